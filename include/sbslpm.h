@@ -1,0 +1,184 @@
+/* sbslpm.h -- C ABI of the MI355X (gfx950) spectral-BSLPM hot path:
+ * cubed-sphere barycentric-Lagrange-interpolation FMM for the sphere Poisson
+ * and Biot-Savart problems, the direct sums, and the backward semi-Lagrangian
+ * BVE step.  Library: spectral-bslpm_amd/libsbslpm.so.
+ *
+ * Conventions (shared with the reference operator API, SURVEY of
+ * fmm_funcs.hpp:16-54, bve_vel.hpp:10-14, inverse_poisson.hpp:10-12,
+ * direct_sum.hpp:8-11, bve_time_step.hpp:7-13):
+ *   - All array arguments are DEVICE pointers owned by the caller, FP64,
+ *     row-major ("LayoutRight"): particle arrays [Pl][m*m] (leaf panel, node
+ *     xi_index*m + eta_index), proxy arrays [P][n], n = (d+1)^2, panels in the
+ *     reference's order (level by level, children (hi,hi),(lo,hi),(lo,lo),(hi,lo)).
+ *   - "accumulate" entry points add into caller-zeroed outputs like the
+ *     reference functors; "assign" ones overwrite.
+ *   - Work is enqueued on the context's stream; functions return after
+ *     launching.  sbslpm_synchronize() waits and reports device-side errors
+ *     (the reference's Kokkos::abort sites) as SBSLPM_E_GEOMETRY.
+ *   - A context is bound to one device and one stream and is not thread-safe.
+ *   - Layout: each leaf carries an m x m Chebyshev (2nd kind) particle grid,
+ *     m = leaf_degree + 1; proxies have degree d.  leaf_degree == d is exactly
+ *     the reference layout (particles are the leaf proxy nodes).
+ */
+#ifndef SBSLPM_H
+#define SBSLPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sbslpm_ctx sbslpm_ctx;
+
+enum {
+  SBSLPM_OK = 0,
+  SBSLPM_E_ARG = 1,      /* invalid argument (reference: std::runtime_error) */
+  SBSLPM_E_GEOMETRY = 2, /* point outside its panel / location failure (reference: Kokkos::abort) */
+  SBSLPM_E_HIP = 3,      /* HIP runtime error */
+  SBSLPM_E_NOMEM = 4
+};
+
+enum { SBSLPM_WEIGHTS_REFERENCE = 0, SBSLPM_WEIGHTS_CLENSHAW_CURTIS = 1, SBSLPM_WEIGHTS_EQUAL_AREA = 2 };
+enum { SBSLPM_RK4_REFERENCE = 0, SBSLPM_RK4_TEXTBOOK = 1 };
+enum { SBSLPM_FP64 = 0, SBSLPM_MIXED = 1 };
+enum { SBSLPM_FLAG_GRAPH = 1 };
+
+/* Run parameters: the hot-path subset of RunConfig (run_config.hpp:6-44). */
+typedef struct sbslpm_params {
+  int levels;         /* RunConfig::levels; leaves at level levels-1 */
+  int interp_degree;  /* RunConfig::interp_degree, 1..10 */
+  int leaf_degree;    /* particle grid degree per leaf (m-1), 1..10; == interp_degree for the reference layout */
+  int weight_rule;    /* SBSLPM_WEIGHTS_*: particle quadrature weights (initialize_cubed_sphere.cpp:275) */
+  int rk4_combine;    /* SBSLPM_RK4_*: departure-point combine (bve_time_step.cpp:288-303) */
+  int precision;      /* SBSLPM_FP64 or SBSLPM_MIXED pair kernels */
+  int tracer_count;   /* RunConfig::tracer_count */
+  int flags;          /* bit 0: capture each BVE step in a hipGraph and replay it */
+  double theta;       /* RunConfig::fmm_theta */
+  double kernel_eps;  /* RunConfig::kernel_eps */
+  double omega;       /* RunConfig::omega */
+  double delta_t;     /* RunConfig::delta_t */
+} sbslpm_params;
+
+/* ---- Host-only set-up (no device needed) ------------------------------------------- */
+typedef struct sbslpm_tree sbslpm_tree;
+/* cubed_sphere_panels_init (initialize_cubed_sphere.cpp:185-234), interp_init (interp_funcs.cpp:24-73),
+ * cube_sphere_spec_points (initialize_cubed_sphere.cpp:280-288) and dual_tree_traversal
+ * (fmm_funcs.cpp:17-123) on the host. */
+int sbslpm_tree_build(const sbslpm_params* params, sbslpm_tree** out);
+void sbslpm_tree_destroy(sbslpm_tree* tree);
+/* Same layout as sbslpm_get_sizes. */
+int sbslpm_tree_sizes(const sbslpm_tree* tree, int64_t* sizes);
+int sbslpm_tree_spec_points(const sbslpm_tree* tree, double* x, double* y, double* z, double* area);
+int sbslpm_tree_interaction_list(const sbslpm_tree* tree, int* list);
+/* ints[P][8] = face, level, parent, child1..child4, is_leaf;
+ * dbls[P][6] = min_xi, max_xi, min_eta, max_eta (in [-1,1]), radius, area (CubedSpherePanel fields). */
+int sbslpm_tree_panels(const sbslpm_tree* tree, int* ints, double* dbls);
+/* point_locate_panel_2 (interp_funcs.cpp:175-203): leaf panel id containing (x,y,z), -1 on failure. */
+int sbslpm_tree_locate(const sbslpm_tree* tree, double x, double y, double z);
+/* Fixed transfer matrices: m2m_hi/m2m_lo [(d+1)*(d+1)] (parent basis a at upper/lower-half child node k,
+ * index a*(d+1)+k) and p2m [(d+1)*m] (proxy basis a at particle node k). Any pointer may be NULL. */
+int sbslpm_tree_transfer_matrices(const sbslpm_tree* tree, double* m2m_hi, double* m2m_lo, double* p2m);
+
+/* ---- Device context ----------------------------------------------------------------- */
+/* Builds the cubed-sphere tree (cubed_sphere_panels_init, initialize_cubed_sphere.cpp:185-234),
+ * the interpolation tables (interp_init, interp_funcs.cpp:24-73), the interaction list
+ * (dual_tree_traversal, fmm_funcs.cpp:17-123) and uploads them with all scratch space. */
+int sbslpm_ctx_create(const sbslpm_params* params, sbslpm_ctx** out);
+void sbslpm_ctx_destroy(sbslpm_ctx* ctx);
+/* stream: a hipStream_t (NULL = default stream). */
+int sbslpm_ctx_set_stream(sbslpm_ctx* ctx, void* stream);
+int sbslpm_ctx_set_precision(sbslpm_ctx* ctx, int precision);
+const char* sbslpm_last_error(const sbslpm_ctx* ctx);
+/* Waits for the stream; returns SBSLPM_E_GEOMETRY once if a kernel flagged a geometry error. */
+int sbslpm_synchronize(sbslpm_ctx* ctx);
+
+/* sizes[0..7] = P (panels), Pl (leaves), n (proxies per panel), m2 (particles per leaf),
+ * I (interactions, RunConfig::fmm_interaction_count), N = Pl*m2, offset = P - Pl, tiles. */
+int sbslpm_get_sizes(const sbslpm_ctx* ctx, int64_t* sizes);
+/* Host copies of the particle positions and weights [Pl][m2]
+ * (cube_sphere_spec_points, initialize_cubed_sphere.cpp:280-288). */
+int sbslpm_spec_points(const sbslpm_ctx* ctx, double* x, double* y, double* z, double* area);
+/* Host copy of the interaction list [I][3] = {target, source, type}, reference order. */
+int sbslpm_interaction_list(const sbslpm_ctx* ctx, int* list);
+
+/* upward_pass (fmm_funcs.cpp:341-355): q_part = area*pots [Pl][m2] (base_pots),
+ * q_proxy [P][n] leaf rows by P2M, parents by M2M.  Assigns every row. */
+int sbslpm_upward_pass(sbslpm_ctx* ctx, const double* area, const double* pots, double* q_part, double* q_proxy);
+
+/* bve_vel_interactions (fmm_interactions/bve_vel.cpp:9-37): Biot-Savart pair sums over the
+ * interaction list with sources displaced by disp (NULL = no displacement).
+ * Accumulates into phi_part_{x,y,z} [Pl][m2] (leaf targets) and phi_proxy_{x,y,z} [P][n]. */
+int sbslpm_bve_vel_interactions(sbslpm_ctx* ctx, const double* disp_part_x, const double* disp_part_y,
+                                const double* disp_part_z, const double* disp_proxy_x, const double* disp_proxy_y,
+                                const double* disp_proxy_z, const double* q_part, const double* q_proxy,
+                                double* phi_part_x, double* phi_part_y, double* phi_part_z, double* phi_proxy_x,
+                                double* phi_proxy_y, double* phi_proxy_z);
+
+/* poisson_fmm_interactions (fmm_interactions/inverse_poisson.cpp:9-33): log-kernel pair sums.
+ * Accumulates into psi_part [Pl][m2] and psi_proxy [P][n]. */
+int sbslpm_poisson_fmm_interactions(sbslpm_ctx* ctx, const double* q_part, const double* q_proxy, double* psi_part,
+                                    double* psi_proxy);
+
+/* downward_pass (fmm_funcs.cpp:357-372): L2L accumulated into phi_proxy [P][n] top-down,
+ * then sols [Pl][m2] = L2P(leaf proxies) + phi_part (phi_part may be NULL). */
+int sbslpm_downward_pass(sbslpm_ctx* ctx, double* phi_proxy, const double* phi_part, double* sols);
+/* downward_pass_3 (fmm_funcs.cpp:374-392): the same for three components. */
+int sbslpm_downward_pass_3(sbslpm_ctx* ctx, double* phi_proxy_x, double* phi_proxy_y, double* phi_proxy_z,
+                           const double* phi_part_x, const double* phi_part_y, const double* phi_part_z,
+                           double* sols_x, double* sols_y, double* sols_z);
+
+/* One Biot-Savart velocity evaluation, upward pass + interactions + downward pass
+ * (the sequence at bve_time_step.cpp:321-344).  vel_{x,y,z} [Pl][m2] assigned. */
+int sbslpm_bve_velocity(sbslpm_ctx* ctx, const double* area, const double* vor, const double* disp_part_x,
+                        const double* disp_part_y, const double* disp_part_z, const double* disp_proxy_x,
+                        const double* disp_proxy_y, const double* disp_proxy_z, double* vel_x, double* vel_y,
+                        double* vel_z);
+/* One sphere Poisson solve (poisson_eq_fmm.cpp:129-147): psi [Pl][m2] assigned. */
+int sbslpm_poisson_solve(sbslpm_ctx* ctx, const double* area, const double* f, double* psi);
+
+/* direct_sum_inv_lap (direct_sum.cpp:8-15): soln[i] += sum_{j != i} -1/(4 pi) log(1 - x_i.x_j)
+ * area_j pots_j for i in [lb, ub) over n points. */
+int sbslpm_direct_sum_inv_lap(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                              const double* area, const double* pots, double* soln, int64_t lb, int64_t ub);
+/* Log direct sum at nt separate targets over ns sources; pairs with x.y >= 1 - 1e-15 skipped
+ * (the FMM kernel's guard, inverse_poisson_impl.hpp:205).  soln [nt] accumulated. */
+int sbslpm_direct_sum_inv_lap_targets(sbslpm_ctx* ctx, int64_t nt, const double* tx, const double* ty,
+                                      const double* tz, int64_t ns, const double* sx, const double* sy,
+                                      const double* sz, const double* area, const double* pots, double* soln);
+/* Biot-Savart direct sum (no reference counterpart; pair kernel of bve_vel_impl.hpp:90-103):
+ * vel[i] += sum_j -1/(4 pi) area_j vor_j (x_i cross y_j)/(1 - x_i.y_j + eps), y_j = P(x_j + disp_j),
+ * for i in [lb, ub); disp may be NULL.  Uses the context's eps. */
+int sbslpm_direct_sum_bve_vel(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                              const double* area, const double* vor, const double* disp_x, const double* disp_y,
+                              const double* disp_z, double* vel_x, double* vel_y, double* vel_z, int64_t lb,
+                              int64_t ub);
+
+/* bve_displacement_upward_pass (bve_time_step.cpp:162-170): new_vor = vor + 2 Omega z - 2 Omega z',
+ * z' of P(x + tau v); disp_part = tau v [Pl][m2]; disp_proxy [P][n] non-leaf rows interpolated. */
+int sbslpm_bve_displacement(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                            const double* vel_x, const double* vel_y, const double* vel_z, const double* vor,
+                            double tau, double* new_vor, double* disp_part_x, double* disp_part_y,
+                            double* disp_part_z, double* disp_proxy_x, double* disp_proxy_y, double* disp_proxy_z);
+/* bve_find_departure_points (bve_time_step.cpp:172-309); vels = 12 pointers
+ * {v0x,v0y,v0z, v1x,v1y,v1z, v2x,v2y,v2z, v3x,v3y,v3z} [Pl][m2]. */
+int sbslpm_bve_find_departure_points(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                     const double* const* vels, double dt, double* dep_x, double* dep_y,
+                                     double* dep_z);
+/* bve_departure_to_target (bve_time_step.cpp:15-67); tracers [Pl][m2][ntr] (may be NULL if ntr == 0). */
+int sbslpm_bve_departure_to_target(sbslpm_ctx* ctx, const double* z, const double* dep_x, const double* dep_y,
+                                   const double* dep_z, const double* vor, const double* tracers, int ntr,
+                                   double* new_vor, double* new_tracers);
+/* bve_back_rk4_step (bve_time_step.cpp:311-419), no forcing (bve_forcing copy,
+ * forcing_funcs.cpp:93-95).  vor [Pl][m2] and tracers [Pl][m2][tracer_count] updated in place;
+ * vel_{x,y,z} receive the stage-0 velocity; proxy_source_vors [P][n] is scratch (may be NULL).
+ * `time` is accepted for signature parity (it only drives forcing). */
+int sbslpm_bve_back_rk4_step(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                             const double* area, double* vor, double* proxy_source_vors, double* vel_x,
+                             double* vel_y, double* vel_z, double* tracers, double time);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBSLPM_H */

@@ -1,0 +1,313 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes front end of the FP64 CPU oracle.
+
+The oracle restates the reference algorithm (see sbslpm_oracle.cpp for the
+file:line of every function).  Only tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg may import this module; the product path never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_sbslpm.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_ppd = ctypes.POINTER(_dp)
+
+
+def build():
+    subprocess.check_call([os.path.join(_HERE, "build.sh")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        for f in ("orc_panel_count", "orc_leaf_count", "orc_interaction_count"):
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+            getattr(L, f).restype = ctypes.c_int
+        L.orc_get_panels.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.orc_get_interactions.argtypes = [ctypes.c_void_p, _ip]
+        L.orc_interp_init.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_cc_weights_textbook.argtypes = [ctypes.c_int, _dp]
+        L.orc_interp_vals_bli.argtypes = [_dp] + [ctypes.c_double] * 6 + [ctypes.c_int]
+        L.orc_interp_vals_bli.restype = ctypes.c_int
+        L.orc_face_from_xyz.argtypes = [ctypes.c_double] * 3
+        L.orc_face_from_xyz.restype = ctypes.c_int
+        L.orc_xyz_from_xieta.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_xyz_from_xieta.restype = ctypes.c_int
+        L.orc_xieta_from_xyz.argtypes = [ctypes.c_double] * 3 + [_dp]
+        L.orc_xieta_from_xyz_face.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _dp]
+        L.orc_xieta_from_xyz_face.restype = ctypes.c_int
+        L.orc_sphere_tri_area.argtypes = [_dp, _dp, _dp]
+        L.orc_sphere_tri_area.restype = ctypes.c_double
+        L.orc_point_locate_panel.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
+        L.orc_point_locate_panel.restype = ctypes.c_int
+        L.orc_spec_points.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_proxy_points.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_upward_pass.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_upward_pass.restype = ctypes.c_int
+        L.orc_downward_pass.argtypes = [ctypes.c_void_p, ctypes.c_int, _ppd, _ppd, _ppd]
+        L.orc_downward_pass.restype = ctypes.c_int
+        L.orc_bve_vel_interactions.argtypes = [ctypes.c_void_p, _ppd, _ppd, _dp, _dp, ctypes.c_double, _ppd, _ppd]
+        L.orc_poisson_fmm_interactions.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_direct_sum_inv_lap.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64]
+        L.orc_direct_sum_bve_vel.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _ppd, ctypes.c_double, _ppd,
+                                             ctypes.c_int64, ctypes.c_int64]
+        L.orc_direct_sum_inv_lap_targets.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp,
+                                                     _dp, _dp, _dp]
+        L.orc_bve_displacement.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _dp, ctypes.c_double,
+                                           ctypes.c_double, _dp, _ppd, _ppd]
+        L.orc_bve_displacement.restype = ctypes.c_int
+        L.orc_bve_find_departure_points.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, ctypes.c_double,
+                                                    ctypes.c_int, _dp, _dp, _dp]
+        L.orc_bve_find_departure_points.restype = ctypes.c_int
+        L.orc_bve_departure_to_target.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                                  ctypes.c_double, _dp, _dp]
+        L.orc_bve_departure_to_target.restype = ctypes.c_int
+        L.orc_bve_back_rk4_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ppd]
+        L.orc_bve_back_rk4_step.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _pp(arrs):
+    if arrs is None:
+        return None
+    ptrs = (_dp * len(arrs))(*[_p(a) for a in arrs])
+    _pp.keep.append((ptrs, arrs))
+    if len(_pp.keep) > 64:
+        del _pp.keep[:32]
+    return ctypes.cast(ptrs, _ppd)
+
+
+_pp.keep = []
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def interp_init(degree, weight_rule=0):
+    tab = np.zeros(((degree + 1) ** 2, 4))
+    lib().orc_interp_init(degree, weight_rule, _p(tab))
+    return tab
+
+
+def cc_weights_textbook(degree):
+    w = np.zeros(degree + 1)
+    lib().orc_cc_weights_textbook(degree, _p(w))
+    return w
+
+
+def interp_vals_bli(xi, eta, min_xi, max_xi, min_eta, max_eta, degree):
+    basis = np.zeros((degree + 1) ** 2)
+    err = lib().orc_interp_vals_bli(_p(basis), xi, eta, min_xi, max_xi, min_eta, max_eta, degree)
+    return basis, err
+
+
+def face_from_xyz(x, y, z):
+    return lib().orc_face_from_xyz(x, y, z)
+
+
+def xyz_from_xieta(xi, eta, face):
+    out = np.zeros(3)
+    err = lib().orc_xyz_from_xieta(xi, eta, face, _p(out))
+    return out, err
+
+
+def xieta_from_xyz(x, y, z, face=None):
+    out = np.zeros(2)
+    if face is None:
+        lib().orc_xieta_from_xyz(x, y, z, _p(out))
+        return out
+    err = lib().orc_xieta_from_xyz_face(x, y, z, face, _p(out))
+    return out, err
+
+
+def sphere_tri_area(a, b, c):
+    return lib().orc_sphere_tri_area(_p(np.asarray(a, float)), _p(np.asarray(b, float)), _p(np.asarray(c, float)))
+
+
+def direct_sum_inv_lap(x, y, z, area, pots, lb=0, ub=None):
+    n = len(x)
+    ub = n if ub is None else ub
+    soln = np.zeros(n)
+    lib().orc_direct_sum_inv_lap(n, _p(x), _p(y), _p(z), _p(area), _p(pots), _p(soln), lb, ub)
+    return soln
+
+
+def direct_sum_bve_vel(x, y, z, area, vor, eps, disp=None, lb=0, ub=None):
+    n = len(x)
+    ub = n if ub is None else ub
+    vel = [np.zeros(n) for _ in range(3)]
+    lib().orc_direct_sum_bve_vel(n, _p(x), _p(y), _p(z), _p(area), _p(vor), _pp(disp), eps, _pp(vel), lb, ub)
+    return vel
+
+
+def direct_sum_inv_lap_targets(tx, ty, tz, sx, sy, sz, area, pots):
+    out = np.zeros(len(tx))
+    lib().orc_direct_sum_inv_lap_targets(len(tx), _p(tx), _p(ty), _p(tz), len(sx), _p(sx), _p(sy), _p(sz),
+                                         _p(area), _p(pots), _p(out))
+    return out
+
+
+class OracleFMM:
+    """Tree + interaction list + passes of the reference FMM on the host.
+
+    levels L (leaves at level L-1), proxy degree d, particle-grid degree
+    md = m-1 (md == d is the reference layout), MAC theta, particle weight rule
+    (0 reference quadrature, 1 Clenshaw-Curtis, 2 equal area).
+    """
+
+    def __init__(self, levels, degree, leaf_degree=None, theta=0.7, weight_rule=0):
+        self.levels, self.d = levels, degree
+        self.md = degree if leaf_degree is None else leaf_degree
+        self.theta, self.weight_rule = theta, weight_rule
+        self.h = lib().orc_create(levels, degree, self.md, theta, weight_rule)
+        if not self.h:
+            raise ValueError("oracle: bad parameters")
+        self.P = lib().orc_panel_count(self.h)
+        self.Pl = lib().orc_leaf_count(self.h)
+        self.offset = self.P - self.Pl
+        self.n = (degree + 1) ** 2
+        self.m2 = (self.md + 1) ** 2
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def interaction_count(self):
+        return lib().orc_interaction_count(self.h)
+
+    def panels(self):
+        ints = np.zeros((self.P, 8), dtype=np.int32)
+        dbls = np.zeros((self.P, 6))
+        lib().orc_get_panels(self.h, ints.ctypes.data_as(_ip), _p(dbls))
+        return ints, dbls
+
+    def interactions(self):
+        out = np.zeros((self.interaction_count, 3), dtype=np.int32)
+        lib().orc_get_interactions(self.h, out.ctypes.data_as(_ip))
+        return out
+
+    def spec_points(self):
+        a = [np.zeros((self.Pl, self.m2)) for _ in range(4)]
+        lib().orc_spec_points(self.h, *[_p(v) for v in a])
+        return a
+
+    def proxy_points(self):
+        a = [np.zeros((self.P, self.n)) for _ in range(3)]
+        lib().orc_proxy_points(self.h, *[_p(v) for v in a])
+        return a
+
+    def point_locate(self, x, y, z):
+        return lib().orc_point_locate_panel(self.h, x, y, z)
+
+    def upward_pass(self, area, pots):
+        qp = np.zeros((self.Pl, self.m2))
+        qx = np.zeros((self.P, self.n))
+        err = lib().orc_upward_pass(self.h, _p(area), _p(pots), _p(qp), _p(qx))
+        if err:
+            raise RuntimeError("oracle upward_pass: geometry error")
+        return qp, qx
+
+    def downward_pass(self, phi_proxy, phi_part=None):
+        ncomp = len(phi_proxy)
+        sols = [np.zeros((self.Pl, self.m2)) for _ in range(ncomp)]
+        err = lib().orc_downward_pass(self.h, ncomp, _pp(phi_proxy), _pp(phi_part), _pp(sols))
+        if err:
+            raise RuntimeError("oracle downward_pass: geometry error")
+        return sols
+
+    def bve_vel_interactions(self, q_part, q_proxy, eps, disp_part=None, disp_proxy=None):
+        if disp_part is None:
+            disp_part = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        if disp_proxy is None:
+            disp_proxy = [np.zeros((self.P, self.n)) for _ in range(3)]
+        pp = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        px = [np.zeros((self.P, self.n)) for _ in range(3)]
+        lib().orc_bve_vel_interactions(self.h, _pp(disp_part), _pp(disp_proxy), _p(q_part), _p(q_proxy), eps,
+                                       _pp(pp), _pp(px))
+        return pp, px
+
+    def poisson_fmm_interactions(self, q_part, q_proxy):
+        pp = np.zeros((self.Pl, self.m2))
+        px = np.zeros((self.P, self.n))
+        lib().orc_poisson_fmm_interactions(self.h, _p(q_part), _p(q_proxy), _p(pp), _p(px))
+        return pp, px
+
+    def bve_velocity(self, area, vor, eps, disp_part=None, disp_proxy=None):
+        qp, qx = self.upward_pass(area, vor)
+        pp, px = self.bve_vel_interactions(qp, qx, eps, disp_part, disp_proxy)
+        return self.downward_pass(px, pp)
+
+    def poisson_solve(self, area, f):
+        qp, qx = self.upward_pass(area, f)
+        pp, px = self.poisson_fmm_interactions(qp, qx)
+        return self.downward_pass([px], [pp])[0]
+
+    def bve_displacement(self, x, y, z, vel, vor, tau, omega):
+        nv = np.zeros((self.Pl, self.m2))
+        dp = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        dx = [np.zeros((self.P, self.n)) for _ in range(3)]
+        err = lib().orc_bve_displacement(self.h, _p(x), _p(y), _p(z), _pp(vel), _p(vor), tau, omega, _p(nv),
+                                         _pp(dp), _pp(dx))
+        if err:
+            raise RuntimeError("oracle bve_displacement: geometry error")
+        return nv, dp, dx
+
+    def bve_find_departure_points(self, x, y, z, vels, dt, combine=0):
+        dep = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        flat = [v for stage in vels for v in stage]
+        err = lib().orc_bve_find_departure_points(self.h, _p(x), _p(y), _p(z), _pp(flat), dt, combine,
+                                                  *[_p(d) for d in dep])
+        if err:
+            raise RuntimeError("oracle departure points: geometry error")
+        return dep
+
+    def bve_departure_to_target(self, z, dep, vor, omega, tracers=None):
+        ntr = 0 if tracers is None else tracers.shape[-1]
+        tr = np.zeros((self.Pl, self.m2, max(ntr, 1))) if tracers is None else tracers
+        nv = np.zeros((self.Pl, self.m2))
+        ntrac = np.zeros_like(tr)
+        err = lib().orc_bve_departure_to_target(self.h, _p(z), *[_p(d) for d in dep], _p(vor), _p(tr), ntr,
+                                                omega, _p(nv), _p(ntrac))
+        if err:
+            raise RuntimeError("oracle departure_to_target: geometry error")
+        return nv, (ntrac if ntr else None)
+
+    def bve_back_rk4_step(self, x, y, z, area, vor, dt, omega, eps, combine=0, tracers=None):
+        """Returns (new vorticity, stage-0 velocity, new tracers)."""
+        vor = vor.copy()
+        ntr = 0 if tracers is None else tracers.shape[-1]
+        tr = np.zeros((self.Pl, self.m2, 1)) if tracers is None else tracers.copy()
+        vel = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        err = lib().orc_bve_back_rk4_step(self.h, _p(x), _p(y), _p(z), _p(area), _p(vor), _p(tr), ntr, dt, omega,
+                                          eps, combine, _pp(vel))
+        if err:
+            raise RuntimeError("oracle bve_back_rk4_step: geometry error")
+        return vor, vel, (tr if ntr else None)

@@ -1,0 +1,966 @@
+// sbslpm_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of the spectral-BSLPM hot path (cubed-sphere BLI FMM,
+// direct sum, backward semi-Lagrangian BVE step).  It is the checker for the
+// HIP library: only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg may load it.  The product path never links or calls it.
+//
+// Every function cites the reference file:line it restates.  The loop
+// structure of the reference is kept on purpose (per-pair recomputation of
+// source node coordinates, per-node barycentric basis evaluation), so that
+// timing this file is timing the reference algorithm on the host cores.
+// Two deliberate differences, both summation-order only:
+//   * accumulation is target-major and sequential per target row instead of
+//     Kokkos::atomic_add (the reference order is non-deterministic anyway);
+//   * the caller-side zeroing of accumulated arrays (bve_time_step.cpp:321,
+//     341-343, 351-354) is done inside the pass that accumulates.
+//
+// Generalisation ("layout A-prime"): each leaf carries an m x m Chebyshev
+// (2nd kind) particle grid of degree m-1, proxies have degree d.  For m == d+1
+// this is exactly the reference layout (particles == leaf proxy nodes,
+// P2M/L2P identities, fmm_funcs_impl.hpp:17-34, 174-188).  For m != d+1 the
+// leaf <-> proxy transfers and the type-dependent interactions follow the
+// reference's particle-tree functions (base_pots_ll / child_panel_interp_3_ll,
+// fmm_funcs_impl.hpp:190-225, 318-364; swe_vel_panel_interaction_ll,
+// swe_vel_impl.hpp:179-287), with particle label coordinates taken from the
+// leaf grid.
+//
+// Build: oracle/build.sh (g++ -O2 -fopenmp -ffp-contract=off).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <queue>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+const double kPi = M_PI;
+const double kNegInv4Pi = -0.07957747154594767;  // bve_vel_impl.hpp:63, inverse_poisson_impl.hpp:48
+
+enum { ORC_OK = 0, ORC_E_GEOMETRY = 1, ORC_E_ARG = 2 };
+
+// ---------------------------------------------------------------- geometry
+
+// cubed_sphere_transforms_impl.hpp:8-32
+int face_from_xyz(double x, double y, double z) {
+  double ax = std::fabs(x), ay = std::fabs(y), az = std::fabs(z);
+  if ((ax >= ay) && (ax >= az)) return x >= 0 ? 0 : 2;
+  if ((ay >= ax) && (ay >= az)) return y >= 0 ? 1 : 3;
+  return z >= 0 ? 4 : 5;
+}
+
+// cubed_sphere_transforms_impl.hpp:34-65 (abort -> error code)
+int xyz_from_xieta(double xi, double eta, int face, double* xyz) {
+  double X = std::tan(xi), Y = std::tan(eta);
+  if (face == 0) {
+    xyz[0] = 1.0 / std::sqrt(1 + X * X + Y * Y); xyz[1] = X * xyz[0]; xyz[2] = Y * xyz[0];
+  } else if (face == 1) {
+    xyz[1] = 1.0 / std::sqrt(1 + X * X + Y * Y); xyz[0] = -X * xyz[1]; xyz[2] = Y * xyz[1];
+  } else if (face == 2) {
+    xyz[0] = -1.0 / std::sqrt(1 + X * X + Y * Y); xyz[1] = X * xyz[0]; xyz[2] = -Y * xyz[0];
+  } else if (face == 3) {
+    xyz[1] = -1.0 / std::sqrt(1 + X * X + Y * Y); xyz[0] = -X * xyz[1]; xyz[2] = -Y * xyz[1];
+  } else if (face == 4) {
+    xyz[2] = 1.0 / std::sqrt(1 + X * X + Y * Y); xyz[0] = -Y * xyz[2]; xyz[1] = X * xyz[2];
+  } else if (face == 5) {
+    xyz[2] = -1.0 / std::sqrt(1 + X * X + Y * Y); xyz[0] = -Y * xyz[2]; xyz[1] = -X * xyz[2];
+  } else {
+    return ORC_E_GEOMETRY;
+  }
+  return ORC_OK;
+}
+
+// cubed_sphere_transforms_impl.hpp:67-98
+void xieta_from_xyz(double x, double y, double z, double* xieta) {
+  double ax = std::fabs(x), ay = std::fabs(y), az = std::fabs(z);
+  if ((ax >= ay) && (ax >= az)) {
+    if (x >= 0) { xieta[0] = std::atan(y / x); xieta[1] = std::atan(z / x); }
+    else        { xieta[0] = std::atan(y / x); xieta[1] = std::atan(-z / x); }
+  } else if ((ay >= ax) && (ay >= az)) {
+    if (y >= 0) { xieta[0] = std::atan(-x / y); xieta[1] = std::atan(z / y); }
+    else        { xieta[0] = std::atan(-x / y); xieta[1] = std::atan(-z / y); }
+  } else {
+    if (z >= 0) { xieta[0] = std::atan(y / z); xieta[1] = std::atan(-x / z); }
+    else        { xieta[0] = std::atan(-y / z); xieta[1] = std::atan(-x / z); }
+  }
+}
+
+// cubed_sphere_transforms_impl.hpp:100-123
+int xieta_from_xyz_face(double x, double y, double z, int face, double* xieta) {
+  if (face == 0)      { xieta[0] = std::atan(y / x);  xieta[1] = std::atan(z / x); }
+  else if (face == 1) { xieta[0] = std::atan(-x / y); xieta[1] = std::atan(z / y); }
+  else if (face == 2) { xieta[0] = std::atan(y / x);  xieta[1] = std::atan(-z / x); }
+  else if (face == 3) { xieta[0] = std::atan(-x / y); xieta[1] = std::atan(-z / y); }
+  else if (face == 4) { xieta[0] = std::atan(y / z);  xieta[1] = std::atan(-x / z); }
+  else if (face == 5) { xieta[0] = std::atan(-y / z); xieta[1] = std::atan(-x / z); }
+  else return ORC_E_GEOMETRY;
+  return ORC_OK;
+}
+
+// general_utils_impl.hpp:179-185
+inline void project_to_sphere(double& x, double& y, double& z) {
+  double n = std::sqrt(x * x + y * y + z * z);
+  x /= n; y /= n; z /= n;
+}
+
+// general_utils.cpp:23-26
+double gcdist(const double* p1, const double* p2) {
+  double dp = std::max(std::min(p1[0] * p2[0] + p1[1] * p2[1] + p1[2] * p2[2], 1.0), -1.0);
+  return std::acos(dp);
+}
+
+// general_utils.cpp:5-21 (l'Huilier)
+double sphere_tri_area(const double* p1, const double* p2, const double* p3) {
+  double p1n[3], p2n[3], p3n[3];
+  std::memcpy(p1n, p1, 3 * sizeof(double));
+  std::memcpy(p2n, p2, 3 * sizeof(double));
+  std::memcpy(p3n, p3, 3 * sizeof(double));
+  p2n[0] -= p3[0], p2n[1] -= p3[1], p2n[2] -= p3[2];
+  p3n[0] -= p1[0], p3n[1] -= p1[1], p3n[2] -= p1[2];
+  p1n[0] -= p2[0], p1n[1] -= p2[1], p1n[2] -= p2[2];
+  double a = std::acos(1 - 0.5 * (p2n[0] * p2n[0] + p2n[1] * p2n[1] + p2n[2] * p2n[2]));
+  double b = std::acos(1 - 0.5 * (p3n[0] * p3n[0] + p3n[1] * p3n[1] + p3n[2] * p3n[2]));
+  double c = std::acos(1 - 0.5 * (p1n[0] * p1n[0] + p1n[1] * p1n[1] + p1n[2] * p1n[2]));
+  double s = (a + b + c) / 2;
+  double z = std::tan(s / 2) * std::tan((s - a) / 2) * std::tan((s - b) / 2) * std::tan((s - c) / 2);
+  return 4 * std::atan(std::sqrt(z));
+}
+
+// ---------------------------------------------------------------- interpolation
+
+// interp_funcs.cpp:11-22
+double bli_coeff(int j, int degree) {
+  if (degree == 0) return 1;
+  if (j == 0) return 0.5;
+  if (j == degree) return 0.5 * std::pow(-1, j);
+  return std::pow(-1, j);
+}
+
+// Clenshaw-Curtis weights on [-1,1] at cos(k pi/N) (Trefethen, "Spectral
+// Methods in MATLAB", clencurt.m).  Not in the reference: used for the particle
+// grid when its degree is odd, where the reference formula does not integrate
+// constants (its weights then do not sum to 2).
+void cc_weights_textbook(int N, double* w) {
+  if (N == 0) { w[0] = 2.0; return; }
+  std::vector<double> v(N + 1, 1.0);
+  for (int i = 0; i <= N; ++i) w[i] = 0;
+  double theta;
+  if (N % 2 == 0) {
+    w[0] = 1.0 / (N * N - 1.0); w[N] = w[0];
+    for (int k = 1; k <= N / 2 - 1; ++k)
+      for (int i = 1; i < N; ++i) { theta = kPi * i / N; v[i] -= 2.0 * std::cos(2.0 * k * theta) / (4.0 * k * k - 1.0); }
+    for (int i = 1; i < N; ++i) { theta = kPi * i / N; v[i] -= std::cos(N * theta) / (N * N - 1.0); }
+  } else {
+    w[0] = 1.0 / ((double)N * N); w[N] = w[0];
+    for (int k = 1; k <= (N - 1) / 2; ++k)
+      for (int i = 1; i < N; ++i) { theta = kPi * i / N; v[i] -= 2.0 * std::cos(2.0 * k * theta) / (4.0 * k * k - 1.0); }
+  }
+  for (int i = 1; i < N; ++i) w[i] = 2.0 * v[i] / N;
+}
+
+// interp_funcs.cpp:24-73.  tab[(d+1)^2][4] = {xi node, eta node, bary weight
+// product, quadrature weight product}.  weight_rule 0 keeps the reference's
+// quadrature formula (:42-60), 1 uses textbook Clenshaw-Curtis, 2 leaves the
+// reference formula in column 3 (equal-area weights are applied by spec_points).
+void interp_init(int degree, int weight_rule, double* tab) {
+  int pps = degree + 1;
+  std::vector<double> cheb(pps, 0), bw(pps, 0);
+  if (degree == 0) {
+    cheb[0] = 0; bw[0] = 1;
+  } else {
+    for (int i = 0; i < pps; i++) {
+      cheb[i] = std::cos(kPi / degree * i);
+      if (std::fabs(cheb[i]) < 1e-16) cheb[i] = 0;
+      bw[i] = bli_coeff(i, degree);
+    }
+  }
+  std::vector<double> cc(pps, 0);
+  if (weight_rule == 1) {
+    cc_weights_textbook(degree, cc.data());
+  } else {
+    std::vector<double> vec(degree / 2 + 1, 2);
+    vec[0] = 1;
+    for (int i = 1; i < degree / 2 + 1; i++) vec[i] = 2.0 / (1 - 4 * i * i);
+    for (int i = 0; i < degree / 2 + 1; i++) {
+      double sum = 0;
+      for (int j = 0; j < degree / 2 + 1; j++) sum += 2 * std::cos(2 * i * j * kPi / degree) / degree * vec[j];
+      cc[i] = sum;
+    }
+    cc[0] *= 0.5;
+    for (int i = 0; i < degree / 2; i++) cc[degree - i] = cc[i];
+  }
+  for (int i = 0; i < pps; i++)
+    for (int j = 0; j < pps; j++) {
+      int index = i * pps + j;
+      tab[index * 4 + 0] = cheb[i];
+      tab[index * 4 + 1] = cheb[j];
+      tab[index * 4 + 2] = bw[i] * bw[j];
+      tab[index * 4 + 3] = cc[i] * cc[j];
+    }
+}
+
+// interp_funcs_impl.hpp:11-112 (Kokkos::abort -> error code, basis zeroed).
+int interp_vals_bli(double* basis, double xi, double eta, double min_xi, double max_xi,
+                    double min_eta, double max_eta, int deg) {
+  double xi_range = 0.5 * (max_xi - min_xi), xi_offset = 0.5 * (max_xi + min_xi);
+  double eta_range = 0.5 * (max_eta - min_eta), eta_offset = 0.5 * (max_eta + min_eta);
+  double cheb_xi[11], cheb_eta[11], bw[11], xf[11], ef[11], val;
+  for (int i = 0; i < (deg + 1) * (deg + 1); i++) basis[i] = 0.0;
+  if (xi < min_xi - 1e-15 || xi > max_xi + 1e-15 || eta < min_eta - 1e-15 || eta > max_eta + 1e-15 ||
+      !(xi == xi) || !(eta == eta))
+    return ORC_E_GEOMETRY;
+  if (deg == 0) {
+    cheb_xi[0] = xi_offset; cheb_eta[0] = eta_offset; bw[0] = 1;
+  } else {
+    for (int i = 0; i < deg + 1; i++) {
+      cheb_xi[i] = std::cos(kPi * i / deg) * xi_range + xi_offset;
+      cheb_eta[i] = std::cos(kPi * i / deg) * eta_range + eta_offset;
+      if (i == 0) bw[i] = 0.5;
+      else if (i == deg) bw[i] = 0.5 * std::pow(-1, i);
+      else bw[i] = std::pow(-1, i);
+    }
+  }
+  bool found = false;
+  for (int i = 0; i < deg + 1; i++) {
+    if (std::fabs(xi - cheb_xi[i]) < 1e-15) {
+      found = true;
+      for (int j = 0; j < deg + 1; j++) xf[j] = 0;
+      xf[i] = 1;
+      break;
+    }
+  }
+  if (!found) {
+    double den = 0;
+    for (int i = 0; i < deg + 1; i++) { val = bw[i] / (xi - cheb_xi[i]); xf[i] = val; den += val; }
+    for (int i = 0; i < deg + 1; i++) xf[i] /= den;
+  }
+  found = false;
+  for (int i = 0; i < deg + 1; i++) {
+    if (std::fabs(eta - cheb_eta[i]) < 1e-15) {
+      found = true;
+      for (int j = 0; j < deg + 1; j++) ef[j] = 0;
+      ef[i] = 1;
+      break;
+    }
+  }
+  if (!found) {
+    double den = 0;
+    for (int i = 0; i < deg + 1; i++) { val = bw[i] / (eta - cheb_eta[i]); ef[i] = val; den += val; }
+    for (int i = 0; i < deg + 1; i++) ef[i] /= den;
+  }
+  for (int i = 0; i < deg + 1; i++)
+    for (int j = 0; j < deg + 1; j++) basis[i * (deg + 1) + j] = xf[i] * ef[j];
+  return ORC_OK;
+}
+
+// ---------------------------------------------------------------- tree
+
+struct Panel {  // initialize_cubed_sphere.hpp:8-29 (edge neighbours omitted: off the path)
+  int face, level, parent, child[4], is_leaf;
+  double min_xi, max_xi, min_eta, max_eta, radius, area;
+};
+
+struct Ctx {
+  int levels, d, md, weight_rule;  // md = particle grid degree m-1
+  double theta;
+  int P, Pl, offset, n, m2;
+  std::vector<int> level_start;
+  std::vector<Panel> pn;
+  std::vector<double> tab_d, tab_m;  // interp tables, proxies and particle grid
+  std::vector<int> ilist;            // [I][3] = target, source, type (reference BFS order)
+  std::vector<int> by_target;        // interaction indices grouped by target, list order kept
+  std::vector<int> tgt_start;        // [P+1]
+};
+
+// initialize_cubed_sphere.cpp:73-140 (children and bounds)
+void refine(Ctx& c, int level, int i) {
+  int level_start = c.level_start[level + 1];
+  int start = level_start + 4 * i;
+  int prev_start = c.level_start[level];
+  Panel& par = c.pn[i + prev_start];
+  for (int j = 0; j < 4; j++) {
+    c.pn[start + j].face = par.face;
+    c.pn[start + j].parent = i + prev_start;
+    c.pn[start + j].level = par.level + 1;
+  }
+  double max_xi = par.max_xi, min_xi = par.min_xi, max_eta = par.max_eta, min_eta = par.min_eta;
+  double mid_xi = 0.5 * (max_xi + min_xi), mid_eta = 0.5 * (max_eta + min_eta);
+  c.pn[start].max_xi = max_xi;     c.pn[start].min_xi = mid_xi;
+  c.pn[start].max_eta = max_eta;   c.pn[start].min_eta = mid_eta;
+  c.pn[start + 1].max_xi = mid_xi; c.pn[start + 1].min_xi = min_xi;
+  c.pn[start + 1].max_eta = max_eta; c.pn[start + 1].min_eta = mid_eta;
+  c.pn[start + 2].max_xi = mid_xi; c.pn[start + 2].min_xi = min_xi;
+  c.pn[start + 2].max_eta = mid_eta; c.pn[start + 2].min_eta = min_eta;
+  c.pn[start + 3].max_xi = max_xi; c.pn[start + 3].min_xi = mid_xi;
+  c.pn[start + 3].max_eta = mid_eta; c.pn[start + 3].min_eta = min_eta;
+  for (int j = 0; j < 4; j++) par.child[j] = start + j;
+}
+
+// initialize_cubed_sphere.cpp:151-182
+void init_2(Ctx& c, int i) {
+  Panel& p = c.pn[i];
+  p.is_leaf = (p.level == c.levels - 1) ? 1 : 0;
+  double max_xi = p.max_xi * kPi / 4.0, min_xi = p.min_xi * kPi / 4.0;
+  double max_eta = p.max_eta * kPi / 4.0, min_eta = p.min_eta * kPi / 4.0;
+  double mid_xi = 0.5 * (max_xi + min_xi), mid_eta = 0.5 * (max_eta + min_eta);
+  double p1[3], p2[3], p3[3], p4[3], pc[3];
+  xyz_from_xieta(max_xi, max_eta, p.face, p1);
+  xyz_from_xieta(max_xi, min_eta, p.face, p2);
+  xyz_from_xieta(min_xi, min_eta, p.face, p3);
+  xyz_from_xieta(min_xi, max_eta, p.face, p4);
+  xyz_from_xieta(mid_xi, mid_eta, p.face, pc);
+  double d1 = gcdist(p1, pc), d2 = gcdist(p2, pc), d3 = gcdist(p3, pc), d4 = gcdist(p4, pc);
+  p.radius = std::fmax(d1, std::fmax(d2, std::fmax(d3, d4)));
+  p.area = sphere_tri_area(p1, p2, p3) + sphere_tri_area(p1, p3, p4);
+}
+
+// initialize_cubed_sphere.cpp:185-234
+void build_tree(Ctx& c) {
+  int L = c.levels;
+  c.P = 2 * ((int)std::pow(4, L) - 1);
+  c.Pl = 6 * (int)std::pow(4, L - 1);
+  c.offset = c.P - c.Pl;
+  c.pn.assign(c.P, Panel());
+  for (auto& p : c.pn) { p.parent = -1; p.child[0] = p.child[1] = p.child[2] = p.child[3] = -1; }
+  for (int i = 0; i < 6; i++) {
+    c.pn[i].face = i; c.pn[i].min_xi = -1; c.pn[i].max_xi = 1;
+    c.pn[i].min_eta = -1; c.pn[i].max_eta = 1; c.pn[i].level = 0; c.pn[i].parent = -1;
+  }
+  c.level_start.assign(L + 1, 0);
+  for (int i = 0; i < L; i++) c.level_start[i + 1] = c.level_start[i] + 6 * (int)std::pow(4, i);
+  for (int i = 0; i < L - 1; i++)
+    for (int j = 0; j < 6 * (int)std::pow(4, i); j++) refine(c, i, j);
+  for (int i = 0; i < c.P; i++) init_2(c, i);
+}
+
+// interp_funcs_impl.hpp:114-142
+int point_locate_panel(const Ctx& c, double x, double y, double z) {
+  double xieta[2];
+  int cur = face_from_xyz(x, y, z);
+  xieta_from_xyz_face(x, y, z, cur, xieta);
+  double xi = xieta[0], eta = xieta[1];
+  while (!c.pn[cur].is_leaf) {
+    double mid_xi = 0.5 * kPi / 4.0 * (c.pn[cur].max_xi + c.pn[cur].min_xi);
+    double mid_eta = 0.5 * kPi / 4.0 * (c.pn[cur].max_eta + c.pn[cur].min_eta);
+    if ((xi >= mid_xi) && (eta >= mid_eta)) cur = c.pn[cur].child[0];
+    else if ((xi <= mid_xi) && (eta >= mid_eta)) cur = c.pn[cur].child[1];
+    else if ((xi <= mid_xi) && (eta <= mid_eta)) cur = c.pn[cur].child[2];
+    else if ((xi >= mid_xi) && (eta <= mid_eta)) cur = c.pn[cur].child[3];
+    else return -1;  // NaN input
+  }
+  return cur;
+}
+
+// fmm_funcs.cpp:17-123 (literal BFS; the pair order is the reference's)
+void dual_tree_traversal(Ctx& c) {
+  std::queue<int> tq, sq;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) { tq.push(i); sq.push(j); }
+  c.ilist.clear();
+  double c1[3], c2[3];
+  while (!tq.empty()) {
+    int t = tq.front(), s = sq.front();
+    tq.pop(); sq.pop();
+    const Panel &pt = c.pn[t], &ps = c.pn[s];
+    double xi1 = kPi / 4.0 * 0.5 * (pt.min_xi + pt.max_xi);
+    double xi2 = kPi / 4.0 * 0.5 * (ps.min_xi + ps.max_xi);
+    double eta1 = kPi / 4.0 * 0.5 * (pt.min_eta + pt.max_eta);
+    double eta2 = kPi / 4.0 * 0.5 * (ps.min_eta + ps.max_eta);
+    xyz_from_xieta(xi1, eta1, pt.face, c1);
+    xyz_from_xieta(xi2, eta2, ps.face, c2);
+    double dist = gcdist(c1, c2);
+    double sep = 100;
+    if (dist > 1e-16) sep = (pt.radius + ps.radius) / dist;
+    if (sep < c.theta) {
+      int type = 0;
+      if (pt.level < c.levels - 1) type += 2;
+      if (ps.level < c.levels - 1) type += 1;
+      c.ilist.push_back(t); c.ilist.push_back(s); c.ilist.push_back(type);
+    } else if (pt.is_leaf && ps.is_leaf) {
+      c.ilist.push_back(t); c.ilist.push_back(s); c.ilist.push_back(0);
+    } else if (pt.is_leaf) {
+      for (int k = 0; k < 4; k++) { tq.push(t); sq.push(ps.child[k]); }
+    } else if (ps.is_leaf) {
+      for (int k = 0; k < 4; k++) { sq.push(s); tq.push(pt.child[k]); }
+    } else if (pt.level <= ps.level) {
+      for (int k = 0; k < 4; k++) { sq.push(s); tq.push(pt.child[k]); }
+    } else {
+      for (int k = 0; k < 4; k++) { tq.push(t); sq.push(ps.child[k]); }
+    }
+  }
+  int I = (int)c.ilist.size() / 3;
+  c.tgt_start.assign(c.P + 1, 0);
+  for (int i = 0; i < I; i++) c.tgt_start[c.ilist[3 * i] + 1]++;
+  for (int p = 0; p < c.P; p++) c.tgt_start[p + 1] += c.tgt_start[p];
+  c.by_target.assign(I, 0);
+  std::vector<int> fill(c.tgt_start.begin(), c.tgt_start.end() - 1);
+  for (int i = 0; i < I; i++) c.by_target[fill[c.ilist[3 * i]]++] = i;
+}
+
+// Panel label-space scale/offset in radians, as the kernels compute them
+// (bve_vel_impl.hpp:75-83).
+inline void panel_frame(const Panel& p, double& xo, double& xs, double& eo, double& es) {
+  xo = kPi / 4 * 0.5 * (p.max_xi + p.min_xi);
+  xs = kPi / 4 * 0.5 * (p.max_xi - p.min_xi);
+  eo = kPi / 4 * 0.5 * (p.max_eta + p.min_eta);
+  es = kPi / 4 * 0.5 * (p.max_eta - p.min_eta);
+}
+
+inline int bli_in_panel(const Panel& p, double xi, double eta, int deg, double* basis) {
+  return interp_vals_bli(basis, xi, eta, kPi / 4.0 * p.min_xi, kPi / 4.0 * p.max_xi,
+                         kPi / 4.0 * p.min_eta, kPi / 4.0 * p.max_eta, deg);
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+void* orc_create(int levels, int degree, int leaf_degree, double theta, int weight_rule) {
+  if (levels < 1 || levels > 9 || degree < 1 || degree > 10 || leaf_degree < 1 || leaf_degree > 10) return nullptr;
+  Ctx* c = new Ctx();
+  c->levels = levels; c->d = degree; c->md = leaf_degree; c->theta = theta; c->weight_rule = weight_rule;
+  c->n = (degree + 1) * (degree + 1);
+  c->m2 = (leaf_degree + 1) * (leaf_degree + 1);
+  c->tab_d.resize(c->n * 4);
+  c->tab_m.resize(c->m2 * 4);
+  interp_init(degree, 0, c->tab_d.data());
+  interp_init(leaf_degree, weight_rule == 1 ? 1 : 0, c->tab_m.data());
+  build_tree(*c);
+  dual_tree_traversal(*c);
+  return c;
+}
+
+void orc_destroy(void* h) { delete static_cast<Ctx*>(h); }
+
+int orc_panel_count(void* h) { return static_cast<Ctx*>(h)->P; }
+int orc_leaf_count(void* h) { return static_cast<Ctx*>(h)->Pl; }
+int orc_interaction_count(void* h) { return (int)static_cast<Ctx*>(h)->ilist.size() / 3; }
+
+// ints[P][8] = face, level, parent, child1..4, is_leaf; dbls[P][6] = min/max xi, min/max eta, radius, area
+void orc_get_panels(void* h, int* ints, double* dbls) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  for (int i = 0; i < c.P; i++) {
+    const Panel& p = c.pn[i];
+    int* r = ints + 8 * i;
+    r[0] = p.face; r[1] = p.level; r[2] = p.parent;
+    r[3] = p.child[0]; r[4] = p.child[1]; r[5] = p.child[2]; r[6] = p.child[3]; r[7] = p.is_leaf;
+    double* q = dbls + 6 * i;
+    q[0] = p.min_xi; q[1] = p.max_xi; q[2] = p.min_eta; q[3] = p.max_eta; q[4] = p.radius; q[5] = p.area;
+  }
+}
+
+void orc_get_interactions(void* h, int* out) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  std::memcpy(out, c.ilist.data(), c.ilist.size() * sizeof(int));
+}
+
+void orc_interp_init(int degree, int weight_rule, double* tab) { interp_init(degree, weight_rule, tab); }
+void orc_cc_weights_textbook(int degree, double* w) { cc_weights_textbook(degree, w); }
+
+int orc_interp_vals_bli(double* basis, double xi, double eta, double min_xi, double max_xi,
+                        double min_eta, double max_eta, int deg) {
+  return interp_vals_bli(basis, xi, eta, min_xi, max_xi, min_eta, max_eta, deg);
+}
+int orc_face_from_xyz(double x, double y, double z) { return face_from_xyz(x, y, z); }
+int orc_xyz_from_xieta(double xi, double eta, int face, double* xyz) { return xyz_from_xieta(xi, eta, face, xyz); }
+void orc_xieta_from_xyz(double x, double y, double z, double* xieta) { xieta_from_xyz(x, y, z, xieta); }
+int orc_xieta_from_xyz_face(double x, double y, double z, int face, double* xieta) {
+  return xieta_from_xyz_face(x, y, z, face, xieta);
+}
+double orc_sphere_tri_area(const double* a, const double* b, const double* c) { return sphere_tri_area(a, b, c); }
+int orc_point_locate_panel(void* h, double x, double y, double z) {
+  return point_locate_panel(*static_cast<Ctx*>(h), x, y, z);
+}
+
+// initialize_cubed_sphere.cpp:254-277 on the particle grid (degree m-1).
+// Weights: cc_i*cc_j*panel_area/4 (:275), or 4 pi/N for weight_rule 2.
+// Outputs [Pl][m2].
+void orc_spec_points(void* h, double* x, double* y, double* z, double* area) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  double N = (double)c.Pl * c.m2;
+  for (int i = c.offset; i < c.P; i++) {
+    const Panel& p = c.pn[i];
+    double max_xi = kPi / 4.0 * p.max_xi, min_xi = kPi / 4.0 * p.min_xi;
+    double max_eta = kPi / 4.0 * p.max_eta, min_eta = kPi / 4.0 * p.min_eta;
+    double xr = 0.5 * (max_xi - min_xi), xo = 0.5 * (max_xi + min_xi);
+    double er = 0.5 * (max_eta - min_eta), eo = 0.5 * (max_eta + min_eta);
+    for (int j = 0; j < c.m2; j++) {
+      double xyz[3];
+      xyz_from_xieta(xo + xr * c.tab_m[4 * j], eo + er * c.tab_m[4 * j + 1], p.face, xyz);
+      size_t o = (size_t)(i - c.offset) * c.m2 + j;
+      x[o] = xyz[0]; y[o] = xyz[1]; z[o] = xyz[2];
+      area[o] = c.weight_rule == 2 ? 4.0 * kPi / N : c.tab_m[4 * j + 3] * p.area / 4.0;
+    }
+  }
+}
+
+// Proxy node positions of every panel, [P][n] (the coordinates the pair
+// kernels recompute per pair, bve_vel_impl.hpp:86-93).
+void orc_proxy_points(void* h, double* x, double* y, double* z) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  for (int i = 0; i < c.P; i++) {
+    double xo, xs, eo, es, xyz[3];
+    panel_frame(c.pn[i], xo, xs, eo, es);
+    for (int j = 0; j < c.n; j++) {
+      xyz_from_xieta(c.tab_d[4 * j] * xs + xo, c.tab_d[4 * j + 1] * es + eo, c.pn[i].face, xyz);
+      x[(size_t)i * c.n + j] = xyz[0]; y[(size_t)i * c.n + j] = xyz[1]; z[(size_t)i * c.n + j] = xyz[2];
+    }
+  }
+}
+
+// fmm_funcs.cpp:341-355 with base_pots (fmm_funcs_impl.hpp:29-33) and
+// child_to_parent (:47-70).  For m != d+1 the leaf proxies come from BLI
+// anterpolation of the particle charges (base_pots_ll, :207-223).
+// q_part[Pl][m2] (= area*pots), q_proxy[P][n]; every row is assigned.
+int orc_upward_pass(void* h, const double* area, const double* pots, double* q_part, double* q_proxy) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+  for (size_t k = 0; k < (size_t)c.Pl * c.m2; k++) q_part[k] = area[k] * pots[k];
+  std::memset(q_proxy, 0, sizeof(double) * (size_t)c.P * c.n);
+  if (c.md == c.d) {
+    std::memcpy(q_proxy + (size_t)c.offset * c.n, q_part, sizeof(double) * (size_t)c.Pl * c.n);
+  } else {
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err)
+    for (int i = 0; i < c.Pl; i++) {
+      const Panel& p = c.pn[c.offset + i];
+      double xo, xs, eo, es, basis[121];
+      panel_frame(p, xo, xs, eo, es);
+      double* out = q_proxy + (size_t)(c.offset + i) * c.n;
+      for (int j = 0; j < c.m2; j++) {
+        double xi = c.tab_m[4 * j] * xs + xo, eta = c.tab_m[4 * j + 1] * es + eo;
+        err |= bli_in_panel(p, xi, eta, c.d, basis);
+        double qj = q_part[(size_t)i * c.m2 + j];
+        for (int k = 0; k < c.n; k++) out[k] += basis[k] * qj;
+      }
+    }
+  }
+  // M2M, finest children first: children at level L-1 .. 1.
+  for (int lev = c.levels - 1; lev >= 1; lev--) {
+    int pb = c.level_start[lev - 1], pe = c.level_start[lev];
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err)
+    for (int par = pb; par < pe; par++) {
+      const Panel& pp = c.pn[par];
+      double basis[121];
+      double* out = q_proxy + (size_t)par * c.n;
+      for (int cc = 0; cc < 4; cc++) {
+        int k = pp.child[cc];
+        double xo, xs, eo, es;
+        panel_frame(c.pn[k], xo, xs, eo, es);
+        for (int i = 0; i < c.n; i++) {
+          double xi = c.tab_d[4 * i] * xs + xo, eta = c.tab_d[4 * i + 1] * es + eo;
+          err |= bli_in_panel(pp, xi, eta, c.d, basis);
+          double qi = q_proxy[(size_t)k * c.n + i];
+          for (int j = 0; j < c.n; j++) out[j] += basis[j] * qi;
+        }
+      }
+    }
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// fmm_funcs.cpp:374-392 (downward_pass_3) with parent_to_child_3
+// (fmm_funcs_impl.hpp:138-171) and child_panel_copy (:182-187).  ncomp fields
+// phi_proxy[ncomp][P][n] are accumulated top-down; sols[ncomp][Pl][m2] are
+// assigned = L2P(leaf proxies) + phi_part (phi_part may be null).
+int orc_downward_pass(void* h, int ncomp, double* const* phi_proxy, const double* const* phi_part,
+                      double* const* sols) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+  for (int lev = 0; lev < c.levels - 1; lev++) {
+    int pb = c.level_start[lev], pe = c.level_start[lev + 1];
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err)
+    for (int i = pb; i < pe; i++) {
+      const Panel& pp = c.pn[i];
+      double basis[121];
+      for (int cc = 0; cc < 4; cc++) {
+        int child = pp.child[cc];
+        double xo, xs, eo, es;
+        panel_frame(c.pn[child], xo, xs, eo, es);
+        for (int k = 0; k < c.n; k++) {
+          double xi = c.tab_d[4 * k] * xs + xo, eta = c.tab_d[4 * k + 1] * es + eo;
+          err |= bli_in_panel(pp, xi, eta, c.d, basis);
+          for (int f = 0; f < ncomp; f++) {
+            double acc = 0;
+            const double* par = phi_proxy[f] + (size_t)i * c.n;
+            for (int l = 0; l < c.n; l++) acc += basis[l] * par[l];
+            phi_proxy[f][(size_t)child * c.n + k] += acc;
+          }
+        }
+      }
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err)
+  for (int i = 0; i < c.Pl; i++) {
+    const Panel& p = c.pn[c.offset + i];
+    double xo, xs, eo, es, basis[121];
+    panel_frame(p, xo, xs, eo, es);
+    for (int j = 0; j < c.m2; j++) {
+      size_t o = (size_t)i * c.m2 + j;
+      if (c.md == c.d) {
+        for (int f = 0; f < ncomp; f++) sols[f][o] = phi_proxy[f][(size_t)(c.offset + i) * c.n + j];
+      } else {
+        double xi = c.tab_m[4 * j] * xs + xo, eta = c.tab_m[4 * j + 1] * es + eo;
+        err |= bli_in_panel(p, xi, eta, c.d, basis);
+        for (int f = 0; f < ncomp; f++) {
+          double acc = 0;
+          const double* lp = phi_proxy[f] + (size_t)(c.offset + i) * c.n;
+          for (int k = 0; k < c.n; k++) acc += basis[k] * lp[k];
+          sols[f][o] = acc;
+        }
+      }
+      if (phi_part)
+        for (int f = 0; f < ncomp; f++) sols[f][o] += phi_part[f][o];
+    }
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// Point set of one side of an interaction: particles of a leaf (types 0/1 for
+// targets, 0/2 for sources) or the proxies of a panel.
+struct Side {
+  int count, deg;
+  const double* tab;
+  bool particles;
+};
+
+static inline Side side_of(const Ctx& c, int panel, bool particles) {
+  Side s;
+  s.particles = particles;
+  s.count = particles ? c.m2 : c.n;
+  s.deg = particles ? c.md : c.d;
+  s.tab = particles ? c.tab_m.data() : c.tab_d.data();
+  (void)panel;
+  return s;
+}
+
+// fmm_interactions/bve_vel.cpp:9-37 + bve_vel_impl.hpp:71-106, with the
+// type-dependent point sets of swe_vel_impl.hpp:188-255.  Sources sit at
+// P(node + disp) (bve_vel_impl.hpp:90-97); targets are undisplaced.
+// phi_part[3][Pl][m2] and phi_proxy[3][P][n] are accumulated.
+void orc_bve_vel_interactions(void* h, const double* const* disp_part, const double* const* disp_proxy,
+                              const double* q_part, const double* q_proxy, double eps,
+                              double* const* phi_part, double* const* phi_proxy) {
+  Ctx& c = *static_cast<Ctx*>(h);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < c.P; t++) {
+    for (int e = c.tgt_start[t]; e < c.tgt_start[t + 1]; e++) {
+      int ii = c.by_target[e];
+      int source = c.ilist[3 * ii + 1], type = c.ilist[3 * ii + 2];
+      bool tpart = (type == 0 || type == 1), spart = (type == 0 || type == 2);
+      Side ts = side_of(c, t, tpart), ss = side_of(c, source, spart);
+      const Panel &pt = c.pn[t], &ps = c.pn[source];
+      double xot, xst, eot, est, xos, xss, eos, ess;
+      panel_frame(pt, xot, xst, eot, est);
+      panel_frame(ps, xos, xss, eos, ess);
+      double* o1; double* o2; double* o3;
+      if (tpart) {
+        size_t b = (size_t)(t - c.offset) * c.m2;
+        o1 = phi_part[0] + b; o2 = phi_part[1] + b; o3 = phi_part[2] + b;
+      } else {
+        size_t b = (size_t)t * c.n;
+        o1 = phi_proxy[0] + b; o2 = phi_proxy[1] + b; o3 = phi_proxy[2] + b;
+      }
+      const double *sq, *dx, *dy, *dz;
+      if (spart) {
+        size_t b = (size_t)(source - c.offset) * c.m2;
+        sq = q_part + b; dx = disp_part[0] + b; dy = disp_part[1] + b; dz = disp_part[2] + b;
+      } else {
+        size_t b = (size_t)source * c.n;
+        sq = q_proxy + b; dx = disp_proxy[0] + b; dy = disp_proxy[1] + b; dz = disp_proxy[2] + b;
+      }
+      double xt[3], xs_[3];
+      for (int j = 0; j < ts.count; j++) {
+        double xi_t = ts.tab[4 * j] * xst + xot, eta_t = ts.tab[4 * j + 1] * est + eot;
+        xyz_from_xieta(xi_t, eta_t, pt.face, xt);
+        double a1 = 0, a2 = 0, a3 = 0;
+        for (int k = 0; k < ss.count; k++) {
+          double xi_s = ss.tab[4 * k] * xss + xos, eta_s = ss.tab[4 * k + 1] * ess + eos;
+          xyz_from_xieta(xi_s, eta_s, ps.face, xs_);
+          double x_s = xs_[0] + dx[k], y_s = xs_[1] + dy[k], z_s = xs_[2] + dz[k];
+          project_to_sphere(x_s, y_s, z_s);
+          double dp = xt[0] * x_s + xt[1] * y_s + xt[2] * z_s;
+          double g = kNegInv4Pi / (1.0 - dp + eps) * sq[k];
+          a1 += g * (xt[1] * z_s - xt[2] * y_s);
+          a2 += g * (xt[2] * x_s - xt[0] * z_s);
+          a3 += g * (xt[0] * y_s - xt[1] * x_s);
+        }
+        o1[j] += a1; o2[j] += a2; o3[j] += a3;
+      }
+    }
+  }
+}
+
+// fmm_interactions/inverse_poisson.cpp:9-33 + inverse_poisson_impl.hpp:181-210,
+// with the type-dependent point sets.  psi_part[Pl][m2], psi_proxy[P][n] accumulated.
+void orc_poisson_fmm_interactions(void* h, const double* q_part, const double* q_proxy,
+                                  double* psi_part, double* psi_proxy) {
+  Ctx& c = *static_cast<Ctx*>(h);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < c.P; t++) {
+    for (int e = c.tgt_start[t]; e < c.tgt_start[t + 1]; e++) {
+      int ii = c.by_target[e];
+      int source = c.ilist[3 * ii + 1], type = c.ilist[3 * ii + 2];
+      bool tpart = (type == 0 || type == 1), spart = (type == 0 || type == 2);
+      Side ts = side_of(c, t, tpart), ss = side_of(c, source, spart);
+      const Panel &pt = c.pn[t], &ps = c.pn[source];
+      double xot, xst, eot, est, xos, xss, eos, ess;
+      panel_frame(pt, xot, xst, eot, est);
+      panel_frame(ps, xos, xss, eos, ess);
+      double* o = tpart ? psi_part + (size_t)(t - c.offset) * c.m2 : psi_proxy + (size_t)t * c.n;
+      const double* sq = spart ? q_part + (size_t)(source - c.offset) * c.m2 : q_proxy + (size_t)source * c.n;
+      double xt[3], xs_[3];
+      for (int j = 0; j < ts.count; j++) {
+        xyz_from_xieta(ts.tab[4 * j] * xst + xot, ts.tab[4 * j + 1] * est + eot, pt.face, xt);
+        double acc = 0;
+        for (int k = 0; k < ss.count; k++) {
+          xyz_from_xieta(ss.tab[4 * k] * xss + xos, ss.tab[4 * k + 1] * ess + eos, ps.face, xs_);
+          double dp = xt[0] * xs_[0] + xt[1] * xs_[1] + xt[2] * xs_[2];
+          if (dp < 1.0 - 1e-15) acc += kNegInv4Pi * std::log(1 - dp) * sq[k];
+        }
+        o[j] += acc;
+      }
+    }
+  }
+}
+
+// direct_sum.cpp:8-15 + direct_sum_impl.hpp:18-27: soln[i] += sum_{j != i}
+// -1/(4 pi) log(1 - x_i.x_j) area_j pots_j for i in [lb, ub).
+void orc_direct_sum_inv_lap(int64_t N, const double* x, const double* y, const double* z,
+                            const double* area, const double* pots, double* soln, int64_t lb, int64_t ub) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = lb; i < ub; i++) {
+    double acc = 0;
+    for (int64_t j = 0; j < N; j++) {
+      if (i != j) {
+        double dp = x[i] * x[j] + y[i] * y[j] + z[i] * z[j];
+        acc += kNegInv4Pi * std::log(1 - dp) * area[j] * pots[j];
+      }
+    }
+    soln[i] += acc;
+  }
+}
+
+// Biot-Savart direct sum with the pair kernel of bve_vel_impl.hpp:90-103 (no
+// reference counterpart: the reference has only the log direct sum).  Sources
+// optionally displaced (disp may be null); vel[3][N] accumulated.
+void orc_direct_sum_bve_vel(int64_t N, const double* x, const double* y, const double* z,
+                            const double* area, const double* vor, const double* const* disp, double eps,
+                            double* const* vel, int64_t lb, int64_t ub) {
+  std::vector<double> sx(N), sy(N), sz(N);
+  for (int64_t j = 0; j < N; j++) {
+    double a = x[j], b = y[j], cc = z[j];
+    if (disp) { a += disp[0][j]; b += disp[1][j]; cc += disp[2][j]; project_to_sphere(a, b, cc); }
+    sx[j] = a; sy[j] = b; sz[j] = cc;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = lb; i < ub; i++) {
+    double a1 = 0, a2 = 0, a3 = 0;
+    for (int64_t j = 0; j < N; j++) {
+      double dp = x[i] * sx[j] + y[i] * sy[j] + z[i] * sz[j];
+      double g = kNegInv4Pi / (1.0 - dp + eps) * area[j] * vor[j];
+      a1 += g * (y[i] * sz[j] - z[i] * sy[j]);
+      a2 += g * (z[i] * sx[j] - x[i] * sz[j]);
+      a3 += g * (x[i] * sy[j] - y[i] * sx[j]);
+    }
+    vel[0][i] += a1; vel[1][i] += a2; vel[2][i] += a3;
+  }
+}
+
+// Log direct sum at separate target points; pairs with x.y >= 1 - 1e-15 are
+// skipped (the FMM kernel's guard, inverse_poisson_impl.hpp:47).
+void orc_direct_sum_inv_lap_targets(int64_t NT, const double* tx, const double* ty, const double* tz,
+                                    int64_t NS, const double* sx, const double* sy, const double* sz,
+                                    const double* area, const double* pots, double* soln) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < NT; i++) {
+    double acc = 0;
+    for (int64_t j = 0; j < NS; j++) {
+      double dp = tx[i] * sx[j] + ty[i] * sy[j] + tz[i] * sz[j];
+      if (dp < 1.0 - 1e-15) acc += kNegInv4Pi * std::log(1 - dp) * area[j] * pots[j];
+    }
+    soln[i] += acc;
+  }
+}
+
+// bve_time_step.cpp:91-104 (bve_disp_update) and :131-159 (bve_disp_interp).
+// Leaf particles: new_vor = vor + 2 Omega z - 2 Omega z', disp_part = tau v.
+// Non-leaf proxies: disp = BLI (particle-grid degree) of the leaf particle
+// displacement at the undisplaced proxy node, in the leaf found by point location.
+int orc_bve_displacement(void* h, const double* x, const double* y, const double* z,
+                         const double* const* vel, const double* vor, double tau, double omega,
+                         double* new_vor, double* const* disp_part, double* const* disp_proxy) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+  for (size_t k = 0; k < (size_t)c.Pl * c.m2; k++) {
+    double a = x[k] + vel[0][k] * tau, b = y[k] + vel[1][k] * tau, cc = z[k] + vel[2][k] * tau;
+    project_to_sphere(a, b, cc);
+    new_vor[k] = vor[k] + 2 * omega * z[k] - 2 * omega * cc;
+    disp_part[0][k] = tau * vel[0][k];
+    disp_part[1][k] = tau * vel[1][k];
+    disp_part[2][k] = tau * vel[2][k];
+  }
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err)
+  for (int i = 0; i < c.offset; i++) {
+    double xo, xs, eo, es, xyz[3], basis[121];
+    panel_frame(c.pn[i], xo, xs, eo, es);
+    for (int j = 0; j < c.n; j++) {
+      double xi = c.tab_d[4 * j] * xs + xo, eta = c.tab_d[4 * j + 1] * es + eo;
+      xyz_from_xieta(xi, eta, c.pn[i].face, xyz);
+      int leaf = point_locate_panel(c, xyz[0], xyz[1], xyz[2]);
+      double a[3] = {0, 0, 0};
+      if (leaf < 0) { err |= ORC_E_GEOMETRY; }
+      else {
+        err |= bli_in_panel(c.pn[leaf], xi, eta, c.md, basis);
+        size_t b = (size_t)(leaf - c.offset) * c.m2;
+        for (int k = 0; k < c.m2; k++)
+          for (int f = 0; f < 3; f++) a[f] += disp_part[f][b + k] * basis[k];
+      }
+      for (int f = 0; f < 3; f++) disp_proxy[f][(size_t)i * c.n + j] = a[f];
+    }
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// Locate + BLI-interpolate nf fields [Pl][m2] at a point (bve_time_step.cpp:220-237).
+static int interp_at(const Ctx& c, double x, double y, double z, int nf, const double* const* fields,
+                     double* out) {
+  double xieta[2], basis[121];
+  xieta_from_xyz(x, y, z, xieta);
+  int leaf = point_locate_panel(c, x, y, z);
+  for (int f = 0; f < nf; f++) out[f] = 0;
+  if (leaf < 0) return ORC_E_GEOMETRY;
+  int err = bli_in_panel(c.pn[leaf], xieta[0], xieta[1], c.md, basis);
+  size_t b = (size_t)(leaf - c.offset) * c.m2;
+  for (int k = 0; k < c.m2; k++)
+    for (int f = 0; f < nf; f++) out[f] += fields[f][b + k] * basis[k];
+  return err;
+}
+
+// bve_time_step.cpp:206-308.  vels = {v0x,v0y,v0z, v1x.., v2x.., v3x..}.
+// combine 0: the reference's sequential combine (:288-303: x - dt/6 k1, then
+// + dt/3 k2, + dt/3 k3, + dt/6 k4, projecting after each); 1: textbook
+// x - dt/6 (k1 + 2 k2 + 2 k3 + k4), projected once.
+int orc_bve_find_departure_points(void* h, const double* x, const double* y, const double* z,
+                                  const double* const* vels, double dt, int combine,
+                                  double* dx, double* dy, double* dz) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+  size_t N = (size_t)c.Pl * c.m2;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : err)
+  for (size_t k = 0; k < N; k++) {
+    double k1[3] = {vels[9][k], vels[10][k], vels[11][k]}, k2[3], k3[3], k4[3];
+    double a = x[k] - 0.5 * dt * k1[0], b = y[k] - 0.5 * dt * k1[1], cc = z[k] - 0.5 * dt * k1[2];
+    project_to_sphere(a, b, cc);
+    err |= interp_at(c, a, b, cc, 3, vels + 6, k2);
+    a = x[k] - 0.5 * dt * k2[0]; b = y[k] - 0.5 * dt * k2[1]; cc = z[k] - 0.5 * dt * k2[2];
+    project_to_sphere(a, b, cc);
+    err |= interp_at(c, a, b, cc, 3, vels + 3, k3);
+    a = x[k] - dt * k3[0]; b = y[k] - dt * k3[1]; cc = z[k] - dt * k3[2];
+    project_to_sphere(a, b, cc);
+    err |= interp_at(c, a, b, cc, 3, vels + 0, k4);
+    if (combine == 0) {
+      a = x[k] - 1.0 / 6.0 * dt * k1[0]; b = y[k] - 1.0 / 6.0 * dt * k1[1]; cc = z[k] - 1.0 / 6.0 * dt * k1[2];
+      project_to_sphere(a, b, cc);
+      a -= -1.0 / 3.0 * dt * k2[0]; b -= -1.0 / 3.0 * dt * k2[1]; cc -= -1.0 / 3.0 * dt * k2[2];
+      project_to_sphere(a, b, cc);
+      a -= -1.0 / 3.0 * dt * k3[0]; b -= -1.0 / 3.0 * dt * k3[1]; cc -= -1.0 / 3.0 * dt * k3[2];
+      project_to_sphere(a, b, cc);
+      a -= -1.0 / 6.0 * dt * k4[0]; b -= -1.0 / 6.0 * dt * k4[1]; cc -= -1.0 / 6.0 * dt * k4[2];
+      project_to_sphere(a, b, cc);
+    } else {
+      a = x[k] - dt / 6.0 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0]);
+      b = y[k] - dt / 6.0 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1]);
+      cc = z[k] - dt / 6.0 * (k1[2] + 2 * k2[2] + 2 * k3[2] + k4[2]);
+      project_to_sphere(a, b, cc);
+    }
+    dx[k] = a; dy[k] = b; dz[k] = cc;
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// bve_time_step.cpp:38-66: new_vor = BLI[vor](dep) + 2 Omega dep_z - 2 Omega z;
+// tracers[Pl][m2][T] interpolated without the Coriolis terms.
+int orc_bve_departure_to_target(void* h, const double* z, const double* dx, const double* dy, const double* dz,
+                                const double* vor, const double* tracers, int ntr, double omega,
+                                double* new_vor, double* new_tracers) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+  size_t N = (size_t)c.Pl * c.m2;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : err)
+  for (size_t k = 0; k < N; k++) {
+    double xieta[2], basis[121];
+    xieta_from_xyz(dx[k], dy[k], dz[k], xieta);
+    int leaf = point_locate_panel(c, dx[k], dy[k], dz[k]);
+    double v = 0;
+    for (int t = 0; t < ntr; t++) new_tracers[k * ntr + t] = 0;
+    if (leaf < 0) { err |= ORC_E_GEOMETRY; new_vor[k] = 0; continue; }
+    err |= bli_in_panel(c.pn[leaf], xieta[0], xieta[1], c.md, basis);
+    size_t b = (size_t)(leaf - c.offset) * c.m2;
+    for (int j = 0; j < c.m2; j++) {
+      v += vor[b + j] * basis[j];
+      for (int t = 0; t < ntr; t++) new_tracers[k * ntr + t] += tracers[(b + j) * ntr + t] * basis[j];
+    }
+    v += 2 * omega * dz[k];
+    new_vor[k] = v - 2 * omega * z[k];
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// One FMM velocity evaluation: upward pass, interactions, downward pass
+// (bve_time_step.cpp:321-344).  vel[3][Pl][m2] assigned.
+static int fmm_velocity(Ctx& c, const double* area, const double* vor_eff, const double* const* disp_part,
+                        const double* const* disp_proxy, double eps, double* const* vel) {
+  size_t NP = (size_t)c.Pl * c.m2, NQ = (size_t)c.P * c.n;
+  std::vector<double> q_part(NP), q_proxy(NQ), pp(3 * NP, 0.0), px(3 * NQ, 0.0);
+  int err = orc_upward_pass(&c, area, vor_eff, q_part.data(), q_proxy.data());
+  double* php[3] = {pp.data(), pp.data() + NP, pp.data() + 2 * NP};
+  double* phx[3] = {px.data(), px.data() + NQ, px.data() + 2 * NQ};
+  orc_bve_vel_interactions(&c, disp_part, disp_proxy, q_part.data(), q_proxy.data(), eps, php, phx);
+  const double* cphp[3] = {php[0], php[1], php[2]};
+  err |= orc_downward_pass(&c, 3, phx, cphp, vel);
+  return err;
+}
+
+// bve_back_rk4_step, bve_time_step.cpp:311-419 (no forcing: bve_forcing is a
+// copy, forcing_funcs.cpp:93-95).  vor[Pl][m2] and tracers[Pl][m2][T] are
+// updated in place; vel[3][Pl][m2] receives the stage-0 velocity.
+int orc_bve_back_rk4_step(void* h, const double* x, const double* y, const double* z, const double* area,
+                          double* vor, double* tracers, int ntr, double dt, double omega, double eps,
+                          int combine, double* const* vel) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  size_t NP = (size_t)c.Pl * c.m2, NQ = (size_t)c.P * c.n;
+  std::vector<double> dpart(3 * NP, 0.0), dprox(3 * NQ, 0.0), vs(9 * NP), dvor(NP);
+  const double* dp3[3] = {dpart.data(), dpart.data() + NP, dpart.data() + 2 * NP};
+  const double* dx3[3] = {dprox.data(), dprox.data() + NQ, dprox.data() + 2 * NQ};
+  double* dp3w[3] = {dpart.data(), dpart.data() + NP, dpart.data() + 2 * NP};
+  double* dx3w[3] = {dprox.data(), dprox.data() + NQ, dprox.data() + 2 * NQ};
+  double* v1[3] = {vs.data(), vs.data() + NP, vs.data() + 2 * NP};
+  double* v2[3] = {vs.data() + 3 * NP, vs.data() + 4 * NP, vs.data() + 5 * NP};
+  double* v3[3] = {vs.data() + 6 * NP, vs.data() + 7 * NP, vs.data() + 8 * NP};
+  int err = fmm_velocity(c, area, vor, dp3, dx3, eps, vel);
+  const double* cv0[3] = {vel[0], vel[1], vel[2]};
+  err |= orc_bve_displacement(&c, x, y, z, cv0, vor, 0.5 * dt, omega, dvor.data(), dp3w, dx3w);
+  err |= fmm_velocity(c, area, dvor.data(), dp3, dx3, eps, v1);
+  const double* cv1[3] = {v1[0], v1[1], v1[2]};
+  err |= orc_bve_displacement(&c, x, y, z, cv1, vor, 0.5 * dt, omega, dvor.data(), dp3w, dx3w);
+  err |= fmm_velocity(c, area, dvor.data(), dp3, dx3, eps, v2);
+  const double* cv2[3] = {v2[0], v2[1], v2[2]};
+  err |= orc_bve_displacement(&c, x, y, z, cv2, vor, dt, omega, dvor.data(), dp3w, dx3w);
+  err |= fmm_velocity(c, area, dvor.data(), dp3, dx3, eps, v3);
+  const double* all[12] = {vel[0], vel[1], vel[2], v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+  std::vector<double> dep(3 * NP), nv(NP), ntr_buf((size_t)NP * (ntr > 0 ? ntr : 1));
+  err |= orc_bve_find_departure_points(&c, x, y, z, all, dt, combine, dep.data(), dep.data() + NP, dep.data() + 2 * NP);
+  err |= orc_bve_departure_to_target(&c, z, dep.data(), dep.data() + NP, dep.data() + 2 * NP, vor, tracers, ntr,
+                                     omega, nv.data(), ntr_buf.data());
+  std::memcpy(vor, nv.data(), NP * sizeof(double));
+  if (ntr > 0) std::memcpy(tracers, ntr_buf.data(), NP * ntr * sizeof(double));
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+void orc_set_threads(int n) { omp_set_num_threads(n); }
+int orc_get_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"

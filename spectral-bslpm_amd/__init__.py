@@ -1,0 +1,6 @@
+"""spectral-BSLPM hot path on MI355X (gfx950): cubed-sphere BLI FMM, direct
+sums and the backward semi-Lagrangian BVE step, as hand-written HIP kernels
+behind a C ABI (include/sbslpm.h, libsbslpm.so)."""
+from ._lib import SbslpmError, exported_symbols, load  # noqa: F401
+from .api import Context, HostTree, RunConfig  # noqa: F401
+from . import workloads  # noqa: F401

@@ -1,0 +1,102 @@
+"""ctypes binding of libsbslpm.so (declarations mirror include/sbslpm.h).
+
+The library is the only compute path: if it is missing this module raises,
+there is no fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsbslpm.so")
+
+SBSLPM_OK, SBSLPM_E_ARG, SBSLPM_E_GEOMETRY, SBSLPM_E_HIP, SBSLPM_E_NOMEM = 0, 1, 2, 3, 4
+ERROR_NAMES = {1: "SBSLPM_E_ARG", 2: "SBSLPM_E_GEOMETRY", 3: "SBSLPM_E_HIP", 4: "SBSLPM_E_NOMEM"}
+
+
+class SbslpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_int),
+        ("interp_degree", ctypes.c_int),
+        ("leaf_degree", ctypes.c_int),
+        ("weight_rule", ctypes.c_int),
+        ("rk4_combine", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("tracer_count", ctypes.c_int),
+        ("flags", ctypes.c_int),
+        ("theta", ctypes.c_double),
+        ("kernel_eps", ctypes.c_double),
+        ("omega", ctypes.c_double),
+        ("delta_t", ctypes.c_double),
+    ]
+
+
+_V = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes); every function returns an int status unless noted
+SIGNATURES = {
+    "sbslpm_tree_build": (_I, [ctypes.POINTER(Params), ctypes.POINTER(_V)]),
+    "sbslpm_tree_destroy": (None, [_V]),
+    "sbslpm_tree_sizes": (_I, [_V, _PI64]),
+    "sbslpm_tree_spec_points": (_I, [_V, _PD, _PD, _PD, _PD]),
+    "sbslpm_tree_interaction_list": (_I, [_V, _PI]),
+    "sbslpm_tree_panels": (_I, [_V, _PI, _PD]),
+    "sbslpm_tree_locate": (_I, [_V, _D, _D, _D]),
+    "sbslpm_tree_transfer_matrices": (_I, [_V, _PD, _PD, _PD]),
+    "sbslpm_ctx_create": (_I, [ctypes.POINTER(Params), ctypes.POINTER(_V)]),
+    "sbslpm_ctx_destroy": (None, [_V]),
+    "sbslpm_ctx_set_stream": (_I, [_V, _V]),
+    "sbslpm_ctx_set_precision": (_I, [_V, _I]),
+    "sbslpm_last_error": (ctypes.c_char_p, [_V]),
+    "sbslpm_synchronize": (_I, [_V]),
+    "sbslpm_get_sizes": (_I, [_V, _PI64]),
+    "sbslpm_spec_points": (_I, [_V, _PD, _PD, _PD, _PD]),
+    "sbslpm_interaction_list": (_I, [_V, _PI]),
+    "sbslpm_upward_pass": (_I, [_V, _V, _V, _V, _V]),
+    "sbslpm_bve_vel_interactions": (_I, [_V] + [_V] * 14),
+    "sbslpm_poisson_fmm_interactions": (_I, [_V, _V, _V, _V, _V]),
+    "sbslpm_downward_pass": (_I, [_V, _V, _V, _V]),
+    "sbslpm_downward_pass_3": (_I, [_V] + [_V] * 9),
+    "sbslpm_bve_velocity": (_I, [_V] + [_V] * 11),
+    "sbslpm_poisson_solve": (_I, [_V, _V, _V, _V]),
+    "sbslpm_direct_sum_inv_lap": (_I, [_V, _I64, _V, _V, _V, _V, _V, _V, _I64, _I64]),
+    "sbslpm_direct_sum_inv_lap_targets": (_I, [_V, _I64, _V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
+    "sbslpm_direct_sum_bve_vel": (_I, [_V, _I64] + [_V] * 11 + [_I64, _I64]),
+    "sbslpm_bve_displacement": (_I, [_V] + [_V] * 7 + [_D] + [_V] * 7),
+    "sbslpm_bve_find_departure_points": (_I, [_V, _V, _V, _V, _V, _D, _V, _V, _V]),
+    "sbslpm_bve_departure_to_target": (_I, [_V, _V, _V, _V, _V, _V, _V, _I, _V, _V]),
+    "sbslpm_bve_back_rk4_step": (_I, [_V] + [_V] * 10 + [_D]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libsbslpm.so (built in-tree by build.py); raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsbslpm.so not built at {LIB_PATH}: run __graft_entry__.build() "
+                               "(or python spectral-bslpm_amd/build.py)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return sorted(SIGNATURES)

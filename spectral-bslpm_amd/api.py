@@ -1,0 +1,315 @@
+"""Python host layer over libsbslpm.so, mirroring the reference's operator API.
+
+Function names and argument meaning follow the reference's free functions
+(fmm_funcs.hpp:16-54, bve_vel.hpp:10-14, inverse_poisson.hpp:10-12,
+direct_sum.hpp:8-11, bve_time_step.hpp:7-13); arrays are torch tensors on the
+GPU (float64, contiguous, [Pl][m2] for particles, [P][n] for proxies).  Errors
+raise SbslpmError instead of aborting like Kokkos::abort / std::runtime_error.
+All work runs on the context's HIP stream, which defaults to torch's current
+stream so torch and library work are ordered.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import Params, SbslpmError, load
+
+WEIGHTS = {"reference": 0, "clenshaw_curtis": 1, "equal_area": 2}
+COMBINE = {"reference": 0, "textbook": 1}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+@dataclass
+class RunConfig:
+    """Hot-path subset of the reference RunConfig (run_config.hpp:6-44)."""
+    levels: int
+    interp_degree: int
+    leaf_degree: int = None          # particle grid degree per leaf; None -> interp_degree (reference layout)
+    fmm_theta: float = 0.7
+    kernel_eps: float = 1e-4         # namelist.txt:16
+    omega: float = 7.292e-5
+    delta_t: float = 3600.0
+    weights: str = "reference"       # reference | clenshaw_curtis | equal_area
+    rk4_combine: str = "reference"   # reference | textbook
+    precision: str = "fp64"          # fp64 | mixed
+    tracer_count: int = 0
+    use_graph: bool = False
+
+    def to_params(self):
+        p = Params()
+        p.levels = self.levels
+        p.interp_degree = self.interp_degree
+        p.leaf_degree = self.interp_degree if self.leaf_degree is None else self.leaf_degree
+        p.weight_rule = WEIGHTS[self.weights]
+        p.rk4_combine = COMBINE[self.rk4_combine]
+        p.precision = PRECISION[self.precision]
+        p.tracer_count = self.tracer_count
+        p.flags = 1 if self.use_graph else 0
+        p.theta = self.fmm_theta
+        p.kernel_eps = self.kernel_eps
+        p.omega = self.omega
+        p.delta_t = self.delta_t
+        return p
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Sizes:
+    def __init__(self, s):
+        self.P, self.Pl, self.n, self.m2, self.I, self.N, self.offset, self.tiles = [int(v) for v in s]
+
+
+class HostTree:
+    """Host-side tree, particle grid and interaction list (no GPU needed)."""
+
+    def __init__(self, cfg: RunConfig):
+        self._lib = load()
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        p = cfg.to_params()
+        rc = self._lib.sbslpm_tree_build(ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise SbslpmError(rc, "tree_build: invalid parameters")
+        self.h = h
+        s = np.zeros(8, dtype=np.int64)
+        self._lib.sbslpm_tree_sizes(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.sizes = Sizes(s)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.sbslpm_tree_destroy(self.h)
+            self.h = None
+
+    def spec_points(self):
+        s = self.sizes
+        out = [np.zeros((s.Pl, s.m2)) for _ in range(4)]
+        self._lib.sbslpm_tree_spec_points(self.h, *[_dp(a) for a in out])
+        return out
+
+    def interaction_list(self):
+        out = np.zeros((self.sizes.I, 3), dtype=np.int32)
+        self._lib.sbslpm_tree_interaction_list(self.h, _ip(out))
+        return out
+
+    def panels(self):
+        ints = np.zeros((self.sizes.P, 8), dtype=np.int32)
+        dbls = np.zeros((self.sizes.P, 6))
+        self._lib.sbslpm_tree_panels(self.h, _ip(ints), _dp(dbls))
+        return ints, dbls
+
+    def locate(self, x, y, z):
+        return self._lib.sbslpm_tree_locate(self.h, float(x), float(y), float(z))
+
+    def transfer_matrices(self):
+        d1 = self.cfg.interp_degree + 1
+        m = (self.cfg.interp_degree if self.cfg.leaf_degree is None else self.cfg.leaf_degree) + 1
+        hi, lo, p2m = np.zeros((d1, d1)), np.zeros((d1, d1)), np.zeros((d1, m))
+        self._lib.sbslpm_tree_transfer_matrices(self.h, _dp(hi), _dp(lo), _dp(p2m))
+        return hi, lo, p2m
+
+
+def _ptr(t):
+    """Device pointer of a contiguous float64 CUDA tensor (None -> NULL)."""
+    if t is None:
+        return None
+    import torch
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 tensor on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """Device context: tree, tables, interaction lists and workspace on one GPU."""
+
+    def __init__(self, cfg: RunConfig, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("sbslpm.Context needs a GPU (HIP device)")
+        self._lib = load()
+        self.cfg = cfg
+        self.torch = torch
+        h = ctypes.c_void_p()
+        p = cfg.to_params()
+        rc = self._lib.sbslpm_ctx_create(ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise SbslpmError(rc, "ctx_create failed")
+        self.h = h
+        s = np.zeros(8, dtype=np.int64)
+        self._lib.sbslpm_get_sizes(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.sizes = Sizes(s)
+        self.set_stream(stream if stream is not None else torch.cuda.current_stream())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.sbslpm_ctx_destroy(self.h)
+            self.h = None
+
+    # ---- plumbing
+    def _check(self, rc, what):
+        if rc:
+            msg = self._lib.sbslpm_last_error(self.h)
+            raise SbslpmError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_stream(self, stream):
+        self.stream = stream
+        self._check(self._lib.sbslpm_ctx_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "set_stream")
+
+    def set_precision(self, precision):
+        self._check(self._lib.sbslpm_ctx_set_precision(self.h, PRECISION[precision]), "set_precision")
+        self.cfg.precision = precision
+
+    def synchronize(self):
+        self._check(self._lib.sbslpm_synchronize(self.h), "synchronize")
+
+    def zeros_part(self, k=None):
+        shape = (self.sizes.Pl, self.sizes.m2)
+        if k is None:
+            return self.torch.zeros(shape, dtype=self.torch.float64, device="cuda")
+        return [self.torch.zeros(shape, dtype=self.torch.float64, device="cuda") for _ in range(k)]
+
+    def zeros_proxy(self, k=None):
+        shape = (self.sizes.P, self.sizes.n)
+        if k is None:
+            return self.torch.zeros(shape, dtype=self.torch.float64, device="cuda")
+        return [self.torch.zeros(shape, dtype=self.torch.float64, device="cuda") for _ in range(k)]
+
+    def spec_points(self):
+        s = self.sizes
+        out = [np.zeros((s.Pl, s.m2)) for _ in range(4)]
+        self._check(self._lib.sbslpm_spec_points(self.h, *[_dp(a) for a in out]), "spec_points")
+        return out
+
+    def interaction_list(self):
+        out = np.zeros((self.sizes.I, 3), dtype=np.int32)
+        self._check(self._lib.sbslpm_interaction_list(self.h, _ip(out)), "interaction_list")
+        return out
+
+    # ---- reference operator API
+    def upward_pass(self, area, pots, q_part=None, q_proxy=None):
+        """upward_pass, fmm_funcs.cpp:341-355."""
+        q_part = self.zeros_part() if q_part is None else q_part
+        q_proxy = self.zeros_proxy() if q_proxy is None else q_proxy
+        self._check(self._lib.sbslpm_upward_pass(self.h, _ptr(area), _ptr(pots), _ptr(q_part), _ptr(q_proxy)),
+                    "upward_pass")
+        return q_part, q_proxy
+
+    def bve_vel_interactions(self, q_part, q_proxy, disp_part=None, disp_proxy=None, phi_part=None,
+                             phi_proxy=None):
+        """bve_vel_interactions, bve_vel.cpp:9-37 (accumulates)."""
+        phi_part = self.zeros_part(3) if phi_part is None else phi_part
+        phi_proxy = self.zeros_proxy(3) if phi_proxy is None else phi_proxy
+        dp = disp_part if disp_part is not None else [None] * 3
+        dx = disp_proxy if disp_proxy is not None else [None] * 3
+        args = [_ptr(a) for a in list(dp) + list(dx) + [q_part, q_proxy] + list(phi_part) + list(phi_proxy)]
+        self._check(self._lib.sbslpm_bve_vel_interactions(self.h, *args), "bve_vel_interactions")
+        return phi_part, phi_proxy
+
+    def poisson_fmm_interactions(self, q_part, q_proxy, psi_part=None, psi_proxy=None):
+        """poisson_fmm_interactions, inverse_poisson.cpp:9-33 (accumulates)."""
+        psi_part = self.zeros_part() if psi_part is None else psi_part
+        psi_proxy = self.zeros_proxy() if psi_proxy is None else psi_proxy
+        self._check(self._lib.sbslpm_poisson_fmm_interactions(self.h, _ptr(q_part), _ptr(q_proxy), _ptr(psi_part),
+                                                              _ptr(psi_proxy)), "poisson_fmm_interactions")
+        return psi_part, psi_proxy
+
+    def downward_pass(self, phi_proxy, phi_part=None, sols=None):
+        """downward_pass, fmm_funcs.cpp:357-372."""
+        sols = self.zeros_part() if sols is None else sols
+        self._check(self._lib.sbslpm_downward_pass(self.h, _ptr(phi_proxy), _ptr(phi_part), _ptr(sols)),
+                    "downward_pass")
+        return sols
+
+    def downward_pass_3(self, phi_proxy, phi_part=None, sols=None):
+        """downward_pass_3, fmm_funcs.cpp:374-392."""
+        sols = self.zeros_part(3) if sols is None else sols
+        pp = phi_part if phi_part is not None else [None] * 3
+        args = [_ptr(a) for a in list(phi_proxy) + list(pp) + list(sols)]
+        self._check(self._lib.sbslpm_downward_pass_3(self.h, *args), "downward_pass_3")
+        return sols
+
+    def bve_velocity(self, area, vor, disp_part=None, disp_proxy=None, vel=None):
+        """Upward pass + interactions + downward pass (bve_time_step.cpp:321-344)."""
+        vel = self.zeros_part(3) if vel is None else vel
+        dp = disp_part if disp_part is not None else [None] * 3
+        dx = disp_proxy if disp_proxy is not None else [None] * 3
+        args = [_ptr(a) for a in [area, vor] + list(dp) + list(dx) + list(vel)]
+        self._check(self._lib.sbslpm_bve_velocity(self.h, *args), "bve_velocity")
+        return vel
+
+    def poisson_solve(self, area, f, psi=None):
+        """poisson_eq_fmm.cpp:129-147."""
+        psi = self.zeros_part() if psi is None else psi
+        self._check(self._lib.sbslpm_poisson_solve(self.h, _ptr(area), _ptr(f), _ptr(psi)), "poisson_solve")
+        return psi
+
+    def direct_sum_inv_lap(self, x, y, z, area, pots, soln=None, lb=0, ub=None):
+        """direct_sum_inv_lap, direct_sum.cpp:8-15 (1-D arrays, accumulates)."""
+        n = x.numel()
+        ub = n if ub is None else ub
+        soln = self.torch.zeros(n, dtype=self.torch.float64, device="cuda") if soln is None else soln
+        self._check(self._lib.sbslpm_direct_sum_inv_lap(self.h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(area),
+                                                        _ptr(pots), _ptr(soln), lb, ub), "direct_sum_inv_lap")
+        return soln
+
+    def direct_sum_inv_lap_targets(self, tx, ty, tz, sx, sy, sz, area, pots, soln=None):
+        nt = tx.numel()
+        soln = self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") if soln is None else soln
+        self._check(self._lib.sbslpm_direct_sum_inv_lap_targets(self.h, nt, _ptr(tx), _ptr(ty), _ptr(tz),
+                                                                sx.numel(), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(area),
+                                                                _ptr(pots), _ptr(soln)),
+                    "direct_sum_inv_lap_targets")
+        return soln
+
+    def direct_sum_bve_vel(self, x, y, z, area, vor, disp=None, vel=None, lb=0, ub=None):
+        n = x.numel()
+        ub = n if ub is None else ub
+        vel = [self.torch.zeros(n, dtype=self.torch.float64, device="cuda") for _ in range(3)] if vel is None else vel
+        d = disp if disp is not None else [None] * 3
+        args = [_ptr(a) for a in [x, y, z, area, vor] + list(d) + list(vel)]
+        self._check(self._lib.sbslpm_direct_sum_bve_vel(self.h, n, *args, lb, ub), "direct_sum_bve_vel")
+        return vel
+
+    def bve_displacement(self, x, y, z, vel, vor, tau, new_vor=None, disp_part=None, disp_proxy=None):
+        """bve_displacement_upward_pass, bve_time_step.cpp:162-170."""
+        new_vor = self.zeros_part() if new_vor is None else new_vor
+        disp_part = self.zeros_part(3) if disp_part is None else disp_part
+        disp_proxy = self.zeros_proxy(3) if disp_proxy is None else disp_proxy
+        a = [_ptr(t) for t in [x, y, z] + list(vel) + [vor]]
+        b = [_ptr(t) for t in [new_vor] + list(disp_part) + list(disp_proxy)]
+        self._check(self._lib.sbslpm_bve_displacement(self.h, *a, float(tau), *b), "bve_displacement")
+        return new_vor, disp_part, disp_proxy
+
+    def bve_find_departure_points(self, x, y, z, vels, dt, dep=None):
+        """bve_find_departure_points, bve_time_step.cpp:172-309; vels = [v0, v1, v2, v3], each 3 tensors."""
+        dep = self.zeros_part(3) if dep is None else dep
+        flat = [t for stage in vels for t in stage]
+        arr = (ctypes.c_void_p * 12)(*[t.data_ptr() for t in flat])
+        self._check(self._lib.sbslpm_bve_find_departure_points(self.h, _ptr(x), _ptr(y), _ptr(z),
+                                                               ctypes.cast(arr, ctypes.c_void_p), float(dt),
+                                                               *[_ptr(t) for t in dep]), "departure_points")
+        return dep
+
+    def bve_departure_to_target(self, z, dep, vor, tracers=None, new_vor=None, new_tracers=None):
+        """bve_departure_to_target, bve_time_step.cpp:15-67."""
+        new_vor = self.zeros_part() if new_vor is None else new_vor
+        ntr = 0 if tracers is None else tracers.shape[-1]
+        if ntr and new_tracers is None:
+            new_tracers = self.torch.zeros_like(tracers)
+        self._check(self._lib.sbslpm_bve_departure_to_target(self.h, _ptr(z), *[_ptr(t) for t in dep], _ptr(vor),
+                                                             _ptr(tracers), ntr, _ptr(new_vor), _ptr(new_tracers)),
+                    "departure_to_target")
+        return new_vor, new_tracers
+
+    def bve_back_rk4_step(self, x, y, z, area, vors, vel, tracers=None, time=0.0, proxy_source_vors=None):
+        """bve_back_rk4_step, bve_time_step.cpp:311-419: vors/tracers updated in place, vel <- stage-0 velocity."""
+        self._check(self._lib.sbslpm_bve_back_rk4_step(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vors),
+                                                       _ptr(proxy_source_vors), *[_ptr(t) for t in vel],
+                                                       _ptr(tracers), float(time)), "bve_back_rk4_step")
+        return vors

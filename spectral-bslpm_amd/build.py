@@ -1,0 +1,54 @@
+"""Build recipe of libsbslpm.so (gfx950) with hipcc, in-tree.
+
+Every HIP translation unit is compiled with packed-FP32 instruction
+generation switched off; the host set-up (tree.cpp) is compiled with
+-ffp-contract=off so its geometry rounds exactly as written.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libsbslpm.so")
+ARCH = os.environ.get("SBSLPM_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["fmm_passes.hip", "fmm_interact.hip", "remap.hip", "direct.hip", "capi.hip"]
+HOST_SOURCES = ["tree.cpp", "tree_api.cpp"]
+
+COMMON = ["-std=c++17", "-O3", "-fPIC", "-I" + CSRC]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + p.stdout)
+    return p.stdout
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        obj = os.path.join(BUILD, s + ".o")
+        jobs.append(["hipcc", "-x", "hip", "-c", os.path.join(CSRC, s), "-o", obj] + COMMON + DEVICE)
+    for s in HOST_SOURCES:
+        obj = os.path.join(BUILD, s + ".o")
+        jobs.append(["hipcc", "-c", os.path.join(CSRC, s), "-o", obj, "-ffp-contract=off"] + COMMON)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    objs = [j[j.index("-o") + 1] for j in jobs]
+    tmp = LIB + ".tmp"
+    _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

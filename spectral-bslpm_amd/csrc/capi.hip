@@ -1,0 +1,563 @@
+// capi.hip -- the C ABI (include/sbslpm.h): context, workspace, entry points
+// and the bve_back_rk4_step orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sbslpm.h"
+#include "device.hpp"
+#include "tree.hpp"
+
+using namespace sbslpm;
+
+struct sbslpm_ctx {
+  sbslpm_params prm{};
+  Tree tree;
+  DevTree dt{};
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  int device = 0;
+  std::string err;
+
+  char* arena = nullptr;
+  TargetTile* tiles = nullptr;
+  SourceSet* sets = nullptr;
+  int ntiles = 0;
+
+  // workspace
+  SrcRec64* rec64 = nullptr;
+  SrcRec32* rec32 = nullptr;
+  double* qpart = nullptr;
+  double* qproxy = nullptr;
+  double* phipart[3] = {};
+  double* phiproxy[3] = {};
+  double* disppart[3] = {};
+  double* dispproxy[3] = {};
+  double* vstage[9] = {};  // RK4 stages 1..3
+  double* newvor = nullptr;
+  double* dep[3] = {};
+  double* vor_next = nullptr;
+  double* tr_next = nullptr;
+  double* scratch = nullptr;  // direct sums, grown on demand
+  int64_t scratch_len = 0;
+
+  // hipGraph of one BVE step, keyed by the caller's pointers.
+  hipGraphExec_t graph = nullptr;
+  const void* graph_key[10] = {};
+};
+
+namespace {
+
+#define HIP_TRY(ctx, expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      if (ctx) (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);              \
+      return SBSLPM_E_HIP;                                                                  \
+    }                                                                                       \
+  } while (0)
+
+int fail(sbslpm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+// Checks the launch status of the last kernels.
+int launched(sbslpm_ctx* c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, SBSLPM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return SBSLPM_OK;
+}
+
+struct Arena {
+  size_t off = 0;
+  template <typename T>
+  size_t take(size_t count) {
+    size_t o = (off + 255) & ~size_t(255);
+    off = o + count * sizeof(T);
+    return o;
+  }
+};
+
+int upload(sbslpm_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return SBSLPM_OK;
+  HIP_TRY(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SBSLPM_OK;
+}
+
+int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double* const* dpart,
+             const double* const* dprox, double* const* vel);
+
+}  // namespace
+
+extern "C" {
+
+int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
+  if (!p || !out) return SBSLPM_E_ARG;
+  *out = nullptr;
+  sbslpm_ctx* c = new (std::nothrow) sbslpm_ctx();
+  if (!c) return SBSLPM_E_NOMEM;
+  c->prm = *p;
+  if (p->precision != SBSLPM_FP64 && p->precision != SBSLPM_MIXED) {
+    delete c;
+    return SBSLPM_E_ARG;
+  }
+  if (p->rk4_combine != SBSLPM_RK4_REFERENCE && p->rk4_combine != SBSLPM_RK4_TEXTBOOK) {
+    delete c;
+    return SBSLPM_E_ARG;
+  }
+  if (p->tracer_count < 0 || !(p->kernel_eps >= 0)) {
+    delete c;
+    return SBSLPM_E_ARG;
+  }
+  TreeConfig cfg{p->levels, p->interp_degree, p->leaf_degree, p->theta, p->weight_rule};
+  std::string msg = build_tree(cfg, c->tree);
+  if (!msg.empty()) {
+    delete c;
+    return SBSLPM_E_ARG;
+  }
+  const Tree& t = c->tree;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return SBSLPM_E_HIP;
+  }
+
+  // Device copies of the constant tables + workspace in one allocation.
+  const size_t Pn = (size_t)t.P * t.n, NP = (size_t)t.npart, U = (size_t)t.U;
+  const int ntr = std::max(p->tracer_count, 1);
+  std::vector<double> hiT(t.n), loT(t.n), p2mT(t.dp1 * t.m);
+  for (int a = 0; a < t.dp1; ++a)
+    for (int k = 0; k < t.dp1; ++k) {
+      hiT[k * t.dp1 + a] = t.m2m_hi[a * t.dp1 + k];
+      loT[k * t.dp1 + a] = t.m2m_lo[a * t.dp1 + k];
+    }
+  for (int a = 0; a < t.dp1; ++a)
+    for (int k = 0; k < t.m; ++k) p2mT[k * t.dp1 + a] = t.p2m[a * t.m + k];
+
+  Arena A;
+  size_t o_pan = A.take<DevPanel>(t.P), o_px = A.take<double>(U), o_py = A.take<double>(U), o_pz = A.take<double>(U);
+  size_t o_chd = A.take<double>(t.dp1), o_chm = A.take<double>(t.m);
+  size_t o_hi = A.take<double>(t.n), o_lo = A.take<double>(t.n), o_hiT = A.take<double>(t.n),
+         o_loT = A.take<double>(t.n);
+  size_t o_p2m = A.take<double>(t.dp1 * t.m), o_p2mT = A.take<double>(t.dp1 * t.m);
+  size_t o_pl = A.take<int>(t.proxy_leaf.size()), o_pxe = A.take<double>(t.proxy_xieta.size());
+  size_t o_err = A.take<int>(4);
+  size_t o_tiles = A.take<TargetTile>(t.tiles.size()), o_sets = A.take<SourceSet>(t.srcsets.size());
+  size_t o_rec64 = A.take<SrcRec64>(U), o_rec32 = A.take<SrcRec32>(U);
+  size_t o_qpart = A.take<double>(NP), o_qproxy = A.take<double>(Pn);
+  size_t o_phipart = A.take<double>(3 * NP), o_phiproxy = A.take<double>(3 * Pn);
+  size_t o_disppart = A.take<double>(3 * NP), o_dispproxy = A.take<double>(3 * Pn);
+  size_t o_vstage = A.take<double>(9 * NP), o_newvor = A.take<double>(NP), o_dep = A.take<double>(3 * NP);
+  size_t o_vnext = A.take<double>(NP), o_trnext = A.take<double>(NP * ntr);
+  size_t total = A.take<char>(256);
+  if (hipMalloc(&c->arena, total) != hipSuccess) {
+    delete c;
+    return SBSLPM_E_NOMEM;
+  }
+  char* b = c->arena;
+  int rc = SBSLPM_OK;
+  rc |= upload(c, b + o_pan, t.dev.data(), t.P * sizeof(DevPanel));
+  rc |= upload(c, b + o_px, t.px.data(), U * sizeof(double));
+  rc |= upload(c, b + o_py, t.py.data(), U * sizeof(double));
+  rc |= upload(c, b + o_pz, t.pz.data(), U * sizeof(double));
+  rc |= upload(c, b + o_chd, t.cheb_d.data(), t.dp1 * sizeof(double));
+  rc |= upload(c, b + o_chm, t.cheb_m.data(), t.m * sizeof(double));
+  rc |= upload(c, b + o_hi, t.m2m_hi.data(), t.n * sizeof(double));
+  rc |= upload(c, b + o_lo, t.m2m_lo.data(), t.n * sizeof(double));
+  rc |= upload(c, b + o_hiT, hiT.data(), t.n * sizeof(double));
+  rc |= upload(c, b + o_loT, loT.data(), t.n * sizeof(double));
+  rc |= upload(c, b + o_p2m, t.p2m.data(), t.p2m.size() * sizeof(double));
+  rc |= upload(c, b + o_p2mT, p2mT.data(), p2mT.size() * sizeof(double));
+  rc |= upload(c, b + o_pl, t.proxy_leaf.data(), t.proxy_leaf.size() * sizeof(int));
+  rc |= upload(c, b + o_pxe, t.proxy_xieta.data(), t.proxy_xieta.size() * sizeof(double));
+  rc |= upload(c, b + o_tiles, t.tiles.data(), t.tiles.size() * sizeof(TargetTile));
+  rc |= upload(c, b + o_sets, t.srcsets.data(), t.srcsets.size() * sizeof(SourceSet));
+  if (rc != SBSLPM_OK || hipMemset(b + o_err, 0, 4 * sizeof(int)) != hipSuccess) {
+    hipFree(c->arena);
+    delete c;
+    return SBSLPM_E_HIP;
+  }
+  DevTree& d = c->dt;
+  d.panels = (const DevPanel*)(b + o_pan);
+  d.px = (const double*)(b + o_px); d.py = (const double*)(b + o_py); d.pz = (const double*)(b + o_pz);
+  d.cheb_d = (const double*)(b + o_chd); d.cheb_m = (const double*)(b + o_chm);
+  d.m2m_hi = (const double*)(b + o_hi); d.m2m_lo = (const double*)(b + o_lo);
+  d.m2m_hiT = (const double*)(b + o_hiT); d.m2m_loT = (const double*)(b + o_loT);
+  d.p2m = (const double*)(b + o_p2m); d.p2mT = (const double*)(b + o_p2mT);
+  d.proxy_leaf = (const int*)(b + o_pl); d.proxy_xieta = (const double*)(b + o_pxe);
+  d.err = (int*)(b + o_err);
+  d.P = t.P; d.Pl = t.Pl; d.offset = t.offset; d.n = t.n; d.dp1 = t.dp1; d.m = t.m; d.m2 = t.m2;
+  d.levels = t.cfg.levels; d.npart = t.npart; d.U = t.U;
+  c->tiles = (TargetTile*)(b + o_tiles);
+  c->sets = (SourceSet*)(b + o_sets);
+  c->ntiles = (int)t.tiles.size();
+  c->rec64 = (SrcRec64*)(b + o_rec64);
+  c->rec32 = (SrcRec32*)(b + o_rec32);
+  c->qpart = (double*)(b + o_qpart);
+  c->qproxy = (double*)(b + o_qproxy);
+  for (int f = 0; f < 3; ++f) {
+    c->phipart[f] = (double*)(b + o_phipart) + f * NP;
+    c->phiproxy[f] = (double*)(b + o_phiproxy) + f * Pn;
+    c->disppart[f] = (double*)(b + o_disppart) + f * NP;
+    c->dispproxy[f] = (double*)(b + o_dispproxy) + f * Pn;
+    c->dep[f] = (double*)(b + o_dep) + f * NP;
+  }
+  for (int k = 0; k < 9; ++k) c->vstage[k] = (double*)(b + o_vstage) + k * NP;
+  c->newvor = (double*)(b + o_newvor);
+  c->vor_next = (double*)(b + o_vnext);
+  c->tr_next = (double*)(b + o_trnext);
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    hipFree(c->arena);
+    delete c;
+    return SBSLPM_E_HIP;
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return SBSLPM_OK;
+}
+
+void sbslpm_ctx_destroy(sbslpm_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  if (c->scratch) hipFree(c->scratch);
+  if (c->arena) hipFree(c->arena);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+int sbslpm_ctx_set_stream(sbslpm_ctx* c, void* stream) {
+  if (!c) return SBSLPM_E_ARG;
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  if (c->graph) {
+    hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return SBSLPM_OK;
+}
+
+int sbslpm_ctx_set_precision(sbslpm_ctx* c, int precision) {
+  if (!c || (precision != SBSLPM_FP64 && precision != SBSLPM_MIXED)) return SBSLPM_E_ARG;
+  if (precision != c->prm.precision && c->graph) {
+    hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  c->prm.precision = precision;
+  return SBSLPM_OK;
+}
+
+const char* sbslpm_last_error(const sbslpm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int sbslpm_synchronize(sbslpm_ctx* c) {
+  if (!c) return SBSLPM_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int flag = 0;
+  HIP_TRY(c, hipMemcpy(&flag, c->dt.err, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) {
+    HIP_TRY(c, hipMemset(c->dt.err, 0, sizeof(int)));
+    return fail(c, SBSLPM_E_GEOMETRY,
+                (flag & kErrLocate) ? "point location failed (non-finite or off-sphere point)"
+                                    : "interpolation point outside its panel");
+  }
+  return SBSLPM_OK;
+}
+
+int sbslpm_get_sizes(const sbslpm_ctx* c, int64_t* s) {
+  if (!c || !s) return SBSLPM_E_ARG;
+  const Tree& t = c->tree;
+  s[0] = t.P; s[1] = t.Pl; s[2] = t.n; s[3] = t.m2; s[4] = t.interaction_count(); s[5] = t.npart;
+  s[6] = t.offset; s[7] = (int64_t)t.tiles.size();
+  return SBSLPM_OK;
+}
+
+int sbslpm_spec_points(const sbslpm_ctx* c, double* x, double* y, double* z, double* area) {
+  if (!c) return SBSLPM_E_ARG;
+  const Tree& t = c->tree;
+  size_t bytes = (size_t)t.npart * sizeof(double);
+  if (x) std::memcpy(x, t.spec_x.data(), bytes);
+  if (y) std::memcpy(y, t.spec_y.data(), bytes);
+  if (z) std::memcpy(z, t.spec_z.data(), bytes);
+  if (area) std::memcpy(area, t.spec_area.data(), bytes);
+  return SBSLPM_OK;
+}
+
+int sbslpm_interaction_list(const sbslpm_ctx* c, int* list) {
+  if (!c || !list) return SBSLPM_E_ARG;
+  std::memcpy(list, c->tree.ilist.data(), c->tree.ilist.size() * sizeof(int));
+  return SBSLPM_OK;
+}
+
+int sbslpm_upward_pass(sbslpm_ctx* c, const double* area, const double* pots, double* q_part, double* q_proxy) {
+  if (!c || !area || !pots || !q_part || !q_proxy) return fail(c, SBSLPM_E_ARG, "upward_pass: null pointer");
+  launch_upward_pass(c->dt, area, pots, q_part, q_proxy, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_vel_interactions(sbslpm_ctx* c, const double* dpx, const double* dpy, const double* dpz,
+                                const double* dxx, const double* dxy, const double* dxz, const double* q_part,
+                                const double* q_proxy, double* ppx, double* ppy, double* ppz, double* pxx,
+                                double* pxy, double* pxz) {
+  if (!c || !q_part || !q_proxy || !ppx || !ppy || !ppz || !pxx || !pxy || !pxz)
+    return fail(c, SBSLPM_E_ARG, "bve_vel_interactions: null pointer");
+  const double* dp[3] = {dpx, dpy, dpz};
+  const double* dx[3] = {dxx, dxy, dxz};
+  double* op[3] = {ppx, ppy, ppz};
+  double* ox[3] = {pxx, pxy, pxz};
+  const double* const* dpp = dpx ? dp : nullptr;
+  const double* const* dxp = dxx ? dx : nullptr;
+  if (c->prm.precision == SBSLPM_MIXED) {
+    launch_source_records32(c->dt, q_part, q_proxy, dpp, dxp, c->rec32, c->stream);
+    launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, op, ox, c->stream);
+  } else {
+    launch_source_records64(c->dt, q_part, q_proxy, dpp, dxp, true, c->rec64, c->stream);
+    launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, c->prm.kernel_eps, op, ox, c->stream);
+  }
+  return launched(c);
+}
+
+int sbslpm_poisson_fmm_interactions(sbslpm_ctx* c, const double* q_part, const double* q_proxy, double* psi_part,
+                                    double* psi_proxy) {
+  if (!c || !q_part || !q_proxy || !psi_part || !psi_proxy)
+    return fail(c, SBSLPM_E_ARG, "poisson_fmm_interactions: null pointer");
+  if (c->prm.precision == SBSLPM_MIXED) {
+    launch_source_records32(c->dt, q_part, q_proxy, nullptr, nullptr, c->rec32, c->stream);
+    launch_poisson_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, psi_part, psi_proxy, c->stream);
+  } else {
+    // the Poisson kernel reads the node coordinates unprojected (inverse_poisson_impl.hpp:199-204)
+    launch_source_records64(c->dt, q_part, q_proxy, nullptr, nullptr, false, c->rec64, c->stream);
+    launch_poisson_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, psi_part, psi_proxy, c->stream);
+  }
+  return launched(c);
+}
+
+int sbslpm_downward_pass(sbslpm_ctx* c, double* phi_proxy, const double* phi_part, double* sols) {
+  if (!c || !phi_proxy || !sols) return fail(c, SBSLPM_E_ARG, "downward_pass: null pointer");
+  double* px[1] = {phi_proxy};
+  const double* pp[1] = {phi_part};
+  double* so[1] = {sols};
+  launch_downward_pass(c->dt, 1, px, pp, so, c->stream);
+  return launched(c);
+}
+
+int sbslpm_downward_pass_3(sbslpm_ctx* c, double* p1, double* p2, double* p3, const double* q1, const double* q2,
+                           const double* q3, double* s1, double* s2, double* s3) {
+  if (!c || !p1 || !p2 || !p3 || !s1 || !s2 || !s3) return fail(c, SBSLPM_E_ARG, "downward_pass_3: null pointer");
+  double* px[3] = {p1, p2, p3};
+  const double* pp[3] = {q1, q2, q3};
+  double* so[3] = {s1, s2, s3};
+  launch_downward_pass(c->dt, 3, px, pp, so, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_velocity(sbslpm_ctx* c, const double* area, const double* vor, const double* dpx, const double* dpy,
+                        const double* dpz, const double* dxx, const double* dxy, const double* dxz, double* vx,
+                        double* vy, double* vz) {
+  if (!c || !area || !vor || !vx || !vy || !vz) return fail(c, SBSLPM_E_ARG, "bve_velocity: null pointer");
+  const double* dp[3] = {dpx, dpy, dpz};
+  const double* dx[3] = {dxx, dxy, dxz};
+  double* v[3] = {vx, vy, vz};
+  return velocity(c, area, vor, dpx ? dp : nullptr, dxx ? dx : nullptr, v);
+}
+
+int sbslpm_poisson_solve(sbslpm_ctx* c, const double* area, const double* f, double* psi) {
+  if (!c || !area || !f || !psi) return fail(c, SBSLPM_E_ARG, "poisson_solve: null pointer");
+  const Tree& t = c->tree;
+  launch_upward_pass(c->dt, area, f, c->qpart, c->qproxy, c->stream);
+  launch_fill_zero(c->phipart[0], t.npart, c->stream);
+  launch_fill_zero(c->phiproxy[0], (int64_t)t.P * t.n, c->stream);
+  int rc = sbslpm_poisson_fmm_interactions(c, c->qpart, c->qproxy, c->phipart[0], c->phiproxy[0]);
+  if (rc) return rc;
+  return sbslpm_downward_pass(c, c->phiproxy[0], c->phipart[0], psi);
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_scratch(sbslpm_ctx* c, int64_t need) {
+  if (need <= c->scratch_len) return SBSLPM_OK;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->scratch) hipFree(c->scratch);
+  c->scratch = nullptr;
+  c->scratch_len = 0;
+  HIP_TRY(c, hipMalloc(&c->scratch, need * sizeof(double)));
+  c->scratch_len = need;
+  return SBSLPM_OK;
+}
+
+int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double* const* dpart,
+             const double* const* dprox, double* const* vel) {
+  const Tree& t = c->tree;
+  const int64_t Pn = (int64_t)t.P * t.n;
+  launch_upward_pass(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
+  for (int f = 0; f < 3; ++f) {
+    launch_fill_zero(c->phipart[f], t.npart, c->stream);
+    launch_fill_zero(c->phiproxy[f], Pn, c->stream);
+  }
+  if (c->prm.precision == SBSLPM_MIXED) {
+    launch_source_records32(c->dt, c->qpart, c->qproxy, dpart, dprox, c->rec32, c->stream);
+    launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, c->phipart,
+                              c->phiproxy, c->stream);
+  } else {
+    launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->rec64, c->stream);
+    launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, c->prm.kernel_eps, c->phipart,
+                          c->phiproxy, c->stream);
+  }
+  const double* pp[3] = {c->phipart[0], c->phipart[1], c->phipart[2]};
+  launch_downward_pass(c->dt, 3, c->phiproxy, pp, vel, c->stream);
+  return launched(c);
+}
+
+// bve_back_rk4_step body (bve_time_step.cpp:318-418) on the context stream.
+int bve_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area, double* vor,
+                  double* const* vel0, double* tracers) {
+  const Tree& t = c->tree;
+  const double dt = c->prm.delta_t, om = c->prm.omega;
+  double* const* v1 = c->vstage;
+  double* const* v2 = c->vstage + 3;
+  double* const* v3 = c->vstage + 6;
+  const double* cdp[3] = {c->disppart[0], c->disppart[1], c->disppart[2]};
+  const double* cdx[3] = {c->dispproxy[0], c->dispproxy[1], c->dispproxy[2]};
+  int rc = velocity(c, area, vor, nullptr, nullptr, vel0);
+  const double* stage_in[3][3] = {{vel0[0], vel0[1], vel0[2]}, {v1[0], v1[1], v1[2]}, {v2[0], v2[1], v2[2]}};
+  double* const* stage_out[3] = {v1, v2, v3};
+  const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
+  for (int s = 0; s < 3 && rc == SBSLPM_OK; ++s) {
+    launch_bve_displacement(c->dt, x, y, z, stage_in[s], vor, taus[s], om, c->newvor, c->disppart, c->dispproxy,
+                            c->stream);
+    rc = velocity(c, area, c->newvor, cdp, cdx, stage_out[s]);
+  }
+  if (rc) return rc;
+  const double* vels[12] = {vel0[0], vel0[1], vel0[2], v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+  launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, c->dep[0], c->dep[1], c->dep[2], c->stream);
+  const int ntr = c->prm.tracer_count;
+  launch_bve_departure_to_target(c->dt, z, c->dep[0], c->dep[1], c->dep[2], vor, tracers, ntr, om, c->vor_next,
+                                 c->tr_next, c->stream);
+  launch_copy(vor, c->vor_next, t.npart, c->stream);
+  if (ntr > 0) launch_copy(tracers, c->tr_next, t.npart * ntr, c->stream);
+  return launched(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbslpm_direct_sum_inv_lap(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                              const double* area, const double* pots, double* soln, int64_t lb, int64_t ub) {
+  if (!c || !x || !y || !z || !area || !pots || !soln || n < 0 || lb < 0 || ub > n || lb > ub)
+    return fail(c, SBSLPM_E_ARG, "direct_sum_inv_lap: bad argument");
+  int64_t nt = ub - lb;
+  if (nt == 0) return SBSLPM_OK;
+  int rc = ensure_scratch(c, direct_partials_size(nt, n, 1));
+  if (rc) return rc;
+  launch_direct_inv_lap(nt, x + lb, y + lb, z + lb, n, x, y, z, area, pots, lb, false, c->scratch, soln + lb,
+                        c->stream);
+  return launched(c);
+}
+
+int sbslpm_direct_sum_inv_lap_targets(sbslpm_ctx* c, int64_t nt, const double* tx, const double* ty,
+                                      const double* tz, int64_t ns, const double* sx, const double* sy,
+                                      const double* sz, const double* area, const double* pots, double* soln) {
+  if (!c || !tx || !ty || !tz || !sx || !sy || !sz || !area || !pots || !soln || nt < 0 || ns < 0)
+    return fail(c, SBSLPM_E_ARG, "direct_sum_inv_lap_targets: bad argument");
+  if (nt == 0) return SBSLPM_OK;
+  int rc = ensure_scratch(c, direct_partials_size(nt, ns, 1));
+  if (rc) return rc;
+  launch_direct_inv_lap(nt, tx, ty, tz, ns, sx, sy, sz, area, pots, 0, true, c->scratch, soln, c->stream);
+  return launched(c);
+}
+
+int sbslpm_direct_sum_bve_vel(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                              const double* area, const double* vor, const double* dx, const double* dy,
+                              const double* dz, double* vx, double* vy, double* vz, int64_t lb, int64_t ub) {
+  if (!c || !x || !y || !z || !area || !vor || !vx || !vy || !vz || n < 0 || lb < 0 || ub > n || lb > ub)
+    return fail(c, SBSLPM_E_ARG, "direct_sum_bve_vel: bad argument");
+  if ((dx || dy || dz) && !(dx && dy && dz)) return fail(c, SBSLPM_E_ARG, "direct_sum_bve_vel: partial disp");
+  int64_t nt = ub - lb;
+  if (nt == 0) return SBSLPM_OK;
+  int rc = ensure_scratch(c, direct_partials_size(nt, n, 3));
+  if (rc) return rc;
+  const double* disp[3] = {dx, dy, dz};
+  double* vel[3] = {vx + lb, vy + lb, vz + lb};
+  launch_direct_bve_vel(nt, x + lb, y + lb, z + lb, n, x, y, z, area, vor, dx ? disp : nullptr, c->prm.kernel_eps,
+                        c->scratch, vel, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_displacement(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* vx,
+                            const double* vy, const double* vz, const double* vor, double tau, double* new_vor,
+                            double* dpx, double* dpy, double* dpz, double* dxx, double* dxy, double* dxz) {
+  if (!c || !x || !y || !z || !vx || !vy || !vz || !vor || !new_vor || !dpx || !dpy || !dpz || !dxx || !dxy || !dxz)
+    return fail(c, SBSLPM_E_ARG, "bve_displacement: null pointer");
+  const double* v[3] = {vx, vy, vz};
+  double* dp[3] = {dpx, dpy, dpz};
+  double* dxp[3] = {dxx, dxy, dxz};
+  launch_bve_displacement(c->dt, x, y, z, v, vor, tau, c->prm.omega, new_vor, dp, dxp, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_find_departure_points(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                     const double* const* vels, double dt, double* dep_x, double* dep_y,
+                                     double* dep_z) {
+  if (!c || !x || !y || !z || !vels || !dep_x || !dep_y || !dep_z)
+    return fail(c, SBSLPM_E_ARG, "bve_find_departure_points: null pointer");
+  for (int i = 0; i < 12; ++i)
+    if (!vels[i]) return fail(c, SBSLPM_E_ARG, "bve_find_departure_points: null velocity");
+  launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, dep_x, dep_y, dep_z, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_departure_to_target(sbslpm_ctx* c, const double* z, const double* dep_x, const double* dep_y,
+                                   const double* dep_z, const double* vor, const double* tracers, int ntr,
+                                   double* new_vor, double* new_tracers) {
+  if (!c || !z || !dep_x || !dep_y || !dep_z || !vor || !new_vor || ntr < 0 || (ntr > 0 && (!tracers || !new_tracers)))
+    return fail(c, SBSLPM_E_ARG, "bve_departure_to_target: bad argument");
+  launch_bve_departure_to_target(c->dt, z, dep_x, dep_y, dep_z, vor, tracers, ntr, c->prm.omega, new_vor,
+                                 new_tracers, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
+                             double* vor, double* proxy_source_vors, double* vx, double* vy, double* vz,
+                             double* tracers, double time) {
+  (void)time;
+  (void)proxy_source_vors;  // the context owns its proxy charge workspace
+  if (!c || !x || !y || !z || !area || !vor || !vx || !vy || !vz || (c->prm.tracer_count > 0 && !tracers))
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step: bad argument");
+  double* vel0[3] = {vx, vy, vz};
+  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH)) return bve_step_body(c, x, y, z, area, vor, vel0, tracers);
+
+  const void* key[10] = {x, y, z, area, vor, vx, vy, vz, tracers, (const void*)(intptr_t)c->prm.precision};
+  if (!c->graph || std::memcmp(key, c->graph_key, sizeof(key)) != 0) {
+    if (c->graph) {
+      hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = bve_step_body(c, x, y, z, area, vor, vel0, tracers);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, SBSLPM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->graph = nullptr;
+      return fail(c, SBSLPM_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    std::memcpy(c->graph_key, key, sizeof(key));
+  }
+  HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+  return SBSLPM_OK;
+}
+
+}  // extern "C"

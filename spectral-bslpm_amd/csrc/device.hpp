@@ -1,0 +1,103 @@
+// device.hpp -- device-side view of the tree and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "geometry.hpp"
+#include "tree.hpp"
+
+namespace sbslpm {
+
+// Device error word bits (Kokkos::abort sites of the reference become flags).
+enum : int {
+  kErrBliBounds = 1,   // interp_vals_bli point outside its panel (interp_funcs_impl.hpp:22-33)
+  kErrLocate = 2,      // point location failed, e.g. NaN (interp_funcs_impl.hpp:138)
+};
+
+// Everything a kernel needs to know about the tree; all pointers are device
+// memory owned by the context.
+struct DevTree {
+  const DevPanel* panels;
+  const double* px;  // unified point coordinates [U]: leaf particles then proxies
+  const double* py;
+  const double* pz;
+  const double* cheb_d;  // [d+1]
+  const double* cheb_m;  // [m]
+  const double* m2m_hi;  // [d+1][d+1] parent basis a at upper-half child node k
+  const double* m2m_lo;
+  const double* m2m_hiT;  // transposes
+  const double* m2m_loT;
+  const double* p2m;   // [d+1][m]
+  const double* p2mT;  // [m][d+1]
+  const int* proxy_leaf;      // [offset*n]
+  const double* proxy_xieta;  // [offset*n][2]
+  int* err;
+  int P, Pl, offset, n, dp1, m, m2, levels;
+  int64_t npart, U;
+};
+
+// Source record of the FP64 pair kernels: displaced, re-projected source
+// position and its charge pre-multiplied by -1/(4 pi).
+struct SrcRec64 {
+  double x, y, z, q;
+};
+
+// Source record of the mixed-precision pair kernels: position relative to the
+// source panel centre and the scaled charge, FP32.
+struct SrcRec32 {
+  float x, y, z, q;
+};
+
+struct LaunchCfg {
+  hipStream_t stream;
+};
+
+// ---- passes (fmm_passes.hip)
+void launch_upward_pass(const DevTree& t, const double* area, const double* pots, double* q_part,
+                        double* q_proxy, hipStream_t s);
+void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
+                          double* const* sols, hipStream_t s);
+void launch_fill_zero(double* p, int64_t count, hipStream_t s);
+
+// ---- interactions (fmm_interact.hip)
+void launch_source_records64(const DevTree& t, const double* q_part, const double* q_proxy,
+                             const double* const* disp_part, const double* const* disp_proxy, bool project,
+                             SrcRec64* rec, hipStream_t s);
+void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                           const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
+                           hipStream_t s);
+void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                               const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s);
+void launch_source_records32(const DevTree& t, const double* q_part, const double* q_proxy,
+                             const double* const* disp_part, const double* const* disp_proxy, SrcRec32* rec,
+                             hipStream_t s);
+void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                               const SrcRec32* rec, double eps, double* const* out_part,
+                               double* const* out_proxy, hipStream_t s);
+void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                                   const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s);
+
+// ---- displacement, departure points, remap (remap.hip)
+void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,
+                             const double* const* vel, const double* vor, double tau, double omega,
+                             double* new_vor, double* const* disp_part, double* const* disp_proxy, hipStream_t s);
+void launch_bve_departure(const DevTree& t, const double* x, const double* y, const double* z,
+                          const double* const* vels /*12*/, double dt, int combine, double* dx, double* dy,
+                          double* dz, hipStream_t s);
+void launch_bve_departure_to_target(const DevTree& t, const double* z, const double* dx, const double* dy,
+                                    const double* dz, const double* vor, const double* tracers, int ntr,
+                                    double omega, double* new_vor, double* new_tracers, hipStream_t s);
+void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
+
+// ---- direct sums (direct.hip)
+int64_t direct_partials_size(int64_t nt, int64_t ns, int ncomp);
+void launch_direct_inv_lap(int64_t nt, const double* tx, const double* ty, const double* tz, int64_t ns,
+                           const double* sx, const double* sy, const double* sz, const double* area,
+                           const double* pots, int64_t self_offset, bool guard_coincident, double* partials,
+                           double* soln, hipStream_t s);
+void launch_direct_bve_vel(int64_t nt, const double* tx, const double* ty, const double* tz, int64_t ns,
+                           const double* sx, const double* sy, const double* sz, const double* area,
+                           const double* vor, const double* const* disp, double eps, double* partials,
+                           double* const* vel, hipStream_t s);
+
+}  // namespace sbslpm

@@ -1,0 +1,298 @@
+// fmm_interact.hip -- pair kernels over the target-major interaction lists.
+//
+// Reference: bve_vel_panel_interaction (fmm_interactions/bve_vel_impl.hpp:71-106)
+// and poisson_panel_interaction (inverse_poisson_impl.hpp:181-210) run one
+// thread per interaction, recompute every source node's map per pair and
+// scatter three atomics per pair.  Here:
+//   * one wave64 owns up to 64 target points of one target panel and walks
+//     that panel's source sets, so each target is accumulated in registers by
+//     one lane (no atomics, fixed summation order);
+//   * displaced, re-projected source positions and scaled charges are built
+//     once per evaluation into a record array (SrcRec64 / SrcRec32);
+//   * every lane of a wave reads the same source record, so the records are
+//     wave-uniform loads (scalar cache) with no LDS and no barriers;
+//   * u = sum_j g_j q_j (x cross y_j) is accumulated as x cross (sum_j g_j q_j y_j),
+//     saving the per-pair cross product (six multiplies) of the reference form.
+// Workgroups of four waves; the workgroup id is remapped so that consecutive
+// tiles (spatially adjacent target panels sharing sources) land on the same
+// XCD and its L2.
+#include "device.hpp"
+
+namespace sbslpm {
+namespace {
+
+struct Ptr3 {
+  double* p[3];
+};
+struct CPtr3 {
+  const double* p[3];
+};
+
+constexpr int kWavesPerBlock = 4;
+
+// Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
+// range of logical workgroups.
+__device__ inline int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, y = b / 8;
+  return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
+}
+
+// 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps
+// (each step squares the relative error; the estimate is good to ~2^-26).
+__device__ inline double rcp64(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ inline void store_u(const DevTree& t, int64_t u, Ptr3 part, Ptr3 proxy, double a, double b, double c) {
+  if (u < t.npart) {
+    part.p[0][u] += a; part.p[1][u] += b; part.p[2][u] += c;
+  } else {
+    int64_t v = u - t.npart;
+    proxy.p[0][v] += a; proxy.p[1][v] += b; proxy.p[2][v] += c;
+  }
+}
+
+// Source records, FP64: y = P(node + disp) (bve_vel_impl.hpp:90-97), q' = -q/(4 pi).
+// Leaf proxy rows are never sources and are left untouched.
+__global__ void k_src64(DevTree t, const double* __restrict__ q_part, const double* __restrict__ q_proxy,
+                        CPtr3 dpart, CPtr3 dprox, int project, SrcRec64* __restrict__ rec) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double x = t.px[u], y = t.py[u], z = t.pz[u], q;
+  if (u < t.npart) {
+    if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
+    q = q_part[u];
+  } else {
+    int64_t v = u - t.npart;
+    if (v / t.n >= t.offset) return;
+    if (dprox.p[0]) { x += dprox.p[0][v]; y += dprox.p[1][v]; z += dprox.p[2][v]; }
+    q = q_proxy[v];
+  }
+  if (project) project_to_sphere(x, y, z);
+  rec[u] = SrcRec64{x, y, z, kNegInv4Pi * q};
+}
+
+// Source records, FP32: position relative to the owning panel's centre.
+__global__ void k_src32(DevTree t, const double* __restrict__ q_part, const double* __restrict__ q_proxy,
+                        CPtr3 dpart, CPtr3 dprox, int project, SrcRec32* __restrict__ rec) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double x = t.px[u], y = t.py[u], z = t.pz[u], q;
+  int panel;
+  if (u < t.npart) {
+    if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
+    q = q_part[u];
+    panel = t.offset + (int)(u / t.m2);
+  } else {
+    int64_t v = u - t.npart;
+    panel = (int)(v / t.n);
+    if (panel >= t.offset) return;
+    if (dprox.p[0]) { x += dprox.p[0][v]; y += dprox.p[1][v]; z += dprox.p[2][v]; }
+    q = q_proxy[v];
+  }
+  if (project) project_to_sphere(x, y, z);
+  const DevPanel& P = t.panels[panel];
+  rec[u] = SrcRec32{(float)(x - P.cx), (float)(y - P.cy), (float)(z - P.cz), (float)(kNegInv4Pi * q)};
+}
+
+// FP64 Biot-Savart pair kernel, the reference's arithmetic:
+// g = -q/(4 pi (1 - x.y + eps)).
+__global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                               const SourceSet* __restrict__ sets,
+                                               const SrcRec64* __restrict__ rec, double eps, Ptr3 part,
+                                               Ptr3 proxy) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  double w0 = 0, w1 = 0, w2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const SrcRec64* __restrict__ r = rec + S.first;
+#pragma unroll 3
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRec64 s = r[k];
+      const double dp = x * s.x + y * s.y + z * s.z;
+      const double g = rcp64((1.0 - dp) + eps) * s.q;
+      w0 = fma(g, s.x, w0);
+      w1 = fma(g, s.y, w1);
+      w2 = fma(g, s.z, w2);
+    }
+  }
+  if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0);
+}
+
+// FP64 log pair kernel: psi += -1/(4 pi) log(1 - x.y) q, pairs with
+// x.y >= 1 - 1e-15 skipped (inverse_poisson_impl.hpp:204-207).
+__global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                               const SourceSet* __restrict__ sets,
+                                               const SrcRec64* __restrict__ rec, double* __restrict__ part,
+                                               double* __restrict__ proxy) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  double acc = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const SrcRec64* __restrict__ r = rec + S.first;
+#pragma unroll 2
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRec64 s = r[k];
+      const double dp = x * s.x + y * s.y + z * s.z;
+      const double l = log(fmax(1.0 - dp, 1e-300));
+      acc += (dp < 1.0 - 1e-15) ? l * s.q : 0.0;
+    }
+  }
+  if (lane < T.count) {
+    if (u < t.npart) part[u] += acc;
+    else proxy[u - t.npart] += acc;
+  }
+}
+
+// Mixed precision Biot-Savart: FP32 pair arithmetic in panel-centred
+// coordinates with 1 - x.y computed as |x - y|^2 / 2, FP32 partial sums per
+// source set flushed into FP64 accumulators, cross product in FP64.
+__global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ sets,
+                                                   const SrcRec32* __restrict__ rec, float eps, Ptr3 part,
+                                                   Ptr3 proxy) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const DevPanel& PT = t.panels[T.panel];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const float ax = (float)(x - PT.cx), ay = (float)(y - PT.cy), az = (float)(z - PT.cz);
+  double w0 = 0, w1 = 0, w2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const DevPanel& PS = t.panels[S.panel];
+    // target relative to the source panel centre: a + (c_t - c_s)
+    const float px = ax + (float)(PT.cx - PS.cx), py = ay + (float)(PT.cy - PS.cy), pz = az + (float)(PT.cz - PS.cz);
+    const SrcRec32* __restrict__ r = rec + S.first;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRec32 s = r[k];
+      const float d0 = s.x - px, d1 = s.y - py, d2 = s.z - pz;
+      const float h = fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps);
+      const float g = __builtin_amdgcn_rcpf(h) * s.q;
+      v0 = fmaf(g, d0, v0);
+      v1 = fmaf(g, d1, v1);
+      v2 = fmaf(g, d2, v2);
+    }
+    w0 += v0; w1 += v1; w2 += v2;
+  }
+  if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0);
+}
+
+// Mixed precision log kernel: log(|x - y|^2 / 2) in FP32, same guard.
+__global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ sets,
+                                                   const SrcRec32* __restrict__ rec, double* __restrict__ part,
+                                                   double* __restrict__ proxy) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const DevPanel& PT = t.panels[T.panel];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const float ax = (float)(t.px[u] - PT.cx), ay = (float)(t.py[u] - PT.cy), az = (float)(t.pz[u] - PT.cz);
+  double acc = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const DevPanel& PS = t.panels[S.panel];
+    const float px = ax + (float)(PT.cx - PS.cx), py = ay + (float)(PT.cy - PS.cy), pz = az + (float)(PT.cz - PS.cz);
+    const SrcRec32* __restrict__ r = rec + S.first;
+    float v = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRec32 s = r[k];
+      const float d0 = s.x - px, d1 = s.y - py, d2 = s.z - pz;
+      const float h = 0.5f * fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+      const float l = __logf(fmaxf(h, 1e-30f));
+      v += (h > 1e-15f) ? l * s.q : 0.f;
+    }
+    acc += v;
+  }
+  if (lane < T.count) {
+    if (u < t.npart) part[u] += acc;
+    else proxy[u - t.npart] += acc;
+  }
+}
+
+inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
+
+inline CPtr3 cptr3(const double* const* a) {
+  CPtr3 r{};
+  if (a)
+    for (int f = 0; f < 3; ++f) r.p[f] = a[f];
+  return r;
+}
+inline Ptr3 ptr3(double* const* a) {
+  Ptr3 r{};
+  for (int f = 0; f < 3; ++f) r.p[f] = a[f];
+  return r;
+}
+
+}  // namespace
+
+void launch_source_records64(const DevTree& t, const double* q_part, const double* q_proxy,
+                             const double* const* disp_part, const double* const* disp_proxy, bool project,
+                             SrcRec64* rec, hipStream_t s) {
+  hipLaunchKernelGGL(k_src64, dim3(blocks_for(t.U, 256)), dim3(256), 0, s, t, q_part, q_proxy, cptr3(disp_part),
+                     cptr3(disp_proxy), project ? 1 : 0, rec);
+}
+
+void launch_source_records32(const DevTree& t, const double* q_part, const double* q_proxy,
+                             const double* const* disp_part, const double* const* disp_proxy, SrcRec32* rec,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_src32, dim3(blocks_for(t.U, 256)), dim3(256), 0, s, t, q_part, q_proxy, cptr3(disp_part),
+                     cptr3(disp_proxy), 1, rec);
+}
+
+void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                           const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
+                           hipStream_t s) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
+                     ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy));
+}
+
+void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                               const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_log64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
+                     ntiles, sets, rec, out_part, out_proxy);
+}
+
+void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                               const SrcRec32* rec, double eps, double* const* out_part,
+                               double* const* out_proxy, hipStream_t s) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
+                     tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy));
+}
+
+void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                                   const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_log_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
+                     tiles, ntiles, sets, rec, out_part, out_proxy);
+}
+
+}  // namespace sbslpm

@@ -1,0 +1,302 @@
+// remap.hip -- BVE source displacement, backward RK4 departure points and the
+// semi-Lagrangian remap (bve_time_step.cpp:15-309), all in FP64.
+//
+// One thread per particle (or proxy node).  Point location descends the
+// quadtree from the face panel (interp_funcs_impl.hpp:114-142); the
+// barycentric basis is evaluated as two 1-D weight vectors (the reference's
+// tensor basis is their outer product, interp_funcs_impl.hpp:105-111).  The
+// kernels are templated on the particle-grid side M so the weight vectors
+// and the m x m gathers stay in registers.  They are gather-bound: per
+// particle and stage, one leaf tile of three velocity components
+// (3 * M^2 * 8 bytes) is read, mostly from L2 because neighbouring particles
+// share leaves.
+#include "device.hpp"
+
+namespace sbslpm {
+namespace {
+
+struct CPtr3 {
+  const double* p[3];
+};
+struct Ptr3 {
+  double* p[3];
+};
+struct Vel12 {
+  const double* p[12];  // stage velocities v0x v0y v0z v1x .. v3z
+};
+
+__device__ inline void flag(const DevTree& t, int bit) { atomicOr(t.err, bit); }
+
+// interp_funcs_impl.hpp:114-142.  Returns the leaf panel id or -1; xe gets the
+// label coordinates on the located face (xieta_from_xyz, :67-98, chooses the
+// same face).
+__device__ inline int locate(const DevTree& t, double x, double y, double z, double* xe) {
+  int cur = face_from_xyz(x, y, z);
+  xieta_from_xyz_face(x, y, z, cur, xe);
+  for (int guard = 0; guard < 16; ++guard) {
+    const DevPanel& p = t.panels[cur];
+    if (p.is_leaf) return cur;
+    int c;
+    if (xe[0] >= p.mid_xi && xe[1] >= p.mid_eta) c = 0;
+    else if (xe[0] <= p.mid_xi && xe[1] >= p.mid_eta) c = 1;
+    else if (xe[0] <= p.mid_xi && xe[1] <= p.mid_eta) c = 2;
+    else if (xe[0] >= p.mid_xi && xe[1] <= p.mid_eta) c = 3;
+    else return -1;
+    cur = p.child0 + c;
+  }
+  return -1;
+}
+
+// 1-D barycentric weights, compile-time degree M-1 (see bli_weights_1d).
+template <int M>
+__device__ inline bool bli_w(double t, double lo, double hi, const double* __restrict__ cheb, double* w) {
+  constexpr int D = M - 1;
+  const double range = 0.5 * (hi - lo), off = 0.5 * (hi + lo);
+  const bool ok = (t >= lo - kSnapTol) && (t <= hi + kSnapTol);
+  double node[M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) node[i] = cheb[i] * range + off;
+  int hit = -1;
+#pragma unroll
+  for (int i = M - 1; i >= 0; --i)
+    if (fabs(t - node[i]) < kSnapTol) hit = i;
+  if (hit >= 0) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) w[i] = (i == hit) ? 1.0 : 0.0;
+    return ok;
+  }
+  double den = 0;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    double bw = (i == 0 || i == D) ? 0.5 : 1.0;
+    if (i & 1) bw = -bw;
+    w[i] = bw / (t - node[i]);
+    den += w[i];
+  }
+  const double inv = 1.0 / den;
+#pragma unroll
+  for (int i = 0; i < M; ++i) w[i] *= inv;
+  return ok;
+}
+
+// Interpolate NF leaf fields at label point (xi, eta) of leaf `leaf`
+// (bve_time_step.cpp:222-237): sum_ac (wx_a * we_c) f[a*M + c].
+template <int M, int NF>
+__device__ inline bool interp_leaf(const DevTree& t, int leaf, double xi, double eta, const double* const* f,
+                                   double* out) {
+  const DevPanel& P = t.panels[leaf];
+  double wx[M], we[M];
+  bool ok = bli_w<M>(xi, P.min_xi, P.max_xi, t.cheb_m, wx);
+  ok = bli_w<M>(eta, P.min_eta, P.max_eta, t.cheb_m, we) && ok;
+  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
+#pragma unroll
+  for (int k = 0; k < NF; ++k) out[k] = 0;
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      const double w = wx[a] * we[c];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) out[k] += f[k][base + a * M + c] * w;
+    }
+  }
+  return ok;
+}
+
+// bve_disp_update, bve_time_step.cpp:91-104.
+__global__ void k_disp_update(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, CPtr3 vel, const double* __restrict__ vor, double tau,
+                              double omega, double* __restrict__ new_vor, Ptr3 disp) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.npart) return;
+  const double vx = vel.p[0][u], vy = vel.p[1][u], vz = vel.p[2][u];
+  double a = x[u] + vx * tau, b = y[u] + vy * tau, c = z[u] + vz * tau;
+  project_to_sphere(a, b, c);
+  new_vor[u] = vor[u] + 2 * omega * z[u] - 2 * omega * c;
+  disp.p[0][u] = tau * vx;
+  disp.p[1][u] = tau * vy;
+  disp.p[2][u] = tau * vz;
+}
+
+// bve_disp_interp, bve_time_step.cpp:131-159: displacement of each non-leaf
+// proxy node interpolated from the leaf that contains it (leaf and label
+// coordinates precomputed on the host, the tree being fixed in label space).
+template <int M>
+__global__ void k_disp_interp(DevTree t, CPtr3 dpart, Ptr3 dprox) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (int64_t)t.offset * t.n) return;
+  const int leaf = t.proxy_leaf[v];
+  double out[3] = {0, 0, 0};
+  if (leaf < 0) {
+    flag(t, kErrLocate);
+  } else if (!interp_leaf<M, 3>(t, leaf, t.proxy_xieta[2 * v], t.proxy_xieta[2 * v + 1], dpart.p, out)) {
+    flag(t, kErrBliBounds);
+  }
+  dprox.p[0][v] = out[0];
+  dprox.p[1][v] = out[1];
+  dprox.p[2][v] = out[2];
+}
+
+template <int M>
+__device__ inline void vel_at(const DevTree& t, double x, double y, double z, const double* const* f, double* k) {
+  double xe[2];
+  const int leaf = locate(t, x, y, z, xe);
+  if (leaf < 0) {
+    flag(t, kErrLocate);
+    k[0] = k[1] = k[2] = 0;
+    return;
+  }
+  if (!interp_leaf<M, 3>(t, leaf, xe[0], xe[1], f, k)) flag(t, kErrBliBounds);
+}
+
+// bve_find_departure_points, bve_time_step.cpp:206-308.
+template <int M>
+__global__ void k_departure(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, Vel12 v, double dt, int combine, double* __restrict__ dx,
+                            double* __restrict__ dy, double* __restrict__ dz) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.npart) return;
+  const double X = x[u], Y = y[u], Z = z[u];
+  double k1[3] = {v.p[9][u], v.p[10][u], v.p[11][u]}, k2[3], k3[3], k4[3];
+  double a = X - 0.5 * dt * k1[0], b = Y - 0.5 * dt * k1[1], c = Z - 0.5 * dt * k1[2];
+  project_to_sphere(a, b, c);
+  vel_at<M>(t, a, b, c, v.p + 6, k2);
+  a = X - 0.5 * dt * k2[0]; b = Y - 0.5 * dt * k2[1]; c = Z - 0.5 * dt * k2[2];
+  project_to_sphere(a, b, c);
+  vel_at<M>(t, a, b, c, v.p + 3, k3);
+  a = X - dt * k3[0]; b = Y - dt * k3[1]; c = Z - dt * k3[2];
+  project_to_sphere(a, b, c);
+  vel_at<M>(t, a, b, c, v.p + 0, k4);
+  if (combine == 0) {
+    // the reference's sequence (:288-303)
+    a = X - 1.0 / 6.0 * dt * k1[0]; b = Y - 1.0 / 6.0 * dt * k1[1]; c = Z - 1.0 / 6.0 * dt * k1[2];
+    project_to_sphere(a, b, c);
+    a -= -1.0 / 3.0 * dt * k2[0]; b -= -1.0 / 3.0 * dt * k2[1]; c -= -1.0 / 3.0 * dt * k2[2];
+    project_to_sphere(a, b, c);
+    a -= -1.0 / 3.0 * dt * k3[0]; b -= -1.0 / 3.0 * dt * k3[1]; c -= -1.0 / 3.0 * dt * k3[2];
+    project_to_sphere(a, b, c);
+    a -= -1.0 / 6.0 * dt * k4[0]; b -= -1.0 / 6.0 * dt * k4[1]; c -= -1.0 / 6.0 * dt * k4[2];
+    project_to_sphere(a, b, c);
+  } else {
+    a = X - dt / 6.0 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0]);
+    b = Y - dt / 6.0 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1]);
+    c = Z - dt / 6.0 * (k1[2] + 2 * k2[2] + 2 * k3[2] + k4[2]);
+    project_to_sphere(a, b, c);
+  }
+  dx[u] = a; dy[u] = b; dz[u] = c;
+}
+
+// bve_departure_to_target, bve_time_step.cpp:38-66.  Tracers [Pl][m2][T].
+template <int M>
+__global__ void k_dep_to_target(DevTree t, const double* __restrict__ z, const double* __restrict__ dx,
+                                const double* __restrict__ dy, const double* __restrict__ dz,
+                                const double* __restrict__ vor, const double* __restrict__ tracers, int ntr,
+                                double omega, double* __restrict__ new_vor, double* __restrict__ new_tracers) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.npart) return;
+  double xe[2];
+  const double X = dx[u], Y = dy[u], Z = dz[u];
+  const int leaf = locate(t, X, Y, Z, xe);
+  if (leaf < 0) {
+    flag(t, kErrLocate);
+    new_vor[u] = 0;
+    for (int k = 0; k < ntr; ++k) new_tracers[u * ntr + k] = 0;
+    return;
+  }
+  const DevPanel& P = t.panels[leaf];
+  double wx[M], we[M];
+  bool ok = bli_w<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx);
+  ok = bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we) && ok;
+  if (!ok) flag(t, kErrBliBounds);
+  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
+  double v = 0;
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int c = 0; c < M; ++c) v += vor[base + a * M + c] * (wx[a] * we[c]);
+  v += 2 * omega * Z;
+  new_vor[u] = v - 2 * omega * z[u];
+  for (int k = 0; k < ntr; ++k) {
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+      for (int c = 0; c < M; ++c) s += tracers[(base + a * M + c) * ntr + k] * (wx[a] * we[c]);
+    new_tracers[u * ntr + k] = s;
+  }
+}
+
+inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
+
+template <template <int> class F, typename... A>
+void dispatch_m(int m, A&&... a) {
+  switch (m) {
+    case 2: F<2>::run(a...); break;
+    case 3: F<3>::run(a...); break;
+    case 4: F<4>::run(a...); break;
+    case 5: F<5>::run(a...); break;
+    case 6: F<6>::run(a...); break;
+    case 7: F<7>::run(a...); break;
+    case 8: F<8>::run(a...); break;
+    case 9: F<9>::run(a...); break;
+    case 10: F<10>::run(a...); break;
+    default: F<11>::run(a...); break;
+  }
+}
+
+template <int M>
+struct DispInterp {
+  static void run(const DevTree& t, CPtr3 dp, Ptr3 dx, hipStream_t s) {
+    int64_t cnt = (int64_t)t.offset * t.n;
+    if (cnt > 0) hipLaunchKernelGGL(k_disp_interp<M>, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, t, dp, dx);
+  }
+};
+
+template <int M>
+struct Departure {
+  static void run(const DevTree& t, const double* x, const double* y, const double* z, Vel12 v, double dt,
+                  int combine, double* dx, double* dy, double* dz, hipStream_t s) {
+    hipLaunchKernelGGL(k_departure<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, v, dt, combine,
+                       dx, dy, dz);
+  }
+};
+
+template <int M>
+struct DepToTarget {
+  static void run(const DevTree& t, const double* z, const double* dx, const double* dy, const double* dz,
+                  const double* vor, const double* tr, int ntr, double omega, double* nv, double* ntrc,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_dep_to_target<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, z, dx, dy, dz, vor,
+                       tr, ntr, omega, nv, ntrc);
+  }
+};
+
+}  // namespace
+
+void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,
+                             const double* const* vel, const double* vor, double tau, double omega,
+                             double* new_vor, double* const* disp_part, double* const* disp_proxy, hipStream_t s) {
+  CPtr3 v{{vel[0], vel[1], vel[2]}};
+  Ptr3 dp{{disp_part[0], disp_part[1], disp_part[2]}};
+  hipLaunchKernelGGL(k_disp_update, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, x, y, z, v, vor, tau, omega,
+                     new_vor, dp);
+  CPtr3 cdp{{disp_part[0], disp_part[1], disp_part[2]}};
+  Ptr3 dx{{disp_proxy[0], disp_proxy[1], disp_proxy[2]}};
+  dispatch_m<DispInterp>(t.m, t, cdp, dx, s);
+}
+
+void launch_bve_departure(const DevTree& t, const double* x, const double* y, const double* z,
+                          const double* const* vels, double dt, int combine, double* dx, double* dy, double* dz,
+                          hipStream_t s) {
+  Vel12 v{};
+  for (int i = 0; i < 12; ++i) v.p[i] = vels[i];
+  dispatch_m<Departure>(t.m, t, x, y, z, v, dt, combine, dx, dy, dz, s);
+}
+
+void launch_bve_departure_to_target(const DevTree& t, const double* z, const double* dx, const double* dy,
+                                    const double* dz, const double* vor, const double* tracers, int ntr,
+                                    double omega, double* new_vor, double* new_tracers, hipStream_t s) {
+  dispatch_m<DepToTarget>(t.m, t, z, dx, dy, dz, vor, tracers, ntr, omega, new_vor, new_tracers, s);
+}
+
+}  // namespace sbslpm

@@ -1,0 +1,86 @@
+// tree.hpp -- host-side cubed-sphere quadtree, dual-tree interaction list and
+// the fixed transfer operators the device passes use.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace sbslpm {
+
+// Panel fields the hot path reads (initialize_cubed_sphere.hpp:8-29; the edge
+// neighbour ids serve only the SWE derivative code and are not built).
+struct HostPanel {
+  int face, level, parent, child[4], is_leaf;
+  double min_xi, max_xi, min_eta, max_eta;  // in [-1, 1]; radians = value * pi/4
+  double radius, area;
+};
+
+// Device copy of a panel: bounds in radians, as the reference kernels convert
+// them (`Kokkos::numbers::pi/4.0 * min_xi`, interp_funcs_impl.hpp:127,
+// bve_time_step.cpp:45), plus the centre used by the mixed-precision kernels.
+struct DevPanel {
+  double min_xi, max_xi, min_eta, max_eta;
+  double mid_xi, mid_eta;  // 0.5*pi/4*(max+min) as point location computes it (interp_funcs_impl.hpp:127-128)
+  double cx, cy, cz, pad0;  // unit-sphere panel centre
+  int face, is_leaf, child0, level;
+};
+
+struct TreeConfig {
+  int levels;       // L; leaves at level L-1
+  int degree;       // proxy degree d (n = (d+1)^2)
+  int leaf_degree;  // particle-grid degree m-1 per leaf (== d: the reference layout)
+  double theta;     // multipole acceptance parameter
+  int weight_rule;  // 0 reference quadrature formula, 1 Clenshaw-Curtis, 2 equal area
+};
+
+// One unit of interaction work: up to 64 target points of one target panel
+// and its source-set range.  Target points live in the unified point index
+// space [0, Pl*m2) particles, [Pl*m2, Pl*m2 + P*n) proxies.
+struct TargetTile {
+  int panel;      // target panel id
+  int first;      // first unified target point index of the tile
+  int count;      // target points in the tile (1..64)
+  int src_begin;  // range in the source-set list
+  int src_end;
+  int pad[3];
+};
+
+// A source point set: `count` consecutive records starting at `first` in the
+// unified point index space, belonging to panel `panel`.
+struct SourceSet {
+  int first, count, panel, pad;
+};
+
+struct Tree {
+  TreeConfig cfg{};
+  int P = 0, Pl = 0, offset = 0, n = 0, dp1 = 0, m = 0, m2 = 0;
+  int64_t npart = 0, U = 0;        // Pl*m2, Pl*m2 + P*n
+  std::vector<int> level_start;    // [L+1]
+  std::vector<HostPanel> pn;       // [P]
+  std::vector<double> cheb_d, cheb_m;  // cos(pi i/deg) as interp_vals_bli evaluates it (:45)
+  std::vector<double> node_d, node_m;  // interp_init nodes cos(pi/deg*i), |.|<1e-16 -> 0 (:33-39)
+  std::vector<double> quad_m;          // 1-D quadrature weights of the particle grid
+  std::vector<int> ilist;              // [I][3] target, source, type in the reference's BFS order
+
+  // Device-facing structures.
+  std::vector<DevPanel> dev;            // [P]
+  std::vector<TargetTile> tiles;        // interaction work units
+  std::vector<SourceSet> srcsets;       // per-tile source lists (CSR via tiles)
+  std::vector<double> px, py, pz;       // unified point coordinates [U] (targets, undisplaced sources)
+  std::vector<double> m2m_hi, m2m_lo;   // [d+1][d+1]: parent basis a at child node k, A[a*(d+1)+k]
+  std::vector<double> p2m;              // [d+1][m]: proxy basis a at particle node k
+  std::vector<int> proxy_leaf;          // [offset*n] leaf containing each non-leaf proxy node
+  std::vector<double> proxy_xieta;      // [offset*n][2] label coordinates of those nodes (radians)
+  std::vector<double> spec_x, spec_y, spec_z, spec_area;  // caller-facing particles [Pl][m2]
+
+  int64_t interaction_count() const { return (int64_t)ilist.size() / 3; }
+};
+
+// Builds everything above on the host; returns an empty string on success or
+// a message describing the invalid parameter.
+std::string build_tree(const TreeConfig& cfg, Tree& t);
+
+// Host twin of the device point location (interp_funcs_impl.hpp:114-142).
+int locate_leaf(const Tree& t, double x, double y, double z);
+
+}  // namespace sbslpm

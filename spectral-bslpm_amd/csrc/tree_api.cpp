@@ -1,0 +1,87 @@
+// tree_api.cpp -- host-only C ABI over the tree builder (no device needed).
+#include <cstring>
+#include <new>
+
+#include "../../include/sbslpm.h"
+#include "tree.hpp"
+
+struct sbslpm_tree {
+  sbslpm::Tree t;
+};
+
+extern "C" {
+
+int sbslpm_tree_build(const sbslpm_params* p, sbslpm_tree** out) {
+  if (!p || !out) return SBSLPM_E_ARG;
+  *out = nullptr;
+  sbslpm_tree* h = new (std::nothrow) sbslpm_tree();
+  if (!h) return SBSLPM_E_NOMEM;
+  sbslpm::TreeConfig cfg{p->levels, p->interp_degree, p->leaf_degree, p->theta, p->weight_rule};
+  if (!sbslpm::build_tree(cfg, h->t).empty()) {
+    delete h;
+    return SBSLPM_E_ARG;
+  }
+  *out = h;
+  return SBSLPM_OK;
+}
+
+void sbslpm_tree_destroy(sbslpm_tree* h) { delete h; }
+
+int sbslpm_tree_sizes(const sbslpm_tree* h, int64_t* s) {
+  if (!h || !s) return SBSLPM_E_ARG;
+  const sbslpm::Tree& t = h->t;
+  s[0] = t.P; s[1] = t.Pl; s[2] = t.n; s[3] = t.m2; s[4] = t.interaction_count(); s[5] = t.npart;
+  s[6] = t.offset; s[7] = (int64_t)t.tiles.size();
+  return SBSLPM_OK;
+}
+
+int sbslpm_tree_spec_points(const sbslpm_tree* h, double* x, double* y, double* z, double* area) {
+  if (!h) return SBSLPM_E_ARG;
+  const sbslpm::Tree& t = h->t;
+  size_t bytes = (size_t)t.npart * sizeof(double);
+  if (x) std::memcpy(x, t.spec_x.data(), bytes);
+  if (y) std::memcpy(y, t.spec_y.data(), bytes);
+  if (z) std::memcpy(z, t.spec_z.data(), bytes);
+  if (area) std::memcpy(area, t.spec_area.data(), bytes);
+  return SBSLPM_OK;
+}
+
+int sbslpm_tree_interaction_list(const sbslpm_tree* h, int* list) {
+  if (!h || !list) return SBSLPM_E_ARG;
+  std::memcpy(list, h->t.ilist.data(), h->t.ilist.size() * sizeof(int));
+  return SBSLPM_OK;
+}
+
+int sbslpm_tree_panels(const sbslpm_tree* h, int* ints, double* dbls) {
+  if (!h) return SBSLPM_E_ARG;
+  const sbslpm::Tree& t = h->t;
+  for (int i = 0; i < t.P; ++i) {
+    const sbslpm::HostPanel& p = t.pn[i];
+    if (ints) {
+      int* r = ints + 8 * i;
+      r[0] = p.face; r[1] = p.level; r[2] = p.parent;
+      r[3] = p.child[0]; r[4] = p.child[1]; r[5] = p.child[2]; r[6] = p.child[3]; r[7] = p.is_leaf;
+    }
+    if (dbls) {
+      double* q = dbls + 6 * i;
+      q[0] = p.min_xi; q[1] = p.max_xi; q[2] = p.min_eta; q[3] = p.max_eta; q[4] = p.radius; q[5] = p.area;
+    }
+  }
+  return SBSLPM_OK;
+}
+
+int sbslpm_tree_locate(const sbslpm_tree* h, double x, double y, double z) {
+  if (!h) return -1;
+  return sbslpm::locate_leaf(h->t, x, y, z);
+}
+
+int sbslpm_tree_transfer_matrices(const sbslpm_tree* h, double* hi, double* lo, double* p2m) {
+  if (!h) return SBSLPM_E_ARG;
+  const sbslpm::Tree& t = h->t;
+  if (hi) std::memcpy(hi, t.m2m_hi.data(), t.m2m_hi.size() * sizeof(double));
+  if (lo) std::memcpy(lo, t.m2m_lo.data(), t.m2m_lo.size() * sizeof(double));
+  if (p2m) std::memcpy(p2m, t.p2m.data(), t.p2m.size() * sizeof(double));
+  return SBSLPM_OK;
+}
+
+}  // extern "C"

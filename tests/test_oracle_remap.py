@@ -1,0 +1,113 @@
+"""Pins the oracle's BVE displacement, departure points, remap and step
+(bve_time_step.cpp:15-419) against exact special cases."""
+import numpy as np
+import pytest
+
+OMEGA = 7.292e-5
+DT = 3600.0
+
+
+def _rot(x, y, z, axis, ang):
+    """Rodrigues rotation of unit vectors by `ang` about unit `axis`."""
+    k = np.asarray(axis, float) / np.linalg.norm(axis)
+    v = np.stack([x, y, z], -1)
+    c, s = np.cos(ang), np.sin(ang)
+    kv = np.cross(np.broadcast_to(k, v.shape), v)
+    return v * c + kv * s + np.outer((v @ k).ravel(), k).reshape(v.shape) * (1 - c)
+
+
+@pytest.fixture(scope="module")
+def setup(orc):
+    o = orc.OracleFMM(3, 8, 7, 0.7, 1)
+    x, y, z, a = o.spec_points()
+    return o, x, y, z, a
+
+
+def test_zero_velocity_is_identity_remap(orc, setup):
+    o, x, y, z, a = setup
+    zero = [np.zeros_like(x) for _ in range(3)]
+    for combine in (0, 1):
+        dep = o.bve_find_departure_points(x, y, z, [zero] * 4, DT, combine)
+        np.testing.assert_allclose(dep[0], x, atol=2e-16)
+        vor = np.sin(3 * x) * np.cos(2 * z) + y
+        tr = np.stack([x * y, z], -1).copy()
+        nv, ntr = o.bve_departure_to_target(z, dep, vor, OMEGA, tr)
+        np.testing.assert_allclose(nv, vor, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(ntr, tr, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("combine", [0, 1])
+def test_solid_body_rotation_departure_points(orc, setup, combine):
+    """v = w x X (steady): textbook RK4 gives the exact backward rotation to
+    O((w dt)^5); the reference combine (x - dt/6 k1, then + dt/3 k2, + dt/3 k3,
+    + dt/6 k4, projecting after each) is evaluated independently here."""
+    o, x, y, z, a = setup
+    w = np.array([0.3, -0.2, 1.0]) * 2e-5
+    vel = lambda X: np.cross(np.broadcast_to(w, X.shape), X)
+    X = np.stack([x, y, z], -1)
+    V = vel(X)
+    stage = [V[..., 0].copy(), V[..., 1].copy(), V[..., 2].copy()]
+    dep = o.bve_find_departure_points(x, y, z, [stage] * 4, DT, combine)
+    dep = np.stack(dep, -1)
+    P = lambda v: v / np.linalg.norm(v, axis=-1, keepdims=True)
+    k1 = V
+    k2 = vel(P(X - 0.5 * DT * k1))
+    k3 = vel(P(X - 0.5 * DT * k2))
+    k4 = vel(P(X - DT * k3))
+    if combine == 1:
+        expect = P(X - DT / 6 * (k1 + 2 * k2 + 2 * k3 + k4))
+        exact = _rot(x, y, z, w, -np.linalg.norm(w) * DT)
+        assert np.max(np.abs(dep - exact)) < 1e-8
+    else:
+        e = P(X - DT / 6 * k1)
+        e = P(e + DT / 3 * k2)
+        e = P(e + DT / 3 * k3)
+        expect = P(e + DT / 6 * k4)
+    # only the BLI error of the interpolated stage velocities separates the two
+    assert np.max(np.abs(dep - expect)) < 1e-9
+
+
+def test_absolute_vorticity_is_conserved(orc, setup):
+    o, x, y, z, a = setup
+    c = 3e-5
+    vor = c - 2 * OMEGA * z  # absolute vorticity == c everywhere
+    rng = np.random.default_rng(7)
+    dep = np.stack([x, y, z], -1) + 0.01 * rng.normal(size=x.shape + (3,))
+    dep /= np.linalg.norm(dep, axis=-1, keepdims=True)
+    dep = [dep[..., i].copy() for i in range(3)]
+    nv, _ = o.bve_departure_to_target(z, dep, vor, OMEGA)
+    np.testing.assert_allclose(nv + 2 * OMEGA * z, c, rtol=0, atol=1e-12)
+
+
+def test_displacement_update_and_proxy_interpolation(orc, setup):
+    o, x, y, z, a = setup
+    tau = 1800.0
+    vel = [1e-6 * (y * z + 1), 1e-6 * x, -2e-6 * x * y]
+    vor = np.cos(x) * z
+    nv, dp, dx = o.bve_displacement(x, y, z, vel, vor, tau, OMEGA)
+    X = np.stack([x, y, z], -1) + tau * np.stack(vel, -1)
+    zp = X[..., 2] / np.linalg.norm(X, axis=-1)
+    np.testing.assert_allclose(nv, vor + 2 * OMEGA * z - 2 * OMEGA * zp, rtol=0, atol=1e-18)
+    for f in range(3):
+        np.testing.assert_allclose(dp[f], tau * vel[f], rtol=0, atol=0)
+    # non-leaf proxies: the displacement field evaluated at the proxy node (BLI accuracy)
+    px, py, pz = o.proxy_points()
+    field = [tau * 1e-6 * (py * pz + 1), tau * 1e-6 * px, -2 * tau * 1e-6 * px * py]
+    for f in range(3):
+        np.testing.assert_allclose(dx[f][: o.offset], field[f][: o.offset], rtol=0, atol=1e-8 * tau * 1e-6)
+
+
+@pytest.mark.parametrize("combine", [0, 1])
+def test_step_keeps_zonal_l1_field(orc, setup, combine):
+    """omega = c z induces solid rotation about z with rate c/2 (psi = c z/2,
+    u = -x cross grad psi); the field is invariant under it, so one step with
+    Omega = 0 returns it to discretisation accuracy, and the stage-0 velocity is
+    (c/2) z_hat cross x."""
+    o, x, y, z, a = setup
+    c = 2e-5
+    vor = c * z
+    nv, vel, _ = o.bve_back_rk4_step(x, y, z, a, vor, DT, 0.0, 1e-10, combine)
+    expect = [-(c / 2) * y, (c / 2) * x, np.zeros_like(z)]
+    err = np.sqrt(sum(np.sum((v - e) ** 2) for v, e in zip(vel, expect)) / sum(np.sum(e ** 2) for e in expect[:2]))
+    assert err < 2e-2
+    assert np.max(np.abs(nv - vor)) < 2e-2 * c
