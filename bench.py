@@ -1,0 +1,241 @@
+"""Benchmark: BVE RK4 steps/s at 6 x 512^2 = 1,572,864 particles on one MI355X,
+mixed precision vs plain FP64, with the accuracy of mixed vs FP64.
+
+Workload: uniform cubed-sphere quadtree with 7 levels (24,576 leaves), an 8 x 8
+Chebyshev particle grid per leaf, proxy degree 8, MAC theta 0.7,
+kernel_eps 1e-4, dt 3600 s, Omega 7.292e-5; initial vorticity = 12 seeded
+Gaussian vortices (synthetic).  One step = bve_back_rk4_step: four FMM
+velocity evaluations with displaced sources, backward RK4 departure points and
+the semi-Lagrangian remap, captured once in a hipGraph and replayed.
+
+Prints one JSON line (see README of the driver contract in the task):
+value = mixed-precision steps/s; the plain FP64 rate, the speed-up, the
+rel-L2 / Linf velocity and vorticity errors of mixed vs FP64, the FP64 FMM
+error vs the FP64 direct sum on 4,096 targets, the pair-kernel roofline
+measured with HIP events on the context stream, and the CPU baseline (the
+oracle restatement of the reference algorithm timed on a bounded sample on
+this host's cores).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "BVE RK4 steps/sec at 1.57M particles, mixed vs FP64 plain, rel-L2 err ≤1e-6"
+PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 (vector and matrix)
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32
+FLOP_BS = 27       # flop per Biot-Savart pair (difference form; div/rcp/sqrt counted as 1)
+
+CONFIGS = {
+    # levels, proxy degree, particle-grid degree, theta
+    "c3": dict(levels=7, interp_degree=8, leaf_degree=7, fmm_theta=0.7),
+    "c1": dict(levels=4, interp_degree=8, leaf_degree=7, fmm_theta=0.7),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pair_count(il, P, Pl, n, m2):
+    t, s, typ = il[:, 0], il[:, 1], il[:, 2]
+    nt = np.where(typ <= 1, m2, n).astype(np.int64)
+    ns = np.where((typ == 0) | (typ == 2), m2, n).astype(np.int64)
+    return int(np.sum(nt * ns))
+
+
+def time_steps(ctx, torch, stream, x, y, z, a, vor0, steps, warmup):
+    vor = vor0.clone()
+    vel = ctx.zeros_part(3)
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            ctx.bve_back_rk4_step(x, y, z, a, vor, vel)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    vor.copy_(vor0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        for _ in range(steps):
+            ctx.bve_back_rk4_step(x, y, z, a, vor, vel)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.synchronize()
+    return wall, e0.elapsed_time(e1) / 1e3, vor, vel
+
+
+def cpu_baseline(cfg, x, y, z, a, vor, budget_s):
+    """Oracle (FP64 restatement of the reference loops) timed on a bounded
+    sample: one upward pass, one downward pass, one displacement, one
+    departure + remap at full size, and the interaction phase on every
+    `stride`-th target panel; one step = 4 x (upward + interactions +
+    downward) + 3 displacements + departure + remap."""
+    from oracle import oracle as O
+    threads = len(os.sched_getaffinity(0))
+    O.set_threads(threads)
+    o = O.OracleFMM(cfg["levels"], cfg["interp_degree"], cfg["leaf_degree"], cfg["fmm_theta"], 1)
+    t0 = time.perf_counter()
+    qp, qx = o.upward_pass(a, vor)
+    t_up = time.perf_counter() - t0
+    il = o.interactions()
+    # pick the sample stride from a small probe so the sample takes ~budget/2
+    stride = 256
+    t0 = time.perf_counter()
+    o.bve_vel_interactions_sample(qp, qx, 1e-4, stride)
+    t_probe = time.perf_counter() - t0
+    frac_probe = np.mean(np.arange(o.P) % stride == 0)
+    est_full = t_probe / max(frac_probe, 1e-9)
+    want = max(0.5 * budget_s, t_probe)
+    stride2 = int(max(1, min(256, round(est_full / want))))
+    t0 = time.perf_counter()
+    pp, px = o.bve_vel_interactions_sample(qp, qx, 1e-4, stride2)
+    t_int_sample = time.perf_counter() - t0
+    sampled = np.isin(il[:, 0], np.arange(0, o.P, stride2))
+    pairs_all = pair_count(il, o.P, o.Pl, o.n, o.m2)
+    pairs_sample = pair_count(il[sampled], o.P, o.Pl, o.n, o.m2)
+    t_int = t_int_sample * pairs_all / max(pairs_sample, 1)
+    t0 = time.perf_counter()
+    vel = o.downward_pass(px, pp)
+    t_down = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vfake = [1e-6 * np.ones_like(x), 1e-6 * y, -1e-6 * x]
+    nv, dpart, dprox = o.bve_displacement(x, y, z, vfake, vor, 1800.0, 7.292e-5)
+    t_disp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dep = o.bve_find_departure_points(x, y, z, [vfake] * 4, 3600.0, 0)
+    t_dep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.bve_departure_to_target(z, dep, vor, 7.292e-5)
+    t_remap = time.perf_counter() - t0
+    step = 4 * (t_up + t_int + t_down) + 3 * t_disp + t_dep + t_remap
+    measured = t_up + t_probe + t_int_sample + t_down + t_disp + t_dep + t_remap
+    sample = (f"full-size upward/downward pass, displacement, departure+remap; interactions of every "
+              f"{stride2}-th target panel ({pairs_sample:.3e} of {pairs_all:.3e} pairs, scaled by pair count); "
+              f"step = 4x(up+interactions+down)+3x displacement+departure+remap; {measured:.1f} s measured")
+    return dict(value=1.0 / step, unit="steps/s", cores=threads, kind="port", sample=sample,
+                seconds_per_step_extrapolated=step)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    sb = importlib.import_module("spectral-bslpm_amd")
+    W = sb.workloads
+    cfgd = CONFIGS[args.config]
+    stream = torch.cuda.Stream()
+    common = dict(weights="clenshaw_curtis", kernel_eps=1e-4, delta_t=3600.0, omega=7.292e-5,
+                  rk4_combine="reference", use_graph=not args.no_graph, **cfgd)
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        ctx64 = sb.Context(sb.RunConfig(precision="fp64", **common))
+        ctxmx = sb.Context(sb.RunConfig(precision="mixed", **common))
+    log(f"setup {time.perf_counter() - t0:.1f} s: N={ctx64.sizes.N} P={ctx64.sizes.P} I={ctx64.sizes.I}")
+    x, y, z, a = ctx64.spec_points()
+    vor_h = W.gaussian_vortices(x, y, z, a, count=12, seed=0)
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    gx, gy, gz, ga, gvor0 = [dev(v) for v in (x, y, z, a, vor_h)]
+    torch.cuda.synchronize()
+
+    wall64, ev64, vor64, vel64 = time_steps(ctx64, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
+    log(f"fp64 : {args.steps / ev64:.3f} steps/s ({1e3 * ev64 / args.steps:.2f} ms/step)")
+    wallmx, evmx, vormx, velmx = time_steps(ctxmx, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
+    log(f"mixed: {args.steps / evmx:.3f} steps/s ({1e3 * evmx / args.steps:.2f} ms/step)")
+
+    # accuracy of mixed vs FP64: one velocity evaluation from the initial field,
+    # and the vorticity after the timed steps (both runs start from the same field)
+    with torch.cuda.stream(stream):
+        v64 = ctx64.bve_velocity(ga, gvor0)
+        vmx = ctxmx.bve_velocity(ga, gvor0)
+    torch.cuda.synchronize()
+    v64h = [v.cpu().numpy() for v in v64]
+    vmxh = [v.cpu().numpy() for v in vmx]
+    acc = dict(
+        vel_rel_l2=W.vec_rel_l2(vmxh, v64h), vel_rel_linf=W.vec_rel_linf(vmxh, v64h),
+        vor_rel_l2_after_steps=W.rel_l2(vormx.cpu().numpy(), vor64.cpu().numpy()),
+        vor_rel_linf_after_steps=W.rel_linf(vormx.cpu().numpy(), vor64.cpu().numpy()))
+    # FP64 FMM vs FP64 direct sum at 4,096 targets (a contiguous block of 64 leaves)
+    rng = np.random.default_rng(0)
+    lb = int(rng.integers(0, ctx64.sizes.Pl - 64)) * ctx64.sizes.m2
+    ub = lb + 4096
+    flat = [t.reshape(-1) for t in (gx, gy, gz, ga, gvor0)]
+    with torch.cuda.stream(stream):
+        ds = ctx64.direct_sum_bve_vel(*flat, lb=lb, ub=ub)
+    torch.cuda.synchronize()
+    dsh = [d.cpu().numpy()[lb:ub] for d in ds]
+    fmm64 = [v.reshape(-1)[lb:ub] for v in v64h]
+    fmmmx = [v.reshape(-1)[lb:ub] for v in vmxh]
+    acc["fp64_fmm_vs_direct_rel_l2"] = W.vec_rel_l2(fmm64, dsh)
+    acc["mixed_fmm_vs_direct_rel_l2"] = W.vec_rel_l2(fmmmx, dsh)
+    log("accuracy", acc)
+
+    # pair-kernel roofline: HIP events around each pair-kernel launch on the context stream
+    il = ctx64.interaction_list()
+    s = ctx64.sizes
+    pairs = pair_count(il, s.P, s.Pl, s.n, s.m2)
+    roof = {}
+    for ctx, name, peak in ((ctx64, "bve_fp64", PEAK_FP64), (ctxmx, "bve_mixed", PEAK_FP32)):
+        ctx.enable_timing(True)
+        with torch.cuda.stream(stream):
+            for _ in range(5):
+                ctx.bve_velocity(ga, gvor0)
+        torch.cuda.synchronize()
+        ms, nl = ctx.kernel_time(name)
+        ctx.enable_timing(False)
+        avg = ms / nl / 1e3
+        achieved = pairs * FLOP_BS / avg / 1e12
+        roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                          frac=round(achieved / peak, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
+                          pairs_per_launch=pairs, flop_per_pair=FLOP_BS)
+    log("roofline", roof)
+
+    cpu = None
+    if args.cpu_budget > 0:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(cfgd, x, y, z, a, vor_h, args.cpu_budget)
+        log(f"cpu baseline {time.perf_counter() - t0:.1f} s: {cpu}")
+
+    value = args.steps / evmx
+    line = {
+        "metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * evmx / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair kernels, f64 accumulation/trajectories",
+        "data": "synthetic (12 seeded Gaussian vortices)",
+        "config": {"workload": f"BVE RK4 step ({args.config}): 6x512^2 particles" if args.config == "c3" else
+                   f"BVE RK4 step ({args.config})", "particles": s.N, "levels": cfgd["levels"],
+                   "proxy_degree": cfgd["interp_degree"], "leaf_grid": cfgd["leaf_degree"] + 1,
+                   "theta": cfgd["fmm_theta"], "interactions": s.I, "pairs_per_evaluation": pairs,
+                   "kernel_eps": 1e-4, "dt": 3600.0, "graph": not args.no_graph},
+        "fp64_steps_per_s": round(args.steps / ev64, 4), "fp64_ms_per_step": round(1e3 * ev64 / args.steps, 3),
+        "speedup_mixed_vs_fp64": round(ev64 / evmx, 3),
+        "accuracy": {k: float(f"{v:.3e}") for k, v in acc.items()},
+        "wall_s": {"fp64": round(wall64, 4), "mixed": round(wallmx, 4)},
+        "roofline": roof["bve_mixed"], "roofline_fp64": roof["bve_fp64"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -86,7 +86,8 @@ int sbslpm_tree_transfer_matrices(const sbslpm_tree* tree, double* m2m_hi, doubl
  * (dual_tree_traversal, fmm_funcs.cpp:17-123) and uploads them with all scratch space. */
 int sbslpm_ctx_create(const sbslpm_params* params, sbslpm_ctx** out);
 void sbslpm_ctx_destroy(sbslpm_ctx* ctx);
-/* stream: a hipStream_t (NULL = default stream). */
+/* stream: a hipStream_t; NULL selects the default (legacy) stream.  A new context uses a
+ * stream of its own.  hipGraph capture (SBSLPM_FLAG_GRAPH) needs a non-default stream. */
 int sbslpm_ctx_set_stream(sbslpm_ctx* ctx, void* stream);
 int sbslpm_ctx_set_precision(sbslpm_ctx* ctx, int precision);
 const char* sbslpm_last_error(const sbslpm_ctx* ctx);
@@ -101,6 +102,15 @@ int sbslpm_get_sizes(const sbslpm_ctx* ctx, int64_t* sizes);
 int sbslpm_spec_points(const sbslpm_ctx* ctx, double* x, double* y, double* z, double* area);
 /* Host copy of the interaction list [I][3] = {target, source, type}, reference order. */
 int sbslpm_interaction_list(const sbslpm_ctx* ctx, int* list);
+
+/* Kernel timing (measurement aid, no reference counterpart): while enabled, every pair-kernel
+ * launch is bracketed by hipEvents on the context stream (eager calls only; not inside a captured
+ * step).  sbslpm_kernel_time waits for the recorded events and returns the summed milliseconds and
+ * the launch count of one kernel class since the last reset:
+ * 0 Biot-Savart FP64, 1 Biot-Savart mixed, 2 log FP64, 3 log mixed. */
+enum { SBSLPM_KERNEL_BVE_FP64 = 0, SBSLPM_KERNEL_BVE_MIXED = 1, SBSLPM_KERNEL_LOG_FP64 = 2, SBSLPM_KERNEL_LOG_MIXED = 3 };
+int sbslpm_ctx_enable_timing(sbslpm_ctx* ctx, int enable);
+int sbslpm_kernel_time(sbslpm_ctx* ctx, int kernel, double* ms, int64_t* launches, int reset);
 
 /* upward_pass (fmm_funcs.cpp:341-355): q_part = area*pots [Pl][m2] (base_pots),
  * q_proxy [P][n] leaf rows by P2M, parents by M2M.  Assigns every row. */

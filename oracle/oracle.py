@@ -59,6 +59,8 @@ def lib():
         L.orc_downward_pass.argtypes = [ctypes.c_void_p, ctypes.c_int, _ppd, _ppd, _ppd]
         L.orc_downward_pass.restype = ctypes.c_int
         L.orc_bve_vel_interactions.argtypes = [ctypes.c_void_p, _ppd, _ppd, _dp, _dp, ctypes.c_double, _ppd, _ppd]
+        L.orc_bve_vel_interactions_sample.argtypes = [ctypes.c_void_p, _ppd, _ppd, _dp, _dp, ctypes.c_double,
+                                                      _ppd, _ppd, ctypes.c_int]
         L.orc_poisson_fmm_interactions.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_direct_sum_inv_lap.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64]
         L.orc_direct_sum_bve_vel.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _ppd, ctypes.c_double, _ppd,
@@ -252,6 +254,16 @@ class OracleFMM:
         px = [np.zeros((self.P, self.n)) for _ in range(3)]
         lib().orc_bve_vel_interactions(self.h, _pp(disp_part), _pp(disp_proxy), _p(q_part), _p(q_proxy), eps,
                                        _pp(pp), _pp(px))
+        return pp, px
+
+    def bve_vel_interactions_sample(self, q_part, q_proxy, eps, stride):
+        """Interactions of target panels t % stride == 0 only (timing samples)."""
+        dp = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        dx = [np.zeros((self.P, self.n)) for _ in range(3)]
+        pp = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        px = [np.zeros((self.P, self.n)) for _ in range(3)]
+        lib().orc_bve_vel_interactions_sample(self.h, _pp(dp), _pp(dx), _p(q_part), _p(q_proxy), eps, _pp(pp),
+                                              _pp(px), int(stride))
         return pp, px
 
     def poisson_fmm_interactions(self, q_part, q_proxy):

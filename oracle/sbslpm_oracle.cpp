@@ -640,12 +640,15 @@ static inline Side side_of(const Ctx& c, int panel, bool particles) {
 // type-dependent point sets of swe_vel_impl.hpp:188-255.  Sources sit at
 // P(node + disp) (bve_vel_impl.hpp:90-97); targets are undisplaced.
 // phi_part[3][Pl][m2] and phi_proxy[3][P][n] are accumulated.
-void orc_bve_vel_interactions(void* h, const double* const* disp_part, const double* const* disp_proxy,
-                              const double* q_part, const double* q_proxy, double eps,
-                              double* const* phi_part, double* const* phi_proxy) {
+// Target panels t with t % stride == 0 only (stride 1: all); used to time a
+// bounded sample of the interaction phase.
+void orc_bve_vel_interactions_sample(void* h, const double* const* disp_part, const double* const* disp_proxy,
+                                     const double* q_part, const double* q_proxy, double eps,
+                                     double* const* phi_part, double* const* phi_proxy, int stride) {
   Ctx& c = *static_cast<Ctx*>(h);
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 1)
   for (int t = 0; t < c.P; t++) {
+    if (t % stride) continue;
     for (int e = c.tgt_start[t]; e < c.tgt_start[t + 1]; e++) {
       int ii = c.by_target[e];
       int source = c.ilist[3 * ii + 1], type = c.ilist[3 * ii + 2];
@@ -691,6 +694,12 @@ void orc_bve_vel_interactions(void* h, const double* const* disp_part, const dou
       }
     }
   }
+}
+
+void orc_bve_vel_interactions(void* h, const double* const* disp_part, const double* const* disp_proxy,
+                              const double* q_part, const double* q_proxy, double eps,
+                              double* const* phi_part, double* const* phi_proxy) {
+  orc_bve_vel_interactions_sample(h, disp_part, disp_proxy, q_part, q_proxy, eps, phi_part, phi_proxy, 1);
 }
 
 // fmm_interactions/inverse_poisson.cpp:9-33 + inverse_poisson_impl.hpp:181-210,

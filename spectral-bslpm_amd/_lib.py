@@ -63,6 +63,8 @@ SIGNATURES = {
     "sbslpm_get_sizes": (_I, [_V, _PI64]),
     "sbslpm_spec_points": (_I, [_V, _PD, _PD, _PD, _PD]),
     "sbslpm_interaction_list": (_I, [_V, _PI]),
+    "sbslpm_ctx_enable_timing": (_I, [_V, _I]),
+    "sbslpm_kernel_time": (_I, [_V, _I, _PD, _PI64, _I]),
     "sbslpm_upward_pass": (_I, [_V, _V, _V, _V, _V]),
     "sbslpm_bve_vel_interactions": (_I, [_V] + [_V] * 14),
     "sbslpm_poisson_fmm_interactions": (_I, [_V, _V, _V, _V, _V]),

@@ -165,6 +165,19 @@ class Context:
         self._check(self._lib.sbslpm_ctx_set_precision(self.h, PRECISION[precision]), "set_precision")
         self.cfg.precision = precision
 
+    KERNELS = {"bve_fp64": 0, "bve_mixed": 1, "log_fp64": 2, "log_mixed": 3}
+
+    def enable_timing(self, on=True):
+        self._check(self._lib.sbslpm_ctx_enable_timing(self.h, 1 if on else 0), "enable_timing")
+
+    def kernel_time(self, kernel, reset=True):
+        """(summed milliseconds, launch count) of one pair-kernel class since the last reset."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self._lib.sbslpm_kernel_time(self.h, self.KERNELS[kernel], ctypes.byref(ms), ctypes.byref(n),
+                                                 1 if reset else 0), "kernel_time")
+        return ms.value, n.value
+
     def synchronize(self):
         self._check(self._lib.sbslpm_synchronize(self.h), "synchronize")
 

@@ -46,6 +46,17 @@ struct sbslpm_ctx {
   double* scratch = nullptr;  // direct sums, grown on demand
   int64_t scratch_len = 0;
 
+  // Kernel timing (sbslpm_ctx_enable_timing).
+  bool timing = false;
+  bool capturing = false;
+  struct Pending {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  double kernel_ms[4] = {};
+  int64_t kernel_launches[4] = {};
+
   // hipGraph of one BVE step, keyed by the caller's pointers.
   hipGraphExec_t graph = nullptr;
   const void* graph_key[10] = {};
@@ -92,6 +103,24 @@ int upload(sbslpm_ctx* c, void* dst, const void* src, size_t bytes) {
 
 int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double* const* dpart,
              const double* const* dprox, double* const* vel);
+
+// Brackets one pair-kernel launch with events when timing is on.
+struct KernelTimer {
+  sbslpm_ctx* c;
+  int kernel;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelTimer(sbslpm_ctx* c_, int k) : c(c_), kernel(k) {
+    if (c->timing && !c->capturing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      hipEventRecord(a, c->stream);
+    else
+      a = b = nullptr;
+  }
+  ~KernelTimer() {
+    if (!a) return;
+    hipEventRecord(b, c->stream);
+    c->pending.push_back({kernel, a, b});
+  }
+};
 
 }  // namespace
 
@@ -226,6 +255,10 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->graph) hipGraphExecDestroy(c->graph);
+  for (auto& p : c->pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
   if (c->scratch) hipFree(c->scratch);
   if (c->arena) hipFree(c->arena);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -234,7 +267,7 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
 
 int sbslpm_ctx_set_stream(sbslpm_ctx* c, void* stream) {
   if (!c) return SBSLPM_E_ARG;
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  c->stream = (hipStream_t)stream;  // NULL: the default (legacy) stream
   if (c->graph) {
     hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -264,6 +297,33 @@ int sbslpm_synchronize(sbslpm_ctx* c) {
     return fail(c, SBSLPM_E_GEOMETRY,
                 (flag & kErrLocate) ? "point location failed (non-finite or off-sphere point)"
                                     : "interpolation point outside its panel");
+  }
+  return SBSLPM_OK;
+}
+
+int sbslpm_ctx_enable_timing(sbslpm_ctx* c, int enable) {
+  if (!c) return SBSLPM_E_ARG;
+  c->timing = enable != 0;
+  return SBSLPM_OK;
+}
+
+int sbslpm_kernel_time(sbslpm_ctx* c, int kernel, double* ms, int64_t* launches, int reset) {
+  if (!c || kernel < 0 || kernel > 3) return SBSLPM_E_ARG;
+  for (auto& p : c->pending) {
+    HIP_TRY(c, hipEventSynchronize(p.b));
+    float t = 0;
+    HIP_TRY(c, hipEventElapsedTime(&t, p.a, p.b));
+    c->kernel_ms[p.kernel] += t;
+    c->kernel_launches[p.kernel] += 1;
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  c->pending.clear();
+  if (ms) *ms = c->kernel_ms[kernel];
+  if (launches) *launches = c->kernel_launches[kernel];
+  if (reset) {
+    c->kernel_ms[kernel] = 0;
+    c->kernel_launches[kernel] = 0;
   }
   return SBSLPM_OK;
 }
@@ -313,9 +373,11 @@ int sbslpm_bve_vel_interactions(sbslpm_ctx* c, const double* dpx, const double* 
   const double* const* dxp = dxx ? dx : nullptr;
   if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records32(c->dt, q_part, q_proxy, dpp, dxp, c->rec32, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, op, ox, c->stream);
   } else {
     launch_source_records64(c->dt, q_part, q_proxy, dpp, dxp, true, c->rec64, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_FP64);
     launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, c->prm.kernel_eps, op, ox, c->stream);
   }
   return launched(c);
@@ -327,10 +389,12 @@ int sbslpm_poisson_fmm_interactions(sbslpm_ctx* c, const double* q_part, const d
     return fail(c, SBSLPM_E_ARG, "poisson_fmm_interactions: null pointer");
   if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records32(c->dt, q_part, q_proxy, nullptr, nullptr, c->rec32, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_LOG_MIXED);
     launch_poisson_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, psi_part, psi_proxy, c->stream);
   } else {
     // the Poisson kernel reads the node coordinates unprojected (inverse_poisson_impl.hpp:199-204)
     launch_source_records64(c->dt, q_part, q_proxy, nullptr, nullptr, false, c->rec64, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_LOG_FP64);
     launch_poisson_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, psi_part, psi_proxy, c->stream);
   }
   return launched(c);
@@ -402,10 +466,12 @@ int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double*
   }
   if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records32(c->dt, c->qpart, c->qproxy, dpart, dprox, c->rec32, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, c->phipart,
                               c->phiproxy, c->stream);
   } else {
     launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->rec64, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_FP64);
     launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, c->prm.kernel_eps, c->phipart,
                           c->phiproxy, c->stream);
   }
@@ -531,7 +597,8 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
   if (!c || !x || !y || !z || !area || !vor || !vx || !vy || !vz || (c->prm.tracer_count > 0 && !tracers))
     return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step: bad argument");
   double* vel0[3] = {vx, vy, vz};
-  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH)) return bve_step_body(c, x, y, z, area, vor, vel0, tracers);
+  // Graph capture needs a non-default stream; on the default stream the step runs eagerly.
+  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH) || !c->stream) return bve_step_body(c, x, y, z, area, vor, vel0, tracers);
 
   const void* key[10] = {x, y, z, area, vor, vx, vy, vz, tracers, (const void*)(intptr_t)c->prm.precision};
   if (!c->graph || std::memcmp(key, c->graph_key, sizeof(key)) != 0) {
@@ -541,7 +608,9 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
     }
     hipGraph_t g = nullptr;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     int rc = bve_step_body(c, x, y, z, area, vor, vel0, tracers);
+    c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
       if (g) hipGraphDestroy(g);
