@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB three times (scale reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -209,6 +211,15 @@ def main():
                           frac=round(achieved / peak, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
                           pairs_per_launch=pairs, flop_per_pair=FLOP_BS)
     log("roofline", roof)
+
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {nbytes} bytes read + written")
 
     cpu = None
     if args.cpu_budget > 0:

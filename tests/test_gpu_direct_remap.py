@@ -43,7 +43,9 @@ def test_direct_sum_inv_lap(c1):
     ctx.synchronize()
     got = to_np(g)
     assert np.all(got[:100] == 0) and np.all(got[5100:] == 0)
-    assert rel(got, expect) < 1e-12
+    # random-sign charges cancel; bound the error by the sum of |contributions|
+    scale = np.abs(orc.direct_sum_inv_lap(x, y, z, area, np.abs(pots), 100, 5100))
+    assert np.max(np.abs(got - expect)) < 1e-13 * np.max(scale)
 
 
 def test_direct_sum_targets_and_bve(c1):
