@@ -134,6 +134,7 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--valu", action="store_true", help="also time the mixed-precision vector-unit pair kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (scale reference for HBM counters)")
     args = ap.parse_args()
@@ -164,6 +165,15 @@ def main():
     log(f"fp64 : {args.steps / ev64:.3f} steps/s ({1e3 * ev64 / args.steps:.2f} ms/step)")
     wallmx, evmx, vormx, velmx = time_steps(ctxmx, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
     log(f"mixed: {args.steps / evmx:.3f} steps/s ({1e3 * evmx / args.steps:.2f} ms/step)")
+    valu = None
+    if args.valu:
+        with torch.cuda.stream(stream):
+            ctxva = sb.Context(sb.RunConfig(precision="mixed_valu", **common))
+        _, evva, vorva, _ = time_steps(ctxva, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
+        valu = dict(steps_per_s=round(args.steps / evva, 4),
+                    vor_rel_l2_after_steps=float(f"{W.rel_l2(vorva.cpu().numpy(), vor64.cpu().numpy()):.3e}"))
+        log(f"mixed (vector unit): {args.steps / evva:.3f} steps/s")
+        del ctxva
 
     # accuracy of mixed vs FP64: one velocity evaluation from the initial field,
     # and the vorticity after the timed steps (both runs start from the same field)
@@ -245,6 +255,8 @@ def main():
         "roofline": roof["bve_mixed"], "roofline_fp64": roof["bve_fp64"],
         "cpu_baseline": cpu,
     }
+    if valu:
+        line["mixed_valu"] = valu
     print(json.dumps(line), flush=True)
 
 

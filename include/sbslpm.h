@@ -41,7 +41,9 @@ enum {
 
 enum { SBSLPM_WEIGHTS_REFERENCE = 0, SBSLPM_WEIGHTS_CLENSHAW_CURTIS = 1, SBSLPM_WEIGHTS_EQUAL_AREA = 2 };
 enum { SBSLPM_RK4_REFERENCE = 0, SBSLPM_RK4_TEXTBOOK = 1 };
-enum { SBSLPM_FP64 = 0, SBSLPM_MIXED = 1 };
+/* SBSLPM_MIXED: FP32 pair arithmetic on the matrix cores (Biot-Savart) / vector unit (log), FP64
+ * accumulation; SBSLPM_MIXED_VALU: the same precision with the Biot-Savart pairs on the vector unit. */
+enum { SBSLPM_FP64 = 0, SBSLPM_MIXED = 1, SBSLPM_MIXED_VALU = 2 };
 enum { SBSLPM_FLAG_GRAPH = 1 };
 
 /* Run parameters: the hot-path subset of RunConfig (run_config.hpp:6-44). */
@@ -51,7 +53,7 @@ typedef struct sbslpm_params {
   int leaf_degree;    /* particle grid degree per leaf (m-1), 1..10; == interp_degree for the reference layout */
   int weight_rule;    /* SBSLPM_WEIGHTS_*: particle quadrature weights (initialize_cubed_sphere.cpp:275) */
   int rk4_combine;    /* SBSLPM_RK4_*: departure-point combine (bve_time_step.cpp:288-303) */
-  int precision;      /* SBSLPM_FP64 or SBSLPM_MIXED pair kernels */
+  int precision;      /* SBSLPM_FP64, SBSLPM_MIXED or SBSLPM_MIXED_VALU pair kernels */
   int tracer_count;   /* RunConfig::tracer_count */
   int flags;          /* bit 0: capture each BVE step in a hipGraph and replay it */
   double theta;       /* RunConfig::fmm_theta */

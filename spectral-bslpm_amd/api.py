@@ -17,7 +17,7 @@ from ._lib import Params, SbslpmError, load
 
 WEIGHTS = {"reference": 0, "clenshaw_curtis": 1, "equal_area": 2}
 COMBINE = {"reference": 0, "textbook": 1}
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_valu": 2}
 
 
 @dataclass
@@ -32,7 +32,7 @@ class RunConfig:
     delta_t: float = 3600.0
     weights: str = "reference"       # reference | clenshaw_curtis | equal_area
     rk4_combine: str = "reference"   # reference | textbook
-    precision: str = "fp64"          # fp64 | mixed
+    precision: str = "fp64"          # fp64 | mixed (matrix cores) | mixed_valu
     tracer_count: int = 0
     use_graph: bool = False
 

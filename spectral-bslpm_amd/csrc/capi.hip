@@ -32,6 +32,7 @@ struct sbslpm_ctx {
   // workspace
   SrcRec64* rec64 = nullptr;
   SrcRec32* rec32 = nullptr;
+  SrcRecMfma* recmf = nullptr;
   double* qpart = nullptr;
   double* qproxy = nullptr;
   double* phipart[3] = {};
@@ -132,7 +133,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   sbslpm_ctx* c = new (std::nothrow) sbslpm_ctx();
   if (!c) return SBSLPM_E_NOMEM;
   c->prm = *p;
-  if (p->precision != SBSLPM_FP64 && p->precision != SBSLPM_MIXED) {
+  if (p->precision < SBSLPM_FP64 || p->precision > SBSLPM_MIXED_VALU) {
     delete c;
     return SBSLPM_E_ARG;
   }
@@ -177,7 +178,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   size_t o_pl = A.take<int>(t.proxy_leaf.size()), o_pxe = A.take<double>(t.proxy_xieta.size());
   size_t o_err = A.take<int>(4);
   size_t o_tiles = A.take<TargetTile>(t.tiles.size()), o_sets = A.take<SourceSet>(t.srcsets.size());
-  size_t o_rec64 = A.take<SrcRec64>(U), o_rec32 = A.take<SrcRec32>(U);
+  size_t o_rec64 = A.take<SrcRec64>(U), o_rec32 = A.take<SrcRec32>(U), o_recmf = A.take<SrcRecMfma>(t.U16);
   size_t o_qpart = A.take<double>(NP), o_qproxy = A.take<double>(Pn);
   size_t o_phipart = A.take<double>(3 * NP), o_phiproxy = A.take<double>(3 * Pn);
   size_t o_disppart = A.take<double>(3 * NP), o_dispproxy = A.take<double>(3 * Pn);
@@ -222,11 +223,13 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   d.err = (int*)(b + o_err);
   d.P = t.P; d.Pl = t.Pl; d.offset = t.offset; d.n = t.n; d.dp1 = t.dp1; d.m = t.m; d.m2 = t.m2;
   d.levels = t.cfg.levels; d.npart = t.npart; d.U = t.U;
+  d.m2p = t.m2p; d.np16 = t.np16; d.U16 = t.U16;
   c->tiles = (TargetTile*)(b + o_tiles);
   c->sets = (SourceSet*)(b + o_sets);
   c->ntiles = (int)t.tiles.size();
   c->rec64 = (SrcRec64*)(b + o_rec64);
   c->rec32 = (SrcRec32*)(b + o_rec32);
+  c->recmf = (SrcRecMfma*)(b + o_recmf);
   c->qpart = (double*)(b + o_qpart);
   c->qproxy = (double*)(b + o_qproxy);
   for (int f = 0; f < 3; ++f) {
@@ -276,7 +279,7 @@ int sbslpm_ctx_set_stream(sbslpm_ctx* c, void* stream) {
 }
 
 int sbslpm_ctx_set_precision(sbslpm_ctx* c, int precision) {
-  if (!c || (precision != SBSLPM_FP64 && precision != SBSLPM_MIXED)) return SBSLPM_E_ARG;
+  if (!c || precision < SBSLPM_FP64 || precision > SBSLPM_MIXED_VALU) return SBSLPM_E_ARG;
   if (precision != c->prm.precision && c->graph) {
     hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -372,6 +375,10 @@ int sbslpm_bve_vel_interactions(sbslpm_ctx* c, const double* dpx, const double* 
   const double* const* dpp = dpx ? dp : nullptr;
   const double* const* dxp = dxx ? dx : nullptr;
   if (c->prm.precision == SBSLPM_MIXED) {
+    launch_source_records_mfma(c->dt, q_part, q_proxy, dpp, dxp, c->recmf, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
+    launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, c->recmf, c->prm.kernel_eps, op, ox, c->stream);
+  } else if (c->prm.precision == SBSLPM_MIXED_VALU) {
     launch_source_records32(c->dt, q_part, q_proxy, dpp, dxp, c->rec32, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, op, ox, c->stream);
@@ -387,7 +394,7 @@ int sbslpm_poisson_fmm_interactions(sbslpm_ctx* c, const double* q_part, const d
                                     double* psi_proxy) {
   if (!c || !q_part || !q_proxy || !psi_part || !psi_proxy)
     return fail(c, SBSLPM_E_ARG, "poisson_fmm_interactions: null pointer");
-  if (c->prm.precision == SBSLPM_MIXED) {
+  if (c->prm.precision != SBSLPM_FP64) {
     launch_source_records32(c->dt, q_part, q_proxy, nullptr, nullptr, c->rec32, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_LOG_MIXED);
     launch_poisson_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, psi_part, psi_proxy, c->stream);
@@ -465,6 +472,11 @@ int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double*
     launch_fill_zero(c->phiproxy[f], Pn, c->stream);
   }
   if (c->prm.precision == SBSLPM_MIXED) {
+    launch_source_records_mfma(c->dt, c->qpart, c->qproxy, dpart, dprox, c->recmf, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
+    launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, c->recmf, c->prm.kernel_eps, c->phipart,
+                             c->phiproxy, c->stream);
+  } else if (c->prm.precision == SBSLPM_MIXED_VALU) {
     launch_source_records32(c->dt, c->qpart, c->qproxy, dpart, dprox, c->rec32, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, c->phipart,

@@ -34,6 +34,8 @@ struct DevTree {
   int* err;
   int P, Pl, offset, n, dp1, m, m2, levels;
   int64_t npart, U;
+  int m2p, np16;  // m2, n rounded up to 16
+  int64_t U16;    // padded record count of the MFMA kernels
 };
 
 // Source record of the FP64 pair kernels: displaced, re-projected source
@@ -46,6 +48,13 @@ struct SrcRec64 {
 // source panel centre and the scaled charge, FP32.
 struct SrcRec32 {
   float x, y, z, q;
+};
+
+// Source record of the MFMA pair kernel, 16-padded per source set:
+// b = position relative to the source-panel centre, beta = |b|^2/2 and the
+// charge-weighted (q b, q) with q = -charge/(4 pi); padding slots are zero.
+struct SrcRecMfma {
+  float bx, by, bz, beta, qbx, qby, qbz, q;
 };
 
 struct LaunchCfg {
@@ -76,6 +85,12 @@ void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int nt
                                double* const* out_proxy, hipStream_t s);
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                    const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s);
+void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
+                                const double* const* disp_part, const double* const* disp_proxy, SrcRecMfma* rec,
+                                hipStream_t s);
+void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                              const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
+                              hipStream_t s);
 
 // ---- displacement, departure points, remap (remap.hip)
 void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,

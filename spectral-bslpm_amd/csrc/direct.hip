@@ -48,6 +48,10 @@ __global__ __launch_bounds__(256) void k_direct_log(int64_t nt, const double* __
                                                     const double* __restrict__ area, const double* __restrict__ pots,
                                                     int64_t self_offset, int guard, int64_t chunk_len,
                                                     double* __restrict__ partials) {
+  // x.y rounded exactly as written in direct_sum_impl.hpp:23 (no fused
+  // multiply-add): for nearly coincident points log(1 - x.y) amplifies a
+  // one-ulp change of x.y by 1/(1 - x.y).
+#pragma clang fp contract(off)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int64_t tile = (int64_t)blockIdx.x * kWaves + wave;

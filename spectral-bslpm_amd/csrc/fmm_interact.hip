@@ -235,6 +235,153 @@ __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* 
   }
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Records of the MFMA kernel (16-padded per source set; padding slots zero).
+__global__ void k_src_mfma(DevTree t, const double* __restrict__ q_part, const double* __restrict__ q_proxy,
+                           CPtr3 dpart, CPtr3 dprox, SrcRecMfma* __restrict__ rec) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= t.U16) return;
+  const int64_t lp = (int64_t)t.Pl * t.m2p;
+  int panel, k;
+  int64_t u;
+  bool valid;
+  if (s < lp) {
+    const int leaf = (int)(s / t.m2p);
+    k = (int)(s % t.m2p);
+    panel = t.offset + leaf;
+    valid = k < t.m2;
+    u = (int64_t)leaf * t.m2 + k;
+  } else {
+    const int64_t v = s - lp;
+    panel = (int)(v / t.np16);
+    k = (int)(v % t.np16);
+    valid = k < t.n;
+    u = t.npart + (int64_t)panel * t.n + k;
+  }
+  SrcRecMfma r{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (valid) {
+    double x = t.px[u], y = t.py[u], z = t.pz[u], q;
+    if (u < t.npart) {
+      if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
+      q = q_part[u];
+    } else {
+      const int64_t v = u - t.npart;
+      if (dprox.p[0]) { x += dprox.p[0][v]; y += dprox.p[1][v]; z += dprox.p[2][v]; }
+      q = q_proxy[v];
+    }
+    project_to_sphere(x, y, z);
+    const DevPanel& P = t.panels[panel];
+    const float bx = (float)(x - P.cx), by = (float)(y - P.cy), bz = (float)(z - P.cz);
+    const double qq = kNegInv4Pi * q;
+    r.bx = bx; r.by = by; r.bz = bz;
+    r.beta = (float)(0.5 * ((double)bx * bx + (double)by * by + (double)bz * bz));
+    r.qbx = (float)(qq * bx); r.qby = (float)(qq * by); r.qbz = (float)(qq * bz); r.q = (float)qq;
+  }
+  rec[s] = r;
+}
+
+// Mixed-precision Biot-Savart pair kernel on the matrix cores.
+//
+// Per wave: up to 4 target tiles of 16 (64 targets of one panel).  For every
+// source set, coordinates are taken relative to the source-panel centre c_s:
+// target p = (x - c_t) + (c_t - c_s), source b (record).  For a 16 x 16 tile
+//   D = alpha_j + beta_i - p_j.b_i = eps + |p_j - b_i|^2 / 2  (= 1 - x.y + eps)
+// is one v_mfma_f32_16x16x4_f32 (A = (b, beta) per source, B = (-p, 1) per
+// target, C = alpha per target column); R = 1/D on the vector unit; then
+// four v_mfma_f32_4x4x1_16b_f32 accumulate W_j = sum_i R_ij (q b_i, q),
+// 4 components x 16 targets per source quarter.  After each source set the
+// quarters are summed across lanes and w_j += W_xyz - p_j * W_q is added in
+// FP64; finally u = x cross w in FP64 (x cross y = x cross (y - x)).
+// Lane layouts: A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k,
+// D[i][j] in lane j + 16 (i / 4), register i % 4 (16x16x4); block b of the
+// 4x4x1 form uses lanes 4 b .. 4 b + 3 (A row / B column = lane % 4, D row =
+// register).
+__global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                  const SourceSet* __restrict__ sets,
+                                                  const float* __restrict__ rec, float eps, Ptr3 part,
+                                                  Ptr3 proxy) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const DevPanel& PT = t.panels[T.panel];
+  const int ntt = (T.count + 15) >> 4;
+  float ax[4], ay[4], az[4];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) {
+    const int j = min(tt * 16 + col, T.count - 1);
+    const int64_t u = (int64_t)T.first + j;
+    ax[tt] = (float)(t.px[u] - PT.cx);
+    ay[tt] = (float)(t.py[u] - PT.cy);
+    az[tt] = (float)(t.pz[u] - PT.cz);
+  }
+  double w[4][3];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) w[tt][0] = w[tt][1] = w[tt][2] = 0.0;
+
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const DevPanel& PS = t.panels[S.panel];
+    const float dx = (float)(PT.cx - PS.cx), dy = (float)(PT.cy - PS.cy), dz = (float)(PT.cz - PS.cz);
+    float px[4], py[4], pz[4], b1[4];
+    v4f c1[4], acc[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      px[tt] = ax[tt] + dx;
+      py[tt] = ay[tt] + dy;
+      pz[tt] = az[tt] + dz;
+      const float alpha = fmaf(0.5f, fmaf(px[tt], px[tt], fmaf(py[tt], py[tt], pz[tt] * pz[tt])), eps);
+      b1[tt] = grp == 0 ? -px[tt] : grp == 1 ? -py[tt] : grp == 2 ? -pz[tt] : 1.0f;
+      c1[tt] = v4f{alpha, alpha, alpha, alpha};
+      acc[tt] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    const float* __restrict__ r = rec + (int64_t)S.first16 * 8;
+    const int nst = (S.count + 15) >> 4;
+    for (int st = 0; st < nst; ++st, r += 128) {
+      const float a1 = r[col * 8 + grp];
+      float bq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bq[q] = r[(4 * grp + q) * 8 + 4 + (lane & 3)];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) {
+        if (tt < ntt) {
+          const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], c1[tt], 0, 0, 0);
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[tt] = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q], __builtin_amdgcn_rcpf(d[q]), acc[tt], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      if (tt < ntt) {
+        v4f v = acc[tt];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          v[c] += __shfl_xor(v[c], 16);
+          v[c] += __shfl_xor(v[c], 32);
+        }
+        w[tt][0] += (double)fmaf(-px[tt], v[3], v[0]);
+        w[tt][1] += (double)fmaf(-py[tt], v[3], v[1]);
+        w[tt][2] += (double)fmaf(-pz[tt], v[3], v[2]);
+      }
+    }
+  }
+  // lane group g stores target tile g
+  const int j = grp * 16 + col;
+  if (grp < ntt && j < T.count) {
+    const double w0 = grp == 0 ? w[0][0] : grp == 1 ? w[1][0] : grp == 2 ? w[2][0] : w[3][0];
+    const double w1 = grp == 0 ? w[0][1] : grp == 1 ? w[1][1] : grp == 2 ? w[2][1] : w[3][1];
+    const double w2 = grp == 0 ? w[0][2] : grp == 1 ? w[1][2] : grp == 2 ? w[2][2] : w[3][2];
+    const int64_t u = (int64_t)T.first + j;
+    const double x = t.px[u], y = t.py[u], z = t.pz[u];
+    store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0);
+  }
+}
+
 inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
 inline CPtr3 cptr3(const double* const* a) {
@@ -286,6 +433,21 @@ void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int nt
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
                      tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy));
+}
+
+void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
+                                const double* const* disp_part, const double* const* disp_proxy, SrcRecMfma* rec,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_src_mfma, dim3(blocks_for(t.U16, 256)), dim3(256), 0, s, t, q_part, q_proxy,
+                     cptr3(disp_part), cptr3(disp_proxy), rec);
+}
+
+void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                              const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
+                              hipStream_t s) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
+                     tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy));
 }
 
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,

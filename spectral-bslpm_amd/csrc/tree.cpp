@@ -206,6 +206,9 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
   traverse(t);
   t.npart = (int64_t)t.Pl * t.m2;
   t.U = t.npart + (int64_t)t.P * t.n;
+  t.m2p = (t.m2 + 15) / 16 * 16;
+  t.np16 = (t.n + 15) / 16 * 16;
+  t.U16 = (int64_t)t.Pl * t.m2p + (int64_t)t.offset * t.np16;
 
   // Device panels.
   t.dev.resize(t.P);
@@ -277,8 +280,15 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
     int s = t.ilist[3 * order[k] + 1];
     SourceSet ss{};
     ss.panel = s;
-    if (t.pn[s].is_leaf) { ss.first = (int)((int64_t)(s - t.offset) * t.m2); ss.count = t.m2; }
-    else { ss.first = (int)(t.npart + (int64_t)s * t.n); ss.count = t.n; }
+    if (t.pn[s].is_leaf) {
+      ss.first = (int)((int64_t)(s - t.offset) * t.m2);
+      ss.count = t.m2;
+      ss.first16 = (int)((int64_t)(s - t.offset) * t.m2p);
+    } else {
+      ss.first = (int)(t.npart + (int64_t)s * t.n);
+      ss.count = t.n;
+      ss.first16 = (int)((int64_t)t.Pl * t.m2p + (int64_t)s * t.np16);
+    }
     t.srcsets[k] = ss;
   }
   t.tiles.clear();
@@ -286,9 +296,11 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
     if (cnt[p + 1] == cnt[p]) continue;
     int npts = t.pn[p].is_leaf ? t.m2 : t.n;
     int64_t base = t.pn[p].is_leaf ? (int64_t)(p - t.offset) * t.m2 : t.npart + (int64_t)p * t.n;
-    for (int f = 0; f < npts; f += 64) {
+    // balanced chunks of at most 64 points (81 -> 41 + 40, i.e. 3 + 3 MFMA tiles of 16)
+    int chunks = (npts + 63) / 64, per = (npts + chunks - 1) / chunks;
+    for (int f = 0; f < npts; f += per) {
       TargetTile tl{};
-      tl.panel = p; tl.first = (int)(base + f); tl.count = std::min(64, npts - f);
+      tl.panel = p; tl.first = (int)(base + f); tl.count = std::min(per, npts - f);
       tl.src_begin = cnt[p]; tl.src_end = cnt[p + 1];
       t.tiles.push_back(tl);
     }

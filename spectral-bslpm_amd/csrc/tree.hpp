@@ -46,15 +46,18 @@ struct TargetTile {
 };
 
 // A source point set: `count` consecutive records starting at `first` in the
-// unified point index space, belonging to panel `panel`.
+// unified point index space, belonging to panel `panel`; `first16` is its
+// first record in the 16-padded record space of the MFMA kernels.
 struct SourceSet {
-  int first, count, panel, pad;
+  int first, count, panel, first16;
 };
 
 struct Tree {
   TreeConfig cfg{};
   int P = 0, Pl = 0, offset = 0, n = 0, dp1 = 0, m = 0, m2 = 0;
   int64_t npart = 0, U = 0;        // Pl*m2, Pl*m2 + P*n
+  int m2p = 0, np16 = 0;           // m2 and n rounded up to multiples of 16
+  int64_t U16 = 0;                 // 16-padded record space: Pl*m2p + offset*np16
   std::vector<int> level_start;    // [L+1]
   std::vector<HostPanel> pn;       // [P]
   std::vector<double> cheb_d, cheb_m;  // cos(pi i/deg) as interp_vals_bli evaluates it (:45)

@@ -74,6 +74,30 @@ def test_bve_vel_interactions(case, displaced):
         assert rel(to_np(gx[f])[nonleaf], ex[f][nonleaf]) < 1e-12
 
 
+@pytest.mark.parametrize("mode", ["mixed", "mixed_valu"])
+def test_bve_vel_interactions_mixed(case, mode):
+    """Mixed-precision pair sums (displaced sources) against the FP64 oracle,
+    per output class (leaf particles, non-leaf proxies)."""
+    ctx, o, t = case["ctx"], case["o"], case["torch"]
+    rp, rx = o.upward_pass(case["a"], case["vor"])
+    rng = np.random.default_rng(13)
+    dp = [1e-3 * rng.normal(size=(o.Pl, o.m2)) for _ in range(3)]
+    dx = [1e-3 * rng.normal(size=(o.P, o.n)) for _ in range(3)]
+    ep, ex = o.bve_vel_interactions(rp, rx, EPS, dp, dx)
+    ctx.set_precision(mode)
+    try:
+        gp, gx = ctx.bve_vel_interactions(to_gpu(t, rp), to_gpu(t, rx), [to_gpu(t, v) for v in dp],
+                                          [to_gpu(t, v) for v in dx])
+        ctx.synchronize()
+    finally:
+        ctx.set_precision("fp64")
+    W = case["sb"].workloads
+    e1 = W.vec_rel_l2([to_np(g) for g in gp], ep)
+    e2 = W.vec_rel_l2([to_np(g)[: o.offset] for g in gx], [v[: o.offset] for v in ex])
+    print(f"  {mode}: particles {e1:.2e}, proxies {e2:.2e}")
+    assert e1 < 1e-6 and e2 < 1e-6
+
+
 def test_poisson_fmm_interactions(case):
     ctx, o, t = case["ctx"], case["o"], case["torch"]
     rp, rx = o.upward_pass(case["a"], case["vor"])
@@ -108,13 +132,14 @@ def test_bve_velocity_fp64_and_mixed(case):
     v64 = [to_np(v) for v in ctx.bve_velocity(ga, gv)]
     ctx.synchronize()
     assert case["sb"].workloads.vec_rel_l2(v64, expect) < 1e-12
-    ctx.set_precision("mixed")
-    vmx = [to_np(v) for v in ctx.bve_velocity(ga, gv)]
-    ctx.synchronize()
-    ctx.set_precision("fp64")
-    err = case["sb"].workloads.vec_rel_l2(vmx, v64)
-    print(f"  mixed vs fp64 velocity rel-L2 {err:.2e}")
-    assert err < 1e-6
+    for mode in ("mixed", "mixed_valu"):
+        ctx.set_precision(mode)
+        vmx = [to_np(v) for v in ctx.bve_velocity(ga, gv)]
+        ctx.synchronize()
+        ctx.set_precision("fp64")
+        err = case["sb"].workloads.vec_rel_l2(vmx, v64)
+        print(f"  {mode} vs fp64 velocity rel-L2 {err:.2e}")
+        assert err < 1e-6
 
 
 def test_poisson_solve_fp64_and_mixed(case):
