@@ -16,6 +16,9 @@ LIB = os.path.join(HERE, "libsbslpm.so")
 ARCH = os.environ.get("SBSLPM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["fmm_passes.hip", "fmm_interact.hip", "remap.hip", "direct.hip", "capi.hip"]
+# MFMA results in VGPRs (gfx950 reads them from the unified register file), so
+# the vector unit can take the reciprocal without accumulator-register copies.
+EXTRA = {"fmm_interact.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HOST_SOURCES = ["tree.cpp", "tree_api.cpp"]
 
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-I" + CSRC]
@@ -35,7 +38,7 @@ def build(verbose=False):
     jobs = []
     for s in HIP_SOURCES:
         obj = os.path.join(BUILD, s + ".o")
-        jobs.append(["hipcc", "-x", "hip", "-c", os.path.join(CSRC, s), "-o", obj] + COMMON + DEVICE)
+        jobs.append(["hipcc", "-x", "hip", "-c", os.path.join(CSRC, s), "-o", obj] + COMMON + DEVICE + EXTRA.get(s, []))
     for s in HOST_SOURCES:
         obj = os.path.join(BUILD, s + ".o")
         jobs.append(["hipcc", "-c", os.path.join(CSRC, s), "-o", obj, "-ffp-contract=off"] + COMMON)

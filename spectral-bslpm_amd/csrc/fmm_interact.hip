@@ -283,20 +283,92 @@ __global__ void k_src_mfma(DevTree t, const double* __restrict__ q_part, const d
 
 // Mixed-precision Biot-Savart pair kernel on the matrix cores.
 //
-// Per wave: up to 4 target tiles of 16 (64 targets of one panel).  For every
-// source set, coordinates are taken relative to the source-panel centre c_s:
-// target p = (x - c_t) + (c_t - c_s), source b (record).  For a 16 x 16 tile
+// Per wave: NTT (1..4) target tiles of 16 (up to 64 targets of one panel).
+// For every source set, coordinates are taken relative to the source-panel
+// centre c_s: target p = (x - c_t) + (c_t - c_s), source b (record).  For a
+// 16 x 16 tile
 //   D = alpha_j + beta_i - p_j.b_i = eps + |p_j - b_i|^2 / 2  (= 1 - x.y + eps)
 // is one v_mfma_f32_16x16x4_f32 (A = (b, beta) per source, B = (-p, 1) per
-// target, C = alpha per target column); R = 1/D on the vector unit; then
-// four v_mfma_f32_4x4x1_16b_f32 accumulate W_j = sum_i R_ij (q b_i, q),
-// 4 components x 16 targets per source quarter.  After each source set the
-// quarters are summed across lanes and w_j += W_xyz - p_j * W_q is added in
-// FP64; finally u = x cross w in FP64 (x cross y = x cross (y - x)).
+// target) plus alpha per target column; R = 1/D on the vector unit; then four
+// v_mfma_f32_4x4x1_16b_f32 accumulate W_j = sum_i R_ij (q b_i, q), 4
+// components x 16 targets per quarter of the source tile.  After each source
+// set every lane adds its quarter's w_j += W_xyz - p_j W_q into FP64; the four
+// quarters are summed across lanes once at the end, and u = x cross w in FP64
+// (x cross y = x cross (y - x)).
 // Lane layouts: A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k,
 // D[i][j] in lane j + 16 (i / 4), register i % 4 (16x16x4); block b of the
 // 4x4x1 form uses lanes 4 b .. 4 b + 3 (A row / B column = lane % 4, D row =
-// register).
+// register).  The next source tile's operands are loaded before the current
+// tile's arithmetic (every index stays inside the set's padded records).
+template <int NTT>
+__device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, const SourceSet* __restrict__ sets,
+                                      const float* __restrict__ rec, float eps, int lane, double (&w)[4][3]) {
+  const int col = lane & 15, grp = lane >> 4;
+  const DevPanel& PT = t.panels[T.panel];
+  float ax[NTT], ay[NTT], az[NTT];
+#pragma unroll
+  for (int tt = 0; tt < NTT; ++tt) {
+    const int j = min(tt * 16 + col, T.count - 1);
+    const int64_t u = (int64_t)T.first + j;
+    ax[tt] = (float)(t.px[u] - PT.cx);
+    ay[tt] = (float)(t.py[u] - PT.cy);
+    az[tt] = (float)(t.pz[u] - PT.cz);
+  }
+  const int off_a = col * 8 + grp;                // A operand: component grp of source col
+  const int off_b = (4 * grp) * 8 + 4 + (lane & 3);  // B2 operand q: (q b, q)[lane%4] of source 4 grp + q
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const DevPanel& PS = t.panels[S.panel];
+    const float dx = (float)(PT.cx - PS.cx), dy = (float)(PT.cy - PS.cy), dz = (float)(PT.cz - PS.cz);
+    float px[NTT], py[NTT], pz[NTT], b1[NTT], alpha[NTT];
+    v4f acc[NTT];
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt) {
+      px[tt] = ax[tt] + dx;
+      py[tt] = ay[tt] + dy;
+      pz[tt] = az[tt] + dz;
+      alpha[tt] = fmaf(0.5f, fmaf(px[tt], px[tt], fmaf(py[tt], py[tt], pz[tt] * pz[tt])), eps);
+      b1[tt] = grp == 0 ? -px[tt] : grp == 1 ? -py[tt] : grp == 2 ? -pz[tt] : 1.0f;
+      acc[tt] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    const float* __restrict__ r = rec + (int64_t)S.first16 * 8;
+    const int nst = (S.count + 15) >> 4;
+    float a1 = r[off_a];
+    float bq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bq[q] = r[off_b + 8 * q];
+    for (int st = 0; st < nst; ++st) {
+      const float* __restrict__ rn = r + 128 * min(st + 1, nst - 1);
+      const float na1 = rn[off_a];
+      float nbq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) nbq[q] = rn[off_b + 8 * q];
+      v4f d[NTT];
+#pragma unroll
+      for (int tt = 0; tt < NTT; ++tt)
+        d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+      for (int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[tt][q] = __builtin_amdgcn_rcpf(d[tt][q] + alpha[tt]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int tt = 0; tt < NTT; ++tt)
+          acc[tt] = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q], d[tt][q], acc[tt], 0, 0, 0);
+      a1 = na1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bq[q] = nbq[q];
+    }
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt) {
+      w[tt][0] += (double)fmaf(-px[tt], acc[tt][3], acc[tt][0]);
+      w[tt][1] += (double)fmaf(-py[tt], acc[tt][3], acc[tt][1]);
+      w[tt][2] += (double)fmaf(-pz[tt], acc[tt][3], acc[tt][2]);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                   const SourceSet* __restrict__ sets,
                                                   const float* __restrict__ rec, float eps, Ptr3 part,
@@ -307,69 +379,24 @@ __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* _
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
-  const DevPanel& PT = t.panels[T.panel];
   const int ntt = (T.count + 15) >> 4;
-  float ax[4], ay[4], az[4];
-#pragma unroll
-  for (int tt = 0; tt < 4; ++tt) {
-    const int j = min(tt * 16 + col, T.count - 1);
-    const int64_t u = (int64_t)T.first + j;
-    ax[tt] = (float)(t.px[u] - PT.cx);
-    ay[tt] = (float)(t.py[u] - PT.cy);
-    az[tt] = (float)(t.pz[u] - PT.cz);
-  }
   double w[4][3];
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) w[tt][0] = w[tt][1] = w[tt][2] = 0.0;
-
-  for (int e = T.src_begin; e < T.src_end; ++e) {
-    const SourceSet S = sets[e];
-    const DevPanel& PS = t.panels[S.panel];
-    const float dx = (float)(PT.cx - PS.cx), dy = (float)(PT.cy - PS.cy), dz = (float)(PT.cz - PS.cz);
-    float px[4], py[4], pz[4], b1[4];
-    v4f c1[4], acc[4];
-#pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-      px[tt] = ax[tt] + dx;
-      py[tt] = ay[tt] + dy;
-      pz[tt] = az[tt] + dz;
-      const float alpha = fmaf(0.5f, fmaf(px[tt], px[tt], fmaf(py[tt], py[tt], pz[tt] * pz[tt])), eps);
-      b1[tt] = grp == 0 ? -px[tt] : grp == 1 ? -py[tt] : grp == 2 ? -pz[tt] : 1.0f;
-      c1[tt] = v4f{alpha, alpha, alpha, alpha};
-      acc[tt] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
-    const float* __restrict__ r = rec + (int64_t)S.first16 * 8;
-    const int nst = (S.count + 15) >> 4;
-    for (int st = 0; st < nst; ++st, r += 128) {
-      const float a1 = r[col * 8 + grp];
-      float bq[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) bq[q] = r[(4 * grp + q) * 8 + 4 + (lane & 3)];
-#pragma unroll
-      for (int tt = 0; tt < 4; ++tt) {
-        if (tt < ntt) {
-          const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], c1[tt], 0, 0, 0);
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            acc[tt] = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q], __builtin_amdgcn_rcpf(d[q]), acc[tt], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-      if (tt < ntt) {
-        v4f v = acc[tt];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          v[c] += __shfl_xor(v[c], 16);
-          v[c] += __shfl_xor(v[c], 32);
-        }
-        w[tt][0] += (double)fmaf(-px[tt], v[3], v[0]);
-        w[tt][1] += (double)fmaf(-py[tt], v[3], v[1]);
-        w[tt][2] += (double)fmaf(-pz[tt], v[3], v[2]);
-      }
-    }
+  switch (ntt) {
+    case 4: bve_mfma_tiles<4>(t, T, sets, rec, eps, lane, w); break;
+    case 3: bve_mfma_tiles<3>(t, T, sets, rec, eps, lane, w); break;
+    case 2: bve_mfma_tiles<2>(t, T, sets, rec, eps, lane, w); break;
+    default: bve_mfma_tiles<1>(t, T, sets, rec, eps, lane, w); break;
   }
+  // sum the four source quarters (lanes col, col+16, col+32, col+48)
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      w[tt][k] += __shfl_xor(w[tt][k], 16);
+      w[tt][k] += __shfl_xor(w[tt][k], 32);
+    }
   // lane group g stores target tile g
   const int j = grp * 16 + col;
   if (grp < ntt && j < T.count) {

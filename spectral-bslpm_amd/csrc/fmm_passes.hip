@@ -22,16 +22,9 @@ struct CPtr3 {
   const double* p[3];
 };
 
-// out(q, p) = sum_r X[q*R + r] * sum_s Y[p*R + s] * in[r*R + s]
-__device__ inline double contract(const double* X, const double* Y, const double* in, int R, int q, int p) {
-  double acc = 0;
-  for (int r = 0; r < R; ++r) {
-    double s = 0;
-    for (int c = 0; c < R; ++c) s += Y[p * R + c] * in[r * R + c];
-    acc += X[q * R + r] * s;
-  }
-  return acc;
-}
+// The transfers are tensor products, out = X in Y^T with small dense X, Y;
+// every kernel below evaluates them as two 1-D sweeps through LDS
+// (R^2 (R + Q) instead of Q^2 R^2 multiply-adds).
 
 // q_part = area * pots (base_pots, fmm_funcs_impl.hpp:29-33).
 __global__ void k_base_pots(int64_t npart, const double* __restrict__ area, const double* __restrict__ pots,
@@ -40,28 +33,36 @@ __global__ void k_base_pots(int64_t npart, const double* __restrict__ area, cons
   if (u < npart) q_part[u] = area[u] * pots[u];
 }
 
-// Leaf proxies from the particle grid: Q_leaf = T Q_part T^T, T[a][k] = l_a(x_k).
-// For the reference layout (m == d+1) T is the identity and this is the
-// base_pots copy into the leaf rows.
+// Leaf proxies from the particle grid: Q_leaf[a][b] = sum_kl T[a][k] T[b][l] q[k][l],
+// T[a][k] = l_a(x_k) (base_pots_ll, fmm_funcs_impl.hpp:207-223, on the leaf grid).
 __global__ __launch_bounds__(128) void k_p2m(DevTree t, const double* __restrict__ q_part,
                                              double* __restrict__ q_proxy) {
-  __shared__ double sin_[kMaxN];
-  __shared__ double sT[kMaxN];
+  __shared__ double sq[kMaxN], sv[kMaxN], sT[kMaxN];
   const int leaf = blockIdx.x;
   const int m = t.m, m2 = t.m2, dp1 = t.dp1, n = t.n;
-  for (int i = threadIdx.x; i < m2; i += blockDim.x) sin_[i] = q_part[(int64_t)leaf * m2 + i];
+  for (int i = threadIdx.x; i < m2; i += blockDim.x) sq[i] = q_part[(int64_t)leaf * m2 + i];
   for (int i = threadIdx.x; i < dp1 * m; i += blockDim.x) sT[i] = t.p2m[i];
   __syncthreads();
-  for (int o = threadIdx.x; o < n; o += blockDim.x) {
-    double v = contract(sT, sT, sin_, m, o / dp1, o % dp1);
+  for (int o = threadIdx.x; o < m * dp1; o += blockDim.x) {  // V[k][b] = sum_l T[b][l] q[k][l]
+    const int k = o / dp1, b = o % dp1;
+    double v = 0;
+    for (int l = 0; l < m; ++l) v += sT[b * m + l] * sq[k * m + l];
+    sv[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < n; o += blockDim.x) {  // Q[a][b] = sum_k T[a][k] V[k][b]
+    const int a = o / dp1, b = o % dp1;
+    double v = 0;
+    for (int k = 0; k < m; ++k) v += sT[a * m + k] * sv[k * dp1 + b];
     q_proxy[(int64_t)(t.offset + leaf) * n + o] = v;
   }
 }
 
-// M2M for all parents of one level (child_to_parent, fmm_funcs_impl.hpp:47-70),
-// assigning the parent rows.
+// M2M for all parents of one level (child_to_parent, fmm_funcs_impl.hpp:47-70):
+// Q_p[a][b] = sum_c sum_kl X_c[a][k] Y_c[b][l] Q_c[k][l], assigning the parent rows.
+// X_c / Y_c = upper- or lower-half matrix in xi / eta (children 1,4 upper in xi; 1,2 upper in eta).
 __global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, double* __restrict__ q) {
-  __shared__ double sq[4 * kMaxN];
+  __shared__ double sq[4 * kMaxN], sv[4 * kMaxN];
   __shared__ double shi[kMaxN], slo[kMaxN];
   const int par = pbegin + blockIdx.x;
   const int n = t.n, dp1 = t.dp1;
@@ -69,61 +70,89 @@ __global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, double* __re
   for (int i = threadIdx.x; i < 4 * n; i += blockDim.x) sq[i] = q[(int64_t)c0 * n + i];
   for (int i = threadIdx.x; i < n; i += blockDim.x) { shi[i] = t.m2m_hi[i]; slo[i] = t.m2m_lo[i]; }
   __syncthreads();
-  for (int o = threadIdx.x; o < n; o += blockDim.x) {
+  for (int o = threadIdx.x; o < 4 * n; o += blockDim.x) {  // V_c[k][b] = sum_l Y_c[b][l] Q_c[k][l]
+    const int c = o / n, k = (o % n) / dp1, b = o % dp1;
+    const double* Y = (c <= 1) ? shi : slo;
+    const double* Q = sq + c * n;
+    double v = 0;
+    for (int l = 0; l < dp1; ++l) v += Y[b * dp1 + l] * Q[k * dp1 + l];
+    sv[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < n; o += blockDim.x) {  // Q_p[a][b] = sum_c sum_k X_c[a][k] V_c[k][b]
     const int a = o / dp1, b = o % dp1;
-    double acc = 0;
+    double v = 0;
     for (int c = 0; c < 4; ++c) {
-      const double* X = (c == 0 || c == 3) ? shi : slo;  // xi: children 1,4 upper half
-      const double* Y = (c <= 1) ? shi : slo;            // eta: children 1,2 upper half
-      acc += contract(X, Y, sq + c * n, dp1, a, b);
+      const double* X = (c == 0 || c == 3) ? shi : slo;
+      for (int k = 0; k < dp1; ++k) v += X[a * dp1 + k] * sv[c * n + k * dp1 + b];
     }
-    q[(int64_t)par * n + o] = acc;
+    q[(int64_t)par * n + o] = v;
   }
 }
 
 // L2L for all parents of one level (parent_to_child_3, fmm_funcs_impl.hpp:138-171),
-// accumulating into the four children.
+// accumulating into the four children:
+// phi_c[k][l] += sum_a X_c[a][k] sum_b Y_c[b][l] phi_p[a][b].  The eta sweep
+// depends only on the eta half, so it is done once for the upper and once
+// for the lower half.
 __global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, Ptr3 phi) {
-  __shared__ double sp[3 * kMaxN];
-  __shared__ double shT[kMaxN], slT[kMaxN];
+  __shared__ double sp[3 * kMaxN], sv[3 * 2 * kMaxN];
+  __shared__ double shi[kMaxN], slo[kMaxN];
   const int par = pbegin + blockIdx.x;
   const int n = t.n, dp1 = t.dp1;
   const int c0 = t.panels[par].child0;
   for (int f = 0; f < ncomp; ++f)
     for (int i = threadIdx.x; i < n; i += blockDim.x) sp[f * n + i] = phi.p[f][(int64_t)par * n + i];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) { shT[i] = t.m2m_hiT[i]; slT[i] = t.m2m_loT[i]; }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) { shi[i] = t.m2m_hi[i]; slo[i] = t.m2m_lo[i]; }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) {
-    const int c = idx / n, node = idx % n;
-    const double* X = (c == 0 || c == 3) ? shT : slT;
-    const double* Y = (c <= 1) ? shT : slT;
-    const int k = node / dp1, l = node % dp1;
-    for (int f = 0; f < ncomp; ++f) {
-      double v = contract(X, Y, sp + f * n, dp1, k, l);
-      phi.p[f][(int64_t)(c0 + c) * n + node] += v;
-    }
+  for (int o = threadIdx.x; o < ncomp * 2 * n; o += blockDim.x) {  // V_h[a][l] = sum_b Y_h[b][l] phi[a][b]
+    const int f = o / (2 * n), h = (o / n) & 1, a = (o % n) / dp1, l = o % dp1;
+    const double* Y = h ? slo : shi;
+    const double* P = sp + f * n;
+    double v = 0;
+    for (int b = 0; b < dp1; ++b) v += Y[b * dp1 + l] * P[a * dp1 + b];
+    sv[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < ncomp * 4 * n; o += blockDim.x) {  // phi_c[k][l] += sum_a X_c[a][k] V_h(c)[a][l]
+    const int f = o / (4 * n), c = (o / n) & 3, k = (o % n) / dp1, l = o % dp1;
+    const double* X = (c == 0 || c == 3) ? shi : slo;
+    const double* V = sv + (f * 2 + (c <= 1 ? 0 : 1)) * n;
+    double v = 0;
+    for (int a = 0; a < dp1; ++a) v += X[a * dp1 + k] * V[a * dp1 + l];
+    phi.p[f][(int64_t)(c0 + c) * n + k * dp1 + l] += v;
   }
 }
 
 // L2P onto the leaf particle grid plus the direct (type 0/1) particle field:
-// child_panel_copy (fmm_funcs_impl.hpp:182-187) for the reference layout,
-// child_panel_interp_3_ll (:341-362) otherwise.
+// child_panel_interp_3_ll (fmm_funcs_impl.hpp:341-362) on the leaf grid:
+// sols[k][l] = sum_a T[a][k] sum_b T[b][l] phi[a][b] (+ part).
 __global__ __launch_bounds__(128) void k_l2p(DevTree t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 sols) {
-  __shared__ double sp[3 * kMaxN];
-  __shared__ double sTt[kMaxN];
+  __shared__ double sp[3 * kMaxN], sv[3 * kMaxN];
+  __shared__ double sT[kMaxN];
   const int leaf = blockIdx.x;
   const int n = t.n, dp1 = t.dp1, m = t.m, m2 = t.m2;
   for (int f = 0; f < ncomp; ++f)
     for (int i = threadIdx.x; i < n; i += blockDim.x) sp[f * n + i] = phi.p[f][(int64_t)(t.offset + leaf) * n + i];
-  for (int i = threadIdx.x; i < dp1 * m; i += blockDim.x) sTt[i] = t.p2mT[i];
+  for (int i = threadIdx.x; i < dp1 * m; i += blockDim.x) sT[i] = t.p2m[i];
   __syncthreads();
-  for (int o = threadIdx.x; o < m2; o += blockDim.x) {
-    for (int f = 0; f < ncomp; ++f) {
-      double v = contract(sTt, sTt, sp + f * n, dp1, o / m, o % m);
-      int64_t u = (int64_t)leaf * m2 + o;
-      if (part.p[f]) v += part.p[f][u];
-      sols.p[f][u] = v;
-    }
+  const int vn = dp1 * m;
+  for (int o = threadIdx.x; o < ncomp * vn; o += blockDim.x) {  // V[a][l] = sum_b T[b][l] phi[a][b]
+    const int f = o / vn, a = (o % vn) / m, l = o % m;
+    const double* P = sp + f * n;
+    double v = 0;
+    for (int b = 0; b < dp1; ++b) v += sT[b * m + l] * P[a * dp1 + b];
+    sv[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < ncomp * m2; o += blockDim.x) {  // sol[k][l] = sum_a T[a][k] V[a][l]
+    const int f = o / m2, k = (o % m2) / m, l = o % m;
+    const double* V = sv + f * vn;
+    double v = 0;
+    for (int a = 0; a < dp1; ++a) v += sT[a * m + k] * V[a * m + l];
+    const int64_t u = (int64_t)leaf * m2 + (o % m2);
+    if (part.p[f]) v += part.p[f][u];
+    sols.p[f][u] = v;
   }
 }
 

@@ -79,8 +79,22 @@ __device__ inline bool bli_w(double t, double lo, double hi, const double* __res
   return ok;
 }
 
-// Interpolate NF leaf fields at label point (xi, eta) of leaf `leaf`
-// (bve_time_step.cpp:222-237): sum_ac (wx_a * we_c) f[a*M + c].
+// sum_a wx[a] sum_c we[c] f[a*M + c]: the reference's tensor basis
+// (bve_time_step.cpp:222-237 sums f * (wx_a * we_c)) applied one direction at a time.
+template <int M>
+__device__ inline double tensor_gather(const double* __restrict__ f, const double* wx, const double* we) {
+  double out = 0;
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) s = fma(f[a * M + c], we[c], s);
+    out = fma(wx[a], s, out);
+  }
+  return out;
+}
+
+// Interpolate NF leaf fields [Pl][M*M] at label point (xi, eta) of leaf `leaf`.
 template <int M, int NF>
 __device__ inline bool interp_leaf(const DevTree& t, int leaf, double xi, double eta, const double* const* f,
                                    double* out) {
@@ -90,16 +104,7 @@ __device__ inline bool interp_leaf(const DevTree& t, int leaf, double xi, double
   ok = bli_w<M>(eta, P.min_eta, P.max_eta, t.cheb_m, we) && ok;
   const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
 #pragma unroll
-  for (int k = 0; k < NF; ++k) out[k] = 0;
-#pragma unroll
-  for (int a = 0; a < M; ++a) {
-#pragma unroll
-    for (int c = 0; c < M; ++c) {
-      const double w = wx[a] * we[c];
-#pragma unroll
-      for (int k = 0; k < NF; ++k) out[k] += f[k][base + a * M + c] * w;
-    }
-  }
+  for (int k = 0; k < NF; ++k) out[k] = tensor_gather<M>(f[k] + base, wx, we);
   return ok;
 }
 
@@ -122,7 +127,7 @@ __global__ void k_disp_update(DevTree t, const double* __restrict__ x, const dou
 // proxy node interpolated from the leaf that contains it (leaf and label
 // coordinates precomputed on the host, the tree being fixed in label space).
 template <int M>
-__global__ void k_disp_interp(DevTree t, CPtr3 dpart, Ptr3 dprox) {
+__global__ __launch_bounds__(256) void k_disp_interp(DevTree t, CPtr3 dpart, Ptr3 dprox) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= (int64_t)t.offset * t.n) return;
   const int leaf = t.proxy_leaf[v];
@@ -151,7 +156,7 @@ __device__ inline void vel_at(const DevTree& t, double x, double y, double z, co
 
 // bve_find_departure_points, bve_time_step.cpp:206-308.
 template <int M>
-__global__ void k_departure(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
+__global__ __launch_bounds__(128) void k_departure(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
                             const double* __restrict__ z, Vel12 v, double dt, int combine, double* __restrict__ dx,
                             double* __restrict__ dy, double* __restrict__ dz) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,7 +193,7 @@ __global__ void k_departure(DevTree t, const double* __restrict__ x, const doubl
 
 // bve_departure_to_target, bve_time_step.cpp:38-66.  Tracers [Pl][m2][T].
 template <int M>
-__global__ void k_dep_to_target(DevTree t, const double* __restrict__ z, const double* __restrict__ dx,
+__global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* __restrict__ z, const double* __restrict__ dx,
                                 const double* __restrict__ dy, const double* __restrict__ dz,
                                 const double* __restrict__ vor, const double* __restrict__ tracers, int ntr,
                                 double omega, double* __restrict__ new_vor, double* __restrict__ new_tracers) {
@@ -209,19 +214,18 @@ __global__ void k_dep_to_target(DevTree t, const double* __restrict__ z, const d
   ok = bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we) && ok;
   if (!ok) flag(t, kErrBliBounds);
   const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
-  double v = 0;
-#pragma unroll
-  for (int a = 0; a < M; ++a)
-#pragma unroll
-    for (int c = 0; c < M; ++c) v += vor[base + a * M + c] * (wx[a] * we[c]);
+  double v = tensor_gather<M>(vor + base, wx, we);
   v += 2 * omega * Z;
   new_vor[u] = v - 2 * omega * z[u];
   for (int k = 0; k < ntr; ++k) {
     double s = 0;
 #pragma unroll
-    for (int a = 0; a < M; ++a)
+    for (int a = 0; a < M; ++a) {
+      double r = 0;
 #pragma unroll
-      for (int c = 0; c < M; ++c) s += tracers[(base + a * M + c) * ntr + k] * (wx[a] * we[c]);
+      for (int c = 0; c < M; ++c) r = fma(tracers[(base + a * M + c) * ntr + k], we[c], r);
+      s = fma(wx[a], r, s);
+    }
     new_tracers[u * ntr + k] = s;
   }
 }
