@@ -1,0 +1,111 @@
+// sbslpm_reference_api.hpp -- the reference's C++ call signatures restated over
+// the C ABI of sbslpm.h, for a drop-in replacement of the Kokkos calls in
+// bve_time_step.cpp / poisson_eq_fmm.cpp / poisson_eq_ds.cpp.
+//
+// `View2<T>` stands in for Kokkos::View<T**, LayoutRight> (device memory):
+// pointer + two extents.  Function names, argument order and the
+// accumulate/assign conventions are the reference's; the interpolation table,
+// panels and interaction list live in the context instead of being passed.
+// These wrappers cover the reference layout (particles are the leaf proxy
+// nodes, leaf_degree == interp_degree), where the particle arrays are the leaf
+// rows of the [P][n] proxy arrays, exactly as in the reference.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "sbslpm.h"
+
+namespace sbslpm_ref {
+
+template <class T>
+struct View2 {
+  T* ptr = nullptr;
+  int64_t e0 = 0, e1 = 0;
+  T* data() const { return ptr; }
+  int extent_int(int i) const { return (int)(i == 0 ? e0 : e1); }
+  T* row(int64_t r) const { return ptr + r * e1; }
+};
+
+// Raises like the reference's std::runtime_error paths (Kokkos::abort sites
+// surface from sbslpm_synchronize as SBSLPM_E_GEOMETRY).
+inline void check(sbslpm_ctx* ctx, int rc, const char* what) {
+  if (rc != SBSLPM_OK) throw std::runtime_error(std::string(what) + ": " + sbslpm_last_error(ctx));
+}
+
+struct Layout {
+  int64_t P, Pl, n, offset;
+};
+
+inline Layout layout(sbslpm_ctx* ctx) {
+  int64_t s[8];
+  check(ctx, sbslpm_get_sizes(ctx, s), "get_sizes");
+  if (s[2] != s[3]) throw std::runtime_error("reference-layout wrappers need leaf_degree == interp_degree");
+  return Layout{s[0], s[1], s[2], s[6]};
+}
+
+// fmm_funcs.hpp:25-27
+inline void upward_pass(sbslpm_ctx* ctx, View2<double>& area, View2<double>& pots,
+                        View2<double>& proxy_source_pots) {
+  Layout L = layout(ctx);
+  check(ctx, sbslpm_upward_pass(ctx, area.data(), pots.data(), proxy_source_pots.row(L.offset),
+                                proxy_source_pots.data()),
+        "upward_pass");
+}
+
+// fmm_interactions/bve_vel.hpp:10-14
+inline void bve_vel_interactions(sbslpm_ctx* ctx, View2<double>& disp_x, View2<double>& disp_y,
+                                 View2<double>& disp_z, View2<double>& target_pots_1, View2<double>& target_pots_2,
+                                 View2<double>& target_pots_3, View2<double>& source_vals) {
+  Layout L = layout(ctx);
+  check(ctx,
+        sbslpm_bve_vel_interactions(ctx, disp_x.row(L.offset), disp_y.row(L.offset), disp_z.row(L.offset),
+                                    disp_x.data(), disp_y.data(), disp_z.data(), source_vals.row(L.offset),
+                                    source_vals.data(), target_pots_1.row(L.offset), target_pots_2.row(L.offset),
+                                    target_pots_3.row(L.offset), target_pots_1.data(), target_pots_2.data(),
+                                    target_pots_3.data()),
+        "bve_vel_interactions");
+}
+
+// fmm_interactions/inverse_poisson.hpp:10-12
+inline void poisson_fmm_interactions(sbslpm_ctx* ctx, View2<double>& target_pots, View2<double>& source_vals) {
+  Layout L = layout(ctx);
+  check(ctx,
+        sbslpm_poisson_fmm_interactions(ctx, source_vals.row(L.offset), source_vals.data(),
+                                        target_pots.row(L.offset), target_pots.data()),
+        "poisson_fmm_interactions");
+}
+
+// fmm_funcs.hpp:29-31 (the leaf rows already hold the leaf-target sums)
+inline void downward_pass(sbslpm_ctx* ctx, View2<double>& proxy_target_pots, View2<double>& sols) {
+  check(ctx, sbslpm_downward_pass(ctx, proxy_target_pots.data(), nullptr, sols.data()), "downward_pass");
+}
+
+// fmm_funcs.hpp:33-36
+inline void downward_pass_3(sbslpm_ctx* ctx, View2<double>& p1, View2<double>& p2, View2<double>& p3,
+                            View2<double>& s1, View2<double>& s2, View2<double>& s3) {
+  check(ctx,
+        sbslpm_downward_pass_3(ctx, p1.data(), p2.data(), p3.data(), nullptr, nullptr, nullptr, s1.data(), s2.data(),
+                               s3.data()),
+        "downward_pass_3");
+}
+
+// direct_sum.hpp:8-11, rows [lb, ub) (RunConfig::oned_lb / oned_ub)
+inline void direct_sum_inv_lap(sbslpm_ctx* ctx, int64_t lb, int64_t ub, int64_t n, double* xcos, double* ycos,
+                               double* zcos, double* area, double* pots, double* soln) {
+  check(ctx, sbslpm_direct_sum_inv_lap(ctx, n, xcos, ycos, zcos, area, pots, soln, lb, ub), "direct_sum_inv_lap");
+}
+
+// bve_time_step.hpp:7-13 (passive_tracers: [Pl][n][tracer_count] device array)
+inline void bve_back_rk4_step(sbslpm_ctx* ctx, View2<double>& xcos, View2<double>& ycos, View2<double>& zcos,
+                              View2<double>& area, View2<double>& vors, View2<double>& proxy_source_vors,
+                              View2<double>& vel_x, View2<double>& vel_y, View2<double>& vel_z,
+                              double* passive_tracers, double time) {
+  check(ctx,
+        sbslpm_bve_back_rk4_step(ctx, xcos.data(), ycos.data(), zcos.data(), area.data(), vors.data(),
+                                 proxy_source_vors.data(), vel_x.data(), vel_y.data(), vel_z.data(), passive_tracers,
+                                 time),
+        "bve_back_rk4_step");
+}
+
+}  // namespace sbslpm_ref

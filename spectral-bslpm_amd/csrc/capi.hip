@@ -26,6 +26,8 @@ struct sbslpm_ctx {
 
   char* arena = nullptr;
   TargetTile* tiles = nullptr;
+  int* zero_rows = nullptr;
+  int nzero_rows = 0;
   SourceSet* sets = nullptr;
   int ntiles = 0;
 
@@ -176,7 +178,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
          o_loT = A.take<double>(t.n);
   size_t o_p2m = A.take<double>(t.dp1 * t.m), o_p2mT = A.take<double>(t.dp1 * t.m);
   size_t o_pl = A.take<int>(t.proxy_leaf.size()), o_pxe = A.take<double>(t.proxy_xieta.size());
-  size_t o_err = A.take<int>(4);
+  size_t o_err = A.take<int>(4), o_zr = A.take<int>(t.zero_rows.size());
   size_t o_tiles = A.take<TargetTile>(t.tiles.size()), o_sets = A.take<SourceSet>(t.srcsets.size());
   size_t o_rec64 = A.take<SrcRec64>(U), o_rec32 = A.take<SrcRec32>(U), o_recmf = A.take<SrcRecMfma>(t.U16);
   size_t o_qpart = A.take<double>(NP), o_qproxy = A.take<double>(Pn);
@@ -207,6 +209,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   rc |= upload(c, b + o_pxe, t.proxy_xieta.data(), t.proxy_xieta.size() * sizeof(double));
   rc |= upload(c, b + o_tiles, t.tiles.data(), t.tiles.size() * sizeof(TargetTile));
   rc |= upload(c, b + o_sets, t.srcsets.data(), t.srcsets.size() * sizeof(SourceSet));
+  rc |= upload(c, b + o_zr, t.zero_rows.data(), t.zero_rows.size() * sizeof(int));
   if (rc != SBSLPM_OK || hipMemset(b + o_err, 0, 4 * sizeof(int)) != hipSuccess) {
     hipFree(c->arena);
     delete c;
@@ -227,6 +230,8 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   c->tiles = (TargetTile*)(b + o_tiles);
   c->sets = (SourceSet*)(b + o_sets);
   c->ntiles = (int)t.tiles.size();
+  c->zero_rows = (int*)(b + o_zr);
+  c->nzero_rows = (int)t.zero_rows.size();
   c->rec64 = (SrcRec64*)(b + o_rec64);
   c->rec32 = (SrcRec32*)(b + o_rec32);
   c->recmf = (SrcRecMfma*)(b + o_recmf);
@@ -466,29 +471,30 @@ int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double*
              const double* const* dprox, double* const* vel) {
   const Tree& t = c->tree;
   const int64_t Pn = (int64_t)t.P * t.n;
+  (void)Pn;
+  // Every leaf particle and every proxy of a target panel is written (assigned)
+  // by exactly one pair-kernel lane, the leaf proxy rows by the last L2L level;
+  // only the remaining proxy rows need clearing.
   launch_upward_pass(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
-  for (int f = 0; f < 3; ++f) {
-    launch_fill_zero(c->phipart[f], t.npart, c->stream);
-    launch_fill_zero(c->phiproxy[f], Pn, c->stream);
-  }
+  launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 3, c->phiproxy, c->stream);
   if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records_mfma(c->dt, c->qpart, c->qproxy, dpart, dprox, c->recmf, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, c->recmf, c->prm.kernel_eps, c->phipart,
-                             c->phiproxy, c->stream);
+                             c->phiproxy, c->stream, true);
   } else if (c->prm.precision == SBSLPM_MIXED_VALU) {
     launch_source_records32(c->dt, c->qpart, c->qproxy, dpart, dprox, c->rec32, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, c->phipart,
-                              c->phiproxy, c->stream);
+                              c->phiproxy, c->stream, true);
   } else {
     launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->rec64, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_FP64);
     launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, c->prm.kernel_eps, c->phipart,
-                          c->phiproxy, c->stream);
+                          c->phiproxy, c->stream, true);
   }
   const double* pp[3] = {c->phipart[0], c->phipart[1], c->phipart[2]};
-  launch_downward_pass(c->dt, 3, c->phiproxy, pp, vel, c->stream);
+  launch_downward_pass(c->dt, 3, c->phiproxy, pp, vel, c->stream, true);
   return launched(c);
 }
 

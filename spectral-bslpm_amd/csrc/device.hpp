@@ -50,11 +50,15 @@ struct SrcRec32 {
   float x, y, z, q;
 };
 
-// Source record of the MFMA pair kernel, 16-padded per source set:
-// b = position relative to the source-panel centre, beta = |b|^2/2 and the
-// charge-weighted (q b, q) with q = -charge/(4 pi); padding slots are zero.
+// Source records of the MFMA pair kernel, in tiles of 16 sources (a source
+// set is padded to whole tiles with zero records).  A tile is 128 floats:
+//   [k*16 + i]                    k = 0..3: (b_x, b_y, b_z, beta) of source i,
+//   [64 + ((i/4)*4 + c)*4 + i%4]  c = 0..3: (q b_x, q b_y, q b_z, q) of source i,
+// with b the position relative to the source-panel centre, beta = |b|^2/2 and
+// q = -charge/(4 pi).  Lane l of a wave then reads its 16x16x4 A operand at
+// [l] and its four 4x4x1 A operands as one float4 at 64 + 4*((l/16)*4 + l%4).
 struct SrcRecMfma {
-  float bx, by, bz, beta, qbx, qby, qbz, q;
+  float v[8];
 };
 
 struct LaunchCfg {
@@ -64,33 +68,41 @@ struct LaunchCfg {
 // ---- passes (fmm_passes.hip)
 void launch_upward_pass(const DevTree& t, const double* area, const double* pots, double* q_part,
                         double* q_proxy, hipStream_t s);
+// assign_leaf_rows: the last L2L level overwrites the leaf proxy rows instead of
+// adding to them (valid when nothing else writes those rows).
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
-                          double* const* sols, hipStream_t s);
+                          double* const* sols, hipStream_t s, bool assign_leaf_rows = false);
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
+// Zeroes rows[0..nrows) (panel ids) of ncomp [P][n] arrays.
+void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s);
 
 // ---- interactions (fmm_interact.hip)
 void launch_source_records64(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, bool project,
                              SrcRec64* rec, hipStream_t s);
+// The pair kernels accumulate into their outputs like the reference functors;
+// with assign = true they overwrite them (each target written by one lane).
 void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s);
+                           hipStream_t s, bool assign = false);
 void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                               const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s);
+                               const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
+                               bool assign = false);
 void launch_source_records32(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, SrcRec32* rec,
                              hipStream_t s);
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec32* rec, double eps, double* const* out_part,
-                               double* const* out_proxy, hipStream_t s);
+                               double* const* out_proxy, hipStream_t s, bool assign = false);
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                                   const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s);
+                                   const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s,
+                                   bool assign = false);
 void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
                                 const double* const* disp_part, const double* const* disp_proxy, SrcRecMfma* rec,
                                 hipStream_t s);
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
-                              hipStream_t s);
+                              hipStream_t s, bool assign = false);
 
 // ---- displacement, departure points, remap (remap.hip)
 void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,

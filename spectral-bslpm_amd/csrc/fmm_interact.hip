@@ -47,13 +47,28 @@ __device__ inline double rcp64(double d) {
   return fma(r, e, r);
 }
 
-__device__ inline void store_u(const DevTree& t, int64_t u, Ptr3 part, Ptr3 proxy, double a, double b, double c) {
+// Target-major ownership: every target point is written by exactly one lane,
+// so `assign` may replace the accumulate of the reference functors.
+__device__ inline void store_u(const DevTree& t, int64_t u, Ptr3 part, Ptr3 proxy, double a, double b, double c,
+                               int assign) {
+  double* o[3];
+  int64_t v = u;
   if (u < t.npart) {
-    part.p[0][u] += a; part.p[1][u] += b; part.p[2][u] += c;
+    o[0] = part.p[0]; o[1] = part.p[1]; o[2] = part.p[2];
   } else {
-    int64_t v = u - t.npart;
-    proxy.p[0][v] += a; proxy.p[1][v] += b; proxy.p[2][v] += c;
+    v = u - t.npart;
+    o[0] = proxy.p[0]; o[1] = proxy.p[1]; o[2] = proxy.p[2];
   }
+  if (assign) {
+    o[0][v] = a; o[1][v] = b; o[2][v] = c;
+  } else {
+    o[0][v] += a; o[1][v] += b; o[2][v] += c;
+  }
+}
+
+__device__ inline void store_s(const DevTree& t, int64_t u, double* part, double* proxy, double a, int assign) {
+  double* o = u < t.npart ? part + u : proxy + (u - t.npart);
+  *o = assign ? a : *o + a;
 }
 
 // Source records, FP64: y = P(node + disp) (bve_vel_impl.hpp:90-97), q' = -q/(4 pi).
@@ -104,7 +119,7 @@ __global__ void k_src32(DevTree t, const double* __restrict__ q_part, const doub
 __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
                                                const SrcRec64* __restrict__ rec, double eps, Ptr3 part,
-                                               Ptr3 proxy) {
+                                               Ptr3 proxy, int assign) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -126,7 +141,7 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
       w2 = fma(g, s.z, w2);
     }
   }
-  if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0);
+  if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0, assign);
 }
 
 // FP64 log pair kernel: psi += -1/(4 pi) log(1 - x.y) q, pairs with
@@ -134,7 +149,7 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
 __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
                                                const SrcRec64* __restrict__ rec, double* __restrict__ part,
-                                               double* __restrict__ proxy) {
+                                               double* __restrict__ proxy, int assign) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -154,10 +169,7 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
       acc += (dp < 1.0 - 1e-15) ? l * s.q : 0.0;
     }
   }
-  if (lane < T.count) {
-    if (u < t.npart) part[u] += acc;
-    else proxy[u - t.npart] += acc;
-  }
+  if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
 
 // Mixed precision Biot-Savart: FP32 pair arithmetic in panel-centred
@@ -166,7 +178,7 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
 __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ sets,
                                                    const SrcRec32* __restrict__ rec, float eps, Ptr3 part,
-                                                   Ptr3 proxy) {
+                                                   Ptr3 proxy, int assign) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -196,14 +208,14 @@ __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* 
     }
     w0 += v0; w1 += v1; w2 += v2;
   }
-  if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0);
+  if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0, assign);
 }
 
 // Mixed precision log kernel: log(|x - y|^2 / 2) in FP32, same guard.
 __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ sets,
                                                    const SrcRec32* __restrict__ rec, double* __restrict__ part,
-                                                   double* __restrict__ proxy) {
+                                                   double* __restrict__ proxy, int assign) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -229,10 +241,7 @@ __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* 
     }
     acc += v;
   }
-  if (lane < T.count) {
-    if (u < t.npart) part[u] += acc;
-    else proxy[u - t.npart] += acc;
-  }
+  if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -259,26 +268,31 @@ __global__ void k_src_mfma(DevTree t, const double* __restrict__ q_part, const d
     valid = k < t.n;
     u = t.npart + (int64_t)panel * t.n + k;
   }
-  SrcRecMfma r{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (valid) {
     double x = t.px[u], y = t.py[u], z = t.pz[u], q;
     if (u < t.npart) {
       if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
       q = q_part[u];
     } else {
-      const int64_t v = u - t.npart;
-      if (dprox.p[0]) { x += dprox.p[0][v]; y += dprox.p[1][v]; z += dprox.p[2][v]; }
-      q = q_proxy[v];
+      const int64_t w = u - t.npart;
+      if (dprox.p[0]) { x += dprox.p[0][w]; y += dprox.p[1][w]; z += dprox.p[2][w]; }
+      q = q_proxy[w];
     }
     project_to_sphere(x, y, z);
     const DevPanel& P = t.panels[panel];
     const float bx = (float)(x - P.cx), by = (float)(y - P.cy), bz = (float)(z - P.cz);
     const double qq = kNegInv4Pi * q;
-    r.bx = bx; r.by = by; r.bz = bz;
-    r.beta = (float)(0.5 * ((double)bx * bx + (double)by * by + (double)bz * bz));
-    r.qbx = (float)(qq * bx); r.qby = (float)(qq * by); r.qbz = (float)(qq * bz); r.q = (float)qq;
+    v[0] = bx; v[1] = by; v[2] = bz;
+    v[3] = (float)(0.5 * ((double)bx * bx + (double)by * by + (double)bz * bz));
+    v[4] = (float)(qq * bx); v[5] = (float)(qq * by); v[6] = (float)(qq * bz); v[7] = (float)qq;
   }
-  rec[s] = r;
+  float* tile = reinterpret_cast<float*>(rec) + (s >> 4) * 128;
+  const int i = (int)(s & 15);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tile[k * 16 + i] = v[k];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) tile[64 + ((i >> 2) * 4 + c) * 4 + (i & 3)] = v[4 + c];
 }
 
 // Mixed-precision Biot-Savart pair kernel on the matrix cores.
@@ -314,8 +328,7 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
     ay[tt] = (float)(t.py[u] - PT.cy);
     az[tt] = (float)(t.pz[u] - PT.cz);
   }
-  const int off_a = col * 8 + grp;                // A operand: component grp of source col
-  const int off_b = (4 * grp) * 8 + 4 + (lane & 3);  // B2 operand q: (q b, q)[lane%4] of source 4 grp + q
+  const int off_b = 64 + 4 * (grp * 4 + (lane & 3));  // float4: (q b, q)[lane % 4] of sources 4 grp + 0..3
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
     const DevPanel& PS = t.panels[S.panel];
@@ -333,16 +346,12 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
     }
     const float* __restrict__ r = rec + (int64_t)S.first16 * 8;
     const int nst = (S.count + 15) >> 4;
-    float a1 = r[off_a];
-    float bq[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) bq[q] = r[off_b + 8 * q];
+    float a1 = r[lane];
+    v4f bq = *reinterpret_cast<const v4f*>(r + off_b);
     for (int st = 0; st < nst; ++st) {
       const float* __restrict__ rn = r + 128 * min(st + 1, nst - 1);
-      const float na1 = rn[off_a];
-      float nbq[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) nbq[q] = rn[off_b + 8 * q];
+      const float na1 = rn[lane];
+      const v4f nbq = *reinterpret_cast<const v4f*>(rn + off_b);
       v4f d[NTT];
 #pragma unroll
       for (int tt = 0; tt < NTT; ++tt)
@@ -357,8 +366,7 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
         for (int tt = 0; tt < NTT; ++tt)
           acc[tt] = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q], d[tt][q], acc[tt], 0, 0, 0);
       a1 = na1;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) bq[q] = nbq[q];
+      bq = nbq;
     }
 #pragma unroll
     for (int tt = 0; tt < NTT; ++tt) {
@@ -372,7 +380,7 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
 __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                   const SourceSet* __restrict__ sets,
                                                   const float* __restrict__ rec, float eps, Ptr3 part,
-                                                  Ptr3 proxy) {
+                                                  Ptr3 proxy, int assign) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
@@ -405,7 +413,7 @@ __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* _
     const double w2 = grp == 0 ? w[0][2] : grp == 1 ? w[1][2] : grp == 2 ? w[2][2] : w[3][2];
     const int64_t u = (int64_t)T.first + j;
     const double x = t.px[u], y = t.py[u], z = t.pz[u];
-    store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0);
+    store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0, assign);
   }
 }
 
@@ -441,25 +449,26 @@ void launch_source_records32(const DevTree& t, const double* q_part, const doubl
 
 void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s) {
+                           hipStream_t s, bool assign) {
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy));
+                     ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0);
 }
 
 void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                               const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s) {
+                               const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
+                               bool assign) {
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_log64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, out_part, out_proxy);
+                     ntiles, sets, rec, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec32* rec, double eps, double* const* out_part,
-                               double* const* out_proxy, hipStream_t s) {
+                               double* const* out_proxy, hipStream_t s, bool assign) {
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
-                     tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy));
+                     tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0);
 }
 
 void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
@@ -471,17 +480,19 @@ void launch_source_records_mfma(const DevTree& t, const double* q_part, const do
 
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
-                              hipStream_t s) {
+                              hipStream_t s, bool assign) {
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
-                     tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy));
+                     tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy),
+                     assign ? 1 : 0);
 }
 
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                                   const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s) {
+                                   const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s,
+                                   bool assign) {
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_log_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
-                     tiles, ntiles, sets, rec, out_part, out_proxy);
+                     tiles, ntiles, sets, rec, out_part, out_proxy, assign ? 1 : 0);
 }
 
 }  // namespace sbslpm

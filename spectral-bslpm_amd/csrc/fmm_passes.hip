@@ -95,7 +95,7 @@ __global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, double* __re
 // phi_c[k][l] += sum_a X_c[a][k] sum_b Y_c[b][l] phi_p[a][b].  The eta sweep
 // depends only on the eta half, so it is done once for the upper and once
 // for the lower half.
-__global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, Ptr3 phi) {
+__global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, Ptr3 phi, int assign) {
   __shared__ double sp[3 * kMaxN], sv[3 * 2 * kMaxN];
   __shared__ double shi[kMaxN], slo[kMaxN];
   const int par = pbegin + blockIdx.x;
@@ -120,7 +120,8 @@ __global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, P
     const double* V = sv + (f * 2 + (c <= 1 ? 0 : 1)) * n;
     double v = 0;
     for (int a = 0; a < dp1; ++a) v += X[a * dp1 + k] * V[a * dp1 + l];
-    phi.p[f][(int64_t)(c0 + c) * n + k * dp1 + l] += v;
+    double* dst = phi.p[f] + (int64_t)(c0 + c) * n + k * dp1 + l;
+    *dst = assign ? v : *dst + v;
   }
 }
 
@@ -167,6 +168,13 @@ __global__ void k_l2p_copy(int64_t npart, int ncomp, CPtr3 leafrows, CPtr3 part,
   }
 }
 
+__global__ void k_zero_rows(const int* __restrict__ rows, int nrows, int n, int ncomp, Ptr3 a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nrows * n) return;
+  const int64_t o = (int64_t)rows[i / n] * n + i % n;
+  for (int f = 0; f < ncomp; ++f) a.p[f][o] = 0.0;
+}
+
 __global__ void k_fill_zero(double* p, int64_t count) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u < count) p[u] = 0.0;
@@ -185,6 +193,14 @@ bool is_identity_p2m(const DevTree& t) { return t.m == t.dp1; }
 
 void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
   if (count > 0) hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
+}
+
+void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s) {
+  if (nrows <= 0) return;
+  Ptr3 a{};
+  for (int f = 0; f < ncomp; ++f) a.p[f] = arrays[f];
+  const int64_t cnt = (int64_t)nrows * n;
+  hipLaunchKernelGGL(k_zero_rows, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, rows, nrows, n, ncomp, a);
 }
 
 void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
@@ -211,7 +227,7 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
 // fmm_funcs.cpp:357-392.  phi_proxy rows are accumulated top-down; sols are
 // assigned = L2P(leaf proxies) + phi_part (phi_part entries may be null).
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
-                          double* const* sols, hipStream_t s) {
+                          double* const* sols, hipStream_t s, bool assign_leaf_rows) {
   Ptr3 phi{}; CPtr3 part{}, leafrows{}; Ptr3 out{};
   for (int f = 0; f < ncomp; ++f) {
     phi.p[f] = phi_proxy[f];
@@ -222,7 +238,8 @@ void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy,
   for (int l = 0; l + 1 < t.levels; ++l) {
     int pbegin = 2 * ((1 << (2 * l)) - 1);
     int count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_l2l, dim3(count), dim3(256), 0, s, t, pbegin, ncomp, phi);
+    const int assign = (assign_leaf_rows && l + 2 == t.levels) ? 1 : 0;
+    hipLaunchKernelGGL(k_l2l, dim3(count), dim3(256), 0, s, t, pbegin, ncomp, phi, assign);
   }
   CPtr3 cphi{};
   for (int f = 0; f < ncomp; ++f) cphi.p[f] = phi_proxy[f];

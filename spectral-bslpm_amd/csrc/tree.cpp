@@ -306,6 +306,12 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
     }
   }
 
+  // Proxy rows that are not interaction targets: non-leaf panels without
+  // tiles, and the leaf rows when there is no L2L level to write them.
+  t.zero_rows.clear();
+  for (int p = 0; p < t.P; ++p)
+    if ((!t.pn[p].is_leaf && cnt[p + 1] == cnt[p]) || (t.pn[p].is_leaf && t.cfg.levels == 1)) t.zero_rows.push_back(p);
+
   // Fixed 1-D transfer matrices.  Children are exact quadrants, so the parent
   // basis at a child node (fmm_funcs_impl.hpp:61-64, 110-113) depends only on
   // whether the child is the upper or lower half in each direction.

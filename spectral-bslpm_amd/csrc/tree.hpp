@@ -72,6 +72,7 @@ struct Tree {
   std::vector<double> px, py, pz;       // unified point coordinates [U] (targets, undisplaced sources)
   std::vector<double> m2m_hi, m2m_lo;   // [d+1][d+1]: parent basis a at child node k, A[a*(d+1)+k]
   std::vector<double> p2m;              // [d+1][m]: proxy basis a at particle node k
+  std::vector<int> zero_rows;           // proxy rows no pair kernel or last-level L2L writes
   std::vector<int> proxy_leaf;          // [offset*n] leaf containing each non-leaf proxy node
   std::vector<double> proxy_xieta;      // [offset*n][2] label coordinates of those nodes (radians)
   std::vector<double> spec_x, spec_y, spec_z, spec_area;  // caller-facing particles [Pl][m2]
