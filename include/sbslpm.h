@@ -149,6 +149,20 @@ int sbslpm_bve_velocity(sbslpm_ctx* ctx, const double* area, const double* vor, 
 /* One sphere Poisson solve (poisson_eq_fmm.cpp:129-147): psi [Pl][m2] assigned. */
 int sbslpm_poisson_solve(sbslpm_ctx* ctx, const double* area, const double* f, double* psi);
 
+/* FMM evaluation at nt arbitrary unit-sphere targets t{x,y,z} [nt] (device arrays), in FP64.
+ * Each target takes the far field of the leaf that contains it, interpolated by BLI from the
+ * leaf's L2L-swept proxy potentials (child_panel_interp_3_ll, fmm_funcs_impl.hpp:318-364), plus
+ * the near field over that leaf's own interactions (the particle-target types 0/1 of
+ * swe_vel_impl.hpp:188-255).  Point location as in interp_funcs_impl.hpp:114-142.
+ * psi [nt] assigned: Poisson solve (log kernel, poisson_eq_fmm.cpp:129-147) at the targets. */
+int sbslpm_poisson_solve_targets(sbslpm_ctx* ctx, const double* area, const double* f, int64_t nt,
+                                 const double* tx, const double* ty, const double* tz, double* psi);
+/* The same for the Biot-Savart velocity of vorticity vor (undisplaced sources, the context's
+ * eps): vel_{x,y,z} [nt] assigned. */
+int sbslpm_bve_velocity_targets(sbslpm_ctx* ctx, const double* area, const double* vor, int64_t nt,
+                                const double* tx, const double* ty, const double* tz, double* vel_x,
+                                double* vel_y, double* vel_z);
+
 /* direct_sum_inv_lap (direct_sum.cpp:8-15): soln[i] += sum_{j != i} -1/(4 pi) log(1 - x_i.x_j)
  * area_j pots_j for i in [lb, ub) over n points. */
 int sbslpm_direct_sum_inv_lap(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,

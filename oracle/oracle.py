@@ -79,6 +79,9 @@ def lib():
         L.orc_bve_back_rk4_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ppd]
         L.orc_bve_back_rk4_step.restype = ctypes.c_int
+        L.orc_eval_targets.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int64,
+                                       _dp, _dp, _dp, _ppd]
+        L.orc_eval_targets.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -281,6 +284,17 @@ class OracleFMM:
         qp, qx = self.upward_pass(area, f)
         pp, px = self.poisson_fmm_interactions(qp, qx)
         return self.downward_pass([px], [pp])[0]
+
+    def eval_targets(self, kind, area, f, tx, ty, tz, eps=0.0):
+        """FMM value at target points (kind "poisson": psi, "bve": velocity
+        components); returns a list of 1 or 3 arrays."""
+        k = {"poisson": 0, "bve": 1}[kind]
+        tx, ty, tz = [np.ascontiguousarray(v, dtype=np.float64) for v in (tx, ty, tz)]
+        out = [np.zeros(tx.size) for _ in range(1 if k == 0 else 3)]
+        err = lib().orc_eval_targets(self.h, k, _p(area), _p(f), eps, tx.size, _p(tx), _p(ty), _p(tz), _pp(out))
+        if err:
+            raise RuntimeError("oracle eval_targets: geometry error")
+        return out
 
     def bve_displacement(self, x, y, z, vel, vor, tau, omega):
         nv = np.zeros((self.Pl, self.m2))

@@ -561,13 +561,9 @@ int orc_upward_pass(void* h, const double* area, const double* pots, double* q_p
   return err ? ORC_E_GEOMETRY : ORC_OK;
 }
 
-// fmm_funcs.cpp:374-392 (downward_pass_3) with parent_to_child_3
-// (fmm_funcs_impl.hpp:138-171) and child_panel_copy (:182-187).  ncomp fields
-// phi_proxy[ncomp][P][n] are accumulated top-down; sols[ncomp][Pl][m2] are
-// assigned = L2P(leaf proxies) + phi_part (phi_part may be null).
-int orc_downward_pass(void* h, int ncomp, double* const* phi_proxy, const double* const* phi_part,
-                      double* const* sols) {
-  Ctx& c = *static_cast<Ctx*>(h);
+// parent_to_child_3 (fmm_funcs_impl.hpp:138-171) over every level
+// (fmm_funcs.cpp:378-389): ncomp fields phi_proxy[ncomp][P][n] accumulated top-down.
+static int l2l_sweep(Ctx& c, int ncomp, double* const* phi_proxy) {
   int err = 0;
   for (int lev = 0; lev < c.levels - 1; lev++) {
     int pb = c.level_start[lev], pe = c.level_start[lev + 1];
@@ -592,6 +588,16 @@ int orc_downward_pass(void* h, int ncomp, double* const* phi_proxy, const double
       }
     }
   }
+  return err;
+}
+
+// fmm_funcs.cpp:374-392 (downward_pass_3): the L2L sweep, then child_panel_copy
+// (fmm_funcs_impl.hpp:182-187).  sols[ncomp][Pl][m2] are assigned
+// = L2P(leaf proxies) + phi_part (phi_part may be null).
+int orc_downward_pass(void* h, int ncomp, double* const* phi_proxy, const double* const* phi_part,
+                      double* const* sols) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = l2l_sweep(c, ncomp, phi_proxy);
 #pragma omp parallel for schedule(dynamic, 16) reduction(| : err)
   for (int i = 0; i < c.Pl; i++) {
     const Panel& p = c.pn[c.offset + i];
@@ -792,6 +798,80 @@ void orc_direct_sum_inv_lap_targets(int64_t NT, const double* tx, const double* 
     }
     soln[i] += acc;
   }
+}
+
+// FMM evaluation at arbitrary points: the `_ll` pass structure
+// (fmm_funcs.cpp:394-419) with targets that are not sources.  A target in
+// leaf L gets
+//   far:  the degree-d BLI of L's proxy potentials after the L2L sweep
+//         (child_panel_interp_3_ll, fmm_funcs_impl.hpp:318-364), and
+//   near: the kernel over the sources of L's own interactions, types 0 and 1
+//         (the particle-target point sets of swe_vel_impl.hpp:188-255).
+// kind 0: Poisson, -1/(4 pi) log(1 - x.y) with the dp < 1 - 1e-15 guard on
+// unprojected nodes (inverse_poisson_impl.hpp:47-48, 199-204); out[0][NT].
+// kind 1: Biot-Savart with undisplaced, re-projected sources
+// (bve_vel_impl.hpp:54-66); out[0..2][NT].
+int orc_eval_targets(void* h, int kind, const double* area, const double* f, double eps, int64_t NT,
+                     const double* tx, const double* ty, const double* tz, double* const* out) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const size_t NP = (size_t)c.Pl * c.m2, Pn = (size_t)c.P * c.n;
+  const int nc = kind == 0 ? 1 : 3;
+  std::vector<double> qp(NP), qx(Pn);
+  int err = orc_upward_pass(h, area, f, qp.data(), qx.data()) != ORC_OK;
+  std::vector<double> pp(3 * NP, 0.0), px(3 * Pn, 0.0);
+  double* ppp[3] = {pp.data(), pp.data() + NP, pp.data() + 2 * NP};
+  double* pxp[3] = {px.data(), px.data() + Pn, px.data() + 2 * Pn};
+  if (kind == 0) {
+    orc_poisson_fmm_interactions(h, qp.data(), qx.data(), ppp[0], pxp[0]);
+  } else {
+    std::vector<double> zp(NP, 0.0), zx(Pn, 0.0);
+    const double* dp[3] = {zp.data(), zp.data(), zp.data()};
+    const double* dx[3] = {zx.data(), zx.data(), zx.data()};
+    orc_bve_vel_interactions(h, dp, dx, qp.data(), qx.data(), eps, ppp, pxp);
+  }
+  err |= l2l_sweep(c, nc, pxp);
+#pragma omp parallel for schedule(dynamic, 8) reduction(| : err)
+  for (int64_t i = 0; i < NT; i++) {
+    const double xt[3] = {tx[i], ty[i], tz[i]};
+    double xieta[2], basis[121], xs_[3];
+    xieta_from_xyz(xt[0], xt[1], xt[2], xieta);
+    const int leaf = point_locate_panel(c, xt[0], xt[1], xt[2]);
+    for (int k = 0; k < nc; k++) out[k][i] = 0;
+    if (leaf < 0) {
+      err |= 1;
+      continue;
+    }
+    err |= bli_in_panel(c.pn[leaf], xieta[0], xieta[1], c.d, basis);
+    double acc[3] = {0, 0, 0};
+    for (int k = 0; k < c.n; k++)
+      for (int q = 0; q < nc; q++) acc[q] += basis[k] * pxp[q][(size_t)leaf * c.n + k];
+    for (int e = c.tgt_start[leaf]; e < c.tgt_start[leaf + 1]; e++) {
+      const int ii = c.by_target[e];
+      const int source = c.ilist[3 * ii + 1], type = c.ilist[3 * ii + 2];
+      const bool spart = type == 0 || type == 2;
+      const Side ss = side_of(c, source, spart);
+      double xos, xss, eos, ess;
+      panel_frame(c.pn[source], xos, xss, eos, ess);
+      const double* sq = spart ? qp.data() + (size_t)(source - c.offset) * c.m2 : qx.data() + (size_t)source * c.n;
+      for (int k = 0; k < ss.count; k++) {
+        xyz_from_xieta(ss.tab[4 * k] * xss + xos, ss.tab[4 * k + 1] * ess + eos, c.pn[source].face, xs_);
+        if (kind == 0) {
+          const double dp = xt[0] * xs_[0] + xt[1] * xs_[1] + xt[2] * xs_[2];
+          if (dp < 1.0 - 1e-15) acc[0] += kNegInv4Pi * std::log(1 - dp) * sq[k];
+        } else {
+          double x_s = xs_[0], y_s = xs_[1], z_s = xs_[2];
+          project_to_sphere(x_s, y_s, z_s);
+          const double dp = xt[0] * x_s + xt[1] * y_s + xt[2] * z_s;
+          const double g = kNegInv4Pi / (1.0 - dp + eps) * sq[k];
+          acc[0] += g * (xt[1] * z_s - xt[2] * y_s);
+          acc[1] += g * (xt[2] * x_s - xt[0] * z_s);
+          acc[2] += g * (xt[0] * y_s - xt[1] * x_s);
+        }
+      }
+    }
+    for (int q = 0; q < nc; q++) out[q][i] = acc[q];
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
 }
 
 // bve_time_step.cpp:91-104 (bve_disp_update) and :131-159 (bve_disp_interp).

@@ -262,6 +262,24 @@ class Context:
         self._check(self._lib.sbslpm_poisson_solve(self.h, _ptr(area), _ptr(f), _ptr(psi)), "poisson_solve")
         return psi
 
+    def poisson_solve_targets(self, area, f, tx, ty, tz, psi=None):
+        """Poisson solve evaluated at arbitrary unit-sphere targets (1-D arrays):
+        leaf far field by BLI (child_panel_interp_3_ll, fmm_funcs_impl.hpp:318-364)
+        plus the leaf's near field."""
+        nt = tx.numel()
+        psi = self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") if psi is None else psi
+        self._check(self._lib.sbslpm_poisson_solve_targets(self.h, _ptr(area), _ptr(f), nt, _ptr(tx), _ptr(ty),
+                                                           _ptr(tz), _ptr(psi)), "poisson_solve_targets")
+        return psi
+
+    def bve_velocity_targets(self, area, vor, tx, ty, tz, vel=None):
+        """Biot-Savart velocity at arbitrary unit-sphere targets (1-D arrays)."""
+        nt = tx.numel()
+        vel = [self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") for _ in range(3)] if vel is None else vel
+        self._check(self._lib.sbslpm_bve_velocity_targets(self.h, _ptr(area), _ptr(vor), nt, _ptr(tx), _ptr(ty),
+                                                          _ptr(tz), *[_ptr(v) for v in vel]), "bve_velocity_targets")
+        return vel
+
     def direct_sum_inv_lap(self, x, y, z, area, pots, soln=None, lb=0, ub=None):
         """direct_sum_inv_lap, direct_sum.cpp:8-15 (1-D arrays, accumulates)."""
         n = x.numel()

@@ -29,7 +29,9 @@ struct sbslpm_ctx {
   int* zero_rows = nullptr;
   int nzero_rows = 0;
   SourceSet* sets = nullptr;
+  int* set_start = nullptr;
   int ntiles = 0;
+  int nonleaf_tiles = 0;
 
   // workspace
   SrcRec64* rec64 = nullptr;
@@ -180,6 +182,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   size_t o_pl = A.take<int>(t.proxy_leaf.size()), o_pxe = A.take<double>(t.proxy_xieta.size());
   size_t o_err = A.take<int>(4), o_zr = A.take<int>(t.zero_rows.size());
   size_t o_tiles = A.take<TargetTile>(t.tiles.size()), o_sets = A.take<SourceSet>(t.srcsets.size());
+  size_t o_ss = A.take<int>(t.set_start.size());
   size_t o_rec64 = A.take<SrcRec64>(U), o_rec32 = A.take<SrcRec32>(U), o_recmf = A.take<SrcRecMfma>(t.U16);
   size_t o_qpart = A.take<double>(NP), o_qproxy = A.take<double>(Pn);
   size_t o_phipart = A.take<double>(3 * NP), o_phiproxy = A.take<double>(3 * Pn);
@@ -210,6 +213,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   rc |= upload(c, b + o_tiles, t.tiles.data(), t.tiles.size() * sizeof(TargetTile));
   rc |= upload(c, b + o_sets, t.srcsets.data(), t.srcsets.size() * sizeof(SourceSet));
   rc |= upload(c, b + o_zr, t.zero_rows.data(), t.zero_rows.size() * sizeof(int));
+  rc |= upload(c, b + o_ss, t.set_start.data(), t.set_start.size() * sizeof(int));
   if (rc != SBSLPM_OK || hipMemset(b + o_err, 0, 4 * sizeof(int)) != hipSuccess) {
     hipFree(c->arena);
     delete c;
@@ -230,6 +234,8 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   c->tiles = (TargetTile*)(b + o_tiles);
   c->sets = (SourceSet*)(b + o_sets);
   c->ntiles = (int)t.tiles.size();
+  c->nonleaf_tiles = t.nonleaf_tiles;
+  c->set_start = (int*)(b + o_ss);
   c->zero_rows = (int*)(b + o_zr);
   c->nzero_rows = (int)t.zero_rows.size();
   c->rec64 = (SrcRec64*)(b + o_rec64);
@@ -450,6 +456,46 @@ int sbslpm_poisson_solve(sbslpm_ctx* c, const double* area, const double* f, dou
   int rc = sbslpm_poisson_fmm_interactions(c, c->qpart, c->qproxy, c->phipart[0], c->phiproxy[0]);
   if (rc) return rc;
   return sbslpm_downward_pass(c, c->phiproxy[0], c->phipart[0], psi);
+}
+
+// Poisson (kind 0) or Biot-Savart (kind 1) FMM at arbitrary targets, FP64:
+// upward pass, pair sums for the non-leaf target panels only, L2L sweep
+// (the last level assigns the leaf rows), then one wave per target.
+static int eval_targets(sbslpm_ctx* c, int kind, const double* area, const double* f, int64_t nt, const double* tx,
+                        const double* ty, const double* tz, double* const* out) {
+  const int nc = kind == 0 ? 1 : 3;
+  launch_upward_pass(c->dt, area, f, c->qpart, c->qproxy, c->stream);
+  launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, nc, c->phiproxy, c->stream);
+  // the Poisson kernel reads unprojected nodes, Biot-Savart re-projects (inverse_poisson_impl.hpp:199-204,
+  // bve_vel_impl.hpp:54-60)
+  launch_source_records64(c->dt, c->qpart, c->qproxy, nullptr, nullptr, kind == 1, c->rec64, c->stream);
+  if (kind == 0)
+    launch_poisson_interact64(c->dt, c->tiles, c->nonleaf_tiles, c->sets, c->rec64, c->phipart[0], c->phiproxy[0],
+                              c->stream, true);
+  else
+    launch_bve_interact64(c->dt, c->tiles, c->nonleaf_tiles, c->sets, c->rec64, c->prm.kernel_eps, c->phipart,
+                          c->phiproxy, c->stream, true);
+  launch_l2l(c->dt, nc, c->phiproxy, c->stream, true);
+  const double* far[3] = {c->phiproxy[0], c->phiproxy[1], c->phiproxy[2]};
+  launch_eval_targets(c->dt, kind, c->set_start, c->sets, c->rec64, far, c->prm.kernel_eps, nt, tx, ty, tz, out,
+                      c->stream);
+  return launched(c);
+}
+
+int sbslpm_poisson_solve_targets(sbslpm_ctx* c, const double* area, const double* f, int64_t nt, const double* tx,
+                                 const double* ty, const double* tz, double* psi) {
+  if (!c || !area || !f || nt < 0 || (nt > 0 && (!tx || !ty || !tz || !psi)))
+    return fail(c, SBSLPM_E_ARG, "poisson_solve_targets: bad argument");
+  double* out[3] = {psi, nullptr, nullptr};
+  return eval_targets(c, 0, area, f, nt, tx, ty, tz, out);
+}
+
+int sbslpm_bve_velocity_targets(sbslpm_ctx* c, const double* area, const double* vor, int64_t nt, const double* tx,
+                                const double* ty, const double* tz, double* vx, double* vy, double* vz) {
+  if (!c || !area || !vor || nt < 0 || (nt > 0 && (!tx || !ty || !tz || !vx || !vy || !vz)))
+    return fail(c, SBSLPM_E_ARG, "bve_velocity_targets: bad argument");
+  double* out[3] = {vx, vy, vz};
+  return eval_targets(c, 1, area, vor, nt, tx, ty, tz, out);
 }
 
 }  // extern "C"

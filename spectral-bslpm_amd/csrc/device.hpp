@@ -38,6 +38,28 @@ struct DevTree {
   int64_t U16;    // padded record count of the MFMA kernels
 };
 
+// interp_funcs_impl.hpp:114-142: descend the quadtree from the face panel.
+// Returns the leaf panel id or -1 (no child takes the point, e.g. NaN); xe gets
+// the label coordinates on the located face (xieta_from_xyz, :67-98, chooses
+// the same face).  Ties on a mid-line go to the first child in the order
+// (hi,hi), (lo,hi), (lo,lo), (hi,lo), as in the reference.
+__device__ inline int locate_leaf(const DevTree& t, double x, double y, double z, double* xe) {
+  int cur = face_from_xyz(x, y, z);
+  xieta_from_xyz_face(x, y, z, cur, xe);
+  for (int guard = 0; guard < 16; ++guard) {
+    const DevPanel& p = t.panels[cur];
+    if (p.is_leaf) return cur;
+    int c;
+    if (xe[0] >= p.mid_xi && xe[1] >= p.mid_eta) c = 0;
+    else if (xe[0] <= p.mid_xi && xe[1] >= p.mid_eta) c = 1;
+    else if (xe[0] <= p.mid_xi && xe[1] <= p.mid_eta) c = 2;
+    else if (xe[0] >= p.mid_xi && xe[1] <= p.mid_eta) c = 3;
+    else return -1;
+    cur = p.child0 + c;
+  }
+  return -1;
+}
+
 // Source record of the FP64 pair kernels: displaced, re-projected source
 // position and its charge pre-multiplied by -1/(4 pi).
 struct SrcRec64 {
@@ -70,6 +92,9 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
                         double* q_proxy, hipStream_t s);
 // assign_leaf_rows: the last L2L level overwrites the leaf proxy rows instead of
 // adding to them (valid when nothing else writes those rows).
+// L2L over every level (no L2P); the last level assigns the leaf rows when
+// assign_leaf_rows.
+void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream_t s, bool assign_leaf_rows);
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
                           double* const* sols, hipStream_t s, bool assign_leaf_rows = false);
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
@@ -103,6 +128,14 @@ void launch_source_records_mfma(const DevTree& t, const double* q_part, const do
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
                               hipStream_t s, bool assign = false);
+
+// ---- evaluation at arbitrary targets (targets.hip)
+// kind 0: log kernel (psi), kind 1: Biot-Savart (3 components).  far[f] are the
+// [P][n] proxy potentials after the L2L sweep (leaf rows = far field of each
+// leaf); set_start[P+1] is the per-target-panel CSR into sets.
+void launch_eval_targets(const DevTree& t, int kind, const int* set_start, const SourceSet* sets,
+                         const SrcRec64* rec, const double* const* far, double eps, int64_t nt, const double* tx,
+                         const double* ty, const double* tz, double* const* out, hipStream_t s);
 
 // ---- displacement, departure points, remap (remap.hip)
 void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,

@@ -226,21 +226,26 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
 
 // fmm_funcs.cpp:357-392.  phi_proxy rows are accumulated top-down; sols are
 // assigned = L2P(leaf proxies) + phi_part (phi_part entries may be null).
-void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
-                          double* const* sols, hipStream_t s, bool assign_leaf_rows) {
-  Ptr3 phi{}; CPtr3 part{}, leafrows{}; Ptr3 out{};
-  for (int f = 0; f < ncomp; ++f) {
-    phi.p[f] = phi_proxy[f];
-    part.p[f] = phi_part ? phi_part[f] : nullptr;
-    leafrows.p[f] = phi_proxy[f] + (int64_t)t.offset * t.n;
-    out.p[f] = sols[f];
-  }
+void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream_t s, bool assign_leaf_rows) {
+  Ptr3 phi{};
+  for (int f = 0; f < ncomp; ++f) phi.p[f] = phi_proxy[f];
   for (int l = 0; l + 1 < t.levels; ++l) {
     int pbegin = 2 * ((1 << (2 * l)) - 1);
     int count = 6 << (2 * l);
     const int assign = (assign_leaf_rows && l + 2 == t.levels) ? 1 : 0;
     hipLaunchKernelGGL(k_l2l, dim3(count), dim3(256), 0, s, t, pbegin, ncomp, phi, assign);
   }
+}
+
+void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
+                          double* const* sols, hipStream_t s, bool assign_leaf_rows) {
+  CPtr3 part{}, leafrows{}; Ptr3 out{};
+  for (int f = 0; f < ncomp; ++f) {
+    part.p[f] = phi_part ? phi_part[f] : nullptr;
+    leafrows.p[f] = phi_proxy[f] + (int64_t)t.offset * t.n;
+    out.p[f] = sols[f];
+  }
+  launch_l2l(t, ncomp, phi_proxy, s, assign_leaf_rows);
   CPtr3 cphi{};
   for (int f = 0; f < ncomp; ++f) cphi.p[f] = phi_proxy[f];
   if (is_identity_p2m(t))

@@ -27,26 +27,6 @@ struct Vel12 {
 
 __device__ inline void flag(const DevTree& t, int bit) { atomicOr(t.err, bit); }
 
-// interp_funcs_impl.hpp:114-142.  Returns the leaf panel id or -1; xe gets the
-// label coordinates on the located face (xieta_from_xyz, :67-98, chooses the
-// same face).
-__device__ inline int locate(const DevTree& t, double x, double y, double z, double* xe) {
-  int cur = face_from_xyz(x, y, z);
-  xieta_from_xyz_face(x, y, z, cur, xe);
-  for (int guard = 0; guard < 16; ++guard) {
-    const DevPanel& p = t.panels[cur];
-    if (p.is_leaf) return cur;
-    int c;
-    if (xe[0] >= p.mid_xi && xe[1] >= p.mid_eta) c = 0;
-    else if (xe[0] <= p.mid_xi && xe[1] >= p.mid_eta) c = 1;
-    else if (xe[0] <= p.mid_xi && xe[1] <= p.mid_eta) c = 2;
-    else if (xe[0] >= p.mid_xi && xe[1] <= p.mid_eta) c = 3;
-    else return -1;
-    cur = p.child0 + c;
-  }
-  return -1;
-}
-
 // 1-D barycentric weights, compile-time degree M-1 (see bli_weights_1d).
 template <int M>
 __device__ inline bool bli_w(double t, double lo, double hi, const double* __restrict__ cheb, double* w) {
@@ -145,7 +125,7 @@ __global__ __launch_bounds__(256) void k_disp_interp(DevTree t, CPtr3 dpart, Ptr
 template <int M>
 __device__ inline void vel_at(const DevTree& t, double x, double y, double z, const double* const* f, double* k) {
   double xe[2];
-  const int leaf = locate(t, x, y, z, xe);
+  const int leaf = locate_leaf(t, x, y, z, xe);
   if (leaf < 0) {
     flag(t, kErrLocate);
     k[0] = k[1] = k[2] = 0;
@@ -201,7 +181,7 @@ __global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* 
   if (u >= t.npart) return;
   double xe[2];
   const double X = dx[u], Y = dy[u], Z = dz[u];
-  const int leaf = locate(t, X, Y, Z, xe);
+  const int leaf = locate_leaf(t, X, Y, Z, xe);
   if (leaf < 0) {
     flag(t, kErrLocate);
     new_vor[u] = 0;

@@ -306,6 +306,10 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
     }
   }
 
+  t.set_start = cnt;
+  t.nonleaf_tiles = 0;
+  while (t.nonleaf_tiles < (int)t.tiles.size() && !t.pn[t.tiles[t.nonleaf_tiles].panel].is_leaf) ++t.nonleaf_tiles;
+
   // Proxy rows that are not interaction targets: non-leaf panels without
   // tiles, and the leaf rows when there is no L2L level to write them.
   t.zero_rows.clear();
