@@ -38,26 +38,21 @@ struct DevTree {
   int64_t U16;    // padded record count of the MFMA kernels
 };
 
-// interp_funcs_impl.hpp:114-142: descend the quadtree from the face panel.
-// Returns the leaf panel id or -1 (no child takes the point, e.g. NaN); xe gets
-// the label coordinates on the located face (xieta_from_xyz, :67-98, chooses
-// the same face).  Ties on a mid-line go to the first child in the order
-// (hi,hi), (lo,hi), (lo,lo), (hi,lo), as in the reference.
+// 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps
+// (each step squares the relative error; the estimate is good to ~2^-26), so
+// the result is within an ulp of the correctly rounded quotient.
+__device__ inline double rcp64(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// interp_funcs_impl.hpp:114-142: leaf containing (x,y,z) or -1; xe gets the
+// label coordinates on the located face (locate_uniform: no panel reads).
 __device__ inline int locate_leaf(const DevTree& t, double x, double y, double z, double* xe) {
-  int cur = face_from_xyz(x, y, z);
-  xieta_from_xyz_face(x, y, z, cur, xe);
-  for (int guard = 0; guard < 16; ++guard) {
-    const DevPanel& p = t.panels[cur];
-    if (p.is_leaf) return cur;
-    int c;
-    if (xe[0] >= p.mid_xi && xe[1] >= p.mid_eta) c = 0;
-    else if (xe[0] <= p.mid_xi && xe[1] >= p.mid_eta) c = 1;
-    else if (xe[0] <= p.mid_xi && xe[1] <= p.mid_eta) c = 2;
-    else if (xe[0] >= p.mid_xi && xe[1] <= p.mid_eta) c = 3;
-    else return -1;
-    cur = p.child0 + c;
-  }
-  return -1;
+  return locate_uniform(t.levels, x, y, z, xe);
 }
 
 // Source record of the FP64 pair kernels: displaced, re-projected source

@@ -37,16 +37,6 @@ __device__ inline int xcd_remap(int b, int nb) {
   return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
 }
 
-// 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps
-// (each step squares the relative error; the estimate is good to ~2^-26).
-__device__ inline double rcp64(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
-
 // Target-major ownership: every target point is written by exactly one lane,
 // so `assign` may replace the accumulate of the reference functors.
 __device__ inline void store_u(const DevTree& t, int64_t u, Ptr3 part, Ptr3 proxy, double a, double b, double c,
@@ -191,9 +181,8 @@ __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* 
   double w0 = 0, w1 = 0, w2 = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
-    const DevPanel& PS = t.panels[S.panel];
     // target relative to the source panel centre: a + (c_t - c_s)
-    const float px = ax + (float)(PT.cx - PS.cx), py = ay + (float)(PT.cy - PS.cy), pz = az + (float)(PT.cz - PS.cz);
+    const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
     const SrcRec32* __restrict__ r = rec + S.first;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f;
 #pragma unroll 4
@@ -227,8 +216,7 @@ __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* 
   double acc = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
-    const DevPanel& PS = t.panels[S.panel];
-    const float px = ax + (float)(PT.cx - PS.cx), py = ay + (float)(PT.cy - PS.cy), pz = az + (float)(PT.cz - PS.cz);
+    const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
     const SrcRec32* __restrict__ r = rec + S.first;
     float v = 0.f;
 #pragma unroll 4
@@ -329,50 +317,68 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
     az[tt] = (float)(t.pz[u] - PT.cz);
   }
   const int off_b = 64 + 4 * (grp * 4 + (lane & 3));  // float4: (q b, q)[lane % 4] of sources 4 grp + 0..3
-  for (int e = T.src_begin; e < T.src_end; ++e) {
-    const SourceSet S = sets[e];
-    const DevPanel& PS = t.panels[S.panel];
-    const float dx = (float)(PT.cx - PS.cx), dy = (float)(PT.cy - PS.cy), dz = (float)(PT.cz - PS.cz);
-    float px[NTT], py[NTT], pz[NTT], b1[NTT], alpha[NTT];
-    v4f acc[NTT];
+  if (T.src_begin >= T.src_end) return;
+  // b1 = (-p_x, -p_y, -p_z, 1)[lane / 16] without branches
+  const float sx = grp == 0 ? -1.f : 0.f, sy = grp == 1 ? -1.f : 0.f, sz = grp == 2 ? -1.f : 0.f;
+  const float s1 = grp == 3 ? 1.f : 0.f;
+  float px[NTT], py[NTT], pz[NTT], b1[NTT], alpha[NTT];
+  v4f acc[NTT];
+  auto begin_set = [&](const SourceSet& S) {
 #pragma unroll
     for (int tt = 0; tt < NTT; ++tt) {
-      px[tt] = ax[tt] + dx;
-      py[tt] = ay[tt] + dy;
-      pz[tt] = az[tt] + dz;
+      px[tt] = ax[tt] + S.ox;
+      py[tt] = ay[tt] + S.oy;
+      pz[tt] = az[tt] + S.oz;
       alpha[tt] = fmaf(0.5f, fmaf(px[tt], px[tt], fmaf(py[tt], py[tt], pz[tt] * pz[tt])), eps);
-      b1[tt] = grp == 0 ? -px[tt] : grp == 1 ? -py[tt] : grp == 2 ? -pz[tt] : 1.0f;
+      b1[tt] = fmaf(sx, px[tt], fmaf(sy, py[tt], fmaf(sz, pz[tt], s1)));
       acc[tt] = v4f{0.f, 0.f, 0.f, 0.f};
     }
-    const float* __restrict__ r = rec + (int64_t)S.first16 * 8;
-    const int nst = (S.count + 15) >> 4;
-    float a1 = r[lane];
-    v4f bq = *reinterpret_cast<const v4f*>(r + off_b);
-    for (int st = 0; st < nst; ++st) {
-      const float* __restrict__ rn = r + 128 * min(st + 1, nst - 1);
-      const float na1 = rn[lane];
-      const v4f nbq = *reinterpret_cast<const v4f*>(rn + off_b);
-      v4f d[NTT];
+  };
+  // One flat loop over the 16-source tiles of all source sets, so that the
+  // operands of the next tile are loaded one tile ahead across set boundaries
+  // and the set records two sets ahead (the loads stay in the loop body).
+  const int last = T.src_end - 1;
+  int e = T.src_begin;
+  SourceSet S = sets[e], Sn = sets[min(e + 1, last)], Snn = sets[min(e + 2, last)];
+  begin_set(S);
+  int nst = (S.count + 15) >> 4, st = 0;
+  const float* __restrict__ r = rec + (int64_t)S.first16 * 8;
+  float a1 = r[lane];
+  v4f bq = *reinterpret_cast<const v4f*>(r + off_b);
+  while (true) {
+    const float* __restrict__ rn = st + 1 < nst ? r + 128 * (st + 1) : rec + (int64_t)Sn.first16 * 8;
+    const float na1 = rn[lane];
+    const v4f nbq = *reinterpret_cast<const v4f*>(rn + off_b);
+    v4f d[NTT];
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt)
+      d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) d[tt][q] = __builtin_amdgcn_rcpf(d[tt][q] + alpha[tt]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int tt = 0; tt < NTT; ++tt)
-        d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q], d[tt][q], acc[tt], 0, 0, 0);
+    a1 = na1;
+    bq = nbq;
+    if (++st == nst) {
 #pragma unroll
-      for (int tt = 0; tt < NTT; ++tt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) d[tt][q] = __builtin_amdgcn_rcpf(d[tt][q] + alpha[tt]);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int tt = 0; tt < NTT; ++tt)
-          acc[tt] = __builtin_amdgcn_mfma_f32_4x4x1f32(bq[q], d[tt][q], acc[tt], 0, 0, 0);
-      a1 = na1;
-      bq = nbq;
-    }
-#pragma unroll
-    for (int tt = 0; tt < NTT; ++tt) {
-      w[tt][0] += (double)fmaf(-px[tt], acc[tt][3], acc[tt][0]);
-      w[tt][1] += (double)fmaf(-py[tt], acc[tt][3], acc[tt][1]);
-      w[tt][2] += (double)fmaf(-pz[tt], acc[tt][3], acc[tt][2]);
+      for (int tt = 0; tt < NTT; ++tt) {
+        w[tt][0] += (double)fmaf(-px[tt], acc[tt][3], acc[tt][0]);
+        w[tt][1] += (double)fmaf(-py[tt], acc[tt][3], acc[tt][1]);
+        w[tt][2] += (double)fmaf(-pz[tt], acc[tt][3], acc[tt][2]);
+      }
+      if (++e > last) break;
+      S = Sn;
+      Sn = Snn;
+      Snn = sets[min(e + 2, last)];
+      begin_set(S);
+      r = rec + (int64_t)S.first16 * 8;
+      nst = (S.count + 15) >> 4;
+      st = 0;
     }
   }
 }

@@ -50,6 +50,37 @@ __host__ __device__ inline void xieta_from_xyz_face(double x, double y, double z
   }
 }
 
+// Point location in the uniform quadtree, the descent of point_locate_panel
+// (interp_funcs_impl.hpp:114-142) done arithmetically instead of by reading
+// panels.  The level-l panel with integer coordinates (i, j) spans label
+// units [-1 + i h, -1 + (i+1) h], h = 2^(1-l), so max + min = (2i+1) h - 2
+// exactly and its mid-line (pi/8)(max + min) rounds exactly as the stored
+// tree's.  Children are numbered 4k + c below their parent's level-relative
+// index k in the order (hi,hi), (lo,hi), (lo,lo), (hi,lo), with the
+// reference's comparison sequence deciding ties.  Returns the leaf panel id
+// (level_start[levels-1] + k) or -1 when no child takes the point (NaN);
+// xe receives the label coordinates on the located face.
+__host__ __device__ inline int locate_uniform(int levels, double x, double y, double z, double* xe) {
+  const int face = face_from_xyz(x, y, z);
+  xieta_from_xyz_face(x, y, z, face, xe);
+  int k = face, i = 0, j = 0;
+  for (int l = 0; l + 1 < levels; ++l) {
+    const double h = ldexp(1.0, 1 - l);
+    const double mx = 0.5 * kPi / 4.0 * ((2 * i + 1) * h - 2.0);
+    const double me = 0.5 * kPi / 4.0 * ((2 * j + 1) * h - 2.0);
+    int c;
+    if (xe[0] >= mx && xe[1] >= me) c = 0;
+    else if (xe[0] <= mx && xe[1] >= me) c = 1;
+    else if (xe[0] <= mx && xe[1] <= me) c = 2;
+    else if (xe[0] >= mx && xe[1] <= me) c = 3;
+    else return -1;
+    i = 2 * i + ((c == 0 || c == 3) ? 1 : 0);
+    j = 2 * j + (c <= 1 ? 1 : 0);
+    k = 4 * k + c;
+  }
+  return 2 * ((1 << (2 * (levels - 1))) - 1) + k;
+}
+
 // general_utils_impl.hpp:179-185
 __host__ __device__ inline void project_to_sphere(double& x, double& y, double& z) {
   double n = sqrt(x * x + y * y + z * z);

@@ -50,7 +50,7 @@ __device__ inline bool bli_w(double t, double lo, double hi, const double* __res
   for (int i = 0; i < M; ++i) {
     double bw = (i == 0 || i == D) ? 0.5 : 1.0;
     if (i & 1) bw = -bw;
-    w[i] = bw / (t - node[i]);
+    w[i] = bw * rcp64(t - node[i]);  // |t - node| >= 1e-15 here
     den += w[i];
   }
   const double inv = 1.0 / den;
@@ -134,6 +134,15 @@ __device__ inline void vel_at(const DevTree& t, double x, double y, double z, co
   if (!interp_leaf<M, 3>(t, leaf, xe[0], xe[1], f, k)) flag(t, kErrBliBounds);
 }
 
+// project_to_sphere (general_utils_impl.hpp:179-185) with one reciprocal of
+// the norm instead of three divisions (each result within an ulp).
+__device__ inline void project_rcp(double& x, double& y, double& z) {
+  const double r = rcp64(sqrt(x * x + y * y + z * z));
+  x *= r;
+  y *= r;
+  z *= r;
+}
+
 // bve_find_departure_points, bve_time_step.cpp:206-308.
 template <int M>
 __global__ __launch_bounds__(128) void k_departure(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
@@ -144,29 +153,29 @@ __global__ __launch_bounds__(128) void k_departure(DevTree t, const double* __re
   const double X = x[u], Y = y[u], Z = z[u];
   double k1[3] = {v.p[9][u], v.p[10][u], v.p[11][u]}, k2[3], k3[3], k4[3];
   double a = X - 0.5 * dt * k1[0], b = Y - 0.5 * dt * k1[1], c = Z - 0.5 * dt * k1[2];
-  project_to_sphere(a, b, c);
+  project_rcp(a, b, c);
   vel_at<M>(t, a, b, c, v.p + 6, k2);
   a = X - 0.5 * dt * k2[0]; b = Y - 0.5 * dt * k2[1]; c = Z - 0.5 * dt * k2[2];
-  project_to_sphere(a, b, c);
+  project_rcp(a, b, c);
   vel_at<M>(t, a, b, c, v.p + 3, k3);
   a = X - dt * k3[0]; b = Y - dt * k3[1]; c = Z - dt * k3[2];
-  project_to_sphere(a, b, c);
+  project_rcp(a, b, c);
   vel_at<M>(t, a, b, c, v.p + 0, k4);
   if (combine == 0) {
     // the reference's sequence (:288-303)
     a = X - 1.0 / 6.0 * dt * k1[0]; b = Y - 1.0 / 6.0 * dt * k1[1]; c = Z - 1.0 / 6.0 * dt * k1[2];
-    project_to_sphere(a, b, c);
+    project_rcp(a, b, c);
     a -= -1.0 / 3.0 * dt * k2[0]; b -= -1.0 / 3.0 * dt * k2[1]; c -= -1.0 / 3.0 * dt * k2[2];
-    project_to_sphere(a, b, c);
+    project_rcp(a, b, c);
     a -= -1.0 / 3.0 * dt * k3[0]; b -= -1.0 / 3.0 * dt * k3[1]; c -= -1.0 / 3.0 * dt * k3[2];
-    project_to_sphere(a, b, c);
+    project_rcp(a, b, c);
     a -= -1.0 / 6.0 * dt * k4[0]; b -= -1.0 / 6.0 * dt * k4[1]; c -= -1.0 / 6.0 * dt * k4[2];
-    project_to_sphere(a, b, c);
+    project_rcp(a, b, c);
   } else {
     a = X - dt / 6.0 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0]);
     b = Y - dt / 6.0 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1]);
     c = Z - dt / 6.0 * (k1[2] + 2 * k2[2] + 2 * k3[2] + k4[2]);
-    project_to_sphere(a, b, c);
+    project_rcp(a, b, c);
   }
   dx[u] = a; dy[u] = b; dz[u] = c;
 }

@@ -159,19 +159,8 @@ void traverse(Tree& t) {
 }  // namespace
 
 int locate_leaf(const Tree& t, double x, double y, double z) {
-  int cur = face_from_xyz(x, y, z);
   double xe[2];
-  xieta_from_xyz_face(x, y, z, cur, xe);
-  while (!t.pn[cur].is_leaf) {
-    const HostPanel& p = t.pn[cur];
-    double mx = 0.5 * kPi / 4.0 * (p.max_xi + p.min_xi), me = 0.5 * kPi / 4.0 * (p.max_eta + p.min_eta);
-    if (xe[0] >= mx && xe[1] >= me) cur = p.child[0];
-    else if (xe[0] <= mx && xe[1] >= me) cur = p.child[1];
-    else if (xe[0] <= mx && xe[1] <= me) cur = p.child[2];
-    else if (xe[0] >= mx && xe[1] <= me) cur = p.child[3];
-    else return -1;
-  }
-  return cur;
+  return locate_uniform(t.cfg.levels, x, y, z, xe);
 }
 
 std::string build_tree(const TreeConfig& cfg, Tree& t) {
@@ -277,9 +266,12 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
   for (int64_t e = 0; e < I; ++e) order[fill[t.ilist[3 * e]]++] = (int)e;
   t.srcsets.resize(I);
   for (int64_t k = 0; k < I; ++k) {
-    int s = t.ilist[3 * order[k] + 1];
+    int s = t.ilist[3 * order[k] + 1], tg = t.ilist[3 * order[k]];
     SourceSet ss{};
     ss.panel = s;
+    ss.ox = (float)(t.dev[tg].cx - t.dev[s].cx);
+    ss.oy = (float)(t.dev[tg].cy - t.dev[s].cy);
+    ss.oz = (float)(t.dev[tg].cz - t.dev[s].cz);
     if (t.pn[s].is_leaf) {
       ss.first = (int)((int64_t)(s - t.offset) * t.m2);
       ss.count = t.m2;

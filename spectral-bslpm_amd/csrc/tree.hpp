@@ -48,8 +48,14 @@ struct TargetTile {
 // A source point set: `count` consecutive records starting at `first` in the
 // unified point index space, belonging to panel `panel`; `first16` is its
 // first record in the 16-padded record space of the MFMA kernels.
+// One interaction seen from its target: the source points [first, first+count)
+// of the unified index space (first16 in the 16-padded MFMA record space) and
+// the target-panel centre minus the source-panel centre, rounded to FP32 (the
+// mixed-precision kernels work in panel-centred coordinates).
 struct SourceSet {
   int first, count, panel, first16;
+  float ox, oy, oz;
+  int pad;
 };
 
 struct Tree {

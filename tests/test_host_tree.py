@@ -80,15 +80,25 @@ def test_transfer_matrices_equal_per_node_bli(sb, orc, d):
         np.testing.assert_allclose(basis, np.outer(p2m[:, k], p2m[:, l]).ravel(), rtol=0, atol=1e-14)
 
 
-def test_host_locate_matches_oracle(sb, orc):
-    h = sb.HostTree(sb.RunConfig(levels=4, interp_degree=4))
-    o = orc.OracleFMM(4, 4, 4, 0.7, 0)
+@pytest.mark.parametrize("L", [1, 2, 4, 6])
+def test_host_locate_matches_oracle(sb, orc, L):
+    """The arithmetic descent against the oracle's panel-reading descent, on
+    random points and on points built on every mid-line, edge and corner of
+    the level-(L-1) grid of each face (the tie rules)."""
+    h = sb.HostTree(sb.RunConfig(levels=L, interp_degree=4))
+    o = orc.OracleFMM(L, 4, 4, 0.7, 0)
     rng = np.random.default_rng(8)
     v = rng.normal(size=(300, 3))
     v /= np.linalg.norm(v, axis=1, keepdims=True)
+    g = np.pi / 4 * np.linspace(-1, 1, 2 ** (L - 1) + 1)
+    for face in range(6):
+        for xi in g:
+            for eta in g[:: max(1, len(g) // 5)]:
+                v = np.vstack([v, orc.xyz_from_xieta(xi, eta, face)[0]])
     for p in v:
         assert h.locate(*p) == o.point_locate(*p)
-    assert h.locate(float("nan"), 0.0, 0.0) == -1
+    if L > 1:
+        assert h.locate(float("nan"), 0.0, 0.0) == -1
 
 
 def test_invalid_parameters_are_rejected(sb):
