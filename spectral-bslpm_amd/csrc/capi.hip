@@ -180,6 +180,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
          o_loT = A.take<double>(t.n);
   size_t o_p2m = A.take<double>(t.dp1 * t.m), o_p2mT = A.take<double>(t.dp1 * t.m);
   size_t o_pl = A.take<int>(t.proxy_leaf.size()), o_pxe = A.take<double>(t.proxy_xieta.size());
+  size_t o_po = A.take<int>(t.proxy_order.size());
   size_t o_err = A.take<int>(4), o_zr = A.take<int>(t.zero_rows.size());
   size_t o_tiles = A.take<TargetTile>(t.tiles.size()), o_sets = A.take<SourceSet>(t.srcsets.size());
   size_t o_ss = A.take<int>(t.set_start.size());
@@ -210,6 +211,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   rc |= upload(c, b + o_p2mT, p2mT.data(), p2mT.size() * sizeof(double));
   rc |= upload(c, b + o_pl, t.proxy_leaf.data(), t.proxy_leaf.size() * sizeof(int));
   rc |= upload(c, b + o_pxe, t.proxy_xieta.data(), t.proxy_xieta.size() * sizeof(double));
+  rc |= upload(c, b + o_po, t.proxy_order.data(), t.proxy_order.size() * sizeof(int));
   rc |= upload(c, b + o_tiles, t.tiles.data(), t.tiles.size() * sizeof(TargetTile));
   rc |= upload(c, b + o_sets, t.srcsets.data(), t.srcsets.size() * sizeof(SourceSet));
   rc |= upload(c, b + o_zr, t.zero_rows.data(), t.zero_rows.size() * sizeof(int));
@@ -227,6 +229,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   d.m2m_hiT = (const double*)(b + o_hiT); d.m2m_loT = (const double*)(b + o_loT);
   d.p2m = (const double*)(b + o_p2m); d.p2mT = (const double*)(b + o_p2mT);
   d.proxy_leaf = (const int*)(b + o_pl); d.proxy_xieta = (const double*)(b + o_pxe);
+  d.proxy_order = (const int*)(b + o_po);
   d.err = (int*)(b + o_err);
   d.P = t.P; d.Pl = t.Pl; d.offset = t.offset; d.n = t.n; d.dp1 = t.dp1; d.m = t.m; d.m2 = t.m2;
   d.levels = t.cfg.levels; d.npart = t.npart; d.U = t.U;

@@ -31,6 +31,7 @@ struct DevTree {
   const double* p2mT;  // [m][d+1]
   const int* proxy_leaf;      // [offset*n]
   const double* proxy_xieta;  // [offset*n][2]
+  const int* proxy_order;     // [offset*n] proxy nodes sorted by leaf
   int* err;
   int P, Pl, offset, n, dp1, m, m2, levels;
   int64_t npart, U;

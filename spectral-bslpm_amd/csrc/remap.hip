@@ -106,10 +106,13 @@ __global__ void k_disp_update(DevTree t, const double* __restrict__ x, const dou
 // bve_disp_interp, bve_time_step.cpp:131-159: displacement of each non-leaf
 // proxy node interpolated from the leaf that contains it (leaf and label
 // coordinates precomputed on the host, the tree being fixed in label space).
+// Threads take the nodes in leaf order, so neighbouring lanes gather the same
+// leaf tile.
 template <int M>
 __global__ __launch_bounds__(256) void k_disp_interp(DevTree t, CPtr3 dpart, Ptr3 dprox) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= (int64_t)t.offset * t.n) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)t.offset * t.n) return;
+  const int64_t v = t.proxy_order[i];
   const int leaf = t.proxy_leaf[v];
   double out[3] = {0, 0, 0};
   if (leaf < 0) {

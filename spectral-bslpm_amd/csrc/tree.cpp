@@ -342,6 +342,10 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
       t.proxy_xieta[2 * o + 1] = eta;
     }
   }
+  t.proxy_order.resize(t.proxy_leaf.size());
+  for (size_t o = 0; o < t.proxy_order.size(); ++o) t.proxy_order[o] = (int)o;
+  std::stable_sort(t.proxy_order.begin(), t.proxy_order.end(),
+                   [&](int a, int b) { return t.proxy_leaf[a] < t.proxy_leaf[b]; });
   return "";
 }
 

@@ -83,6 +83,7 @@ struct Tree {
   std::vector<int> zero_rows;           // proxy rows no pair kernel or last-level L2L writes
   std::vector<int> proxy_leaf;          // [offset*n] leaf containing each non-leaf proxy node
   std::vector<double> proxy_xieta;      // [offset*n][2] label coordinates of those nodes (radians)
+  std::vector<int> proxy_order;         // [offset*n] non-leaf proxy nodes sorted by their leaf (gather locality)
   std::vector<double> spec_x, spec_y, spec_z, spec_area;  // caller-facing particles [Pl][m2]
 
   int64_t interaction_count() const { return (int64_t)ilist.size() / 3; }
