@@ -132,6 +132,25 @@ int sbslpm_bve_vel_interactions(sbslpm_ctx* ctx, const double* disp_part_x, cons
 int sbslpm_poisson_fmm_interactions(sbslpm_ctx* ctx, const double* q_part, const double* q_proxy, double* psi_part,
                                     double* psi_proxy);
 
+/* swe_vel_interactions (fmm_interactions/swe_vel.cpp:9-35, pair kernel swe_vel_impl.hpp:12-76):
+ * Biot-Savart term of the vorticity charges plus the irrotational term of the divergence charges,
+ * u += g qz (x cross y) + g qd (y - x (x.y)), g = -1/(4 pi (1 - x.y + eps)), y = P(node + disp).
+ * Displacements may be NULL.  Accumulates into phi_part [3][Pl][m2] and phi_proxy [3][P][n]. */
+int sbslpm_swe_vel_interactions(sbslpm_ctx* ctx, const double* disp_part_x, const double* disp_part_y,
+                                const double* disp_part_z, const double* disp_proxy_x, const double* disp_proxy_y,
+                                const double* disp_proxy_z, const double* qz_part, const double* qz_proxy,
+                                const double* qd_part, const double* qd_proxy, double* phi_part_x, double* phi_part_y,
+                                double* phi_part_z, double* phi_proxy_x, double* phi_proxy_y, double* phi_proxy_z);
+/* swe_vel_interactions_2 (swe_vel.cpp:37-67, pair kernel swe_vel_impl.hpp:78-147): the same with
+ * the divergence term taken at the undisplaced node y0 and the opposite sign,
+ * u += g qz (x cross y) + g0 qd (y0 - x (x.y0)), g0 = +1/(4 pi (1 - x.y0 + eps)). */
+int sbslpm_swe_vel_interactions_2(sbslpm_ctx* ctx, const double* disp_part_x, const double* disp_part_y,
+                                  const double* disp_part_z, const double* disp_proxy_x, const double* disp_proxy_y,
+                                  const double* disp_proxy_z, const double* qz_part, const double* qz_proxy,
+                                  const double* qd_part, const double* qd_proxy, double* phi_part_x,
+                                  double* phi_part_y, double* phi_part_z, double* phi_proxy_x, double* phi_proxy_y,
+                                  double* phi_proxy_z);
+
 /* downward_pass (fmm_funcs.cpp:357-372): L2L accumulated into phi_proxy [P][n] top-down,
  * then sols [Pl][m2] = L2P(leaf proxies) + phi_part (phi_part may be NULL). */
 int sbslpm_downward_pass(sbslpm_ctx* ctx, double* phi_proxy, const double* phi_part, double* sols);
@@ -146,6 +165,13 @@ int sbslpm_bve_velocity(sbslpm_ctx* ctx, const double* area, const double* vor, 
                         const double* disp_part_y, const double* disp_part_z, const double* disp_proxy_x,
                         const double* disp_proxy_y, const double* disp_proxy_z, double* vel_x, double* vel_y,
                         double* vel_z);
+/* One shallow-water velocity evaluation (swe_time_step.cpp:640-653): upward passes of vor and div,
+ * swe_vel_interactions (variant 1) or swe_vel_interactions_2 (variant 2), downward pass.
+ * vel_{x,y,z} [Pl][m2] assigned; displacements may be NULL. */
+int sbslpm_swe_velocity(sbslpm_ctx* ctx, int variant, const double* area, const double* vor, const double* div,
+                        const double* disp_part_x, const double* disp_part_y, const double* disp_part_z,
+                        const double* disp_proxy_x, const double* disp_proxy_y, const double* disp_proxy_z,
+                        double* vel_x, double* vel_y, double* vel_z);
 /* One sphere Poisson solve (poisson_eq_fmm.cpp:129-147): psi [Pl][m2] assigned. */
 int sbslpm_poisson_solve(sbslpm_ctx* ctx, const double* area, const double* f, double* psi);
 

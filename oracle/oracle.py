@@ -79,6 +79,8 @@ def lib():
         L.orc_bve_back_rk4_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ppd]
         L.orc_bve_back_rk4_step.restype = ctypes.c_int
+        L.orc_swe_vel_interactions.argtypes = [ctypes.c_void_p, ctypes.c_int, _ppd, _ppd, _dp, _dp, _dp, _dp,
+                                               ctypes.c_double, _ppd, _ppd]
         L.orc_eval_targets.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int64,
                                        _dp, _dp, _dp, _ppd]
         L.orc_eval_targets.restype = ctypes.c_int
@@ -284,6 +286,26 @@ class OracleFMM:
         qp, qx = self.upward_pass(area, f)
         pp, px = self.poisson_fmm_interactions(qp, qx)
         return self.downward_pass([px], [pp])[0]
+
+    def swe_vel_interactions(self, variant, qz_part, qz_proxy, qd_part, qd_proxy, eps, disp_part=None,
+                             disp_proxy=None):
+        """swe_vel_interactions (variant 1) / swe_vel_interactions_2 (variant 2); returns (phi_part, phi_proxy)."""
+        if disp_part is None:
+            disp_part = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        if disp_proxy is None:
+            disp_proxy = [np.zeros((self.P, self.n)) for _ in range(3)]
+        pp = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        px = [np.zeros((self.P, self.n)) for _ in range(3)]
+        lib().orc_swe_vel_interactions(self.h, variant, _pp(disp_part), _pp(disp_proxy), _p(qz_part), _p(qz_proxy),
+                                       _p(qd_part), _p(qd_proxy), eps, _pp(pp), _pp(px))
+        return pp, px
+
+    def swe_velocity(self, variant, area, vor, div, eps, disp_part=None, disp_proxy=None):
+        """Upward passes of vor and div, SWE interactions, downward pass (swe_time_step.cpp:655-672)."""
+        qzp, qzx = self.upward_pass(area, vor)
+        qdp, qdx = self.upward_pass(area, div)
+        pp, px = self.swe_vel_interactions(variant, qzp, qzx, qdp, qdx, eps, disp_part, disp_proxy)
+        return self.downward_pass(px, pp)
 
     def eval_targets(self, kind, area, f, tx, ty, tz, eps=0.0):
         """FMM value at target points (kind "poisson": psi, "bve": velocity

@@ -708,6 +708,80 @@ void orc_bve_vel_interactions(void* h, const double* const* disp_part, const dou
   orc_bve_vel_interactions_sample(h, disp_part, disp_proxy, q_part, q_proxy, eps, phi_part, phi_proxy, 1);
 }
 
+// fmm_interactions/swe_vel.cpp:9-67 with the pair kernels of
+// swe_vel_impl.hpp:12-76 (variant 1, swe_vel_panel_interaction) and :78-147
+// (variant 2, swe_vel_panel_interaction_2), on the type-dependent point sets.
+// Source y = P(node + disp), y0 = the undisplaced node.
+//   variant 1: g = -1/(4 pi (1 - x.y + eps));
+//              u += g qz (x cross y) + g qd (y - x (x.y))
+//   variant 2: u += -g' qz (x cross y) + g'' qd (y0 - x (x.y0)),
+//              g' = 1/(4 pi (1 - x.y + eps)), g'' = 1/(4 pi (1 - x.y0 + eps))
+// phi_part[3][Pl][m2] and phi_proxy[3][P][n] are accumulated.
+void orc_swe_vel_interactions(void* h, int variant, const double* const* disp_part, const double* const* disp_proxy,
+                              const double* qz_part, const double* qz_proxy, const double* qd_part,
+                              const double* qd_proxy, double eps, double* const* phi_part, double* const* phi_proxy) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const double kInv4Pi = -kNegInv4Pi;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < c.P; t++) {
+    for (int e = c.tgt_start[t]; e < c.tgt_start[t + 1]; e++) {
+      int ii = c.by_target[e];
+      int source = c.ilist[3 * ii + 1], type = c.ilist[3 * ii + 2];
+      bool tpart = (type == 0 || type == 1), spart = (type == 0 || type == 2);
+      Side ts = side_of(c, t, tpart), ss = side_of(c, source, spart);
+      const Panel &pt = c.pn[t], &ps = c.pn[source];
+      double xot, xst, eot, est, xos, xss, eos, ess;
+      panel_frame(pt, xot, xst, eot, est);
+      panel_frame(ps, xos, xss, eos, ess);
+      double *o1, *o2, *o3;
+      if (tpart) {
+        size_t b = (size_t)(t - c.offset) * c.m2;
+        o1 = phi_part[0] + b; o2 = phi_part[1] + b; o3 = phi_part[2] + b;
+      } else {
+        size_t b = (size_t)t * c.n;
+        o1 = phi_proxy[0] + b; o2 = phi_proxy[1] + b; o3 = phi_proxy[2] + b;
+      }
+      const double *qz, *qd, *dx, *dy, *dz;
+      if (spart) {
+        size_t b = (size_t)(source - c.offset) * c.m2;
+        qz = qz_part + b; qd = qd_part + b;
+        dx = disp_part[0] + b; dy = disp_part[1] + b; dz = disp_part[2] + b;
+      } else {
+        size_t b = (size_t)source * c.n;
+        qz = qz_proxy + b; qd = qd_proxy + b;
+        dx = disp_proxy[0] + b; dy = disp_proxy[1] + b; dz = disp_proxy[2] + b;
+      }
+      double xt[3], y0[3];
+      for (int j = 0; j < ts.count; j++) {
+        xyz_from_xieta(ts.tab[4 * j] * xst + xot, ts.tab[4 * j + 1] * est + eot, pt.face, xt);
+        double a1 = 0, a2 = 0, a3 = 0;
+        for (int k = 0; k < ss.count; k++) {
+          xyz_from_xieta(ss.tab[4 * k] * xss + xos, ss.tab[4 * k + 1] * ess + eos, ps.face, y0);
+          double x_s = y0[0] + dx[k], y_s = y0[1] + dy[k], z_s = y0[2] + dz[k];
+          project_to_sphere(x_s, y_s, z_s);
+          const double dp = xt[0] * x_s + xt[1] * y_s + xt[2] * z_s;
+          double g1, g2, v0, v1, v2;
+          if (variant == 1) {
+            const double g = kNegInv4Pi / (1.0 - dp + eps);
+            g1 = g * qz[k];
+            g2 = g * qd[k];
+            v0 = x_s - xt[0] * dp; v1 = y_s - xt[1] * dp; v2 = z_s - xt[2] * dp;
+          } else {
+            const double dp2 = xt[0] * y0[0] + xt[1] * y0[1] + xt[2] * y0[2];
+            g1 = -(kInv4Pi / (1.0 - dp + eps)) * qz[k];
+            g2 = kInv4Pi / (1.0 - dp2 + eps) * qd[k];
+            v0 = y0[0] - xt[0] * dp2; v1 = y0[1] - xt[1] * dp2; v2 = y0[2] - xt[2] * dp2;
+          }
+          a1 += g1 * (xt[1] * z_s - xt[2] * y_s) + g2 * v0;
+          a2 += g1 * (xt[2] * x_s - xt[0] * z_s) + g2 * v1;
+          a3 += g1 * (xt[0] * y_s - xt[1] * x_s) + g2 * v2;
+        }
+        o1[j] += a1; o2[j] += a2; o3[j] += a3;
+      }
+    }
+  }
+}
+
 // fmm_interactions/inverse_poisson.cpp:9-33 + inverse_poisson_impl.hpp:181-210,
 // with the type-dependent point sets.  psi_part[Pl][m2], psi_proxy[P][n] accumulated.
 void orc_poisson_fmm_interactions(void* h, const double* q_part, const double* q_proxy,

@@ -72,6 +72,9 @@ SIGNATURES = {
     "sbslpm_downward_pass_3": (_I, [_V] + [_V] * 9),
     "sbslpm_bve_velocity": (_I, [_V] + [_V] * 11),
     "sbslpm_poisson_solve": (_I, [_V, _V, _V, _V]),
+    "sbslpm_swe_vel_interactions": (_I, [_V] * 17),
+    "sbslpm_swe_vel_interactions_2": (_I, [_V] * 17),
+    "sbslpm_swe_velocity": (_I, [_V, _I] + [_V] * 12),
     "sbslpm_poisson_solve_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V]),
     "sbslpm_bve_velocity_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
     "sbslpm_direct_sum_inv_lap": (_I, [_V, _I64, _V, _V, _V, _V, _V, _V, _I64, _I64]),

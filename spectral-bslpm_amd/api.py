@@ -224,6 +224,29 @@ class Context:
         self._check(self._lib.sbslpm_bve_vel_interactions(self.h, *args), "bve_vel_interactions")
         return phi_part, phi_proxy
 
+    def swe_vel_interactions(self, qz_part, qz_proxy, qd_part, qd_proxy, disp_part=None, disp_proxy=None,
+                             phi_part=None, phi_proxy=None, variant=1):
+        """swe_vel_interactions (variant 1, swe_vel.cpp:9-35) or swe_vel_interactions_2 (variant 2,
+        swe_vel.cpp:37-67); accumulates."""
+        phi_part = self.zeros_part(3) if phi_part is None else phi_part
+        phi_proxy = self.zeros_proxy(3) if phi_proxy is None else phi_proxy
+        dp = disp_part if disp_part is not None else [None] * 3
+        dx = disp_proxy if disp_proxy is not None else [None] * 3
+        args = [_ptr(a) for a in list(dp) + list(dx) + [qz_part, qz_proxy, qd_part, qd_proxy] + list(phi_part)
+                + list(phi_proxy)]
+        fn = self._lib.sbslpm_swe_vel_interactions if variant == 1 else self._lib.sbslpm_swe_vel_interactions_2
+        self._check(fn(self.h, *args), "swe_vel_interactions")
+        return phi_part, phi_proxy
+
+    def swe_velocity(self, area, vor, div, disp_part=None, disp_proxy=None, vel=None, variant=1):
+        """Upward passes of vor and div + SWE interactions + downward pass (swe_time_step.cpp:640-653)."""
+        vel = self.zeros_part(3) if vel is None else vel
+        dp = disp_part if disp_part is not None else [None] * 3
+        dx = disp_proxy if disp_proxy is not None else [None] * 3
+        args = [_ptr(a) for a in [area, vor, div] + list(dp) + list(dx) + list(vel)]
+        self._check(self._lib.sbslpm_swe_velocity(self.h, variant, *args), "swe_velocity")
+        return vel
+
     def poisson_fmm_interactions(self, q_part, q_proxy, psi_part=None, psi_proxy=None):
         """poisson_fmm_interactions, inverse_poisson.cpp:9-33 (accumulates)."""
         psi_part = self.zeros_part() if psi_part is None else psi_part

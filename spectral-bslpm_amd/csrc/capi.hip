@@ -50,6 +50,11 @@ struct sbslpm_ctx {
   double* tr_next = nullptr;
   double* scratch = nullptr;  // direct sums, grown on demand
   int64_t scratch_len = 0;
+  // shallow-water workspace, allocated on the first SWE call
+  char* swe_arena = nullptr;
+  SrcRecSwe* recswe = nullptr;
+  double* qdpart = nullptr;
+  double* qdproxy = nullptr;
 
   // Kernel timing (sbslpm_ctx_enable_timing).
   bool timing = false;
@@ -277,6 +282,7 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
     hipEventDestroy(p.b);
   }
   if (c->scratch) hipFree(c->scratch);
+  if (c->swe_arena) hipFree(c->swe_arena);
   if (c->arena) hipFree(c->arena);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
@@ -459,6 +465,80 @@ int sbslpm_poisson_solve(sbslpm_ctx* c, const double* area, const double* f, dou
   int rc = sbslpm_poisson_fmm_interactions(c, c->qpart, c->qproxy, c->phipart[0], c->phiproxy[0]);
   if (rc) return rc;
   return sbslpm_downward_pass(c, c->phiproxy[0], c->phipart[0], psi);
+}
+
+// Shallow-water workspace: records [U] and the divergence charges.
+static int ensure_swe(sbslpm_ctx* c) {
+  if (c->swe_arena) return SBSLPM_OK;
+  const Tree& t = c->tree;
+  Arena A;
+  const size_t o_rec = A.take<SrcRecSwe>(t.U), o_qp = A.take<double>(t.npart),
+               o_qx = A.take<double>((size_t)t.P * t.n), total = A.take<char>(256);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (hipMalloc(&c->swe_arena, total) != hipSuccess) {
+    c->swe_arena = nullptr;
+    return fail(c, SBSLPM_E_NOMEM, "shallow-water workspace: out of device memory");
+  }
+  c->recswe = (SrcRecSwe*)(c->swe_arena + o_rec);
+  c->qdpart = (double*)(c->swe_arena + o_qp);
+  c->qdproxy = (double*)(c->swe_arena + o_qx);
+  return SBSLPM_OK;
+}
+
+static int swe_interactions(sbslpm_ctx* c, int variant, const double* dpx, const double* dpy, const double* dpz,
+                            const double* dxx, const double* dxy, const double* dxz, const double* qz_part,
+                            const double* qz_proxy, const double* qd_part, const double* qd_proxy, double* const* op,
+                            double* const* ox, bool assign) {
+  int rc = ensure_swe(c);
+  if (rc) return rc;
+  const double* dp[3] = {dpx, dpy, dpz};
+  const double* dx[3] = {dxx, dxy, dxz};
+  launch_source_records_swe(c->dt, variant, qz_part, qz_proxy, qd_part, qd_proxy, dpx ? dp : nullptr,
+                            dxx ? dx : nullptr, c->recswe, c->stream);
+  launch_swe_interact64(c->dt, variant, c->tiles, c->ntiles, c->sets, c->recswe, c->prm.kernel_eps, op, ox, c->stream,
+                        assign);
+  return launched(c);
+}
+
+int sbslpm_swe_vel_interactions(sbslpm_ctx* c, const double* dpx, const double* dpy, const double* dpz,
+                                const double* dxx, const double* dxy, const double* dxz, const double* qz_part,
+                                const double* qz_proxy, const double* qd_part, const double* qd_proxy, double* ppx,
+                                double* ppy, double* ppz, double* pxx, double* pxy, double* pxz) {
+  if (!c || !qz_part || !qz_proxy || !qd_part || !qd_proxy || !ppx || !ppy || !ppz || !pxx || !pxy || !pxz)
+    return fail(c, SBSLPM_E_ARG, "swe_vel_interactions: null pointer");
+  double* op[3] = {ppx, ppy, ppz};
+  double* ox[3] = {pxx, pxy, pxz};
+  return swe_interactions(c, 1, dpx, dpy, dpz, dxx, dxy, dxz, qz_part, qz_proxy, qd_part, qd_proxy, op, ox, false);
+}
+
+int sbslpm_swe_vel_interactions_2(sbslpm_ctx* c, const double* dpx, const double* dpy, const double* dpz,
+                                  const double* dxx, const double* dxy, const double* dxz, const double* qz_part,
+                                  const double* qz_proxy, const double* qd_part, const double* qd_proxy, double* ppx,
+                                  double* ppy, double* ppz, double* pxx, double* pxy, double* pxz) {
+  if (!c || !qz_part || !qz_proxy || !qd_part || !qd_proxy || !ppx || !ppy || !ppz || !pxx || !pxy || !pxz)
+    return fail(c, SBSLPM_E_ARG, "swe_vel_interactions_2: null pointer");
+  double* op[3] = {ppx, ppy, ppz};
+  double* ox[3] = {pxx, pxy, pxz};
+  return swe_interactions(c, 2, dpx, dpy, dpz, dxx, dxy, dxz, qz_part, qz_proxy, qd_part, qd_proxy, op, ox, false);
+}
+
+int sbslpm_swe_velocity(sbslpm_ctx* c, int variant, const double* area, const double* vor, const double* div,
+                        const double* dpx, const double* dpy, const double* dpz, const double* dxx, const double* dxy,
+                        const double* dxz, double* vx, double* vy, double* vz) {
+  if (!c || !area || !vor || !div || !vx || !vy || !vz || (variant != 1 && variant != 2))
+    return fail(c, SBSLPM_E_ARG, "swe_velocity: bad argument");
+  int rc = ensure_swe(c);
+  if (rc) return rc;
+  launch_upward_pass(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
+  launch_upward_pass(c->dt, area, div, c->qdpart, c->qdproxy, c->stream);
+  launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, 3, c->phiproxy, c->stream);
+  rc = swe_interactions(c, variant, dpx, dpy, dpz, dxx, dxy, dxz, c->qpart, c->qproxy, c->qdpart, c->qdproxy,
+                        c->phipart, c->phiproxy, true);
+  if (rc) return rc;
+  const double* pp[3] = {c->phipart[0], c->phipart[1], c->phipart[2]};
+  double* vel[3] = {vx, vy, vz};
+  launch_downward_pass(c->dt, 3, c->phiproxy, pp, vel, c->stream, true);
+  return launched(c);
 }
 
 // Poisson (kind 0) or Biot-Savart (kind 1) FMM at arbitrary targets, FP64:

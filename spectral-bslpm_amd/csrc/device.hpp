@@ -39,6 +39,17 @@ struct DevTree {
   int64_t U16;    // padded record count of the MFMA kernels
 };
 
+// Pair kernels run one wave per target tile, four waves per workgroup.
+constexpr int kWavesPerBlock = 4;
+
+// Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
+// range of logical workgroups (neighbouring target tiles share sources, which
+// then stay in one XCD's L2).
+__device__ inline int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, y = b / 8;
+  return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
+}
+
 // 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps
 // (each step squares the relative error; the estimate is good to ~2^-26), so
 // the result is within an ulp of the correctly rounded quotient.
@@ -60,6 +71,13 @@ __device__ inline int locate_leaf(const DevTree& t, double x, double y, double z
 // position and its charge pre-multiplied by -1/(4 pi).
 struct SrcRec64 {
   double x, y, z, q;
+};
+
+// Source record of the FP64 shallow-water pair kernels: displaced, re-projected
+// position with its vorticity charge x -1/(4 pi), the undisplaced node and the
+// divergence charge x -+1/(4 pi) (variant 1 / 2).
+struct SrcRecSwe {
+  double x, y, z, qz, x0, y0, z0, qd;
 };
 
 // Source record of the mixed-precision pair kernels: position relative to the
@@ -124,6 +142,14 @@ void launch_source_records_mfma(const DevTree& t, const double* q_part, const do
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
                               hipStream_t s, bool assign = false);
+
+// ---- shallow-water pair sums (swe.hip); variant 1 or 2
+void launch_source_records_swe(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
+                               const double* qd_part, const double* qd_proxy, const double* const* disp_part,
+                               const double* const* disp_proxy, SrcRecSwe* rec, hipStream_t s);
+void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                           const SrcRecSwe* rec, double eps, double* const* out_part, double* const* out_proxy,
+                           hipStream_t s, bool assign = false);
 
 // ---- evaluation at arbitrary targets (targets.hip)
 // kind 0: log kernel (psi), kind 1: Biot-Savart (3 components).  far[f] are the

@@ -28,15 +28,6 @@ struct CPtr3 {
   const double* p[3];
 };
 
-constexpr int kWavesPerBlock = 4;
-
-// Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
-// range of logical workgroups.
-__device__ inline int xcd_remap(int b, int nb) {
-  const int per = nb / 8, rem = nb % 8, x = b % 8, y = b / 8;
-  return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
-}
-
 // Target-major ownership: every target point is written by exactly one lane,
 // so `assign` may replace the accumulate of the reference functors.
 __device__ inline void store_u(const DevTree& t, int64_t u, Ptr3 part, Ptr3 proxy, double a, double b, double c,

@@ -1,0 +1,143 @@
+// swe.hip -- shallow-water velocity pair sums (fmm_interactions/swe_vel.cpp:9-67,
+// swe_vel_impl.hpp:12-147), FP64, on the same target-major tiles as the
+// Biot-Savart kernels.
+//
+// Per pair, with x the target, y = P(node + disp) the displaced source and
+// y0 the undisplaced node:
+//   variant 1 (swe_vel_panel_interaction):
+//     u += g qz (x cross y) + g qd (y - x (x.y)),      g = -1/(4 pi (1 - x.y + eps))
+//   variant 2 (swe_vel_panel_interaction_2):
+//     u += g qz (x cross y) + g0 qd (y0 - x (x.y0)),   g0 = +1/(4 pi (1 - x.y0 + eps))
+// The kernel accumulates W = sum g qz y, D = sum gd y', S = sum gd (x.y')
+// (y' = y or y0) and forms u = x cross W + D - x S once per target, the same
+// sums regrouped.
+#include "device.hpp"
+
+namespace sbslpm {
+namespace {
+
+struct CPtr3 {
+  const double* p[3];
+};
+struct Ptr3 {
+  double* p[3];
+};
+
+// Source records: displaced re-projected position, undisplaced node, and the
+// charges pre-multiplied by the variant's constants.
+__global__ void k_src_swe(DevTree t, const double* __restrict__ qz_part, const double* __restrict__ qz_proxy,
+                          const double* __restrict__ qd_part, const double* __restrict__ qd_proxy, CPtr3 dpart,
+                          CPtr3 dprox, double qd_scale, SrcRecSwe* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  const double x0 = t.px[u], y0 = t.py[u], z0 = t.pz[u];
+  double x = x0, y = y0, z = z0, qz, qd;
+  if (u < t.npart) {
+    if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
+    qz = qz_part[u];
+    qd = qd_part[u];
+  } else {
+    const int64_t v = u - t.npart;
+    if (v / t.n >= t.offset) return;  // leaf proxy rows are never sources
+    if (dprox.p[0]) { x += dprox.p[0][v]; y += dprox.p[1][v]; z += dprox.p[2][v]; }
+    qz = qz_proxy[v];
+    qd = qd_proxy[v];
+  }
+  project_to_sphere(x, y, z);
+  rec[u] = SrcRecSwe{x, y, z, kNegInv4Pi * qz, x0, y0, z0, qd_scale * qd};
+}
+
+template <int VARIANT>
+__global__ __launch_bounds__(256) void k_swe64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                               const SourceSet* __restrict__ sets, const SrcRecSwe* __restrict__ rec,
+                                               double eps, Ptr3 part, Ptr3 proxy, int assign) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  double w0 = 0, w1 = 0, w2 = 0, d0 = 0, d1 = 0, d2 = 0, sd = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const SrcRecSwe* __restrict__ r = rec + S.first;
+#pragma unroll 2
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRecSwe s = r[k];
+      const double dp = x * s.x + y * s.y + z * s.z;
+      const double g = rcp64((1.0 - dp) + eps);
+      const double gz = g * s.qz;
+      w0 = fma(gz, s.x, w0);
+      w1 = fma(gz, s.y, w1);
+      w2 = fma(gz, s.z, w2);
+      if (VARIANT == 1) {
+        const double gd = g * s.qd;
+        d0 = fma(gd, s.x, d0);
+        d1 = fma(gd, s.y, d1);
+        d2 = fma(gd, s.z, d2);
+        sd = fma(gd, dp, sd);
+      } else {
+        const double dp0 = x * s.x0 + y * s.y0 + z * s.z0;
+        const double gd = rcp64((1.0 - dp0) + eps) * s.qd;
+        d0 = fma(gd, s.x0, d0);
+        d1 = fma(gd, s.y0, d1);
+        d2 = fma(gd, s.z0, d2);
+        sd = fma(gd, dp0, sd);
+      }
+    }
+  }
+  if (lane >= T.count) return;
+  const double a = (y * w2 - z * w1) + (d0 - x * sd);
+  const double b = (z * w0 - x * w2) + (d1 - y * sd);
+  const double c = (x * w1 - y * w0) + (d2 - z * sd);
+  double* o[3];
+  int64_t v = u;
+  if (u < t.npart) {
+    o[0] = part.p[0]; o[1] = part.p[1]; o[2] = part.p[2];
+  } else {
+    v = u - t.npart;
+    o[0] = proxy.p[0]; o[1] = proxy.p[1]; o[2] = proxy.p[2];
+  }
+  if (assign) {
+    o[0][v] = a; o[1][v] = b; o[2][v] = c;
+  } else {
+    o[0][v] += a; o[1][v] += b; o[2][v] += c;
+  }
+}
+
+}  // namespace
+
+void launch_source_records_swe(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
+                               const double* qd_part, const double* qd_proxy, const double* const* disp_part,
+                               const double* const* disp_proxy, SrcRecSwe* rec, hipStream_t s) {
+  CPtr3 dp{}, dx{};
+  for (int f = 0; f < 3; ++f) {
+    dp.p[f] = disp_part ? disp_part[f] : nullptr;
+    dx.p[f] = disp_proxy ? disp_proxy[f] : nullptr;
+  }
+  // variant 1: g qd with g = -1/(4 pi ...); variant 2: +1/(4 pi ...)
+  const double qd_scale = variant == 1 ? kNegInv4Pi : -kNegInv4Pi;
+  hipLaunchKernelGGL(k_src_swe, dim3((unsigned)((t.U + 255) / 256)), dim3(256), 0, s, t, qz_part, qz_proxy, qd_part,
+                     qd_proxy, dp, dx, qd_scale, rec);
+}
+
+void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                           const SrcRecSwe* rec, double eps, double* const* out_part, double* const* out_proxy,
+                           hipStream_t s, bool assign) {
+  if (ntiles == 0) return;
+  Ptr3 op{}, ox{};
+  for (int f = 0; f < 3; ++f) {
+    op.p[f] = out_part[f];
+    ox.p[f] = out_proxy[f];
+  }
+  const unsigned blocks = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
+  if (variant == 1)
+    hipLaunchKernelGGL(k_swe64<1>, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec, eps, op, ox,
+                       assign ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_swe64<2>, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec, eps, op, ox,
+                       assign ? 1 : 0);
+}
+
+}  // namespace sbslpm
