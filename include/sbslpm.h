@@ -221,6 +221,23 @@ int sbslpm_bve_find_departure_points(sbslpm_ctx* ctx, const double* x, const dou
 int sbslpm_bve_departure_to_target(sbslpm_ctx* ctx, const double* z, const double* dep_x, const double* dep_y,
                                    const double* dep_z, const double* vor, const double* tracers, int ntr,
                                    double* new_vor, double* new_tracers);
+/* swe_displacement_upward_pass (swe_time_step.cpp:265-276): disp_part = tau v [Pl][m2]
+ * (swe_disp_update, :194-210); disp_proxy [P][n] non-leaf rows interpolated from the leaf that
+ * contains each undisplaced node (swe_disp_interp, :212-263). */
+int sbslpm_swe_displacement(sbslpm_ctx* ctx, const double* vel_x, const double* vel_y, const double* vel_z,
+                            double tau, double* disp_part_x, double* disp_part_y, double* disp_part_z,
+                            double* disp_proxy_x, double* disp_proxy_y, double* disp_proxy_z);
+/* swe_find_departure_points (swe_time_step.cpp:422-556): the backward RK4 search of
+ * bve_find_departure_points, same arguments. */
+int sbslpm_swe_find_departure_points(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                     const double* const* vels, double dt, double* dep_x, double* dep_y,
+                                     double* dep_z);
+/* swe_departure_to_target (swe_time_step.cpp:19-74): vor, div, h [Pl][m2] and tracers [Pl][m2][ntr]
+ * interpolated at the departure points (no Coriolis terms); outputs assigned. */
+int sbslpm_swe_departure_to_target(sbslpm_ctx* ctx, const double* dep_x, const double* dep_y, const double* dep_z,
+                                   const double* vor, const double* div, const double* h, const double* tracers,
+                                   int ntr, double* new_vor, double* new_div, double* new_h, double* new_tracers);
+
 /* bve_back_rk4_step (bve_time_step.cpp:311-419), no forcing (bve_forcing copy,
  * forcing_funcs.cpp:93-95).  vor [Pl][m2] and tracers [Pl][m2][tracer_count] updated in place;
  * vel_{x,y,z} receive the stage-0 velocity; proxy_source_vors [P][n] is scratch (may be NULL).

@@ -81,6 +81,9 @@ def lib():
         L.orc_bve_back_rk4_step.restype = ctypes.c_int
         L.orc_swe_vel_interactions.argtypes = [ctypes.c_void_p, ctypes.c_int, _ppd, _ppd, _dp, _dp, _dp, _dp,
                                                ctypes.c_double, _ppd, _ppd]
+        L.orc_swe_departure_to_target.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _dp, ctypes.c_int, _ppd,
+                                                  _dp]
+        L.orc_swe_departure_to_target.restype = ctypes.c_int
         L.orc_eval_targets.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int64,
                                        _dp, _dp, _dp, _ppd]
         L.orc_eval_targets.restype = ctypes.c_int
@@ -347,6 +350,18 @@ class OracleFMM:
         if err:
             raise RuntimeError("oracle departure_to_target: geometry error")
         return nv, (ntrac if ntr else None)
+
+    def swe_departure_to_target(self, dep, fields, tracers=None):
+        """swe_departure_to_target: fields = (vor, div, h); returns (3 new fields, new tracers)."""
+        ntr = 0 if tracers is None else tracers.shape[-1]
+        tr = np.zeros((self.Pl, self.m2, max(ntr, 1))) if tracers is None else tracers
+        out = [np.zeros((self.Pl, self.m2)) for _ in range(3)]
+        ntrac = np.zeros_like(tr)
+        err = lib().orc_swe_departure_to_target(self.h, *[_p(d) for d in dep], _pp(list(fields)), _p(tr), ntr,
+                                                _pp(out), _p(ntrac))
+        if err:
+            raise RuntimeError("oracle swe_departure_to_target: geometry error")
+        return out, (ntrac if ntr else None)
 
     def bve_back_rk4_step(self, x, y, z, area, vor, dt, omega, eps, combine=0, tracers=None):
         """Returns (new vorticity, stage-0 velocity, new tracers)."""

@@ -1072,6 +1072,32 @@ int orc_bve_departure_to_target(void* h, const double* z, const double* dx, cons
   return err ? ORC_E_GEOMETRY : ORC_OK;
 }
 
+// swe_time_step.cpp:19-74 (swe_departure_to_target): vor, div, h [Pl][m2] and
+// tracers [Pl][m2][T] interpolated at the departure points, no Coriolis terms.
+int orc_swe_departure_to_target(void* h, const double* dx, const double* dy, const double* dz,
+                                const double* const* fields, const double* tracers, int ntr,
+                                double* const* new_fields, double* new_tracers) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+  size_t N = (size_t)c.Pl * c.m2;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : err)
+  for (size_t k = 0; k < N; k++) {
+    double xieta[2], basis[121];
+    xieta_from_xyz(dx[k], dy[k], dz[k], xieta);
+    int leaf = point_locate_panel(c, dx[k], dy[k], dz[k]);
+    for (int f = 0; f < 3; f++) new_fields[f][k] = 0;
+    for (int t = 0; t < ntr; t++) new_tracers[k * ntr + t] = 0;
+    if (leaf < 0) { err |= ORC_E_GEOMETRY; continue; }
+    err |= bli_in_panel(c.pn[leaf], xieta[0], xieta[1], c.md, basis);
+    size_t b = (size_t)(leaf - c.offset) * c.m2;
+    for (int j = 0; j < c.m2; j++) {
+      for (int f = 0; f < 3; f++) new_fields[f][k] += fields[f][b + j] * basis[j];
+      for (int t = 0; t < ntr; t++) new_tracers[k * ntr + t] += tracers[(b + j) * ntr + t] * basis[j];
+    }
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
 // One FMM velocity evaluation: upward pass, interactions, downward pass
 // (bve_time_step.cpp:321-344).  vel[3][Pl][m2] assigned.
 static int fmm_velocity(Ctx& c, const double* area, const double* vor_eff, const double* const* disp_part,

@@ -75,6 +75,9 @@ SIGNATURES = {
     "sbslpm_swe_vel_interactions": (_I, [_V] * 17),
     "sbslpm_swe_vel_interactions_2": (_I, [_V] * 17),
     "sbslpm_swe_velocity": (_I, [_V, _I] + [_V] * 12),
+    "sbslpm_swe_displacement": (_I, [_V, _V, _V, _V, ctypes.c_double] + [_V] * 6),
+    "sbslpm_swe_find_departure_points": (_I, [_V, _V, _V, _V, _V, ctypes.c_double, _V, _V, _V]),
+    "sbslpm_swe_departure_to_target": (_I, [_V] * 8 + [_I] + [_V] * 4),
     "sbslpm_poisson_solve_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V]),
     "sbslpm_bve_velocity_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
     "sbslpm_direct_sum_inv_lap": (_I, [_V, _I64, _V, _V, _V, _V, _V, _V, _I64, _I64]),

@@ -361,6 +361,35 @@ class Context:
                     "departure_to_target")
         return new_vor, new_tracers
 
+    def swe_displacement(self, vel, tau, disp_part=None, disp_proxy=None):
+        """swe_displacement_upward_pass, swe_time_step.cpp:265-276."""
+        disp_part = self.zeros_part(3) if disp_part is None else disp_part
+        disp_proxy = self.zeros_proxy(3) if disp_proxy is None else disp_proxy
+        self._check(self._lib.sbslpm_swe_displacement(self.h, *[_ptr(v) for v in vel], float(tau),
+                                                      *[_ptr(v) for v in list(disp_part) + list(disp_proxy)]),
+                    "swe_displacement")
+        return disp_part, disp_proxy
+
+    def swe_find_departure_points(self, x, y, z, vels, dt, dep=None):
+        """swe_find_departure_points, swe_time_step.cpp:422-556."""
+        dep = self.zeros_part(3) if dep is None else dep
+        flat = [t for stage in vels for t in stage]
+        arr = (ctypes.c_void_p * 12)(*[t.data_ptr() for t in flat])
+        self._check(self._lib.sbslpm_swe_find_departure_points(self.h, _ptr(x), _ptr(y), _ptr(z),
+                                                               ctypes.cast(arr, ctypes.c_void_p), float(dt),
+                                                               *[_ptr(t) for t in dep]), "swe_departure_points")
+        return dep
+
+    def swe_departure_to_target(self, dep, vor, div, h, tracers=None):
+        """swe_departure_to_target, swe_time_step.cpp:19-74; returns (new_vor, new_div, new_h, new_tracers)."""
+        out = [self.zeros_part() for _ in range(3)]
+        ntr = 0 if tracers is None else tracers.shape[-1]
+        new_tracers = self.torch.zeros_like(tracers) if ntr else None
+        self._check(self._lib.sbslpm_swe_departure_to_target(self.h, *[_ptr(t) for t in dep], _ptr(vor), _ptr(div),
+                                                             _ptr(h), _ptr(tracers), ntr, *[_ptr(o) for o in out],
+                                                             _ptr(new_tracers)), "swe_departure_to_target")
+        return out[0], out[1], out[2], new_tracers
+
     def bve_back_rk4_step(self, x, y, z, area, vors, vel, tracers=None, time=0.0, proxy_source_vors=None):
         """bve_back_rk4_step, bve_time_step.cpp:311-419: vors/tracers updated in place, vel <- stage-0 velocity."""
         self._check(self._lib.sbslpm_bve_back_rk4_step(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vors),

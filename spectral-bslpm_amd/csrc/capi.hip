@@ -736,6 +736,40 @@ int sbslpm_bve_departure_to_target(sbslpm_ctx* c, const double* z, const double*
   return launched(c);
 }
 
+int sbslpm_swe_displacement(sbslpm_ctx* c, const double* vx, const double* vy, const double* vz, double tau,
+                            double* dpx, double* dpy, double* dpz, double* dxx, double* dxy, double* dxz) {
+  if (!c || !vx || !vy || !vz || !dpx || !dpy || !dpz || !dxx || !dxy || !dxz)
+    return fail(c, SBSLPM_E_ARG, "swe_displacement: null pointer");
+  const double* v[3] = {vx, vy, vz};
+  double* dp[3] = {dpx, dpy, dpz};
+  double* dxp[3] = {dxx, dxy, dxz};
+  launch_bve_displacement(c->dt, nullptr, nullptr, nullptr, v, nullptr, tau, 0.0, nullptr, dp, dxp, c->stream);
+  return launched(c);
+}
+
+int sbslpm_swe_find_departure_points(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                     const double* const* vels, double dt, double* dep_x, double* dep_y,
+                                     double* dep_z) {
+  if (!c || !x || !y || !z || !vels || !dep_x || !dep_y || !dep_z)
+    return fail(c, SBSLPM_E_ARG, "swe_find_departure_points: null pointer");
+  for (int i = 0; i < 12; ++i)
+    if (!vels[i]) return fail(c, SBSLPM_E_ARG, "swe_find_departure_points: null velocity");
+  launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, dep_x, dep_y, dep_z, c->stream);
+  return launched(c);
+}
+
+int sbslpm_swe_departure_to_target(sbslpm_ctx* c, const double* dep_x, const double* dep_y, const double* dep_z,
+                                   const double* vor, const double* div, const double* h, const double* tracers,
+                                   int ntr, double* new_vor, double* new_div, double* new_h, double* new_tracers) {
+  if (!c || !dep_x || !dep_y || !dep_z || !vor || !div || !h || !new_vor || !new_div || !new_h || ntr < 0 ||
+      (ntr > 0 && (!tracers || !new_tracers)))
+    return fail(c, SBSLPM_E_ARG, "swe_departure_to_target: bad argument");
+  const double* f[3] = {vor, div, h};
+  double* nf[3] = {new_vor, new_div, new_h};
+  launch_swe_departure_to_target(c->dt, dep_x, dep_y, dep_z, f, tracers, ntr, nf, new_tracers, c->stream);
+  return launched(c);
+}
+
 int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                              double* vor, double* proxy_source_vors, double* vx, double* vy, double* vz,
                              double* tracers, double time) {

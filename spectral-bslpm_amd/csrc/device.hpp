@@ -169,6 +169,10 @@ void launch_bve_departure(const DevTree& t, const double* x, const double* y, co
 void launch_bve_departure_to_target(const DevTree& t, const double* z, const double* dx, const double* dy,
                                     const double* dz, const double* vor, const double* tracers, int ntr,
                                     double omega, double* new_vor, double* new_tracers, hipStream_t s);
+// swe_departure_to_target: fields = {vor, div, h} [Pl][m2], no Coriolis terms.
+void launch_swe_departure_to_target(const DevTree& t, const double* dx, const double* dy, const double* dz,
+                                    const double* const* fields, const double* tracers, int ntr,
+                                    double* const* new_fields, double* new_tracers, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
 
 // ---- direct sums (direct.hip)

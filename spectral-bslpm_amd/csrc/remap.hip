@@ -95,9 +95,11 @@ __global__ void k_disp_update(DevTree t, const double* __restrict__ x, const dou
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.npart) return;
   const double vx = vel.p[0][u], vy = vel.p[1][u], vz = vel.p[2][u];
-  double a = x[u] + vx * tau, b = y[u] + vy * tau, c = z[u] + vz * tau;
-  project_to_sphere(a, b, c);
-  new_vor[u] = vor[u] + 2 * omega * z[u] - 2 * omega * c;
+  if (new_vor) {  // null for the shallow-water displacement (swe_disp_update, swe_time_step.cpp:194-210)
+    double a = x[u] + vx * tau, b = y[u] + vy * tau, c = z[u] + vz * tau;
+    project_to_sphere(a, b, c);
+    new_vor[u] = vor[u] + 2 * omega * z[u] - 2 * omega * c;
+  }
   disp.p[0][u] = tau * vx;
   disp.p[1][u] = tau * vy;
   disp.p[2][u] = tau * vz;
@@ -183,12 +185,20 @@ __global__ __launch_bounds__(128) void k_departure(DevTree t, const double* __re
   dx[u] = a; dy[u] = b; dz[u] = c;
 }
 
-// bve_departure_to_target, bve_time_step.cpp:38-66.  Tracers [Pl][m2][T].
-template <int M>
+// bve_departure_to_target (bve_time_step.cpp:38-66; NF = 1, Omega terms) and
+// swe_departure_to_target (swe_time_step.cpp:47-74; NF = 3 fields vor, div, h,
+// no Omega terms): fields and tracers [Pl][m2][T] interpolated at the
+// departure points.
+struct Fields3 {
+  const double* in[3];
+  double* out[3];
+};
+
+template <int M, int NF>
 __global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* __restrict__ z, const double* __restrict__ dx,
-                                const double* __restrict__ dy, const double* __restrict__ dz,
-                                const double* __restrict__ vor, const double* __restrict__ tracers, int ntr,
-                                double omega, double* __restrict__ new_vor, double* __restrict__ new_tracers) {
+                                const double* __restrict__ dy, const double* __restrict__ dz, Fields3 f,
+                                const double* __restrict__ tracers, int ntr, double omega, int coriolis,
+                                double* __restrict__ new_tracers) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.npart) return;
   double xe[2];
@@ -196,7 +206,8 @@ __global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* 
   const int leaf = locate_leaf(t, X, Y, Z, xe);
   if (leaf < 0) {
     flag(t, kErrLocate);
-    new_vor[u] = 0;
+#pragma unroll
+    for (int q = 0; q < NF; ++q) f.out[q][u] = 0;
     for (int k = 0; k < ntr; ++k) new_tracers[u * ntr + k] = 0;
     return;
   }
@@ -206,9 +217,15 @@ __global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* 
   ok = bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we) && ok;
   if (!ok) flag(t, kErrBliBounds);
   const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
-  double v = tensor_gather<M>(vor + base, wx, we);
-  v += 2 * omega * Z;
-  new_vor[u] = v - 2 * omega * z[u];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    double v = tensor_gather<M>(f.in[q] + base, wx, we);
+    if (coriolis) {
+      v += 2 * omega * Z;
+      v -= 2 * omega * z[u];
+    }
+    f.out[q][u] = v;
+  }
   for (int k = 0; k < ntr; ++k) {
     double s = 0;
 #pragma unroll
@@ -259,11 +276,14 @@ struct Departure {
 
 template <int M>
 struct DepToTarget {
-  static void run(const DevTree& t, const double* z, const double* dx, const double* dy, const double* dz,
-                  const double* vor, const double* tr, int ntr, double omega, double* nv, double* ntrc,
-                  hipStream_t s) {
-    hipLaunchKernelGGL(k_dep_to_target<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, z, dx, dy, dz, vor,
-                       tr, ntr, omega, nv, ntrc);
+  static void run(const DevTree& t, int nf, const double* z, const double* dx, const double* dy, const double* dz,
+                  Fields3 f, const double* tr, int ntr, double omega, int coriolis, double* ntrc, hipStream_t s) {
+    if (nf == 1)
+      hipLaunchKernelGGL((k_dep_to_target<M, 1>), dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, z, dx, dy, dz, f,
+                         tr, ntr, omega, coriolis, ntrc);
+    else
+      hipLaunchKernelGGL((k_dep_to_target<M, 3>), dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, z, dx, dy, dz, f,
+                         tr, ntr, omega, coriolis, ntrc);
   }
 };
 
@@ -292,7 +312,15 @@ void launch_bve_departure(const DevTree& t, const double* x, const double* y, co
 void launch_bve_departure_to_target(const DevTree& t, const double* z, const double* dx, const double* dy,
                                     const double* dz, const double* vor, const double* tracers, int ntr,
                                     double omega, double* new_vor, double* new_tracers, hipStream_t s) {
-  dispatch_m<DepToTarget>(t.m, t, z, dx, dy, dz, vor, tracers, ntr, omega, new_vor, new_tracers, s);
+  Fields3 f{{vor, nullptr, nullptr}, {new_vor, nullptr, nullptr}};
+  dispatch_m<DepToTarget>(t.m, t, 1, z, dx, dy, dz, f, tracers, ntr, omega, 1, new_tracers, s);
+}
+
+void launch_swe_departure_to_target(const DevTree& t, const double* dx, const double* dy, const double* dz,
+                                    const double* const* fields, const double* tracers, int ntr,
+                                    double* const* new_fields, double* new_tracers, hipStream_t s) {
+  Fields3 f{{fields[0], fields[1], fields[2]}, {new_fields[0], new_fields[1], new_fields[2]}};
+  dispatch_m<DepToTarget>(t.m, t, 3, nullptr, dx, dy, dz, f, tracers, ntr, 0.0, 0, new_tracers, s);
 }
 
 }  // namespace sbslpm

@@ -64,3 +64,35 @@ def test_swe_velocity(case, variant):
     err = case["sb"].workloads.vec_rel_l2([to_np(g) for g in got], expect)
     print(f"  variant {variant}: velocity rel-L2 {err:.2e}")
     assert err < 1e-12
+
+
+def test_swe_displacement_and_remap(case):
+    """swe_displacement_upward_pass (swe_time_step.cpp:265-276), swe_find_departure_points
+    (:422-556) and swe_departure_to_target (:19-74) against the oracle."""
+    ctx, o, t = case["ctx"], case["o"], case["t"]
+    W = case["sb"].workloads
+    x, y, z = case["x"], case["y"], case["z"]
+    vel = W.rh4_velocity(x, y, z)
+    gvel = [to_gpu(t, v) for v in vel]
+    gdp, gdx = ctx.swe_displacement(gvel, 1800.0)
+    ctx.synchronize()
+    _, dp, dx = o.bve_displacement(x, y, z, vel, np.zeros_like(x), 1800.0, 0.0)
+    for f in range(3):
+        np.testing.assert_array_equal(to_np(gdp[f]), dp[f])
+        np.testing.assert_allclose(to_np(gdx[f])[: o.offset], dx[f][: o.offset], rtol=0, atol=1e-16)
+    stages = [vel, [1.1 * v for v in vel], [0.9 * v for v in vel], [1.05 * v for v in vel]]
+    gst = [[to_gpu(t, v) for v in s] for s in stages]
+    gx, gy, gz = [to_gpu(t, v) for v in (x, y, z)]
+    gdep = ctx.swe_find_departure_points(gx, gy, gz, gst, 3600.0)
+    ctx.synchronize()
+    dep = o.bve_find_departure_points(x, y, z, stages, 3600.0, 0)
+    for f in range(3):
+        np.testing.assert_allclose(to_np(gdep[f]), dep[f], rtol=0, atol=1e-14)
+    fields = [case["vor"], case["div"], 1.0 + 0.1 * z]
+    tr = np.stack([x * y, z], -1).copy()
+    got = ctx.swe_departure_to_target(gdep, *[to_gpu(t, v) for v in fields], tracers=to_gpu(t, tr))
+    ctx.synchronize()
+    exp, etr = o.swe_departure_to_target([to_np(g) for g in gdep], fields, tr)
+    for g, e in zip(got[:3], exp):
+        assert W.rel_l2(to_np(g), e) < 1e-13
+    assert W.rel_l2(to_np(got[3]), etr) < 1e-13

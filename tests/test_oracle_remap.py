@@ -111,3 +111,26 @@ def test_step_keeps_zonal_l1_field(orc, setup, combine):
     err = np.sqrt(sum(np.sum((v - e) ** 2) for v, e in zip(vel, expect)) / sum(np.sum(e ** 2) for e in expect[:2]))
     assert err < 2e-2
     assert np.max(np.abs(nv - vor)) < 2e-2 * c
+
+
+def test_swe_remap_identity_and_rotation(orc, setup):
+    """swe_departure_to_target (swe_time_step.cpp:19-74): identity at the nodes,
+    and under a small solid-body rotation each field is interpolated to the
+    accuracy of the degree-(m-1) grid (smooth fields, no Coriolis terms)."""
+    o, x, y, z, a = setup
+    f = [np.sin(3 * x) * np.cos(2 * z) + y, x * z - 0.3, np.cos(x + 2 * y)]
+    tr = np.stack([x * y, z], -1).copy()
+    out, ntr = o.swe_departure_to_target([x, y, z], f, tr)
+    for g, h in zip(out, f):
+        np.testing.assert_allclose(g, h, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(ntr, tr, rtol=0, atol=1e-15)
+    d = _rot(x, y, z, [0.2, -0.4, 0.9], 0.01)
+    dep = [d[..., i].copy() for i in range(3)]
+    out, _ = o.swe_departure_to_target(dep, f)
+    exact = [np.sin(3 * dep[0]) * np.cos(2 * dep[2]) + dep[1], dep[0] * dep[2] - 0.3, np.cos(dep[0] + 2 * dep[1])]
+    for g, h in zip(out, exact):
+        assert np.max(np.abs(g - h)) < 1e-5
+    # with no rotation rate the BVE remap of each field is the same interpolation
+    for g, h in zip(out, f):
+        nv, _ = o.bve_departure_to_target(z, dep, h, 0.0)
+        np.testing.assert_allclose(g, nv, rtol=0, atol=1e-15)
