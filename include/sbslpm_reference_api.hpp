@@ -76,6 +76,21 @@ inline void poisson_fmm_interactions(sbslpm_ctx* ctx, View2<double>& target_pots
         "poisson_fmm_interactions");
 }
 
+// fmm_interactions/swe_vel.hpp:11-23 (swe_vel_interactions, swe_vel_interactions_2)
+inline void swe_vel_interactions(sbslpm_ctx* ctx, View2<double>& disp_x, View2<double>& disp_y, View2<double>& disp_z,
+                                 View2<double>& target_pots_1, View2<double>& target_pots_2,
+                                 View2<double>& target_pots_3, View2<double>& source_vals_vor,
+                                 View2<double>& source_vals_div, int variant = 1) {
+  Layout L = layout(ctx);
+  auto fn = variant == 1 ? sbslpm_swe_vel_interactions : sbslpm_swe_vel_interactions_2;
+  check(ctx,
+        fn(ctx, disp_x.row(L.offset), disp_y.row(L.offset), disp_z.row(L.offset), disp_x.data(), disp_y.data(),
+           disp_z.data(), source_vals_vor.row(L.offset), source_vals_vor.data(), source_vals_div.row(L.offset),
+           source_vals_div.data(), target_pots_1.row(L.offset), target_pots_2.row(L.offset),
+           target_pots_3.row(L.offset), target_pots_1.data(), target_pots_2.data(), target_pots_3.data()),
+        "swe_vel_interactions");
+}
+
 // fmm_funcs.hpp:29-31 (the leaf rows already hold the leaf-target sums)
 inline void downward_pass(sbslpm_ctx* ctx, View2<double>& proxy_target_pots, View2<double>& sols) {
   check(ctx, sbslpm_downward_pass(ctx, proxy_target_pots.data(), nullptr, sols.data()), "downward_pass");
