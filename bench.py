@@ -126,12 +126,112 @@ def cpu_baseline(cfg, x, y, z, a, vor, budget_s):
                 seconds_per_step_extrapolated=step)
 
 
+ENSEMBLE = [  # (particles per face side, levels, particle-grid side m); proxy degree 8, theta 0.7
+    (96, 5, 6), (128, 5, 8), (192, 6, 6)]
+
+
+def bench_ensemble(args, torch, sb):
+    """Ensemble of 48 BVE members, 16 at each of 6x96^2, 6x128^2 and 6x192^2:
+    one context (tree, interaction list, tiles) per resolution, each step of
+    its 16 members in shared kernel launches, captured in a hipGraph.  Member
+    fields: the 12-vortex field plus seeded spherical-harmonic noise (l <= 24,
+    1e-3 of the field RMS).  Reports member-steps/s over all 48 members and
+    the speed-up over stepping the members one at a time."""
+    W = sb.workloads
+    nmem = 16
+    stream = torch.cuda.Stream()
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    groups = []
+    for side, levels, m in ENSEMBLE:
+        kw = dict(levels=levels, interp_degree=8, leaf_degree=m - 1, fmm_theta=0.7, weights="clenshaw_curtis",
+                  kernel_eps=1e-4, delta_t=3600.0, omega=7.292e-5, rk4_combine="reference",
+                  use_graph=not args.no_graph, precision="mixed")
+        with torch.cuda.stream(stream):
+            ctx = sb.Context(sb.RunConfig(**kw))
+        x, y, z, a = ctx.spec_points()
+        base = W.gaussian_vortices(x, y, z, a, count=12, seed=0)
+        rms = float(np.sqrt(np.mean(base ** 2)))
+        mem = []
+        for j in range(nmem):
+            noise = W.random_sh_forcing(x, y, z, lmax=24, seed=100 + j)[0]
+            mem.append(base + 1e-3 * rms * noise / float(np.sqrt(np.mean(noise ** 2))))
+        vors = dev(np.stack(mem))
+        g = dict(side=side, ctx=ctx, kw=kw, xyz=[dev(v) for v in (x, y, z, a)], vors0=vors,
+                 vors=vors.clone(), vel=[torch.zeros_like(vors) for _ in range(3)])
+        groups.append(g)
+    torch.cuda.synchronize()
+
+    def run(steps):
+        with torch.cuda.stream(stream):
+            for _ in range(steps):
+                for g in groups:
+                    g["ctx"].bve_back_rk4_step_ensemble(*g["xyz"], g["vors"], g["vel"])
+    run(args.warmup)
+    torch.cuda.synchronize()
+    for g in groups:
+        g["vors"].copy_(g["vors0"])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+    run(args.steps)
+    with torch.cuda.stream(stream):
+        e1.record(stream)
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3
+    total_members = nmem * len(groups)
+    value = total_members * args.steps / sec
+
+    # the same members stepped one at a time (same contexts, single-member entry point)
+    one = []
+    for g in groups:
+        v1 = g["vors0"][0].clone()
+        vel1 = g["ctx"].zeros_part(3)
+        with torch.cuda.stream(stream):
+            g["ctx"].bve_back_rk4_step(*g["xyz"], v1, vel1)
+        torch.cuda.synchronize()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            f0.record(stream)
+            for _ in range(args.steps):
+                g["ctx"].bve_back_rk4_step(*g["xyz"], v1, vel1)
+            f1.record(stream)
+        torch.cuda.synchronize()
+        one.append(f0.elapsed_time(f1) / 1e3 / args.steps)
+    seq = nmem * sum(one)  # seconds per step of all 48 members, one member at a time
+    # accuracy: member 0 of each resolution after the timed steps against an FP64 single-member run
+    acc = {}
+    for g in groups:
+        kw = dict(g["kw"], precision="fp64", use_graph=False)
+        c64 = sb.Context(sb.RunConfig(**kw))
+        v64 = g["vors0"][0].clone()
+        vel64 = c64.zeros_part(3)
+        for _ in range(args.steps):
+            c64.bve_back_rk4_step(*g["xyz"], v64, vel64)
+        c64.synchronize()
+        acc[f"6x{g['side']}^2"] = float(f"{W.rel_l2(g['vors'][0].cpu().numpy(), v64.cpu().numpy()):.3e}")
+        del c64
+    line = {"metric": "BVE ensemble member-steps/sec, 48 members at 6x96^2, 6x128^2, 6x192^2",
+            "value": round(value, 3), "unit": "member-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * sec / args.steps, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair kernels, f64 accumulation/trajectories",
+            "data": "synthetic (12 seeded Gaussian vortices + seeded SH noise per member)",
+            "config": {"workload": "BVE ensemble (c4): 3 x 16 members", "members": total_members,
+                       "resolutions": [f"6x{s}^2 (L={l}, m={m}, d=8)" for s, l, m in ENSEMBLE],
+                       "graph": not args.no_graph},
+            "one_member_at_a_time_ms_per_step": round(1e3 * seq, 3),
+            "speedup_vs_one_member_at_a_time": round(seq / (sec / args.steps), 3),
+            "member0_vor_rel_l2_vs_fp64": acc}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4"],
+                    help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c4: the 48-member ensemble")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--valu", action="store_true", help="also time the mixed-precision vector-unit pair kernel")
@@ -145,6 +245,9 @@ def main():
 
     import torch
     sb = importlib.import_module("spectral-bslpm_amd")
+    if args.config == "c4":
+        bench_ensemble(args, torch, sb)
+        return
     W = sb.workloads
     cfgd = CONFIGS[args.config]
     stream = torch.cuda.Stream()
@@ -220,6 +323,16 @@ def main():
         roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                           frac=round(achieved / peak, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
                           pairs_per_launch=pairs, flop_per_pair=FLOP_BS)
+    # HBM bytes per launch of the pair kernels, from the committed FETCH_SIZE / WRITE_SIZE
+    # runs of this workload (scaled by the copy-kernel factors recorded with them)
+    tfile = os.path.join(REPO, "profiles", "bve_step_c3_hbm_traffic.json")
+    if args.config == "c3" and os.path.exists(tfile):
+        with open(tfile) as fh:
+            tr = json.load(fh)["kernels"]
+        for name, kern in (("bve_fp64", "k_bve64"), ("bve_mixed", "k_bve_mfma")):
+            if tr.get(kern):
+                roof[name]["traffic"] = tr[kern]["hbm_bytes"]
+                roof[name]["traffic_kernel"] = kern
     log("roofline", roof)
 
     if args.copy_probe > 0:

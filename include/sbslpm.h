@@ -221,6 +221,17 @@ int sbslpm_bve_find_departure_points(sbslpm_ctx* ctx, const double* x, const dou
 int sbslpm_bve_departure_to_target(sbslpm_ctx* ctx, const double* z, const double* dep_x, const double* dep_y,
                                    const double* dep_z, const double* vor, const double* tracers, int ntr,
                                    double* new_vor, double* new_tracers);
+/* Ensemble form of bve_back_rk4_step: nmem members on the context's grid (shared x, y, z, area
+ * [Pl][m2]), each advanced by one backward RK4 step exactly as bve_back_rk4_step would, with every
+ * kernel launch covering all members (the tree, interaction list and work tiles are shared).
+ * vor [nmem][Pl][m2] is updated in place; vel_{x,y,z} [nmem][Pl][m2] receive the stage-0
+ * velocities.  No tracers.  Needs levels >= 2.  The workspace for nmem members is allocated on
+ * the first call with that count; with SBSLPM_FLAG_GRAPH and a non-default stream the step is
+ * captured once in a hipGraph and replayed. */
+int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* ctx, int nmem, const double* x, const double* y, const double* z,
+                                      const double* area, double* vor, double* vel_x, double* vel_y, double* vel_z,
+                                      double time);
+
 /* swe_displacement_upward_pass (swe_time_step.cpp:265-276): disp_part = tau v [Pl][m2]
  * (swe_disp_update, :194-210); disp_proxy [P][n] non-leaf rows interpolated from the leaf that
  * contains each undisplaced node (swe_disp_interp, :212-263). */

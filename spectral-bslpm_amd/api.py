@@ -361,6 +361,16 @@ class Context:
                     "departure_to_target")
         return new_vor, new_tracers
 
+    def bve_back_rk4_step_ensemble(self, x, y, z, area, vors, vel, time=0.0):
+        """One backward RK4 step of every member of an ensemble (vors [members, Pl, m2] updated in
+        place, vel = 3 tensors [members, Pl, m2] receive the stage-0 velocities); all members share
+        each kernel launch."""
+        nmem = vors.shape[0]
+        self._check(self._lib.sbslpm_bve_back_rk4_step_ensemble(self.h, nmem, _ptr(x), _ptr(y), _ptr(z), _ptr(area),
+                                                                _ptr(vors), *[_ptr(t) for t in vel], float(time)),
+                    "bve_back_rk4_step_ensemble")
+        return vors
+
     def swe_displacement(self, vel, tau, disp_part=None, disp_proxy=None):
         """swe_displacement_upward_pass, swe_time_step.cpp:265-276."""
         disp_part = self.zeros_part(3) if disp_part is None else disp_part

@@ -15,6 +15,26 @@
 
 using namespace sbslpm;
 
+// Per-evaluation state of a BVE step for `members` stacked members: particle
+// arrays [members][Pl][m2], proxy arrays [members][P][n], records per member.
+struct Work {
+  int members = 0;
+  SrcRec64* rec64 = nullptr;
+  SrcRec32* rec32 = nullptr;
+  SrcRecMfma* recmf = nullptr;
+  double* qpart = nullptr;
+  double* qproxy = nullptr;
+  double* phipart[3] = {};
+  double* phiproxy[3] = {};
+  double* disppart[3] = {};
+  double* dispproxy[3] = {};
+  double* vstage[9] = {};
+  double* newvor = nullptr;
+  double* dep[3] = {};
+  double* vor_next = nullptr;
+  double* tr_next = nullptr;
+};
+
 struct sbslpm_ctx {
   sbslpm_params prm{};
   Tree tree;
@@ -50,6 +70,11 @@ struct sbslpm_ctx {
   double* tr_next = nullptr;
   double* scratch = nullptr;  // direct sums, grown on demand
   int64_t scratch_len = 0;
+  Work w1;                     // the single-member workspace (aliases the arena buffers above)
+  Work wens;                   // ensemble workspace, allocated on demand
+  char* ens_arena = nullptr;
+  hipGraphExec_t ens_graph = nullptr;
+  const void* ens_key[10] = {};
   // shallow-water workspace, allocated on the first SWE call
   char* swe_arena = nullptr;
   SrcRecSwe* recswe = nullptr;
@@ -111,8 +136,8 @@ int upload(sbslpm_ctx* c, void* dst, const void* src, size_t bytes) {
   return SBSLPM_OK;
 }
 
-int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double* const* dpart,
-             const double* const* dprox, double* const* vel);
+int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const double* vor,
+             const double* const* dpart, const double* const* dprox, double* const* vel);
 
 // Brackets one pair-kernel launch with events when timing is on.
 struct KernelTimer {
@@ -262,6 +287,18 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   c->newvor = (double*)(b + o_newvor);
   c->vor_next = (double*)(b + o_vnext);
   c->tr_next = (double*)(b + o_trnext);
+  {
+    Work& w = c->w1;
+    w.members = 1;
+    w.rec64 = c->rec64; w.rec32 = c->rec32; w.recmf = c->recmf;
+    w.qpart = c->qpart; w.qproxy = c->qproxy;
+    for (int f = 0; f < 3; ++f) {
+      w.phipart[f] = c->phipart[f]; w.phiproxy[f] = c->phiproxy[f];
+      w.disppart[f] = c->disppart[f]; w.dispproxy[f] = c->dispproxy[f]; w.dep[f] = c->dep[f];
+    }
+    for (int k = 0; k < 9; ++k) w.vstage[k] = c->vstage[k];
+    w.newvor = c->newvor; w.vor_next = c->vor_next; w.tr_next = c->tr_next;
+  }
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     hipFree(c->arena);
     delete c;
@@ -283,6 +320,8 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
   }
   if (c->scratch) hipFree(c->scratch);
   if (c->swe_arena) hipFree(c->swe_arena);
+  if (c->ens_graph) hipGraphExecDestroy(c->ens_graph);
+  if (c->ens_arena) hipFree(c->ens_arena);
   if (c->arena) hipFree(c->arena);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
@@ -453,7 +492,7 @@ int sbslpm_bve_velocity(sbslpm_ctx* c, const double* area, const double* vor, co
   const double* dp[3] = {dpx, dpy, dpz};
   const double* dx[3] = {dxx, dxy, dxz};
   double* v[3] = {vx, vy, vz};
-  return velocity(c, area, vor, dpx ? dp : nullptr, dxx ? dx : nullptr, v);
+  return velocity(c, c->w1, 1, area, vor, dpx ? dp : nullptr, dxx ? dx : nullptr, v);
 }
 
 int sbslpm_poisson_solve(sbslpm_ctx* c, const double* area, const double* f, double* psi) {
@@ -529,8 +568,8 @@ int sbslpm_swe_velocity(sbslpm_ctx* c, int variant, const double* area, const do
     return fail(c, SBSLPM_E_ARG, "swe_velocity: bad argument");
   int rc = ensure_swe(c);
   if (rc) return rc;
-  launch_upward_pass(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
-  launch_upward_pass(c->dt, area, div, c->qdpart, c->qdproxy, c->stream);
+  launch_upward_fused(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
+  launch_upward_fused(c->dt, area, div, c->qdpart, c->qdproxy, c->stream);
   launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, 3, c->phiproxy, c->stream);
   rc = swe_interactions(c, variant, dpx, dpy, dpz, dxx, dxy, dxz, c->qpart, c->qproxy, c->qdpart, c->qdproxy,
                         c->phipart, c->phiproxy, true);
@@ -596,64 +635,65 @@ int ensure_scratch(sbslpm_ctx* c, int64_t need) {
   return SBSLPM_OK;
 }
 
-int velocity(sbslpm_ctx* c, const double* area, const double* vor, const double* const* dpart,
-             const double* const* dprox, double* const* vel) {
+int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const double* vor,
+             const double* const* dpart, const double* const* dprox, double* const* vel) {
   const Tree& t = c->tree;
   const int64_t Pn = (int64_t)t.P * t.n;
-  (void)Pn;
   // Every leaf particle and every proxy of a target panel is written (assigned)
-  // by exactly one pair-kernel lane, the leaf proxy rows by the last L2L level;
-  // only the remaining proxy rows need clearing.
-  launch_upward_pass(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
-  launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 3, c->phiproxy, c->stream);
+  // by exactly one pair-kernel lane; leaf proxy rows are neither sources nor
+  // targets (the fused passes keep them in LDS); only the remaining proxy rows
+  // need clearing.
+  launch_upward_fused(c->dt, area, vor, w.qpart, w.qproxy, c->stream, nmem);
+  launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 3, w.phiproxy, c->stream, nmem, Pn);
   if (c->prm.precision == SBSLPM_MIXED) {
-    launch_source_records_mfma(c->dt, c->qpart, c->qproxy, dpart, dprox, c->recmf, c->stream);
+    launch_source_records_mfma(c->dt, w.qpart, w.qproxy, dpart, dprox, w.recmf, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
-    launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, c->recmf, c->prm.kernel_eps, c->phipart,
-                             c->phiproxy, c->stream, true);
+    launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, c->prm.kernel_eps, w.phipart,
+                             w.phiproxy, c->stream, true, nmem);
   } else if (c->prm.precision == SBSLPM_MIXED_VALU) {
-    launch_source_records32(c->dt, c->qpart, c->qproxy, dpart, dprox, c->rec32, c->stream);
+    launch_source_records32(c->dt, w.qpart, w.qproxy, dpart, dprox, w.rec32, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
-    launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, c->prm.kernel_eps, c->phipart,
-                              c->phiproxy, c->stream, true);
+    launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, w.rec32, c->prm.kernel_eps, w.phipart,
+                              w.phiproxy, c->stream, true, nmem);
   } else {
-    launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->rec64, c->stream);
+    launch_source_records64(c->dt, w.qpart, w.qproxy, dpart, dprox, true, w.rec64, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_FP64);
-    launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, c->prm.kernel_eps, c->phipart,
-                          c->phiproxy, c->stream, true);
+    launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, w.rec64, c->prm.kernel_eps, w.phipart, w.phiproxy,
+                          c->stream, true, nmem);
   }
-  const double* pp[3] = {c->phipart[0], c->phipart[1], c->phipart[2]};
-  launch_downward_pass(c->dt, 3, c->phiproxy, pp, vel, c->stream, true);
+  const double* pp[3] = {w.phipart[0], w.phipart[1], w.phipart[2]};
+  launch_downward_pass(c->dt, 3, w.phiproxy, pp, vel, c->stream, true, nmem);
   return launched(c);
 }
 
-// bve_back_rk4_step body (bve_time_step.cpp:318-418) on the context stream.
-int bve_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area, double* vor,
-                  double* const* vel0, double* tracers) {
+// bve_back_rk4_step body (bve_time_step.cpp:318-418) on the context stream,
+// for nmem stacked members (tracers only for a single member).
+int bve_step_body(sbslpm_ctx* c, const Work& w, int nmem, const double* x, const double* y, const double* z,
+                  const double* area, double* vor, double* const* vel0, double* tracers) {
   const Tree& t = c->tree;
   const double dt = c->prm.delta_t, om = c->prm.omega;
-  double* const* v1 = c->vstage;
-  double* const* v2 = c->vstage + 3;
-  double* const* v3 = c->vstage + 6;
-  const double* cdp[3] = {c->disppart[0], c->disppart[1], c->disppart[2]};
-  const double* cdx[3] = {c->dispproxy[0], c->dispproxy[1], c->dispproxy[2]};
-  int rc = velocity(c, area, vor, nullptr, nullptr, vel0);
+  double* const* v1 = w.vstage;
+  double* const* v2 = w.vstage + 3;
+  double* const* v3 = w.vstage + 6;
+  const double* cdp[3] = {w.disppart[0], w.disppart[1], w.disppart[2]};
+  const double* cdx[3] = {w.dispproxy[0], w.dispproxy[1], w.dispproxy[2]};
+  int rc = velocity(c, w, nmem, area, vor, nullptr, nullptr, vel0);
   const double* stage_in[3][3] = {{vel0[0], vel0[1], vel0[2]}, {v1[0], v1[1], v1[2]}, {v2[0], v2[1], v2[2]}};
   double* const* stage_out[3] = {v1, v2, v3};
   const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
   for (int s = 0; s < 3 && rc == SBSLPM_OK; ++s) {
-    launch_bve_displacement(c->dt, x, y, z, stage_in[s], vor, taus[s], om, c->newvor, c->disppart, c->dispproxy,
-                            c->stream);
-    rc = velocity(c, area, c->newvor, cdp, cdx, stage_out[s]);
+    launch_bve_displacement(c->dt, x, y, z, stage_in[s], vor, taus[s], om, w.newvor, w.disppart, w.dispproxy,
+                            c->stream, nmem);
+    rc = velocity(c, w, nmem, area, w.newvor, cdp, cdx, stage_out[s]);
   }
   if (rc) return rc;
   const double* vels[12] = {vel0[0], vel0[1], vel0[2], v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
-  launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, c->dep[0], c->dep[1], c->dep[2], c->stream);
-  const int ntr = c->prm.tracer_count;
-  launch_bve_departure_to_target(c->dt, z, c->dep[0], c->dep[1], c->dep[2], vor, tracers, ntr, om, c->vor_next,
-                                 c->tr_next, c->stream);
-  launch_copy(vor, c->vor_next, t.npart, c->stream);
-  if (ntr > 0) launch_copy(tracers, c->tr_next, t.npart * ntr, c->stream);
+  launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, w.dep[0], w.dep[1], w.dep[2], c->stream, nmem);
+  const int ntr = nmem == 1 ? c->prm.tracer_count : 0;
+  launch_bve_departure_to_target(c->dt, z, w.dep[0], w.dep[1], w.dep[2], vor, tracers, ntr, om, w.vor_next,
+                                 w.tr_next, c->stream, nmem);
+  launch_copy(vor, w.vor_next, t.npart * nmem, c->stream);
+  if (ntr > 0) launch_copy(tracers, w.tr_next, t.npart * ntr, c->stream);
   return launched(c);
 }
 
@@ -779,7 +819,7 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
     return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step: bad argument");
   double* vel0[3] = {vx, vy, vz};
   // Graph capture needs a non-default stream; on the default stream the step runs eagerly.
-  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH) || !c->stream) return bve_step_body(c, x, y, z, area, vor, vel0, tracers);
+  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH) || !c->stream) return bve_step_body(c, c->w1, 1, x, y, z, area, vor, vel0, tracers);
 
   const void* key[10] = {x, y, z, area, vor, vx, vy, vz, tracers, (const void*)(intptr_t)c->prm.precision};
   if (!c->graph || std::memcmp(key, c->graph_key, sizeof(key)) != 0) {
@@ -790,7 +830,7 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
     hipGraph_t g = nullptr;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    int rc = bve_step_body(c, x, y, z, area, vor, vel0, tracers);
+    int rc = bve_step_body(c, c->w1, 1, x, y, z, area, vor, vel0, tracers);
     c->capturing = false;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -807,6 +847,95 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
     std::memcpy(c->graph_key, key, sizeof(key));
   }
   HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+  return SBSLPM_OK;
+}
+
+// Ensemble workspace for nmem members (grown on demand, never shrunk).
+static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
+  if (c->ens_arena && c->wens.members >= nmem) return SBSLPM_OK;
+  const Tree& t = c->tree;
+  const size_t NP = (size_t)t.npart * nmem, Pn = (size_t)t.P * t.n * nmem, U = (size_t)t.U * nmem,
+               U16 = (size_t)t.U16 * nmem;
+  Arena A;
+  const size_t o_r64 = A.take<SrcRec64>(c->prm.precision == SBSLPM_FP64 ? U : 0);
+  const size_t o_r32 = A.take<SrcRec32>(c->prm.precision == SBSLPM_MIXED_VALU ? U : 0);
+  const size_t o_rmf = A.take<SrcRecMfma>(c->prm.precision == SBSLPM_MIXED ? U16 : 0);
+  const size_t o_qp = A.take<double>(NP), o_qx = A.take<double>(Pn);
+  const size_t o_pp = A.take<double>(3 * NP), o_px = A.take<double>(3 * Pn);
+  const size_t o_dp = A.take<double>(3 * NP), o_dx = A.take<double>(3 * Pn);
+  const size_t o_vs = A.take<double>(9 * NP), o_nv = A.take<double>(NP), o_dep = A.take<double>(3 * NP);
+  const size_t o_vn = A.take<double>(NP), total = A.take<char>(256);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->ens_graph) {
+    hipGraphExecDestroy(c->ens_graph);
+    c->ens_graph = nullptr;
+  }
+  if (c->ens_arena) hipFree(c->ens_arena);
+  c->ens_arena = nullptr;
+  c->wens = Work{};
+  char* b = nullptr;
+  if (hipMalloc(&b, total) != hipSuccess) return fail(c, SBSLPM_E_NOMEM, "ensemble workspace: out of device memory");
+  c->ens_arena = b;
+  Work& w = c->wens;
+  w.members = nmem;
+  w.rec64 = (SrcRec64*)(b + o_r64); w.rec32 = (SrcRec32*)(b + o_r32); w.recmf = (SrcRecMfma*)(b + o_rmf);
+  w.qpart = (double*)(b + o_qp); w.qproxy = (double*)(b + o_qx);
+  for (int f = 0; f < 3; ++f) {
+    w.phipart[f] = (double*)(b + o_pp) + f * NP;
+    w.phiproxy[f] = (double*)(b + o_px) + f * Pn;
+    w.disppart[f] = (double*)(b + o_dp) + f * NP;
+    w.dispproxy[f] = (double*)(b + o_dx) + f * Pn;
+    w.dep[f] = (double*)(b + o_dep) + f * NP;
+  }
+  for (int k = 0; k < 9; ++k) w.vstage[k] = (double*)(b + o_vs) + k * NP;
+  w.newvor = (double*)(b + o_nv);
+  w.vor_next = (double*)(b + o_vn);
+  return SBSLPM_OK;
+}
+
+// Stacked per-member arrays are [member][...] with the member as the slowest
+// index; component arrays of the workspace are [component][member][...].  The
+// kernels index member j of a component array at j * (member size), so the
+// workspace keeps every component's members contiguous: phipart[f] + j * NP.
+int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* c, int nmem, const double* x, const double* y, const double* z,
+                                      const double* area, double* vor, double* vx, double* vy, double* vz,
+                                      double time) {
+  (void)time;
+  if (!c || nmem < 1 || nmem > 65535 || !x || !y || !z || !area || !vor || !vx || !vy || !vz)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_ensemble: bad argument");
+  if (c->tree.cfg.levels < 2) return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_ensemble: needs levels >= 2");
+  int rc = ensure_ensemble(c, nmem);
+  if (rc) return rc;
+  double* vel0[3] = {vx, vy, vz};
+  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH) || !c->stream)
+    return bve_step_body(c, c->wens, nmem, x, y, z, area, vor, vel0, nullptr);
+  const void* key[10] = {x,  y,  z,  area, vor, vx, vy, vz, (const void*)(intptr_t)nmem,
+                         (const void*)(intptr_t)c->prm.precision};
+  if (!c->ens_graph || std::memcmp(key, c->ens_key, sizeof(key)) != 0) {
+    if (c->ens_graph) {
+      hipGraphExecDestroy(c->ens_graph);
+      c->ens_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    rc = bve_step_body(c, c->wens, nmem, x, y, z, area, vor, vel0, nullptr);
+    c->capturing = false;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, SBSLPM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->ens_graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->ens_graph = nullptr;
+      return fail(c, SBSLPM_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    std::memcpy(c->ens_key, key, sizeof(key));
+  }
+  HIP_TRY(c, hipGraphLaunch(c->ens_graph, c->stream));
   return SBSLPM_OK;
 }
 

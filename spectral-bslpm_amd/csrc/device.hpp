@@ -39,6 +39,13 @@ struct DevTree {
   int64_t U16;    // padded record count of the MFMA kernels
 };
 
+// Ensembles: kernels launched with gridDim.y = members take member
+// blockIdx.y's slice of every per-member array (particle arrays [Pl][m2],
+// proxy arrays [P][n], source records), stacked member after member.  With
+// gridDim.y = 1 the offsets are zero.
+__device__ inline int64_t member_part(const DevTree& t) { return (int64_t)blockIdx.y * t.npart; }
+__device__ inline int64_t member_proxy(const DevTree& t) { return (int64_t)blockIdx.y * t.P * t.n; }
+
 // Pair kernels run one wave per target tile, four waves per workgroup.
 constexpr int kWavesPerBlock = 4;
 
@@ -59,6 +66,14 @@ __device__ inline double rcp64(double d) {
   r = fma(r, e, r);
   e = fma(-d, r, 1.0);
   return fma(r, e, r);
+}
+
+// 1/d in FP64 with one Newton-Raphson step on the hardware estimate (relative
+// error about the square of the estimate's, i.e. an ulp or two), for the pair
+// kernels where d = (1 - x.y) + eps >= eps.
+__device__ inline double rcp64_pair(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
 }
 
 // interp_funcs_impl.hpp:114-142: leaf containing (x,y,z) or -1; xe gets the
@@ -108,40 +123,46 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
 // adding to them (valid when nothing else writes those rows).
 // L2L over every level (no L2P); the last level assigns the leaf rows when
 // assign_leaf_rows.
-void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream_t s, bool assign_leaf_rows);
+void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream_t s, bool assign_leaf_rows,
+                int nmem = 1);
+// Upward pass with base_pots + P2M + first M2M fused; the leaf rows of q_proxy
+// are not written (they are never interaction sources).
+void launch_upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
+                         hipStream_t s, int nmem = 1);
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
-                          double* const* sols, hipStream_t s, bool assign_leaf_rows = false);
+                          double* const* sols, hipStream_t s, bool assign_leaf_rows = false, int nmem = 1);
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
 // Zeroes rows[0..nrows) (panel ids) of ncomp [P][n] arrays.
-void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s);
+void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s,
+                      int nmem = 1, int64_t member_stride = 0);
 
 // ---- interactions (fmm_interact.hip)
 void launch_source_records64(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, bool project,
-                             SrcRec64* rec, hipStream_t s);
+                             SrcRec64* rec, hipStream_t s, int nmem = 1);
 // The pair kernels accumulate into their outputs like the reference functors;
 // with assign = true they overwrite them (each target written by one lane).
 void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s, bool assign = false);
+                           hipStream_t s, bool assign = false, int nmem = 1);
 void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
                                bool assign = false);
 void launch_source_records32(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, SrcRec32* rec,
-                             hipStream_t s);
+                             hipStream_t s, int nmem = 1);
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec32* rec, double eps, double* const* out_part,
-                               double* const* out_proxy, hipStream_t s, bool assign = false);
+                               double* const* out_proxy, hipStream_t s, bool assign = false, int nmem = 1);
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                    const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s,
                                    bool assign = false);
 void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
                                 const double* const* disp_part, const double* const* disp_proxy, SrcRecMfma* rec,
-                                hipStream_t s);
+                                hipStream_t s, int nmem = 1);
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
-                              hipStream_t s, bool assign = false);
+                              hipStream_t s, bool assign = false, int nmem = 1);
 
 // ---- shallow-water pair sums (swe.hip); variant 1 or 2
 void launch_source_records_swe(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
@@ -162,13 +183,14 @@ void launch_eval_targets(const DevTree& t, int kind, const int* set_start, const
 // ---- displacement, departure points, remap (remap.hip)
 void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,
                              const double* const* vel, const double* vor, double tau, double omega,
-                             double* new_vor, double* const* disp_part, double* const* disp_proxy, hipStream_t s);
+                             double* new_vor, double* const* disp_part, double* const* disp_proxy, hipStream_t s,
+                             int nmem = 1);
 void launch_bve_departure(const DevTree& t, const double* x, const double* y, const double* z,
                           const double* const* vels /*12*/, double dt, int combine, double* dx, double* dy,
-                          double* dz, hipStream_t s);
+                          double* dz, hipStream_t s, int nmem = 1);
 void launch_bve_departure_to_target(const DevTree& t, const double* z, const double* dx, const double* dy,
                                     const double* dz, const double* vor, const double* tracers, int ntr,
-                                    double omega, double* new_vor, double* new_tracers, hipStream_t s);
+                                    double omega, double* new_vor, double* new_tracers, hipStream_t s, int nmem = 1);
 // swe_departure_to_target: fields = {vor, div, h} [Pl][m2], no Coriolis terms.
 void launch_swe_departure_to_target(const DevTree& t, const double* dx, const double* dy, const double* dz,
                                     const double* const* fields, const double* tracers, int ntr,

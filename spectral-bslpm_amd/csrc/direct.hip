@@ -31,14 +31,6 @@ Plan plan(int64_t nt, int64_t ns) {
   return p;
 }
 
-__device__ inline double rcp64(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
-
 // guard 0: skip j == self_offset + i (the reference's i != j);
 // guard 1: skip x.y >= 1 - 1e-15 (the FMM kernel's coincidence guard).
 __global__ __launch_bounds__(256) void k_direct_log(int64_t nt, const double* __restrict__ tx,

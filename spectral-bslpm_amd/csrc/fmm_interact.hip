@@ -28,6 +28,27 @@ struct CPtr3 {
   const double* p[3];
 };
 
+// Member slices of a batched launch (gridDim.y members): charges and
+// displacements of the sources, outputs of the targets.
+__device__ inline void member_sources(const DevTree& t, const double* __restrict__& q_part,
+                                      const double* __restrict__& q_proxy, CPtr3& dpart,
+                                      CPtr3& dprox) {
+  const int64_t mp = member_part(t), mx = member_proxy(t);
+  q_part += mp;
+  q_proxy += mx;
+  for (int f = 0; f < 3; ++f) {
+    if (dpart.p[f]) dpart.p[f] += mp;
+    if (dprox.p[f]) dprox.p[f] += mx;
+  }
+}
+__device__ inline void member_targets(const DevTree& t, Ptr3& part, Ptr3& proxy) {
+  const int64_t mp = member_part(t), mx = member_proxy(t);
+  for (int f = 0; f < 3; ++f) {
+    if (part.p[f]) part.p[f] += mp;
+    if (proxy.p[f]) proxy.p[f] += mx;
+  }
+}
+
 // Target-major ownership: every target point is written by exactly one lane,
 // so `assign` may replace the accumulate of the reference functors.
 __device__ inline void store_u(const DevTree& t, int64_t u, Ptr3 part, Ptr3 proxy, double a, double b, double c,
@@ -58,6 +79,8 @@ __global__ void k_src64(DevTree t, const double* __restrict__ q_part, const doub
                         CPtr3 dpart, CPtr3 dprox, int project, SrcRec64* __restrict__ rec) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.U) return;
+  member_sources(t, q_part, q_proxy, dpart, dprox);
+  rec += (int64_t)blockIdx.y * t.U;
   double x = t.px[u], y = t.py[u], z = t.pz[u], q;
   if (u < t.npart) {
     if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
@@ -77,6 +100,8 @@ __global__ void k_src32(DevTree t, const double* __restrict__ q_part, const doub
                         CPtr3 dpart, CPtr3 dprox, int project, SrcRec32* __restrict__ rec) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.U) return;
+  member_sources(t, q_part, q_proxy, dpart, dprox);
+  rec += (int64_t)blockIdx.y * t.U;
   double x = t.px[u], y = t.py[u], z = t.pz[u], q;
   int panel;
   if (u < t.npart) {
@@ -101,6 +126,8 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
                                                const SourceSet* __restrict__ sets,
                                                const SrcRec64* __restrict__ rec, double eps, Ptr3 part,
                                                Ptr3 proxy, int assign) {
+  member_targets(t, part, proxy);
+  rec += (int64_t)blockIdx.y * t.U;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -116,7 +143,7 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
     for (int k = 0; k < S.count; ++k) {
       const SrcRec64 s = r[k];
       const double dp = x * s.x + y * s.y + z * s.z;
-      const double g = rcp64((1.0 - dp) + eps) * s.q;
+      const double g = rcp64_pair((1.0 - dp) + eps) * s.q;
       w0 = fma(g, s.x, w0);
       w1 = fma(g, s.y, w1);
       w2 = fma(g, s.z, w2);
@@ -160,6 +187,8 @@ __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* 
                                                    const SourceSet* __restrict__ sets,
                                                    const SrcRec32* __restrict__ rec, float eps, Ptr3 part,
                                                    Ptr3 proxy, int assign) {
+  member_targets(t, part, proxy);
+  rec += (int64_t)blockIdx.y * t.U;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -230,6 +259,8 @@ __global__ void k_src_mfma(DevTree t, const double* __restrict__ q_part, const d
                            CPtr3 dpart, CPtr3 dprox, SrcRecMfma* __restrict__ rec) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= t.U16) return;
+  member_sources(t, q_part, q_proxy, dpart, dprox);
+  rec += (int64_t)blockIdx.y * t.U16;
   const int64_t lp = (int64_t)t.Pl * t.m2p;
   int panel, k;
   int64_t u;
@@ -378,6 +409,8 @@ __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* _
                                                   const SourceSet* __restrict__ sets,
                                                   const float* __restrict__ rec, float eps, Ptr3 part,
                                                   Ptr3 proxy, int assign) {
+  member_targets(t, part, proxy);
+  rec += (int64_t)blockIdx.y * t.U16 * 8;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
@@ -432,23 +465,23 @@ inline Ptr3 ptr3(double* const* a) {
 
 void launch_source_records64(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, bool project,
-                             SrcRec64* rec, hipStream_t s) {
-  hipLaunchKernelGGL(k_src64, dim3(blocks_for(t.U, 256)), dim3(256), 0, s, t, q_part, q_proxy, cptr3(disp_part),
+                             SrcRec64* rec, hipStream_t s, int nmem) {
+  hipLaunchKernelGGL(k_src64, dim3(blocks_for(t.U, 256), nmem), dim3(256), 0, s, t, q_part, q_proxy, cptr3(disp_part),
                      cptr3(disp_proxy), project ? 1 : 0, rec);
 }
 
 void launch_source_records32(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, SrcRec32* rec,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_src32, dim3(blocks_for(t.U, 256)), dim3(256), 0, s, t, q_part, q_proxy, cptr3(disp_part),
+                             hipStream_t s, int nmem) {
+  hipLaunchKernelGGL(k_src32, dim3(blocks_for(t.U, 256), nmem), dim3(256), 0, s, t, q_part, q_proxy, cptr3(disp_part),
                      cptr3(disp_proxy), 1, rec);
 }
 
 void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s, bool assign) {
+                           hipStream_t s, bool assign, int nmem) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
+  hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, kWavesPerBlock), nmem), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
                      ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0);
 }
 
@@ -462,24 +495,24 @@ void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int nt
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec32* rec, double eps, double* const* out_part,
-                               double* const* out_proxy, hipStream_t s, bool assign) {
+                               double* const* out_proxy, hipStream_t s, bool assign, int nmem) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
+  hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, kWavesPerBlock), nmem), dim3(64 * kWavesPerBlock), 0, s, t,
                      tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0);
 }
 
 void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
                                 const double* const* disp_part, const double* const* disp_proxy, SrcRecMfma* rec,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_src_mfma, dim3(blocks_for(t.U16, 256)), dim3(256), 0, s, t, q_part, q_proxy,
+                                hipStream_t s, int nmem) {
+  hipLaunchKernelGGL(k_src_mfma, dim3(blocks_for(t.U16, 256), nmem), dim3(256), 0, s, t, q_part, q_proxy,
                      cptr3(disp_part), cptr3(disp_proxy), rec);
 }
 
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
-                              hipStream_t s, bool assign) {
+                              hipStream_t s, bool assign, int nmem) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
+  hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, kWavesPerBlock), nmem), dim3(64 * kWavesPerBlock), 0, s, t,
                      tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy),
                      assign ? 1 : 0);
 }

@@ -22,6 +22,13 @@ struct CPtr3 {
   const double* p[3];
 };
 
+// Component f of a pointer triple as a select, so that the triple stays in
+// registers (dynamic indexing of a kernel-argument array goes to scratch).
+template <typename P3>
+__device__ inline auto comp(const P3& a, int f) -> decltype(a.p[0]) {
+  return f == 0 ? a.p[0] : (f == 1 ? a.p[1] : a.p[2]);
+}
+
 // The transfers are tensor products, out = X in Y^T with small dense X, Y;
 // every kernel below evaluates them as two 1-D sweeps through LDS
 // (R^2 (R + Q) instead of Q^2 R^2 multiply-adds).
@@ -67,6 +74,7 @@ __global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, double* __re
   const int par = pbegin + blockIdx.x;
   const int n = t.n, dp1 = t.dp1;
   const int c0 = t.panels[par].child0;
+  q += member_proxy(t);
   for (int i = threadIdx.x; i < 4 * n; i += blockDim.x) sq[i] = q[(int64_t)c0 * n + i];
   for (int i = threadIdx.x; i < n; i += blockDim.x) { shi[i] = t.m2m_hi[i]; slo[i] = t.m2m_lo[i]; }
   __syncthreads();
@@ -90,6 +98,68 @@ __global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, double* __re
   }
 }
 
+// base_pots + P2M + the first M2M level fused, one workgroup per parent of
+// the leaves: q_part = area * pots is written for the pair kernels, the four
+// leaves' proxy charges stay in LDS (leaf proxy rows are never interaction
+// sources, so they are not written) and the parent row is assigned.
+__global__ __launch_bounds__(256) void k_p2m_m2m(DevTree t, int pbegin, const double* __restrict__ area,
+                                                 const double* __restrict__ pots, double* __restrict__ q_part,
+                                                 double* __restrict__ q) {
+  __shared__ double sq[4 * kMaxN], sv[4 * kMaxN], sQ[4 * kMaxN], sw[4 * kMaxN];
+  __shared__ double shi[kMaxN], slo[kMaxN], sT[kMaxN];
+  const int par = pbegin + blockIdx.x;
+  const int n = t.n, dp1 = t.dp1, m = t.m, m2 = t.m2;
+  const int leaf0 = t.panels[par].child0 - t.offset;
+  pots += member_part(t);
+  q_part += member_part(t);
+  q += member_proxy(t);
+  for (int i = threadIdx.x; i < 4 * m2; i += blockDim.x) {
+    const int64_t u = (int64_t)leaf0 * m2 + i;
+    const double v = area[u] * pots[u];
+    q_part[u] = v;
+    sq[i] = v;
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) { shi[i] = t.m2m_hi[i]; slo[i] = t.m2m_lo[i]; }
+  for (int i = threadIdx.x; i < dp1 * m; i += blockDim.x) sT[i] = t.p2m[i];
+  __syncthreads();
+  const double* Q = sq;
+  if (m != dp1) {  // P2M per leaf: Q_c[a][b] = sum_kl T[a][k] T[b][l] q_c[k][l]
+    const int vn = m * dp1;
+    for (int o = threadIdx.x; o < 4 * vn; o += blockDim.x) {
+      const int c = o / vn, k = (o % vn) / dp1, b = o % dp1;
+      double v = 0;
+      for (int l = 0; l < m; ++l) v += sT[b * m + l] * sq[c * m2 + k * m + l];
+      sv[o] = v;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < 4 * n; o += blockDim.x) {
+      const int c = o / n, a = (o % n) / dp1, b = o % dp1;
+      double v = 0;
+      for (int k = 0; k < m; ++k) v += sT[a * m + k] * sv[c * vn + k * dp1 + b];
+      sQ[o] = v;
+    }
+    __syncthreads();
+    Q = sQ;
+  }
+  for (int o = threadIdx.x; o < 4 * n; o += blockDim.x) {  // V_c[k][b] = sum_l Y_c[b][l] Q_c[k][l]
+    const int c = o / n, k = (o % n) / dp1, b = o % dp1;
+    const double* Y = (c <= 1) ? shi : slo;
+    double v = 0;
+    for (int l = 0; l < dp1; ++l) v += Y[b * dp1 + l] * Q[c * n + k * dp1 + l];
+    sw[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < n; o += blockDim.x) {  // Q_p[a][b] = sum_c sum_k X_c[a][k] V_c[k][b]
+    const int a = o / dp1, b = o % dp1;
+    double v = 0;
+    for (int c = 0; c < 4; ++c) {
+      const double* X = (c == 0 || c == 3) ? shi : slo;
+      for (int k = 0; k < dp1; ++k) v += X[a * dp1 + k] * sw[c * n + k * dp1 + b];
+    }
+    q[(int64_t)par * n + o] = v;
+  }
+}
+
 // L2L for all parents of one level (parent_to_child_3, fmm_funcs_impl.hpp:138-171),
 // accumulating into the four children:
 // phi_c[k][l] += sum_a X_c[a][k] sum_b Y_c[b][l] phi_p[a][b].  The eta sweep
@@ -101,8 +171,9 @@ __global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, P
   const int par = pbegin + blockIdx.x;
   const int n = t.n, dp1 = t.dp1;
   const int c0 = t.panels[par].child0;
+  const int64_t mx = member_proxy(t);
   for (int f = 0; f < ncomp; ++f)
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sp[f * n + i] = phi.p[f][(int64_t)par * n + i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sp[f * n + i] = comp(phi, f)[mx + (int64_t)par * n + i];
   for (int i = threadIdx.x; i < n; i += blockDim.x) { shi[i] = t.m2m_hi[i]; slo[i] = t.m2m_lo[i]; }
   __syncthreads();
   for (int o = threadIdx.x; o < ncomp * 2 * n; o += blockDim.x) {  // V_h[a][l] = sum_b Y_h[b][l] phi[a][b]
@@ -120,7 +191,7 @@ __global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, P
     const double* V = sv + (f * 2 + (c <= 1 ? 0 : 1)) * n;
     double v = 0;
     for (int a = 0; a < dp1; ++a) v += X[a * dp1 + k] * V[a * dp1 + l];
-    double* dst = phi.p[f] + (int64_t)(c0 + c) * n + k * dp1 + l;
+    double* dst = comp(phi, f) + mx + (int64_t)(c0 + c) * n + k * dp1 + l;
     *dst = assign ? v : *dst + v;
   }
 }
@@ -133,8 +204,10 @@ __global__ __launch_bounds__(128) void k_l2p(DevTree t, int ncomp, CPtr3 phi, CP
   __shared__ double sT[kMaxN];
   const int leaf = blockIdx.x;
   const int n = t.n, dp1 = t.dp1, m = t.m, m2 = t.m2;
+  const int64_t mp = member_part(t), mx = member_proxy(t);
   for (int f = 0; f < ncomp; ++f)
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sp[f * n + i] = phi.p[f][(int64_t)(t.offset + leaf) * n + i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      sp[f * n + i] = comp(phi, f)[mx + (int64_t)(t.offset + leaf) * n + i];
   for (int i = threadIdx.x; i < dp1 * m; i += blockDim.x) sT[i] = t.p2m[i];
   __syncthreads();
   const int vn = dp1 * m;
@@ -151,28 +224,33 @@ __global__ __launch_bounds__(128) void k_l2p(DevTree t, int ncomp, CPtr3 phi, CP
     const double* V = sv + f * vn;
     double v = 0;
     for (int a = 0; a < dp1; ++a) v += sT[a * m + k] * V[a * m + l];
-    const int64_t u = (int64_t)leaf * m2 + (o % m2);
-    if (part.p[f]) v += part.p[f][u];
-    sols.p[f][u] = v;
+    const int64_t u = mp + (int64_t)leaf * m2 + (o % m2);
+    const double* pf = comp(part, f);
+    if (pf) v += pf[u];
+    comp(sols, f)[u] = v;
   }
 }
 
 // Identity L2P of the reference layout: sols = leaf proxy rows (+ particle field).
-__global__ void k_l2p_copy(int64_t npart, int ncomp, CPtr3 leafrows, CPtr3 part, Ptr3 sols) {
+__global__ void k_l2p_copy(int64_t npart, int ncomp, CPtr3 leafrows, CPtr3 part, Ptr3 sols, int64_t proxy_stride) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= npart) return;
+  const int64_t ux = (int64_t)blockIdx.y * proxy_stride + u;
+  u += (int64_t)blockIdx.y * npart;
   for (int f = 0; f < ncomp; ++f) {
-    double v = leafrows.p[f][u];
-    if (part.p[f]) v += part.p[f][u];
-    sols.p[f][u] = v;
+    double v = comp(leafrows, f)[ux];
+    const double* pf = comp(part, f);
+    if (pf) v += pf[u];
+    comp(sols, f)[u] = v;
   }
 }
 
-__global__ void k_zero_rows(const int* __restrict__ rows, int nrows, int n, int ncomp, Ptr3 a) {
+__global__ void k_zero_rows(const int* __restrict__ rows, int nrows, int n, int ncomp, Ptr3 a, int64_t member_stride) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)nrows * n) return;
-  const int64_t o = (int64_t)rows[i / n] * n + i % n;
-  for (int f = 0; f < ncomp; ++f) a.p[f][o] = 0.0;
+  const int64_t o = (int64_t)blockIdx.y * member_stride + (int64_t)rows[i / n] * n + i % n;
+  for (int f = 0; f < 3; ++f)
+    if (f < ncomp) a.p[f][o] = 0.0;
 }
 
 __global__ void k_fill_zero(double* p, int64_t count) {
@@ -195,12 +273,14 @@ void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
   if (count > 0) hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
 }
 
-void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s) {
+void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s, int nmem,
+                      int64_t member_stride) {
   if (nrows <= 0) return;
   Ptr3 a{};
   for (int f = 0; f < ncomp; ++f) a.p[f] = arrays[f];
   const int64_t cnt = (int64_t)nrows * n;
-  hipLaunchKernelGGL(k_zero_rows, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, rows, nrows, n, ncomp, a);
+  hipLaunchKernelGGL(k_zero_rows, dim3(blocks_for(cnt, 256), nmem), dim3(256), 0, s, rows, nrows, n, ncomp, a,
+                     member_stride);
 }
 
 void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
@@ -224,35 +304,55 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
   }
 }
 
+// Upward pass with base_pots, P2M and the first M2M level fused: q_part and
+// the non-leaf rows of q_proxy are assigned, the leaf rows are not written.
+void launch_upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
+                         hipStream_t s, int nmem) {
+  if (t.levels < 2) {
+    launch_upward_pass(t, area, pots, q_part, q_proxy, s);
+    return;
+  }
+  {
+    const int l = t.levels - 2;
+    const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
+    hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
+  }
+  for (int l = t.levels - 3; l >= 0; --l) {
+    const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
+    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, q_proxy);
+  }
+}
+
 // fmm_funcs.cpp:357-392.  phi_proxy rows are accumulated top-down; sols are
 // assigned = L2P(leaf proxies) + phi_part (phi_part entries may be null).
-void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream_t s, bool assign_leaf_rows) {
+void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream_t s, bool assign_leaf_rows,
+                int nmem) {
   Ptr3 phi{};
   for (int f = 0; f < ncomp; ++f) phi.p[f] = phi_proxy[f];
   for (int l = 0; l + 1 < t.levels; ++l) {
     int pbegin = 2 * ((1 << (2 * l)) - 1);
     int count = 6 << (2 * l);
     const int assign = (assign_leaf_rows && l + 2 == t.levels) ? 1 : 0;
-    hipLaunchKernelGGL(k_l2l, dim3(count), dim3(256), 0, s, t, pbegin, ncomp, phi, assign);
+    hipLaunchKernelGGL(k_l2l, dim3(count, nmem), dim3(256), 0, s, t, pbegin, ncomp, phi, assign);
   }
 }
 
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
-                          double* const* sols, hipStream_t s, bool assign_leaf_rows) {
+                          double* const* sols, hipStream_t s, bool assign_leaf_rows, int nmem) {
   CPtr3 part{}, leafrows{}; Ptr3 out{};
   for (int f = 0; f < ncomp; ++f) {
     part.p[f] = phi_part ? phi_part[f] : nullptr;
     leafrows.p[f] = phi_proxy[f] + (int64_t)t.offset * t.n;
     out.p[f] = sols[f];
   }
-  launch_l2l(t, ncomp, phi_proxy, s, assign_leaf_rows);
+  launch_l2l(t, ncomp, phi_proxy, s, assign_leaf_rows, nmem);
   CPtr3 cphi{};
   for (int f = 0; f < ncomp; ++f) cphi.p[f] = phi_proxy[f];
   if (is_identity_p2m(t))
-    hipLaunchKernelGGL(k_l2p_copy, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t.npart, ncomp, leafrows, part,
-                       out);
+    hipLaunchKernelGGL(k_l2p_copy, dim3(blocks_for(t.npart, 256), nmem), dim3(256), 0, s, t.npart, ncomp, leafrows,
+                       part, out, (int64_t)t.P * t.n);
   else
-    hipLaunchKernelGGL(k_l2p, dim3(t.Pl), dim3(128), 0, s, t, ncomp, cphi, part, out);
+    hipLaunchKernelGGL(k_l2p, dim3(t.Pl, nmem), dim3(128), 0, s, t, ncomp, cphi, part, out);
 }
 
 }  // namespace sbslpm

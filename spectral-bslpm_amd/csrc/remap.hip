@@ -94,6 +94,17 @@ __global__ void k_disp_update(DevTree t, const double* __restrict__ x, const dou
                               double omega, double* __restrict__ new_vor, Ptr3 disp) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.npart) return;
+  {  // member slice (x, y, z are shared by the members)
+    const int64_t mp = member_part(t);
+    for (int f = 0; f < 3; ++f) {
+      vel.p[f] += mp;
+      disp.p[f] += mp;
+    }
+    if (new_vor) {
+      vor += mp;
+      new_vor += mp;
+    }
+  }
   const double vx = vel.p[0][u], vy = vel.p[1][u], vz = vel.p[2][u];
   if (new_vor) {  // null for the shallow-water displacement (swe_disp_update, swe_time_step.cpp:194-210)
     double a = x[u] + vx * tau, b = y[u] + vy * tau, c = z[u] + vz * tau;
@@ -114,6 +125,10 @@ template <int M>
 __global__ __launch_bounds__(256) void k_disp_interp(DevTree t, CPtr3 dpart, Ptr3 dprox) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)t.offset * t.n) return;
+  for (int f = 0; f < 3; ++f) {
+    dpart.p[f] += member_part(t);
+    dprox.p[f] += member_proxy(t);
+  }
   const int64_t v = t.proxy_order[i];
   const int leaf = t.proxy_leaf[v];
   double out[3] = {0, 0, 0};
@@ -155,6 +170,13 @@ __global__ __launch_bounds__(128) void k_departure(DevTree t, const double* __re
                             double* __restrict__ dy, double* __restrict__ dz) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.npart) return;
+  {  // member slice (x, y, z are shared by the members)
+    const int64_t mp = member_part(t);
+    for (int i = 0; i < 12; ++i) v.p[i] += mp;
+    dx += mp;
+    dy += mp;
+    dz += mp;
+  }
   const double X = x[u], Y = y[u], Z = z[u];
   double k1[3] = {v.p[9][u], v.p[10][u], v.p[11][u]}, k2[3], k3[3], k4[3];
   double a = X - 0.5 * dt * k1[0], b = Y - 0.5 * dt * k1[1], c = Z - 0.5 * dt * k1[2];
@@ -201,6 +223,20 @@ __global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* 
                                 double* __restrict__ new_tracers) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.npart) return;
+  {  // member slice (z is shared by the members)
+    const int64_t mp = member_part(t);
+    dx += mp;
+    dy += mp;
+    dz += mp;
+    for (int q = 0; q < NF; ++q) {
+      f.in[q] += mp;
+      f.out[q] += mp;
+    }
+    if (ntr > 0) {
+      tracers += mp * ntr;
+      new_tracers += mp * ntr;
+    }
+  }
   double xe[2];
   const double X = dx[u], Y = dy[u], Z = dz[u];
   const int leaf = locate_leaf(t, X, Y, Z, xe);
@@ -259,17 +295,17 @@ void dispatch_m(int m, A&&... a) {
 
 template <int M>
 struct DispInterp {
-  static void run(const DevTree& t, CPtr3 dp, Ptr3 dx, hipStream_t s) {
+  static void run(const DevTree& t, CPtr3 dp, Ptr3 dx, hipStream_t s, int nmem) {
     int64_t cnt = (int64_t)t.offset * t.n;
-    if (cnt > 0) hipLaunchKernelGGL(k_disp_interp<M>, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, t, dp, dx);
+    if (cnt > 0) hipLaunchKernelGGL(k_disp_interp<M>, dim3(blocks_for(cnt, 256), nmem), dim3(256), 0, s, t, dp, dx);
   }
 };
 
 template <int M>
 struct Departure {
   static void run(const DevTree& t, const double* x, const double* y, const double* z, Vel12 v, double dt,
-                  int combine, double* dx, double* dy, double* dz, hipStream_t s) {
-    hipLaunchKernelGGL(k_departure<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, v, dt, combine,
+                  int combine, double* dx, double* dy, double* dz, hipStream_t s, int nmem) {
+    hipLaunchKernelGGL(k_departure<M>, dim3(blocks_for(t.npart, 128), nmem), dim3(128), 0, s, t, x, y, z, v, dt, combine,
                        dx, dy, dz);
   }
 };
@@ -277,12 +313,13 @@ struct Departure {
 template <int M>
 struct DepToTarget {
   static void run(const DevTree& t, int nf, const double* z, const double* dx, const double* dy, const double* dz,
-                  Fields3 f, const double* tr, int ntr, double omega, int coriolis, double* ntrc, hipStream_t s) {
+                  Fields3 f, const double* tr, int ntr, double omega, int coriolis, double* ntrc, hipStream_t s,
+                  int nmem) {
     if (nf == 1)
-      hipLaunchKernelGGL((k_dep_to_target<M, 1>), dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, z, dx, dy, dz, f,
+      hipLaunchKernelGGL((k_dep_to_target<M, 1>), dim3(blocks_for(t.npart, 128), nmem), dim3(128), 0, s, t, z, dx, dy, dz, f,
                          tr, ntr, omega, coriolis, ntrc);
     else
-      hipLaunchKernelGGL((k_dep_to_target<M, 3>), dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, z, dx, dy, dz, f,
+      hipLaunchKernelGGL((k_dep_to_target<M, 3>), dim3(blocks_for(t.npart, 128), nmem), dim3(128), 0, s, t, z, dx, dy, dz, f,
                          tr, ntr, omega, coriolis, ntrc);
   }
 };
@@ -291,36 +328,37 @@ struct DepToTarget {
 
 void launch_bve_displacement(const DevTree& t, const double* x, const double* y, const double* z,
                              const double* const* vel, const double* vor, double tau, double omega,
-                             double* new_vor, double* const* disp_part, double* const* disp_proxy, hipStream_t s) {
+                             double* new_vor, double* const* disp_part, double* const* disp_proxy, hipStream_t s,
+                             int nmem) {
   CPtr3 v{{vel[0], vel[1], vel[2]}};
   Ptr3 dp{{disp_part[0], disp_part[1], disp_part[2]}};
-  hipLaunchKernelGGL(k_disp_update, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, x, y, z, v, vor, tau, omega,
+  hipLaunchKernelGGL(k_disp_update, dim3(blocks_for(t.npart, 256), nmem), dim3(256), 0, s, t, x, y, z, v, vor, tau, omega,
                      new_vor, dp);
   CPtr3 cdp{{disp_part[0], disp_part[1], disp_part[2]}};
   Ptr3 dx{{disp_proxy[0], disp_proxy[1], disp_proxy[2]}};
-  dispatch_m<DispInterp>(t.m, t, cdp, dx, s);
+  dispatch_m<DispInterp>(t.m, t, cdp, dx, s, nmem);
 }
 
 void launch_bve_departure(const DevTree& t, const double* x, const double* y, const double* z,
                           const double* const* vels, double dt, int combine, double* dx, double* dy, double* dz,
-                          hipStream_t s) {
+                          hipStream_t s, int nmem) {
   Vel12 v{};
   for (int i = 0; i < 12; ++i) v.p[i] = vels[i];
-  dispatch_m<Departure>(t.m, t, x, y, z, v, dt, combine, dx, dy, dz, s);
+  dispatch_m<Departure>(t.m, t, x, y, z, v, dt, combine, dx, dy, dz, s, nmem);
 }
 
 void launch_bve_departure_to_target(const DevTree& t, const double* z, const double* dx, const double* dy,
                                     const double* dz, const double* vor, const double* tracers, int ntr,
-                                    double omega, double* new_vor, double* new_tracers, hipStream_t s) {
+                                    double omega, double* new_vor, double* new_tracers, hipStream_t s, int nmem) {
   Fields3 f{{vor, nullptr, nullptr}, {new_vor, nullptr, nullptr}};
-  dispatch_m<DepToTarget>(t.m, t, 1, z, dx, dy, dz, f, tracers, ntr, omega, 1, new_tracers, s);
+  dispatch_m<DepToTarget>(t.m, t, 1, z, dx, dy, dz, f, tracers, ntr, omega, 1, new_tracers, s, nmem);
 }
 
 void launch_swe_departure_to_target(const DevTree& t, const double* dx, const double* dy, const double* dz,
                                     const double* const* fields, const double* tracers, int ntr,
                                     double* const* new_fields, double* new_tracers, hipStream_t s) {
   Fields3 f{{fields[0], fields[1], fields[2]}, {new_fields[0], new_fields[1], new_fields[2]}};
-  dispatch_m<DepToTarget>(t.m, t, 3, nullptr, dx, dy, dz, f, tracers, ntr, 0.0, 0, new_tracers, s);
+  dispatch_m<DepToTarget>(t.m, t, 3, nullptr, dx, dy, dz, f, tracers, ntr, 0.0, 0, new_tracers, s, 1);
 }
 
 }  // namespace sbslpm

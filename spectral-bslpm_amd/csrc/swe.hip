@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_swe64(DevTree t, const TargetTile* __re
     for (int k = 0; k < S.count; ++k) {
       const SrcRecSwe s = r[k];
       const double dp = x * s.x + y * s.y + z * s.z;
-      const double g = rcp64((1.0 - dp) + eps);
+      const double g = rcp64_pair((1.0 - dp) + eps);
       const double gz = g * s.qz;
       w0 = fma(gz, s.x, w0);
       w1 = fma(gz, s.y, w1);
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_swe64(DevTree t, const TargetTile* __re
         sd = fma(gd, dp, sd);
       } else {
         const double dp0 = x * s.x0 + y * s.y0 + z * s.z0;
-        const double gd = rcp64((1.0 - dp0) + eps) * s.qd;
+        const double gd = rcp64_pair((1.0 - dp0) + eps) * s.qd;
         d0 = fma(gd, s.x0, d0);
         d1 = fma(gd, s.y0, d1);
         d2 = fma(gd, s.z0, d2);
