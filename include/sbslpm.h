@@ -232,6 +232,22 @@ int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* ctx, int nmem, const double* x
                                       const double* area, double* vor, double* vel_x, double* vel_y, double* vel_z,
                                       double time);
 
+/* Adjoint (transpose) of the Biot-Savart velocity evaluation with undisplaced sources
+ * (sbslpm_bve_velocity with NULL displacements, the first RK4 stage): for a cotangent
+ * ubar_{x,y,z} [Pl][m2] of the velocity, vor_bar [Pl][m2] is assigned
+ *   vor_bar_s = -(area_s / (4 pi)) x_s . sum_t g(x_s, x_t) (ubar_t cross x_t),
+ *   g = 1/(1 - x.y + eps),
+ * evaluated with the same tree, interaction list and passes (FP64).  No reference
+ * counterpart; it is the transpose of the operator of bve_vel_impl.hpp:90-103 up to
+ * the FMM truncation. */
+int sbslpm_bve_velocity_adjoint(sbslpm_ctx* ctx, const double* area, const double* ubar_x, const double* ubar_y,
+                                const double* ubar_z, double* vor_bar);
+/* Transpose of the remap interpolation of bve_departure_to_target (bve_time_step.cpp:38-66)
+ * with respect to the field: vor_bar [Pl][m2] += B^T new_vor_bar, B the BLI weights at the
+ * departure points.  Accumulated with FP64 atomic adds (summation order not fixed). */
+int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* ctx, const double* dep_x, const double* dep_y,
+                                           const double* dep_z, const double* new_vor_bar, double* vor_bar);
+
 /* swe_displacement_upward_pass (swe_time_step.cpp:265-276): disp_part = tau v [Pl][m2]
  * (swe_disp_update, :194-210); disp_proxy [P][n] non-leaf rows interpolated from the leaf that
  * contains each undisplaced node (swe_disp_interp, :212-263). */

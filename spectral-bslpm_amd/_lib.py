@@ -76,6 +76,8 @@ SIGNATURES = {
     "sbslpm_swe_vel_interactions_2": (_I, [_V] * 17),
     "sbslpm_swe_velocity": (_I, [_V, _I] + [_V] * 12),
     "sbslpm_bve_back_rk4_step_ensemble": (_I, [_V, _I] + [_V] * 8 + [ctypes.c_double]),
+    "sbslpm_bve_velocity_adjoint": (_I, [_V] * 6),
+    "sbslpm_bve_departure_to_target_adjoint": (_I, [_V] * 6),
     "sbslpm_swe_displacement": (_I, [_V, _V, _V, _V, ctypes.c_double] + [_V] * 6),
     "sbslpm_swe_find_departure_points": (_I, [_V, _V, _V, _V, _V, ctypes.c_double, _V, _V, _V]),
     "sbslpm_swe_departure_to_target": (_I, [_V] * 8 + [_I] + [_V] * 4),

@@ -371,6 +371,21 @@ class Context:
                     "bve_back_rk4_step_ensemble")
         return vors
 
+    def bve_velocity_adjoint(self, area, ubar, vor_bar=None):
+        """Transpose of the undisplaced velocity evaluation: vor_bar = K^T ubar (ubar = 3 tensors)."""
+        vor_bar = self.zeros_part() if vor_bar is None else vor_bar
+        self._check(self._lib.sbslpm_bve_velocity_adjoint(self.h, _ptr(area), *[_ptr(u) for u in ubar],
+                                                          _ptr(vor_bar)), "bve_velocity_adjoint")
+        return vor_bar
+
+    def bve_departure_to_target_adjoint(self, dep, new_vor_bar, vor_bar=None):
+        """Transpose of the remap interpolation: vor_bar += B^T new_vor_bar."""
+        vor_bar = self.zeros_part() if vor_bar is None else vor_bar
+        self._check(self._lib.sbslpm_bve_departure_to_target_adjoint(self.h, *[_ptr(d) for d in dep],
+                                                                     _ptr(new_vor_bar), _ptr(vor_bar)),
+                    "bve_departure_to_target_adjoint")
+        return vor_bar
+
     def swe_displacement(self, vel, tau, disp_part=None, disp_proxy=None):
         """swe_displacement_upward_pass, swe_time_step.cpp:265-276."""
         disp_part = self.zeros_part(3) if disp_part is None else disp_part

@@ -70,6 +70,16 @@ struct sbslpm_ctx {
   double* tr_next = nullptr;
   double* scratch = nullptr;  // direct sums, grown on demand
   int64_t scratch_len = 0;
+  // adjoint workspace, allocated on the first adjoint call
+  char* adj_arena = nullptr;
+  double* adj_ones = nullptr;
+  double* adj_c[3] = {};
+  double* adj_qp[3] = {};
+  double* adj_qx[3] = {};
+  double* adj_vp[3] = {};
+  double* adj_vx[3] = {};
+  double* adj_v[3] = {};
+  void* adj_rec = nullptr;
   Work w1;                     // the single-member workspace (aliases the arena buffers above)
   Work wens;                   // ensemble workspace, allocated on demand
   char* ens_arena = nullptr;
@@ -320,6 +330,7 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
   }
   if (c->scratch) hipFree(c->scratch);
   if (c->swe_arena) hipFree(c->swe_arena);
+  if (c->adj_arena) hipFree(c->adj_arena);
   if (c->ens_graph) hipGraphExecDestroy(c->ens_graph);
   if (c->ens_arena) hipFree(c->ens_arena);
   if (c->arena) hipFree(c->arena);
@@ -848,6 +859,62 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
   }
   HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
   return SBSLPM_OK;
+}
+
+// Adjoint workspace: ones, charges, upward-pass output, V and records.
+static int ensure_adjoint(sbslpm_ctx* c) {
+  if (c->adj_arena) return SBSLPM_OK;
+  const Tree& t = c->tree;
+  const size_t NP = (size_t)t.npart, Pn = (size_t)t.P * t.n;
+  Arena A;
+  const size_t o1 = A.take<double>(NP), oc = A.take<double>(3 * NP), oqp = A.take<double>(3 * NP),
+               oqx = A.take<double>(3 * Pn), ovp = A.take<double>(3 * NP), ovx = A.take<double>(3 * Pn),
+               ov = A.take<double>(3 * NP), orec = A.take<char>(t.U * gvec_record_bytes()), total = A.take<char>(256);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  char* b = nullptr;
+  if (hipMalloc(&b, total) != hipSuccess) return fail(c, SBSLPM_E_NOMEM, "adjoint workspace: out of device memory");
+  c->adj_arena = b;
+  c->adj_ones = (double*)(b + o1);
+  for (int f = 0; f < 3; ++f) {
+    c->adj_c[f] = (double*)(b + oc) + f * NP;
+    c->adj_qp[f] = (double*)(b + oqp) + f * NP;
+    c->adj_qx[f] = (double*)(b + oqx) + f * Pn;
+    c->adj_vp[f] = (double*)(b + ovp) + f * NP;
+    c->adj_vx[f] = (double*)(b + ovx) + f * Pn;
+    c->adj_v[f] = (double*)(b + ov) + f * NP;
+  }
+  c->adj_rec = b + orec;
+  launch_fill_value(c->adj_ones, (int64_t)NP, 1.0, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_velocity_adjoint(sbslpm_ctx* c, const double* area, const double* ubx, const double* uby,
+                                const double* ubz, double* vor_bar) {
+  if (!c || !area || !ubx || !uby || !ubz || !vor_bar) return fail(c, SBSLPM_E_ARG, "bve_velocity_adjoint: null pointer");
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  const Tree& t = c->tree;
+  const double* ub[3] = {ubx, uby, ubz};
+  launch_cross_charges(c->dt, ub, c->adj_c, c->stream);
+  for (int f = 0; f < 3; ++f) launch_upward_pass(c->dt, c->adj_ones, c->adj_c[f], c->adj_qp[f], c->adj_qx[f], c->stream);
+  launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 3, c->adj_vx, c->stream);
+  const double* cp[3] = {c->adj_qp[0], c->adj_qp[1], c->adj_qp[2]};
+  const double* cx[3] = {c->adj_qx[0], c->adj_qx[1], c->adj_qx[2]};
+  launch_gvec_interact64(c->dt, c->tiles, c->ntiles, c->sets, cp, cx, c->adj_rec, c->prm.kernel_eps, c->adj_vp,
+                         c->adj_vx, c->stream);
+  const double* vp[3] = {c->adj_vp[0], c->adj_vp[1], c->adj_vp[2]};
+  launch_downward_pass(c->dt, 3, c->adj_vx, vp, c->adj_v, c->stream, true);
+  const double* V[3] = {c->adj_v[0], c->adj_v[1], c->adj_v[2]};
+  launch_adjoint_finish(c->dt, area, V, vor_bar, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* c, const double* dep_x, const double* dep_y,
+                                           const double* dep_z, const double* new_vor_bar, double* vor_bar) {
+  if (!c || !dep_x || !dep_y || !dep_z || !new_vor_bar || !vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_departure_to_target_adjoint: null pointer");
+  launch_remap_adjoint(c->dt, dep_x, dep_y, dep_z, new_vor_bar, vor_bar, c->stream);
+  return launched(c);
 }
 
 // Ensemble workspace for nmem members (grown on demand, never shrunk).

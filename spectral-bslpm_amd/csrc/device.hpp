@@ -132,6 +132,7 @@ void launch_upward_fused(const DevTree& t, const double* area, const double* pot
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
                           double* const* sols, hipStream_t s, bool assign_leaf_rows = false, int nmem = 1);
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
+void launch_fill_value(double* p, int64_t count, double v, hipStream_t s);
 // Zeroes rows[0..nrows) (panel ids) of ncomp [P][n] arrays.
 void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s,
                       int nmem = 1, int64_t member_stride = 0);
@@ -171,6 +172,17 @@ void launch_source_records_swe(const DevTree& t, int variant, const double* qz_p
 void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRecSwe* rec, double eps, double* const* out_part, double* const* out_proxy,
                            hipStream_t s, bool assign = false);
+
+// ---- adjoints (adjoint.hip)
+void launch_cross_charges(const DevTree& t, const double* const* ubar, double* const* c, hipStream_t s);
+void launch_gvec_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                            const double* const* c_part, const double* const* c_proxy, void* rec, double eps,
+                            double* const* out_part, double* const* out_proxy, hipStream_t s);
+size_t gvec_record_bytes();
+void launch_adjoint_finish(const DevTree& t, const double* area, const double* const* V, double* wbar,
+                           hipStream_t s);
+void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
+                          double* wbar, hipStream_t s);
 
 // ---- evaluation at arbitrary targets (targets.hip)
 // kind 0: log kernel (psi), kind 1: Biot-Savart (3 components).  far[f] are the

@@ -258,6 +258,11 @@ __global__ void k_fill_zero(double* p, int64_t count) {
   if (u < count) p[u] = 0.0;
 }
 
+__global__ void k_fill_value(double* p, int64_t count, double v) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < count) p[u] = v;
+}
+
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t count) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u < count) dst[u] = src[u];
@@ -271,6 +276,10 @@ bool is_identity_p2m(const DevTree& t) { return t.m == t.dp1; }
 
 void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
   if (count > 0) hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
+}
+
+void launch_fill_value(double* p, int64_t count, double v, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_fill_value, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count, v);
 }
 
 void launch_zero_rows(const int* rows, int nrows, int n, int ncomp, double* const* arrays, hipStream_t s, int nmem,

@@ -1,0 +1,117 @@
+"""Adjoints of the BVE velocity evaluation and of the remap, and their
+torch.autograd wrappers.
+
+* velocity: the FMM adjoint against the transpose of the FP64 direct sum
+  (bve_vel_impl.hpp:90-103 summed over all particles, numpy), and the
+  dot-product test <K w, ubar> = <w, K^T ubar>, both to the FMM truncation;
+* remap: the dot-product test <B w, v> = <w, B^T v> to rounding;
+* autograd: directional derivatives of a scalar loss against central finite
+  differences along random directions."""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import to_gpu, to_np
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-4
+
+
+@pytest.fixture(scope="module")
+def setup(sb, torch_gpu):
+    print("\n[case adjoint L3 d8 m8]", flush=True)
+    ctx = sb.Context(sb.RunConfig(levels=3, interp_degree=8, leaf_degree=7, weights="clenshaw_curtis",
+                                  kernel_eps=EPS, omega=0.0, delta_t=3600.0))
+    x, y, z, a = ctx.spec_points()
+    return ctx, x, y, z, a
+
+
+def direct_adjoint(x, y, z, a, ub):
+    X = np.stack([v.ravel() for v in (x, y, z)], 1)
+    U = np.stack([v.ravel() for v in ub], 1)
+    Y = X / np.linalg.norm(X, axis=1, keepdims=True)
+    g = 1.0 / (1.0 - Y @ X.T + EPS)            # [s, t] = g(x_t, y_s)
+    c = np.cross(U, X)                          # ubar_t cross x_t
+    V = g @ c
+    return (-(a.ravel() / (4 * np.pi)) * np.sum(Y * V, 1)).reshape(x.shape)
+
+
+def test_velocity_adjoint_against_direct_transpose(setup, sb, torch_gpu):
+    ctx, x, y, z, a = setup
+    t = torch_gpu
+    rng = np.random.default_rng(3)
+    ub = [rng.normal(size=x.shape) for _ in range(3)]
+    got = to_np(ctx.bve_velocity_adjoint(to_gpu(t, a), [to_gpu(t, u) for u in ub]))
+    ctx.synchronize()
+    ref = direct_adjoint(x, y, z, a, ub)
+    err = sb.workloads.rel_l2(got, ref)
+    print(f"  adjoint vs direct transpose rel-L2 {err:.2e}")
+    assert err < 1e-6
+
+
+def test_velocity_dot_product(setup, sb, torch_gpu):
+    ctx, x, y, z, a = setup
+    t = torch_gpu
+    rng = np.random.default_rng(4)
+    w = sb.workloads.gaussian_vortices(x, y, z, a, seed=9)
+    ub = [rng.normal(size=x.shape) for _ in range(3)]
+    ga = to_gpu(t, a)
+    Kw = [to_np(v) for v in ctx.bve_velocity(ga, to_gpu(t, w))]
+    KTu = to_np(ctx.bve_velocity_adjoint(ga, [to_gpu(t, u) for u in ub]))
+    ctx.synchronize()
+    lhs = sum(float(np.sum(k * u)) for k, u in zip(Kw, ub))
+    rhs = float(np.sum(w * KTu))
+    print(f"  <Kw,u> {lhs:.12e}  <w,K^T u> {rhs:.12e}")
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+def test_remap_dot_product(setup, sb, torch_gpu):
+    ctx, x, y, z, a = setup
+    t = torch_gpu
+    rng = np.random.default_rng(5)
+    # departure points: a small rotation of the particles (all inside the sphere's panels)
+    ang = 0.02
+    c, s = np.cos(ang), np.sin(ang)
+    dep = [c * x - s * y, s * x + c * y, z.copy()]
+    w = rng.normal(size=x.shape)
+    v = rng.normal(size=x.shape)
+    gdep = [to_gpu(t, d) for d in dep]
+    Bw, _ = ctx.bve_departure_to_target(to_gpu(t, z), gdep, to_gpu(t, w))  # omega = 0: no Coriolis terms
+    BTv = ctx.bve_departure_to_target_adjoint(gdep, to_gpu(t, v))
+    ctx.synchronize()
+    lhs, rhs = float(np.sum(to_np(Bw) * v)), float(np.sum(w * to_np(BTv)))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+def test_autograd_matches_finite_differences(setup, sb, torch_gpu):
+    ctx, x, y, z, a = setup
+    t = torch_gpu
+    ag = importlib.import_module("spectral-bslpm_amd.autograd")
+    rng = np.random.default_rng(6)
+    ga, gz = to_gpu(t, a), to_gpu(t, z)
+    w0 = to_gpu(t, sb.workloads.gaussian_vortices(x, y, z, a, seed=10))
+    wts = [to_gpu(t, rng.normal(size=x.shape)) for _ in range(3)]
+    ang = 0.015
+    c, s = np.cos(ang), np.sin(ang)
+    dep = [to_gpu(t, d) for d in (c * x - s * y, s * x + c * y, z.copy())]
+
+    def loss(w):
+        vel = ag.bve_velocity(ctx, ga, w)
+        nw = ag.remap(ctx, gz, dep, w)
+        return sum((v * q).sum() for v, q in zip(vel, wts)) + 0.5 * (nw * nw).sum()
+
+    w = w0.clone().requires_grad_(True)
+    L = loss(w)
+    L.backward()
+    grad = w.grad.detach()
+    for k in range(4):
+        d = to_gpu(t, rng.normal(size=x.shape))
+        h = 1e-3
+        with t.no_grad():
+            fd = (loss(w0 + h * d) - loss(w0 - h * d)) / (2 * h)
+        an = (grad * d).sum()
+        rel = abs(float(fd - an)) / abs(float(fd))
+        print(f"  direction {k}: fd {float(fd):.10e} adjoint {float(an):.10e} rel {rel:.2e}")
+        assert rel < 1e-6
