@@ -225,13 +225,92 @@ def bench_ensemble(args, torch, sb):
     print(json.dumps(line), flush=True)
 
 
+def bench_adjoint(args, torch, sb):
+    """Gradient of one BVE step at 6x512^2 from the hand-written reverse pass
+    (sbslpm_bve_back_rk4_step_adjoint: the step's forward recomputed inside,
+    then the FP64 reverse pass), against the forward step itself (hipGraph
+    replay, FP64 and mixed), with one central finite-difference check of the
+    gradient along a random direction at this size."""
+    W = sb.workloads
+    cfgd = CONFIGS["c3"]
+    stream = torch.cuda.Stream()
+    common = dict(weights="clenshaw_curtis", kernel_eps=1e-4, delta_t=3600.0, omega=7.292e-5,
+                  rk4_combine="reference", **cfgd)
+    with torch.cuda.stream(stream):
+        ctx64 = sb.Context(sb.RunConfig(precision="fp64", use_graph=not args.no_graph, **common))
+        ctxmx = sb.Context(sb.RunConfig(precision="mixed", use_graph=not args.no_graph, **common))
+    x, y, z, a = ctx64.spec_points()
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    vor_h = W.gaussian_vortices(x, y, z, a, count=12, seed=0)
+    rng = np.random.default_rng(1)
+    gx, gy, gz, ga, gvor0, r = [dev(v) for v in (x, y, z, a, vor_h, rng.normal(size=x.shape))]
+    _, ev64, _, _ = time_steps(ctx64, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
+    _, evmx, _, _ = time_steps(ctxmx, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
+    fwd64, fwdmx = ev64 / args.steps, evmx / args.steps
+    log(f"forward step: fp64 {1e3 * fwd64:.2f} ms, mixed {1e3 * fwdmx:.2f} ms")
+
+    def time_adjoint(ctx):
+        out = ctx.zeros_part()
+        with torch.cuda.stream(stream):
+            for _ in range(max(args.warmup, 1)):
+                ctx.bve_back_rk4_step_adjoint(gx, gy, gz, ga, gvor0, r, out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            for _ in range(args.steps):
+                ctx.bve_back_rk4_step_adjoint(gx, gy, gz, ga, gvor0, r, out)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        ctx.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / args.steps, out
+
+    adj64, grad = time_adjoint(ctx64)
+    adjmx, gradmx = time_adjoint(ctxmx)
+    log(f"step adjoint: fp64 forward {1e3 * adj64:.2f} ms, mixed forward {1e3 * adjmx:.2f} ms")
+
+    # central finite difference of L(w) = sum r * step(w) along one random direction
+    d = dev(rng.normal(size=x.shape))
+    h = 1e-5 * float(gvor0.norm()) / float(d.norm())
+    vel = ctx64.zeros_part(3)
+
+    def loss(w):
+        v = w.clone()
+        with torch.cuda.stream(stream):
+            ctx64.bve_back_rk4_step(gx, gy, gz, ga, v, vel)
+        torch.cuda.synchronize()
+        return float((r * v).sum())
+
+    fd = (loss(gvor0 + h * d) - loss(gvor0 - h * d)) / (2 * h)
+    an = float((grad * d).sum())
+    fd_rel = abs(fd - an) / abs(fd)
+    mixed_vs_fp64 = W.rel_l2(gradmx.cpu().numpy(), grad.cpu().numpy())
+    log(f"finite difference {fd:.10e} adjoint {an:.10e} rel {fd_rel:.2e}; gradient mixed vs fp64 {mixed_vs_fp64:.2e}")
+    s = ctx64.sizes
+    line = {"metric": "BVE step gradient (hand-written adjoint) cost / forward step cost at 6x512^2",
+            "value": round(adj64 / fwd64, 4), "unit": "x forward step (fp64)", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * adj64, 3), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 reverse pass",
+            "data": "synthetic (12 seeded Gaussian vortices, seeded cotangent)",
+            "config": {"workload": "BVE RK4 step gradient: 6x512^2 particles", "particles": s.N,
+                       "levels": cfgd["levels"], "proxy_degree": cfgd["interp_degree"],
+                       "leaf_grid": cfgd["leaf_degree"] + 1, "theta": cfgd["fmm_theta"]},
+            "forward_fp64_ms": round(1e3 * fwd64, 3), "forward_mixed_ms": round(1e3 * fwdmx, 3),
+            "adjoint_fp64_forward_ms": round(1e3 * adj64, 3), "adjoint_mixed_forward_ms": round(1e3 * adjmx, 3),
+            "adjoint_over_mixed_forward": round(adjmx / fwdmx, 4),
+            "fd_check": {"rel_err": float(f"{fd_rel:.3e}"), "relative_step": 1e-5},
+            "gradient_mixed_vs_fp64_rel_l2": float(f"{mixed_vs_fp64:.3e}")}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4"],
-                    help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c4: the 48-member ensemble")
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4", "adjoint"],
+                    help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c4: the 48-member ensemble, "
+                         "adjoint: cost of the step gradient at 6x512^2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--valu", action="store_true", help="also time the mixed-precision vector-unit pair kernel")
@@ -247,6 +326,9 @@ def main():
     sb = importlib.import_module("spectral-bslpm_amd")
     if args.config == "c4":
         bench_ensemble(args, torch, sb)
+        return
+    if args.config == "adjoint":
+        bench_adjoint(args, torch, sb)
         return
     W = sb.workloads
     cfgd = CONFIGS[args.config]

@@ -234,14 +234,28 @@ int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* ctx, int nmem, const double* x
 
 /* Adjoint (transpose) of the Biot-Savart velocity evaluation with undisplaced sources
  * (sbslpm_bve_velocity with NULL displacements, the first RK4 stage): for a cotangent
- * ubar_{x,y,z} [Pl][m2] of the velocity, vor_bar [Pl][m2] is assigned
+ * ubar_{x,y,z} [Pl][m2] of the velocity, vor_bar [Pl][m2] is assigned K^T ubar, K the FP64 FMM
+ * operator of the context (tree, interaction list, upward and downward passes) -- its exact
+ * transpose, which approximates
  *   vor_bar_s = -(area_s / (4 pi)) x_s . sum_t g(x_s, x_t) (ubar_t cross x_t),
- *   g = 1/(1 - x.y + eps),
- * evaluated with the same tree, interaction list and passes (FP64).  No reference
- * counterpart; it is the transpose of the operator of bve_vel_impl.hpp:90-103 up to
- * the FMM truncation. */
+ *   g = 1/(1 - x.y + eps)
+ * to the FMM truncation.  No reference counterpart; it transposes the operator of
+ * bve_vel_interactions (bve_vel.cpp:9-37, bve_vel_impl.hpp:71-106) with the passes of
+ * fmm_funcs.cpp:341-392. */
 int sbslpm_bve_velocity_adjoint(sbslpm_ctx* ctx, const double* area, const double* ubar_x, const double* ubar_y,
                                 const double* ubar_z, double* vor_bar);
+/* Gradient of one bve_back_rk4_step (bve_time_step.cpp:311-419, no tracers) with respect to the
+ * vorticity before the step: vor_bar [Pl][m2] is assigned J^T new_vor_bar, J the Jacobian of
+ * vor -> vor after the step at the state vor (which is not modified).  The step's forward pass
+ * is recomputed inside in the context's precision; the reverse pass is FP64 and differentiates
+ * the four FMM velocity evaluations in the charges and the source displacements, the
+ * displacement updates, the backward RK4 departure search (BLI interpolation differentiated in
+ * the point, re-projections, the configured combine rule) and the remap with its Coriolis
+ * terms.  Point location is piecewise constant and contributes nothing.  No reference
+ * counterpart (the reference has no adjoint); for 4D-Var style gradients through many steps. */
+int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                     const double* area, const double* vor, const double* new_vor_bar,
+                                     double* vor_bar);
 /* Transpose of the remap interpolation of bve_departure_to_target (bve_time_step.cpp:38-66)
  * with respect to the field: vor_bar [Pl][m2] += B^T new_vor_bar, B the BLI weights at the
  * departure points.  Accumulated with FP64 atomic adds (summation order not fixed). */

@@ -372,10 +372,21 @@ class Context:
         return vors
 
     def bve_velocity_adjoint(self, area, ubar, vor_bar=None):
-        """Transpose of the undisplaced velocity evaluation: vor_bar = K^T ubar (ubar = 3 tensors)."""
+        """Transpose of the undisplaced velocity evaluation (exact transpose of the FP64 FMM operator):
+        vor_bar = K^T ubar (ubar = 3 tensors)."""
         vor_bar = self.zeros_part() if vor_bar is None else vor_bar
         self._check(self._lib.sbslpm_bve_velocity_adjoint(self.h, _ptr(area), *[_ptr(u) for u in ubar],
                                                           _ptr(vor_bar)), "bve_velocity_adjoint")
+        return vor_bar
+
+    def bve_back_rk4_step_adjoint(self, x, y, z, area, vor, new_vor_bar, vor_bar=None):
+        """Gradient of one bve_back_rk4_step with respect to the vorticity before it: vor_bar =
+        (d new_vor / d vor)^T new_vor_bar at the state vor (not modified; the step's forward
+        pass is recomputed inside)."""
+        vor_bar = self.zeros_part() if vor_bar is None else vor_bar
+        self._check(self._lib.sbslpm_bve_back_rk4_step_adjoint(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area),
+                                                               _ptr(vor), _ptr(new_vor_bar), _ptr(vor_bar)),
+                    "bve_back_rk4_step_adjoint")
         return vor_bar
 
     def bve_departure_to_target_adjoint(self, dep, new_vor_bar, vor_bar=None):

@@ -1,6 +1,6 @@
 """torch.autograd.Function wrappers whose backward passes are the hand-written
 adjoint kernels of the library (sbslpm_bve_velocity_adjoint,
-sbslpm_bve_departure_to_target_adjoint).
+sbslpm_bve_departure_to_target_adjoint, sbslpm_bve_back_rk4_step_adjoint).
 
 * ``bve_velocity(ctx, area, vor)`` -- the Biot-Savart velocity of the
   vorticity with undisplaced sources (the first RK4 stage of
@@ -9,9 +9,13 @@ sbslpm_bve_departure_to_target_adjoint).
   (bve_time_step.cpp:38-66) at given departure points; differentiable in
   ``vor`` (the departure points are constants of this function).
 
-Both are linear in the differentiated argument, so their Jacobian-vector and
-vector-Jacobian products are exact up to the FMM truncation (velocity) or
-rounding (remap).
+* ``bve_steps(ctx, x, y, z, area, vor0, nsteps, every)`` -- ``nsteps``
+  bve_back_rk4_step calls (bve_time_step.cpp:311-419) from ``vor0``;
+  differentiable in ``vor0`` through the full nonlinear step (trajectories
+  included), checkpointed every ``every`` steps.
+
+The first two are linear in the differentiated argument; their backward
+passes are the exact transposes of the library's FP64 operators.
 """
 import torch
 
@@ -55,3 +59,51 @@ def bve_velocity(sbctx, area, vor):
 def remap(sbctx, z, dep, vor):
     """new_vor = B(dep) vor + 2 Omega (dep_z - z); backward: vor_bar = B(dep)^T new_vor_bar."""
     return _Remap.apply(sbctx, z, dep[0], dep[1], dep[2], vor)
+
+
+class _BVESteps(torch.autograd.Function):
+    """Forward: nsteps steps, keeping the state at every `every`-th step (the
+    checkpoints).  Backward: segment by segment from the last, the states
+    inside the segment are recomputed from its checkpoint, then the step
+    adjoints run in reverse order.  Memory: nsteps/every + every states; work:
+    one extra forward step per step plus the step adjoints (each of which
+    recomputes its own step's stage velocities)."""
+
+    @staticmethod
+    def forward(fctx, sbctx, x, y, z, area, vor0, nsteps, every):
+        vel = sbctx.zeros_part(3)
+        v = vor0.detach().clone()
+        checkpoints = [v.clone()]
+        for n in range(nsteps):
+            sbctx.bve_back_rk4_step(x, y, z, area, v, vel)
+            if (n + 1) % every == 0 and n + 1 < nsteps:
+                checkpoints.append(v.clone())
+        fctx.sbctx, fctx.nsteps, fctx.every = sbctx, nsteps, every
+        fctx.geom = (x, y, z, area)
+        fctx.checkpoints = checkpoints
+        return v
+
+    @staticmethod
+    def backward(fctx, g):
+        sbctx, nsteps, every = fctx.sbctx, fctx.nsteps, fctx.every
+        x, y, z, area = fctx.geom
+        vel = sbctx.zeros_part(3)
+        gbar = g.contiguous()
+        for seg in reversed(range(len(fctx.checkpoints))):
+            start, end = seg * every, min((seg + 1) * every, nsteps)
+            states = [fctx.checkpoints[seg]]
+            v = states[0].clone()
+            for _ in range(start, end - 1):
+                sbctx.bve_back_rk4_step(x, y, z, area, v, vel)
+                states.append(v.clone())
+            for n in reversed(range(start, end)):
+                gbar = sbctx.bve_back_rk4_step_adjoint(x, y, z, area, states[n - start], gbar)
+        return None, None, None, None, None, gbar, None, None
+
+
+def bve_steps(sbctx, x, y, z, area, vor0, nsteps, every=1):
+    """Vorticity after nsteps backward RK4 steps; backward: the hand-written step
+    adjoints with checkpoints every `every` steps."""
+    if nsteps < 1 or every < 1:
+        raise ValueError("nsteps and every must be positive")
+    return _BVESteps.apply(sbctx, x, y, z, area, vor0, nsteps, every)

@@ -70,16 +70,26 @@ struct sbslpm_ctx {
   double* tr_next = nullptr;
   double* scratch = nullptr;  // direct sums, grown on demand
   int64_t scratch_len = 0;
-  // adjoint workspace, allocated on the first adjoint call
+  // adjoint workspace and source-major interaction list, allocated on the
+  // first adjoint call
   char* adj_arena = nullptr;
+  TargetTile* adj_tiles = nullptr;
+  SourceSet* adj_targets = nullptr;
+  int* adj_zero_rows = nullptr;
+  int adj_ntiles = 0;
+  int adj_nzero = 0;
   double* adj_ones = nullptr;
-  double* adj_c[3] = {};
-  double* adj_qp[3] = {};
-  double* adj_qx[3] = {};
-  double* adj_vp[3] = {};
-  double* adj_vx[3] = {};
-  double* adj_v[3] = {};
-  void* adj_rec = nullptr;
+  double* adj_psip[3] = {};  // particle copies of the cotangent (upward-pass output)
+  double* adj_psix[3] = {};  // its anterpolation to the proxy rows
+  void* adj_trec = nullptr;  // target records
+  SrcRec64* adj_srec = nullptr;
+  double* adj_qbp = nullptr;  // charge cotangent at source rows: particles, proxies
+  double* adj_qbx = nullptr;
+  double* adj_qb = nullptr;  // dL/dq_part after the downward pass
+  double* adj_dbp[3] = {};   // displacement cotangent: particles, proxies
+  double* adj_dbx[3] = {};
+  double* adj_v[12] = {};    // stage velocities v0..v3 of the step adjoint
+  double* adj_vb[12] = {};   // and their cotangents
   Work w1;                     // the single-member workspace (aliases the arena buffers above)
   Work wens;                   // ensemble workspace, allocated on demand
   char* ens_arena = nullptr;
@@ -861,30 +871,82 @@ int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
   return SBSLPM_OK;
 }
 
-// Adjoint workspace: ones, charges, upward-pass output, V and records.
+// Adjoint workspace and the source-major interaction list (built on the host
+// on the first adjoint call).
 static int ensure_adjoint(sbslpm_ctx* c) {
   if (c->adj_arena) return SBSLPM_OK;
   const Tree& t = c->tree;
+  SourceMajor sm;
+  build_source_major(t, sm);
   const size_t NP = (size_t)t.npart, Pn = (size_t)t.P * t.n;
   Arena A;
-  const size_t o1 = A.take<double>(NP), oc = A.take<double>(3 * NP), oqp = A.take<double>(3 * NP),
-               oqx = A.take<double>(3 * Pn), ovp = A.take<double>(3 * NP), ovx = A.take<double>(3 * Pn),
-               ov = A.take<double>(3 * NP), orec = A.take<char>(t.U * gvec_record_bytes()), total = A.take<char>(256);
+  const size_t o_tiles = A.take<TargetTile>(sm.tiles.size()), o_tg = A.take<SourceSet>(sm.targets.size()),
+               o_zr = A.take<int>(sm.zero_rows.size()), o1 = A.take<double>(NP), o_pp = A.take<double>(3 * NP),
+               o_px = A.take<double>(3 * Pn), o_tr = A.take<char>(t.U * adj_target_record_bytes()),
+               o_sr = A.take<SrcRec64>(t.U), o_qbp = A.take<double>(NP), o_qbx = A.take<double>(Pn),
+               o_qb = A.take<double>(NP), o_dbp = A.take<double>(3 * NP), o_dbx = A.take<double>(3 * Pn),
+               o_v = A.take<double>(12 * NP), o_vb = A.take<double>(12 * NP), total = A.take<char>(256);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   char* b = nullptr;
   if (hipMalloc(&b, total) != hipSuccess) return fail(c, SBSLPM_E_NOMEM, "adjoint workspace: out of device memory");
   c->adj_arena = b;
+  c->adj_tiles = (TargetTile*)(b + o_tiles);
+  c->adj_targets = (SourceSet*)(b + o_tg);
+  c->adj_zero_rows = (int*)(b + o_zr);
+  c->adj_ntiles = (int)sm.tiles.size();
+  c->adj_nzero = (int)sm.zero_rows.size();
   c->adj_ones = (double*)(b + o1);
+  c->adj_trec = b + o_tr;
+  c->adj_srec = (SrcRec64*)(b + o_sr);
+  c->adj_qbp = (double*)(b + o_qbp);
+  c->adj_qbx = (double*)(b + o_qbx);
+  c->adj_qb = (double*)(b + o_qb);
   for (int f = 0; f < 3; ++f) {
-    c->adj_c[f] = (double*)(b + oc) + f * NP;
-    c->adj_qp[f] = (double*)(b + oqp) + f * NP;
-    c->adj_qx[f] = (double*)(b + oqx) + f * Pn;
-    c->adj_vp[f] = (double*)(b + ovp) + f * NP;
-    c->adj_vx[f] = (double*)(b + ovx) + f * Pn;
-    c->adj_v[f] = (double*)(b + ov) + f * NP;
+    c->adj_psip[f] = (double*)(b + o_pp) + f * NP;
+    c->adj_psix[f] = (double*)(b + o_px) + f * Pn;
+    c->adj_dbp[f] = (double*)(b + o_dbp) + f * NP;
+    c->adj_dbx[f] = (double*)(b + o_dbx) + f * Pn;
   }
-  c->adj_rec = b + orec;
+  for (int i = 0; i < 12; ++i) {
+    c->adj_v[i] = (double*)(b + o_v) + i * NP;
+    c->adj_vb[i] = (double*)(b + o_vb) + i * NP;
+  }
+  int rc = upload(c, c->adj_tiles, sm.tiles.data(), sm.tiles.size() * sizeof(TargetTile));
+  if (!rc) rc = upload(c, c->adj_targets, sm.targets.data(), sm.targets.size() * sizeof(SourceSet));
+  if (!rc) rc = upload(c, c->adj_zero_rows, sm.zero_rows.data(), sm.zero_rows.size() * sizeof(int));
+  if (rc) return rc;
   launch_fill_value(c->adj_ones, (int64_t)NP, 1.0, c->stream);
+  return launched(c);
+}
+
+// Exact transpose of velocity() in FP64 for the particle-velocity cotangent
+// vbar: assigns adj_qb = dL/dq_part (q_part = area * vor_eff) and, when the
+// sources are displaced (dpart, dprox given), adj_dbp = dL/d disp_part
+// including the path through the interpolated proxy displacements.  The
+// charges (area, vor_eff) enter only the displacement part.
+static int fmm_adjoint(sbslpm_ctx* c, const double* area, const double* vor_eff, const double* const* dpart,
+                const double* const* dprox, const double* const* vbar) {
+  const Tree& t = c->tree;
+  const bool pos = dpart != nullptr;
+  for (int f = 0; f < 3; ++f)
+    launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
+  const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
+  launch_adj_target_records(c->dt, vbar, psix, c->adj_trec, c->stream);
+  // source positions (and, for the displacement part, charges) as the forward records
+  if (pos) launch_upward_fused(c->dt, area, vor_eff, c->qpart, c->qproxy, c->stream);
+  launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->adj_srec, c->stream);
+  launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 1, &c->adj_qbx, c->stream);
+  if (pos) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, c->adj_dbx, c->stream);
+  launch_bve_adjoint64(c->dt, c->adj_tiles, c->adj_ntiles, c->adj_targets, c->adj_trec, c->adj_srec,
+                       c->prm.kernel_eps, dpart, dprox, c->adj_qbp, c->adj_qbx, c->adj_dbp, c->adj_dbx, c->stream);
+  // charges: particle sources directly, proxy sources through P2M/M2M, whose
+  // transposes are L2L/L2P
+  const double* qbp[1] = {c->adj_qbp};
+  launch_downward_pass(c->dt, 1, &c->adj_qbx, qbp, &c->adj_qb, c->stream, true);
+  if (pos) {
+    const double* dbx[3] = {c->adj_dbx[0], c->adj_dbx[1], c->adj_dbx[2]};
+    launch_disp_interp_adjoint(c->dt, dbx, c->adj_dbp, c->stream);
+  }
   return launched(c);
 }
 
@@ -893,19 +955,62 @@ int sbslpm_bve_velocity_adjoint(sbslpm_ctx* c, const double* area, const double*
   if (!c || !area || !ubx || !uby || !ubz || !vor_bar) return fail(c, SBSLPM_E_ARG, "bve_velocity_adjoint: null pointer");
   int rc = ensure_adjoint(c);
   if (rc) return rc;
-  const Tree& t = c->tree;
   const double* ub[3] = {ubx, uby, ubz};
-  launch_cross_charges(c->dt, ub, c->adj_c, c->stream);
-  for (int f = 0; f < 3; ++f) launch_upward_pass(c->dt, c->adj_ones, c->adj_c[f], c->adj_qp[f], c->adj_qx[f], c->stream);
-  launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 3, c->adj_vx, c->stream);
-  const double* cp[3] = {c->adj_qp[0], c->adj_qp[1], c->adj_qp[2]};
-  const double* cx[3] = {c->adj_qx[0], c->adj_qx[1], c->adj_qx[2]};
-  launch_gvec_interact64(c->dt, c->tiles, c->ntiles, c->sets, cp, cx, c->adj_rec, c->prm.kernel_eps, c->adj_vp,
-                         c->adj_vx, c->stream);
-  const double* vp[3] = {c->adj_vp[0], c->adj_vp[1], c->adj_vp[2]};
-  launch_downward_pass(c->dt, 3, c->adj_vx, vp, c->adj_v, c->stream, true);
-  const double* V[3] = {c->adj_v[0], c->adj_v[1], c->adj_v[2]};
-  launch_adjoint_finish(c->dt, area, V, vor_bar, c->stream);
+  rc = fmm_adjoint(c, area, nullptr, nullptr, nullptr, ub);
+  if (rc) return rc;
+  launch_fill_zero(vor_bar, c->tree.npart, c->stream);
+  launch_stage_adjoint(c->dt, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0, area, c->adj_qb, nullptr, vor_bar,
+                       nullptr, c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                     const double* area, const double* vor, const double* new_vor_bar,
+                                     double* vor_bar) {
+  if (!c || !x || !y || !z || !area || !vor || !new_vor_bar || !vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint: null pointer");
+  if (vor_bar == vor || vor_bar == new_vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint: vor_bar must not alias the inputs");
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  const Tree& t = c->tree;
+  const Work& w = c->w1;
+  const double dt = c->prm.delta_t, om = c->prm.omega;
+  const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
+  double* const* V = c->adj_v;
+  double* const* VB = c->adj_vb;
+  const double* cdp[3] = {w.disppart[0], w.disppart[1], w.disppart[2]};
+  const double* cdx[3] = {w.dispproxy[0], w.dispproxy[1], w.dispproxy[2]};
+  // forward, keeping the four stage velocities
+  rc = velocity(c, w, 1, area, vor, nullptr, nullptr, V);
+  for (int s = 0; s < 3 && rc == SBSLPM_OK; ++s) {
+    const double* vin[3] = {V[3 * s], V[3 * s + 1], V[3 * s + 2]};
+    launch_bve_displacement(c->dt, x, y, z, vin, vor, taus[s], om, w.newvor, w.disppart, w.dispproxy, c->stream);
+    rc = velocity(c, w, 1, area, w.newvor, cdp, cdx, V + 3 * (s + 1));
+  }
+  if (rc) return rc;
+  // reverse: remap and departure search
+  launch_fill_zero(vor_bar, t.npart, c->stream);
+  launch_fill_zero(VB[0], 9 * t.npart, c->stream);  // v0..v2 cotangents (contiguous)
+  const double* cv[12];
+  for (int i = 0; i < 12; ++i) cv[i] = V[i];
+  launch_departure_adjoint(c->dt, x, y, z, cv, vor, new_vor_bar, dt, c->prm.rk4_combine, om, VB, vor_bar, c->stream);
+  // stages 3, 2, 1: velocity at displaced sources, then the displacement update
+  for (int s = 2; s >= 0 && rc == SBSLPM_OK; --s) {
+    const double* vin[3] = {V[3 * s], V[3 * s + 1], V[3 * s + 2]};
+    launch_bve_displacement(c->dt, x, y, z, vin, vor, taus[s], om, w.newvor, w.disppart, w.dispproxy, c->stream);
+    const double* vb[3] = {VB[3 * (s + 1)], VB[3 * (s + 1) + 1], VB[3 * (s + 1) + 2]};
+    rc = fmm_adjoint(c, area, w.newvor, cdp, cdx, vb);
+    const double* db[3] = {c->adj_dbp[0], c->adj_dbp[1], c->adj_dbp[2]};
+    launch_stage_adjoint(c->dt, x, y, z, vin, taus[s], om, area, c->adj_qb, db, vor_bar, VB + 3 * s, c->stream);
+  }
+  if (rc) return rc;
+  // stage 0: undisplaced sources
+  const double* vb0[3] = {VB[0], VB[1], VB[2]};
+  rc = fmm_adjoint(c, area, nullptr, nullptr, nullptr, vb0);
+  if (rc) return rc;
+  launch_stage_adjoint(c->dt, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0, area, c->adj_qb, nullptr, vor_bar, nullptr,
+                       c->stream);
   return launched(c);
 }
 

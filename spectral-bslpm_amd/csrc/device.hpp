@@ -173,16 +173,29 @@ void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tile
                            const SrcRecSwe* rec, double eps, double* const* out_part, double* const* out_proxy,
                            hipStream_t s, bool assign = false);
 
-// ---- adjoints (adjoint.hip)
-void launch_cross_charges(const DevTree& t, const double* const* ubar, double* const* c, hipStream_t s);
-void launch_gvec_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                            const double* const* c_part, const double* const* c_proxy, void* rec, double eps,
-                            double* const* out_part, double* const* out_proxy, hipStream_t s);
-size_t gvec_record_bytes();
-void launch_adjoint_finish(const DevTree& t, const double* area, const double* const* V, double* wbar,
-                           hipStream_t s);
+// ---- adjoints (adjoint.hip: FMM transpose; remap_adjoint.hip: remap side)
+size_t adj_target_record_bytes();
+// Target records (x_t, psi_t cross x_t) over the unified index space.
+void launch_adj_target_records(const DevTree& t, const double* const* psi_part, const double* const* psi_prox,
+                               void* rec, hipStream_t s);
+// Transposed pair sums over the source-major list: assigns qbar (dL/dq, every
+// source row) and, when dpart is given, dbar (dL/d displacement) of every
+// source row.
+void launch_bve_adjoint64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
+                          const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
+                          const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
+                          double* const* dbar_prox, hipStream_t s);
 void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
                           double* wbar, hipStream_t s);
+void launch_departure_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                              const double* const* vels /*12*/, const double* vor, const double* new_vor_bar,
+                              double dt, int combine, double omega, double* const* vel_bar /*12*/, double* vor_bar,
+                              hipStream_t s);
+void launch_disp_interp_adjoint(const DevTree& t, const double* const* dbar_proxy, double* const* dbar_part,
+                                hipStream_t s);
+void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                          const double* const* vel, double tau, double omega, const double* area, const double* qbar,
+                          const double* const* dbar, double* vor_bar, double* const* vbar, hipStream_t s);
 
 // ---- evaluation at arbitrary targets (targets.hip)
 // kind 0: log kernel (psi), kind 1: Biot-Savart (3 components).  far[f] are the

@@ -10,10 +10,12 @@
 // particle and stage, one leaf tile of three velocity components
 // (3 * M^2 * 8 bytes) is read, mostly from L2 because neighbouring particles
 // share leaves.
-#include "device.hpp"
+#include "bli.hpp"
 
 namespace sbslpm {
 namespace {
+
+using namespace bli;
 
 struct CPtr3 {
   const double* p[3];
@@ -24,55 +26,6 @@ struct Ptr3 {
 struct Vel12 {
   const double* p[12];  // stage velocities v0x v0y v0z v1x .. v3z
 };
-
-__device__ inline void flag(const DevTree& t, int bit) { atomicOr(t.err, bit); }
-
-// 1-D barycentric weights, compile-time degree M-1 (see bli_weights_1d).
-template <int M>
-__device__ inline bool bli_w(double t, double lo, double hi, const double* __restrict__ cheb, double* w) {
-  constexpr int D = M - 1;
-  const double range = 0.5 * (hi - lo), off = 0.5 * (hi + lo);
-  const bool ok = (t >= lo - kSnapTol) && (t <= hi + kSnapTol);
-  double node[M];
-#pragma unroll
-  for (int i = 0; i < M; ++i) node[i] = cheb[i] * range + off;
-  int hit = -1;
-#pragma unroll
-  for (int i = M - 1; i >= 0; --i)
-    if (fabs(t - node[i]) < kSnapTol) hit = i;
-  if (hit >= 0) {
-#pragma unroll
-    for (int i = 0; i < M; ++i) w[i] = (i == hit) ? 1.0 : 0.0;
-    return ok;
-  }
-  double den = 0;
-#pragma unroll
-  for (int i = 0; i < M; ++i) {
-    double bw = (i == 0 || i == D) ? 0.5 : 1.0;
-    if (i & 1) bw = -bw;
-    w[i] = bw * rcp64(t - node[i]);  // |t - node| >= 1e-15 here
-    den += w[i];
-  }
-  const double inv = 1.0 / den;
-#pragma unroll
-  for (int i = 0; i < M; ++i) w[i] *= inv;
-  return ok;
-}
-
-// sum_a wx[a] sum_c we[c] f[a*M + c]: the reference's tensor basis
-// (bve_time_step.cpp:222-237 sums f * (wx_a * we_c)) applied one direction at a time.
-template <int M>
-__device__ inline double tensor_gather(const double* __restrict__ f, const double* wx, const double* we) {
-  double out = 0;
-#pragma unroll
-  for (int a = 0; a < M; ++a) {
-    double s = 0;
-#pragma unroll
-    for (int c = 0; c < M; ++c) s = fma(f[a * M + c], we[c], s);
-    out = fma(wx[a], s, out);
-  }
-  return out;
-}
 
 // Interpolate NF leaf fields [Pl][M*M] at label point (xi, eta) of leaf `leaf`.
 template <int M, int NF>
@@ -152,15 +105,6 @@ __device__ inline void vel_at(const DevTree& t, double x, double y, double z, co
     return;
   }
   if (!interp_leaf<M, 3>(t, leaf, xe[0], xe[1], f, k)) flag(t, kErrBliBounds);
-}
-
-// project_to_sphere (general_utils_impl.hpp:179-185) with one reciprocal of
-// the norm instead of three divisions (each result within an ulp).
-__device__ inline void project_rcp(double& x, double& y, double& z) {
-  const double r = rcp64(sqrt(x * x + y * y + z * z));
-  x *= r;
-  y *= r;
-  z *= r;
 }
 
 // bve_find_departure_points, bve_time_step.cpp:206-308.
@@ -272,24 +216,6 @@ __global__ __launch_bounds__(128) void k_dep_to_target(DevTree t, const double* 
       s = fma(wx[a], r, s);
     }
     new_tracers[u * ntr + k] = s;
-  }
-}
-
-inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
-
-template <template <int> class F, typename... A>
-void dispatch_m(int m, A&&... a) {
-  switch (m) {
-    case 2: F<2>::run(a...); break;
-    case 3: F<3>::run(a...); break;
-    case 4: F<4>::run(a...); break;
-    case 5: F<5>::run(a...); break;
-    case 6: F<6>::run(a...); break;
-    case 7: F<7>::run(a...); break;
-    case 8: F<8>::run(a...); break;
-    case 9: F<9>::run(a...); break;
-    case 10: F<10>::run(a...); break;
-    default: F<11>::run(a...); break;
   }
 }
 

@@ -1,0 +1,333 @@
+// remap_adjoint.hip -- reverse-mode kernels of the BVE step's remap side
+// (bve_time_step.cpp:15-309), FP64:
+//   * k_remap_adj       -- transpose of the remap interpolation in the field;
+//   * k_departure_adj   -- the backward RK4 departure search and the remap,
+//                          differentiated in the stage velocities, the field
+//                          and the departure points (recomputed per thread);
+//   * k_disp_interp_adj -- transpose of the proxy-displacement interpolation;
+//   * k_stage_adj       -- reverse of the per-stage displacement update.
+// Point location is piecewise constant and contributes nothing; the BLI
+// interpolant is differentiated in its label coordinates (bli_wd) and those
+// in the point (xieta_gradient), the re-projections through
+// project_adjoint.  Transposed interpolations scatter into the leaf nodes
+// with FP64 atomic adds (summation order not fixed).
+#include "bli.hpp"
+
+namespace sbslpm {
+namespace {
+
+using namespace bli;
+
+struct CPtr3 {
+  const double* p[3];
+};
+struct Ptr3 {
+  double* p[3];
+};
+struct CVel12 {
+  const double* p[12];
+};
+struct Vel12 {
+  double* p[12];
+};
+
+// field_bar[leaf nodes] += (wx outer we) * g
+template <int M>
+__device__ inline void scatter_leaf(double* __restrict__ dst, const double* wx, const double* we, double g) {
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    const double ga = wx[a] * g;
+#pragma unroll
+    for (int c = 0; c < M; ++c) atomicAdd(dst + a * M + c, ga * we[c]);
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(128) void k_remap_adj(DevTree t, const double* __restrict__ dx,
+                                                   const double* __restrict__ dy, const double* __restrict__ dz,
+                                                   const double* __restrict__ newbar, double* __restrict__ wbar) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.npart) return;
+  double xe[2];
+  const int leaf = locate_leaf(t, dx[u], dy[u], dz[u], xe);
+  if (leaf < 0) {
+    flag(t, kErrLocate);
+    return;
+  }
+  const DevPanel& P = t.panels[leaf];
+  double wx[M], we[M];
+  bool ok = bli_w<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx);
+  ok = bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we) && ok;
+  if (!ok) flag(t, kErrBliBounds);
+  scatter_leaf<M>(wbar + (int64_t)(leaf - t.offset) * (M * M), wx, we, newbar[u]);
+}
+
+// Forward value of three leaf fields at a point (as vel_at in the departure search).
+template <int M>
+__device__ inline void interp3_at(const DevTree& t, double x, double y, double z, const double* const* f, double* k) {
+  double xe[2];
+  const int leaf = locate_leaf(t, x, y, z, xe);
+  if (leaf < 0) {
+    k[0] = k[1] = k[2] = 0;
+    return;
+  }
+  const DevPanel& P = t.panels[leaf];
+  double wx[M], we[M];
+  bli_w<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx);
+  bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we);
+  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) k[q] = tensor_gather<M>(f[q] + base, wx, we);
+}
+
+// Reverse of k = BLI[f](p) for NF leaf fields at point p: fbar[nodes] += B kbar
+// (when fbar is given) and pbar += sum_q kbar_q grad_p f_q(p).
+template <int M, int NF>
+__device__ inline void interp_reverse(const DevTree& t, double x, double y, double z, const double* const* f,
+                                      const double* kbar, double* const* fbar, double* pbar) {
+  double xe[2];
+  const int leaf = locate_leaf(t, x, y, z, xe);
+  if (leaf < 0) {
+    flag(t, kErrLocate);
+    return;
+  }
+  const DevPanel& P = t.panels[leaf];
+  double wx[M], we[M], dwx[M], dwe[M];
+  bool ok = bli_wd<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx, dwx);
+  ok = bli_wd<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we, dwe) && ok;
+  if (!ok) flag(t, kErrBliBounds);
+  double gxi[3], geta[3];
+  xieta_gradient(x, y, z, P.face, gxi, geta);
+  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
+  double sxi = 0, seta = 0;  // sum_q kbar_q d f_q / d xi, d eta
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const double* __restrict__ F = f[q] + base;
+    double dxi = 0, deta = 0;
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+      double s = 0, sd = 0;
+#pragma unroll
+      for (int c = 0; c < M; ++c) {
+        s = fma(F[a * M + c], we[c], s);
+        sd = fma(F[a * M + c], dwe[c], sd);
+      }
+      dxi = fma(dwx[a], s, dxi);
+      deta = fma(wx[a], sd, deta);
+    }
+    sxi = fma(kbar[q], dxi, sxi);
+    seta = fma(kbar[q], deta, seta);
+    if (fbar) scatter_leaf<M>(fbar[q] + base, wx, we, kbar[q]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) pbar[k] += sxi * gxi[k] + seta * geta[k];
+}
+
+// Reverse of k_departure followed by k_dep_to_target (Omega terms included)
+// for one arrival particle.  Inputs: the four stage velocities v (v0..v3,
+// as in the departure search), the field before the step and the cotangent
+// of the field after it.  vor_bar and vbar[0..8] (v0..v2) are accumulated,
+// vbar[9..11] (v3, read only at the arrival node) assigned.
+template <int M>
+__global__ __launch_bounds__(128) void k_departure_adj(DevTree t, const double* __restrict__ x,
+                                                       const double* __restrict__ y, const double* __restrict__ z,
+                                                       CVel12 v, const double* __restrict__ vor,
+                                                       const double* __restrict__ nbar, double dt, int combine,
+                                                       double omega, Vel12 vbar, double* __restrict__ vor_bar) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.npart) return;
+  const double X = x[u], Y = y[u], Z = z[u];
+  // forward, the arithmetic of k_departure
+  double k1[3] = {v.p[9][u], v.p[10][u], v.p[11][u]}, k2[3], k3[3], k4[3];
+  double w2[3] = {X - 0.5 * dt * k1[0], Y - 0.5 * dt * k1[1], Z - 0.5 * dt * k1[2]};
+  double a = w2[0], b = w2[1], c = w2[2];
+  project_rcp(a, b, c);
+  const double p2[3] = {a, b, c};
+  interp3_at<M>(t, a, b, c, v.p + 6, k2);
+  double w3[3] = {X - 0.5 * dt * k2[0], Y - 0.5 * dt * k2[1], Z - 0.5 * dt * k2[2]};
+  a = w3[0]; b = w3[1]; c = w3[2];
+  project_rcp(a, b, c);
+  const double p3[3] = {a, b, c};
+  interp3_at<M>(t, a, b, c, v.p + 3, k3);
+  double w4[3] = {X - dt * k3[0], Y - dt * k3[1], Z - dt * k3[2]};
+  a = w4[0]; b = w4[1]; c = w4[2];
+  project_rcp(a, b, c);
+  const double p4[3] = {a, b, c};
+  interp3_at<M>(t, a, b, c, v.p + 0, k4);
+  double c1[3], c2[3], c3[3], c4[3];  // pre-projection points of the combine
+  if (combine == 0) {
+    a = X - 1.0 / 6.0 * dt * k1[0]; b = Y - 1.0 / 6.0 * dt * k1[1]; c = Z - 1.0 / 6.0 * dt * k1[2];
+    c1[0] = a; c1[1] = b; c1[2] = c;
+    project_rcp(a, b, c);
+    a -= -1.0 / 3.0 * dt * k2[0]; b -= -1.0 / 3.0 * dt * k2[1]; c -= -1.0 / 3.0 * dt * k2[2];
+    c2[0] = a; c2[1] = b; c2[2] = c;
+    project_rcp(a, b, c);
+    a -= -1.0 / 3.0 * dt * k3[0]; b -= -1.0 / 3.0 * dt * k3[1]; c -= -1.0 / 3.0 * dt * k3[2];
+    c3[0] = a; c3[1] = b; c3[2] = c;
+    project_rcp(a, b, c);
+    a -= -1.0 / 6.0 * dt * k4[0]; b -= -1.0 / 6.0 * dt * k4[1]; c -= -1.0 / 6.0 * dt * k4[2];
+    c4[0] = a; c4[1] = b; c4[2] = c;
+    project_rcp(a, b, c);
+  } else {
+    a = X - dt / 6.0 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0]);
+    b = Y - dt / 6.0 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1]);
+    c = Z - dt / 6.0 * (k1[2] + 2 * k2[2] + 2 * k3[2] + k4[2]);
+    c4[0] = a; c4[1] = b; c4[2] = c;
+    project_rcp(a, b, c);
+  }
+  // remap: new = BLI[vor](dep) + 2 Omega dep_z - 2 Omega z
+  const double g = nbar[u];
+  double db[3] = {0, 0, 2 * omega * g};
+  {
+    double* fb[1] = {vor_bar};
+    const double* f[1] = {vor};
+    interp_reverse<M, 1>(t, a, b, c, f, &g, fb, db);
+  }
+  // combine
+  double kb1[3], kb2[3], kb3[3], kb4[3];
+  if (combine == 0) {
+    project_adjoint(c4[0], c4[1], c4[2], db);
+    for (int i = 0; i < 3; ++i) kb4[i] = dt / 6.0 * db[i];
+    project_adjoint(c3[0], c3[1], c3[2], db);
+    for (int i = 0; i < 3; ++i) kb3[i] = dt / 3.0 * db[i];
+    project_adjoint(c2[0], c2[1], c2[2], db);
+    for (int i = 0; i < 3; ++i) kb2[i] = dt / 3.0 * db[i];
+    project_adjoint(c1[0], c1[1], c1[2], db);
+    for (int i = 0; i < 3; ++i) kb1[i] = -dt / 6.0 * db[i];
+  } else {
+    project_adjoint(c4[0], c4[1], c4[2], db);
+    for (int i = 0; i < 3; ++i) {
+      kb1[i] = -dt / 6.0 * db[i];
+      kb2[i] = -dt / 3.0 * db[i];
+      kb3[i] = -dt / 3.0 * db[i];
+      kb4[i] = -dt / 6.0 * db[i];
+    }
+  }
+  // k4 = BLI[v0](P(X - dt k3))
+  double pb[3] = {0, 0, 0};
+  interp_reverse<M, 3>(t, p4[0], p4[1], p4[2], v.p + 0, kb4, vbar.p + 0, pb);
+  project_adjoint(w4[0], w4[1], w4[2], pb);
+  for (int i = 0; i < 3; ++i) kb3[i] -= dt * pb[i];
+  // k3 = BLI[v1](P(X - dt/2 k2))
+  pb[0] = pb[1] = pb[2] = 0;
+  interp_reverse<M, 3>(t, p3[0], p3[1], p3[2], v.p + 3, kb3, vbar.p + 3, pb);
+  project_adjoint(w3[0], w3[1], w3[2], pb);
+  for (int i = 0; i < 3; ++i) kb2[i] -= 0.5 * dt * pb[i];
+  // k2 = BLI[v2](P(X - dt/2 k1))
+  pb[0] = pb[1] = pb[2] = 0;
+  interp_reverse<M, 3>(t, p2[0], p2[1], p2[2], v.p + 6, kb2, vbar.p + 6, pb);
+  project_adjoint(w2[0], w2[1], w2[2], pb);
+  for (int i = 0; i < 3; ++i) kb1[i] -= 0.5 * dt * pb[i];
+  // k1 = v3 at the arrival node
+  vbar.p[9][u] = kb1[0];
+  vbar.p[10][u] = kb1[1];
+  vbar.p[11][u] = kb1[2];
+}
+
+// Transpose of k_disp_interp: dbar_part[leaf nodes] += B dbar_prox (non-leaf proxies).
+template <int M>
+__global__ __launch_bounds__(256) void k_disp_interp_adj(DevTree t, CPtr3 dprox, Ptr3 dpart) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)t.offset * t.n) return;
+  const int64_t v = t.proxy_order[i];
+  const int leaf = t.proxy_leaf[v];
+  if (leaf < 0) return;  // flagged by the forward interpolation
+  const double g0 = dprox.p[0][v], g1 = dprox.p[1][v], g2 = dprox.p[2][v];
+  if (g0 == 0.0 && g1 == 0.0 && g2 == 0.0) return;  // not a source
+  const DevPanel& P = t.panels[leaf];
+  double wx[M], we[M];
+  bli_w<M>(t.proxy_xieta[2 * v], P.min_xi, P.max_xi, t.cheb_m, wx);
+  bli_w<M>(t.proxy_xieta[2 * v + 1], P.min_eta, P.max_eta, t.cheb_m, we);
+  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
+  scatter_leaf<M>(dpart.p[0] + base, wx, we, g0);
+  scatter_leaf<M>(dpart.p[1] + base, wx, we, g1);
+  scatter_leaf<M>(dpart.p[2] + base, wx, we, g2);
+}
+
+// Reverse of bve_disp_update (bve_time_step.cpp:91-104) for one stage, given
+// qbar = dL/dq_part (q_part = area * new_vor) and dbar = dL/d disp_part:
+//   vor_bar += area qbar,
+//   vbar += tau (dbar - 2 Omega area qbar dP_z(x + tau v)/dw).
+// Without vbar only the charge part is applied (the undisplaced stage).
+__global__ void k_stage_adj(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, CPtr3 vel, double tau, double omega,
+                            const double* __restrict__ area, const double* __restrict__ qbar, CPtr3 dbar,
+                            double* __restrict__ vor_bar, Ptr3 vbar) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.npart) return;
+  const double nb = area[u] * qbar[u];
+  vor_bar[u] += nb;
+  if (!vbar.p[0]) return;
+  double g[3] = {0, 0, -2 * omega * nb};
+  project_adjoint(x[u] + vel.p[0][u] * tau, y[u] + vel.p[1][u] * tau, z[u] + vel.p[2][u] * tau, g);
+  for (int f = 0; f < 3; ++f) vbar.p[f][u] += tau * (dbar.p[f][u] + g[f]);
+}
+
+template <int M>
+struct RemapAdj {
+  static void run(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* nb,
+                  double* wb, hipStream_t s) {
+    hipLaunchKernelGGL(k_remap_adj<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, dx, dy, dz, nb, wb);
+  }
+};
+
+template <int M>
+struct DepartureAdj {
+  static void run(const DevTree& t, const double* x, const double* y, const double* z, CVel12 v, const double* vor,
+                  const double* nb, double dt, int combine, double omega, Vel12 vb, double* wb, hipStream_t s) {
+    hipLaunchKernelGGL(k_departure_adj<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, v, vor, nb,
+                       dt, combine, omega, vb, wb);
+  }
+};
+
+template <int M>
+struct DispInterpAdj {
+  static void run(const DevTree& t, CPtr3 dx, Ptr3 dp, hipStream_t s) {
+    const int64_t cnt = (int64_t)t.offset * t.n;
+    if (cnt > 0) hipLaunchKernelGGL(k_disp_interp_adj<M>, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, t, dx, dp);
+  }
+};
+
+}  // namespace
+
+void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
+                          double* wbar, hipStream_t s) {
+  dispatch_m<RemapAdj>(t.m, t, dx, dy, dz, newbar, wbar, s);
+}
+
+void launch_departure_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                              const double* const* vels, const double* vor, const double* new_vor_bar, double dt,
+                              int combine, double omega, double* const* vel_bar, double* vor_bar, hipStream_t s) {
+  CVel12 v{};
+  Vel12 vb{};
+  for (int i = 0; i < 12; ++i) {
+    v.p[i] = vels[i];
+    vb.p[i] = vel_bar[i];
+  }
+  dispatch_m<DepartureAdj>(t.m, t, x, y, z, v, vor, new_vor_bar, dt, combine, omega, vb, vor_bar, s);
+}
+
+void launch_disp_interp_adjoint(const DevTree& t, const double* const* dbar_proxy, double* const* dbar_part,
+                                hipStream_t s) {
+  CPtr3 dx{{dbar_proxy[0], dbar_proxy[1], dbar_proxy[2]}};
+  Ptr3 dp{{dbar_part[0], dbar_part[1], dbar_part[2]}};
+  dispatch_m<DispInterpAdj>(t.m, t, dx, dp, s);
+}
+
+void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                          const double* const* vel, double tau, double omega, const double* area, const double* qbar,
+                          const double* const* dbar, double* vor_bar, double* const* vbar, hipStream_t s) {
+  CPtr3 v{}, d{};
+  Ptr3 vb{};
+  if (vbar) {
+    for (int f = 0; f < 3; ++f) {
+      v.p[f] = vel[f];
+      d.p[f] = dbar[f];
+      vb.p[f] = vbar[f];
+    }
+  }
+  hipLaunchKernelGGL(k_stage_adj, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, x, y, z, v, tau, omega, area,
+                     qbar, d, vor_bar, vb);
+}
+
+}  // namespace sbslpm

@@ -349,4 +349,42 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
   return "";
 }
 
+void build_source_major(const Tree& t, SourceMajor& sm) {
+  const int64_t I = t.interaction_count();
+  std::vector<int> cnt(t.P + 1, 0);
+  for (int64_t e = 0; e < I; ++e) cnt[t.ilist[3 * e + 1] + 1]++;
+  for (int p = 0; p < t.P; ++p) cnt[p + 1] += cnt[p];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  sm.targets.assign(I, SourceSet{});
+  for (int64_t e = 0; e < I; ++e) {
+    const int tg = t.ilist[3 * e], s = t.ilist[3 * e + 1];
+    SourceSet ts{};
+    ts.panel = tg;
+    if (t.pn[tg].is_leaf) {
+      ts.first = (int)((int64_t)(tg - t.offset) * t.m2);
+      ts.count = t.m2;
+    } else {
+      ts.first = (int)(t.npart + (int64_t)tg * t.n);
+      ts.count = t.n;
+    }
+    sm.targets[fill[s]++] = ts;
+  }
+  sm.tiles.clear();
+  for (int p = 0; p < t.P; ++p) {
+    if (cnt[p + 1] == cnt[p]) continue;
+    const int npts = t.pn[p].is_leaf ? t.m2 : t.n;
+    const int64_t base = t.pn[p].is_leaf ? (int64_t)(p - t.offset) * t.m2 : t.npart + (int64_t)p * t.n;
+    const int chunks = (npts + 63) / 64, per = (npts + chunks - 1) / chunks;
+    for (int f = 0; f < npts; f += per) {
+      TargetTile tl{};
+      tl.panel = p; tl.first = (int)(base + f); tl.count = std::min(per, npts - f);
+      tl.src_begin = cnt[p]; tl.src_end = cnt[p + 1];
+      sm.tiles.push_back(tl);
+    }
+  }
+  sm.zero_rows.clear();
+  for (int p = 0; p < t.P; ++p)
+    if ((!t.pn[p].is_leaf && cnt[p + 1] == cnt[p]) || (t.pn[p].is_leaf && t.cfg.levels == 1)) sm.zero_rows.push_back(p);
+}
+
 }  // namespace sbslpm

@@ -93,6 +93,21 @@ struct Tree {
 // a message describing the invalid parameter.
 std::string build_tree(const TreeConfig& cfg, Tree& t);
 
+// The interaction list seen from its sources, for the transposed (adjoint)
+// pair kernel: tiles of at most 64 source points of one source panel
+// (particles of a leaf, proxies of a non-leaf panel) whose [src_begin,
+// src_end) range indexes `targets`, each entry naming the target points
+// [first, first+count) of one interaction (particles of a leaf target,
+// proxies of a non-leaf target).  `zero_rows` lists the proxy rows that are
+// no interaction source (and the leaf rows when levels == 1), whose
+// cotangents are zero.
+struct SourceMajor {
+  std::vector<TargetTile> tiles;
+  std::vector<SourceSet> targets;
+  std::vector<int> zero_rows;
+};
+void build_source_major(const Tree& t, SourceMajor& sm);
+
 // Host twin of the device point location (interp_funcs_impl.hpp:114-142).
 int locate_leaf(const Tree& t, double x, double y, double z);
 
