@@ -81,6 +81,14 @@ int sbslpm_tree_locate(const sbslpm_tree* tree, double x, double y, double z);
 /* Fixed transfer matrices: m2m_hi/m2m_lo [(d+1)*(d+1)] (parent basis a at upper/lower-half child node k,
  * index a*(d+1)+k) and p2m [(d+1)*m] (proxy basis a at particle node k). Any pointer may be NULL. */
 int sbslpm_tree_transfer_matrices(const sbslpm_tree* tree, double* m2m_hi, double* m2m_lo, double* p2m);
+/* Unique particle positions, cubed_sphere_2d_to_1d (initialize_cubed_sphere.cpp:290-324):
+ * particles in [leaf][node] order; one within 1e-15 in every coordinate of an earlier unique
+ * point maps to the first such point, otherwise it becomes the next unique point.
+ * two_d_to_1d [Pl][m2] receives the map, counts/x1d/y1d/z1d (each at least Pl*m2 long, may be
+ * NULL) the multiplicity and position of each unique point, *n1d their number (the reference's
+ * point_count). */
+int sbslpm_tree_dedupe(const sbslpm_tree* tree, int* two_d_to_1d, int* counts, double* x1d, double* y1d,
+                       double* z1d, int64_t* n1d);
 
 /* ---- Device context ----------------------------------------------------------------- */
 /* Builds the cubed-sphere tree (cubed_sphere_panels_init, initialize_cubed_sphere.cpp:185-234),
@@ -261,6 +269,18 @@ int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* ctx, const double* x, const dou
  * departure points.  Accumulated with FP64 atomic adds (summation order not fixed). */
 int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* ctx, const double* dep_x, const double* dep_y,
                                            const double* dep_z, const double* new_vor_bar, double* vor_bar);
+
+/* BLI interpolation of nfields (1..8) leaf fields [Pl][m2] at n arbitrary points (x, y, z on
+ * the unit sphere): out[q][i] = field q interpolated at point i on the leaf that contains it.
+ * The functor of latlon_interp_host / interp_to_latlon_host (interp_funcs.hpp:22-77) for any
+ * point set; the particle grid's degree m-1 replaces interp_degree (equal in the reference
+ * layout). */
+int sbslpm_interp_points(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z, int nfields,
+                         const double* const* fields, double* const* out);
+/* Zonal and meridional velocity from Cartesian velocity at n points, xyz_vel_to_uv_vel
+ * (general_utils_impl.hpp:195-229; zero at the poles). */
+int sbslpm_xyz_vel_to_uv_vel(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz, double* u, double* v);
 
 /* swe_displacement_upward_pass (swe_time_step.cpp:265-276): disp_part = tau v [Pl][m2]
  * (swe_disp_update, :194-210); disp_proxy [P][n] non-leaf rows interpolated from the leaf that

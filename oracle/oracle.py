@@ -87,6 +87,11 @@ def lib():
         L.orc_eval_targets.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int64,
                                        _dp, _dp, _dp, _ppd]
         L.orc_eval_targets.restype = ctypes.c_int
+        L.orc_interp_points.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int, _ppd, _ppd]
+        L.orc_interp_points.restype = ctypes.c_int
+        L.orc_xyz_vel_to_uv_vel.argtypes = [ctypes.c_int64] + [_dp] * 8
+        L.orc_dedupe.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp]
+        L.orc_dedupe.restype = ctypes.c_int64
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -374,3 +379,31 @@ class OracleFMM:
         if err:
             raise RuntimeError("oracle bve_back_rk4_step: geometry error")
         return vor, vel, (tr if ntr else None)
+
+    def interp_points(self, x, y, z, fields):
+        """latlon_interp_host at arbitrary points (1-D x, y, z); fields [Pl][m2] each."""
+        x, y, z = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z)]
+        fields = [np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+        out = [np.zeros(x.size) for _ in fields]
+        err = lib().orc_interp_points(self.h, x.size, _p(x), _p(y), _p(z), len(fields), _pp(fields), _pp(out))
+        if err:
+            raise RuntimeError("oracle interp_points: geometry error")
+        return out
+
+
+def xyz_vel_to_uv_vel(x, y, z, vx, vy, vz):
+    a = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z, vx, vy, vz)]
+    u, v = np.zeros(a[0].size), np.zeros(a[0].size)
+    lib().orc_xyz_vel_to_uv_vel(a[0].size, *[_p(t) for t in a], _p(u), _p(v))
+    return u, v
+
+
+def dedupe(x, y, z):
+    """cubed_sphere_2d_to_1d, the reference's O(N^2) scan: (map, counts, x1d, y1d, z1d)."""
+    x, y, z = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z)]
+    n = x.size
+    m = np.zeros(n, dtype=np.int32)
+    c = np.zeros(n, dtype=np.int32)
+    o = [np.zeros(n) for _ in range(3)]
+    k = lib().orc_dedupe(n, _p(x), _p(y), _p(z), m.ctypes.data_as(_ip), c.ctypes.data_as(_ip), *[_p(t) for t in o])
+    return m, c[:k].copy(), o[0][:k].copy(), o[1][:k].copy(), o[2][:k].copy()

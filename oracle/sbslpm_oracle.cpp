@@ -1149,6 +1149,59 @@ int orc_bve_back_rk4_step(void* h, const double* x, const double* y, const doubl
   return err ? ORC_E_GEOMETRY : ORC_OK;
 }
 
+// latlon_interp_host (interp_funcs.hpp:36-59) at arbitrary points: locate,
+// BLI basis on the leaf, sum over the leaf's nodes; nf fields [Pl][m2].
+int orc_interp_points(void* h, int64_t n, const double* x, const double* y, const double* z, int nf,
+                      const double* const* fields, double* const* out) {
+  const Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+#pragma omp parallel for reduction(| : err)
+  for (int64_t i = 0; i < n; i++) {
+    double v[16];
+    err |= interp_at(c, x[i], y[i], z[i], nf, fields, v);
+    for (int f = 0; f < nf; f++) out[f][i] = v[f];
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// xyz_vel_to_uv_vel (general_utils_impl.hpp:211-228).
+void orc_xyz_vel_to_uv_vel(int64_t n, const double* x, const double* y, const double* z, const double* vx,
+                           const double* vy, const double* vz, double* u, double* v) {
+  for (int64_t i = 0; i < n; i++) {
+    if (std::abs(z[i]) < 1 - 1e-16) {
+      u[i] = (-y[i] * vx[i] + x[i] * vy[i]) / std::sqrt(x[i] * x[i] + y[i] * y[i]);
+      v[i] = -((x[i] * vx[i] + y[i] * vy[i]) * z[i] - (x[i] * x[i] + y[i] * y[i]) * vz[i]) /
+             std::sqrt(x[i] * x[i] + y[i] * y[i]);
+    } else {
+      u[i] = 0;
+      v[i] = 0;
+    }
+  }
+}
+
+// cubed_sphere_2d_to_1d (initialize_cubed_sphere.cpp:290-324), the reference's
+// scan: each point in order is compared with every unique point found so far
+// and joins the first within 1e-15 in all three coordinates.  O(N^2): small N.
+int64_t orc_dedupe(int64_t n, const double* x, const double* y, const double* z, int* two_d_to_1d, int* counts,
+                   double* x1d, double* y1d, double* z1d) {
+  int64_t found = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t k = 0;
+    for (; k < found; k++)
+      if (std::abs(x[i] - x1d[k]) < 1e-15 && std::abs(y[i] - y1d[k]) < 1e-15 && std::abs(z[i] - z1d[k]) < 1e-15) break;
+    if (k == found) {
+      x1d[found] = x[i];
+      y1d[found] = y[i];
+      z1d[found] = z[i];
+      counts[found] = 0;
+      found++;
+    }
+    counts[k] += 1;
+    two_d_to_1d[i] = (int)k;
+  }
+  return found;
+}
+
 void orc_set_threads(int n) { omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 

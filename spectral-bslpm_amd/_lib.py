@@ -54,6 +54,7 @@ SIGNATURES = {
     "sbslpm_tree_panels": (_I, [_V, _PI, _PD]),
     "sbslpm_tree_locate": (_I, [_V, _D, _D, _D]),
     "sbslpm_tree_transfer_matrices": (_I, [_V, _PD, _PD, _PD]),
+    "sbslpm_tree_dedupe": (_I, [_V, _PI, _PI, _PD, _PD, _PD, _PI64]),
     "sbslpm_ctx_create": (_I, [ctypes.POINTER(Params), ctypes.POINTER(_V)]),
     "sbslpm_ctx_destroy": (None, [_V]),
     "sbslpm_ctx_set_stream": (_I, [_V, _V]),
@@ -80,6 +81,8 @@ SIGNATURES = {
     "sbslpm_bve_back_rk4_step_adjoint": (_I, [_V] * 8),
     "sbslpm_bve_departure_to_target_adjoint": (_I, [_V] * 6),
     "sbslpm_swe_displacement": (_I, [_V, _V, _V, _V, ctypes.c_double] + [_V] * 6),
+    "sbslpm_interp_points": (_I, [_V, ctypes.c_int64, _V, _V, _V, _I, _V, _V]),
+    "sbslpm_xyz_vel_to_uv_vel": (_I, [_V, ctypes.c_int64] + [_V] * 8),
     "sbslpm_swe_find_departure_points": (_I, [_V, _V, _V, _V, _V, ctypes.c_double, _V, _V, _V]),
     "sbslpm_swe_departure_to_target": (_I, [_V] * 8 + [_I] + [_V] * 4),
     "sbslpm_poisson_solve_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V]),

@@ -114,6 +114,21 @@ class HostTree:
         self._lib.sbslpm_tree_transfer_matrices(self.h, _dp(hi), _dp(lo), _dp(p2m))
         return hi, lo, p2m
 
+    def dedupe(self):
+        """cubed_sphere_2d_to_1d (initialize_cubed_sphere.cpp:290-324): (two_d_to_1d [Pl, m2],
+        counts [n1d], x1d, y1d, z1d [n1d]) of the unique particle positions."""
+        s = self.sizes
+        m = np.zeros((s.Pl, s.m2), dtype=np.int32)
+        cnt = np.zeros(s.N, dtype=np.int32)
+        xyz = [np.zeros(s.N) for _ in range(3)]
+        n1d = np.zeros(1, dtype=np.int64)
+        rc = self._lib.sbslpm_tree_dedupe(self.h, _ip(m), _ip(cnt), *[_dp(a) for a in xyz],
+                                          n1d.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        if rc:
+            raise SbslpmError(rc, "tree_dedupe")
+        k = int(n1d[0])
+        return m, cnt[:k].copy(), xyz[0][:k].copy(), xyz[1][:k].copy(), xyz[2][:k].copy()
+
 
 def _ptr(t):
     """Device pointer of a contiguous float64 CUDA tensor (None -> NULL)."""
@@ -378,6 +393,29 @@ class Context:
         self._check(self._lib.sbslpm_bve_velocity_adjoint(self.h, _ptr(area), *[_ptr(u) for u in ubar],
                                                           _ptr(vor_bar)), "bve_velocity_adjoint")
         return vor_bar
+
+    def interp_points(self, x, y, z, fields, out=None):
+        """BLI of leaf fields [Pl, m2] (a list of up to 8 tensors) at the points (x, y, z) (1-D
+        tensors of one length): latlon_interp_host, interp_funcs.hpp:22-60."""
+        n = x.numel()
+        out = [self.torch.zeros(n, dtype=self.torch.float64, device=x.device) for _ in fields] if out is None else out
+        fa = (ctypes.c_void_p * len(fields))(*[f.data_ptr() for f in fields])
+        oa = (ctypes.c_void_p * len(out))(*[o.data_ptr() for o in out])
+        for t in list(fields) + list(out):
+            _ptr(t)
+        self._check(self._lib.sbslpm_interp_points(self.h, n, _ptr(x), _ptr(y), _ptr(z), len(fields),
+                                                   ctypes.cast(fa, ctypes.c_void_p), ctypes.cast(oa, ctypes.c_void_p)),
+                    "interp_points")
+        return out
+
+    def xyz_vel_to_uv_vel(self, x, y, z, vel, u=None, v=None):
+        """Zonal and meridional velocity, xyz_vel_to_uv_vel (general_utils_impl.hpp:195-229)."""
+        u = self.torch.zeros_like(vel[0]) if u is None else u
+        v = self.torch.zeros_like(vel[0]) if v is None else v
+        self._check(self._lib.sbslpm_xyz_vel_to_uv_vel(self.h, vel[0].numel(), _ptr(x), _ptr(y), _ptr(z),
+                                                       *[_ptr(t) for t in vel], _ptr(u), _ptr(v)),
+                    "xyz_vel_to_uv_vel")
+        return u, v
 
     def bve_back_rk4_step_adjoint(self, x, y, z, area, vor, new_vor_bar, vor_bar=None):
         """Gradient of one bve_back_rk4_step with respect to the vorticity before it: vor_bar =

@@ -831,6 +831,24 @@ int sbslpm_swe_departure_to_target(sbslpm_ctx* c, const double* dep_x, const dou
   return launched(c);
 }
 
+int sbslpm_interp_points(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z, int nfields,
+                         const double* const* fields, double* const* out) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !z)) || nfields < 1 || nfields > max_interp_fields() || !fields || !out)
+    return fail(c, SBSLPM_E_ARG, "interp_points: bad argument");
+  for (int q = 0; q < nfields; ++q)
+    if (!fields[q] || !out[q]) return fail(c, SBSLPM_E_ARG, "interp_points: null field");
+  launch_interp_points(c->dt, n, x, y, z, nfields, fields, out, c->stream);
+  return launched(c);
+}
+
+int sbslpm_xyz_vel_to_uv_vel(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz, double* u, double* v) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !u || !v)))
+    return fail(c, SBSLPM_E_ARG, "xyz_vel_to_uv_vel: bad argument");
+  launch_xyz_vel_to_uv(n, x, y, z, vx, vy, vz, u, v, c->stream);
+  return launched(c);
+}
+
 int sbslpm_bve_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                              double* vor, double* proxy_source_vors, double* vx, double* vy, double* vz,
                              double* tracers, double time) {

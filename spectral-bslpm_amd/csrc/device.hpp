@@ -197,6 +197,14 @@ void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, co
                           const double* const* vel, double tau, double omega, const double* area, const double* qbar,
                           const double* const* dbar, double* vor_bar, double* const* vbar, hipStream_t s);
 
+// ---- output operators (output.hip)
+int max_interp_fields();
+// out[q][i] = BLI of leaf field q [Pl][m2] at point i (n points, nf <= max_interp_fields()).
+void launch_interp_points(const DevTree& t, int64_t n, const double* px, const double* py, const double* pz, int nf,
+                          const double* const* fields, double* const* out, hipStream_t s);
+void launch_xyz_vel_to_uv(int64_t n, const double* x, const double* y, const double* z, const double* vx,
+                          const double* vy, const double* vz, double* u, double* v, hipStream_t s);
+
 // ---- evaluation at arbitrary targets (targets.hip)
 // kind 0: log kernel (psi), kind 1: Biot-Savart (3 components).  far[f] are the
 // [P][n] proxy potentials after the L2L sweep (leaf rows = far field of each

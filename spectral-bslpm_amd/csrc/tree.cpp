@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <unordered_map>
 #include <utility>
 
 #include "geometry.hpp"
@@ -385,6 +386,53 @@ void build_source_major(const Tree& t, SourceMajor& sm) {
   sm.zero_rows.clear();
   for (int p = 0; p < t.P; ++p)
     if ((!t.pn[p].is_leaf && cnt[p + 1] == cnt[p]) || (t.pn[p].is_leaf && t.cfg.levels == 1)) sm.zero_rows.push_back(p);
+}
+
+}  // namespace sbslpm
+
+namespace sbslpm {
+
+void dedupe_points(const Tree& t, std::vector<int>& two_d_to_1d, std::vector<int>& counts, std::vector<double>& x1d,
+                   std::vector<double>& y1d, std::vector<double>& z1d) {
+  constexpr double kTol = 1e-15, kCell = 1e-12;
+  const int64_t N = t.npart;
+  two_d_to_1d.assign(N, -1);
+  counts.clear();
+  x1d.clear();
+  y1d.clear();
+  z1d.clear();
+  auto cell = [&](double v) { return (int64_t)std::floor(v / kCell); };
+  auto key = [](int64_t a, int64_t b, int64_t c) {
+    return (uint64_t)(a * 73856093) ^ (uint64_t)(b * 19349663) ^ (uint64_t)(c * 83492791);
+  };
+  std::unordered_multimap<uint64_t, int> grid;
+  grid.reserve((size_t)N);
+  for (int64_t u = 0; u < N; ++u) {
+    const double x = t.spec_x[u], y = t.spec_y[u], z = t.spec_z[u];
+    const int64_t cx = cell(x), cy = cell(y), cz = cell(z);
+    int best = -1;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          auto range = grid.equal_range(key(cx + dx, cy + dy, cz + dz));
+          for (auto it = range.first; it != range.second; ++it) {
+            const int k = it->second;
+            if (std::fabs(x - x1d[k]) < kTol && std::fabs(y - y1d[k]) < kTol && std::fabs(z - z1d[k]) < kTol &&
+                (best < 0 || k < best))
+              best = k;
+          }
+        }
+    if (best < 0) {
+      best = (int)x1d.size();
+      x1d.push_back(x);
+      y1d.push_back(y);
+      z1d.push_back(z);
+      counts.push_back(0);
+      grid.emplace(key(cx, cy, cz), best);
+    }
+    counts[best] += 1;
+    two_d_to_1d[u] = best;
+  }
 }
 
 }  // namespace sbslpm

@@ -108,6 +108,14 @@ struct SourceMajor {
 };
 void build_source_major(const Tree& t, SourceMajor& sm);
 
+// Unique particle positions (cubed_sphere_2d_to_1d, initialize_cubed_sphere.cpp:290-324):
+// particles taken in [leaf][node] order; one within 1e-15 in every coordinate of an
+// earlier unique point maps to the first such point, otherwise it becomes a new
+// unique point.  counts[k] = particles mapped to unique point k.  Spatial hashing
+// replaces the reference's scan over all earlier points (same result).
+void dedupe_points(const Tree& t, std::vector<int>& two_d_to_1d, std::vector<int>& counts, std::vector<double>& x1d,
+                   std::vector<double>& y1d, std::vector<double>& z1d);
+
 // Host twin of the device point location (interp_funcs_impl.hpp:114-142).
 int locate_leaf(const Tree& t, double x, double y, double z);
 

@@ -1,6 +1,7 @@
 // tree_api.cpp -- host-only C ABI over the tree builder (no device needed).
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/sbslpm.h"
 #include "tree.hpp"
@@ -81,6 +82,21 @@ int sbslpm_tree_transfer_matrices(const sbslpm_tree* h, double* hi, double* lo, 
   if (hi) std::memcpy(hi, t.m2m_hi.data(), t.m2m_hi.size() * sizeof(double));
   if (lo) std::memcpy(lo, t.m2m_lo.data(), t.m2m_lo.size() * sizeof(double));
   if (p2m) std::memcpy(p2m, t.p2m.data(), t.p2m.size() * sizeof(double));
+  return SBSLPM_OK;
+}
+
+int sbslpm_tree_dedupe(const sbslpm_tree* h, int* two_d_to_1d, int* counts, double* x1d, double* y1d, double* z1d,
+                       int64_t* n1d) {
+  if (!h || !two_d_to_1d || !n1d) return SBSLPM_E_ARG;
+  std::vector<int> map, cnt;
+  std::vector<double> x, y, z;
+  sbslpm::dedupe_points(h->t, map, cnt, x, y, z);
+  std::memcpy(two_d_to_1d, map.data(), map.size() * sizeof(int));
+  if (counts) std::memcpy(counts, cnt.data(), cnt.size() * sizeof(int));
+  if (x1d) std::memcpy(x1d, x.data(), x.size() * sizeof(double));
+  if (y1d) std::memcpy(y1d, y.data(), y.size() * sizeof(double));
+  if (z1d) std::memcpy(z1d, z.data(), z.size() * sizeof(double));
+  *n1d = (int64_t)x.size();
   return SBSLPM_OK;
 }
 
