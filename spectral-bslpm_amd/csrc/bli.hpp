@@ -180,6 +180,70 @@ __device__ inline void xieta_gradient(double x, double y, double z, int face, do
   geta[id] -= N2 * r2;
 }
 
+// LDS doubles per wave used by wave_scatter_leaf<M, NF>.
+template <int M, int NF>
+constexpr int wave_scatter_lds() {
+  return 64 * (2 * M + NF);
+}
+
+// Transposed leaf interpolation summed over the wave before it reaches memory:
+// every lane holding a point (leaf >= 0) contributes g[q] wx[a] we[c] to node
+// a*M + c of field q of its leaf.  The wave takes one distinct leaf at a time:
+// the lanes on that leaf stage wx, we and g in the wave's LDS slice `lds`
+// (wave_scatter_lds<M, NF>() doubles), then each lane sums the staged
+// contributions to its nodes and issues one atomic add per node and field.
+// Neighbouring particles mostly share a leaf, so a wave makes a few passes
+// instead of 64 * M * M * NF contended atomics.  Every lane of the wave must
+// call it (lanes without a point pass leaf = -1); the loop bounds are
+// wave-uniform.
+template <int M, int NF>
+__device__ inline void wave_scatter_leaf(double* __restrict__ lds, int leaf, const double* wx, const double* we,
+                                         const double* g, double* const* dst, int offset) {
+  const int lane = threadIdx.x & 63;
+  double* sx = lds;
+  double* se = lds + 64 * M;
+  double* sg = lds + 128 * M;
+  uint64_t todo = __ballot(leaf >= 0);
+  while (todo) {
+    const int L = __shfl(leaf, __builtin_ctzll(todo));
+    const bool mine = (leaf == L);
+    const uint64_t mm = __ballot(mine);
+    if (mine) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) {
+        sx[lane * M + i] = wx[i];
+        se[lane * M + i] = we[i];
+      }
+#pragma unroll
+      for (int q = 0; q < NF; ++q) sg[lane * NF + q] = g[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int64_t base = (int64_t)(L - offset) * (M * M);
+    for (int j = lane; j < M * M; j += 64) {
+      const int a = j / M, c = j % M;
+      double acc[NF];
+#pragma unroll
+      for (int q = 0; q < NF; ++q) acc[q] = 0;
+      uint64_t b = mm;
+      while (b) {
+        const int l = __builtin_ctzll(b);
+        b &= b - 1;
+        const double w = sx[l * M + a] * se[l * M + c];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) acc[q] = fma(w, sg[l * NF + q], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < NF; ++q) atomicAdd(dst[q] + base + j, acc[q]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    todo &= ~mm;
+  }
+}
+
 inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
 // Runs F<M>::run(args...) for the particle-grid side m (2..11).

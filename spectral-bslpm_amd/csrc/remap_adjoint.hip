@@ -9,8 +9,10 @@
 // Point location is piecewise constant and contributes nothing; the BLI
 // interpolant is differentiated in its label coordinates (bli_wd) and those
 // in the point (xieta_gradient), the re-projections through
-// project_adjoint.  Transposed interpolations scatter into the leaf nodes
-// with FP64 atomic adds (summation order not fixed).
+// project_adjoint.  Transposed interpolations are summed per wave and leaf
+// in LDS (wave_scatter_leaf) and added to the leaf nodes with FP64 atomics
+// (summation order not fixed).  Threads past the end take the last point's
+// data and scatter nothing, so every lane reaches the wave-wide scatters.
 #include "bli.hpp"
 
 namespace sbslpm {
@@ -31,35 +33,28 @@ struct Vel12 {
   double* p[12];
 };
 
-// field_bar[leaf nodes] += (wx outer we) * g
-template <int M>
-__device__ inline void scatter_leaf(double* __restrict__ dst, const double* wx, const double* we, double g) {
-#pragma unroll
-  for (int a = 0; a < M; ++a) {
-    const double ga = wx[a] * g;
-#pragma unroll
-    for (int c = 0; c < M; ++c) atomicAdd(dst + a * M + c, ga * we[c]);
-  }
-}
-
 template <int M>
 __global__ __launch_bounds__(128) void k_remap_adj(DevTree t, const double* __restrict__ dx,
                                                    const double* __restrict__ dy, const double* __restrict__ dz,
                                                    const double* __restrict__ newbar, double* __restrict__ wbar) {
-  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= t.npart) return;
-  double xe[2];
-  const int leaf = locate_leaf(t, dx[u], dy[u], dz[u], xe);
+  __shared__ double lds[2][wave_scatter_lds<M, 1>()];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < t.npart;
+  const int64_t u = valid ? i : t.npart - 1;
+  double xe[2], wx[M], we[M];
+  int leaf = locate_leaf(t, dx[u], dy[u], dz[u], xe);
   if (leaf < 0) {
-    flag(t, kErrLocate);
-    return;
+    if (valid) flag(t, kErrLocate);
+  } else {
+    const DevPanel& P = t.panels[leaf];
+    bool ok = bli_w<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx);
+    ok = bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we) && ok;
+    if (!ok && valid) flag(t, kErrBliBounds);
   }
-  const DevPanel& P = t.panels[leaf];
-  double wx[M], we[M];
-  bool ok = bli_w<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx);
-  ok = bli_w<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we) && ok;
-  if (!ok) flag(t, kErrBliBounds);
-  scatter_leaf<M>(wbar + (int64_t)(leaf - t.offset) * (M * M), wx, we, newbar[u]);
+  if (!valid) leaf = -1;
+  const double g = newbar[u];
+  double* dst[1] = {wbar};
+  wave_scatter_leaf<M, 1>(lds[threadIdx.x >> 6], leaf, wx, we, &g, dst, t.offset);
 }
 
 // Forward value of three leaf fields at a point (as vel_at in the departure search).
@@ -80,22 +75,24 @@ __device__ inline void interp3_at(const DevTree& t, double x, double y, double z
   for (int q = 0; q < 3; ++q) k[q] = tensor_gather<M>(f[q] + base, wx, we);
 }
 
-// Reverse of k = BLI[f](p) for NF leaf fields at point p: fbar[nodes] += B kbar
-// (when fbar is given) and pbar += sum_q kbar_q grad_p f_q(p).
+// Reverse of k = BLI[f](p) for NF leaf fields at point p, first part:
+// pbar += sum_q kbar_q grad_p f_q(p); returns the leaf (-1 when location
+// fails) and the 1-D weights for the transposed interpolation fbar += B kbar,
+// which the caller applies with wave_scatter_leaf.
 template <int M, int NF>
-__device__ inline void interp_reverse(const DevTree& t, double x, double y, double z, const double* const* f,
-                                      const double* kbar, double* const* fbar, double* pbar) {
+__device__ inline int interp_grad(const DevTree& t, double x, double y, double z, const double* const* f,
+                                  const double* kbar, double* pbar, double* wx, double* we, bool report) {
   double xe[2];
   const int leaf = locate_leaf(t, x, y, z, xe);
   if (leaf < 0) {
-    flag(t, kErrLocate);
-    return;
+    if (report) flag(t, kErrLocate);
+    return -1;
   }
   const DevPanel& P = t.panels[leaf];
-  double wx[M], we[M], dwx[M], dwe[M];
+  double dwx[M], dwe[M];
   bool ok = bli_wd<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx, dwx);
   ok = bli_wd<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we, dwe) && ok;
-  if (!ok) flag(t, kErrBliBounds);
+  if (!ok && report) flag(t, kErrBliBounds);
   double gxi[3], geta[3];
   xieta_gradient(x, y, z, P.face, gxi, geta);
   const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
@@ -117,10 +114,10 @@ __device__ inline void interp_reverse(const DevTree& t, double x, double y, doub
     }
     sxi = fma(kbar[q], dxi, sxi);
     seta = fma(kbar[q], deta, seta);
-    if (fbar) scatter_leaf<M>(fbar[q] + base, wx, we, kbar[q]);
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) pbar[k] += sxi * gxi[k] + seta * geta[k];
+  return leaf;
 }
 
 // Reverse of k_departure followed by k_dep_to_target (Omega terms included)
@@ -134,8 +131,11 @@ __global__ __launch_bounds__(128) void k_departure_adj(DevTree t, const double* 
                                                        CVel12 v, const double* __restrict__ vor,
                                                        const double* __restrict__ nbar, double dt, int combine,
                                                        double omega, Vel12 vbar, double* __restrict__ vor_bar) {
-  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= t.npart) return;
+  __shared__ double lds[2][wave_scatter_lds<M, 3>()];
+  double* wl = lds[threadIdx.x >> 6];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < t.npart;
+  const int64_t u = valid ? i : t.npart - 1;
   const double X = x[u], Y = y[u], Z = z[u];
   // forward, the arithmetic of k_departure
   double k1[3] = {v.p[9][u], v.p[10][u], v.p[11][u]}, k2[3], k3[3], k4[3];
@@ -175,73 +175,80 @@ __global__ __launch_bounds__(128) void k_departure_adj(DevTree t, const double* 
     c4[0] = a; c4[1] = b; c4[2] = c;
     project_rcp(a, b, c);
   }
+  double wx[M], we[M];
   // remap: new = BLI[vor](dep) + 2 Omega dep_z - 2 Omega z
   const double g = nbar[u];
   double db[3] = {0, 0, 2 * omega * g};
   {
-    double* fb[1] = {vor_bar};
     const double* f[1] = {vor};
-    interp_reverse<M, 1>(t, a, b, c, f, &g, fb, db);
+    double* fb[1] = {vor_bar};
+    int leaf = interp_grad<M, 1>(t, a, b, c, f, &g, db, wx, we, valid);
+    wave_scatter_leaf<M, 1>(wl, valid ? leaf : -1, wx, we, &g, fb, t.offset);
   }
   // combine
   double kb1[3], kb2[3], kb3[3], kb4[3];
   if (combine == 0) {
     project_adjoint(c4[0], c4[1], c4[2], db);
-    for (int i = 0; i < 3; ++i) kb4[i] = dt / 6.0 * db[i];
+    for (int k = 0; k < 3; ++k) kb4[k] = dt / 6.0 * db[k];
     project_adjoint(c3[0], c3[1], c3[2], db);
-    for (int i = 0; i < 3; ++i) kb3[i] = dt / 3.0 * db[i];
+    for (int k = 0; k < 3; ++k) kb3[k] = dt / 3.0 * db[k];
     project_adjoint(c2[0], c2[1], c2[2], db);
-    for (int i = 0; i < 3; ++i) kb2[i] = dt / 3.0 * db[i];
+    for (int k = 0; k < 3; ++k) kb2[k] = dt / 3.0 * db[k];
     project_adjoint(c1[0], c1[1], c1[2], db);
-    for (int i = 0; i < 3; ++i) kb1[i] = -dt / 6.0 * db[i];
+    for (int k = 0; k < 3; ++k) kb1[k] = -dt / 6.0 * db[k];
   } else {
     project_adjoint(c4[0], c4[1], c4[2], db);
-    for (int i = 0; i < 3; ++i) {
-      kb1[i] = -dt / 6.0 * db[i];
-      kb2[i] = -dt / 3.0 * db[i];
-      kb3[i] = -dt / 3.0 * db[i];
-      kb4[i] = -dt / 6.0 * db[i];
+    for (int k = 0; k < 3; ++k) {
+      kb1[k] = -dt / 6.0 * db[k];
+      kb2[k] = -dt / 3.0 * db[k];
+      kb3[k] = -dt / 3.0 * db[k];
+      kb4[k] = -dt / 6.0 * db[k];
     }
   }
   // k4 = BLI[v0](P(X - dt k3))
   double pb[3] = {0, 0, 0};
-  interp_reverse<M, 3>(t, p4[0], p4[1], p4[2], v.p + 0, kb4, vbar.p + 0, pb);
+  int leaf = interp_grad<M, 3>(t, p4[0], p4[1], p4[2], v.p + 0, kb4, pb, wx, we, valid);
+  wave_scatter_leaf<M, 3>(wl, valid ? leaf : -1, wx, we, kb4, vbar.p + 0, t.offset);
   project_adjoint(w4[0], w4[1], w4[2], pb);
-  for (int i = 0; i < 3; ++i) kb3[i] -= dt * pb[i];
+  for (int k = 0; k < 3; ++k) kb3[k] -= dt * pb[k];
   // k3 = BLI[v1](P(X - dt/2 k2))
   pb[0] = pb[1] = pb[2] = 0;
-  interp_reverse<M, 3>(t, p3[0], p3[1], p3[2], v.p + 3, kb3, vbar.p + 3, pb);
+  leaf = interp_grad<M, 3>(t, p3[0], p3[1], p3[2], v.p + 3, kb3, pb, wx, we, valid);
+  wave_scatter_leaf<M, 3>(wl, valid ? leaf : -1, wx, we, kb3, vbar.p + 3, t.offset);
   project_adjoint(w3[0], w3[1], w3[2], pb);
-  for (int i = 0; i < 3; ++i) kb2[i] -= 0.5 * dt * pb[i];
+  for (int k = 0; k < 3; ++k) kb2[k] -= 0.5 * dt * pb[k];
   // k2 = BLI[v2](P(X - dt/2 k1))
   pb[0] = pb[1] = pb[2] = 0;
-  interp_reverse<M, 3>(t, p2[0], p2[1], p2[2], v.p + 6, kb2, vbar.p + 6, pb);
+  leaf = interp_grad<M, 3>(t, p2[0], p2[1], p2[2], v.p + 6, kb2, pb, wx, we, valid);
+  wave_scatter_leaf<M, 3>(wl, valid ? leaf : -1, wx, we, kb2, vbar.p + 6, t.offset);
   project_adjoint(w2[0], w2[1], w2[2], pb);
-  for (int i = 0; i < 3; ++i) kb1[i] -= 0.5 * dt * pb[i];
+  for (int k = 0; k < 3; ++k) kb1[k] -= 0.5 * dt * pb[k];
   // k1 = v3 at the arrival node
-  vbar.p[9][u] = kb1[0];
-  vbar.p[10][u] = kb1[1];
-  vbar.p[11][u] = kb1[2];
+  if (valid) {
+    vbar.p[9][u] = kb1[0];
+    vbar.p[10][u] = kb1[1];
+    vbar.p[11][u] = kb1[2];
+  }
 }
 
-// Transpose of k_disp_interp: dbar_part[leaf nodes] += B dbar_prox (non-leaf proxies).
+// Transpose of k_disp_interp: dbar_part[leaf nodes] += B dbar_prox (non-leaf
+// proxies, taken in leaf order so that a wave mostly shares one leaf).
 template <int M>
 __global__ __launch_bounds__(256) void k_disp_interp_adj(DevTree t, CPtr3 dprox, Ptr3 dpart) {
+  __shared__ double lds[4][wave_scatter_lds<M, 3>()];
+  const int64_t cnt = (int64_t)t.offset * t.n;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)t.offset * t.n) return;
-  const int64_t v = t.proxy_order[i];
-  const int leaf = t.proxy_leaf[v];
-  if (leaf < 0) return;  // flagged by the forward interpolation
-  const double g0 = dprox.p[0][v], g1 = dprox.p[1][v], g2 = dprox.p[2][v];
-  if (g0 == 0.0 && g1 == 0.0 && g2 == 0.0) return;  // not a source
-  const DevPanel& P = t.panels[leaf];
+  const bool valid = i < cnt;
+  const int64_t v = t.proxy_order[valid ? i : cnt - 1];
+  int leaf = valid ? t.proxy_leaf[v] : -1;  // a failed location was flagged by the forward interpolation
   double wx[M], we[M];
-  bli_w<M>(t.proxy_xieta[2 * v], P.min_xi, P.max_xi, t.cheb_m, wx);
-  bli_w<M>(t.proxy_xieta[2 * v + 1], P.min_eta, P.max_eta, t.cheb_m, we);
-  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
-  scatter_leaf<M>(dpart.p[0] + base, wx, we, g0);
-  scatter_leaf<M>(dpart.p[1] + base, wx, we, g1);
-  scatter_leaf<M>(dpart.p[2] + base, wx, we, g2);
+  const double g[3] = {dprox.p[0][v], dprox.p[1][v], dprox.p[2][v]};
+  if (leaf >= 0) {
+    const DevPanel& P = t.panels[leaf];
+    bli_w<M>(t.proxy_xieta[2 * v], P.min_xi, P.max_xi, t.cheb_m, wx);
+    bli_w<M>(t.proxy_xieta[2 * v + 1], P.min_eta, P.max_eta, t.cheb_m, we);
+  }
+  wave_scatter_leaf<M, 3>(lds[threadIdx.x >> 6], leaf, wx, we, g, dpart.p, t.offset);
 }
 
 // Reverse of bve_disp_update (bve_time_step.cpp:91-104) for one stage, given
