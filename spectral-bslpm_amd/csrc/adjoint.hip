@@ -33,9 +33,10 @@ struct Ptr3 {
   double* p[3];
 };
 
-// Target record: undisplaced target node and c = psi cross x.
+// Target record: undisplaced target node and c = psi cross x (48 bytes, read
+// with wave-uniform scalar loads).
 struct TgtRec {
-  double x, y, z, pad0, c0, c1, c2, pad1;
+  double x, y, z, c0, c1, c2;
 };
 
 __global__ void k_tgt_rec(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, TgtRec* __restrict__ rec) {
@@ -50,7 +51,7 @@ __global__ void k_tgt_rec(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, TgtRec* __r
     a = psi_prox.p[0][v]; b = psi_prox.p[1][v]; d = psi_prox.p[2][v];
   }
   const double x = t.px[u], y = t.py[u], z = t.pz[u];
-  rec[u] = TgtRec{x, y, z, 0.0, b * z - d * y, d * x - a * z, a * y - b * x, 0.0};
+  rec[u] = TgtRec{x, y, z, b * z - d * y, d * x - a * z, a * y - b * x};
 }
 
 // Transposed pair sums.  POS: also the position cotangent (displaced stages).
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(256) void k_bve_adj64(DevTree t, const TargetTile* 
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet G = targets[e];
     const TgtRec* __restrict__ r = trec + G.first;
-#pragma unroll 2
+#pragma unroll 4
     for (int k = 0; k < G.count; ++k) {
       const TgtRec R = r[k];
       // the forward kernel's denominator, x = target, y = source
