@@ -81,6 +81,19 @@ int sbslpm_tree_locate(const sbslpm_tree* tree, double x, double y, double z);
 /* Fixed transfer matrices: m2m_hi/m2m_lo [(d+1)*(d+1)] (parent basis a at upper/lower-half child node k,
  * index a*(d+1)+k) and p2m [(d+1)*m] (proxy basis a at particle node k). Any pointer may be NULL. */
 int sbslpm_tree_transfer_matrices(const sbslpm_tree* tree, double* m2m_hi, double* m2m_lo, double* p2m);
+/* Host-only view of the irregular particle tree (same construction as sbslpm_irreg_create). */
+typedef struct sbslpm_irreg_tree sbslpm_irreg_tree;
+int sbslpm_irreg_tree_build(int64_t n, const double* x, const double* y, const double* z, int leaf_size,
+                            int interp_degree, double theta, sbslpm_irreg_tree** out);
+void sbslpm_irreg_tree_destroy(sbslpm_irreg_tree* tree);
+int sbslpm_irreg_tree_sizes(const sbslpm_irreg_tree* tree, int64_t* sizes);
+/* ints [P][10]: face, level, parent, child x4, is_leaf, point_count, first sorted point;
+ * dbls [P][5]: min_xi, max_xi, min_eta, max_eta (radians), radius */
+int sbslpm_irreg_tree_panels(const sbslpm_irreg_tree* tree, int* ints, double* dbls);
+/* perm [N]: input index of each leaf-sorted point (leaves in panel order, points in the
+ * reference's leaf_panel_point_ids order) */
+int sbslpm_irreg_tree_points(const sbslpm_irreg_tree* tree, int* perm);
+int sbslpm_irreg_tree_interaction_list(const sbslpm_irreg_tree* tree, int* list);
 /* Unique particle positions, cubed_sphere_2d_to_1d (initialize_cubed_sphere.cpp:290-324):
  * particles in [leaf][node] order; one within 1e-15 in every coordinate of an earlier unique
  * point maps to the first such point, otherwise it becomes the next unique point.
@@ -269,6 +282,44 @@ int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* ctx, const double* x, const dou
  * departure points.  Accumulated with FP64 atomic adds (summation order not fixed). */
 int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* ctx, const double* dep_x, const double* dep_y,
                                            const double* dep_z, const double* new_vor_bar, double* vor_bar);
+
+/* ---- Irregular particle tree (the lat-lon path, SURVEY layout B) -------------------------
+ * Points in any order (the reference's [lat][lon] arrays flattened, index i*lon_count + j).
+ * Face panels split into quadrants while they hold more than leaf_size points
+ * (initialize_cube_sphere_irreg_points, initialize_cubed_sphere.cpp:446-592; panel bounds in
+ * radians); interactions from dual_tree_traversal_irreg (fmm_funcs.cpp:125-231), whose type
+ * bits mark panels above the deepest level (proxies) rather than non-leaves.  The device handle
+ * uses the context's stream, theta and kernel_eps; point arrays are device pointers in the
+ * caller's order, proxy arrays [P][(d+1)^2] in the tree's panel order (the reference's). */
+typedef struct sbslpm_irreg sbslpm_irreg;
+int sbslpm_irreg_create(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z, int leaf_size,
+                        int interp_degree, sbslpm_irreg** out);  /* x, y, z: host arrays */
+void sbslpm_irreg_destroy(sbslpm_irreg* h);
+/* sizes: P, deepest level, (d+1)^2, points, interactions, work tiles */
+int sbslpm_irreg_sizes(const sbslpm_irreg* h, int64_t* sizes);
+/* upward_pass_ll (fmm_funcs.cpp:394-404): proxy_source_pots assigned (the reference
+ * accumulates into caller-zeroed rows); base_pots_ll by BLI anterpolation of each leaf's
+ * points, child_to_parent_ll by the fixed quadrant matrices (no race). */
+int sbslpm_irreg_upward_pass(sbslpm_irreg* h, const double* area, const double* pots, double* proxy_source_pots);
+/* downward_pass_3_ll (fmm_funcs.cpp:406-419): proxy rows pushed to the children in place,
+ * then sols += BLI interpolation of each leaf's proxies at its points. */
+int sbslpm_irreg_downward_pass_3(sbslpm_irreg* h, double* proxy_1, double* proxy_2, double* proxy_3, double* sols_1,
+                                 double* sols_2, double* sols_3);
+/* swe_vel_interactions_ll (swe_vel.cpp:69-100, swe_vel_impl.hpp:149-288): target_pots [P][n] and
+ * vel [N] accumulated; point sources carry vor*area, div*area. */
+int sbslpm_irreg_swe_vel_interactions(sbslpm_irreg* h, double* target_pots_1, double* target_pots_2,
+                                      double* target_pots_3, double* vel_x, double* vel_y, double* vel_z,
+                                      const double* proxy_vors, const double* proxy_divs, const double* vors,
+                                      const double* divs, const double* area);
+/* The lat-lon shallow-water velocity sequence (upward passes of vor and div, interactions,
+ * downward pass) fused; vel assigned. */
+int sbslpm_irreg_swe_velocity(sbslpm_irreg* h, const double* area, const double* vors, const double* divs,
+                              double* vel_x, double* vel_y, double* vel_z);
+/* Biot-Savart velocity of a vorticity field on the irregular tree (the kernel of
+ * bve_vel_impl.hpp:90-103 with undisplaced sources); vel assigned.  No reference counterpart
+ * on this tree. */
+int sbslpm_irreg_bve_velocity(sbslpm_irreg* h, const double* area, const double* vor, double* vel_x, double* vel_y,
+                              double* vel_z);
 
 /* BLI interpolation of nfields (1..8) leaf fields [Pl][m2] at n arbitrary points (x, y, z on
  * the unit sphere): out[q][i] = field q interpolated at point i on the leaf that contains it.

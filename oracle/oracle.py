@@ -92,6 +92,18 @@ def lib():
         L.orc_xyz_vel_to_uv_vel.argtypes = [ctypes.c_int64] + [_dp] * 8
         L.orc_dedupe.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp]
         L.orc_dedupe.restype = ctypes.c_int64
+        L.orc_irreg_create.restype = ctypes.c_void_p
+        L.orc_irreg_create.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.orc_irreg_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_irreg_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+        L.orc_irreg_panels.argtypes = [ctypes.c_void_p, _ip, _dp, _ip]
+        L.orc_irreg_interactions.argtypes = [ctypes.c_void_p, _ip]
+        L.orc_irreg_upward_pass.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_irreg_upward_pass.restype = ctypes.c_int
+        L.orc_irreg_downward_pass_3.argtypes = [ctypes.c_void_p, _ppd, _ppd]
+        L.orc_irreg_downward_pass_3.restype = ctypes.c_int
+        L.orc_irreg_swe_vel_interactions.argtypes = [ctypes.c_void_p, _ppd, _ppd, _dp, _dp, _dp, _dp, _dp,
+                                                     ctypes.c_double]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -407,3 +419,67 @@ def dedupe(x, y, z):
     o = [np.zeros(n) for _ in range(3)]
     k = lib().orc_dedupe(n, _p(x), _p(y), _p(z), m.ctypes.data_as(_ip), c.ctypes.data_as(_ip), *[_p(t) for t in o])
     return m, c[:k].copy(), o[0][:k].copy(), o[1][:k].copy(), o[2][:k].copy()
+
+
+class OracleIrreg:
+    """The irregular particle tree of the lat-lon path and its _ll passes
+    (initialize_cubed_sphere.cpp:446-592, fmm_funcs.cpp:125-231, 394-419,
+    swe_vel_impl.hpp:149-288), restated sequentially."""
+
+    def __init__(self, x, y, z, leaf_size, degree, theta=0.7):
+        self.x, self.y, self.z = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z)]
+        self.N = self.x.size
+        self.degree, self.n = degree, (degree + 1) ** 2
+        self.h = lib().orc_irreg_create(self.N, _p(self.x), _p(self.y), _p(self.z), leaf_size, degree, theta)
+        s = np.zeros(3, dtype=np.int64)
+        lib().orc_irreg_sizes(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.P, self.levels, self.I = [int(v) for v in s]
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib().orc_irreg_destroy(self.h)
+            self.h = None
+
+    def panels(self):
+        ints = np.zeros((self.P, 9), dtype=np.int32)
+        dbls = np.zeros((self.P, 5))
+        pts = np.zeros(self.N, dtype=np.int32)
+        lib().orc_irreg_panels(self.h, ints.ctypes.data_as(_ip), _p(dbls), pts.ctypes.data_as(_ip))
+        return ints, dbls, pts
+
+    def interactions(self):
+        out = np.zeros((self.I, 3), dtype=np.int32)
+        lib().orc_irreg_interactions(self.h, out.ctypes.data_as(_ip))
+        return out
+
+    def upward_pass(self, area, pots):
+        q = np.zeros((self.P, self.n))
+        a, p = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (area, pots)]
+        if lib().orc_irreg_upward_pass(self.h, _p(a), _p(p), _p(q)):
+            raise RuntimeError("oracle irreg upward pass: geometry error")
+        return q
+
+    def downward_pass_3(self, proxy, sols=None):
+        proxy = [np.ascontiguousarray(v, dtype=np.float64).copy() for v in proxy]
+        sols = [np.zeros(self.N) for _ in range(3)] if sols is None else [np.array(v, dtype=np.float64) for v in sols]
+        if lib().orc_irreg_downward_pass_3(self.h, _pp(proxy), _pp(sols)):
+            raise RuntimeError("oracle irreg downward pass: geometry error")
+        return proxy, sols
+
+    def swe_vel_interactions(self, proxy_vors, proxy_divs, vors, divs, area, eps):
+        tp = [np.zeros((self.P, self.n)) for _ in range(3)]
+        vel = [np.zeros(self.N) for _ in range(3)]
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (proxy_vors, proxy_divs)]
+        b = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (vors, divs, area)]
+        lib().orc_irreg_swe_vel_interactions(self.h, _pp(tp), _pp(vel), _p(a[0]), _p(a[1]), _p(b[0]), _p(b[1]),
+                                             _p(b[2]), eps)
+        return tp, vel
+
+    def swe_velocity(self, area, vors, divs, eps):
+        """The lat-lon SWE velocity sequence: upward passes of both charges, interactions,
+        downward pass (swe_time_step_ll.cpp's use of the _ll functors)."""
+        qv = self.upward_pass(area, vors)
+        qd = self.upward_pass(area, divs)
+        tp, vel = self.swe_vel_interactions(qv, qd, vors, divs, area, eps)
+        _, vel = self.downward_pass_3(tp, vel)
+        return vel

@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <queue>
+#include <deque>
 #include <vector>
 #include <algorithm>
 #include <omp.h>
@@ -1200,6 +1201,289 @@ int64_t orc_dedupe(int64_t n, const double* x, const double* y, const double* z,
     two_d_to_1d[i] = (int)k;
   }
   return found;
+}
+
+// ---- irregular particle tree and its _ll passes -------------------------------
+// initialize_cube_sphere_irreg_points (initialize_cubed_sphere.cpp:446-592),
+// dual_tree_traversal_irreg (fmm_funcs.cpp:125-231), base_pots_ll /
+// child_to_parent_ll / parent_to_child_3_ll / child_panel_interp_3_ll
+// (fmm_funcs_impl.hpp:190-364) and swe_vel_panel_interaction_ll
+// (swe_vel_impl.hpp:149-288), restated sequentially: every functor is run
+// panel by panel or pair by pair in index order (no atomics, so the race of
+// child_to_parent_ll, fmm_funcs_impl.hpp:258, does not arise).
+struct IrrPanel {
+  int face, level, parent, child[4], is_leaf, point_count;
+  double min_xi, max_xi, min_eta, max_eta, radius;
+};
+struct IrrCtx {
+  std::vector<IrrPanel> pn;
+  std::vector<std::vector<int>> pts;
+  std::vector<int> level_start, ilist;
+  std::vector<double> x, y, z;
+  int levels = 0, degree = 0, leaf_size = 0;
+  double theta = 0;
+};
+
+void* orc_irreg_create(int64_t n, const double* x, const double* y, const double* z, int leaf_size, int degree,
+                       double theta) {
+  IrrCtx* c = new IrrCtx();
+  c->x.assign(x, x + n); c->y.assign(y, y + n); c->z.assign(z, z + n);
+  c->degree = degree; c->leaf_size = leaf_size; c->theta = theta;
+  auto& P = c->pn;
+  P.resize(6);
+  c->pts.resize(6);
+  for (int i = 0; i < 6; i++) {
+    P[i] = IrrPanel{i, 0, -1, {-1, -1, -1, -1}, 1, 0, -M_PI / 4.0, M_PI / 4.0, -M_PI / 4.0, M_PI / 4.0, 0};
+  }
+  for (int64_t i = 0; i < n; i++) {
+    int f = face_from_xyz(x[i], y[i], z[i]);
+    P[f].point_count += 1;
+    c->pts[f].push_back((int)i);
+  }
+  for (size_t i = 0; i < P.size(); i++) {
+    if (P[i].point_count > leaf_size) {
+      P[i].is_leaf = 0;
+      double min_xi = P[i].min_xi, max_xi = P[i].max_xi, min_eta = P[i].min_eta, max_eta = P[i].max_eta;
+      double mid_xi = 0.5 * (min_xi + max_xi), mid_eta = 0.5 * (min_eta + max_eta);
+      IrrPanel s1{}, s2{}, s3{}, s4{};
+      s1.min_xi = mid_xi; s1.max_xi = max_xi; s1.min_eta = mid_eta; s1.max_eta = max_eta;
+      s2.min_xi = min_xi; s2.max_xi = mid_xi; s2.min_eta = mid_eta; s2.max_eta = max_eta;
+      s3.min_xi = min_xi; s3.max_xi = mid_xi; s3.min_eta = min_eta; s3.max_eta = mid_eta;
+      s4.min_xi = mid_xi; s4.max_xi = max_xi; s4.min_eta = min_eta; s4.max_eta = mid_eta;
+      int start = (int)P.size();
+      P.push_back(s1); P.push_back(s2); P.push_back(s3); P.push_back(s4);
+      for (int j = 0; j < 4; j++) {
+        P[start + j].level = P[i].level + 1;
+        P[start + j].face = P[i].face;
+        P[start + j].parent = (int)i;
+        P[start + j].point_count = 0;
+        P[start + j].is_leaf = 1;
+        for (int k = 0; k < 4; k++) P[start + j].child[k] = -1;
+        P[i].child[j] = start + j;
+      }
+      c->pts.resize(start + 4);
+      for (int idx : c->pts[i]) {
+        double xieta[2];
+        xieta_from_xyz(x[idx], y[idx], z[idx], xieta);
+        int which;
+        if (xieta[0] < mid_xi) which = xieta[1] < mid_eta ? 2 : 1;
+        else which = xieta[1] < mid_eta ? 3 : 0;
+        P[start + which].point_count += 1;
+        c->pts[start + which].push_back(idx);
+      }
+    }
+  }
+  for (auto& p : P) {
+    double mid_xi = (p.min_xi + p.max_xi) * 0.5, mid_eta = (p.min_eta + p.max_eta) * 0.5;
+    double pc[3], p1[3], p2[3], p3[3], p4[3];
+    xyz_from_xieta(mid_xi, mid_eta, p.face, pc);
+    xyz_from_xieta(p.max_xi, p.min_eta, p.face, p1);
+    xyz_from_xieta(p.max_xi, p.max_eta, p.face, p2);
+    xyz_from_xieta(p.min_xi, p.max_eta, p.face, p3);
+    xyz_from_xieta(p.min_xi, p.min_eta, p.face, p4);
+    double d1 = std::acos(pc[0] * p1[0] + pc[1] * p1[1] + pc[2] * p1[2]);
+    double d2 = std::acos(pc[0] * p2[0] + pc[1] * p2[1] + pc[2] * p2[2]);
+    double d3 = std::acos(pc[0] * p3[0] + pc[1] * p3[1] + pc[2] * p3[2]);
+    double d4 = std::acos(pc[0] * p4[0] + pc[1] * p4[1] + pc[2] * p4[2]);
+    p.radius = std::max(std::max(std::max(d1, d2), d3), d4);
+  }
+  c->levels = 0;
+  for (auto& p : P) c->levels = std::max(c->levels, p.level);
+  c->level_start.assign(c->levels + 2, (int)P.size());
+  int target = 0;
+  for (size_t i = 0; i < P.size(); i++)
+    if (P[i].level == target) { c->level_start[target] = (int)i; target++; }
+  // traversal
+  std::deque<int> tq, sq;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) { tq.push_back(i); sq.push_back(j); }
+  while (!tq.empty()) {
+    int it = tq.front(), is = sq.front();
+    tq.pop_front(); sq.pop_front();
+    const IrrPanel &T = P[it], &S = P[is];
+    double c1[3], c2[3];
+    xyz_from_xieta(0.5 * (T.min_xi + T.max_xi), 0.5 * (T.min_eta + T.max_eta), T.face, c1);
+    xyz_from_xieta(0.5 * (S.min_xi + S.max_xi), 0.5 * (S.min_eta + S.max_eta), S.face, c2);
+    double dist = gcdist(c1, c2), sep = 100;
+    if (dist > 1e-16) sep = (T.radius + S.radius) / dist;
+    auto push4 = [&](int t0, const int* ts, int s0, const int* ss) {
+      for (int k = 0; k < 4; k++) { tq.push_back(ts ? ts[k] : t0); sq.push_back(ss ? ss[k] : s0); }
+    };
+    if (sep < theta) {
+      int type = (T.level < c->levels ? 2 : 0) + (S.level < c->levels ? 1 : 0);
+      c->ilist.insert(c->ilist.end(), {it, is, type});
+    } else if (T.is_leaf && S.is_leaf) {
+      c->ilist.insert(c->ilist.end(), {it, is, 0});
+    } else if (T.is_leaf) {
+      push4(it, nullptr, 0, S.child);
+    } else if (S.is_leaf) {
+      push4(0, T.child, is, nullptr);
+    } else if (T.level <= S.level) {
+      push4(0, T.child, is, nullptr);
+    } else {
+      push4(it, nullptr, 0, S.child);
+    }
+  }
+  return c;
+}
+
+void orc_irreg_destroy(void* h) { delete static_cast<IrrCtx*>(h); }
+
+void orc_irreg_sizes(void* h, int64_t* s) {
+  const IrrCtx& c = *static_cast<IrrCtx*>(h);
+  s[0] = (int64_t)c.pn.size(); s[1] = c.levels; s[2] = (int64_t)c.ilist.size() / 3;
+}
+
+// ints [P][9]: face, level, parent, child x4, is_leaf, point_count; dbls [P][5]: bounds, radius;
+// pts: the leaf point lists concatenated in panel order.
+void orc_irreg_panels(void* h, int* ints, double* dbls, int* pts) {
+  const IrrCtx& c = *static_cast<IrrCtx*>(h);
+  int64_t o = 0;
+  for (size_t i = 0; i < c.pn.size(); i++) {
+    const IrrPanel& p = c.pn[i];
+    int r[9] = {p.face, p.level, p.parent, p.child[0], p.child[1], p.child[2], p.child[3], p.is_leaf, p.point_count};
+    std::memcpy(ints + 9 * i, r, sizeof(r));
+    double d[5] = {p.min_xi, p.max_xi, p.min_eta, p.max_eta, p.radius};
+    std::memcpy(dbls + 5 * i, d, sizeof(d));
+    if (p.is_leaf)
+      for (int id : c.pts[i]) pts[o++] = id;
+  }
+}
+
+void orc_irreg_interactions(void* h, int* out) {
+  const IrrCtx& c = *static_cast<IrrCtx*>(h);
+  std::memcpy(out, c.ilist.data(), c.ilist.size() * sizeof(int));
+}
+
+static void irr_node(const IrrPanel& p, int j, int k, int degree, double* xi, double* eta) {
+  double xo = 0.5 * (p.min_xi + p.max_xi), xs = 0.5 * (p.max_xi - p.min_xi);
+  double eo = 0.5 * (p.min_eta + p.max_eta), es = 0.5 * (p.max_eta - p.min_eta);
+  *xi = std::cos(M_PI / degree * j) * xs + xo;
+  *eta = std::cos(M_PI / degree * k) * es + eo;
+}
+
+// upward_pass_ll (fmm_funcs.cpp:394-404): proxy [P][n] accumulated (caller zeroes).
+int orc_irreg_upward_pass(void* h, const double* area, const double* pots, double* proxy) {
+  const IrrCtx& c = *static_cast<IrrCtx*>(h);
+  const int d = c.degree, n = (d + 1) * (d + 1);
+  int err = 0;
+  for (size_t i = 0; i < c.pn.size(); i++) {  // base_pots_ll
+    if (!c.pn[i].is_leaf) continue;
+    for (int idx : c.pts[i]) {
+      double xieta[2], basis[121];
+      xieta_from_xyz(c.x[idx], c.y[idx], c.z[idx], xieta);
+      err |= interp_vals_bli(basis, xieta[0], xieta[1], c.pn[i].min_xi, c.pn[i].max_xi, c.pn[i].min_eta,
+                             c.pn[i].max_eta, d);
+      for (int k = 0; k < n; k++) proxy[i * n + k] += basis[k] * pots[idx] * area[idx];
+    }
+  }
+  for (int l = c.levels; l > 0; l--) {  // child_to_parent_ll
+    for (int i = c.level_start[l]; i < c.level_start[l + 1]; i++) {
+      const IrrPanel& ch = c.pn[i];
+      const IrrPanel& par = c.pn[ch.parent];
+      for (int j = 0; j <= d; j++)
+        for (int k = 0; k <= d; k++) {
+          double xi, eta, basis[121];
+          irr_node(ch, j, k, d, &xi, &eta);
+          err |= interp_vals_bli(basis, xi, eta, par.min_xi, par.max_xi, par.min_eta, par.max_eta, d);
+          for (int m = 0; m < n; m++) proxy[ch.parent * n + m] += basis[m] * proxy[i * n + j * (d + 1) + k];
+        }
+    }
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// downward_pass_3_ll (fmm_funcs.cpp:406-419): proxy rows pushed to children in place,
+// then sols[f][point] += leaf interpolation.
+int orc_irreg_downward_pass_3(void* h, double* const* proxy, double* const* sols) {
+  const IrrCtx& c = *static_cast<IrrCtx*>(h);
+  const int d = c.degree, n = (d + 1) * (d + 1);
+  int err = 0;
+  for (int l = 0; l < c.levels; l++) {
+    for (int i = c.level_start[l]; i < c.level_start[l + 1]; i++) {
+      const IrrPanel& par = c.pn[i];
+      if (par.is_leaf) continue;
+      for (int q = 0; q < 4; q++) {
+        const int ch = par.child[q];
+        for (int k = 0; k < n; k++) {
+          double xi, eta, basis[121];
+          irr_node(c.pn[ch], k / (d + 1), k % (d + 1), d, &xi, &eta);
+          err |= interp_vals_bli(basis, xi, eta, par.min_xi, par.max_xi, par.min_eta, par.max_eta, d);
+          for (int f = 0; f < 3; f++)
+            for (int m = 0; m < n; m++) proxy[f][ch * n + k] += basis[m] * proxy[f][i * n + m];
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < c.pn.size(); i++) {
+    if (!c.pn[i].is_leaf) continue;
+    for (int idx : c.pts[i]) {
+      double xieta[2], basis[121];
+      xieta_from_xyz(c.x[idx], c.y[idx], c.z[idx], xieta);
+      err |= interp_vals_bli(basis, xieta[0], xieta[1], c.pn[i].min_xi, c.pn[i].max_xi, c.pn[i].min_eta,
+                             c.pn[i].max_eta, d);
+      for (int f = 0; f < 3; f++)
+        for (int k = 0; k < n; k++) sols[f][idx] += basis[k] * proxy[f][i * n + k];
+    }
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// swe_vel_interactions_ll (swe_vel.cpp:69-100): target_pots [3][P][n] and vel [3][N] accumulated.
+void orc_irreg_swe_vel_interactions(void* h, double* const* target_pots, double* const* vel, const double* proxy_vors,
+                                    const double* proxy_divs, const double* vors, const double* divs,
+                                    const double* area, double eps) {
+  const IrrCtx& c = *static_cast<IrrCtx*>(h);
+  const int d = c.degree, n = (d + 1) * (d + 1);
+  const int I = (int)c.ilist.size() / 3;
+  for (int e = 0; e < I; e++) {
+    const int tp = c.ilist[3 * e], sp = c.ilist[3 * e + 1], it = c.ilist[3 * e + 2];
+    std::vector<double> tx, ty, tz, sx, sy, sz, sv, sd;
+    std::vector<int> tid;
+    if (it == 0 || it == 1) {
+      for (int idx : c.pts[tp]) { tx.push_back(c.x[idx]); ty.push_back(c.y[idx]); tz.push_back(c.z[idx]); tid.push_back(idx); }
+    } else {
+      for (int j = 0; j <= d; j++)
+        for (int k = 0; k <= d; k++) {
+          const IrrPanel& p = c.pn[tp];
+          double xo = 0.5 * (p.min_xi + p.max_xi), xs = 0.5 * (p.max_xi - p.min_xi);
+          double eo = 0.5 * (p.min_eta + p.max_eta), es = 0.5 * (p.max_eta - p.min_eta), v[3];
+          xyz_from_xieta(std::cos(M_PI * j / d) * xs + xo, std::cos(M_PI * k / d) * es + eo, p.face, v);
+          tx.push_back(v[0]); ty.push_back(v[1]); tz.push_back(v[2]); tid.push_back(j * (d + 1) + k);
+        }
+    }
+    if (it == 0 || it == 2) {
+      for (int idx : c.pts[sp]) {
+        sx.push_back(c.x[idx]); sy.push_back(c.y[idx]); sz.push_back(c.z[idx]);
+        sv.push_back(vors[idx] * area[idx]); sd.push_back(divs[idx] * area[idx]);
+      }
+    } else {
+      for (int j = 0; j <= d; j++)
+        for (int k = 0; k <= d; k++) {
+          const IrrPanel& p = c.pn[sp];
+          double xo = 0.5 * (p.min_xi + p.max_xi), xs = 0.5 * (p.max_xi - p.min_xi);
+          double eo = 0.5 * (p.min_eta + p.max_eta), es = 0.5 * (p.max_eta - p.min_eta), v[3];
+          xyz_from_xieta(std::cos(M_PI * j / d) * xs + xo, std::cos(M_PI * k / d) * es + eo, p.face, v);
+          sx.push_back(v[0]); sy.push_back(v[1]); sz.push_back(v[2]);
+          sv.push_back(proxy_vors[sp * n + j * (d + 1) + k]); sd.push_back(proxy_divs[sp * n + j * (d + 1) + k]);
+        }
+    }
+    for (size_t j = 0; j < tx.size(); j++) {
+      for (size_t k = 0; k < sx.size(); k++) {
+        double dp = tx[j] * sx[k] + ty[j] * sy[k] + tz[j] * sz[k];
+        double gf = 0.07957747154594767 / (1.0 - dp + eps);
+        double gf1 = -gf * sv[k], gf2 = gf * sd[k];
+        double vx = gf1 * (ty[j] * sz[k] - tz[j] * sy[k]) + gf2 * (sx[k] - tx[j] * dp);
+        double vy = gf1 * (tz[j] * sx[k] - tx[j] * sz[k]) + gf2 * (sy[k] - ty[j] * dp);
+        double vz = gf1 * (tx[j] * sy[k] - ty[j] * sx[k]) + gf2 * (sz[k] - tz[j] * dp);
+        if (it == 0 || it == 1) {
+          vel[0][tid[j]] += vx; vel[1][tid[j]] += vy; vel[2][tid[j]] += vz;
+        } else {
+          target_pots[0][tp * n + tid[j]] += vx; target_pots[1][tp * n + tid[j]] += vy; target_pots[2][tp * n + tid[j]] += vz;
+        }
+      }
+    }
+  }
 }
 
 void orc_set_threads(int n) { omp_set_num_threads(n); }

@@ -55,6 +55,12 @@ SIGNATURES = {
     "sbslpm_tree_locate": (_I, [_V, _D, _D, _D]),
     "sbslpm_tree_transfer_matrices": (_I, [_V, _PD, _PD, _PD]),
     "sbslpm_tree_dedupe": (_I, [_V, _PI, _PI, _PD, _PD, _PD, _PI64]),
+    "sbslpm_irreg_tree_build": (_I, [ctypes.c_int64, _PD, _PD, _PD, _I, _I, ctypes.c_double, _V]),
+    "sbslpm_irreg_tree_destroy": (None, [_V]),
+    "sbslpm_irreg_tree_sizes": (_I, [_V, _PI64]),
+    "sbslpm_irreg_tree_panels": (_I, [_V, _PI, _PD]),
+    "sbslpm_irreg_tree_points": (_I, [_V, _PI]),
+    "sbslpm_irreg_tree_interaction_list": (_I, [_V, _PI]),
     "sbslpm_ctx_create": (_I, [ctypes.POINTER(Params), ctypes.POINTER(_V)]),
     "sbslpm_ctx_destroy": (None, [_V]),
     "sbslpm_ctx_set_stream": (_I, [_V, _V]),
@@ -82,6 +88,14 @@ SIGNATURES = {
     "sbslpm_bve_departure_to_target_adjoint": (_I, [_V] * 6),
     "sbslpm_swe_displacement": (_I, [_V, _V, _V, _V, ctypes.c_double] + [_V] * 6),
     "sbslpm_interp_points": (_I, [_V, ctypes.c_int64, _V, _V, _V, _I, _V, _V]),
+    "sbslpm_irreg_create": (_I, [_V, ctypes.c_int64, _PD, _PD, _PD, _I, _I, _V]),
+    "sbslpm_irreg_destroy": (None, [_V]),
+    "sbslpm_irreg_sizes": (_I, [_V, _PI64]),
+    "sbslpm_irreg_upward_pass": (_I, [_V] * 4),
+    "sbslpm_irreg_downward_pass_3": (_I, [_V] * 7),
+    "sbslpm_irreg_swe_vel_interactions": (_I, [_V] * 12),
+    "sbslpm_irreg_swe_velocity": (_I, [_V] * 7),
+    "sbslpm_irreg_bve_velocity": (_I, [_V] * 6),
     "sbslpm_xyz_vel_to_uv_vel": (_I, [_V, ctypes.c_int64] + [_V] * 8),
     "sbslpm_swe_find_departure_points": (_I, [_V, _V, _V, _V, _V, ctypes.c_double, _V, _V, _V]),
     "sbslpm_swe_departure_to_target": (_I, [_V] * 8 + [_I] + [_V] * 4),

@@ -130,6 +130,45 @@ class HostTree:
         return m, cnt[:k].copy(), xyz[0][:k].copy(), xyz[1][:k].copy(), xyz[2][:k].copy()
 
 
+class HostIrregTree:
+    """Host view of the irregular particle tree (initialize_cubed_sphere.cpp:446-592,
+    fmm_funcs.cpp:125-231); points x, y, z are flat arrays in any order."""
+
+    def __init__(self, x, y, z, leaf_size, interp_degree, theta=0.7):
+        self._lib = load()
+        a = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z)]
+        h = ctypes.c_void_p()
+        rc = self._lib.sbslpm_irreg_tree_build(a[0].size, *[_dp(v) for v in a], int(leaf_size), int(interp_degree),
+                                               float(theta), ctypes.byref(h))
+        if rc:
+            raise SbslpmError(rc, "irreg_tree_build: invalid parameters")
+        self.h = h
+        s = np.zeros(6, dtype=np.int64)
+        self._lib.sbslpm_irreg_tree_sizes(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.P, self.levels, self.n, self.N, self.I, self.tiles = [int(v) for v in s]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.sbslpm_irreg_tree_destroy(self.h)
+            self.h = None
+
+    def panels(self):
+        ints = np.zeros((self.P, 10), dtype=np.int32)
+        dbls = np.zeros((self.P, 5))
+        self._lib.sbslpm_irreg_tree_panels(self.h, _ip(ints), _dp(dbls))
+        return ints, dbls
+
+    def points(self):
+        perm = np.zeros(self.N, dtype=np.int32)
+        self._lib.sbslpm_irreg_tree_points(self.h, _ip(perm))
+        return perm
+
+    def interaction_list(self):
+        out = np.zeros((self.I, 3), dtype=np.int32)
+        self._lib.sbslpm_irreg_tree_interaction_list(self.h, _ip(out))
+        return out
+
+
 def _ptr(t):
     """Device pointer of a contiguous float64 CUDA tensor (None -> NULL)."""
     if t is None:
@@ -394,6 +433,11 @@ class Context:
                                                           _ptr(vor_bar)), "bve_velocity_adjoint")
         return vor_bar
 
+    def irregular(self, x, y, z, leaf_size):
+        """Irregular particle tree over host points x, y, z (flat, any order) using this
+        context's stream, interp_degree, theta and kernel_eps."""
+        return IrregContext(self, x, y, z, leaf_size)
+
     def interp_points(self, x, y, z, fields, out=None):
         """BLI of leaf fields [Pl, m2] (a list of up to 8 tensors) at the points (x, y, z) (1-D
         tensors of one length): latlon_interp_host, interp_funcs.hpp:22-60."""
@@ -470,3 +514,61 @@ class Context:
                                                        _ptr(proxy_source_vors), *[_ptr(t) for t in vel],
                                                        _ptr(tracers), float(time)), "bve_back_rk4_step")
         return vors
+
+
+class IrregContext:
+    """Device handle of an irregular particle tree (the lat-lon path's _ll operators)."""
+
+    def __init__(self, ctx, x, y, z, leaf_size):
+        self.ctx, self._lib, self.torch = ctx, ctx._lib, ctx.torch
+        a = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z)]
+        h = ctypes.c_void_p()
+        ctx._check(self._lib.sbslpm_irreg_create(ctx.h, a[0].size, *[_dp(v) for v in a], int(leaf_size),
+                                                 int(ctx.cfg.interp_degree), ctypes.byref(h)), "irreg_create")
+        self.h = h
+        s = np.zeros(6, dtype=np.int64)
+        self._lib.sbslpm_irreg_sizes(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.P, self.levels, self.n, self.N, self.I, self.tiles = [int(v) for v in s]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.sbslpm_irreg_destroy(self.h)
+            self.h = None
+
+    def _zeros(self, *shape):
+        return self.torch.zeros(shape, dtype=self.torch.float64, device="cuda")
+
+    def upward_pass(self, area, pots, proxy=None):
+        """upward_pass_ll: proxy charges [P, n] (assigned)."""
+        proxy = self._zeros(self.P, self.n) if proxy is None else proxy
+        self.ctx._check(self._lib.sbslpm_irreg_upward_pass(self.h, _ptr(area), _ptr(pots), _ptr(proxy)),
+                        "irreg_upward_pass")
+        return proxy
+
+    def downward_pass_3(self, proxy, sols=None):
+        """downward_pass_3_ll: proxy (3 tensors [P, n]) pushed down in place; sols += L2P."""
+        sols = [self._zeros(self.N) for _ in range(3)] if sols is None else sols
+        self.ctx._check(self._lib.sbslpm_irreg_downward_pass_3(self.h, *[_ptr(p) for p in proxy],
+                                                               *[_ptr(t) for t in sols]), "irreg_downward_pass_3")
+        return sols
+
+    def swe_vel_interactions(self, proxy_vors, proxy_divs, vors, divs, area, target_pots=None, vel=None):
+        """swe_vel_interactions_ll: target_pots (3 x [P, n]) and vel (3 x [N]) accumulated."""
+        target_pots = [self._zeros(self.P, self.n) for _ in range(3)] if target_pots is None else target_pots
+        vel = [self._zeros(self.N) for _ in range(3)] if vel is None else vel
+        self.ctx._check(self._lib.sbslpm_irreg_swe_vel_interactions(
+            self.h, *[_ptr(t) for t in target_pots], *[_ptr(t) for t in vel], _ptr(proxy_vors), _ptr(proxy_divs),
+            _ptr(vors), _ptr(divs), _ptr(area)), "irreg_swe_vel_interactions")
+        return target_pots, vel
+
+    def swe_velocity(self, area, vors, divs, vel=None):
+        vel = [self._zeros(self.N) for _ in range(3)] if vel is None else vel
+        self.ctx._check(self._lib.sbslpm_irreg_swe_velocity(self.h, _ptr(area), _ptr(vors), _ptr(divs),
+                                                            *[_ptr(t) for t in vel]), "irreg_swe_velocity")
+        return vel
+
+    def bve_velocity(self, area, vor, vel=None):
+        vel = [self._zeros(self.N) for _ in range(3)] if vel is None else vel
+        self.ctx._check(self._lib.sbslpm_irreg_bve_velocity(self.h, _ptr(area), _ptr(vor), *[_ptr(t) for t in vel]),
+                        "irreg_bve_velocity")
+        return vel

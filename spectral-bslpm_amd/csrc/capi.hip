@@ -11,6 +11,7 @@
 
 #include "../../include/sbslpm.h"
 #include "device.hpp"
+#include "irreg_tree.hpp"
 #include "tree.hpp"
 
 using namespace sbslpm;
@@ -115,6 +116,34 @@ struct sbslpm_ctx {
   // hipGraph of one BVE step, keyed by the caller's pointers.
   hipGraphExec_t graph = nullptr;
   const void* graph_key[10] = {};
+};
+
+// Irregular particle tree on the device (the lat-lon path), bound to a context
+// whose stream and error word it uses.
+struct sbslpm_irreg {
+  sbslpm_ctx* ctx = nullptr;
+  IrregTree tree;
+  DevTree dt{};
+  char* arena = nullptr;
+  int* perm = nullptr;
+  int* leaf_first = nullptr;
+  int* leaf_count = nullptr;
+  int* point_leaf = nullptr;
+  int* leaves = nullptr;
+  int nleaves = 0;
+  int* nonleaf = nullptr;
+  TargetTile* tiles = nullptr;
+  int ntiles = 0;
+  SourceSet* sets = nullptr;
+  int* zero_rows = nullptr;
+  int nzero = 0;
+  // workspace: sorted point arrays [N], proxy arrays [P][n]
+  double* q[2] = {};
+  double* qx[2] = {};
+  double* part[3] = {};
+  double* prox[3] = {};
+  SrcRecSwe* recswe = nullptr;
+  SrcRec64* rec64 = nullptr;
 };
 
 namespace {
@@ -829,6 +858,233 @@ int sbslpm_swe_departure_to_target(sbslpm_ctx* c, const double* dep_x, const dou
   double* nf[3] = {new_vor, new_div, new_h};
   launch_swe_departure_to_target(c->dt, dep_x, dep_y, dep_z, f, tracers, ntr, nf, new_tracers, c->stream);
   return launched(c);
+}
+
+// ---- irregular particle tree (lat-lon path) ----------------------------------
+
+int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z, int leaf_size,
+                        int interp_degree, sbslpm_irreg** out) {
+  if (!c || !out || !x || !y || !z) return fail(c, SBSLPM_E_ARG, "irreg_create: null argument");
+  *out = nullptr;
+  sbslpm_irreg* h = new (std::nothrow) sbslpm_irreg();
+  if (!h) return fail(c, SBSLPM_E_NOMEM, "irreg_create: out of host memory");
+  std::string msg = build_irreg_tree(x, y, z, n, leaf_size, interp_degree, c->prm.theta, h->tree);
+  if (!msg.empty()) {
+    delete h;
+    return fail(c, SBSLPM_E_ARG, "irreg_create: " + msg);
+  }
+  h->ctx = c;
+  const IrregTree& t = h->tree;
+  const size_t N = (size_t)t.N, Pn = (size_t)t.P * t.n, U = (size_t)t.U;
+  std::vector<int> count(t.P, 0), leaves;
+  for (int p = 0; p < t.P; ++p) {
+    count[p] = t.pn[p].point_count;
+    if (t.pn[p].is_leaf) leaves.push_back(p);
+  }
+  Arena A;
+  const size_t o_pan = A.take<DevPanel>(t.P), o_px = A.take<double>(3 * U), o_cheb = A.take<double>(t.dp1),
+               o_hi = A.take<double>(t.dp1 * t.dp1), o_lo = A.take<double>(t.dp1 * t.dp1), o_perm = A.take<int>(N),
+               o_lf = A.take<int>(t.P), o_lc = A.take<int>(t.P), o_pl = A.take<int>(N),
+               o_lv = A.take<int>(leaves.size()), o_nl = A.take<int>(t.nonleaf.size()),
+               o_tl = A.take<TargetTile>(t.tiles.size()), o_st = A.take<SourceSet>(t.sets.size()),
+               o_zr = A.take<int>(t.zero_rows.size()), o_q = A.take<double>(2 * N), o_qx = A.take<double>(2 * Pn),
+               o_part = A.take<double>(3 * N), o_prox = A.take<double>(3 * Pn), o_rs = A.take<SrcRecSwe>(U),
+               o_r64 = A.take<SrcRec64>(U), total = A.take<char>(256);
+  char* b = nullptr;
+  if (hipMalloc(&b, total) != hipSuccess) {
+    delete h;
+    return fail(c, SBSLPM_E_NOMEM, "irreg_create: out of device memory");
+  }
+  h->arena = b;
+  double* pxyz = (double*)(b + o_px);
+  int rc = upload(c, b + o_pan, t.dev.data(), t.P * sizeof(DevPanel));
+  if (!rc) rc = upload(c, pxyz, t.px.data(), U * sizeof(double));
+  if (!rc) rc = upload(c, pxyz + U, t.py.data(), U * sizeof(double));
+  if (!rc) rc = upload(c, pxyz + 2 * U, t.pz.data(), U * sizeof(double));
+  if (!rc) rc = upload(c, b + o_cheb, t.cheb.data(), t.dp1 * sizeof(double));
+  if (!rc) rc = upload(c, b + o_hi, t.m2m_hi.data(), t.m2m_hi.size() * sizeof(double));
+  if (!rc) rc = upload(c, b + o_lo, t.m2m_lo.data(), t.m2m_lo.size() * sizeof(double));
+  if (!rc) rc = upload(c, b + o_perm, t.perm.data(), N * sizeof(int));
+  if (!rc) rc = upload(c, b + o_lf, t.leaf_first.data(), t.P * sizeof(int));
+  if (!rc) rc = upload(c, b + o_lc, count.data(), t.P * sizeof(int));
+  if (!rc) rc = upload(c, b + o_pl, t.point_leaf.data(), N * sizeof(int));
+  if (!rc) rc = upload(c, b + o_lv, leaves.data(), leaves.size() * sizeof(int));
+  if (!rc) rc = upload(c, b + o_nl, t.nonleaf.data(), t.nonleaf.size() * sizeof(int));
+  if (!rc) rc = upload(c, b + o_tl, t.tiles.data(), t.tiles.size() * sizeof(TargetTile));
+  if (!rc) rc = upload(c, b + o_st, t.sets.data(), t.sets.size() * sizeof(SourceSet));
+  if (!rc) rc = upload(c, b + o_zr, t.zero_rows.data(), t.zero_rows.size() * sizeof(int));
+  if (rc) {
+    hipFree(b);
+    delete h;
+    return rc;
+  }
+  DevTree& d = h->dt;
+  d = c->dt;  // error word; the rest below
+  d.panels = (const DevPanel*)(b + o_pan);
+  d.px = pxyz; d.py = pxyz + U; d.pz = pxyz + 2 * U;
+  d.cheb_d = d.cheb_m = (const double*)(b + o_cheb);
+  d.m2m_hi = (const double*)(b + o_hi);
+  d.m2m_lo = (const double*)(b + o_lo);
+  d.m2m_hiT = d.m2m_loT = nullptr;
+  d.p2m = d.p2mT = nullptr;
+  d.proxy_leaf = nullptr; d.proxy_xieta = nullptr; d.proxy_order = nullptr;
+  d.P = t.P; d.Pl = (int)leaves.size(); d.offset = t.P;  // every proxy row may be an interaction source
+  d.n = t.n; d.dp1 = t.dp1; d.m = 0; d.m2 = 0; d.levels = t.levels;
+  d.npart = t.N; d.U = t.U; d.m2p = 0; d.np16 = 0; d.U16 = 0;
+  h->perm = (int*)(b + o_perm);
+  h->leaf_first = (int*)(b + o_lf);
+  h->leaf_count = (int*)(b + o_lc);
+  h->point_leaf = (int*)(b + o_pl);
+  h->leaves = (int*)(b + o_lv);
+  h->nleaves = (int)leaves.size();
+  h->nonleaf = (int*)(b + o_nl);
+  h->tiles = (TargetTile*)(b + o_tl);
+  h->ntiles = (int)t.tiles.size();
+  h->sets = (SourceSet*)(b + o_st);
+  h->zero_rows = (int*)(b + o_zr);
+  h->nzero = (int)t.zero_rows.size();
+  for (int k = 0; k < 2; ++k) {
+    h->q[k] = (double*)(b + o_q) + k * N;
+    h->qx[k] = (double*)(b + o_qx) + k * Pn;
+  }
+  for (int f = 0; f < 3; ++f) {
+    h->part[f] = (double*)(b + o_part) + f * N;
+    h->prox[f] = (double*)(b + o_prox) + f * Pn;
+  }
+  h->recswe = (SrcRecSwe*)(b + o_rs);
+  h->rec64 = (SrcRec64*)(b + o_r64);
+  *out = h;
+  return SBSLPM_OK;
+}
+
+void sbslpm_irreg_destroy(sbslpm_irreg* h) {
+  if (!h) return;
+  if (h->ctx && h->ctx->stream) hipStreamSynchronize(h->ctx->stream);
+  if (h->arena) hipFree(h->arena);
+  delete h;
+}
+
+int sbslpm_irreg_sizes(const sbslpm_irreg* h, int64_t* s) {
+  if (!h || !s) return SBSLPM_E_ARG;
+  const IrregTree& t = h->tree;
+  s[0] = t.P; s[1] = t.levels; s[2] = t.n; s[3] = t.N; s[4] = t.interaction_count(); s[5] = h->ntiles;
+  return SBSLPM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Proxy charges of every panel from point values: P2M at the leaves, then M2M
+// bottom-up over the non-leaf panels (base_pots_ll + child_to_parent_ll).
+void irreg_upward(sbslpm_irreg* h, const double* area, const double* pots, double* q_sorted, double* proxy) {
+  const IrregTree& t = h->tree;
+  hipStream_t s = h->ctx->stream;
+  launch_gather_charges(t.N, h->perm, area, pots, q_sorted, s);
+  launch_p2m_ll(h->dt, h->leaves, h->nleaves, h->leaf_first, h->leaf_count, q_sorted, proxy, s);
+  for (int l = t.levels; l >= 0; --l) {
+    const int a = t.level_nonleaf[l], b = t.level_nonleaf[l + 1];
+    launch_m2m_list(h->dt, h->nonleaf + a, b - a, proxy, s);
+  }
+}
+
+// L2L top-down over the non-leaf panels (accumulating into the children), then
+// out_sorted = L2P(leaf rows) + part_sorted (parent_to_child_3_ll +
+// child_panel_interp_3_ll).
+void irreg_downward(sbslpm_irreg* h, double* const* proxy, const double* const* part_sorted, double* const* out_sorted) {
+  const IrregTree& t = h->tree;
+  hipStream_t s = h->ctx->stream;
+  for (int l = 0; l <= t.levels; ++l) {
+    const int a = t.level_nonleaf[l], b = t.level_nonleaf[l + 1];
+    launch_l2l_list(h->dt, h->nonleaf + a, b - a, 3, proxy, s);
+  }
+  const double* cp[3] = {proxy[0], proxy[1], proxy[2]};
+  launch_l2p_ll(h->dt, h->point_leaf, 3, cp, part_sorted, out_sorted, s);
+}
+
+// Pair sums of the lat-lon shallow-water kernel into the workspace (assigned).
+void irreg_swe_pairs(sbslpm_irreg* h, const double* qz, const double* qzx, const double* qd, const double* qdx) {
+  const IrregTree& t = h->tree;
+  hipStream_t s = h->ctx->stream;
+  launch_source_records_swe(h->dt, 3, qz, qzx, qd, qdx, nullptr, nullptr, h->recswe, s);
+  launch_zero_rows(h->zero_rows, h->nzero, t.n, 3, h->prox, s);
+  launch_swe_interact64(h->dt, 3, h->tiles, h->ntiles, h->sets, h->recswe, h->ctx->prm.kernel_eps, h->part, h->prox,
+                        s, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbslpm_irreg_upward_pass(sbslpm_irreg* h, const double* area, const double* pots, double* proxy_source_pots) {
+  if (!h || !area || !pots || !proxy_source_pots) return h ? fail(h->ctx, SBSLPM_E_ARG, "irreg_upward_pass: null") : SBSLPM_E_ARG;
+  irreg_upward(h, area, pots, h->q[0], proxy_source_pots);
+  return launched(h->ctx);
+}
+
+int sbslpm_irreg_downward_pass_3(sbslpm_irreg* h, double* proxy_1, double* proxy_2, double* proxy_3, double* sols_1,
+                                 double* sols_2, double* sols_3) {
+  if (!h || !proxy_1 || !proxy_2 || !proxy_3 || !sols_1 || !sols_2 || !sols_3)
+    return h ? fail(h->ctx, SBSLPM_E_ARG, "irreg_downward_pass_3: null") : SBSLPM_E_ARG;
+  double* px[3] = {proxy_1, proxy_2, proxy_3};
+  irreg_downward(h, px, nullptr, h->part);
+  const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
+  double* so[3] = {sols_1, sols_2, sols_3};
+  launch_scatter(h->tree.N, 3, h->perm, cp, so, true, h->ctx->stream);
+  return launched(h->ctx);
+}
+
+int sbslpm_irreg_swe_vel_interactions(sbslpm_irreg* h, double* target_pots_1, double* target_pots_2,
+                                      double* target_pots_3, double* vel_x, double* vel_y, double* vel_z,
+                                      const double* proxy_vors, const double* proxy_divs, const double* vors,
+                                      const double* divs, const double* area) {
+  if (!h || !target_pots_1 || !target_pots_2 || !target_pots_3 || !vel_x || !vel_y || !vel_z || !proxy_vors ||
+      !proxy_divs || !vors || !divs || !area)
+    return h ? fail(h->ctx, SBSLPM_E_ARG, "irreg_swe_vel_interactions: null") : SBSLPM_E_ARG;
+  const IrregTree& t = h->tree;
+  hipStream_t s = h->ctx->stream;
+  launch_gather_charges(t.N, h->perm, area, vors, h->q[0], s);
+  launch_gather_charges(t.N, h->perm, area, divs, h->q[1], s);
+  irreg_swe_pairs(h, h->q[0], proxy_vors, h->q[1], proxy_divs);
+  double* tp[3] = {target_pots_1, target_pots_2, target_pots_3};
+  const double* cx[3] = {h->prox[0], h->prox[1], h->prox[2]};
+  launch_add((int64_t)t.P * t.n, 3, tp, cx, s);
+  const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
+  double* vo[3] = {vel_x, vel_y, vel_z};
+  launch_scatter(t.N, 3, h->perm, cp, vo, true, s);
+  return launched(h->ctx);
+}
+
+int sbslpm_irreg_swe_velocity(sbslpm_irreg* h, const double* area, const double* vors, const double* divs,
+                              double* vel_x, double* vel_y, double* vel_z) {
+  if (!h || !area || !vors || !divs || !vel_x || !vel_y || !vel_z)
+    return h ? fail(h->ctx, SBSLPM_E_ARG, "irreg_swe_velocity: null") : SBSLPM_E_ARG;
+  irreg_upward(h, area, vors, h->q[0], h->qx[0]);
+  irreg_upward(h, area, divs, h->q[1], h->qx[1]);
+  irreg_swe_pairs(h, h->q[0], h->qx[0], h->q[1], h->qx[1]);
+  const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
+  irreg_downward(h, h->prox, cp, h->part);
+  double* vo[3] = {vel_x, vel_y, vel_z};
+  launch_scatter(h->tree.N, 3, h->perm, cp, vo, false, h->ctx->stream);
+  return launched(h->ctx);
+}
+
+int sbslpm_irreg_bve_velocity(sbslpm_irreg* h, const double* area, const double* vor, double* vel_x, double* vel_y,
+                              double* vel_z) {
+  if (!h || !area || !vor || !vel_x || !vel_y || !vel_z)
+    return h ? fail(h->ctx, SBSLPM_E_ARG, "irreg_bve_velocity: null") : SBSLPM_E_ARG;
+  const IrregTree& t = h->tree;
+  hipStream_t s = h->ctx->stream;
+  irreg_upward(h, area, vor, h->q[0], h->qx[0]);
+  launch_source_records64(h->dt, h->q[0], h->qx[0], nullptr, nullptr, true, h->rec64, s);
+  launch_zero_rows(h->zero_rows, h->nzero, t.n, 3, h->prox, s);
+  launch_bve_interact64(h->dt, h->tiles, h->ntiles, h->sets, h->rec64, h->ctx->prm.kernel_eps, h->part, h->prox, s,
+                        true);
+  const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
+  irreg_downward(h, h->prox, cp, h->part);
+  double* vo[3] = {vel_x, vel_y, vel_z};
+  launch_scatter(t.N, 3, h->perm, cp, vo, false, s);
+  return launched(h->ctx);
 }
 
 int sbslpm_interp_points(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z, int nfields,

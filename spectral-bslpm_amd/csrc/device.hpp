@@ -165,7 +165,8 @@ void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int nti
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
                               hipStream_t s, bool assign = false, int nmem = 1);
 
-// ---- shallow-water pair sums (swe.hip); variant 1 or 2
+// ---- shallow-water pair sums (swe.hip); variant 1 or 2, 3 = the lat-lon form
+// (swe_vel_impl.hpp:265-276: variant 1's kernel with the divergence term's sign reversed)
 void launch_source_records_swe(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
                                const double* qd_part, const double* qd_proxy, const double* const* disp_part,
                                const double* const* disp_proxy, SrcRecSwe* rec, hipStream_t s);
@@ -196,6 +197,20 @@ void launch_disp_interp_adjoint(const DevTree& t, const double* const* dbar_prox
 void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
                           const double* const* vel, double tau, double omega, const double* area, const double* qbar,
                           const double* const* dbar, double* vor_bar, double* const* vbar, hipStream_t s);
+
+// ---- irregular particle tree (irreg.hip; M2M / L2L over panel lists in fmm_passes.hip)
+void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s);
+void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp, double* const* phi_proxy,
+                     hipStream_t s);
+void launch_gather_charges(int64_t n, const int* perm, const double* area, const double* pots, double* q,
+                           hipStream_t s);
+void launch_scatter(int64_t n, int ncomp, const int* perm, const double* const* in, double* const* out,
+                    bool accumulate, hipStream_t s);
+void launch_add(int64_t n, int ncomp, double* const* a, const double* const* b, hipStream_t s);
+void launch_p2m_ll(const DevTree& t, const int* leaves, int nleaves, const int* leaf_first, const int* leaf_count,
+                   const double* q, double* proxy, hipStream_t s);
+void launch_l2p_ll(const DevTree& t, const int* point_leaf, int ncomp, const double* const* phi,
+                   const double* const* part, double* const* out, hipStream_t s);
 
 // ---- output operators (output.hip)
 int max_interp_fields();

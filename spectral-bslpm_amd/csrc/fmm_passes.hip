@@ -68,10 +68,13 @@ __global__ __launch_bounds__(128) void k_p2m(DevTree t, const double* __restrict
 // M2M for all parents of one level (child_to_parent, fmm_funcs_impl.hpp:47-70):
 // Q_p[a][b] = sum_c sum_kl X_c[a][k] Y_c[b][l] Q_c[k][l], assigning the parent rows.
 // X_c / Y_c = upper- or lower-half matrix in xi / eta (children 1,4 upper in xi; 1,2 upper in eta).
-__global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, double* __restrict__ q) {
+// Parents are pbegin + blockIdx.x, or list[blockIdx.x] when a list is given
+// (the irregular tree's non-leaf panels of one level).
+__global__ __launch_bounds__(128) void k_m2m(DevTree t, int pbegin, const int* __restrict__ list,
+                                             double* __restrict__ q) {
   __shared__ double sq[4 * kMaxN], sv[4 * kMaxN];
   __shared__ double shi[kMaxN], slo[kMaxN];
-  const int par = pbegin + blockIdx.x;
+  const int par = list ? list[blockIdx.x] : pbegin + blockIdx.x;
   const int n = t.n, dp1 = t.dp1;
   const int c0 = t.panels[par].child0;
   q += member_proxy(t);
@@ -165,10 +168,11 @@ __global__ __launch_bounds__(256) void k_p2m_m2m(DevTree t, int pbegin, const do
 // phi_c[k][l] += sum_a X_c[a][k] sum_b Y_c[b][l] phi_p[a][b].  The eta sweep
 // depends only on the eta half, so it is done once for the upper and once
 // for the lower half.
-__global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, int ncomp, Ptr3 phi, int assign) {
+__global__ __launch_bounds__(256) void k_l2l(DevTree t, int pbegin, const int* __restrict__ list, int ncomp, Ptr3 phi,
+                                             int assign) {
   __shared__ double sp[3 * kMaxN], sv[3 * 2 * kMaxN];
   __shared__ double shi[kMaxN], slo[kMaxN];
-  const int par = pbegin + blockIdx.x;
+  const int par = list ? list[blockIdx.x] : pbegin + blockIdx.x;
   const int n = t.n, dp1 = t.dp1;
   const int c0 = t.panels[par].child0;
   const int64_t mx = member_proxy(t);
@@ -309,7 +313,7 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
   for (int l = t.levels - 2; l >= 0; --l) {
     int pbegin = 2 * ((1 << (2 * l)) - 1);
     int count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, t, pbegin, q_proxy);
+    hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
   }
 }
 
@@ -328,7 +332,7 @@ void launch_upward_fused(const DevTree& t, const double* area, const double* pot
   }
   for (int l = t.levels - 3; l >= 0; --l) {
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, q_proxy);
+    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
   }
 }
 
@@ -342,7 +346,7 @@ void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream
     int pbegin = 2 * ((1 << (2 * l)) - 1);
     int count = 6 << (2 * l);
     const int assign = (assign_leaf_rows && l + 2 == t.levels) ? 1 : 0;
-    hipLaunchKernelGGL(k_l2l, dim3(count, nmem), dim3(256), 0, s, t, pbegin, ncomp, phi, assign);
+    hipLaunchKernelGGL(k_l2l, dim3(count, nmem), dim3(256), 0, s, t, pbegin, (const int*)nullptr, ncomp, phi, assign);
   }
 }
 
@@ -362,6 +366,17 @@ void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy,
                        part, out, (int64_t)t.P * t.n);
   else
     hipLaunchKernelGGL(k_l2p, dim3(t.Pl, nmem), dim3(128), 0, s, t, ncomp, cphi, part, out);
+}
+
+void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, t, 0, parents, q_proxy);
+}
+
+void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp, double* const* phi_proxy,
+                     hipStream_t s) {
+  Ptr3 phi{};
+  for (int f = 0; f < ncomp; ++f) phi.p[f] = phi_proxy[f];
+  if (count > 0) hipLaunchKernelGGL(k_l2l, dim3(count), dim3(256), 0, s, t, 0, parents, ncomp, phi, 0);
 }
 
 }  // namespace sbslpm

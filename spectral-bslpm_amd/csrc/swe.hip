@@ -8,6 +8,8 @@
 //     u += g qz (x cross y) + g qd (y - x (x.y)),      g = -1/(4 pi (1 - x.y + eps))
 //   variant 2 (swe_vel_panel_interaction_2):
 //     u += g qz (x cross y) + g0 qd (y0 - x (x.y0)),   g0 = +1/(4 pi (1 - x.y0 + eps))
+//   variant 3 (swe_vel_panel_interaction_ll, swe_vel_impl.hpp:265-276, no displacement):
+//     u += g qz (x cross y) - g qd (y - x (x.y)),      g = -1/(4 pi (1 - x.y + eps))
 // The kernel accumulates W = sum g qz y, D = sum gd y', S = sum gd (x.y')
 // (y' = y or y0) and forms u = x cross W + D - x S once per target, the same
 // sums regrouped.
@@ -116,7 +118,7 @@ void launch_source_records_swe(const DevTree& t, int variant, const double* qz_p
     dp.p[f] = disp_part ? disp_part[f] : nullptr;
     dx.p[f] = disp_proxy ? disp_proxy[f] : nullptr;
   }
-  // variant 1: g qd with g = -1/(4 pi ...); variant 2: +1/(4 pi ...)
+  // variant 1: g qd with g = -1/(4 pi ...); variants 2 and 3: +1/(4 pi ...)
   const double qd_scale = variant == 1 ? kNegInv4Pi : -kNegInv4Pi;
   hipLaunchKernelGGL(k_src_swe, dim3((unsigned)((t.U + 255) / 256)), dim3(256), 0, s, t, qz_part, qz_proxy, qd_part,
                      qd_proxy, dp, dx, qd_scale, rec);
@@ -132,7 +134,7 @@ void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tile
     ox.p[f] = out_proxy[f];
   }
   const unsigned blocks = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
-  if (variant == 1)
+  if (variant != 2)
     hipLaunchKernelGGL(k_swe64<1>, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec, eps, op, ox,
                        assign ? 1 : 0);
   else

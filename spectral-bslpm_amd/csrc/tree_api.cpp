@@ -4,10 +4,15 @@
 #include <vector>
 
 #include "../../include/sbslpm.h"
+#include "irreg_tree.hpp"
 #include "tree.hpp"
 
 struct sbslpm_tree {
   sbslpm::Tree t;
+};
+
+struct sbslpm_irreg_tree {
+  sbslpm::IrregTree t;
 };
 
 extern "C" {
@@ -97,6 +102,59 @@ int sbslpm_tree_dedupe(const sbslpm_tree* h, int* two_d_to_1d, int* counts, doub
   if (y1d) std::memcpy(y1d, y.data(), y.size() * sizeof(double));
   if (z1d) std::memcpy(z1d, z.data(), z.size() * sizeof(double));
   *n1d = (int64_t)x.size();
+  return SBSLPM_OK;
+}
+
+int sbslpm_irreg_tree_build(int64_t n, const double* x, const double* y, const double* z, int leaf_size,
+                            int interp_degree, double theta, sbslpm_irreg_tree** out) {
+  if (!out || !x || !y || !z) return SBSLPM_E_ARG;
+  *out = nullptr;
+  sbslpm_irreg_tree* h = new (std::nothrow) sbslpm_irreg_tree();
+  if (!h) return SBSLPM_E_NOMEM;
+  if (!sbslpm::build_irreg_tree(x, y, z, n, leaf_size, interp_degree, theta, h->t).empty()) {
+    delete h;
+    return SBSLPM_E_ARG;
+  }
+  *out = h;
+  return SBSLPM_OK;
+}
+
+void sbslpm_irreg_tree_destroy(sbslpm_irreg_tree* h) { delete h; }
+
+int sbslpm_irreg_tree_sizes(const sbslpm_irreg_tree* h, int64_t* s) {
+  if (!h || !s) return SBSLPM_E_ARG;
+  const sbslpm::IrregTree& t = h->t;
+  s[0] = t.P; s[1] = t.levels; s[2] = t.n; s[3] = t.N; s[4] = t.interaction_count(); s[5] = (int64_t)t.tiles.size();
+  return SBSLPM_OK;
+}
+
+int sbslpm_irreg_tree_panels(const sbslpm_irreg_tree* h, int* ints, double* dbls) {
+  if (!h) return SBSLPM_E_ARG;
+  const sbslpm::IrregTree& t = h->t;
+  for (int i = 0; i < t.P; ++i) {
+    const sbslpm::IrregPanel& p = t.pn[i];
+    if (ints) {
+      int* r = ints + 10 * i;
+      r[0] = p.face; r[1] = p.level; r[2] = p.parent; r[3] = p.child[0]; r[4] = p.child[1]; r[5] = p.child[2];
+      r[6] = p.child[3]; r[7] = p.is_leaf; r[8] = p.point_count; r[9] = t.leaf_first[i];
+    }
+    if (dbls) {
+      double* q = dbls + 5 * i;
+      q[0] = p.min_xi; q[1] = p.max_xi; q[2] = p.min_eta; q[3] = p.max_eta; q[4] = p.radius;
+    }
+  }
+  return SBSLPM_OK;
+}
+
+int sbslpm_irreg_tree_points(const sbslpm_irreg_tree* h, int* perm) {
+  if (!h || !perm) return SBSLPM_E_ARG;
+  std::memcpy(perm, h->t.perm.data(), h->t.perm.size() * sizeof(int));
+  return SBSLPM_OK;
+}
+
+int sbslpm_irreg_tree_interaction_list(const sbslpm_irreg_tree* h, int* list) {
+  if (!h || !list) return SBSLPM_E_ARG;
+  std::memcpy(list, h->t.ilist.data(), h->t.ilist.size() * sizeof(int));
   return SBSLPM_OK;
 }
 
