@@ -81,6 +81,10 @@ int sbslpm_tree_locate(const sbslpm_tree* tree, double x, double y, double z);
 /* Fixed transfer matrices: m2m_hi/m2m_lo [(d+1)*(d+1)] (parent basis a at upper/lower-half child node k,
  * index a*(d+1)+k) and p2m [(d+1)*m] (proxy basis a at particle node k). Any pointer may be NULL. */
 int sbslpm_tree_transfer_matrices(const sbslpm_tree* tree, double* m2m_hi, double* m2m_lo, double* p2m);
+/* cubed_sphere_midpoints (initialize_cubed_sphere.cpp:10-62): the 6 * 4^levels cell centres of the
+ * equiangular grid with 2^levels cells per face edge, [face][xi][eta], and their areas (two
+ * spherical triangles over the cell corners); faces 0..5 (the reference passes face+1, :35). */
+int sbslpm_cubed_sphere_midpoints(int levels, double* x, double* y, double* z, double* area);
 /* Host-only view of the irregular particle tree (same construction as sbslpm_irreg_create). */
 typedef struct sbslpm_irreg_tree sbslpm_irreg_tree;
 int sbslpm_irreg_tree_build(int64_t n, const double* x, const double* y, const double* z, int leaf_size,

@@ -55,6 +55,7 @@ SIGNATURES = {
     "sbslpm_tree_locate": (_I, [_V, _D, _D, _D]),
     "sbslpm_tree_transfer_matrices": (_I, [_V, _PD, _PD, _PD]),
     "sbslpm_tree_dedupe": (_I, [_V, _PI, _PI, _PD, _PD, _PD, _PI64]),
+    "sbslpm_cubed_sphere_midpoints": (_I, [_I, _PD, _PD, _PD, _PD]),
     "sbslpm_irreg_tree_build": (_I, [ctypes.c_int64, _PD, _PD, _PD, _I, _I, ctypes.c_double, _V]),
     "sbslpm_irreg_tree_destroy": (None, [_V]),
     "sbslpm_irreg_tree_sizes": (_I, [_V, _PI64]),

@@ -130,6 +130,17 @@ class HostTree:
         return m, cnt[:k].copy(), xyz[0][:k].copy(), xyz[1][:k].copy(), xyz[2][:k].copy()
 
 
+def cubed_sphere_midpoints(levels):
+    """Cell centres and areas of the equiangular grid with 2^levels cells per face edge
+    (initialize_cubed_sphere.cpp:10-62): x, y, z, area, each [6 * 4^levels]."""
+    n = 6 * 4 ** levels
+    out = [np.zeros(n) for _ in range(4)]
+    rc = load().sbslpm_cubed_sphere_midpoints(int(levels), *[_dp(a) for a in out])
+    if rc:
+        raise SbslpmError(rc, "cubed_sphere_midpoints: levels must be in 0..14")
+    return out
+
+
 class HostIrregTree:
     """Host view of the irregular particle tree (initialize_cubed_sphere.cpp:446-592,
     fmm_funcs.cpp:125-231); points x, y, z are flat arrays in any order."""

@@ -159,6 +159,8 @@ void traverse(Tree& t) {
 
 }  // namespace
 
+double sphere_tri_area(const double* p1, const double* p2, const double* p3) { return tri_area(p1, p2, p3); }
+
 int locate_leaf(const Tree& t, double x, double y, double z) {
   double xe[2];
   return locate_uniform(t.cfg.levels, x, y, z, xe);

@@ -116,6 +116,9 @@ void build_source_major(const Tree& t, SourceMajor& sm);
 void dedupe_points(const Tree& t, std::vector<int>& two_d_to_1d, std::vector<int>& counts, std::vector<double>& x1d,
                    std::vector<double>& y1d, std::vector<double>& z1d);
 
+// Area of the spherical triangle p1 p2 p3 (sphere_tri_area, general_utils.cpp:5-21).
+double sphere_tri_area(const double* p1, const double* p2, const double* p3);
+
 // Host twin of the device point location (interp_funcs_impl.hpp:114-142).
 int locate_leaf(const Tree& t, double x, double y, double z);
 

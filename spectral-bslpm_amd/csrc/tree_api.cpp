@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "../../include/sbslpm.h"
+#include "geometry.hpp"
 #include "irreg_tree.hpp"
 #include "tree.hpp"
 
@@ -155,6 +156,31 @@ int sbslpm_irreg_tree_points(const sbslpm_irreg_tree* h, int* perm) {
 int sbslpm_irreg_tree_interaction_list(const sbslpm_irreg_tree* h, int* list) {
   if (!h || !list) return SBSLPM_E_ARG;
   std::memcpy(list, h->t.ilist.data(), h->t.ilist.size() * sizeof(int));
+  return SBSLPM_OK;
+}
+
+// cubed_sphere_midpoints (initialize_cubed_sphere.cpp:10-62): 6 * 4^levels cell centres of
+// the equiangular grid with 2^levels cells per face edge, ordered [face][xi][eta], areas
+// from two spherical triangles over the cell corners.  The reference passes face i+1
+// to xyz_from_xieta (:35), which aborts at face 6; here the faces are 0..5.
+int sbslpm_cubed_sphere_midpoints(int levels, double* x, double* y, double* z, double* area) {
+  if (levels < 0 || levels > 14 || !x || !y || !z || !area) return SBSLPM_E_ARG;
+  const int side = 1 << levels;
+  const double between = sbslpm::kPi / (2.0 * side), off = between / 2.0;
+  for (int f = 0; f < 6; ++f)
+    for (int j = 0; j < side; ++j)
+      for (int k = 0; k < side; ++k) {
+        const double xi = -sbslpm::kPi / 4.0 + off + between * j, eta = -sbslpm::kPi / 4.0 + off + between * k;
+        double c[3], p1[3], p2[3], p3[3], p4[3];
+        sbslpm::xyz_from_xieta(xi, eta, f, c);
+        sbslpm::xyz_from_xieta(xi - off, eta - off, f, p1);
+        sbslpm::xyz_from_xieta(xi + off, eta - off, f, p2);
+        sbslpm::xyz_from_xieta(xi + off, eta + off, f, p3);
+        sbslpm::xyz_from_xieta(xi - off, eta + off, f, p4);
+        const int64_t o = ((int64_t)f * side + j) * side + k;
+        x[o] = c[0]; y[o] = c[1]; z[o] = c[2];
+        area[o] = sbslpm::sphere_tri_area(p1, p2, p3) + sbslpm::sphere_tri_area(p1, p3, p4);
+      }
   return SBSLPM_OK;
 }
 

@@ -107,3 +107,22 @@ def test_bad_input(sb):
     X[0, 3] = math.nan
     with pytest.raises(sb.SbslpmError):
         sb.HostIrregTree(*X, 10, 6)
+
+
+@pytest.mark.parametrize("levels", [0, 2, 4])
+def test_midpoints(sb, orc, levels):
+    x, y, z, a = sb.cubed_sphere_midpoints(levels)
+    side = 2 ** levels
+    assert x.size == 6 * side * side
+    np.testing.assert_allclose(x * x + y * y + z * z, 1.0, rtol=0, atol=1e-15)
+    assert abs(a.sum() - 4 * math.pi) < 1e-12 and np.all(a > 0)
+    # each centre sits at the centre of its cell in label coordinates
+    h = math.pi / (2 * side)
+    for f in range(6):
+        for j in (0, side - 1):
+            for k in (0, side // 2):
+                o = (f * side + j) * side + k
+                assert orc.face_from_xyz(x[o], y[o], z[o]) == f
+                xe, _ = orc.xieta_from_xyz(x[o], y[o], z[o], f)
+                assert abs(xe[0] - (-math.pi / 4 + h / 2 + h * j)) < 1e-14
+                assert abs(xe[1] - (-math.pi / 4 + h / 2 + h * k)) < 1e-14
