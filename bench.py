@@ -156,6 +156,11 @@ def bench_ensemble(args, torch, sb):
             noise = W.random_sh_forcing(x, y, z, lmax=24, seed=100 + j)[0]
             mem.append(base + 1e-3 * rms * noise / float(np.sqrt(np.mean(noise ** 2))))
         vors = dev(np.stack(mem))
+        if not args.no_tune:
+            with torch.cuda.stream(stream):
+                _, shape = ctx.autotune(dev(a), vors[0].contiguous())
+            torch.cuda.synchronize()
+            log(f"6x{side}^2: pair-kernel launch shape {shape}")
         g = dict(side=side, ctx=ctx, kw=kw, xyz=[dev(v) for v in (x, y, z, a)], vors0=vors,
                  vors=vors.clone(), vel=[torch.zeros_like(vors) for _ in range(3)])
         groups.append(g)
@@ -221,7 +226,8 @@ def bench_ensemble(args, torch, sb):
                        "graph": not args.no_graph},
             "one_member_at_a_time_ms_per_step": round(1e3 * seq, 3),
             "speedup_vs_one_member_at_a_time": round(seq / (sec / args.steps), 3),
-            "member0_vor_rel_l2_vs_fp64": acc}
+            "member0_vor_rel_l2_vs_fp64": acc,
+            "pair_kernel_launch_shapes": {f"6x{g['side']}^2": list(g["ctx"].pair_shape()) for g in groups}}
     print(json.dumps(line), flush=True)
 
 
@@ -313,6 +319,7 @@ def main():
                          "adjoint: cost of the step gradient at 6x512^2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the pair-kernel launch-shape autotuner")
     ap.add_argument("--valu", action="store_true", help="also time the mixed-precision vector-unit pair kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (scale reference for HBM counters)")
@@ -345,6 +352,15 @@ def main():
     dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
     gx, gy, gz, ga, gvor0 = [dev(v) for v in (x, y, z, a, vor_h)]
     torch.cuda.synchronize()
+    tuned = {}
+    if not args.no_tune:
+        with torch.cuda.stream(stream):
+            for name, ctx in (("fp64", ctx64), ("mixed", ctxmx)):
+                table, shape = ctx.autotune(ga, gvor0)
+                tuned[name] = {"waves_per_workgroup": shape[0], "xcd_contiguous": bool(shape[1]),
+                               "candidates_ms": [[int(w), int(c), round(ms, 4)] for w, c, ms in table]}
+        torch.cuda.synchronize()
+        log("autotune", tuned)
 
     wall64, ev64, vor64, vel64 = time_steps(ctx64, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
     log(f"fp64 : {args.steps / ev64:.3f} steps/s ({1e3 * ev64 / args.steps:.2f} ms/step)")
@@ -448,6 +464,7 @@ def main():
         "accuracy": {k: float(f"{v:.3e}") for k, v in acc.items()},
         "wall_s": {"fp64": round(wall64, 4), "mixed": round(wallmx, 4)},
         "roofline": roof["bve_mixed"], "roofline_fp64": roof["bve_fp64"],
+        "pair_kernel_launch_shape": tuned,
         "cpu_baseline": cpu,
     }
     if valu:

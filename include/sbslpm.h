@@ -287,7 +287,17 @@ int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* ctx, const double* x, const dou
 int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* ctx, const double* dep_x, const double* dep_y,
                                            const double* dep_z, const double* new_vor_bar, double* vor_bar);
 
-/* ---- Irregular particle tree (the lat-lon path, SURVEY layout B) -------------------------
+/* Autotuner of the Biot-Savart pair kernels' launch shape (no reference counterpart): times
+ * the context's pair kernel (its precision) on its own interaction list with the records of
+ * (area, vor), for every candidate of waves per workgroup (1, 2, 4) x XCD-contiguous tile
+ * mapping (off, on), and keeps the fastest for all later evaluations, steps and ensembles
+ * (captured graphs are dropped and recaptured).  report [ncand][3] (may be NULL) receives
+ * waves, xcd, ms per launch; *ncand the number of candidates timed.  Results do not depend on
+ * the shape: every target is still summed by one lane in the same order. */
+int sbslpm_autotune(sbslpm_ctx* ctx, const double* area, const double* vor, double* report, int* ncand);
+int sbslpm_get_pair_shape(const sbslpm_ctx* ctx, int* waves, int* xcd);
+
+/* ---- Irregular particle tree (the lat-lon path) -------------------------
  * Points in any order (the reference's [lat][lon] arrays flattened, index i*lon_count + j).
  * Face panels split into quadrants while they hold more than leaf_size points
  * (initialize_cube_sphere_irreg_points, initialize_cubed_sphere.cpp:446-592; panel bounds in

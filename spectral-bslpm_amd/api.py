@@ -444,6 +444,19 @@ class Context:
                                                           _ptr(vor_bar)), "bve_velocity_adjoint")
         return vor_bar
 
+    def autotune(self, area, vor):
+        """Times the pair kernel's launch shapes on this context's interaction list and keeps the
+        fastest; returns [(waves, xcd, ms per launch), ...] and the chosen (waves, xcd)."""
+        rep = np.zeros((6, 3))
+        n = ctypes.c_int(0)
+        self._check(self._lib.sbslpm_autotune(self.h, _ptr(area), _ptr(vor), _dp(rep), ctypes.byref(n)), "autotune")
+        return [tuple(r) for r in rep[:n.value]], self.pair_shape()
+
+    def pair_shape(self):
+        w, x = ctypes.c_int(0), ctypes.c_int(0)
+        self._lib.sbslpm_get_pair_shape(self.h, ctypes.byref(w), ctypes.byref(x))
+        return w.value, x.value
+
     def irregular(self, x, y, z, leaf_size):
         """Irregular particle tree over host points x, y, z (flat, any order) using this
         context's stream, interp_degree, theta and kernel_eps."""

@@ -113,6 +113,9 @@ struct sbslpm_ctx {
   double kernel_ms[4] = {};
   int64_t kernel_launches[4] = {};
 
+  // Launch shape of the Biot-Savart pair kernels (sbslpm_autotune).
+  PairShape shape{};
+
   // hipGraph of one BVE step, keyed by the caller's pointers.
   hipGraphExec_t graph = nullptr;
   const void* graph_key[10] = {};
@@ -699,17 +702,17 @@ int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const d
     launch_source_records_mfma(c->dt, w.qpart, w.qproxy, dpart, dprox, w.recmf, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, c->prm.kernel_eps, w.phipart,
-                             w.phiproxy, c->stream, true, nmem);
+                             w.phiproxy, c->stream, true, nmem, c->shape);
   } else if (c->prm.precision == SBSLPM_MIXED_VALU) {
     launch_source_records32(c->dt, w.qpart, w.qproxy, dpart, dprox, w.rec32, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, w.rec32, c->prm.kernel_eps, w.phipart,
-                              w.phiproxy, c->stream, true, nmem);
+                              w.phiproxy, c->stream, true, nmem, c->shape);
   } else {
     launch_source_records64(c->dt, w.qpart, w.qproxy, dpart, dprox, true, w.rec64, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_FP64);
     launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, w.rec64, c->prm.kernel_eps, w.phipart, w.phiproxy,
-                          c->stream, true, nmem);
+                          c->stream, true, nmem, c->shape);
   }
   const double* pp[3] = {w.phipart[0], w.phipart[1], w.phipart[2]};
   launch_downward_pass(c->dt, 3, w.phiproxy, pp, vel, c->stream, true, nmem);
@@ -861,6 +864,88 @@ int sbslpm_swe_departure_to_target(sbslpm_ctx* c, const double* dep_x, const dou
 }
 
 // ---- irregular particle tree (lat-lon path) ----------------------------------
+
+// ---- autotuner -----------------------------------------------------------------
+
+int sbslpm_autotune(sbslpm_ctx* c, const double* area, const double* vor, double* report, int* ncand) {
+  if (!c || !area || !vor) return fail(c, SBSLPM_E_ARG, "autotune: null argument");
+  if (c->capturing) return fail(c, SBSLPM_E_ARG, "autotune: not allowed during graph capture");
+  const Work& w = c->w1;
+  // the pair kernel's inputs: the context's records of (area, vor), undisplaced
+  launch_upward_fused(c->dt, area, vor, w.qpart, w.qproxy, c->stream);
+  const int prec = c->prm.precision;
+  if (prec == SBSLPM_MIXED)
+    launch_source_records_mfma(c->dt, w.qpart, w.qproxy, nullptr, nullptr, w.recmf, c->stream);
+  else if (prec == SBSLPM_MIXED_VALU)
+    launch_source_records32(c->dt, w.qpart, w.qproxy, nullptr, nullptr, w.rec32, c->stream);
+  else
+    launch_source_records64(c->dt, w.qpart, w.qproxy, nullptr, nullptr, true, w.rec64, c->stream);
+  auto launch = [&](PairShape sh) {
+    if (prec == SBSLPM_MIXED)
+      launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, c->prm.kernel_eps, w.phipart, w.phiproxy,
+                               c->stream, true, 1, sh);
+    else if (prec == SBSLPM_MIXED_VALU)
+      launch_bve_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, w.rec32, c->prm.kernel_eps, w.phipart,
+                                w.phiproxy, c->stream, true, 1, sh);
+    else
+      launch_bve_interact64(c->dt, c->tiles, c->ntiles, c->sets, w.rec64, c->prm.kernel_eps, w.phipart, w.phiproxy,
+                            c->stream, true, 1, sh);
+  };
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  const int waves[3] = {1, 2, 4};
+  PairShape best = c->shape;
+  float best_ms = 1e30f;
+  int k = 0;
+  for (int xcd = 0; xcd < 2; ++xcd)
+    for (int wi = 0; wi < 3; ++wi) {
+      const PairShape sh{waves[wi], xcd};
+      launch(sh);  // warm-up
+      hipEventRecord(e0, c->stream);
+      for (int r = 0; r < 3; ++r) launch(sh);
+      hipEventRecord(e1, c->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) break;
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      ms /= 3;
+      if (report) {
+        report[3 * k] = sh.waves;
+        report[3 * k + 1] = sh.xcd;
+        report[3 * k + 2] = ms;
+      }
+      ++k;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = sh;
+      }
+    }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ncand) *ncand = k;
+  int rc = launched(c);
+  if (rc) return rc;
+  if (best.waves != c->shape.waves || best.xcd != c->shape.xcd) {
+    c->shape = best;
+    // captured graphs hold the old launch shape
+    if (c->graph) {
+      hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+    }
+    if (c->ens_graph) {
+      hipGraphExecDestroy(c->ens_graph);
+      c->ens_graph = nullptr;
+    }
+  }
+  return SBSLPM_OK;
+}
+
+int sbslpm_get_pair_shape(const sbslpm_ctx* c, int* waves, int* xcd) {
+  if (!c || !waves || !xcd) return SBSLPM_E_ARG;
+  *waves = c->shape.waves;
+  *xcd = c->shape.xcd;
+  return SBSLPM_OK;
+}
 
 int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z, int leaf_size,
                         int interp_degree, sbslpm_irreg** out) {

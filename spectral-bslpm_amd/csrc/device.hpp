@@ -49,12 +49,26 @@ __device__ inline int64_t member_proxy(const DevTree& t) { return (int64_t)block
 // Pair kernels run one wave per target tile, four waves per workgroup.
 constexpr int kWavesPerBlock = 4;
 
+// Launch shape of the Biot-Savart pair kernels, chosen by the context's
+// autotuner: waves (tiles) per workgroup and whether workgroups are remapped
+// so that each XCD takes a contiguous range of tiles.
+struct PairShape {
+  int waves = kWavesPerBlock;
+  int xcd = 1;
+};
+
 // Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
 // range of logical workgroups (neighbouring target tiles share sources, which
 // then stay in one XCD's L2).
 __device__ inline int xcd_remap(int b, int nb) {
   const int per = nb / 8, rem = nb % 8, x = b % 8, y = b / 8;
   return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
+}
+
+// Tile of this wave under a PairShape launch (blockDim.x = 64 * waves).
+__device__ inline int shaped_tile(int wave, int xcd) {
+  const int b = xcd ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  return b * (int)(blockDim.x >> 6) + wave;
 }
 
 // 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps
@@ -145,7 +159,7 @@ void launch_source_records64(const DevTree& t, const double* q_part, const doubl
 // with assign = true they overwrite them (each target written by one lane).
 void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s, bool assign = false, int nmem = 1);
+                           hipStream_t s, bool assign = false, int nmem = 1, PairShape shape = {});
 void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
                                bool assign = false);
@@ -154,7 +168,8 @@ void launch_source_records32(const DevTree& t, const double* q_part, const doubl
                              hipStream_t s, int nmem = 1);
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec32* rec, double eps, double* const* out_part,
-                               double* const* out_proxy, hipStream_t s, bool assign = false, int nmem = 1);
+                               double* const* out_proxy, hipStream_t s, bool assign = false, int nmem = 1,
+                               PairShape shape = {});
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                    const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s,
                                    bool assign = false);
@@ -163,7 +178,7 @@ void launch_source_records_mfma(const DevTree& t, const double* q_part, const do
                                 hipStream_t s, int nmem = 1);
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
-                              hipStream_t s, bool assign = false, int nmem = 1);
+                              hipStream_t s, bool assign = false, int nmem = 1, PairShape shape = {});
 
 // ---- shallow-water pair sums (swe.hip); variant 1 or 2, 3 = the lat-lon form
 // (swe_vel_impl.hpp:265-276: variant 1's kernel with the divergence term's sign reversed)

@@ -125,12 +125,12 @@ __global__ void k_src32(DevTree t, const double* __restrict__ q_part, const doub
 __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
                                                const SrcRec64* __restrict__ rec, double eps, Ptr3 part,
-                                               Ptr3 proxy, int assign) {
+                                               Ptr3 proxy, int assign, int xcd) {
   member_targets(t, part, proxy);
   rec += (int64_t)blockIdx.y * t.U;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int tile = shaped_tile(wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
@@ -186,12 +186,12 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
 __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ sets,
                                                    const SrcRec32* __restrict__ rec, float eps, Ptr3 part,
-                                                   Ptr3 proxy, int assign) {
+                                                   Ptr3 proxy, int assign, int xcd) {
   member_targets(t, part, proxy);
   rec += (int64_t)blockIdx.y * t.U;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int tile = shaped_tile(wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const DevPanel& PT = t.panels[T.panel];
@@ -408,13 +408,13 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
 __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                   const SourceSet* __restrict__ sets,
                                                   const float* __restrict__ rec, float eps, Ptr3 part,
-                                                  Ptr3 proxy, int assign) {
+                                                  Ptr3 proxy, int assign, int xcd) {
   member_targets(t, part, proxy);
   rec += (int64_t)blockIdx.y * t.U16 * 8;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int tile = shaped_tile(wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int ntt = (T.count + 15) >> 4;
@@ -479,10 +479,10 @@ void launch_source_records32(const DevTree& t, const double* q_part, const doubl
 
 void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s, bool assign, int nmem) {
+                           hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, kWavesPerBlock), nmem), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0);
+  hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t, tiles,
+                     ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 
 void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
@@ -495,10 +495,10 @@ void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int nt
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                const SrcRec32* rec, double eps, double* const* out_part,
-                               double* const* out_proxy, hipStream_t s, bool assign, int nmem) {
+                               double* const* out_proxy, hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, kWavesPerBlock), nmem), dim3(64 * kWavesPerBlock), 0, s, t,
-                     tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0);
+  hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t,
+                     tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 
 void launch_source_records_mfma(const DevTree& t, const double* q_part, const double* q_proxy,
@@ -510,11 +510,11 @@ void launch_source_records_mfma(const DevTree& t, const double* q_part, const do
 
 void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
-                              hipStream_t s, bool assign, int nmem) {
+                              hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, kWavesPerBlock), nmem), dim3(64 * kWavesPerBlock), 0, s, t,
+  hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t,
                      tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy),
-                     assign ? 1 : 0);
+                     assign ? 1 : 0, shape.xcd);
 }
 
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
