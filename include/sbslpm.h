@@ -264,15 +264,17 @@ int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* ctx, int nmem, const double* x
  * transpose, which approximates
  *   vor_bar_s = -(area_s / (4 pi)) x_s . sum_t g(x_s, x_t) (ubar_t cross x_t),
  *   g = 1/(1 - x.y + eps)
- * to the FMM truncation.  No reference counterpart; it transposes the operator of
- * bve_vel_interactions (bve_vel.cpp:9-37, bve_vel_impl.hpp:71-106) with the passes of
- * fmm_funcs.cpp:341-392. */
+ * to the FMM truncation.  On a mixed-precision context the transposed pair sums run in FP32
+ * (panel-centred |x - y|^2 / 2 form, FP64 accumulation across interactions), the passes in
+ * FP64.  No reference counterpart; it transposes the operator of bve_vel_interactions
+ * (bve_vel.cpp:9-37, bve_vel_impl.hpp:71-106) with the passes of fmm_funcs.cpp:341-392. */
 int sbslpm_bve_velocity_adjoint(sbslpm_ctx* ctx, const double* area, const double* ubar_x, const double* ubar_y,
                                 const double* ubar_z, double* vor_bar);
 /* Gradient of one bve_back_rk4_step (bve_time_step.cpp:311-419, no tracers) with respect to the
  * vorticity before the step: vor_bar [Pl][m2] is assigned J^T new_vor_bar, J the Jacobian of
  * vor -> vor after the step at the state vor (which is not modified).  The step's forward pass
- * is recomputed inside in the context's precision; the reverse pass is FP64 and differentiates
+ * is recomputed inside in the context's precision; the reverse pass is FP64 (its transposed
+ * pair sums FP32 on a mixed-precision context) and differentiates
  * the four FMM velocity evaluations in the charges and the source displacements, the
  * displacement updates, the backward RK4 departure search (BLI interpolation differentiated in
  * the point, re-projections, the configured combine rule) and the remap with its Coriolis

@@ -104,9 +104,122 @@ __global__ __launch_bounds__(256) void k_bve_adj64(DevTree t, const TargetTile* 
   }
 }
 
+// FP32 target record: node relative to its panel centre and c = psi cross x.
+struct TgtRec32 {
+  float ax, ay, az, c0, c1, c2;
+};
+
+__global__ void k_tgt_rec32(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, TgtRec32* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double a, b, d;
+  int panel;
+  if (u < t.npart) {
+    a = psi_part.p[0][u]; b = psi_part.p[1][u]; d = psi_part.p[2][u];
+    panel = t.offset + (int)(u / t.m2);
+  } else {
+    const int64_t v = u - t.npart;
+    panel = (int)(v / t.n);
+    if (panel >= t.offset) return;
+    a = psi_prox.p[0][v]; b = psi_prox.p[1][v]; d = psi_prox.p[2][v];
+  }
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const DevPanel& P = t.panels[panel];
+  rec[u] = TgtRec32{(float)(x - P.cx), (float)(y - P.cy), (float)(z - P.cz), (float)(b * z - d * y),
+                    (float)(d * x - a * z), (float)(a * y - b * x)};
+}
+
+// Transposed pair sums in FP32 (mixed-precision contexts): 1 - x.y as
+// |x - y|^2 / 2 in panel-centred coordinates like the forward mixed kernel,
+// FP32 partial sums per target set flushed into FP64 accumulators, the
+// source-side finish in FP64.
+template <bool POS>
+__global__ __launch_bounds__(256) void k_bve_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ targets,
+                                                   const TgtRec32* __restrict__ trec, const SrcRec64* __restrict__ srec,
+                                                   float eps, CPtr3 dpart, CPtr3 dprox, double* __restrict__ qbar_part,
+                                                   double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const SrcRec64 S = srec[u];
+  const DevPanel& PS = t.panels[T.panel];
+  const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
+  const float yx = (float)S.x, yy = (float)S.y, yz = (float)S.z;
+  double aq = 0, a0 = 0, a1 = 0, a2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet G = targets[e];
+    const TgtRec32* __restrict__ r = trec + G.first;
+    // source relative to the target panel's centre
+    const float qx = bx - G.ox, qy = by - G.oy, qz = bz - G.oz;
+    float sq = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < G.count; ++k) {
+      const TgtRec32 R = r[k];
+      const float d0 = R.ax - qx, d1 = R.ay - qy, d2 = R.az - qz;  // x_t - y_s
+      const float g = __builtin_amdgcn_rcpf(fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps));
+      const float dc = fmaf(yx, R.c0, fmaf(yy, R.c1, yz * R.c2));
+      sq = fmaf(g, dc, sq);
+      if (POS) {
+        const float h = g * dc;
+        s0 = fmaf(g, fmaf(h, d0 + yx, R.c0), s0);
+        s1 = fmaf(g, fmaf(h, d1 + yy, R.c1), s1);
+        s2 = fmaf(g, fmaf(h, d2 + yz, R.c2), s2);
+      }
+    }
+    aq += sq;
+    if (POS) {
+      a0 += s0; a1 += s1; a2 += s2;
+    }
+  }
+  if (lane >= T.count) return;
+  const bool part = u < t.npart;
+  const int64_t v = part ? u : u - t.npart;
+  (part ? qbar_part : qbar_prox)[v] = kNegInv4Pi * aq;
+  if (POS) {
+    const double w0 = t.px[u] + (part ? dpart.p[0] : dprox.p[0])[v];
+    const double w1 = t.py[u] + (part ? dpart.p[1] : dprox.p[1])[v];
+    const double w2 = t.pz[u] + (part ? dpart.p[2] : dprox.p[2])[v];
+    const double ir = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const double yb0 = S.q * a0, yb1 = S.q * a1, yb2 = S.q * a2;
+    const double yy2 = S.x * yb0 + S.y * yb1 + S.z * yb2;
+    (part ? dbar_part.p[0] : dbar_prox.p[0])[v] = (yb0 - S.x * yy2) * ir;
+    (part ? dbar_part.p[1] : dbar_prox.p[1])[v] = (yb1 - S.y * yy2) * ir;
+    (part ? dbar_part.p[2] : dbar_prox.p[2])[v] = (yb2 - S.z * yy2) * ir;
+  }
+}
+
 }  // namespace
 
-size_t adj_target_record_bytes() { return sizeof(TgtRec); }
+size_t adj_target_record_bytes() { return sizeof(TgtRec) > sizeof(TgtRec32) ? sizeof(TgtRec) : sizeof(TgtRec32); }
+
+void launch_adj_target_records32(const DevTree& t, const double* const* psi_part, const double* const* psi_prox,
+                                 void* rec, hipStream_t s) {
+  CPtr3 pp{{psi_part[0], psi_part[1], psi_part[2]}}, px{{psi_prox[0], psi_prox[1], psi_prox[2]}};
+  hipLaunchKernelGGL(k_tgt_rec32, dim3((unsigned)((t.U + 255) / 256)), dim3(256), 0, s, t, pp, px,
+                     static_cast<TgtRec32*>(rec));
+}
+
+void launch_bve_adjoint32(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
+                          const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
+                          const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
+                          double* const* dbar_prox, hipStream_t s) {
+  if (ntiles == 0) return;
+  const dim3 grid((unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock)), block(64 * kWavesPerBlock);
+  const TgtRec32* r = static_cast<const TgtRec32*>(trec);
+  if (dpart) {
+    CPtr3 dp{{dpart[0], dpart[1], dpart[2]}}, dx{{dprox[0], dprox[1], dprox[2]}};
+    Ptr3 bp{{dbar_part[0], dbar_part[1], dbar_part[2]}}, bx{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
+    hipLaunchKernelGGL(k_bve_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps, dp, dx,
+                       qbar_part, qbar_prox, bp, bx);
+  } else {
+    hipLaunchKernelGGL(k_bve_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps, CPtr3{},
+                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{});
+  }
+}
 
 void launch_adj_target_records(const DevTree& t, const double* const* psi_part, const double* const* psi_prox,
                                void* rec, hipStream_t s) {

@@ -1290,14 +1290,19 @@ static int fmm_adjoint(sbslpm_ctx* c, const double* area, const double* vor_eff,
   for (int f = 0; f < 3; ++f)
     launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
   const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
-  launch_adj_target_records(c->dt, vbar, psix, c->adj_trec, c->stream);
+  const bool fp32 = c->prm.precision != SBSLPM_FP64;  // mixed contexts take the FP32 pair sums
+  if (fp32)
+    launch_adj_target_records32(c->dt, vbar, psix, c->adj_trec, c->stream);
+  else
+    launch_adj_target_records(c->dt, vbar, psix, c->adj_trec, c->stream);
   // source positions (and, for the displacement part, charges) as the forward records
   if (pos) launch_upward_fused(c->dt, area, vor_eff, c->qpart, c->qproxy, c->stream);
   launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->adj_srec, c->stream);
   launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 1, &c->adj_qbx, c->stream);
   if (pos) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, c->adj_dbx, c->stream);
-  launch_bve_adjoint64(c->dt, c->adj_tiles, c->adj_ntiles, c->adj_targets, c->adj_trec, c->adj_srec,
-                       c->prm.kernel_eps, dpart, dprox, c->adj_qbp, c->adj_qbx, c->adj_dbp, c->adj_dbx, c->stream);
+  (fp32 ? launch_bve_adjoint32 : launch_bve_adjoint64)(c->dt, c->adj_tiles, c->adj_ntiles, c->adj_targets,
+                                                       c->adj_trec, c->adj_srec, c->prm.kernel_eps, dpart, dprox,
+                                                       c->adj_qbp, c->adj_qbx, c->adj_dbp, c->adj_dbx, c->stream);
   // charges: particle sources directly, proxy sources through P2M/M2M, whose
   // transposes are L2L/L2P
   const double* qbp[1] = {c->adj_qbp};

@@ -201,6 +201,13 @@ void launch_bve_adjoint64(const DevTree& t, const TargetTile* tiles, int ntiles,
                           const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
                           double* const* dbar_prox, hipStream_t s);
+// FP32 variants for mixed-precision contexts (panel-centred |x - y|^2 / 2 form).
+void launch_adj_target_records32(const DevTree& t, const double* const* psi_part, const double* const* psi_prox,
+                                 void* rec, hipStream_t s);
+void launch_bve_adjoint32(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
+                          const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
+                          const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
+                          double* const* dbar_prox, hipStream_t s);
 void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
                           double* wbar, hipStream_t s);
 void launch_departure_adjoint(const DevTree& t, const double* x, const double* y, const double* z,

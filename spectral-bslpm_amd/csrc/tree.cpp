@@ -363,6 +363,10 @@ void build_source_major(const Tree& t, SourceMajor& sm) {
     const int tg = t.ilist[3 * e], s = t.ilist[3 * e + 1];
     SourceSet ts{};
     ts.panel = tg;
+    // target-panel centre minus source-panel centre (the FP32 adjoint works in panel-centred coordinates)
+    ts.ox = (float)(t.dev[tg].cx - t.dev[s].cx);
+    ts.oy = (float)(t.dev[tg].cy - t.dev[s].cy);
+    ts.oz = (float)(t.dev[tg].cz - t.dev[s].cz);
     if (t.pn[tg].is_leaf) {
       ts.first = (int)((int64_t)(tg - t.offset) * t.m2);
       ts.count = t.m2;

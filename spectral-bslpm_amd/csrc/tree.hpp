@@ -98,7 +98,8 @@ std::string build_tree(const TreeConfig& cfg, Tree& t);
 // (particles of a leaf, proxies of a non-leaf panel) whose [src_begin,
 // src_end) range indexes `targets`, each entry naming the target points
 // [first, first+count) of one interaction (particles of a leaf target,
-// proxies of a non-leaf target).  `zero_rows` lists the proxy rows that are
+// proxies of a non-leaf target) and, in ox/oy/oz, the target-panel centre
+// minus the source-panel centre in FP32.  `zero_rows` lists the proxy rows that are
 // no interaction source (and the leaf rows when levels == 1), whose
 // cotangents are zero.
 struct SourceMajor {

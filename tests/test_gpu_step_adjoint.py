@@ -140,3 +140,28 @@ def test_step_gradient_against_direct_sum_reference(sb, torch_gpu):
     err = sb.workloads.rel_l2(got, ref)
     print(f"  gradient rel-L2 vs direct sum {err:.2e}")
     assert err < 1e-4
+
+
+def test_mixed_precision_gradient_matches_fp64(sb, torch_gpu):
+    """A mixed-precision context runs the transposed pair sums in FP32 on its own (FP32)
+    trajectory; the gradient stays within the 1e-6 relative bar of the FP64 one."""
+    t = torch_gpu
+    print("\n[case step adjoint mixed vs fp64]", flush=True)
+    kw = dict(levels=4, interp_degree=8, leaf_degree=7, weights="clenshaw_curtis", kernel_eps=EPS, omega=OMEGA,
+              delta_t=DT)
+    c64 = sb.Context(sb.RunConfig(precision="fp64", **kw))
+    cmx = sb.Context(sb.RunConfig(precision="mixed", **kw))
+    x, y, z, a = c64.spec_points()
+    g = [to_gpu(t, q) for q in (x, y, z, a)]
+    w = to_gpu(t, sb.workloads.gaussian_vortices(x, y, z, a, seed=31))
+    r = to_gpu(t, np.random.default_rng(32).normal(size=x.shape))
+    ub = [r, 0.5 * r, -r]
+    g64, gmx = c64.bve_back_rk4_step_adjoint(*g, w, r), cmx.bve_back_rk4_step_adjoint(*g, w, r)
+    v64, vmx = c64.bve_velocity_adjoint(g[3], ub), cmx.bve_velocity_adjoint(g[3], ub)
+    c64.synchronize()
+    cmx.synchronize()
+    W = sb.workloads
+    e_step = W.rel_l2(to_np(gmx), to_np(g64))
+    e_vel = W.rel_l2(to_np(vmx), to_np(v64))
+    print(f"  mixed vs fp64: velocity adjoint {e_vel:.2e}, step gradient {e_step:.2e}")
+    assert e_vel < 1e-6 and e_step < 1e-6
