@@ -344,6 +344,11 @@ int sbslpm_irreg_bve_velocity(sbslpm_irreg* h, const double* area, const double*
  * layout). */
 int sbslpm_interp_points(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z, int nfields,
                          const double* const* fields, double* const* out);
+/* Transpose of sbslpm_interp_points: field_bar[q] [Pl][m2] += B^T out_bar[q] (n points), with
+ * the transposed interpolations summed per wave and leaf and added with FP64 atomics (order not
+ * fixed).  No reference counterpart; the observation operator's adjoint for gradients. */
+int sbslpm_interp_points_adjoint(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                                 int nfields, const double* const* out_bar, double* const* field_bar);
 /* Zonal and meridional velocity from Cartesian velocity at n points, xyz_vel_to_uv_vel
  * (general_utils_impl.hpp:195-229; zero at the poles). */
 int sbslpm_xyz_vel_to_uv_vel(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,

@@ -89,6 +89,7 @@ SIGNATURES = {
     "sbslpm_bve_departure_to_target_adjoint": (_I, [_V] * 6),
     "sbslpm_swe_displacement": (_I, [_V, _V, _V, _V, ctypes.c_double] + [_V] * 6),
     "sbslpm_interp_points": (_I, [_V, ctypes.c_int64, _V, _V, _V, _I, _V, _V]),
+    "sbslpm_interp_points_adjoint": (_I, [_V, ctypes.c_int64, _V, _V, _V, _I, _V, _V]),
     "sbslpm_autotune": (_I, [_V, _V, _V, _PD, _PI]),
     "sbslpm_get_pair_shape": (_I, [_V, _PI, _PI]),
     "sbslpm_irreg_create": (_I, [_V, ctypes.c_int64, _PD, _PD, _PD, _I, _I, _V]),

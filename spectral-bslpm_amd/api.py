@@ -476,6 +476,16 @@ class Context:
                     "interp_points")
         return out
 
+    def interp_points_adjoint(self, x, y, z, out_bar, field_bar=None):
+        """Transpose of interp_points: field_bar[q] [Pl, m2] += B^T out_bar[q]."""
+        field_bar = [self.zeros_part() for _ in out_bar] if field_bar is None else field_bar
+        ob = (ctypes.c_void_p * len(out_bar))(*[_ptr(o).value for o in out_bar])
+        fb = (ctypes.c_void_p * len(field_bar))(*[_ptr(f).value for f in field_bar])
+        self._check(self._lib.sbslpm_interp_points_adjoint(self.h, x.numel(), _ptr(x), _ptr(y), _ptr(z), len(out_bar),
+                                                           ctypes.cast(ob, ctypes.c_void_p),
+                                                           ctypes.cast(fb, ctypes.c_void_p)), "interp_points_adjoint")
+        return field_bar
+
     def xyz_vel_to_uv_vel(self, x, y, z, vel, u=None, v=None):
         """Zonal and meridional velocity, xyz_vel_to_uv_vel (general_utils_impl.hpp:195-229)."""
         u = self.torch.zeros_like(vel[0]) if u is None else u

@@ -107,3 +107,38 @@ def bve_steps(sbctx, x, y, z, area, vor0, nsteps, every=1):
     if nsteps < 1 or every < 1:
         raise ValueError("nsteps and every must be positive")
     return _BVESteps.apply(sbctx, x, y, z, area, vor0, nsteps, every)
+
+
+class _InterpPoints(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, sbctx, x, y, z, field):
+        out = sbctx.interp_points(x, y, z, [field.contiguous()])[0]
+        fctx.sbctx = sbctx
+        fctx.save_for_backward(x, y, z)
+        return out
+
+    @staticmethod
+    def backward(fctx, g):
+        x, y, z = fctx.saved_tensors
+        fb = fctx.sbctx.interp_points_adjoint(x, y, z, [g.contiguous()])[0]
+        return None, None, None, None, fb
+
+
+def interp_points(sbctx, x, y, z, field):
+    """BLI of a leaf field [Pl, m2] at points (1-D x, y, z): latlon_interp_host's functor
+    (interp_funcs.hpp:22-60); backward: the transposed interpolation."""
+    return _InterpPoints.apply(sbctx, x, y, z, field)
+
+
+def bve_4dvar(sbctx, x, y, z, area, vor0, nsteps, every, obs_xyz, obs_values):
+    """Loss and gradient of a vorticity 4D-Var problem on the BVE: the initial field vor0 is
+    advanced nsteps steps and compared with observations at the points obs_xyz (three 1-D
+    tensors) at the final time, L = 1/2 sum (H(vor_N) - obs)^2, H the BLI at the points.
+    The gradient comes from the hand-written step adjoints (checkpoints every `every` steps)
+    and the transposed observation operator.  Returns (L, dL/dvor0)."""
+    w = vor0.detach().clone().requires_grad_(True)
+    vn = bve_steps(sbctx, x, y, z, area, w, nsteps, every)
+    hx = interp_points(sbctx, obs_xyz[0], obs_xyz[1], obs_xyz[2], vn)
+    loss = 0.5 * ((hx - obs_values) ** 2).sum()
+    loss.backward()
+    return loss.detach(), w.grad.detach()

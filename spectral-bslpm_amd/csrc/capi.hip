@@ -1182,6 +1182,18 @@ int sbslpm_interp_points(sbslpm_ctx* c, int64_t n, const double* x, const double
   return launched(c);
 }
 
+int sbslpm_interp_points_adjoint(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                                 int nfields, const double* const* out_bar, double* const* field_bar) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !z)) || nfields < 1 || nfields > max_interp_fields() || !out_bar ||
+      !field_bar)
+    return fail(c, SBSLPM_E_ARG, "interp_points_adjoint: bad argument");
+  for (int q = 0; q < nfields; ++q) {
+    if (!out_bar[q] || !field_bar[q]) return fail(c, SBSLPM_E_ARG, "interp_points_adjoint: null field");
+    launch_interp_points_adjoint(c->dt, n, x, y, z, out_bar[q], field_bar[q], c->stream);
+  }
+  return launched(c);
+}
+
 int sbslpm_xyz_vel_to_uv_vel(sbslpm_ctx* c, int64_t n, const double* x, const double* y, const double* z,
                              const double* vx, const double* vy, const double* vz, double* u, double* v) {
   if (!c || n < 0 || (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !u || !v)))

@@ -210,6 +210,9 @@ void launch_bve_adjoint32(const DevTree& t, const TargetTile* tiles, int ntiles,
                           double* const* dbar_prox, hipStream_t s);
 void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
                           double* wbar, hipStream_t s);
+// field_bar [Pl][m2] += B^T out_bar, B the BLI interpolation at n points.
+void launch_interp_points_adjoint(const DevTree& t, int64_t n, const double* x, const double* y, const double* z,
+                                  const double* out_bar, double* field_bar, hipStream_t s);
 void launch_departure_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
                               const double* const* vels /*12*/, const double* vor, const double* new_vor_bar,
                               double dt, int combine, double omega, double* const* vel_bar /*12*/, double* vor_bar,

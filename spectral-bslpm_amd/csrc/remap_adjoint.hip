@@ -33,14 +33,17 @@ struct Vel12 {
   double* p[12];
 };
 
+// Transpose of the BLI interpolation of a leaf field at n points:
+// wbar[leaf nodes] += B newbar (the remap at the departure points, n = Pl*m2,
+// or an observation operator at any points).
 template <int M>
-__global__ __launch_bounds__(128) void k_remap_adj(DevTree t, const double* __restrict__ dx,
+__global__ __launch_bounds__(128) void k_remap_adj(DevTree t, int64_t n, const double* __restrict__ dx,
                                                    const double* __restrict__ dy, const double* __restrict__ dz,
                                                    const double* __restrict__ newbar, double* __restrict__ wbar) {
   __shared__ double lds[2][wave_scatter_lds<M, 1>()];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < t.npart;
-  const int64_t u = valid ? i : t.npart - 1;
+  const bool valid = i < n;
+  const int64_t u = valid ? i : n - 1;
   double xe[2], wx[M], we[M];
   int leaf = locate_leaf(t, dx[u], dy[u], dz[u], xe);
   if (leaf < 0) {
@@ -272,9 +275,9 @@ __global__ void k_stage_adj(DevTree t, const double* __restrict__ x, const doubl
 
 template <int M>
 struct RemapAdj {
-  static void run(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* nb,
+  static void run(const DevTree& t, int64_t n, const double* dx, const double* dy, const double* dz, const double* nb,
                   double* wb, hipStream_t s) {
-    hipLaunchKernelGGL(k_remap_adj<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, dx, dy, dz, nb, wb);
+    hipLaunchKernelGGL(k_remap_adj<M>, dim3(blocks_for(n, 128)), dim3(128), 0, s, t, n, dx, dy, dz, nb, wb);
   }
 };
 
@@ -299,7 +302,12 @@ struct DispInterpAdj {
 
 void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
                           double* wbar, hipStream_t s) {
-  dispatch_m<RemapAdj>(t.m, t, dx, dy, dz, newbar, wbar, s);
+  dispatch_m<RemapAdj>(t.m, t, t.npart, dx, dy, dz, newbar, wbar, s);
+}
+
+void launch_interp_points_adjoint(const DevTree& t, int64_t n, const double* x, const double* y, const double* z,
+                                  const double* out_bar, double* field_bar, hipStream_t s) {
+  if (n > 0) dispatch_m<RemapAdj>(t.m, t, n, x, y, z, out_bar, field_bar, s);
 }
 
 void launch_departure_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
