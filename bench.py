@@ -250,6 +250,13 @@ def bench_adjoint(args, torch, sb):
     vor_h = W.gaussian_vortices(x, y, z, a, count=12, seed=0)
     rng = np.random.default_rng(1)
     gx, gy, gz, ga, gvor0, r = [dev(v) for v in (x, y, z, a, vor_h, rng.normal(size=x.shape))]
+    for ctx in (ctx64, ctxmx):
+        if args.shape:
+            ctx.set_pair_shape(*[int(v) for v in args.shape.split(",")])
+        elif not args.no_tune:
+            with torch.cuda.stream(stream):
+                ctx.autotune(ga, gvor0)
+    torch.cuda.synchronize()
     _, ev64, _, _ = time_steps(ctx64, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
     _, evmx, _, _ = time_steps(ctxmx, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
     fwd64, fwdmx = ev64 / args.steps, evmx / args.steps
@@ -385,6 +392,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the pair-kernel launch-shape autotuner")
+    ap.add_argument("--shape", default=None, metavar="WAVES,XCD",
+                    help="fix the pair-kernel launch shape instead of autotuning (e.g. 4,0)")
     ap.add_argument("--valu", action="store_true", help="also time the mixed-precision vector-unit pair kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (scale reference for HBM counters)")
@@ -421,14 +430,18 @@ def main():
     gx, gy, gz, ga, gvor0 = [dev(v) for v in (x, y, z, a, vor_h)]
     torch.cuda.synchronize()
     tuned = {}
-    if not args.no_tune:
-        with torch.cuda.stream(stream):
-            for name, ctx in (("fp64", ctx64), ("mixed", ctxmx)):
+    for name, ctx in (("fp64", ctx64), ("mixed", ctxmx)):
+        if args.shape:
+            w_, x_ = [int(v) for v in args.shape.split(",")]
+            ctx.set_pair_shape(w_, x_)
+            tuned[name] = {"waves_per_workgroup": w_, "xcd_contiguous": bool(x_), "fixed": True}
+        elif not args.no_tune:
+            with torch.cuda.stream(stream):
                 table, shape = ctx.autotune(ga, gvor0)
-                tuned[name] = {"waves_per_workgroup": shape[0], "xcd_contiguous": bool(shape[1]),
-                               "candidates_ms": [[int(w), int(c), round(ms, 4)] for w, c, ms in table]}
-        torch.cuda.synchronize()
-        log("autotune", tuned)
+            tuned[name] = {"waves_per_workgroup": shape[0], "xcd_contiguous": bool(shape[1]),
+                           "candidates_ms": [[int(w), int(c), round(ms, 4)] for w, c, ms in table]}
+    torch.cuda.synchronize()
+    log("launch shape", tuned)
 
     wall64, ev64, vor64, vel64 = time_steps(ctx64, torch, stream, gx, gy, gz, ga, gvor0, args.steps, args.warmup)
     log(f"fp64 : {args.steps / ev64:.3f} steps/s ({1e3 * ev64 / args.steps:.2f} ms/step)")

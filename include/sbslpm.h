@@ -293,10 +293,13 @@ int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* ctx, const double* dep_x,
  * the context's pair kernel (its precision) on its own interaction list with the records of
  * (area, vor), for every candidate of waves per workgroup (1, 2, 4) x XCD-contiguous tile
  * mapping (off, on), and keeps the fastest for all later evaluations, steps and ensembles
- * (captured graphs are dropped and recaptured).  report [ncand][3] (may be NULL) receives
+ * and the step adjoint's transposed pair sums (captured graphs are dropped and recaptured).
+ * report [ncand][3] (may be NULL) receives
  * waves, xcd, ms per launch; *ncand the number of candidates timed.  Results do not depend on
  * the shape: every target is still summed by one lane in the same order. */
 int sbslpm_autotune(sbslpm_ctx* ctx, const double* area, const double* vor, double* report, int* ncand);
+/* Sets the launch shape directly (waves 1, 2 or 4; xcd 0 or 1), e.g. a shape tuned earlier. */
+int sbslpm_set_pair_shape(sbslpm_ctx* ctx, int waves, int xcd);
 int sbslpm_get_pair_shape(const sbslpm_ctx* ctx, int* waves, int* xcd);
 
 /* ---- Irregular particle tree (the lat-lon path) -------------------------

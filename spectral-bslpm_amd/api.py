@@ -452,6 +452,9 @@ class Context:
         self._check(self._lib.sbslpm_autotune(self.h, _ptr(area), _ptr(vor), _dp(rep), ctypes.byref(n)), "autotune")
         return [tuple(r) for r in rep[:n.value]], self.pair_shape()
 
+    def set_pair_shape(self, waves, xcd):
+        self._check(self._lib.sbslpm_set_pair_shape(self.h, int(waves), int(xcd)), "set_pair_shape")
+
     def pair_shape(self):
         w, x = ctypes.c_int(0), ctypes.c_int(0)
         self._lib.sbslpm_get_pair_shape(self.h, ctypes.byref(w), ctypes.byref(x))

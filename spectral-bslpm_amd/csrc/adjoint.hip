@@ -60,10 +60,10 @@ __global__ __launch_bounds__(256) void k_bve_adj64(DevTree t, const TargetTile* 
                                                    const SourceSet* __restrict__ targets,
                                                    const TgtRec* __restrict__ trec, const SrcRec64* __restrict__ srec,
                                                    double eps, CPtr3 dpart, CPtr3 dprox, double* __restrict__ qbar_part,
-                                                   double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox) {
+                                                   double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox, int xcd) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int tile = shaped_tile(wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
@@ -138,10 +138,10 @@ __global__ __launch_bounds__(256) void k_bve_adj32(DevTree t, const TargetTile* 
                                                    const SourceSet* __restrict__ targets,
                                                    const TgtRec32* __restrict__ trec, const SrcRec64* __restrict__ srec,
                                                    float eps, CPtr3 dpart, CPtr3 dprox, double* __restrict__ qbar_part,
-                                                   double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox) {
+                                                   double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox, int xcd) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int tile = shaped_tile(wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
@@ -206,18 +206,18 @@ void launch_adj_target_records32(const DevTree& t, const double* const* psi_part
 void launch_bve_adjoint32(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
                           const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
-                          double* const* dbar_prox, hipStream_t s) {
+                          double* const* dbar_prox, hipStream_t s, PairShape shape) {
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock)), block(64 * kWavesPerBlock);
+  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const TgtRec32* r = static_cast<const TgtRec32*>(trec);
   if (dpart) {
     CPtr3 dp{{dpart[0], dpart[1], dpart[2]}}, dx{{dprox[0], dprox[1], dprox[2]}};
     Ptr3 bp{{dbar_part[0], dbar_part[1], dbar_part[2]}}, bx{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
     hipLaunchKernelGGL(k_bve_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps, dp, dx,
-                       qbar_part, qbar_prox, bp, bx);
+                       qbar_part, qbar_prox, bp, bx, shape.xcd);
   } else {
     hipLaunchKernelGGL(k_bve_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps, CPtr3{},
-                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{});
+                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{}, shape.xcd);
   }
 }
 
@@ -231,18 +231,18 @@ void launch_adj_target_records(const DevTree& t, const double* const* psi_part, 
 void launch_bve_adjoint64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
                           const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
-                          double* const* dbar_prox, hipStream_t s) {
+                          double* const* dbar_prox, hipStream_t s, PairShape shape) {
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock)), block(64 * kWavesPerBlock);
+  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const TgtRec* r = static_cast<const TgtRec*>(trec);
   if (dpart) {
     CPtr3 dp{{dpart[0], dpart[1], dpart[2]}}, dx{{dprox[0], dprox[1], dprox[2]}};
     Ptr3 bp{{dbar_part[0], dbar_part[1], dbar_part[2]}}, bx{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
     hipLaunchKernelGGL(k_bve_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, dp, dx,
-                       qbar_part, qbar_prox, bp, bx);
+                       qbar_part, qbar_prox, bp, bx, shape.xcd);
   } else {
     hipLaunchKernelGGL(k_bve_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, CPtr3{},
-                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{});
+                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{}, shape.xcd);
   }
 }
 

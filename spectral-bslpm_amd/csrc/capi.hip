@@ -940,6 +940,25 @@ int sbslpm_autotune(sbslpm_ctx* c, const double* area, const double* vor, double
   return SBSLPM_OK;
 }
 
+int sbslpm_set_pair_shape(sbslpm_ctx* c, int waves, int xcd) {
+  if (!c || !(waves == 1 || waves == 2 || waves == 4) || !(xcd == 0 || xcd == 1))
+    return fail(c, SBSLPM_E_ARG, "set_pair_shape: waves must be 1, 2 or 4 and xcd 0 or 1");
+  if (c->capturing) return fail(c, SBSLPM_E_ARG, "set_pair_shape: not allowed during graph capture");
+  if (waves != c->shape.waves || xcd != c->shape.xcd) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->shape = PairShape{waves, xcd};
+    if (c->graph) {
+      hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+    }
+    if (c->ens_graph) {
+      hipGraphExecDestroy(c->ens_graph);
+      c->ens_graph = nullptr;
+    }
+  }
+  return SBSLPM_OK;
+}
+
 int sbslpm_get_pair_shape(const sbslpm_ctx* c, int* waves, int* xcd) {
   if (!c || !waves || !xcd) return SBSLPM_E_ARG;
   *waves = c->shape.waves;
@@ -1314,7 +1333,8 @@ static int fmm_adjoint(sbslpm_ctx* c, const double* area, const double* vor_eff,
   if (pos) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, c->adj_dbx, c->stream);
   (fp32 ? launch_bve_adjoint32 : launch_bve_adjoint64)(c->dt, c->adj_tiles, c->adj_ntiles, c->adj_targets,
                                                        c->adj_trec, c->adj_srec, c->prm.kernel_eps, dpart, dprox,
-                                                       c->adj_qbp, c->adj_qbx, c->adj_dbp, c->adj_dbx, c->stream);
+                                                       c->adj_qbp, c->adj_qbx, c->adj_dbp, c->adj_dbx, c->stream,
+                                                       c->shape);
   // charges: particle sources directly, proxy sources through P2M/M2M, whose
   // transposes are L2L/L2P
   const double* qbp[1] = {c->adj_qbp};

@@ -51,10 +51,12 @@ constexpr int kWavesPerBlock = 4;
 
 // Launch shape of the Biot-Savart pair kernels, chosen by the context's
 // autotuner: waves (tiles) per workgroup and whether workgroups are remapped
-// so that each XCD takes a contiguous range of tiles.
+// so that each XCD takes a contiguous range of tiles.  The default is the
+// shape the autotuner picks for the 6 x 512^2 workload (four waves, tiles
+// dealt round-robin over the XCDs by the hardware).
 struct PairShape {
   int waves = kWavesPerBlock;
-  int xcd = 1;
+  int xcd = 0;
 };
 
 // Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
@@ -200,14 +202,14 @@ void launch_adj_target_records(const DevTree& t, const double* const* psi_part, 
 void launch_bve_adjoint64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
                           const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
-                          double* const* dbar_prox, hipStream_t s);
+                          double* const* dbar_prox, hipStream_t s, PairShape shape = {});
 // FP32 variants for mixed-precision contexts (panel-centred |x - y|^2 / 2 form).
 void launch_adj_target_records32(const DevTree& t, const double* const* psi_part, const double* const* psi_prox,
                                  void* rec, hipStream_t s);
 void launch_bve_adjoint32(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* targets,
                           const void* trec, const SrcRec64* srec, double eps, const double* const* dpart,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
-                          double* const* dbar_prox, hipStream_t s);
+                          double* const* dbar_prox, hipStream_t s, PairShape shape = {});
 void launch_remap_adjoint(const DevTree& t, const double* dx, const double* dy, const double* dz, const double* newbar,
                           double* wbar, hipStream_t s);
 // field_bar [Pl][m2] += B^T out_bar, B the BLI interpolation at n points.
