@@ -6,9 +6,10 @@
 // pointer + two extents.  Function names, argument order and the
 // accumulate/assign conventions are the reference's; the interpolation table,
 // panels and interaction list live in the context instead of being passed.
-// These wrappers cover the reference layout (particles are the leaf proxy
-// nodes, leaf_degree == interp_degree), where the particle arrays are the leaf
-// rows of the [P][n] proxy arrays, exactly as in the reference.
+// The uniform-tree wrappers cover the reference layout (particles are the
+// leaf proxy nodes, leaf_degree == interp_degree), where the particle arrays
+// are the leaf rows of the [P][n] proxy arrays, exactly as in the reference;
+// the _ll wrappers take the irregular tree's handle.
 #pragma once
 #include <cstdint>
 #include <stdexcept>
@@ -121,6 +122,40 @@ inline void bve_back_rk4_step(sbslpm_ctx* ctx, View2<double>& xcos, View2<double
                                  proxy_source_vors.data(), vel_x.data(), vel_y.data(), vel_z.data(), passive_tracers,
                                  time),
         "bve_back_rk4_step");
+}
+
+// ---- the lat-lon path (irregular particle tree).  The tree, its panels and
+// leaf_panel_points live in the sbslpm_irreg handle (sbslpm_irreg_create over
+// the same points); the point arrays are the reference's [lat][lon] arrays.
+
+// fmm_funcs.hpp:37-39 (proxy_source_pots is assigned; the reference accumulates
+// into caller-zeroed rows)
+inline void upward_pass_ll(sbslpm_ctx* ctx, sbslpm_irreg* tree, View2<double>& area, View2<double>& pots,
+                           View2<double>& proxy_source_pots) {
+  check(ctx, sbslpm_irreg_upward_pass(tree, area.data(), pots.data(), proxy_source_pots.data()), "upward_pass_ll");
+}
+
+// fmm_funcs.hpp:41-45
+inline void downward_pass_3_ll(sbslpm_ctx* ctx, sbslpm_irreg* tree, View2<double>& proxy_target_pots_1,
+                               View2<double>& proxy_target_pots_2, View2<double>& proxy_target_pots_3,
+                               View2<double>& sols_1, View2<double>& sols_2, View2<double>& sols_3) {
+  check(ctx,
+        sbslpm_irreg_downward_pass_3(tree, proxy_target_pots_1.data(), proxy_target_pots_2.data(),
+                                     proxy_target_pots_3.data(), sols_1.data(), sols_2.data(), sols_3.data()),
+        "downward_pass_3_ll");
+}
+
+// fmm_interactions/swe_vel.hpp:25-29
+inline void swe_vel_interactions_ll(sbslpm_ctx* ctx, sbslpm_irreg* tree, View2<double>& target_pots_1,
+                                    View2<double>& target_pots_2, View2<double>& target_pots_3, View2<double>& vel_x,
+                                    View2<double>& vel_y, View2<double>& vel_z, View2<double>& source_vals_vor,
+                                    View2<double>& source_vals_div, View2<double>& vors, View2<double>& divs,
+                                    View2<double>& area) {
+  check(ctx,
+        sbslpm_irreg_swe_vel_interactions(tree, target_pots_1.data(), target_pots_2.data(), target_pots_3.data(),
+                                          vel_x.data(), vel_y.data(), vel_z.data(), source_vals_vor.data(),
+                                          source_vals_div.data(), vors.data(), divs.data(), area.data()),
+        "swe_vel_interactions_ll");
 }
 
 }  // namespace sbslpm_ref
