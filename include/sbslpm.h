@@ -283,6 +283,16 @@ int sbslpm_bve_velocity_adjoint(sbslpm_ctx* ctx, const double* area, const doubl
 int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
                                      const double* area, const double* vor, const double* new_vor_bar,
                                      double* vor_bar);
+/* One bve_back_rk4_step (no tracers) that keeps what its reverse pass needs: vor_out [Pl][m2]
+ * receives the vorticity after the step (must not alias vor), stages [12][Pl][m2] the four
+ * stage velocities v0x v0y v0z ... v3z.  Bit-identical to sbslpm_bve_back_rk4_step's result. */
+int sbslpm_bve_back_rk4_step_record(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                    const double* area, const double* vor, double* vor_out, double* stages);
+/* sbslpm_bve_back_rk4_step_adjoint with the stage velocities of the step supplied (from
+ * sbslpm_bve_back_rk4_step_record on the same state) instead of recomputed. */
+int sbslpm_bve_back_rk4_step_adjoint_stages(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                            const double* area, const double* vor, const double* stages,
+                                            const double* new_vor_bar, double* vor_bar);
 /* Transpose of the remap interpolation of bve_departure_to_target (bve_time_step.cpp:38-66)
  * with respect to the field: vor_bar [Pl][m2] += B^T new_vor_bar, B the BLI weights at the
  * departure points.  Accumulated with FP64 atomic adds (summation order not fixed). */

@@ -86,6 +86,8 @@ SIGNATURES = {
     "sbslpm_bve_back_rk4_step_ensemble": (_I, [_V, _I] + [_V] * 8 + [ctypes.c_double]),
     "sbslpm_bve_velocity_adjoint": (_I, [_V] * 6),
     "sbslpm_bve_back_rk4_step_adjoint": (_I, [_V] * 8),
+    "sbslpm_bve_back_rk4_step_record": (_I, [_V] * 8),
+    "sbslpm_bve_back_rk4_step_adjoint_stages": (_I, [_V] * 9),
     "sbslpm_bve_departure_to_target_adjoint": (_I, [_V] * 6),
     "sbslpm_swe_displacement": (_I, [_V, _V, _V, _V, ctypes.c_double] + [_V] * 6),
     "sbslpm_interp_points": (_I, [_V, ctypes.c_int64, _V, _V, _V, _I, _V, _V]),

@@ -508,6 +508,23 @@ class Context:
                     "bve_back_rk4_step_adjoint")
         return vor_bar
 
+    def bve_back_rk4_step_record(self, x, y, z, area, vor, vor_out=None, stages=None):
+        """One step keeping the stage velocities: returns (vor_out, stages [12, Pl, m2])."""
+        vor_out = self.zeros_part() if vor_out is None else vor_out
+        if stages is None:
+            stages = self.torch.zeros((12,) + tuple(vor.shape), dtype=self.torch.float64, device="cuda")
+        self._check(self._lib.sbslpm_bve_back_rk4_step_record(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vor),
+                                                              _ptr(vor_out), _ptr(stages)), "bve_back_rk4_step_record")
+        return vor_out, stages
+
+    def bve_back_rk4_step_adjoint_stages(self, x, y, z, area, vor, stages, new_vor_bar, vor_bar=None):
+        """bve_back_rk4_step_adjoint with the step's recorded stage velocities."""
+        vor_bar = self.zeros_part() if vor_bar is None else vor_bar
+        self._check(self._lib.sbslpm_bve_back_rk4_step_adjoint_stages(
+            self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vor), _ptr(stages), _ptr(new_vor_bar), _ptr(vor_bar)),
+            "bve_back_rk4_step_adjoint_stages")
+        return vor_bar
+
     def bve_departure_to_target_adjoint(self, dep, new_vor_bar, vor_bar=None):
         """Transpose of the remap interpolation: vor_bar += B^T new_vor_bar."""
         vor_bar = self.zeros_part() if vor_bar is None else vor_bar

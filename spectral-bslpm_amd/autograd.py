@@ -63,41 +63,55 @@ def remap(sbctx, z, dep, vor):
 
 class _BVESteps(torch.autograd.Function):
     """Forward: nsteps steps, keeping the state at every `every`-th step (the
-    checkpoints).  Backward: segment by segment from the last, the states
-    inside the segment are recomputed from its checkpoint, then the step
-    adjoints run in reverse order.  Memory: nsteps/every + every states; work:
-    one extra forward step per step plus the step adjoints (each of which
-    recomputes its own step's stage velocities)."""
+    checkpoints) and, for the last segment, each step's stage velocities.
+    Backward: segment by segment from the last, the segment's steps are rerun
+    from its checkpoint with their stage velocities recorded
+    (sbslpm_bve_back_rk4_step_record), then the reverse passes run in reverse
+    order on the recorded stages (sbslpm_bve_back_rk4_step_adjoint_stages).
+    Memory: nsteps/every states + every x (state + 12 stage fields); work: the
+    forward run, one recorded rerun of every segment but the last, and the
+    reverse passes."""
 
     @staticmethod
     def forward(fctx, sbctx, x, y, z, area, vor0, nsteps, every):
-        vel = sbctx.zeros_part(3)
+        nseg = (nsteps + every - 1) // every
+        last = (nseg - 1) * every
         v = vor0.detach().clone()
-        checkpoints = [v.clone()]
+        checkpoints, tail = [v.clone()], []
+        vel = None
         for n in range(nsteps):
-            sbctx.bve_back_rk4_step(x, y, z, area, v, vel)
+            if n >= last:
+                nv, stages = sbctx.bve_back_rk4_step_record(x, y, z, area, v)
+                tail.append((v, stages))
+                v = nv
+            else:
+                vel = sbctx.zeros_part(3) if vel is None else vel
+                sbctx.bve_back_rk4_step(x, y, z, area, v, vel)
             if (n + 1) % every == 0 and n + 1 < nsteps:
                 checkpoints.append(v.clone())
         fctx.sbctx, fctx.nsteps, fctx.every = sbctx, nsteps, every
         fctx.geom = (x, y, z, area)
-        fctx.checkpoints = checkpoints
+        fctx.checkpoints, fctx.tail = checkpoints, tail
         return v
 
     @staticmethod
     def backward(fctx, g):
         sbctx, nsteps, every = fctx.sbctx, fctx.nsteps, fctx.every
         x, y, z, area = fctx.geom
-        vel = sbctx.zeros_part(3)
         gbar = g.contiguous()
         for seg in reversed(range(len(fctx.checkpoints))):
             start, end = seg * every, min((seg + 1) * every, nsteps)
-            states = [fctx.checkpoints[seg]]
-            v = states[0].clone()
-            for _ in range(start, end - 1):
-                sbctx.bve_back_rk4_step(x, y, z, area, v, vel)
-                states.append(v.clone())
-            for n in reversed(range(start, end)):
-                gbar = sbctx.bve_back_rk4_step_adjoint(x, y, z, area, states[n - start], gbar)
+            if seg == len(fctx.checkpoints) - 1:
+                rec = fctx.tail
+            else:
+                rec, v = [], fctx.checkpoints[seg]
+                for _ in range(start, end):
+                    nv, stages = sbctx.bve_back_rk4_step_record(x, y, z, area, v)
+                    rec.append((v, stages))
+                    v = nv
+            for state, stages in reversed(rec):
+                gbar = sbctx.bve_back_rk4_step_adjoint_stages(x, y, z, area, state, stages, gbar)
+        fctx.tail = None
         return None, None, None, None, None, gbar, None, None
 
 

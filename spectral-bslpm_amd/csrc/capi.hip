@@ -1360,37 +1360,44 @@ int sbslpm_bve_velocity_adjoint(sbslpm_ctx* c, const double* area, const double*
   return launched(c);
 }
 
-int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* c, const double* x, const double* y, const double* z,
-                                     const double* area, const double* vor, const double* new_vor_bar,
-                                     double* vor_bar) {
-  if (!c || !x || !y || !z || !area || !vor || !new_vor_bar || !vor_bar)
-    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint: null pointer");
-  if (vor_bar == vor || vor_bar == new_vor_bar)
-    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint: vor_bar must not alias the inputs");
-  int rc = ensure_adjoint(c);
-  if (rc) return rc;
-  const Tree& t = c->tree;
+}  // extern "C"
+
+namespace {
+
+// The four stage velocities of one bve_back_rk4_step into V[0..11] (v0..v3),
+// with the context's precision and kernels.
+int step_stages(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area, const double* vor,
+                double* const* V) {
   const Work& w = c->w1;
   const double dt = c->prm.delta_t, om = c->prm.omega;
   const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
-  double* const* V = c->adj_v;
-  double* const* VB = c->adj_vb;
   const double* cdp[3] = {w.disppart[0], w.disppart[1], w.disppart[2]};
   const double* cdx[3] = {w.dispproxy[0], w.dispproxy[1], w.dispproxy[2]};
-  // forward, keeping the four stage velocities
-  rc = velocity(c, w, 1, area, vor, nullptr, nullptr, V);
+  int rc = velocity(c, w, 1, area, vor, nullptr, nullptr, V);
   for (int s = 0; s < 3 && rc == SBSLPM_OK; ++s) {
     const double* vin[3] = {V[3 * s], V[3 * s + 1], V[3 * s + 2]};
     launch_bve_displacement(c->dt, x, y, z, vin, vor, taus[s], om, w.newvor, w.disppart, w.dispproxy, c->stream);
     rc = velocity(c, w, 1, area, w.newvor, cdp, cdx, V + 3 * (s + 1));
   }
-  if (rc) return rc;
-  // reverse: remap and departure search
+  return rc;
+}
+
+// Reverse pass of one step given its stage velocities V (see
+// sbslpm_bve_back_rk4_step_adjoint).
+int step_reverse(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
+                 const double* vor, const double* const* V, const double* new_vor_bar, double* vor_bar) {
+  const Tree& t = c->tree;
+  const Work& w = c->w1;
+  const double dt = c->prm.delta_t, om = c->prm.omega;
+  const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
+  double* const* VB = c->adj_vb;
+  const double* cdp[3] = {w.disppart[0], w.disppart[1], w.disppart[2]};
+  const double* cdx[3] = {w.dispproxy[0], w.dispproxy[1], w.dispproxy[2]};
+  int rc = SBSLPM_OK;
+  // remap and departure search
   launch_fill_zero(vor_bar, t.npart, c->stream);
   launch_fill_zero(VB[0], 9 * t.npart, c->stream);  // v0..v2 cotangents (contiguous)
-  const double* cv[12];
-  for (int i = 0; i < 12; ++i) cv[i] = V[i];
-  launch_departure_adjoint(c->dt, x, y, z, cv, vor, new_vor_bar, dt, c->prm.rk4_combine, om, VB, vor_bar, c->stream);
+  launch_departure_adjoint(c->dt, x, y, z, V, vor, new_vor_bar, dt, c->prm.rk4_combine, om, VB, vor_bar, c->stream);
   // stages 3, 2, 1: velocity at displaced sources, then the displacement update
   for (int s = 2; s >= 0 && rc == SBSLPM_OK; --s) {
     const double* vin[3] = {V[3 * s], V[3 * s + 1], V[3 * s + 2]};
@@ -1408,6 +1415,59 @@ int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* c, const double* x, const doubl
   launch_stage_adjoint(c->dt, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0, area, c->adj_qb, nullptr, vor_bar, nullptr,
                        c->stream);
   return launched(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbslpm_bve_back_rk4_step_adjoint(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                     const double* area, const double* vor, const double* new_vor_bar,
+                                     double* vor_bar) {
+  if (!c || !x || !y || !z || !area || !vor || !new_vor_bar || !vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint: null pointer");
+  if (vor_bar == vor || vor_bar == new_vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint: vor_bar must not alias the inputs");
+  int rc = ensure_adjoint(c);
+  if (!rc) rc = step_stages(c, x, y, z, area, vor, c->adj_v);
+  if (rc) return rc;
+  const double* cv[12];
+  for (int i = 0; i < 12; ++i) cv[i] = c->adj_v[i];
+  return step_reverse(c, x, y, z, area, vor, cv, new_vor_bar, vor_bar);
+}
+
+int sbslpm_bve_back_rk4_step_record(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                    const double* area, const double* vor, double* vor_out, double* stages) {
+  if (!c || !x || !y || !z || !area || !vor || !vor_out || !stages)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_record: null pointer");
+  if (vor_out == vor) return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_record: vor_out must not alias vor");
+  const int64_t N = c->tree.npart;
+  double* V[12];
+  for (int i = 0; i < 12; ++i) V[i] = stages + i * N;
+  int rc = step_stages(c, x, y, z, area, vor, V);
+  if (rc) return rc;
+  const Work& w = c->w1;
+  const double* cv[12];
+  for (int i = 0; i < 12; ++i) cv[i] = V[i];
+  launch_bve_departure(c->dt, x, y, z, cv, c->prm.delta_t, c->prm.rk4_combine, w.dep[0], w.dep[1], w.dep[2], c->stream);
+  launch_bve_departure_to_target(c->dt, z, w.dep[0], w.dep[1], w.dep[2], vor, nullptr, 0, c->prm.omega, vor_out, nullptr,
+                                 c->stream);
+  return launched(c);
+}
+
+int sbslpm_bve_back_rk4_step_adjoint_stages(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                            const double* area, const double* vor, const double* stages,
+                                            const double* new_vor_bar, double* vor_bar) {
+  if (!c || !x || !y || !z || !area || !vor || !stages || !new_vor_bar || !vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint_stages: null pointer");
+  if (vor_bar == vor || vor_bar == new_vor_bar)
+    return fail(c, SBSLPM_E_ARG, "bve_back_rk4_step_adjoint_stages: vor_bar must not alias the inputs");
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  const int64_t N = c->tree.npart;
+  const double* cv[12];
+  for (int i = 0; i < 12; ++i) cv[i] = stages + i * N;
+  return step_reverse(c, x, y, z, area, vor, cv, new_vor_bar, vor_bar);
 }
 
 int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* c, const double* dep_x, const double* dep_y,

@@ -165,3 +165,22 @@ def test_mixed_precision_gradient_matches_fp64(sb, torch_gpu):
     e_vel = W.rel_l2(to_np(vmx), to_np(v64))
     print(f"  mixed vs fp64: velocity adjoint {e_vel:.2e}, step gradient {e_step:.2e}")
     assert e_vel < 1e-6 and e_step < 1e-6
+
+
+def test_recorded_step_and_stage_adjoint(sb, torch_gpu):
+    """The recording step equals bve_back_rk4_step bit for bit, and the reverse pass on the
+    recorded stages equals the step adjoint (which recomputes them) to rounding."""
+    t = torch_gpu
+    print("\n[case recorded step]", flush=True)
+    ctx, x, y, z, a = make(sb, "reference")
+    g = [to_gpu(t, q) for q in (x, y, z, a)]
+    w = to_gpu(t, sb.workloads.gaussian_vortices(x, y, z, a, seed=51))
+    r = to_gpu(t, np.random.default_rng(52).normal(size=x.shape))
+    ref = w.clone()
+    ctx.bve_back_rk4_step(*g, ref, ctx.zeros_part(3))
+    out, stages = ctx.bve_back_rk4_step_record(*g, w)
+    g1 = ctx.bve_back_rk4_step_adjoint(*g, w, r)
+    g2 = ctx.bve_back_rk4_step_adjoint_stages(*g, w, stages, r)
+    ctx.synchronize()
+    np.testing.assert_array_equal(to_np(out), to_np(ref))
+    np.testing.assert_allclose(to_np(g2), to_np(g1), rtol=0, atol=1e-13 * float(g1.abs().max()))
