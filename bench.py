@@ -32,6 +32,7 @@ METRIC = "BVE RK4 steps/sec at 1.57M particles, mixed vs FP64 plain, rel-L2 err 
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 (vector and matrix)
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32
 FLOP_BS = 27       # flop per Biot-Savart pair (difference form; div/rcp/sqrt counted as 1)
+FLOP_LOG = 13      # flop per log (Poisson) pair, same convention
 
 CONFIGS = {
     # levels, proxy degree, particle-grid degree, theta
@@ -381,13 +382,176 @@ def bench_4dvar(args, torch, sb):
     print(json.dumps(line), flush=True)
 
 
+C2 = dict(levels=6, interp_degree=10, leaf_degree=7, fmm_theta=0.6)
+
+
+def copy_probe(args, torch):
+    """A plain device copy of --copy-probe MiB, three times: the scale reference for the
+    HBM byte counters read in the same profiled run."""
+    if args.copy_probe <= 0:
+        return
+    nbytes = args.copy_probe << 20
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {nbytes} bytes read + written")
+
+
+def tangent_gradient(fun, tx, ty, tz, h=1e-4):
+    """Surface gradient of a function of unit vectors by fourth-order central differences
+    in arc length along the great circles through each point in two tangent directions."""
+    T = np.stack([tx, ty, tz], 1)
+    ref = np.where(np.abs(T[:, 2:3]) < 0.9, np.array([[0.0, 0.0, 1.0]]), np.array([[1.0, 0.0, 0.0]]))
+    e1 = np.cross(T, ref)
+    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+    e2 = np.cross(T, e1)
+    grad = np.zeros_like(T)
+    for e in (e1, e2):
+        def at(s):
+            P = np.cos(s) * T + np.sin(s) * e
+            return fun(P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy())
+        d = (8 * (at(h) - at(-h)) - (at(2 * h) - at(-2 * h))) / (12 * h)
+        grad += d[:, None] * e
+    return grad
+
+
+def bench_c2(args, torch, sb):
+    """Sphere Poisson FMM solve at 6x256^2 = 393,216 particles: L=6 (6,144 leaves), an
+    8x8 Clenshaw-Curtis particle grid per leaf, proxy degree 10, theta 0.6.  Forcing
+    f = sum a_lm Y_lm, 1 <= l <= 32, a_lm ~ N(0, (1+l)^-2) (seeded; zero mean).  Times
+    poisson_solve (upward pass, log pair sums, downward pass) in FP64 -- the value -- and
+    in mixed precision; checks the stream function and the Biot-Savart velocity (eps 0)
+    at 4,096 seeded targets against the FP64 direct sums at those targets and against the
+    closed forms psi = sum a_lm Y_lm / (l(l+1)), u = -x cross grad psi; log pair-kernel
+    roofline from HIP events on the context stream; the oracle's full solve as the CPU
+    baseline."""
+    W = sb.workloads
+    stream = torch.cuda.Stream()
+    common = dict(weights="clenshaw_curtis", kernel_eps=0.0, **C2)
+    with torch.cuda.stream(stream):
+        ctx64 = sb.Context(sb.RunConfig(precision="fp64", **common))
+        ctxmx = sb.Context(sb.RunConfig(precision="mixed", **common))
+    s = ctx64.sizes
+    x, y, z, a = ctx64.spec_points()
+    f_h = W.random_sh_forcing(x, y, z, lmax=32, seed=1)[0]
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    ga, gf = dev(a), dev(f_h)
+    torch.cuda.synchronize()
+
+    def time_solves(ctx):
+        psi = ctx.zeros_part()
+        with torch.cuda.stream(stream):
+            for _ in range(max(args.warmup, 1)):
+                ctx.poisson_solve(ga, gf, psi)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            for _ in range(args.steps):
+                ctx.poisson_solve(ga, gf, psi)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        ctx.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / args.steps, psi
+
+    t64, psi64 = time_solves(ctx64)
+    tmx, psimx = time_solves(ctxmx)
+    log(f"poisson solve: fp64 {1e3 * t64:.3f} ms, mixed {1e3 * tmx:.3f} ms")
+
+    # accuracy at 4,096 seeded targets: FMM vs the FP64 direct sums there and the closed forms
+    tx, ty, tz = W.random_unit_vectors(4096, seed=2)
+    gt = [dev(v) for v in (tx, ty, tz)]
+    flat = [dev(v.ravel().copy()) for v in (x, y, z, a, f_h)]
+    with torch.cuda.stream(stream):
+        p_fmm = ctx64.poisson_solve_targets(ga, gf, *gt)
+        p_ds = ctx64.direct_sum_inv_lap_targets(*gt, *flat)
+        u_fmm = ctx64.bve_velocity_targets(ga, gf, *gt)
+        u_ds = ctx64.direct_sum_bve_vel_targets(*gt, *flat)
+        p_fmm_mx = ctxmx.poisson_solve_targets(ga, gf, *gt)
+    torch.cuda.synchronize()
+    ctx64.synchronize()
+    ctxmx.synchronize()
+    h = lambda v: v.cpu().numpy()
+    p_fmm, p_ds, p_fmm_mx = h(p_fmm), h(p_ds), h(p_fmm_mx)
+    u_fmm, u_ds = [h(v) for v in u_fmm], [h(v) for v in u_ds]
+    p_exact = W.random_sh_forcing(tx, ty, tz, lmax=32, seed=1)[1]
+    grad = tangent_gradient(lambda px, py, pz: W.random_sh_forcing(px, py, pz, lmax=32, seed=1)[1], tx, ty, tz)
+    u_exact = list((-np.cross(np.stack([tx, ty, tz], 1), grad)).T)
+    acc = dict(psi_fmm_vs_direct_rel_l2=W.rel_l2(p_fmm, p_ds), psi_fmm_vs_direct_rel_linf=W.rel_linf(p_fmm, p_ds),
+               vel_fmm_vs_direct_rel_l2=W.vec_rel_l2(u_fmm, u_ds), vel_fmm_vs_direct_rel_linf=W.vec_rel_linf(u_fmm, u_ds),
+               psi_direct_vs_closed_form_rel_l2=W.rel_l2(p_ds, p_exact),
+               vel_direct_vs_closed_form_rel_l2=W.vec_rel_l2(u_ds, u_exact),
+               psi_mixed_targets_vs_direct_rel_l2=W.rel_l2(p_fmm_mx, p_ds),
+               psi_mixed_vs_fp64_particles_rel_l2=W.rel_l2(h(psimx), h(psi64)))
+    log("accuracy", acc)
+
+    # log pair-kernel roofline: HIP events around each pair-kernel launch on the context stream
+    pairs = pair_count(ctx64.interaction_list(), s.P, s.Pl, s.n, s.m2)
+    roof = {}
+    for ctx, name, peak in ((ctx64, "log_fp64", PEAK_FP64), (ctxmx, "log_mixed", PEAK_FP32)):
+        ctx.enable_timing(True)
+        with torch.cuda.stream(stream):
+            for _ in range(5):
+                ctx.poisson_solve(ga, gf)
+        torch.cuda.synchronize()
+        ms, nl = ctx.kernel_time(name)
+        ctx.enable_timing(False)
+        avg = ms / nl / 1e3
+        achieved = pairs * FLOP_LOG / avg / 1e12
+        roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                          frac=round(achieved / peak, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
+                          pairs_per_launch=pairs, flop_per_pair=FLOP_LOG)
+    tfile = os.path.join(REPO, "profiles", "poisson_c2_hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tr = json.load(fh)["kernels"]
+        for name, kern in (("log_fp64", "k_log64"), ("log_mixed", "k_log_mixed")):
+            if tr.get(kern):
+                roof[name]["traffic"] = tr[kern]["hbm_bytes"]
+                roof[name]["traffic_kernel"] = kern
+    log("roofline", roof)
+    copy_probe(args, torch)
+
+    cpu = None
+    if args.cpu_budget > 0:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        O.set_threads(threads)
+        o = O.OracleFMM(C2["levels"], C2["interp_degree"], C2["leaf_degree"], C2["fmm_theta"], 1)
+        t0 = time.perf_counter()
+        o.poisson_solve(a, f_h)
+        sec = time.perf_counter() - t0
+        cpu = dict(value=round(1.0 / sec, 5), unit="solves/s", cores=threads, kind="port",
+                   sample=f"one full oracle Poisson solve at 6x256^2 (upward pass, all {pairs:.3e} log pairs, "
+                          f"downward pass), {sec:.1f} s")
+        log(f"cpu baseline: {cpu}")
+
+    line = {"metric": "Sphere Poisson FMM solves/sec at 6x256^2 = 393,216 particles (FP64)",
+            "value": round(1.0 / t64, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(1e3 * t64, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded spherical-harmonic forcing, l <= 32; 4,096 seeded targets)",
+            "config": {"workload": "Sphere Poisson solve (c2): 6x256^2 particles", "particles": s.N,
+                       "levels": C2["levels"], "proxy_degree": C2["interp_degree"],
+                       "leaf_grid": C2["leaf_degree"] + 1, "theta": C2["fmm_theta"], "interactions": s.I,
+                       "pairs_per_solve": pairs, "targets": 4096},
+            "mixed_solves_per_s": round(1.0 / tmx, 4), "mixed_ms_per_solve": round(1e3 * tmx, 4),
+            "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+            "accuracy": {k: float(f"{v:.3e}") for k, v in acc.items()},
+            "roofline": roof["log_fp64"], "roofline_mixed": roof["log_mixed"], "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4", "adjoint", "4dvar"],
-                    help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c4: the 48-member ensemble, "
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "adjoint", "4dvar"],
+                    help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c2: the 6x256^2 Poisson solve, "
+                         "c4: the 48-member ensemble, "
                          "adjoint: cost of the step gradient at 6x512^2, 4dvar: BVE 4D-Var gradient at 6x192^2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
@@ -413,6 +577,9 @@ def main():
         return
     if args.config == "4dvar":
         bench_4dvar(args, torch, sb)
+        return
+    if args.config == "c2":
+        bench_c2(args, torch, sb)
         return
     W = sb.workloads
     cfgd = CONFIGS[args.config]
@@ -514,14 +681,7 @@ def main():
                 roof[name]["traffic_kernel"] = kern
     log("roofline", roof)
 
-    if args.copy_probe > 0:
-        nbytes = args.copy_probe << 20
-        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy probe: 3 x {nbytes} bytes read + written")
+    copy_probe(args, torch)
 
     cpu = None
     if args.cpu_budget > 0:

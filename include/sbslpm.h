@@ -231,6 +231,14 @@ int sbslpm_direct_sum_bve_vel(sbslpm_ctx* ctx, int64_t n, const double* x, const
                               const double* disp_z, double* vel_x, double* vel_y, double* vel_z, int64_t lb,
                               int64_t ub);
 
+/* Biot-Savart direct sum at nt separate targets over ns undisplaced sources (no reference
+ * counterpart; pair kernel of bve_vel_impl.hpp:63-66 with the context's eps):
+ * vel[i] += sum_j -1/(4 pi) area_j vor_j (t_i cross y_j)/(1 - t_i.y_j + eps), vel_{x,y,z} [nt]. */
+int sbslpm_direct_sum_bve_vel_targets(sbslpm_ctx* ctx, int64_t nt, const double* tx, const double* ty,
+                                      const double* tz, int64_t ns, const double* sx, const double* sy,
+                                      const double* sz, const double* area, const double* vor, double* vel_x,
+                                      double* vel_y, double* vel_z);
+
 /* bve_displacement_upward_pass (bve_time_step.cpp:162-170): new_vor = vor + 2 Omega z - 2 Omega z',
  * z' of P(x + tau v); disp_part = tau v [Pl][m2]; disp_proxy [P][n] non-leaf rows interpolated. */
 int sbslpm_bve_displacement(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,

@@ -67,6 +67,8 @@ def lib():
                                              ctypes.c_int64, ctypes.c_int64]
         L.orc_direct_sum_inv_lap_targets.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp,
                                                      _dp, _dp, _dp]
+        L.orc_direct_sum_bve_vel_targets.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp, _dp,
+                                                     _dp, _dp, ctypes.c_double, _ppd]
         L.orc_bve_displacement.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _dp, ctypes.c_double,
                                            ctypes.c_double, _dp, _ppd, _ppd]
         L.orc_bve_displacement.restype = ctypes.c_int
@@ -198,6 +200,15 @@ def direct_sum_inv_lap_targets(tx, ty, tz, sx, sy, sz, area, pots):
     lib().orc_direct_sum_inv_lap_targets(len(tx), _p(tx), _p(ty), _p(tz), len(sx), _p(sx), _p(sy), _p(sz),
                                          _p(area), _p(pots), _p(out))
     return out
+
+
+def direct_sum_bve_vel_targets(tx, ty, tz, sx, sy, sz, area, vor, eps):
+    """Biot-Savart velocity at separate targets over undisplaced sources (kernel of
+    bve_vel_impl.hpp:63-66); returns [vx, vy, vz]."""
+    vel = [np.zeros(len(tx)) for _ in range(3)]
+    lib().orc_direct_sum_bve_vel_targets(len(tx), _p(tx), _p(ty), _p(tz), len(sx), _p(sx), _p(sy), _p(sz),
+                                         _p(area), _p(vor), float(eps), _pp(vel))
+    return vel
 
 
 class OracleFMM:

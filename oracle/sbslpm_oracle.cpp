@@ -875,6 +875,25 @@ void orc_direct_sum_inv_lap_targets(int64_t NT, const double* tx, const double* 
   }
 }
 
+// Biot-Savart direct sum at separate target points over undisplaced sources,
+// the pair kernel of bve_vel_impl.hpp:63-66; vel[3][NT] accumulated.
+void orc_direct_sum_bve_vel_targets(int64_t NT, const double* tx, const double* ty, const double* tz, int64_t NS,
+                                    const double* sx, const double* sy, const double* sz, const double* area,
+                                    const double* vor, double eps, double* const* vel) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < NT; i++) {
+    double a1 = 0, a2 = 0, a3 = 0;
+    for (int64_t j = 0; j < NS; j++) {
+      double dp = tx[i] * sx[j] + ty[i] * sy[j] + tz[i] * sz[j];
+      double g = kNegInv4Pi / (1.0 - dp + eps) * area[j] * vor[j];
+      a1 += g * (ty[i] * sz[j] - tz[i] * sy[j]);
+      a2 += g * (tz[i] * sx[j] - tx[i] * sz[j]);
+      a3 += g * (tx[i] * sy[j] - ty[i] * sx[j]);
+    }
+    vel[0][i] += a1; vel[1][i] += a2; vel[2][i] += a3;
+  }
+}
+
 // FMM evaluation at arbitrary points: the `_ll` pass structure
 // (fmm_funcs.cpp:394-419) with targets that are not sources.  A target in
 // leaf L gets

@@ -111,6 +111,7 @@ SIGNATURES = {
     "sbslpm_direct_sum_inv_lap": (_I, [_V, _I64, _V, _V, _V, _V, _V, _V, _I64, _I64]),
     "sbslpm_direct_sum_inv_lap_targets": (_I, [_V, _I64, _V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
     "sbslpm_direct_sum_bve_vel": (_I, [_V, _I64] + [_V] * 11 + [_I64, _I64]),
+    "sbslpm_direct_sum_bve_vel_targets": (_I, [_V, _I64, _V, _V, _V, _I64] + [_V] * 8),
     "sbslpm_bve_displacement": (_I, [_V] + [_V] * 7 + [_D] + [_V] * 7),
     "sbslpm_bve_find_departure_points": (_I, [_V, _V, _V, _V, _V, _D, _V, _V, _V]),
     "sbslpm_bve_departure_to_target": (_I, [_V, _V, _V, _V, _V, _V, _V, _I, _V, _V]),

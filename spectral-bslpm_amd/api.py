@@ -386,6 +386,17 @@ class Context:
                     "direct_sum_inv_lap_targets")
         return soln
 
+    def direct_sum_bve_vel_targets(self, tx, ty, tz, sx, sy, sz, area, vor, vel=None):
+        """Biot-Savart direct sum at separate targets over undisplaced sources (the
+        context's eps); vel (three 1-D tensors) accumulated."""
+        nt = tx.numel()
+        vel = [self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") for _ in range(3)] if vel is None else vel
+        self._check(self._lib.sbslpm_direct_sum_bve_vel_targets(self.h, nt, _ptr(tx), _ptr(ty), _ptr(tz), sx.numel(),
+                                                                _ptr(sx), _ptr(sy), _ptr(sz), _ptr(area), _ptr(vor),
+                                                                *[_ptr(v) for v in vel]),
+                    "direct_sum_bve_vel_targets")
+        return vel
+
     def direct_sum_bve_vel(self, x, y, z, area, vor, disp=None, vel=None, lb=0, ub=None):
         n = x.numel()
         ub = n if ub is None else ub

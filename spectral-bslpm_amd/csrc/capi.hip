@@ -796,6 +796,21 @@ int sbslpm_direct_sum_bve_vel(sbslpm_ctx* c, int64_t n, const double* x, const d
   return launched(c);
 }
 
+int sbslpm_direct_sum_bve_vel_targets(sbslpm_ctx* c, int64_t nt, const double* tx, const double* ty,
+                                      const double* tz, int64_t ns, const double* sx, const double* sy,
+                                      const double* sz, const double* area, const double* vor, double* vx,
+                                      double* vy, double* vz) {
+  if (!c || !tx || !ty || !tz || !sx || !sy || !sz || !area || !vor || !vx || !vy || !vz || nt < 0 || ns < 0)
+    return fail(c, SBSLPM_E_ARG, "direct_sum_bve_vel_targets: bad argument");
+  if (nt == 0) return SBSLPM_OK;
+  int rc = ensure_scratch(c, direct_partials_size(nt, ns, 3));
+  if (rc) return rc;
+  double* vel[3] = {vx, vy, vz};
+  launch_direct_bve_vel(nt, tx, ty, tz, ns, sx, sy, sz, area, vor, nullptr, c->prm.kernel_eps, c->scratch, vel,
+                        c->stream);
+  return launched(c);
+}
+
 int sbslpm_bve_displacement(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* vx,
                             const double* vy, const double* vz, const double* vor, double tau, double* new_vor,
                             double* dpx, double* dpy, double* dpz, double* dxx, double* dxy, double* dxz) {

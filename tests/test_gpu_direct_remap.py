@@ -64,6 +64,17 @@ def test_direct_sum_targets_and_bve(c1):
     gv = ctx.direct_sum_bve_vel(*[to_gpu(t, q) for q in flat], to_gpu(t, vor), [to_gpu(t, d) for d in disp])
     ctx.synchronize()
     assert W.vec_rel_l2([to_np(q) for q in gv], ev) < 1e-12
+    # Biot-Savart at separate targets: the oracle's sum, and on particles lb..ub the particle sum
+    ev = orc.direct_sum_bve_vel_targets(tx, ty, tz, *flat, vor, EPS)
+    gv = ctx.direct_sum_bve_vel_targets(*[to_gpu(t, q) for q in (tx, ty, tz)], *[to_gpu(t, q) for q in flat],
+                                        to_gpu(t, vor))
+    lb, ub = 300, 1100
+    gs = ctx.direct_sum_bve_vel_targets(*[to_gpu(t, q[lb:ub].copy()) for q in flat[:3]],
+                                        *[to_gpu(t, q) for q in flat], to_gpu(t, vor))
+    ep = orc.direct_sum_bve_vel(*flat, vor, EPS, None, lb, ub)
+    ctx.synchronize()
+    assert W.vec_rel_l2([to_np(q) for q in gv], ev) < 1e-12
+    assert W.vec_rel_l2([to_np(q) for q in gs], [e[lb:ub] for e in ep]) < 1e-12
 
 
 def test_displacement(c1):

@@ -101,3 +101,43 @@ def test_targets_nan_is_a_geometry_error(orc):
     t = np.array([np.nan, 0.0])
     with pytest.raises(RuntimeError):
         o.eval_targets("poisson", a, np.ones_like(a), t, np.array([0.0, 1.0]), np.array([1.0, 0.0]))
+
+
+def test_bs_direct_targets_is_rotated_gradient_of_log_sum(orc, sb):
+    """With eps = 0 the Biot-Savart sum at a target t is -t x grad_t of the log sum
+    psi_h(t) = sum_j -1/(4 pi) log(1 - t.y_j) q_j (ambient gradient; its radial part
+    drops out of the cross product).  The gradient is taken by central differences of
+    the log direct sum at targets moved off the sphere, so a wrong sign, a swapped
+    cross-product order or a dropped weight in the Biot-Savart sum fails here."""
+    W = sb.workloads
+    o = orc.OracleFMM(2, 6, 5, 0.7, 1)
+    x, y, z, a = [v.ravel().copy() for v in o.spec_points()]
+    f = W.random_sh_forcing(x, y, z, lmax=6, seed=3)[0]
+    T = np.stack(W.random_unit_vectors(64, seed=5), 1)
+    tx, ty, tz = [T[:, i].copy() for i in range(3)]
+    vel = orc.direct_sum_bve_vel_targets(tx, ty, tz, x, y, z, a, f, 0.0)
+    h = 1e-6
+    psi_at = lambda P: orc.direct_sum_inv_lap_targets(P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy(), x, y, z, a, f)
+    grad = np.zeros_like(T)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = h
+        # fourth-order central difference
+        grad[:, k] = (8 * (psi_at(T + e) - psi_at(T - e)) - (psi_at(T + 2 * e) - psi_at(T - 2 * e))) / (12 * h)
+    expect = list((-np.cross(T, grad)).T)
+    assert W.vec_rel_l2(vel, expect) < 1e-8
+    # the opposite orientation is far off
+    assert W.vec_rel_l2(vel, [-v for v in expect]) > 1.0
+
+
+def test_bs_direct_targets_at_particles_equals_particle_sum(orc, sb):
+    """Targets placed on particles lb..ub reproduce the particle direct sum there (the
+    self pair and coincident edge nodes contribute t x t = 0)."""
+    W = sb.workloads
+    o = orc.OracleFMM(2, 6, 5, 0.7, 1)
+    x, y, z, a = [v.ravel().copy() for v in o.spec_points()]
+    f = W.gaussian_vortices(x, y, z, a, seed=4)
+    lb, ub = 100, 400
+    got = orc.direct_sum_bve_vel_targets(x[lb:ub].copy(), y[lb:ub].copy(), z[lb:ub].copy(), x, y, z, a, f, EPS)
+    ref = orc.direct_sum_bve_vel(x, y, z, a, f, EPS, None, lb, ub)
+    assert W.vec_rel_l2(got, [r[lb:ub] for r in ref]) < 1e-14
