@@ -33,6 +33,8 @@ PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 (vector and matrix)
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32
 FLOP_BS = 27       # flop per Biot-Savart pair (difference form; div/rcp/sqrt counted as 1)
 FLOP_LOG = 13      # flop per log (Poisson) pair, same convention
+FLOP_SWE = 40      # shallow-water pair (variant 1): the Biot-Savart 27 + q_d (1) + y - x(x.y) (6) + accumulation (6)
+FLOP_LAP = 30      # regularised-Laplacian pair: dot (5), t^2 (1), erf/exp terms and the polynomial (20), acc (4)
 
 CONFIGS = {
     # levels, proxy degree, particle-grid degree, theta
@@ -544,13 +546,109 @@ def bench_c2(args, torch, sb):
     print(json.dumps(line), flush=True)
 
 
+def bench_swe(args, torch, sb):
+    """Shallow-water RK4 step (swe_back_rk4_step) at 6x192^2 = 221,184 particles: L=6 (6,144
+    leaves), a 6x6 particle grid per leaf that is also the proxy grid (degree 5, the reference
+    layout), theta 0.7, eps 1e-4, dt 900 s, Omega 7.292e-5, the reference's cone mountain
+    (2000 m, e^{-16 r^2}), the reference indexing rule.  Initial state (lengths in Earth radii,
+    time in seconds): zonal flow u = 20 cos(lat) m/s (vor = 2 U z, div = 0) and the balanced
+    geopotential g h0 / a^2 - (Omega U + U^2/2) z^2 for h0 = 5960 m.  One step = 4 shallow-water
+    FMM velocities, 2 regularised-Laplacian FMM evaluations, the panel Laplacian, 3 tendency
+    passes, departure search and remap.  Reports steps/s, the two pair kernels' rooflines (HIP
+    events) and one full oracle step on the host cores as the CPU baseline."""
+    W = sb.workloads
+    kw = dict(levels=6, interp_degree=5, leaf_degree=5, fmm_theta=0.7, weights="reference", kernel_eps=1e-4,
+              delta_t=900.0, omega=7.292e-5, topography="cone", swe_rule="reference")
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx = sb.Context(sb.RunConfig(**kw))
+    s = ctx.sizes
+    x, y, z, a = ctx.spec_points()
+    U, Om, g0 = 20.0 / 6371000.0, 7.292e-5, 9.80616
+    vor_h = 2 * U * z
+    div_h = np.zeros_like(z)
+    h_h = g0 * 5960.0 / 6371000.0 ** 2 - (Om * U + 0.5 * U * U) * z * z
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    gx, gy, gz, ga = [dev(v) for v in (x, y, z, a)]
+    state0 = [dev(v) for v in (vor_h, div_h, h_h)]
+    st = [v.clone() for v in state0]
+    vel, lap = ctx.zeros_part(3), ctx.zeros_part()
+
+    def run(n):
+        with torch.cuda.stream(stream):
+            for _ in range(n):
+                ctx.swe_back_rk4_step(gx, gy, gz, ga, *st, vel=vel, height_lap=lap)
+
+    run(max(args.warmup, 1))
+    torch.cuda.synchronize()
+    for v, v0 in zip(st, state0):
+        v.copy_(v0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+    run(args.steps)
+    with torch.cuda.stream(stream):
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / args.steps
+    log(f"swe step {1e3 * sec:.3f} ms")
+    finite = all(bool(torch.isfinite(v).all()) for v in st)
+
+    pairs = pair_count(ctx.interaction_list(), s.P, s.Pl, s.n, s.m2)
+    roof = {}
+    ctx.enable_timing(True)
+    for v, v0 in zip(st, state0):
+        v.copy_(v0)
+    run(2)
+    torch.cuda.synchronize()
+    for name, flop in (("swe_fp64", FLOP_SWE), ("lap_fp64", FLOP_LAP)):
+        ms, nl = ctx.kernel_time(name)
+        avg = ms / nl / 1e3
+        achieved = pairs * flop / avg / 1e12
+        roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP64, unit="TFLOP/s",
+                          frac=round(achieved / PEAK_FP64, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
+                          launches_per_step=nl // 2, pairs_per_launch=pairs, flop_per_pair=flop)
+    ctx.enable_timing(False)
+    log("roofline", roof)
+
+    cpu = None
+    if args.cpu_budget > 0:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        O.set_threads(threads)
+        o = O.OracleFMM(kw["levels"], kw["interp_degree"], kw["leaf_degree"], kw["fmm_theta"], 0)
+        t0 = time.perf_counter()
+        o.swe_back_rk4_step(x.ravel(), y.ravel(), z.ravel(), a, vor_h, div_h, h_h, kw["delta_t"], Om,
+                            kw["kernel_eps"], 0, 1, 0)
+        osec = time.perf_counter() - t0
+        cpu = dict(value=round(1.0 / osec, 5), unit="steps/s", cores=threads, kind="port",
+                   sample=f"one full oracle shallow-water step at 6x192^2 ({osec:.1f} s)")
+        log(f"cpu baseline: {cpu}")
+
+    line = {"metric": "SWE RK4 steps/sec at 6x192^2 = 221,184 particles (FP64)",
+            "value": round(1.0 / sec, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(1e3 * sec, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (zonal flow 20 cos(lat) m/s, balanced geopotential, cone mountain)",
+            "config": {"workload": "Shallow-water RK4 step: 6x192^2 particles", "particles": s.N,
+                       "levels": kw["levels"], "proxy_degree": kw["interp_degree"], "leaf_grid": kw["leaf_degree"] + 1,
+                       "theta": kw["fmm_theta"], "interactions": s.I, "pairs_per_evaluation": pairs,
+                       "dt": kw["delta_t"], "topography": "cone", "swe_rule": "reference"},
+            "state_finite_after_steps": finite,
+            "roofline": roof["swe_fp64"], "roofline_laplacian": roof["lap_fp64"], "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "adjoint", "4dvar"],
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "adjoint", "4dvar", "swe"],
                     help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c2: the 6x256^2 Poisson solve, "
+                         "swe: the shallow-water step at 6x192^2, "
                          "c4: the 48-member ensemble, "
                          "adjoint: cost of the step gradient at 6x512^2, 4dvar: BVE 4D-Var gradient at 6x192^2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
@@ -580,6 +678,9 @@ def main():
         return
     if args.config == "c2":
         bench_c2(args, torch, sb)
+        return
+    if args.config == "swe":
+        bench_swe(args, torch, sb)
         return
     W = sb.workloads
     cfgd = CONFIGS[args.config]

@@ -41,6 +41,14 @@ enum {
 
 enum { SBSLPM_WEIGHTS_REFERENCE = 0, SBSLPM_WEIGHTS_CLENSHAW_CURTIS = 1, SBSLPM_WEIGHTS_EQUAL_AREA = 2 };
 enum { SBSLPM_RK4_REFERENCE = 0, SBSLPM_RK4_TEXTBOOK = 1 };
+/* Shallow-water step indexing.  REFERENCE reproduces swe_back_rk4_step as written: the velocity
+ * gradient converts vectors with cubed_sphere_panels(i).face for leaf-local i
+ * (swe_time_step.cpp:109-111) and applies the metric factors twice (deriv_funcs_impl.hpp:291-292 and
+ * cubed_sphere_transforms_impl.hpp:161-162), its face-4/5 pole nodes return (g_xi, g_eta, 0); the
+ * topography reads the displacement of panel i (topography.cpp:33-35); the stage-4 state is advanced
+ * by dt/2 (swe_time_step.cpp:761).  LEAF uses the leaf's own face and displacement, the surface
+ * gradient, and dt at stage 4. */
+enum { SBSLPM_SWE_RULE_REFERENCE = 0, SBSLPM_SWE_RULE_LEAF = 1 };
 /* SBSLPM_MIXED: FP32 pair arithmetic on the matrix cores (Biot-Savart) / vector unit (log), FP64
  * accumulation; SBSLPM_MIXED_VALU: the same precision with the Biot-Savart pairs on the vector unit. */
 enum { SBSLPM_FP64 = 0, SBSLPM_MIXED = 1, SBSLPM_MIXED_VALU = 2 };
@@ -60,6 +68,8 @@ typedef struct sbslpm_params {
   double kernel_eps;  /* RunConfig::kernel_eps */
   double omega;       /* RunConfig::omega */
   double delta_t;     /* RunConfig::delta_t */
+  int topography;     /* RunConfig::topo_type: 0 none, 1 "cone" (topography.cpp:6-56) */
+  int swe_rule;       /* SBSLPM_SWE_RULE_*: the shallow-water step's gradient/topography indexing */
 } sbslpm_params;
 
 /* ---- Host-only set-up (no device needed) ------------------------------------------- */
@@ -134,8 +144,16 @@ int sbslpm_interaction_list(const sbslpm_ctx* ctx, int* list);
  * launch is bracketed by hipEvents on the context stream (eager calls only; not inside a captured
  * step).  sbslpm_kernel_time waits for the recorded events and returns the summed milliseconds and
  * the launch count of one kernel class since the last reset:
- * 0 Biot-Savart FP64, 1 Biot-Savart mixed, 2 log FP64, 3 log mixed. */
-enum { SBSLPM_KERNEL_BVE_FP64 = 0, SBSLPM_KERNEL_BVE_MIXED = 1, SBSLPM_KERNEL_LOG_FP64 = 2, SBSLPM_KERNEL_LOG_MIXED = 3 };
+ * 0 Biot-Savart FP64, 1 Biot-Savart mixed, 2 log FP64, 3 log mixed, 4 shallow-water FP64,
+ * 5 regularised Laplacian FP64. */
+enum {
+  SBSLPM_KERNEL_BVE_FP64 = 0,
+  SBSLPM_KERNEL_BVE_MIXED = 1,
+  SBSLPM_KERNEL_LOG_FP64 = 2,
+  SBSLPM_KERNEL_LOG_MIXED = 3,
+  SBSLPM_KERNEL_SWE_FP64 = 4,
+  SBSLPM_KERNEL_LAP_FP64 = 5
+};
 int sbslpm_ctx_enable_timing(sbslpm_ctx* ctx, int enable);
 int sbslpm_kernel_time(sbslpm_ctx* ctx, int kernel, double* ms, int64_t* launches, int reset);
 
@@ -391,6 +409,44 @@ int sbslpm_swe_find_departure_points(sbslpm_ctx* ctx, const double* x, const dou
 int sbslpm_swe_departure_to_target(sbslpm_ctx* ctx, const double* dep_x, const double* dep_y, const double* dep_z,
                                    const double* vor, const double* div, const double* h, const double* tracers,
                                    int ntr, double* new_vor, double* new_div, double* new_h, double* new_tracers);
+
+/* ---- Shallow-water step ------------------------------------------------------------------ */
+/* laplacian (deriv_funcs.cpp:27-37, panel_laplacian deriv_funcs_impl.hpp:338-389): spectral surface
+ * Laplacian of f on every leaf's particle grid; lap [Pl][m2] assigned. */
+int sbslpm_laplacian(sbslpm_ctx* ctx, const double* f, double* lap);
+/* laplacian_interactions (fmm_interactions/laplacian.cpp:6-35, laplacian_impl.hpp:11-69): the
+ * regularised-Laplacian kernel (eps = 0.1) over the interaction list, sources at P(node + disp)
+ * (disp may be NULL); phi_part [Pl][m2], phi_proxy [P][n] accumulated. */
+int sbslpm_laplacian_interactions(sbslpm_ctx* ctx, const double* disp_part_x, const double* disp_part_y,
+                                  const double* disp_part_z, const double* disp_proxy_x, const double* disp_proxy_y,
+                                  const double* disp_proxy_z, const double* q_part, const double* q_proxy,
+                                  double* phi_part, double* phi_proxy);
+/* apply_topography (topography.cpp:47-56): eff = height (no topography) or the cone mountain at
+ * P(x + disp) (disp: [Pl][m2] particle and [P][n] proxy displacements, NULL = zero); the
+ * displacement index follows the context's swe_rule. */
+int sbslpm_apply_topography(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                            const double* disp_part_x, const double* disp_part_y, const double* disp_part_z,
+                            const double* disp_proxy_x, const double* disp_proxy_y, const double* disp_proxy_z,
+                            const double* height, double* eff_height);
+/* swe_tendency_computation (swe_time_step.cpp:76-161) with the context's omega and swe_rule:
+ * vor_t, div_t, h_t [Pl][m2] assigned. */
+int sbslpm_swe_tendency(sbslpm_ctx* ctx, const double* x, const double* y, const double* z, const double* vel_x,
+                        const double* vel_y, const double* vel_z, const double* vor, const double* div,
+                        const double* height, const double* height_lap, double* vor_t, double* div_t, double* h_t);
+/* swe_height_correction (swe_time_step.cpp:558-604): height_lap -= 80 BLI[h + dt h_t](P(x + dt vel)). */
+int sbslpm_swe_height_correction(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                 const double* vel_x, const double* vel_y, const double* vel_z, const double* height,
+                                 const double* h_t, double dt, double* height_lap);
+/* swe_back_rk4_step (swe_time_step.cpp:606-811): vors, divs, height [Pl][m2] and tracers
+ * [Pl][m2][tracer_count] updated in place; vel_{x,y,z} receive the stage-1 velocity and height_lap
+ * the stage-1 surface Laplacian.  Variant-1 SWE kernel with the context's kernel_eps, omega,
+ * delta_t, rk4_combine, topography and swe_rule.  The stage-4 Laplacian, height correction and
+ * tendencies and the k2..k4 terms of swe_compute_arrival_values reach no output there
+ * (arrival = state + dt k1, :415-417) and are not computed.  `time` is accepted for signature
+ * parity (it only drives the commented-out forcing). */
+int sbslpm_swe_back_rk4_step(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                             const double* area, double* vors, double* divs, double* height, double* vel_x,
+                             double* vel_y, double* vel_z, double* tracers, double* height_lap, double time);
 
 /* bve_back_rk4_step (bve_time_step.cpp:311-419), no forcing (bve_forcing copy,
  * forcing_funcs.cpp:93-95).  vor [Pl][m2] and tracers [Pl][m2][tracer_count] updated in place;

@@ -86,6 +86,19 @@ def lib():
         L.orc_swe_departure_to_target.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _dp, ctypes.c_int, _ppd,
                                                   _dp]
         L.orc_swe_departure_to_target.restype = ctypes.c_int
+        L.orc_panel_laplacian.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_velocity_gradient.argtypes = [ctypes.c_void_p, _ppd, ctypes.c_int, _ppd]
+        L.orc_swe_tendency.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                       ctypes.c_int, _dp, _dp, _dp]
+        L.orc_apply_topography.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _ppd, _dp, ctypes.c_int,
+                                           ctypes.c_int, _dp]
+        L.orc_laplacian_interactions.argtypes = [ctypes.c_void_p, _ppd, _ppd, _dp, _dp, _dp, _dp]
+        L.orc_laplacian_interactions.restype = None
+        L.orc_swe_height_correction.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _ppd, _dp, _dp, ctypes.c_double,
+                                                _dp]
+        L.orc_swe_back_rk4_step.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, _ppd, _dp]
         L.orc_eval_targets.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int64,
                                        _dp, _dp, _dp, _ppd]
         L.orc_eval_targets.restype = ctypes.c_int
@@ -402,6 +415,79 @@ class OracleFMM:
         if err:
             raise RuntimeError("oracle bve_back_rk4_step: geometry error")
         return vor, vel, (tr if ntr else None)
+
+    # ---- shallow-water stepper (swe_time_step.cpp) ----
+    def _f(self, a):
+        return np.ascontiguousarray(a, dtype=np.float64).reshape(self.Pl, self.m2)
+
+    def _zp(self, k=None):
+        return np.zeros((self.Pl, self.m2)) if k is None else [np.zeros((self.Pl, self.m2)) for _ in range(k)]
+
+    def panel_laplacian(self, f):
+        """laplacian (deriv_funcs.cpp:27-37) on every leaf's particle grid."""
+        lap = self._zp()
+        lib().orc_panel_laplacian(self.h, _p(self._f(f)), _p(lap))
+        return lap
+
+    def velocity_gradient(self, vel, rule=0):
+        """[9] leaf gradients d vel_a / d x_b at 3a+b (single_panel_grad, deriv_funcs_impl.hpp:272-305)."""
+        g = self._zp(9)
+        if lib().orc_velocity_gradient(self.h, _pp([self._f(v) for v in vel]), rule, _pp(g)):
+            raise RuntimeError("oracle velocity_gradient: geometry error")
+        return g
+
+    def swe_tendency(self, x, y, z, vel, vor, div, height, lap, omega, rule=0):
+        """swe_tendency_computation (swe_time_step.cpp:76-161): (vor_t, div_t, h_t)."""
+        out = self._zp(3)
+        err = lib().orc_swe_tendency(self.h, _p(x), _p(y), _p(z), _pp([self._f(v) for v in vel]), _p(self._f(vor)),
+                                     _p(self._f(div)), _p(self._f(height)), _p(self._f(lap)), omega, rule,
+                                     *[_p(o) for o in out])
+        if err:
+            raise RuntimeError("oracle swe_tendency: geometry error")
+        return out
+
+    def apply_topography(self, x, y, z, height, topo=1, rule=0, disp_part=None, disp_proxy=None):
+        """apply_topography / cone_mountain (topography.cpp:6-56)."""
+        dp = self._zp(3) if disp_part is None else [self._f(d) for d in disp_part]
+        dx = [np.zeros((self.P, self.n)) for _ in range(3)] if disp_proxy is None else \
+            [np.ascontiguousarray(d, dtype=np.float64) for d in disp_proxy]
+        eff = self._zp()
+        if lib().orc_apply_topography(self.h, _p(x), _p(y), _p(z), _pp(dp), _pp(dx), _p(self._f(height)), topo,
+                                      rule, _p(eff)):
+            raise RuntimeError("oracle apply_topography: bad argument")
+        return eff
+
+    def laplacian_interactions(self, q_part, q_proxy, disp_part=None, disp_proxy=None):
+        """laplacian_interactions (laplacian.cpp:6-35); returns (phi_part, phi_proxy)."""
+        dp = self._zp(3) if disp_part is None else [self._f(d) for d in disp_part]
+        dx = [np.zeros((self.P, self.n)) for _ in range(3)] if disp_proxy is None else \
+            [np.ascontiguousarray(d, dtype=np.float64) for d in disp_proxy]
+        pp, px = self._zp(), np.zeros((self.P, self.n))
+        lib().orc_laplacian_interactions(self.h, _pp(dp), _pp(dx), _p(self._f(q_part)),
+                                         _p(np.ascontiguousarray(q_proxy, dtype=np.float64)), _p(pp), _p(px))
+        return pp, px
+
+    def swe_height_correction(self, x, y, z, vel, height, h_t, dt, lap):
+        """swe_height_correction (swe_time_step.cpp:558-604); returns the corrected lap."""
+        lap = self._f(lap).copy()
+        if lib().orc_swe_height_correction(self.h, _p(x), _p(y), _p(z), _pp([self._f(v) for v in vel]),
+                                           _p(self._f(height)), _p(self._f(h_t)), dt, _p(lap)):
+            raise RuntimeError("oracle swe_height_correction: geometry error")
+        return lap
+
+    def swe_back_rk4_step(self, x, y, z, area, vors, divs, height, dt, omega, eps, combine=0, topo=0, rule=0,
+                          tracers=None):
+        """swe_back_rk4_step (swe_time_step.cpp:606-811).  Returns (vors, divs, height,
+        stage-1 velocity, stage-1 surface Laplacian, tracers)."""
+        v, d, hh = [self._f(a).copy() for a in (vors, divs, height)]
+        ntr = 0 if tracers is None else tracers.shape[-1]
+        tr = np.zeros((self.Pl, self.m2, 1)) if tracers is None else np.ascontiguousarray(tracers, dtype=np.float64).copy()
+        vel, lap = self._zp(3), self._zp()
+        err = lib().orc_swe_back_rk4_step(self.h, _p(x), _p(y), _p(z), _p(area), _p(v), _p(d), _p(hh), _p(tr), ntr,
+                                          dt, omega, eps, combine, topo, rule, _pp(vel), _p(lap))
+        if err:
+            raise RuntimeError(f"oracle swe_back_rk4_step: error {err}")
+        return v, d, hh, vel, lap, (tr if ntr else None)
 
     def interp_points(self, x, y, z, fields):
         """latlon_interp_host at arbitrary points (1-D x, y, z); fields [Pl][m2] each."""

@@ -1505,6 +1505,424 @@ void orc_irreg_swe_vel_interactions(void* h, double* const* target_pots, double*
   }
 }
 
+// ================================================================ SWE stepper
+// swe_back_rk4_step (swe_time_step.cpp:606-811) and the functions it calls.
+
+// bli_deriv_xi / bli_deriv_eta (deriv_funcs_impl.hpp:8-92): barycentric
+// differentiation along one direction of a leaf's Chebyshev grid of degree deg,
+// nodes cos(pi i/deg) * half + mid (computed, not snapped), weights 1/2,
+// (-1)^i, (-1)^deg/2, in the reference's difference form
+//   d_i = sum_{k != i} (f_k - f_i) w_k / (x_i - x_k) / w_i.
+// dir 0 differentiates along the first index (xi), dir 1 along the second;
+// index a*(deg+1)+b.
+static void cheb_deriv(const double* f, int deg, double lo, double hi, int dir, double* out) {
+  const int m = deg + 1;
+  const double half = 0.5 * (hi - lo), mid = 0.5 * (hi + lo);
+  auto w = [deg](int i) {
+    if (i == 0) return 0.5;
+    if (i == deg) return 0.5 * ((deg & 1) ? -1.0 : 1.0);
+    return (i & 1) ? -1.0 : 1.0;
+  };
+  for (int a = 0; a < m; a++)
+    for (int b = 0; b < m; b++) {
+      const int i = dir == 0 ? a : b, idx = a * m + b;
+      const double xi = std::cos(kPi * i / deg) * half + mid, fi = f[idx];
+      double acc = 0;
+      for (int k = 0; k < m; k++) {
+        if (k == i) continue;
+        const int idx2 = dir == 0 ? k * m + b : a * m + k;
+        const double xk = std::cos(kPi * k / deg) * half + mid;
+        acc += (f[idx2] - fi) * w(k) / (xi - xk);
+      }
+      out[idx] = acc / w(i);
+    }
+}
+
+// xyzvec_from_xietavec (cubed_sphere_transforms_impl.hpp:153-239): the
+// (xi, eta) pair is mapped with the metric factors, then the 2x2 system for
+// the (colatitude, longitude) components is solved (here by Cramer's rule)
+// and rotated to Cartesian components.  At the pole nodes of faces 4 and 5 the
+// reference returns (g_xi, g_eta, 0) (:194-198, :215-219); with pole_fix the
+// surface gradient there is returned instead: (-g_eta, g_xi, 0) on face 4
+// (d/dxi = +y, d/deta = -x) and (g_eta, g_xi, 0) on face 5.
+static int xyzvec_from_xietavec(double& ox, double& oy, double& oz, double xd, double ed, int face, double xi,
+                                double eta, bool pole_fix) {
+  const double X = std::tan(xi), Y = std::tan(eta), X2 = X * X, Y2 = Y * Y;
+  const double C = std::sqrt(1 + X2), D = std::sqrt(1 + Y2);
+  const double gx = D * xd + X * Y / D * ed, ge = X * Y / C * xd + C * ed;
+  const double isqd = 1.0 / std::sqrt(1 + X2 + Y2);
+  double xyz[3];
+  if (xyz_from_xieta(xi, eta, face, xyz)) return ORC_E_GEOMETRY;
+  const double x = xyz[0], y = xyz[1], z = xyz[2];
+  const double sqc = std::sqrt(x * x + y * y);
+  double m00, m01, m10, m11;
+  if (face <= 3) {
+    m00 = 0; m01 = C * D * isqd; m10 = -1.0; m11 = X * Y * isqd;
+  } else {
+    if (std::fabs(xi) < 1e-16 && std::fabs(eta) < 1e-16) {  // at the pole
+      if (!pole_fix) { ox = gx; oy = ge; }
+      else if (face == 4) { ox = -ge; oy = gx; }
+      else { ox = ge; oy = gx; }
+      oz = 0;
+      return ORC_OK;
+    }
+    const double isqc2 = 1.0 / std::sqrt(X2 + Y2), s = face == 4 ? 1.0 : -1.0;
+    m00 = s * isqc2 * D * X; m01 = -s * isqc2 * D * Y * isqd;
+    m10 = s * isqc2 * C * Y; m11 = s * isqc2 * C * X * isqd;
+  }
+  const double det = m00 * m11 - m01 * m10;
+  const double colat = (gx * m11 - m01 * ge) / det, lon = (m00 * ge - m10 * gx) / det;
+  ox = x * z / sqc * colat - y / sqc * lon;
+  oy = y * z / sqc * colat + x / sqc * lon;
+  oz = -sqc * colat;
+  return ORC_OK;
+}
+
+// single_panel_grad (deriv_funcs_impl.hpp:272-305) of f on the grid of leaf
+// panel p, the vector conversion done for face `face`.  With `metric_twice`
+// (the reference) the metric combination D f_xi + XY/D f_eta, XY/C f_xi + C f_eta
+// is applied to the derivatives before xyzvec_from_xietavec, which applies it
+// again; without it the raw derivatives go in, which gives the surface gradient.
+static int panel_grad(const Ctx& c, const Panel& p, int face, bool metric_twice, const double* f, double* gx,
+                      double* gy, double* gz) {
+  const int deg = c.md, m = deg + 1;
+  double xo, xs, eo, es;
+  panel_frame(p, xo, xs, eo, es);
+  double fx[121], fe[121];
+  cheb_deriv(f, deg, xo - xs, xo + xs, 0, fx);
+  cheb_deriv(f, deg, eo - es, eo + es, 1, fe);
+  int err = 0;
+  for (int a = 0; a < m; a++) {
+    const double xi = std::cos(kPi * a / deg) * xs + xo;
+    for (int b = 0; b < m; b++) {
+      const double eta = std::cos(kPi * b / deg) * es + eo;
+      const int k = a * m + b;
+      const double X = std::tan(xi), Y = std::tan(eta), C = std::sqrt(1 + X * X), D = std::sqrt(1 + Y * Y);
+      const double xc = metric_twice ? D * fx[k] + X * Y / D * fe[k] : fx[k];
+      const double ec = metric_twice ? X * Y / C * fx[k] + C * fe[k] : fe[k];
+      err |= xyzvec_from_xietavec(gx[k], gy[k], gz[k], xc, ec, face, xi, eta, !metric_twice);
+    }
+  }
+  return err;
+}
+
+// laplacian / panel_laplacian (deriv_funcs.cpp:27-37, deriv_funcs_impl.hpp:338-389):
+// lap = delta (f_xx / C^2 + f_ee / D^2 + 2 X Y / (C^2 D^2) f_xe) on each leaf,
+// f_xe = d/dxi of d/deta f.  f, lap [Pl][m2].
+int orc_panel_laplacian(void* h, const double* f, double* lap) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const int deg = c.md, m = deg + 1;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.Pl; i++) {
+    double xo, xs, eo, es, fx[121], fe[121], fxx[121], fxe[121], fee[121];
+    panel_frame(c.pn[i + c.offset], xo, xs, eo, es);
+    const double* fi = f + (size_t)i * c.m2;
+    cheb_deriv(fi, deg, xo - xs, xo + xs, 0, fx);
+    cheb_deriv(fi, deg, eo - es, eo + es, 1, fe);
+    cheb_deriv(fx, deg, xo - xs, xo + xs, 0, fxx);
+    cheb_deriv(fe, deg, xo - xs, xo + xs, 0, fxe);
+    cheb_deriv(fe, deg, eo - es, eo + es, 1, fee);
+    for (int a = 0; a < m; a++) {
+      const double xi = std::cos(kPi * a / deg) * xs + xo;
+      for (int b = 0; b < m; b++) {
+        const double eta = std::cos(kPi * b / deg) * es + eo;
+        const int k = a * m + b;
+        const double X = std::tan(xi), Y = std::tan(eta), C2 = 1 + X * X, D2 = 1 + Y * Y, delta = 1 + X * X + Y * Y;
+        lap[(size_t)i * c.m2 + k] = delta * (fxx[k] / C2 + fee[k] / D2 + 2.0 * X * Y / (C2 * D2) * fxe[k]);
+      }
+    }
+  }
+  return ORC_OK;
+}
+
+// The leaf gradient of the velocity components (swe_time_step.cpp:109-111):
+// rule 0 is the reference (vectors converted with cubed_sphere_panels(i).face,
+// i the leaf-local index, the metric applied twice, the pole nodes' shortcut);
+// rule 1 converts with the leaf's own face from the raw derivatives and maps
+// the pole nodes correctly: the surface gradient.
+// grad[9][Pl][m2]: d(vel_a)/d(x_b) at 3a+b.
+int orc_velocity_gradient(void* h, const double* const* vel, int rule, double* const* grad) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
+  for (int i = 0; i < c.Pl; i++) {
+    const Panel& p = c.pn[i + c.offset];
+    const int face = rule == 0 ? c.pn[i].face : p.face;
+    const size_t b = (size_t)i * c.m2;
+    for (int a = 0; a < 3; a++)
+      err |= panel_grad(c, p, face, rule == 0, vel[a] + b, grad[3 * a] + b, grad[3 * a + 1] + b, grad[3 * a + 2] + b);
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// swe_tendency_computation (swe_time_step.cpp:76-161), per particle:
+//   vor_t = -div (vor + 2 Omega z) - 2 Omega u_z
+//   div_t = 2 Omega z vor - sum_ab G_ab G_ba - lap - |u|^2 - 2 Omega (-y u_x + x u_y) [x^2+y^2 > 1e-16]
+//   h_t   = -h div
+int orc_swe_tendency(void* h, const double* x, const double* y, const double* z, const double* const* vel,
+                     const double* vor, const double* div, const double* height, const double* lap, double omega,
+                     int rule, double* vor_t, double* div_t, double* h_t) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const size_t NP = (size_t)c.Pl * c.m2;
+  std::vector<double> g(9 * NP);
+  double* gp[9];
+  for (int k = 0; k < 9; k++) gp[k] = g.data() + k * NP;
+  int err = orc_velocity_gradient(h, vel, rule, gp);
+#pragma omp parallel for schedule(static)
+  for (size_t k = 0; k < NP; k++) {
+    const double G11 = gp[0][k], G12 = gp[1][k], G13 = gp[2][k], G21 = gp[3][k], G22 = gp[4][k], G23 = gp[5][k],
+                 G31 = gp[6][k], G32 = gp[7][k], G33 = gp[8][k];
+    const double ux = vel[0][k], uy = vel[1][k], uz = vel[2][k];
+    const double abs_vor = vor[k] + 2.0 * omega * z[k];
+    vor_t[k] = -div[k] * abs_vor - 2.0 * omega * uz;
+    double d = 2.0 * omega * z[k] * vor[k];
+    const double gc = G11 * G11 + G12 * G21 + G13 * G31 + G21 * G12 + G22 * G22 + G23 * G32 + G31 * G13 +
+                      G32 * G23 + G33 * G33;
+    d -= gc;
+    d -= lap[k];
+    d -= ux * ux + uy * uy + uz * uz;
+    if (x[k] * x[k] + y[k] * y[k] > 1e-16) d -= 2.0 * omega * (-y[k] * ux + x[k] * uy);
+    div_t[k] = d;
+    h_t[k] = -height[k] * div[k];
+  }
+  return err;
+}
+
+// cone_mountain (topography.cpp:6-45) at P(x + disp): eff = h + h0 exp(-16 r^2),
+// h0 = 2000 / 6371000, r^2 = (lon - pi/2)^2 + (lat - pi/6)^2, lon in [0, 2 pi).
+// topo 0: eff = h (apply_topography's copy, :47-55).  The displacement read at
+// leaf-local i, node j is disp[i][j] of the full [P][n] array (rule 0, the
+// reference: panel i, which is not leaf i; needs m2 == n) or the leaf's own
+// (rule 1).
+int orc_apply_topography(void* h, const double* x, const double* y, const double* z, const double* const* disp_part,
+                         const double* const* disp_proxy, const double* height, int topo, int rule, double* eff) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const size_t NP = (size_t)c.Pl * c.m2;
+  if (topo == 0) {
+    std::memcpy(eff, height, NP * sizeof(double));
+    return ORC_OK;
+  }
+  if (rule == 0 && c.m2 != c.n) return ORC_E_ARG;
+  const double h0 = 2000.0 / 6371000.0, lc = 0.5 * kPi, tc = kPi / 6.0;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.Pl; i++)
+    for (int j = 0; j < c.m2; j++) {
+      const size_t k = (size_t)i * c.m2 + j;
+      double d[3];
+      for (int f = 0; f < 3; f++) {
+        if (rule == 1 || i >= c.offset) {
+          const size_t kk = rule == 1 ? k : (size_t)(i - c.offset) * c.m2 + j;
+          d[f] = disp_part[f][kk];
+        } else {
+          d[f] = disp_proxy[f][(size_t)i * c.n + j];
+        }
+      }
+      double a = x[k] + d[0], b = y[k] + d[1], cc = z[k] + d[2];
+      project_to_sphere(a, b, cc);
+      const double lat = kPi / 2.0 - std::atan2(std::sqrt(a * a + b * b), cc);
+      double lon = std::atan2(b, a);
+      lon += 2 * kPi;
+      lon = std::fmod(lon, 2 * kPi);
+      const double r2 = (lon - lc) * (lon - lc) + (lat - tc) * (lat - tc);
+      eff[k] = height[k] + h0 * std::exp(-16.0 * r2);
+    }
+  return ORC_OK;
+}
+
+// lap_panel_interaction (laplacian_impl.hpp:11-69) on the type-dependent point
+// sets, sources at P(node + disp): for x.y > 1 - 30 eps (eps = 0.1),
+//   K = [e^{-1/eps} pi^{-3/2} 2t(-1 + 3eps + t^2)
+//        + (1 + erf(t/eps)) e^{(t^2-1)/eps} / (sqrt(eps) pi) (2eps^2 + 2t^2(t^2-1) + eps(7t^2-1))] / eps^{5/2},
+// t = x.y; phi += K q.  phi_part [Pl][m2], phi_proxy [P][n] accumulated.
+void orc_laplacian_interactions(void* h, const double* const* disp_part, const double* const* disp_proxy,
+                                const double* q_part, const double* q_proxy, double* phi_part, double* phi_proxy) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const double eps = 0.1, c1 = std::exp(-1.0 / eps) * std::pow(kPi, -1.5), c2 = 1.0 / (std::sqrt(eps) * kPi),
+               c3 = 1.0 / std::pow(eps, 2.5);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < c.P; t++) {
+    for (int e = c.tgt_start[t]; e < c.tgt_start[t + 1]; e++) {
+      int ii = c.by_target[e];
+      int source = c.ilist[3 * ii + 1], type = c.ilist[3 * ii + 2];
+      bool tpart = (type == 0 || type == 1), spart = (type == 0 || type == 2);
+      Side ts = side_of(c, t, tpart), ss = side_of(c, source, spart);
+      const Panel &pt = c.pn[t], &ps = c.pn[source];
+      double xot, xst, eot, est, xos, xss, eos, ess;
+      panel_frame(pt, xot, xst, eot, est);
+      panel_frame(ps, xos, xss, eos, ess);
+      double* o = tpart ? phi_part + (size_t)(t - c.offset) * c.m2 : phi_proxy + (size_t)t * c.n;
+      const size_t sb = spart ? (size_t)(source - c.offset) * c.m2 : (size_t)source * c.n;
+      const double* sq = (spart ? q_part : q_proxy) + sb;
+      const double* dx = (spart ? disp_part[0] : disp_proxy[0]) + sb;
+      const double* dy = (spart ? disp_part[1] : disp_proxy[1]) + sb;
+      const double* dz = (spart ? disp_part[2] : disp_proxy[2]) + sb;
+      double xt[3], y0[3];
+      for (int j = 0; j < ts.count; j++) {
+        xyz_from_xieta(ts.tab[4 * j] * xst + xot, ts.tab[4 * j + 1] * est + eot, pt.face, xt);
+        double acc = 0;
+        for (int k = 0; k < ss.count; k++) {
+          xyz_from_xieta(ss.tab[4 * k] * xss + xos, ss.tab[4 * k + 1] * ess + eos, ps.face, y0);
+          double a = y0[0] + dx[k], b = y0[1] + dy[k], cc = y0[2] + dz[k];
+          project_to_sphere(a, b, cc);
+          const double dp = xt[0] * a + xt[1] * b + xt[2] * cc;
+          if (dp > 1.0 - 30 * eps) {
+            const double dp2 = dp * dp;
+            const double h1 = c1 * (2.0 * dp * (-1.0 + 3.0 * eps + dp2));
+            const double h2 = (1.0 + std::erf(dp / eps)) * std::exp((dp2 - 1) / eps) * c2 *
+                              (2.0 * eps * eps + 2.0 * dp2 * (dp2 - 1.0) + eps * (7.0 * dp2 - 1.0));
+            acc += (h1 + h2) * c3 * sq[k];
+          }
+        }
+        o[j] += acc;
+      }
+    }
+  }
+}
+
+// swe_apply_tendencies (swe_time_step.cpp:163-192): f' = f + dt f_t for vor,
+// div, h; area' = area (1 + dt div).  area/new_area may be null (arrival values,
+// swe_time_step.cpp:404-406).
+void orc_swe_apply_tendencies(void* h, const double* const* f, const double* const* ft, const double* area,
+                              double dt, double* const* out, double* new_area) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const size_t NP = (size_t)c.Pl * c.m2;
+  for (size_t k = 0; k < NP; k++) {
+    for (int q = 0; q < 3; q++) out[q][k] = f[q][k] + dt * ft[q][k];
+    if (area && new_area) new_area[k] = area[k] * (1.0 + dt * f[1][k]);
+  }
+}
+
+// swe_height_correction (swe_time_step.cpp:558-604): at P(x + dt vel) the BLI
+// of h + dt h_t; lap -= 8 new_h / 0.1.
+int orc_swe_height_correction(void* h, const double* x, const double* y, const double* z, const double* const* vel,
+                              const double* height, const double* h_t, double dt, double* lap) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const size_t NP = (size_t)c.Pl * c.m2;
+  std::vector<double> hh(NP);
+  for (size_t k = 0; k < NP; k++) hh[k] = height[k] + dt * h_t[k];
+  const double* f1[1] = {hh.data()};
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : err)
+  for (size_t k = 0; k < NP; k++) {
+    double a = x[k] + dt * vel[0][k], b = y[k] + dt * vel[1][k], cc = z[k] + dt * vel[2][k];
+    project_to_sphere(a, b, cc);
+    double nh;
+    err |= interp_at(c, a, b, cc, 1, f1, &nh);
+    lap[k] -= 8.0 * nh / 0.1;
+  }
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// swe_displacement_upward_pass (swe_time_step.cpp:265-276): the BVE
+// displacement without the vorticity update.
+static int swe_displacement(Ctx& c, const double* x, const double* y, const double* z, const double* const* vel,
+                            double tau, double* const* dpart, double* const* dprox) {
+  std::vector<double> dummy((size_t)c.Pl * c.m2), zero((size_t)c.Pl * c.m2, 0.0);
+  return orc_bve_displacement(&c, x, y, z, vel, zero.data(), tau, 0.0, dummy.data(), dpart, dprox);
+}
+
+// Upward passes of vor and div (area-weighted), swe_vel_interactions (variant 1),
+// downward_pass_3 (swe_time_step.cpp:655-672).  vel[3][Pl][m2] assigned.
+static int swe_fmm_velocity(Ctx& c, const double* area, const double* vor, const double* div,
+                            const double* const* dpart, const double* const* dprox, double eps, double* const* vel) {
+  const size_t NP = (size_t)c.Pl * c.m2, NQ = (size_t)c.P * c.n;
+  std::vector<double> qzp(NP), qzx(NQ), qdp(NP), qdx(NQ), pp(3 * NP, 0.0), px(3 * NQ, 0.0);
+  int err = orc_upward_pass(&c, area, vor, qzp.data(), qzx.data());
+  err |= orc_upward_pass(&c, area, div, qdp.data(), qdx.data());
+  double* php[3] = {pp.data(), pp.data() + NP, pp.data() + 2 * NP};
+  double* phx[3] = {px.data(), px.data() + NQ, px.data() + 2 * NQ};
+  orc_swe_vel_interactions(&c, 1, dpart, dprox, qzp.data(), qzx.data(), qdp.data(), qdx.data(), eps, php, phx);
+  const double* cphp[3] = {php[0], php[1], php[2]};
+  err |= orc_downward_pass(&c, 3, phx, cphp, vel);
+  return err;
+}
+
+// Upward pass of area * f, laplacian_interactions, downward pass
+// (swe_time_step.cpp:695-701).  lap [Pl][m2] assigned.
+static int lap_fmm(Ctx& c, const double* area, const double* f, const double* const* dpart,
+                   const double* const* dprox, double* lap) {
+  const size_t NP = (size_t)c.Pl * c.m2, NQ = (size_t)c.P * c.n;
+  std::vector<double> qp(NP), qx(NQ), pp(NP, 0.0), px(NQ, 0.0);
+  int err = orc_upward_pass(&c, area, f, qp.data(), qx.data());
+  orc_laplacian_interactions(&c, dpart, dprox, qp.data(), qx.data(), pp.data(), px.data());
+  double* phx[1] = {px.data()};
+  const double* cpp[1] = {pp.data()};
+  double* out[1] = {lap};
+  err |= orc_downward_pass(&c, 1, phx, cpp, out);
+  return err;
+}
+
+// swe_back_rk4_step (swe_time_step.cpp:606-811).  vors, divs, height [Pl][m2]
+// and tracers [Pl][m2][T] are updated in place; vel[3] receives the stage-1
+// velocity and height_lap the stage-1 surface Laplacian.  rule 0 is the
+// reference (leaf-local face index in the velocity gradient, panel-i
+// displacement in the topography, stage-4 state advanced by dt/2); rule 1
+// uses the leaf's own face and displacement and dt at stage 4.  The stage-4
+// surface Laplacian, height correction and tendencies, and the k2..k4 part of
+// swe_compute_arrival_values, do not reach any output (the arrival values are
+// state + dt k1, :415-417) and are not computed.
+int orc_swe_back_rk4_step(void* h, const double* x, const double* y, const double* z, const double* area,
+                          double* vors, double* divs, double* height, double* tracers, int ntr, double dt,
+                          double omega, double eps, int combine, int topo, int rule, double* const* vel,
+                          double* height_lap) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  const size_t NP = (size_t)c.Pl * c.m2, NQ = (size_t)c.P * c.n;
+  std::vector<double> dpart(3 * NP, 0.0), dprox(3 * NQ, 0.0), eff(NP), vs(9 * NP), lap(NP), tend(9 * NP),
+      st(3 * NP), da(NP);
+  double* dpw[3] = {dpart.data(), dpart.data() + NP, dpart.data() + 2 * NP};
+  double* dxw[3] = {dprox.data(), dprox.data() + NQ, dprox.data() + 2 * NQ};
+  const double* dp3[3] = {dpw[0], dpw[1], dpw[2]};
+  const double* dx3[3] = {dxw[0], dxw[1], dxw[2]};
+  double* V[4][3] = {{vel[0], vel[1], vel[2]},
+                     {vs.data(), vs.data() + NP, vs.data() + 2 * NP},
+                     {vs.data() + 3 * NP, vs.data() + 4 * NP, vs.data() + 5 * NP},
+                     {vs.data() + 6 * NP, vs.data() + 7 * NP, vs.data() + 8 * NP}};
+  double* T[3][3];
+  for (int s = 0; s < 3; s++)
+    for (int q = 0; q < 3; q++) T[s][q] = tend.data() + (3 * s + q) * NP;
+  double* S[3] = {st.data(), st.data() + NP, st.data() + 2 * NP};  // displaced vor, div, h
+  const double* state[3] = {vors, divs, height};
+  // stage 1
+  int err = orc_apply_topography(h, x, y, z, dp3, dx3, height, topo, rule, eff.data());
+  err |= orc_panel_laplacian(h, eff.data(), height_lap);
+  err |= swe_fmm_velocity(c, area, vors, divs, dp3, dx3, eps, V[0]);
+  const double* cv[3] = {V[0][0], V[0][1], V[0][2]};
+  err |= orc_swe_tendency(h, x, y, z, cv, vors, divs, height, height_lap, omega, rule, T[0][0], T[0][1], T[0][2]);
+  const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
+  for (int s = 1; s <= 3; s++) {
+    const double* vprev[3] = {V[s - 1][0], V[s - 1][1], V[s - 1][2]};
+    err |= swe_displacement(c, x, y, z, vprev, taus[s - 1], dpw, dxw);
+    const double* tp[3] = {T[s - 1][0], T[s - 1][1], T[s - 1][2]};
+    const double tau_apply = (s == 3 && rule == 0) ? 0.5 * dt : taus[s - 1];
+    orc_swe_apply_tendencies(h, state, tp, area, tau_apply, S, da.data());
+    err |= swe_fmm_velocity(c, da.data(), S[0], S[1], dp3, dx3, eps, V[s]);
+    if (s == 3) break;
+    err |= orc_apply_topography(h, x, y, z, dp3, dx3, S[2], topo, rule, eff.data());
+    err |= lap_fmm(c, da.data(), eff.data(), dp3, dx3, lap.data());
+    err |= orc_swe_height_correction(h, x, y, z, vprev, height, T[s - 1][2], taus[s - 1], lap.data());
+    const double* cvs[3] = {V[s][0], V[s][1], V[s][2]};
+    err |= orc_swe_tendency(h, x, y, z, cvs, vors, divs, height, lap.data(), omega, rule, T[s][0], T[s][1], T[s][2]);
+  }
+  // arrival values, departure points, remap
+  std::vector<double> arr(3 * NP), dep(3 * NP), nf(3 * NP), ntr_buf(NP * (ntr > 0 ? ntr : 1));
+  double* A[3] = {arr.data(), arr.data() + NP, arr.data() + 2 * NP};
+  const double* t1[3] = {T[0][0], T[0][1], T[0][2]};
+  orc_swe_apply_tendencies(h, state, t1, nullptr, dt, A, nullptr);
+  const double* all[12] = {V[0][0], V[0][1], V[0][2], V[1][0], V[1][1], V[1][2],
+                           V[2][0], V[2][1], V[2][2], V[3][0], V[3][1], V[3][2]};
+  err |= orc_bve_find_departure_points(h, x, y, z, all, dt, combine, dep.data(), dep.data() + NP, dep.data() + 2 * NP);
+  const double* ca[3] = {A[0], A[1], A[2]};
+  double* N3[3] = {nf.data(), nf.data() + NP, nf.data() + 2 * NP};
+  err |= orc_swe_departure_to_target(h, dep.data(), dep.data() + NP, dep.data() + 2 * NP, ca, tracers, ntr, N3,
+                                     ntr_buf.data());
+  std::memcpy(vors, N3[0], NP * sizeof(double));
+  std::memcpy(divs, N3[1], NP * sizeof(double));
+  std::memcpy(height, N3[2], NP * sizeof(double));
+  if (ntr > 0) std::memcpy(tracers, ntr_buf.data(), NP * ntr * sizeof(double));
+  if (err & ORC_E_ARG) return ORC_E_ARG;
+  return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
 void orc_set_threads(int n) { omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 

@@ -33,6 +33,8 @@ class Params(ctypes.Structure):
         ("kernel_eps", ctypes.c_double),
         ("omega", ctypes.c_double),
         ("delta_t", ctypes.c_double),
+        ("topography", ctypes.c_int),
+        ("swe_rule", ctypes.c_int),
     ]
 
 
@@ -112,6 +114,12 @@ SIGNATURES = {
     "sbslpm_direct_sum_inv_lap_targets": (_I, [_V, _I64, _V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
     "sbslpm_direct_sum_bve_vel": (_I, [_V, _I64] + [_V] * 11 + [_I64, _I64]),
     "sbslpm_direct_sum_bve_vel_targets": (_I, [_V, _I64, _V, _V, _V, _I64] + [_V] * 8),
+    "sbslpm_laplacian": (_I, [_V, _V, _V]),
+    "sbslpm_laplacian_interactions": (_I, [_V] * 11),
+    "sbslpm_apply_topography": (_I, [_V] * 12),
+    "sbslpm_swe_tendency": (_I, [_V] * 14),
+    "sbslpm_swe_height_correction": (_I, [_V] * 9 + [_D, _V]),
+    "sbslpm_swe_back_rk4_step": (_I, [_V] * 13 + [_D]),
     "sbslpm_bve_displacement": (_I, [_V] + [_V] * 7 + [_D] + [_V] * 7),
     "sbslpm_bve_find_departure_points": (_I, [_V, _V, _V, _V, _V, _D, _V, _V, _V]),
     "sbslpm_bve_departure_to_target": (_I, [_V, _V, _V, _V, _V, _V, _V, _I, _V, _V]),

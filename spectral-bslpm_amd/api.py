@@ -17,6 +17,8 @@ from ._lib import Params, SbslpmError, load
 
 WEIGHTS = {"reference": 0, "clenshaw_curtis": 1, "equal_area": 2}
 COMBINE = {"reference": 0, "textbook": 1}
+TOPOGRAPHY = {"none": 0, "cone": 1}
+SWE_RULE = {"reference": 0, "leaf": 1}
 PRECISION = {"fp64": 0, "mixed": 1, "mixed_valu": 2}
 
 
@@ -35,6 +37,8 @@ class RunConfig:
     precision: str = "fp64"          # fp64 | mixed (matrix cores) | mixed_valu
     tracer_count: int = 0
     use_graph: bool = False
+    topography: str = "none"         # RunConfig::topo_type: none | cone
+    swe_rule: str = "reference"      # reference | leaf (shallow-water step indexing, see sbslpm.h)
 
     def to_params(self):
         p = Params()
@@ -50,6 +54,8 @@ class RunConfig:
         p.kernel_eps = self.kernel_eps
         p.omega = self.omega
         p.delta_t = self.delta_t
+        p.topography = TOPOGRAPHY[self.topography]
+        p.swe_rule = SWE_RULE[self.swe_rule]
         return p
 
 
@@ -230,7 +236,7 @@ class Context:
         self._check(self._lib.sbslpm_ctx_set_precision(self.h, PRECISION[precision]), "set_precision")
         self.cfg.precision = precision
 
-    KERNELS = {"bve_fp64": 0, "bve_mixed": 1, "log_fp64": 2, "log_mixed": 3}
+    KERNELS = {"bve_fp64": 0, "bve_mixed": 1, "log_fp64": 2, "log_mixed": 3, "swe_fp64": 4, "lap_fp64": 5}
 
     def enable_timing(self, on=True):
         self._check(self._lib.sbslpm_ctx_enable_timing(self.h, 1 if on else 0), "enable_timing")
@@ -552,6 +558,61 @@ class Context:
                                                       *[_ptr(v) for v in list(disp_part) + list(disp_proxy)]),
                     "swe_displacement")
         return disp_part, disp_proxy
+
+    # ---- shallow-water step (swe_time_step.cpp) ----
+    def laplacian(self, f, lap=None):
+        """laplacian (deriv_funcs.cpp:27-37): spectral surface Laplacian on each leaf."""
+        lap = self.zeros_part() if lap is None else lap
+        self._check(self._lib.sbslpm_laplacian(self.h, _ptr(f), _ptr(lap)), "laplacian")
+        return lap
+
+    def laplacian_interactions(self, q_part, q_proxy, disp_part=None, disp_proxy=None, phi_part=None,
+                               phi_proxy=None):
+        """laplacian_interactions (laplacian.cpp:6-35); accumulates."""
+        phi_part = self.zeros_part() if phi_part is None else phi_part
+        phi_proxy = self.zeros_proxy() if phi_proxy is None else phi_proxy
+        dp = [None] * 3 if disp_part is None else list(disp_part)
+        dx = [None] * 3 if disp_proxy is None else list(disp_proxy)
+        self._check(self._lib.sbslpm_laplacian_interactions(self.h, *[_ptr(d) for d in dp + dx], _ptr(q_part),
+                                                            _ptr(q_proxy), _ptr(phi_part), _ptr(phi_proxy)),
+                    "laplacian_interactions")
+        return phi_part, phi_proxy
+
+    def apply_topography(self, x, y, z, height, disp_part=None, disp_proxy=None, eff=None):
+        """apply_topography (topography.cpp:47-56) with the context's topography and swe_rule."""
+        eff = self.zeros_part() if eff is None else eff
+        dp = [None] * 3 if disp_part is None else list(disp_part)
+        dx = [None] * 3 if disp_proxy is None else list(disp_proxy)
+        self._check(self._lib.sbslpm_apply_topography(self.h, _ptr(x), _ptr(y), _ptr(z), *[_ptr(d) for d in dp + dx],
+                                                      _ptr(height), _ptr(eff)), "apply_topography")
+        return eff
+
+    def swe_tendency(self, x, y, z, vel, vor, div, height, height_lap, out=None):
+        """swe_tendency_computation (swe_time_step.cpp:76-161): (vor_t, div_t, h_t)."""
+        out = self.zeros_part(3) if out is None else out
+        self._check(self._lib.sbslpm_swe_tendency(self.h, _ptr(x), _ptr(y), _ptr(z), *[_ptr(v) for v in vel],
+                                                  _ptr(vor), _ptr(div), _ptr(height), _ptr(height_lap),
+                                                  *[_ptr(o) for o in out]), "swe_tendency")
+        return out
+
+    def swe_height_correction(self, x, y, z, vel, height, h_t, dt, height_lap):
+        """swe_height_correction (swe_time_step.cpp:558-604); height_lap updated in place."""
+        self._check(self._lib.sbslpm_swe_height_correction(self.h, _ptr(x), _ptr(y), _ptr(z), *[_ptr(v) for v in vel],
+                                                           _ptr(height), _ptr(h_t), float(dt), _ptr(height_lap)),
+                    "swe_height_correction")
+        return height_lap
+
+    def swe_back_rk4_step(self, x, y, z, area, vors, divs, height, vel=None, tracers=None, height_lap=None,
+                          time=0.0):
+        """swe_back_rk4_step (swe_time_step.cpp:606-811): vors, divs, height (and tracers) updated
+        in place; returns (stage-1 velocity, stage-1 surface Laplacian)."""
+        vel = self.zeros_part(3) if vel is None else vel
+        height_lap = self.zeros_part() if height_lap is None else height_lap
+        self._check(self._lib.sbslpm_swe_back_rk4_step(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vors),
+                                                       _ptr(divs), _ptr(height), *[_ptr(v) for v in vel],
+                                                       _ptr(tracers), _ptr(height_lap), float(time)),
+                    "swe_back_rk4_step")
+        return vel, height_lap
 
     def swe_find_departure_points(self, x, y, z, vels, dt, dep=None):
         """swe_find_departure_points, swe_time_step.cpp:422-556."""

@@ -101,6 +101,13 @@ struct sbslpm_ctx {
   SrcRecSwe* recswe = nullptr;
   double* qdpart = nullptr;
   double* qdproxy = nullptr;
+  // shallow-water step workspace (stage displacements, velocities, tendencies,
+  // displaced state, surface Laplacian, arrival values), allocated on the first step
+  char* sws_arena = nullptr;
+  struct SweStep {
+    double *dpart[3], *dprox[3], *eff, *lap, *v[3][3], *tend[3][3], *st[3], *da, *arr[3], *dep[3], *ntr;
+    double *qp, *qx, *pp, *px;
+  } sws{};
 
   // Kernel timing (sbslpm_ctx_enable_timing).
   bool timing = false;
@@ -110,8 +117,8 @@ struct sbslpm_ctx {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
-  double kernel_ms[4] = {};
-  int64_t kernel_launches[4] = {};
+  double kernel_ms[6] = {};
+  int64_t kernel_launches[6] = {};
 
   // Launch shape of the Biot-Savart pair kernels (sbslpm_autotune).
   PairShape shape{};
@@ -372,6 +379,7 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
   }
   if (c->scratch) hipFree(c->scratch);
   if (c->swe_arena) hipFree(c->swe_arena);
+  if (c->sws_arena) hipFree(c->sws_arena);
   if (c->adj_arena) hipFree(c->adj_arena);
   if (c->ens_graph) hipGraphExecDestroy(c->ens_graph);
   if (c->ens_arena) hipFree(c->ens_arena);
@@ -423,7 +431,7 @@ int sbslpm_ctx_enable_timing(sbslpm_ctx* c, int enable) {
 }
 
 int sbslpm_kernel_time(sbslpm_ctx* c, int kernel, double* ms, int64_t* launches, int reset) {
-  if (!c || kernel < 0 || kernel > 3) return SBSLPM_E_ARG;
+  if (!c || kernel < 0 || kernel > 5) return SBSLPM_E_ARG;
   for (auto& p : c->pending) {
     HIP_TRY(c, hipEventSynchronize(p.b));
     float t = 0;
@@ -587,8 +595,11 @@ static int swe_interactions(sbslpm_ctx* c, int variant, const double* dpx, const
   const double* dx[3] = {dxx, dxy, dxz};
   launch_source_records_swe(c->dt, variant, qz_part, qz_proxy, qd_part, qd_proxy, dpx ? dp : nullptr,
                             dxx ? dx : nullptr, c->recswe, c->stream);
-  launch_swe_interact64(c->dt, variant, c->tiles, c->ntiles, c->sets, c->recswe, c->prm.kernel_eps, op, ox, c->stream,
-                        assign);
+  {
+    KernelTimer tm(c, SBSLPM_KERNEL_SWE_FP64);
+    launch_swe_interact64(c->dt, variant, c->tiles, c->ntiles, c->sets, c->recswe, c->prm.kernel_eps, op, ox,
+                          c->stream, assign);
+  }
   return launched(c);
 }
 
@@ -875,6 +886,207 @@ int sbslpm_swe_departure_to_target(sbslpm_ctx* c, const double* dep_x, const dou
   const double* f[3] = {vor, div, h};
   double* nf[3] = {new_vor, new_div, new_h};
   launch_swe_departure_to_target(c->dt, dep_x, dep_y, dep_z, f, tracers, ntr, nf, new_tracers, c->stream);
+  return launched(c);
+}
+
+// ---- shallow-water step (swe_time_step.cpp) -----------------------------------
+
+static int ensure_swe_step(sbslpm_ctx* c) {
+  if (c->sws_arena) return SBSLPM_OK;
+  const Tree& t = c->tree;
+  const size_t NP = (size_t)t.npart, NQ = (size_t)t.P * t.n, T = (size_t)std::max(c->prm.tracer_count, 0);
+  Arena A;
+  size_t o_dp[3], o_dx[3], o_v[3][3], o_t[3][3], o_st[3], o_arr[3], o_dep[3];
+  for (int i = 0; i < 3; ++i) o_dp[i] = A.take<double>(NP);
+  for (int i = 0; i < 3; ++i) o_dx[i] = A.take<double>(NQ);
+  const size_t o_eff = A.take<double>(NP), o_lap = A.take<double>(NP);
+  for (int s = 0; s < 3; ++s)
+    for (int i = 0; i < 3; ++i) o_v[s][i] = A.take<double>(NP);
+  for (int s = 0; s < 3; ++s)
+    for (int i = 0; i < 3; ++i) o_t[s][i] = A.take<double>(NP);
+  for (int i = 0; i < 3; ++i) o_st[i] = A.take<double>(NP);
+  const size_t o_da = A.take<double>(NP);
+  for (int i = 0; i < 3; ++i) o_arr[i] = A.take<double>(NP);
+  for (int i = 0; i < 3; ++i) o_dep[i] = A.take<double>(NP);
+  const size_t o_ntr = A.take<double>(NP * T), o_qp = A.take<double>(NP), o_qx = A.take<double>(NQ),
+               o_pp = A.take<double>(NP), o_px = A.take<double>(NQ), total = A.take<char>(256);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (hipMalloc(&c->sws_arena, total) != hipSuccess) {
+    c->sws_arena = nullptr;
+    return fail(c, SBSLPM_E_NOMEM, "shallow-water step workspace: out of device memory");
+  }
+  char* b = c->sws_arena;
+  auto& w = c->sws;
+  for (int i = 0; i < 3; ++i) {
+    w.dpart[i] = (double*)(b + o_dp[i]);
+    w.dprox[i] = (double*)(b + o_dx[i]);
+    w.st[i] = (double*)(b + o_st[i]);
+    w.arr[i] = (double*)(b + o_arr[i]);
+    w.dep[i] = (double*)(b + o_dep[i]);
+    for (int s = 0; s < 3; ++s) {
+      w.v[s][i] = (double*)(b + o_v[s][i]);
+      w.tend[s][i] = (double*)(b + o_t[s][i]);
+    }
+  }
+  w.eff = (double*)(b + o_eff);
+  w.lap = (double*)(b + o_lap);
+  w.da = (double*)(b + o_da);
+  w.ntr = (double*)(b + o_ntr);
+  w.qp = (double*)(b + o_qp);
+  w.qx = (double*)(b + o_qx);
+  w.pp = (double*)(b + o_pp);
+  w.px = (double*)(b + o_px);
+  return SBSLPM_OK;
+}
+
+// Upward pass of area * f, the regularised-Laplacian pair sums over displaced
+// sources, downward pass (swe_time_step.cpp:695-701): lap [Pl][m2] assigned.
+static int laplacian_fmm(sbslpm_ctx* c, const double* area, const double* f, const double* const* dpart,
+                         const double* const* dprox, double* lap) {
+  const Tree& t = c->tree;
+  auto& w = c->sws;
+  launch_upward_fused(c->dt, area, f, w.qp, w.qx, c->stream);
+  launch_source_records64(c->dt, w.qp, w.qx, dpart, dprox, true, c->rec64, c->stream);
+  double* px1[1] = {w.px};
+  launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 1, px1, c->stream);
+  {
+    KernelTimer tm(c, SBSLPM_KERNEL_LAP_FP64);
+    launch_laplacian_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, w.pp, w.px, c->stream, true);
+  }
+  const double* pp1[1] = {w.pp};
+  double* out[1] = {lap};
+  launch_downward_pass(c->dt, 1, px1, pp1, out, c->stream, true);
+  return launched(c);
+}
+
+static bool swe_rule_ok(sbslpm_ctx* c) {
+  return !(c->prm.topography && c->prm.swe_rule == SBSLPM_SWE_RULE_REFERENCE && c->tree.m2 != c->tree.n);
+}
+
+int sbslpm_laplacian(sbslpm_ctx* c, const double* f, double* lap) {
+  if (!c || !f || !lap) return fail(c, SBSLPM_E_ARG, "laplacian: null pointer");
+  launch_panel_laplacian(c->dt, f, lap, c->stream);
+  return launched(c);
+}
+
+int sbslpm_laplacian_interactions(sbslpm_ctx* c, const double* dpx, const double* dpy, const double* dpz,
+                                  const double* dxx, const double* dxy, const double* dxz, const double* q_part,
+                                  const double* q_proxy, double* phi_part, double* phi_proxy) {
+  if (!c || !q_part || !q_proxy || !phi_part || !phi_proxy)
+    return fail(c, SBSLPM_E_ARG, "laplacian_interactions: null pointer");
+  if ((dpx || dpy || dpz || dxx || dxy || dxz) && !(dpx && dpy && dpz && dxx && dxy && dxz))
+    return fail(c, SBSLPM_E_ARG, "laplacian_interactions: partial displacement");
+  const double* dp[3] = {dpx, dpy, dpz};
+  const double* dx[3] = {dxx, dxy, dxz};
+  launch_source_records64(c->dt, q_part, q_proxy, dpx ? dp : nullptr, dxx ? dx : nullptr, true, c->rec64, c->stream);
+  launch_laplacian_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, phi_part, phi_proxy, c->stream, false);
+  return launched(c);
+}
+
+int sbslpm_apply_topography(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* dpx,
+                            const double* dpy, const double* dpz, const double* dxx, const double* dxy,
+                            const double* dxz, const double* height, double* eff) {
+  if (!c || !x || !y || !z || !height || !eff) return fail(c, SBSLPM_E_ARG, "apply_topography: null pointer");
+  if (!swe_rule_ok(c))
+    return fail(c, SBSLPM_E_ARG, "apply_topography: the reference rule needs leaf_degree == interp_degree");
+  if (!c->prm.topography) {
+    launch_copy(eff, height, c->tree.npart, c->stream);
+    return launched(c);
+  }
+  const double* dp[3] = {dpx, dpy, dpz};
+  const double* dx[3] = {dxx, dxy, dxz};
+  launch_topography(c->dt, x, y, z, dp, dx, height, c->prm.swe_rule, eff, c->stream);
+  return launched(c);
+}
+
+int sbslpm_swe_tendency(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* vx,
+                        const double* vy, const double* vz, const double* vor, const double* div, const double* h,
+                        const double* lap, double* vor_t, double* div_t, double* h_t) {
+  if (!c || !x || !y || !z || !vx || !vy || !vz || !vor || !div || !h || !lap || !vor_t || !div_t || !h_t)
+    return fail(c, SBSLPM_E_ARG, "swe_tendency: null pointer");
+  const double* v[3] = {vx, vy, vz};
+  double* o[3] = {vor_t, div_t, h_t};
+  launch_swe_tendency(c->dt, x, y, z, v, vor, div, h, lap, c->prm.omega, c->prm.swe_rule, o, c->stream);
+  return launched(c);
+}
+
+int sbslpm_swe_height_correction(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* vx,
+                                 const double* vy, const double* vz, const double* h, const double* h_t, double dt,
+                                 double* lap) {
+  if (!c || !x || !y || !z || !vx || !vy || !vz || !h || !h_t || !lap)
+    return fail(c, SBSLPM_E_ARG, "swe_height_correction: null pointer");
+  const double* v[3] = {vx, vy, vz};
+  launch_height_correction(c->dt, x, y, z, v, h, h_t, dt, lap, c->stream);
+  return launched(c);
+}
+
+int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
+                             double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
+                             double* tracers, double* height_lap, double time) {
+  (void)time;
+  if (!c || !x || !y || !z || !area || !vors || !divs || !height || !vx || !vy || !vz || !height_lap)
+    return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: null pointer");
+  const int ntr = c->prm.tracer_count;
+  if (ntr > 0 && !tracers) return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: tracers required (tracer_count > 0)");
+  if (!swe_rule_ok(c))
+    return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: the reference topography rule needs leaf_degree == interp_degree");
+  int rc = ensure_swe_step(c);
+  if (rc) return rc;
+  const Tree& t = c->tree;
+  auto& w = c->sws;
+  const int rule = c->prm.swe_rule, topo = c->prm.topography;
+  const double dt = c->prm.delta_t;
+  const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
+  const double* state[3] = {vors, divs, height};
+  double* vel1[3] = {vx, vy, vz};
+  const double* cdp[3] = {w.dpart[0], w.dpart[1], w.dpart[2]};
+  const double* cdx[3] = {w.dprox[0], w.dprox[1], w.dprox[2]};
+  // stage 1: undisplaced; the spectral Laplacian of the effective height
+  const double* eff = height;
+  if (topo) {
+    launch_topography(c->dt, x, y, z, nullptr, nullptr, height, rule, w.eff, c->stream);
+    eff = w.eff;
+  }
+  launch_panel_laplacian(c->dt, eff, height_lap, c->stream);
+  rc = sbslpm_swe_velocity(c, 1, area, vors, divs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vx, vy, vz);
+  if (rc) return rc;
+  launch_swe_tendency(c->dt, x, y, z, vel1, vors, divs, height, height_lap, c->prm.omega, rule, w.tend[0],
+                      c->stream);
+  // stages 2..4
+  for (int s = 1; s <= 3; ++s) {
+    double* const* vprev = s == 1 ? vel1 : w.v[s - 2];
+    const double* cvprev[3] = {vprev[0], vprev[1], vprev[2]};
+    rc = sbslpm_swe_displacement(c, vprev[0], vprev[1], vprev[2], taus[s - 1], w.dpart[0], w.dpart[1], w.dpart[2],
+                                 w.dprox[0], w.dprox[1], w.dprox[2]);
+    if (rc) return rc;
+    const double* tp[3] = {w.tend[s - 1][0], w.tend[s - 1][1], w.tend[s - 1][2]};
+    const double tau_apply = (s == 3 && rule == SBSLPM_SWE_RULE_REFERENCE) ? 0.5 * dt : taus[s - 1];
+    launch_apply_tendencies(t.npart, state, tp, area, tau_apply, w.st, w.da, c->stream);
+    rc = sbslpm_swe_velocity(c, 1, w.da, w.st[0], w.st[1], w.dpart[0], w.dpart[1], w.dpart[2], w.dprox[0], w.dprox[1],
+                             w.dprox[2], w.v[s - 1][0], w.v[s - 1][1], w.v[s - 1][2]);
+    if (rc) return rc;
+    if (s == 3) break;
+    const double* effs = w.st[2];
+    if (topo) {
+      launch_topography(c->dt, x, y, z, cdp, cdx, w.st[2], rule, w.eff, c->stream);
+      effs = w.eff;
+    }
+    rc = laplacian_fmm(c, w.da, effs, cdp, cdx, w.lap);
+    if (rc) return rc;
+    launch_height_correction(c->dt, x, y, z, cvprev, height, w.tend[s - 1][2], taus[s - 1], w.lap, c->stream);
+    const double* cv[3] = {w.v[s - 1][0], w.v[s - 1][1], w.v[s - 1][2]};
+    launch_swe_tendency(c->dt, x, y, z, cv, vors, divs, height, w.lap, c->prm.omega, rule, w.tend[s], c->stream);
+  }
+  // arrival values (state + dt k1), departure points, remap
+  const double* t1[3] = {w.tend[0][0], w.tend[0][1], w.tend[0][2]};
+  launch_apply_tendencies(t.npart, state, t1, nullptr, dt, w.arr, nullptr, c->stream);
+  const double* vels[12] = {vx, vy, vz, w.v[0][0], w.v[0][1], w.v[0][2],
+                            w.v[1][0], w.v[1][1], w.v[1][2], w.v[2][0], w.v[2][1], w.v[2][2]};
+  launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, w.dep[0], w.dep[1], w.dep[2], c->stream);
+  const double* arr[3] = {w.arr[0], w.arr[1], w.arr[2]};
+  double* out[3] = {vors, divs, height};
+  launch_swe_departure_to_target(c->dt, w.dep[0], w.dep[1], w.dep[2], arr, tracers, ntr, out, w.ntr, c->stream);
+  if (ntr > 0) launch_copy(tracers, w.ntr, t.npart * ntr, c->stream);
   return launched(c);
 }
 

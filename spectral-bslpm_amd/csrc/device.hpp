@@ -272,6 +272,23 @@ void launch_swe_departure_to_target(const DevTree& t, const double* dx, const do
                                     double* const* new_fields, double* new_tracers, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
 
+// ---- shallow-water step (swe_step.hip, fmm_interact.hip)
+void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                                 const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
+                                 bool assign = false);
+void launch_panel_laplacian(const DevTree& t, const double* f, double* lap, hipStream_t s);
+void launch_swe_tendency(const DevTree& t, const double* x, const double* y, const double* z,
+                         const double* const* vel, const double* vor, const double* div, const double* h,
+                         const double* lap, double omega, int rule, double* const* out, hipStream_t s);
+void launch_topography(const DevTree& t, const double* x, const double* y, const double* z,
+                       const double* const* disp_part, const double* const* disp_proxy, const double* h, int rule,
+                       double* eff, hipStream_t s);
+void launch_apply_tendencies(int64_t n, const double* const* f, const double* const* ft, const double* area,
+                             double tau, double* const* out, double* new_area, hipStream_t s);
+void launch_height_correction(const DevTree& t, const double* x, const double* y, const double* z,
+                              const double* const* vel, const double* h, const double* ht, double tau, double* lap,
+                              hipStream_t s);
+
 // ---- direct sums (direct.hip)
 int64_t direct_partials_size(int64_t nt, int64_t ns, int ncomp);
 void launch_direct_inv_lap(int64_t nt, const double* tx, const double* ty, const double* tz, int64_t ns,

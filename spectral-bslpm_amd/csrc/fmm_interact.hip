@@ -16,6 +16,8 @@
 // Workgroups of four waves; the workgroup id is remapped so that consecutive
 // tiles (spatially adjacent target panels sharing sources) land on the same
 // XCD and its L2.
+#include <cmath>
+
 #include "device.hpp"
 
 namespace sbslpm {
@@ -175,6 +177,45 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
       const double dp = x * s.x + y * s.y + z * s.z;
       const double l = log(fmax(1.0 - dp, 1e-300));
       acc += (dp < 1.0 - 1e-15) ? l * s.q : 0.0;
+    }
+  }
+  if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
+}
+
+// Regularised-Laplacian pair kernel (lap_panel_interaction,
+// laplacian_impl.hpp:46-66, eps = 0.1): for t = x.y > 1 - 30 eps,
+//   K(t) = [c1 2t(-1 + 3 eps + t^2)
+//           + (1 + erf(t/eps)) e^{(t^2-1)/eps} c2 (2 eps^2 + 2t^2(t^2-1) + eps(7t^2-1))] c3,
+// c1 = e^{-1/eps} pi^{-3/2}, c2 = 1/(sqrt(eps) pi), c3 = eps^{-5/2}; phi += K q.
+// Sources are displaced and projected; the records hold q' = -q/(4 pi), so c3
+// arrives multiplied by -4 pi.
+__global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                               const SourceSet* __restrict__ sets,
+                                               const SrcRec64* __restrict__ rec, double eps, double c1, double c2,
+                                               double c3, double* __restrict__ part, double* __restrict__ proxy,
+                                               int assign) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const double cut = 1.0 - 30 * eps, inv_eps = 1.0 / eps;
+  double acc = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const SrcRec64* __restrict__ r = rec + S.first;
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRec64 s = r[k];
+      const double dp = x * s.x + y * s.y + z * s.z;
+      if (dp > cut) {
+        const double dp2 = dp * dp;
+        const double h1 = c1 * (2.0 * dp * (-1.0 + 3.0 * eps + dp2));
+        const double h2 = (1.0 + erf(dp * inv_eps)) * exp((dp2 - 1) * inv_eps) * c2 *
+                          (2.0 * eps * eps + 2.0 * dp2 * (dp2 - 1.0) + eps * (7.0 * dp2 - 1.0));
+        acc = fma((h1 + h2) * c3, s.q, acc);
+      }
     }
   }
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
@@ -491,6 +532,17 @@ void launch_poisson_interact64(const DevTree& t, const TargetTile* tiles, int nt
   if (ntiles == 0) return;
   hipLaunchKernelGGL(k_log64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
                      ntiles, sets, rec, out_part, out_proxy, assign ? 1 : 0);
+}
+
+void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                                 const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
+                                 bool assign) {
+  if (ntiles == 0) return;
+  const double eps = 0.1, pi = 3.14159265358979323846;
+  const double c1 = std::exp(-1.0 / eps) * std::pow(pi, -1.5), c2 = 1.0 / (std::sqrt(eps) * pi);
+  const double c3 = -4.0 * pi / std::pow(eps, 2.5);
+  hipLaunchKernelGGL(k_lap64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
+                     ntiles, sets, rec, eps, c1, c2, c3, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
