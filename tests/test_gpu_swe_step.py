@@ -143,7 +143,26 @@ def test_step(case, rule, topo):
                 vel=W.vec_rel_l2([to_np(v) for v in vel], evel), lap=W.rel_l2(to_np(lap), elap),
                 tracers=W.rel_l2(to_np(gtr), etr))
     print(f"  step rule {rule} topo {topo}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    assert max(errs.values()) < 1e-9
+    assert errs["vel"] < 1e-12 and errs["lap"] < 1e-12
+    # The remap and the height correction locate moved points in leaves; particles whose
+    # points land within rounding of a leaf edge may be interpolated from either neighbour,
+    # so the step amplifies FP64 rounding there.  The bar is the oracle's own spread under
+    # 1e-13 relative perturbations of the input state (three draws), times 10.  With the
+    # reference's cone the stage-2..4 divergence is dominated by the Laplacian of the
+    # mountain, which the GPU's and the host's exp/atan2 round differently at grid scale, so
+    # the height is perturbed too in that case.
+    spread = dict(vor=0.0, div=0.0, h=0.0, tracers=0.0)
+    for _ in range(3):
+        pv = case["vor"] * (1 + 1e-13 * rng.normal(size=x.shape))
+        pd = case["div"] * (1 + 1e-13 * rng.normal(size=x.shape))
+        ph = case["h"] * (1 + 1e-13 * rng.normal(size=x.shape)) if topo == "cone" else case["h"]
+        r = o.swe_back_rk4_step(x.ravel(), y.ravel(), z.ravel(), a, pv, pd, ph, DT, OMEGA, EPS, 0,
+                                1 if topo == "cone" else 0, 0 if rule == "reference" else 1, tr)
+        for k, name in ((0, "vor"), (1, "div"), (2, "h"), (5, "tracers")):
+            spread[name] = max(spread[name], W.rel_l2(r[k], (ev, ed, eh, None, None, etr)[k]))
+    print("    oracle spread: " + ", ".join(f"{k} {v:.2e}" for k, v in spread.items()))
+    for name in spread:
+        assert errs[name] <= 10 * spread[name] + 1e-10, name
 
 
 def test_step_at_rest(case):
