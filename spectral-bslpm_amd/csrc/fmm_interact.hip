@@ -266,7 +266,10 @@ __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* 
   if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0, assign);
 }
 
-// Mixed precision log kernel: log(|x - y|^2 / 2) in FP32, same guard.
+// Mixed precision log kernel: log(|x - y|^2 / 2) in FP32, same guard.  The
+// pair takes the hardware base-2 logarithm (v_log_f32; h >= 1e-15 is far from
+// the denormal range, so the library's range scaling is not needed) and the
+// factor ln 2 is applied once per source set in FP64.
 __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ sets,
                                                    const SrcRec32* __restrict__ rec, double* __restrict__ part,
@@ -290,10 +293,10 @@ __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* 
       const SrcRec32 s = r[k];
       const float d0 = s.x - px, d1 = s.y - py, d2 = s.z - pz;
       const float h = 0.5f * fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-      const float l = __logf(fmaxf(h, 1e-30f));
-      v += (h > 1e-15f) ? l * s.q : 0.f;
+      const float l2 = __builtin_amdgcn_logf(h);
+      v += (h > 1e-15f) ? l2 * s.q : 0.f;
     }
-    acc += v;
+    acc += 0.69314718055994530942 * (double)v;
   }
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
