@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "fastlog.hpp"
 
 namespace sbslpm {
 namespace {
@@ -154,8 +155,13 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
   if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0, assign);
 }
 
+struct HwRcp64 {
+  __device__ double operator()(double d) const { return __builtin_amdgcn_rcp(d); }
+};
+
 // FP64 log pair kernel: psi += -1/(4 pi) log(1 - x.y) q, pairs with
-// x.y >= 1 - 1e-15 skipped (inverse_poisson_impl.hpp:204-207).
+// x.y >= 1 - 1e-15 skipped (inverse_poisson_impl.hpp:204-207).  The logarithm
+// is log_pos (within 2 ulp of the correctly rounded one).
 __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
                                                const SrcRec64* __restrict__ rec, double* __restrict__ part,
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
     for (int k = 0; k < S.count; ++k) {
       const SrcRec64 s = r[k];
       const double dp = x * s.x + y * s.y + z * s.z;
-      const double l = log(fmax(1.0 - dp, 1e-300));
+      const double l = log_pos(fmax(1.0 - dp, 1e-300), HwRcp64{});
       acc += (dp < 1.0 - 1e-15) ? l * s.q : 0.0;
     }
   }
