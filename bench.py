@@ -620,6 +620,47 @@ def bench_swe(args, torch, sb):
     ctx.enable_timing(False)
     log("roofline", roof)
 
+    # the same step on a mixed-precision context (FP32 shallow-water and Laplacian pairs)
+    with torch.cuda.stream(stream):
+        ctxm = sb.Context(sb.RunConfig(precision="mixed", **kw))
+    stm = [v.clone() for v in state0]
+    velm, lapm = ctxm.zeros_part(3), ctxm.zeros_part()
+
+    def run_m(n):
+        with torch.cuda.stream(stream):
+            for _ in range(n):
+                for v, v0 in zip(stm, state0):
+                    v.copy_(v0)
+                ctxm.swe_back_rk4_step(gx, gy, gz, ga, *stm, vel=velm, height_lap=lapm)
+
+    run_m(max(args.warmup, 1))
+    torch.cuda.synchronize()
+    f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        f0.record(stream)
+    run_m(args.steps)
+    with torch.cuda.stream(stream):
+        f1.record(stream)
+    torch.cuda.synchronize()
+    ctxm.synchronize()
+    secm = f0.elapsed_time(f1) / 1e3 / args.steps
+    log(f"swe step mixed {1e3 * secm:.3f} ms")
+    ctxm.enable_timing(True)
+    run_m(2)
+    torch.cuda.synchronize()
+    for name, flop in (("swe_mixed", FLOP_SWE), ("lap_mixed", FLOP_LAP)):
+        ms, nl = ctxm.kernel_time(name)
+        avg = ms / nl / 1e3
+        achieved = pairs * flop / avg / 1e12
+        roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32, unit="TFLOP/s",
+                          frac=round(achieved / PEAK_FP32, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
+                          launches_per_step=nl // 2, pairs_per_launch=pairs, flop_per_pair=flop)
+    ctxm.enable_timing(False)
+    # stage-1 velocity and Laplacian of one step from the same state, mixed vs FP64
+    vel_err = W.vec_rel_l2([v.cpu().numpy() for v in velm], [v.cpu().numpy() for v in vel])
+    lap_err = W.rel_l2(lapm.cpu().numpy(), lap.cpu().numpy())
+    log(f"mixed vs fp64: stage-1 velocity {vel_err:.2e}, Laplacian {lap_err:.2e}")
+
     cpu = None
     if args.cpu_budget > 0:
         from oracle import oracle as O
@@ -644,7 +685,11 @@ def bench_swe(args, torch, sb):
                        "theta": kw["fmm_theta"], "interactions": s.I, "pairs_per_evaluation": pairs,
                        "dt": kw["delta_t"], "topography": "cone", "swe_rule": "reference"},
             "state_finite_after_steps": finite,
-            "roofline": roof["swe_fp64"], "roofline_laplacian": roof["lap_fp64"], "cpu_baseline": cpu}
+            "mixed_steps_per_s": round(1.0 / secm, 4), "mixed_ms_per_step": round(1e3 * secm, 4),
+            "speedup_mixed_vs_fp64": round(sec / secm, 3),
+            "mixed_vs_fp64_stage1_velocity_rel_l2": float(f"{vel_err:.3e}"),
+            "roofline": roof["swe_fp64"], "roofline_laplacian": roof["lap_fp64"],
+            "roofline_mixed": roof["swe_mixed"], "roofline_laplacian_mixed": roof["lap_mixed"], "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

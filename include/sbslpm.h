@@ -145,14 +145,16 @@ int sbslpm_interaction_list(const sbslpm_ctx* ctx, int* list);
  * step).  sbslpm_kernel_time waits for the recorded events and returns the summed milliseconds and
  * the launch count of one kernel class since the last reset:
  * 0 Biot-Savart FP64, 1 Biot-Savart mixed, 2 log FP64, 3 log mixed, 4 shallow-water FP64,
- * 5 regularised Laplacian FP64. */
+ * 5 regularised Laplacian FP64, 6 shallow-water mixed, 7 regularised Laplacian mixed. */
 enum {
   SBSLPM_KERNEL_BVE_FP64 = 0,
   SBSLPM_KERNEL_BVE_MIXED = 1,
   SBSLPM_KERNEL_LOG_FP64 = 2,
   SBSLPM_KERNEL_LOG_MIXED = 3,
   SBSLPM_KERNEL_SWE_FP64 = 4,
-  SBSLPM_KERNEL_LAP_FP64 = 5
+  SBSLPM_KERNEL_LAP_FP64 = 5,
+  SBSLPM_KERNEL_SWE_MIXED = 6,
+  SBSLPM_KERNEL_LAP_MIXED = 7
 };
 int sbslpm_ctx_enable_timing(sbslpm_ctx* ctx, int enable);
 int sbslpm_kernel_time(sbslpm_ctx* ctx, int kernel, double* ms, int64_t* launches, int reset);
@@ -416,7 +418,9 @@ int sbslpm_swe_departure_to_target(sbslpm_ctx* ctx, const double* dep_x, const d
 int sbslpm_laplacian(sbslpm_ctx* ctx, const double* f, double* lap);
 /* laplacian_interactions (fmm_interactions/laplacian.cpp:6-35, laplacian_impl.hpp:11-69): the
  * regularised-Laplacian kernel (eps = 0.1) over the interaction list, sources at P(node + disp)
- * (disp may be NULL); phi_part [Pl][m2], phi_proxy [P][n] accumulated. */
+ * (disp may be NULL); phi_part [Pl][m2], phi_proxy [P][n] accumulated.  On a mixed-precision
+ * context the pairs are FP32 (FP64 accumulation per source set), as are the shallow-water pairs of
+ * variants 1 and 3 (sbslpm_swe_vel_interactions, sbslpm_swe_velocity, the step). */
 int sbslpm_laplacian_interactions(sbslpm_ctx* ctx, const double* disp_part_x, const double* disp_part_y,
                                   const double* disp_part_z, const double* disp_proxy_x, const double* disp_proxy_y,
                                   const double* disp_proxy_z, const double* q_part, const double* q_proxy,

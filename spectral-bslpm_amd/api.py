@@ -236,7 +236,8 @@ class Context:
         self._check(self._lib.sbslpm_ctx_set_precision(self.h, PRECISION[precision]), "set_precision")
         self.cfg.precision = precision
 
-    KERNELS = {"bve_fp64": 0, "bve_mixed": 1, "log_fp64": 2, "log_mixed": 3, "swe_fp64": 4, "lap_fp64": 5}
+    KERNELS = {"bve_fp64": 0, "bve_mixed": 1, "log_fp64": 2, "log_mixed": 3, "swe_fp64": 4, "lap_fp64": 5,
+               "swe_mixed": 6, "lap_mixed": 7}
 
     def enable_timing(self, on=True):
         self._check(self._lib.sbslpm_ctx_enable_timing(self.h, 1 if on else 0), "enable_timing")

@@ -99,6 +99,7 @@ struct sbslpm_ctx {
   // shallow-water workspace, allocated on the first SWE call
   char* swe_arena = nullptr;
   SrcRecSwe* recswe = nullptr;
+  SrcRecSwe32* recswe32 = nullptr;
   double* qdpart = nullptr;
   double* qdproxy = nullptr;
   // shallow-water step workspace (stage displacements, velocities, tendencies,
@@ -117,8 +118,8 @@ struct sbslpm_ctx {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
-  double kernel_ms[6] = {};
-  int64_t kernel_launches[6] = {};
+  double kernel_ms[8] = {};
+  int64_t kernel_launches[8] = {};
 
   // Launch shape of the Biot-Savart pair kernels (sbslpm_autotune).
   PairShape shape{};
@@ -431,7 +432,7 @@ int sbslpm_ctx_enable_timing(sbslpm_ctx* c, int enable) {
 }
 
 int sbslpm_kernel_time(sbslpm_ctx* c, int kernel, double* ms, int64_t* launches, int reset) {
-  if (!c || kernel < 0 || kernel > 5) return SBSLPM_E_ARG;
+  if (!c || kernel < 0 || kernel > 7) return SBSLPM_E_ARG;
   for (auto& p : c->pending) {
     HIP_TRY(c, hipEventSynchronize(p.b));
     float t = 0;
@@ -572,7 +573,7 @@ static int ensure_swe(sbslpm_ctx* c) {
   if (c->swe_arena) return SBSLPM_OK;
   const Tree& t = c->tree;
   Arena A;
-  const size_t o_rec = A.take<SrcRecSwe>(t.U), o_qp = A.take<double>(t.npart),
+  const size_t o_rec = A.take<SrcRecSwe>(t.U), o_rec32 = A.take<SrcRecSwe32>(t.U), o_qp = A.take<double>(t.npart),
                o_qx = A.take<double>((size_t)t.P * t.n), total = A.take<char>(256);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (hipMalloc(&c->swe_arena, total) != hipSuccess) {
@@ -580,6 +581,7 @@ static int ensure_swe(sbslpm_ctx* c) {
     return fail(c, SBSLPM_E_NOMEM, "shallow-water workspace: out of device memory");
   }
   c->recswe = (SrcRecSwe*)(c->swe_arena + o_rec);
+  c->recswe32 = (SrcRecSwe32*)(c->swe_arena + o_rec32);
   c->qdpart = (double*)(c->swe_arena + o_qp);
   c->qdproxy = (double*)(c->swe_arena + o_qx);
   return SBSLPM_OK;
@@ -593,6 +595,15 @@ static int swe_interactions(sbslpm_ctx* c, int variant, const double* dpx, const
   if (rc) return rc;
   const double* dp[3] = {dpx, dpy, dpz};
   const double* dx[3] = {dxx, dxy, dxz};
+  if (c->prm.precision != SBSLPM_FP64 && variant != 2) {
+    // FP32 pairs on the vector unit for variants 1 and 3 (variant 2 stays FP64)
+    launch_source_records_swe32(c->dt, variant, qz_part, qz_proxy, qd_part, qd_proxy, dpx ? dp : nullptr,
+                                dxx ? dx : nullptr, c->recswe32, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_SWE_MIXED);
+    launch_swe_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->recswe32, c->prm.kernel_eps, op, ox, c->stream,
+                              assign);
+    return launched(c);
+  }
   launch_source_records_swe(c->dt, variant, qz_part, qz_proxy, qd_part, qd_proxy, dpx ? dp : nullptr,
                             dxx ? dx : nullptr, c->recswe, c->stream);
   {
@@ -946,10 +957,14 @@ static int laplacian_fmm(sbslpm_ctx* c, const double* area, const double* f, con
   const Tree& t = c->tree;
   auto& w = c->sws;
   launch_upward_fused(c->dt, area, f, w.qp, w.qx, c->stream);
-  launch_source_records64(c->dt, w.qp, w.qx, dpart, dprox, true, c->rec64, c->stream);
   double* px1[1] = {w.px};
   launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 1, px1, c->stream);
-  {
+  if (c->prm.precision != SBSLPM_FP64) {
+    launch_source_records32(c->dt, w.qp, w.qx, dpart, dprox, c->rec32, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_LAP_MIXED);
+    launch_laplacian_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, w.pp, w.px, c->stream, true);
+  } else {
+    launch_source_records64(c->dt, w.qp, w.qx, dpart, dprox, true, c->rec64, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_LAP_FP64);
     launch_laplacian_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, w.pp, w.px, c->stream, true);
   }
@@ -978,6 +993,12 @@ int sbslpm_laplacian_interactions(sbslpm_ctx* c, const double* dpx, const double
     return fail(c, SBSLPM_E_ARG, "laplacian_interactions: partial displacement");
   const double* dp[3] = {dpx, dpy, dpz};
   const double* dx[3] = {dxx, dxy, dxz};
+  if (c->prm.precision != SBSLPM_FP64) {
+    launch_source_records32(c->dt, q_part, q_proxy, dpx ? dp : nullptr, dxx ? dx : nullptr, c->rec32, c->stream);
+    launch_laplacian_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, phi_part, phi_proxy, c->stream,
+                                    false);
+    return launched(c);
+  }
   launch_source_records64(c->dt, q_part, q_proxy, dpx ? dp : nullptr, dxx ? dx : nullptr, true, c->rec64, c->stream);
   launch_laplacian_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, phi_part, phi_proxy, c->stream, false);
   return launched(c);

@@ -117,6 +117,13 @@ struct SrcRec32 {
   float x, y, z, q;
 };
 
+// Source record of the mixed-precision shallow-water pair kernel: position
+// relative to the source panel centre and the two scaled charges, FP32
+// (padded to 32 bytes for whole scalar loads).
+struct SrcRecSwe32 {
+  float x, y, z, qz, qd, pad0, pad1, pad2;
+};
+
 // Source records of the MFMA pair kernel, in tiles of 16 sources (a source
 // set is padded to whole tiles with zero records).  A tile is 128 floats:
 //   [k*16 + i]                    k = 0..3: (b_x, b_y, b_z, beta) of source i,
@@ -276,6 +283,17 @@ void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
 void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                                  const SrcRec64* rec, double* out_part, double* out_proxy, hipStream_t s,
                                  bool assign = false);
+// Mixed precision (FP32 pairs, FP64 per-set accumulation) forms of the
+// shallow-water (variants 1 and 3) and regularised-Laplacian pair sums.
+void launch_source_records_swe32(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
+                                 const double* qd_part, const double* qd_proxy, const double* const* disp_part,
+                                 const double* const* disp_proxy, SrcRecSwe32* rec, hipStream_t s);
+void launch_swe_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                               const SrcRecSwe32* rec, double eps, double* const* out_part,
+                               double* const* out_proxy, hipStream_t s, bool assign);
+void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                                     const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s,
+                                     bool assign = false);
 void launch_panel_laplacian(const DevTree& t, const double* f, double* lap, hipStream_t s);
 void launch_swe_tendency(const DevTree& t, const double* x, const double* y, const double* z,
                          const double* const* vel, const double* vor, const double* div, const double* h,

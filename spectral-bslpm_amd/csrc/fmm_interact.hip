@@ -232,6 +232,50 @@ __global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __re
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
 
+// Mixed-precision regularised-Laplacian pairs: h = |x - y|^2 / 2 = 1 - t in
+// FP32 from panel-centred coordinates, t^2 - 1 = -h (2 - h), the exponential on
+// the hardware base-2 exponential (its argument lies in [-10 log2 e, 0]), erf
+// only where |t| < 0.6 (elsewhere it is +-1 in FP32 too).  FP32 sums per
+// source set; c3 (times -4 pi, undoing the record's -1/(4 pi)) applied per set
+// in FP64.
+__global__ __launch_bounds__(256) void k_lap_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ sets,
+                                                   const SrcRec32* __restrict__ rec, float eps, float c1, float c2,
+                                                   double c3, double* __restrict__ part, double* __restrict__ proxy,
+                                                   int assign) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const DevPanel& PT = t.panels[T.panel];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const float ax = (float)(t.px[u] - PT.cx), ay = (float)(t.py[u] - PT.cy), az = (float)(t.pz[u] - PT.cz);
+  const float inv_eps = 1.0f / eps, k_exp = 1.44269504088896341f / eps, cut = 1.0f - 30.0f * eps;
+  double acc = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
+    const SrcRec32* __restrict__ r = rec + S.first;
+    float v = 0.f;
+#pragma unroll 2
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRec32 s = r[k];
+      const float d0 = s.x - px, d1 = s.y - py, d2 = s.z - pz;
+      const float h = 0.5f * fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+      const float tt = 1.0f - h, tm1 = -h * (2.0f - h), t2 = 1.0f + tm1;
+      const float sv = tt * inv_eps;
+      const float ef = sv >= 6.0f ? 1.0f : (sv <= -6.0f ? -1.0f : erff(sv));
+      const float ex = __builtin_amdgcn_exp2f(tm1 * k_exp);
+      const float h1 = c1 * (2.0f * tt * (3.0f * eps + tm1));
+      const float h2 = (1.0f + ef) * ex * c2 * (2.0f * eps * eps + 2.0f * t2 * tm1 + eps * (7.0f * t2 - 1.0f));
+      v = (tt > cut) ? fmaf(h1 + h2, s.q, v) : v;
+    }
+    acc += c3 * (double)v;
+  }
+  if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
+}
+
 // Mixed precision Biot-Savart: FP32 pair arithmetic in panel-centred
 // coordinates with 1 - x.y computed as |x - y|^2 / 2, FP32 partial sums per
 // source set flushed into FP64 accumulators, cross product in FP64.
@@ -557,6 +601,17 @@ void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int 
   const double c3 = -4.0 * pi / std::pow(eps, 2.5);
   hipLaunchKernelGGL(k_lap64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
                      ntiles, sets, rec, eps, c1, c2, c3, out_part, out_proxy, assign ? 1 : 0);
+}
+
+void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                                     const SrcRec32* rec, double* out_part, double* out_proxy, hipStream_t s,
+                                     bool assign) {
+  if (ntiles == 0) return;
+  const double eps = 0.1, pi = 3.14159265358979323846;
+  const double c1 = std::exp(-1.0 / eps) * std::pow(pi, -1.5), c2 = 1.0 / (std::sqrt(eps) * pi);
+  const double c3 = -4.0 * pi / std::pow(eps, 2.5);
+  hipLaunchKernelGGL(k_lap_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
+                     ntiles, sets, rec, (float)eps, (float)c1, (float)c2, c3, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,

@@ -108,7 +108,125 @@ __global__ __launch_bounds__(256) void k_swe64(DevTree t, const TargetTile* __re
   }
 }
 
+// Mixed-precision records: position relative to the source panel centre (FP32)
+// and the two charges with the variant's constants, FP32.
+__global__ void k_src_swe32(DevTree t, const double* __restrict__ qz_part, const double* __restrict__ qz_proxy,
+                            const double* __restrict__ qd_part, const double* __restrict__ qd_proxy, CPtr3 dpart,
+                            CPtr3 dprox, double qd_scale, SrcRecSwe32* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double x = t.px[u], y = t.py[u], z = t.pz[u], qz, qd;
+  int panel;
+  if (u < t.npart) {
+    if (dpart.p[0]) { x += dpart.p[0][u]; y += dpart.p[1][u]; z += dpart.p[2][u]; }
+    qz = qz_part[u];
+    qd = qd_part[u];
+    panel = t.offset + (int)(u / t.m2);
+  } else {
+    const int64_t v = u - t.npart;
+    panel = (int)(v / t.n);
+    if (panel >= t.offset) return;  // leaf proxy rows are never sources
+    if (dprox.p[0]) { x += dprox.p[0][v]; y += dprox.p[1][v]; z += dprox.p[2][v]; }
+    qz = qz_proxy[v];
+    qd = qd_proxy[v];
+  }
+  project_to_sphere(x, y, z);
+  const DevPanel& P = t.panels[panel];
+  rec[u] = SrcRecSwe32{(float)(x - P.cx), (float)(y - P.cy), (float)(z - P.cz), (float)(kNegInv4Pi * qz),
+                       (float)(qd_scale * qd), 0.f, 0.f, 0.f};
+}
+
+// Mixed-precision shallow-water pairs (variants 1 and 3): with d = y - x and
+// h = |d|^2 / 2 = 1 - x.y computed in FP32 from panel-centred coordinates,
+// x cross y = x cross d and y - x (x.y) = d + x h, so
+//   u = x cross sum(g qz d) + sum(g qd d) + x sum(g qd h),   g = 1/(h + eps).
+// FP32 partial sums per source set are added into FP64; the cross product and
+// the output are FP64.
+__global__ __launch_bounds__(256) void k_swe_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ sets,
+                                                   const SrcRecSwe32* __restrict__ rec, float eps, Ptr3 part,
+                                                   Ptr3 proxy, int assign) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const DevPanel& PT = t.panels[T.panel];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const float ax = (float)(x - PT.cx), ay = (float)(y - PT.cy), az = (float)(z - PT.cz);
+  double w0 = 0, w1 = 0, w2 = 0, d0 = 0, d1 = 0, d2 = 0, sd = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
+    const SrcRecSwe32* __restrict__ r = rec + S.first;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f, eh = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRecSwe32 s = r[k];
+      const float c0 = s.x - px, c1 = s.y - py, c2 = s.z - pz;
+      const float h = 0.5f * fmaf(c0, c0, fmaf(c1, c1, c2 * c2));
+      const float g = __builtin_amdgcn_rcpf(h + eps);
+      const float gz = g * s.qz, gd = g * s.qd;
+      v0 = fmaf(gz, c0, v0);
+      v1 = fmaf(gz, c1, v1);
+      v2 = fmaf(gz, c2, v2);
+      e0 = fmaf(gd, c0, e0);
+      e1 = fmaf(gd, c1, e1);
+      e2 = fmaf(gd, c2, e2);
+      eh = fmaf(gd, h, eh);
+    }
+    w0 += v0; w1 += v1; w2 += v2;
+    d0 += e0; d1 += e1; d2 += e2;
+    sd += eh;
+  }
+  if (lane >= T.count) return;
+  const double a = (y * w2 - z * w1) + (d0 + x * sd);
+  const double b = (z * w0 - x * w2) + (d1 + y * sd);
+  const double c = (x * w1 - y * w0) + (d2 + z * sd);
+  double* o[3];
+  int64_t v = u;
+  if (u < t.npart) {
+    o[0] = part.p[0]; o[1] = part.p[1]; o[2] = part.p[2];
+  } else {
+    v = u - t.npart;
+    o[0] = proxy.p[0]; o[1] = proxy.p[1]; o[2] = proxy.p[2];
+  }
+  if (assign) {
+    o[0][v] = a; o[1][v] = b; o[2][v] = c;
+  } else {
+    o[0][v] += a; o[1][v] += b; o[2][v] += c;
+  }
+}
+
 }  // namespace
+
+void launch_source_records_swe32(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
+                                 const double* qd_part, const double* qd_proxy, const double* const* disp_part,
+                                 const double* const* disp_proxy, SrcRecSwe32* rec, hipStream_t s) {
+  CPtr3 dp{}, dx{};
+  for (int f = 0; f < 3; ++f) {
+    dp.p[f] = disp_part ? disp_part[f] : nullptr;
+    dx.p[f] = disp_proxy ? disp_proxy[f] : nullptr;
+  }
+  const double qd_scale = variant == 1 ? kNegInv4Pi : -kNegInv4Pi;
+  hipLaunchKernelGGL(k_src_swe32, dim3((unsigned)((t.U + 255) / 256)), dim3(256), 0, s, t, qz_part, qz_proxy,
+                     qd_part, qd_proxy, dp, dx, qd_scale, rec);
+}
+
+void launch_swe_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                               const SrcRecSwe32* rec, double eps, double* const* out_part,
+                               double* const* out_proxy, hipStream_t s, bool assign) {
+  if (ntiles == 0) return;
+  Ptr3 op{}, ox{};
+  for (int f = 0; f < 3; ++f) {
+    op.p[f] = out_part[f];
+    ox.p[f] = out_proxy[f];
+  }
+  const unsigned blocks = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
+  hipLaunchKernelGGL(k_swe_mixed, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec,
+                     (float)eps, op, ox, assign ? 1 : 0);
+}
 
 void launch_source_records_swe(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
                                const double* qd_part, const double* qd_proxy, const double* const* disp_part,

@@ -33,7 +33,9 @@ def case(request, sb, orc, torch_gpu):
     ax = np.array([0.2, 0.4, -0.7]) * 0.01
     dpart = [ax[1] * z - ax[2] * y, ax[2] * x - ax[0] * z, ax[0] * y - ax[1] * x]
     _, _, dprox = o.bve_displacement(x, y, z, dpart, np.zeros_like(x), 1.0, 0.0)
-    return dict(ctx=ctx, o=o, x=x, y=y, z=z, a=a, vor=vor, div=div, dpart=dpart, dprox=dprox, t=torch_gpu, sb=sb)
+    ctx_mixed = sb.Context(sb.RunConfig(kernel_eps=EPS, precision="mixed", **kw))
+    return dict(ctx=ctx, ctx_mixed=ctx_mixed, o=o, x=x, y=y, z=z, a=a, vor=vor, div=div, dpart=dpart, dprox=dprox,
+                t=torch_gpu, sb=sb)
 
 
 @pytest.mark.parametrize("variant", [1, 2])
@@ -96,3 +98,20 @@ def test_swe_displacement_and_remap(case):
     for g, e in zip(got[:3], exp):
         assert W.rel_l2(to_np(g), e) < 1e-13
     assert W.rel_l2(to_np(got[3]), etr) < 1e-13
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_swe_velocity_mixed_context(case, variant):
+    """On a mixed-precision context variants 1 and 3 run FP32 pairs (FP64 accumulation per
+    source set); variant 2 stays FP64.  Bar: 1e-6 relative (FP32 pair arithmetic in
+    panel-centred coordinates, as for the Biot-Savart mixed kernels)."""
+    sb, o, t = case["sb"], case["o"], case["t"]
+    ctx = case["ctx_mixed"]
+    expect = o.swe_velocity(variant, case["a"], case["vor"], case["div"], EPS, case["dpart"], case["dprox"])
+    got = ctx.swe_velocity(*[to_gpu(t, case[k]) for k in ("a", "vor", "div")],
+                           disp_part=[to_gpu(t, v) for v in case["dpart"]],
+                           disp_proxy=[to_gpu(t, v) for v in case["dprox"]], variant=variant)
+    ctx.synchronize()
+    e = sb.workloads.vec_rel_l2([to_np(g) for g in got], expect)
+    print(f"  mixed context, variant {variant}: {e:.2e}")
+    assert e < (1e-6 if variant == 1 else 1e-12)

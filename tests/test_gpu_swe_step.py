@@ -177,3 +177,31 @@ def test_step_at_rest(case):
         assert np.abs(to_np(gv)).max() == 0 and np.abs(to_np(gd)).max() == 0
         assert np.abs(to_np(gh) / H0 - 1).max() < 1e-14
         assert np.abs(to_np(tr) - 1).max() < 1e-14
+
+
+def test_mixed_context_laplacian_and_step(case, sb):
+    """A mixed-precision context runs the regularised-Laplacian and shallow-water pairs in FP32
+    (FP64 accumulation per source set): the Laplacian pair sums and the step's stage-1 velocity
+    agree with the FP64 oracle to 1e-6; the spectral stage-1 Laplacian stays FP64 (1e-12)."""
+    o, W = case["o"], case["W"]
+    x, y, z, a = case["x"], case["y"], case["z"], case["a"]
+    kw = CASES[[k for k in CASES if (CASES[k]["leaf_degree"] == o.md and CASES[k]["interp_degree"] == o.d)][0]]
+    ctx = sb.Context(sb.RunConfig(kernel_eps=EPS, omega=OMEGA, delta_t=DT, swe_rule="leaf", precision="mixed", **kw))
+    vel = [1e-6 * (y * z + 0.5), 2e-6 * np.sin(3 * x), -2e-6 * x * y]
+    _, dp, dx = o.bve_displacement(x, y, z, vel, 0 * x, 900.0, 0.0)
+    qp, qx = o.upward_pass(a, case["h"])
+    ep, ex = o.laplacian_interactions(qp, qx, dp, dx)
+    gp, gx = ctx.laplacian_interactions(*g(case, qp, qx), g(case, *dp), g(case, *dx))
+    ctx.synchronize()
+    e1 = W.rel_l2(to_np(gp), ep)
+    e2 = W.rel_l2(to_np(gx)[: o.offset], ex[: o.offset])
+    ev, ed, eh, evel, elap, _ = o.swe_back_rk4_step(x.ravel(), y.ravel(), z.ravel(), a, case["vor"], case["div"],
+                                                    case["h"], DT, OMEGA, EPS, 0, 0, 1)
+    gv, gd, gh = g(case, case["vor"], case["div"], case["h"])
+    gvel, glap = ctx.swe_back_rk4_step(*g(case, x, y, z, a), gv, gd, gh)
+    ctx.synchronize()
+    e3 = W.vec_rel_l2([to_np(v) for v in gvel], evel)
+    e4 = W.rel_l2(to_np(glap), elap)
+    finite = all(np.isfinite(to_np(v)).all() for v in (gv, gd, gh))
+    print(f"  mixed: laplacian pairs {e1:.2e} / {e2:.2e}, stage-1 velocity {e3:.2e}, stage-1 Laplacian {e4:.2e}")
+    assert e1 < 1e-6 and e2 < 1e-6 and e3 < 1e-6 and e4 < 1e-12 and finite
