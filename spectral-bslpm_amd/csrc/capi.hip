@@ -1049,6 +1049,15 @@ int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
     return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: null pointer");
   const int ntr = c->prm.tracer_count;
   if (ntr > 0 && !tracers) return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: tracers required (tracer_count > 0)");
+  {
+    const void* outs[4] = {vx, vy, vz, height_lap};
+    const void* ins[4] = {vors, divs, height, area};
+    for (const void* o : outs)
+      for (const void* i : ins)
+        if (o == i) return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: velocity / height_lap must not alias the state");
+    if (vors == divs || vors == height || divs == height)
+      return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: vors, divs and height must be distinct arrays");
+  }
   if (!swe_rule_ok(c))
     return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: the reference topography rule needs leaf_degree == interp_degree");
   int rc = ensure_swe_step(c);

@@ -205,3 +205,13 @@ def test_mixed_context_laplacian_and_step(case, sb):
     finite = all(np.isfinite(to_np(v)).all() for v in (gv, gd, gh))
     print(f"  mixed: laplacian pairs {e1:.2e} / {e2:.2e}, stage-1 velocity {e3:.2e}, stage-1 Laplacian {e4:.2e}")
     assert e1 < 1e-6 and e2 < 1e-6 and e3 < 1e-6 and e4 < 1e-12 and finite
+
+
+def test_step_rejects_aliased_outputs(case):
+    ctx, W = case["ctxs"]["leaf", "none"], case["W"]
+    x, y, z, a = case["x"], case["y"], case["z"], case["a"]
+    gv, gd, gh = g(case, case["vor"], case["div"], case["h"])
+    with pytest.raises(case["sb"].SbslpmError):
+        ctx.swe_back_rk4_step(*g(case, x, y, z, a), gv, gd, gh, height_lap=gh)
+    with pytest.raises(case["sb"].SbslpmError):
+        ctx.swe_back_rk4_step(*g(case, x, y, z, a), gv, gv, gh)
