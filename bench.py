@@ -557,17 +557,20 @@ def bench_swe(args, torch, sb):
     g/a^2 factor, so the divergence grows by orders of magnitude per step), so every timed step
     starts from the initial state again (a device copy of three fields, inside the timing).
     One step = 4 shallow-water FMM velocities, 2 regularised-Laplacian FMM evaluations, the
-    panel Laplacian, 3 tendency passes, departure search and remap.  Reports steps/s, the two
-    pair kernels' rooflines (HIP events) and one full oracle step on the host cores as the CPU
-    baseline."""
+    panel Laplacian, 3 tendency passes, departure search and remap, captured in a hipGraph and
+    replayed.  Reports FP64 and mixed-precision steps/s, the two pair kernels' rooflines (HIP
+    events around eager launches on contexts without the graph), the mixed-vs-FP64 stage-1
+    velocity, and one full oracle step on the host cores as the CPU baseline."""
     W = sb.workloads
     kw = dict(levels=6, interp_degree=5, leaf_degree=5, fmm_theta=0.7, weights="reference", kernel_eps=1e-4,
               delta_t=900.0, omega=7.292e-5, topography="cone", swe_rule="reference")
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
-        ctx = sb.Context(sb.RunConfig(**kw))
-    s = ctx.sizes
-    x, y, z, a = ctx.spec_points()
+        probe = sb.Context(sb.RunConfig(**kw))
+    s = probe.sizes
+    x, y, z, a = probe.spec_points()
+    pairs = pair_count(probe.interaction_list(), s.P, s.Pl, s.n, s.m2)
+    del probe
     A, Om, g0 = 6371000.0, 7.292e-5, 9.80616
     u0 = 20.0 / A
     vor_h = 2 * u0 * z
@@ -576,90 +579,55 @@ def bench_swe(args, torch, sb):
     dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
     gx, gy, gz, ga = [dev(v) for v in (x, y, z, a)]
     state0 = [dev(v) for v in (vor_h, div_h, h_h)]
-    st = [v.clone() for v in state0]
-    vel, lap = ctx.zeros_part(3), ctx.zeros_part()
 
-    def run(n):
+    def leg(precision):
+        """Steps/s on a graph-captured context; rooflines and the stage-1 fields from an eager one."""
         with torch.cuda.stream(stream):
-            for _ in range(n):
-                for v, v0 in zip(st, state0):
-                    v.copy_(v0)
-                ctx.swe_back_rk4_step(gx, gy, gz, ga, *st, vel=vel, height_lap=lap)
+            cg = sb.Context(sb.RunConfig(precision=precision, use_graph=not args.no_graph, **kw))
+            ce = sb.Context(sb.RunConfig(precision=precision, **kw))
+        st = [v.clone() for v in state0]
+        vel, lap = cg.zeros_part(3), cg.zeros_part()
 
-    run(max(args.warmup, 1))
-    torch.cuda.synchronize()
-    for v, v0 in zip(st, state0):
-        v.copy_(v0)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(stream):
-        e0.record(stream)
-    run(args.steps)
-    with torch.cuda.stream(stream):
-        e1.record(stream)
-    torch.cuda.synchronize()
-    ctx.synchronize()
-    sec = e0.elapsed_time(e1) / 1e3 / args.steps
-    log(f"swe step {1e3 * sec:.3f} ms")
-    finite = all(bool(torch.isfinite(v).all()) for v in st)
+        def run(ctx, n):
+            with torch.cuda.stream(stream):
+                for _ in range(n):
+                    for v, v0 in zip(st, state0):
+                        v.copy_(v0)
+                    ctx.swe_back_rk4_step(gx, gy, gz, ga, *st, vel=vel, height_lap=lap)
 
-    pairs = pair_count(ctx.interaction_list(), s.P, s.Pl, s.n, s.m2)
-    roof = {}
-    ctx.enable_timing(True)
-    for v, v0 in zip(st, state0):
-        v.copy_(v0)
-    run(2)
-    torch.cuda.synchronize()
-    for name, flop in (("swe_fp64", FLOP_SWE), ("lap_fp64", FLOP_LAP)):
-        ms, nl = ctx.kernel_time(name)
-        avg = ms / nl / 1e3
-        achieved = pairs * flop / avg / 1e12
-        roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP64, unit="TFLOP/s",
-                          frac=round(achieved / PEAK_FP64, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
-                          launches_per_step=nl // 2, pairs_per_launch=pairs, flop_per_pair=flop)
-    ctx.enable_timing(False)
-    log("roofline", roof)
-
-    # the same step on a mixed-precision context (FP32 shallow-water and Laplacian pairs)
-    with torch.cuda.stream(stream):
-        ctxm = sb.Context(sb.RunConfig(precision="mixed", **kw))
-    stm = [v.clone() for v in state0]
-    velm, lapm = ctxm.zeros_part(3), ctxm.zeros_part()
-
-    def run_m(n):
+        run(cg, max(args.warmup, 1))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         with torch.cuda.stream(stream):
-            for _ in range(n):
-                for v, v0 in zip(stm, state0):
-                    v.copy_(v0)
-                ctxm.swe_back_rk4_step(gx, gy, gz, ga, *stm, vel=velm, height_lap=lapm)
+            e0.record(stream)
+        run(cg, args.steps)
+        with torch.cuda.stream(stream):
+            e1.record(stream)
+        torch.cuda.synchronize()
+        cg.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / args.steps
+        finite = all(bool(torch.isfinite(v).all()) for v in st)
+        ce.enable_timing(True)
+        run(ce, 2)
+        torch.cuda.synchronize()
+        ce.synchronize()
+        roof, peak = {}, PEAK_FP64 if precision == "fp64" else PEAK_FP32
+        tag = "fp64" if precision == "fp64" else "mixed"
+        for kern, flop in (("swe", FLOP_SWE), ("lap", FLOP_LAP)):
+            ms, nl = ce.kernel_time(f"{kern}_{tag}")
+            avg = ms / nl / 1e3
+            achieved = pairs * flop / avg / 1e12
+            roof[kern] = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                              frac=round(achieved / peak, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
+                              launches_per_step=nl // 2, pairs_per_launch=pairs, flop_per_pair=flop)
+        ce.enable_timing(False)
+        log(f"swe step {precision}: {1e3 * sec:.3f} ms", roof)
+        return sec, roof, [v.cpu().numpy() for v in vel], lap.cpu().numpy(), finite
 
-    run_m(max(args.warmup, 1))
-    torch.cuda.synchronize()
-    f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(stream):
-        f0.record(stream)
-    run_m(args.steps)
-    with torch.cuda.stream(stream):
-        f1.record(stream)
-    torch.cuda.synchronize()
-    ctxm.synchronize()
-    secm = f0.elapsed_time(f1) / 1e3 / args.steps
-    log(f"swe step mixed {1e3 * secm:.3f} ms")
-    ctxm.enable_timing(True)
-    run_m(2)
-    torch.cuda.synchronize()
-    for name, flop in (("swe_mixed", FLOP_SWE), ("lap_mixed", FLOP_LAP)):
-        ms, nl = ctxm.kernel_time(name)
-        avg = ms / nl / 1e3
-        achieved = pairs * flop / avg / 1e12
-        roof[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32, unit="TFLOP/s",
-                          frac=round(achieved / PEAK_FP32, 4), traffic=None, avg_launch_ms=round(ms / nl, 4),
-                          launches_per_step=nl // 2, pairs_per_launch=pairs, flop_per_pair=flop)
-    ctxm.enable_timing(False)
-    # stage-1 velocity and Laplacian of one step from the same state, mixed vs FP64
-    vel_err = W.vec_rel_l2([v.cpu().numpy() for v in velm], [v.cpu().numpy() for v in vel])
-    lap_err = W.rel_l2(lapm.cpu().numpy(), lap.cpu().numpy())
-    log(f"mixed vs fp64: stage-1 velocity {vel_err:.2e}, Laplacian {lap_err:.2e}")
+    sec, roof, vel64, lap64, finite = leg("fp64")
+    secm, roofm, velm, lapm, finitem = leg("mixed")
+    vel_err = W.vec_rel_l2(velm, vel64)
+    log(f"mixed vs fp64: stage-1 velocity {vel_err:.2e}, Laplacian {W.rel_l2(lapm, lap64):.2e}")
 
     cpu = None
     if args.cpu_budget > 0:
@@ -683,13 +651,14 @@ def bench_swe(args, torch, sb):
             "config": {"workload": "Shallow-water RK4 step: 6x192^2 particles", "particles": s.N,
                        "levels": kw["levels"], "proxy_degree": kw["interp_degree"], "leaf_grid": kw["leaf_degree"] + 1,
                        "theta": kw["fmm_theta"], "interactions": s.I, "pairs_per_evaluation": pairs,
-                       "dt": kw["delta_t"], "topography": "cone", "swe_rule": "reference"},
-            "state_finite_after_steps": finite,
+                       "dt": kw["delta_t"], "topography": "cone", "swe_rule": "reference",
+                       "graph": not args.no_graph},
+            "state_finite_after_steps": finite and finitem,
             "mixed_steps_per_s": round(1.0 / secm, 4), "mixed_ms_per_step": round(1e3 * secm, 4),
             "speedup_mixed_vs_fp64": round(sec / secm, 3),
             "mixed_vs_fp64_stage1_velocity_rel_l2": float(f"{vel_err:.3e}"),
-            "roofline": roof["swe_fp64"], "roofline_laplacian": roof["lap_fp64"],
-            "roofline_mixed": roof["swe_mixed"], "roofline_laplacian_mixed": roof["lap_mixed"], "cpu_baseline": cpu}
+            "roofline": roof["swe"], "roofline_laplacian": roof["lap"],
+            "roofline_mixed": roofm["swe"], "roofline_laplacian_mixed": roofm["lap"], "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

@@ -63,7 +63,7 @@ typedef struct sbslpm_params {
   int rk4_combine;    /* SBSLPM_RK4_*: departure-point combine (bve_time_step.cpp:288-303) */
   int precision;      /* SBSLPM_FP64, SBSLPM_MIXED or SBSLPM_MIXED_VALU pair kernels */
   int tracer_count;   /* RunConfig::tracer_count */
-  int flags;          /* bit 0: capture each BVE step in a hipGraph and replay it */
+  int flags;          /* bit 0: capture each BVE / shallow-water step in a hipGraph and replay it */
   double theta;       /* RunConfig::fmm_theta */
   double kernel_eps;  /* RunConfig::kernel_eps */
   double omega;       /* RunConfig::omega */
@@ -446,8 +446,9 @@ int sbslpm_swe_height_correction(sbslpm_ctx* ctx, const double* x, const double*
  * the stage-1 surface Laplacian.  Variant-1 SWE kernel with the context's kernel_eps, omega,
  * delta_t, rk4_combine, topography and swe_rule.  The stage-4 Laplacian, height correction and
  * tendencies and the k2..k4 terms of swe_compute_arrival_values reach no output there
- * (arrival = state + dt k1, :415-417) and are not computed.  `time` is accepted for signature
- * parity (it only drives the commented-out forcing). */
+ * (arrival = state + dt k1, :415-417) and are not computed.  With SBSLPM_FLAG_GRAPH on a
+ * non-default stream the step is captured in a hipGraph per set of pointers and replayed.  `time`
+ * is accepted for signature parity (it only drives the commented-out forcing). */
 int sbslpm_swe_back_rk4_step(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
                              const double* area, double* vors, double* divs, double* height, double* vel_x,
                              double* vel_y, double* vel_z, double* tracers, double* height_lap, double time);

@@ -105,6 +105,8 @@ struct sbslpm_ctx {
   // shallow-water step workspace (stage displacements, velocities, tendencies,
   // displaced state, surface Laplacian, arrival values), allocated on the first step
   char* sws_arena = nullptr;
+  hipGraphExec_t swe_graph = nullptr;  // captured shallow-water step, keyed by the caller's pointers
+  const void* swe_graph_key[12] = {};
   struct SweStep {
     double *dpart[3], *dprox[3], *eff, *lap, *v[3][3], *tend[3][3], *st[3], *da, *arr[3], *dep[3], *ntr;
     double *qp, *qx, *pp, *px;
@@ -380,6 +382,7 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
   }
   if (c->scratch) hipFree(c->scratch);
   if (c->swe_arena) hipFree(c->swe_arena);
+  if (c->swe_graph) hipGraphExecDestroy(c->swe_graph);
   if (c->sws_arena) hipFree(c->sws_arena);
   if (c->adj_arena) hipFree(c->adj_arena);
   if (c->ens_graph) hipGraphExecDestroy(c->ens_graph);
@@ -1041,6 +1044,10 @@ int sbslpm_swe_height_correction(sbslpm_ctx* c, const double* x, const double* y
   return launched(c);
 }
 
+static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
+                         double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
+                         double* tracers, double* height_lap);
+
 int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                              double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
                              double* tracers, double* height_lap, double time) {
@@ -1060,8 +1067,48 @@ int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
   }
   if (!swe_rule_ok(c))
     return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step: the reference topography rule needs leaf_degree == interp_degree");
-  int rc = ensure_swe_step(c);
+  int rc = ensure_swe(c);
+  if (!rc) rc = ensure_swe_step(c);
   if (rc) return rc;
+  // Graph capture needs a non-default stream; on the default stream the step runs eagerly.
+  if (!(c->prm.flags & SBSLPM_FLAG_GRAPH) || !c->stream)
+    return swe_step_body(c, x, y, z, area, vors, divs, height, vx, vy, vz, tracers, height_lap);
+  const void* key[12] = {x, y, z, area, vors, divs, height, vx, vy, vz, tracers, height_lap};
+  if (!c->swe_graph || std::memcmp(key, c->swe_graph_key, sizeof(key)) != 0) {
+    if (c->swe_graph) {
+      hipGraphExecDestroy(c->swe_graph);
+      c->swe_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    rc = swe_step_body(c, x, y, z, area, vors, divs, height, vx, vy, vz, tracers, height_lap);
+    c->capturing = false;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, SBSLPM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->swe_graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->swe_graph = nullptr;
+      return fail(c, SBSLPM_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    std::memcpy(c->swe_graph_key, key, sizeof(key));
+  }
+  HIP_TRY(c, hipGraphLaunch(c->swe_graph, c->stream));
+  return SBSLPM_OK;
+}
+
+// The kernels of one shallow-water step (eager or under capture); the
+// workspaces exist already.
+static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
+                         double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
+                         double* tracers, double* height_lap) {
+  int rc = SBSLPM_OK;
+  const int ntr = c->prm.tracer_count;
   const Tree& t = c->tree;
   auto& w = c->sws;
   const int rule = c->prm.swe_rule, topo = c->prm.topography;

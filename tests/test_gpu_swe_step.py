@@ -215,3 +215,39 @@ def test_step_rejects_aliased_outputs(case):
         ctx.swe_back_rk4_step(*g(case, x, y, z, a), gv, gd, gh, height_lap=gh)
     with pytest.raises(case["sb"].SbslpmError):
         ctx.swe_back_rk4_step(*g(case, x, y, z, a), gv, gv, gh)
+
+
+def test_step_graph_replay_matches_eager(case, sb, torch_gpu):
+    """With use_graph on a non-default stream the step is captured once and replayed; three
+    replays, each from the same state with eager work between them, equal the eager step to
+    the bit (the step's kernels are deterministic: no atomics)."""
+    t, o, W = torch_gpu, case["o"], case["W"]
+    x, y, z, a = case["x"], case["y"], case["z"], case["a"]
+    kw = CASES[[k for k in CASES if (CASES[k]["leaf_degree"] == o.md and CASES[k]["interp_degree"] == o.d)][0]]
+    stream = t.cuda.Stream()
+    with t.cuda.stream(stream):
+        cg = sb.Context(sb.RunConfig(kernel_eps=EPS, omega=OMEGA, delta_t=DT, swe_rule="leaf", topography="cone",
+                                     tracer_count=2, use_graph=True, **kw))
+    ce = case["ctxs"]["leaf", "cone"]
+    rng = np.random.default_rng(11)
+    tr0 = rng.normal(size=x.shape + (2,))
+    geo = g(case, x, y, z, a)
+    ev, ed, eh, etr = g(case, case["vor"], case["div"], case["h"], tr0)
+    evel, elap = ce.swe_back_rk4_step(*geo, ev, ed, eh, tracers=etr)
+    ce.synchronize()
+    expect = [to_np(v) for v in (ev, ed, eh, etr, *evel, elap)]
+    state0 = g(case, case["vor"], case["div"], case["h"], tr0)
+    st = [v.clone() for v in state0]
+    vel, lap = cg.zeros_part(3), cg.zeros_part()
+    for rep in range(3):
+        with t.cuda.stream(stream):
+            for v, v0 in zip(st, state0):
+                v.copy_(v0)
+            cg.swe_back_rk4_step(*geo, *st[:3], vel=vel, tracers=st[3], height_lap=lap)
+        stream.synchronize()
+        cg.synchronize()
+        got = [to_np(v) for v in (*st, *vel, lap)]
+        same = all(np.array_equal(gv, e) for gv, e in zip(got, expect))
+        print(f"  replay {rep}: bit-identical to the eager step: {same}")
+        assert same
+        _ = (st[0] * 2).sum().item()  # eager work between replays
