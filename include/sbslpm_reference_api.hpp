@@ -124,6 +124,25 @@ inline void bve_back_rk4_step(sbslpm_ctx* ctx, View2<double>& xcos, View2<double
         "bve_back_rk4_step");
 }
 
+// swe_back_rk4_step (swe_time_step.cpp:606-811).  interp_vals, the panels and
+// the interaction list are the context's; topo_type and the indexing rule are
+// the context's sbslpm_params.topography / swe_rule.
+inline void swe_back_rk4_step(sbslpm_ctx* ctx, View2<double>& xcos, View2<double>& ycos, View2<double>& zcos,
+                              View2<double>& area, View2<double>& vors, View2<double>& divs, View2<double>& height,
+                              View2<double>& vel_x, View2<double>& vel_y, View2<double>& vel_z,
+                              double* passive_tracers, View2<double>& height_lapl, double time) {
+  check(ctx,
+        sbslpm_swe_back_rk4_step(ctx, xcos.data(), ycos.data(), zcos.data(), area.data(), vors.data(), divs.data(),
+                                 height.data(), vel_x.data(), vel_y.data(), vel_z.data(), passive_tracers,
+                                 height_lapl.data(), time),
+        "swe_back_rk4_step");
+}
+
+// laplacian (deriv_funcs.cpp:27-37): the spectral surface Laplacian on each leaf.
+inline void laplacian(sbslpm_ctx* ctx, View2<double>& laplacian_vals, View2<double>& func_vals) {
+  check(ctx, sbslpm_laplacian(ctx, func_vals.data(), laplacian_vals.data()), "laplacian");
+}
+
 // ---- the lat-lon path (irregular particle tree).  The tree, its panels and
 // leaf_panel_points live in the sbslpm_irreg handle (sbslpm_irreg_create over
 // the same points); the point arrays are the reference's [lat][lon] arrays.
