@@ -25,6 +25,15 @@
 // swe_vel_impl.hpp:179-287), with particle label coordinates taken from the
 // leaf grid.
 //
+// How each part is pinned (checks that do not restate it) is in the tests:
+// tests/test_oracle_*.py (closed forms, brute force, invariants).  The
+// shallow-water step's reference indexing rule (orc_velocity_gradient rule 0:
+// the leaf-local face index, the metric applied twice, the pole shortcut of
+// xyzvec_from_xietavec) is pinned through its parts only -- the barycentric
+// differentiation and the vector conversion are checked against closed forms
+// and the two rules agree where the reference's is exact -- so its composition
+// is "partially pinned": it follows the reference's lines.
+//
 // Build: oracle/build.sh (g++ -O2 -fopenmp -ffp-contract=off).
 #include <cmath>
 #include <cstdint>
