@@ -148,7 +148,7 @@ def bench_ensemble(args, torch, sb):
     for side, levels, m in ENSEMBLE:
         kw = dict(levels=levels, interp_degree=8, leaf_degree=m - 1, fmm_theta=0.7, weights="clenshaw_curtis",
                   kernel_eps=1e-4, delta_t=3600.0, omega=7.292e-5, rk4_combine="reference",
-                  use_graph=not args.no_graph, precision="mixed")
+                  use_graph=not args.no_graph, precision="mixed", multi_rhs=not args.no_multi_rhs)
         with torch.cuda.stream(stream):
             ctx = sb.Context(sb.RunConfig(**kw))
         x, y, z, a = ctx.spec_points()
@@ -226,7 +226,7 @@ def bench_ensemble(args, torch, sb):
             "data": "synthetic (12 seeded Gaussian vortices + seeded SH noise per member)",
             "config": {"workload": "BVE ensemble (c4): 3 x 16 members", "members": total_members,
                        "resolutions": [f"6x{s}^2 (L={l}, m={m}, d=8)" for s, l, m in ENSEMBLE],
-                       "graph": not args.no_graph},
+                       "graph": not args.no_graph, "multi_rhs_first_stage": not args.no_multi_rhs},
             "one_member_at_a_time_ms_per_step": round(1e3 * seq, 3),
             "speedup_vs_one_member_at_a_time": round(seq / (sec / args.steps), 3),
             "member0_vor_rel_l2_vs_fp64": acc,
@@ -674,6 +674,8 @@ def main():
                          "adjoint: cost of the step gradient at 6x512^2, 4dvar: BVE 4D-Var gradient at 6x192^2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-multi-rhs", action="store_true",
+                    help="c4: first RK4 stage on the per-member pair kernel instead of the multi-RHS product")
     ap.add_argument("--no-tune", action="store_true", help="skip the pair-kernel launch-shape autotuner")
     ap.add_argument("--shape", default=None, metavar="WAVES,XCD",
                     help="fix the pair-kernel launch shape instead of autotuning (e.g. 4,0)")

@@ -52,7 +52,13 @@ enum { SBSLPM_SWE_RULE_REFERENCE = 0, SBSLPM_SWE_RULE_LEAF = 1 };
 /* SBSLPM_MIXED: FP32 pair arithmetic on the matrix cores (Biot-Savart) / vector unit (log), FP64
  * accumulation; SBSLPM_MIXED_VALU: the same precision with the Biot-Savart pairs on the vector unit. */
 enum { SBSLPM_FP64 = 0, SBSLPM_MIXED = 1, SBSLPM_MIXED_VALU = 2 };
-enum { SBSLPM_FLAG_GRAPH = 1 };
+/* SBSLPM_FLAG_GRAPH: capture steps in a hipGraph (needs a non-default stream).
+ * SBSLPM_FLAG_MULTI_RHS: on a SBSLPM_MIXED context, the first RK4 stage of an ensemble step
+ * (sources and targets at the same positions for every member) computes each kernel matrix once
+ * and applies it to all members' charges as one multi-right-hand-side product on the matrix
+ * cores.  Members then agree with the single-member step to FP32 rounding (another summation
+ * order) instead of bit for bit. */
+enum { SBSLPM_FLAG_GRAPH = 1, SBSLPM_FLAG_MULTI_RHS = 2 };
 
 /* Run parameters: the hot-path subset of RunConfig (run_config.hpp:6-44). */
 typedef struct sbslpm_params {
@@ -145,7 +151,8 @@ int sbslpm_interaction_list(const sbslpm_ctx* ctx, int* list);
  * step).  sbslpm_kernel_time waits for the recorded events and returns the summed milliseconds and
  * the launch count of one kernel class since the last reset:
  * 0 Biot-Savart FP64, 1 Biot-Savart mixed, 2 log FP64, 3 log mixed, 4 shallow-water FP64,
- * 5 regularised Laplacian FP64, 6 shallow-water mixed, 7 regularised Laplacian mixed. */
+ * 5 regularised Laplacian FP64, 6 shallow-water mixed, 7 regularised Laplacian mixed,
+ * 8 Biot-Savart mixed, multi-right-hand-side form (ensemble first stage, SBSLPM_FLAG_MULTI_RHS). */
 enum {
   SBSLPM_KERNEL_BVE_FP64 = 0,
   SBSLPM_KERNEL_BVE_MIXED = 1,
@@ -154,7 +161,8 @@ enum {
   SBSLPM_KERNEL_SWE_FP64 = 4,
   SBSLPM_KERNEL_LAP_FP64 = 5,
   SBSLPM_KERNEL_SWE_MIXED = 6,
-  SBSLPM_KERNEL_LAP_MIXED = 7
+  SBSLPM_KERNEL_LAP_MIXED = 7,
+  SBSLPM_KERNEL_BVE_MRHS = 8
 };
 int sbslpm_ctx_enable_timing(sbslpm_ctx* ctx, int enable);
 int sbslpm_kernel_time(sbslpm_ctx* ctx, int kernel, double* ms, int64_t* launches, int reset);

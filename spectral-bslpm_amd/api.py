@@ -37,6 +37,7 @@ class RunConfig:
     precision: str = "fp64"          # fp64 | mixed (matrix cores) | mixed_valu
     tracer_count: int = 0
     use_graph: bool = False
+    multi_rhs: bool = False          # ensembles: first stage as one multi-RHS product (mixed only)
     topography: str = "none"         # RunConfig::topo_type: none | cone
     swe_rule: str = "reference"      # reference | leaf (shallow-water step indexing, see sbslpm.h)
 
@@ -49,7 +50,7 @@ class RunConfig:
         p.rk4_combine = COMBINE[self.rk4_combine]
         p.precision = PRECISION[self.precision]
         p.tracer_count = self.tracer_count
-        p.flags = 1 if self.use_graph else 0
+        p.flags = (1 if self.use_graph else 0) | (2 if self.multi_rhs else 0)
         p.theta = self.fmm_theta
         p.kernel_eps = self.kernel_eps
         p.omega = self.omega
@@ -237,7 +238,7 @@ class Context:
         self.cfg.precision = precision
 
     KERNELS = {"bve_fp64": 0, "bve_mixed": 1, "log_fp64": 2, "log_mixed": 3, "swe_fp64": 4, "lap_fp64": 5,
-               "swe_mixed": 6, "lap_mixed": 7}
+               "swe_mixed": 6, "lap_mixed": 7, "bve_mrhs": 8}
 
     def enable_timing(self, on=True):
         self._check(self._lib.sbslpm_ctx_enable_timing(self.h, 1 if on else 0), "enable_timing")

@@ -23,6 +23,7 @@ struct Work {
   SrcRec64* rec64 = nullptr;
   SrcRec32* rec32 = nullptr;
   SrcRecMfma* recmf = nullptr;
+  float* qcol = nullptr;  // multi-RHS charge columns (ensembles with SBSLPM_FLAG_MULTI_RHS)
   double* qpart = nullptr;
   double* qproxy = nullptr;
   double* phipart[3] = {};
@@ -120,8 +121,8 @@ struct sbslpm_ctx {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
-  double kernel_ms[8] = {};
-  int64_t kernel_launches[8] = {};
+  double kernel_ms[9] = {};
+  int64_t kernel_launches[9] = {};
 
   // Launch shape of the Biot-Savart pair kernels (sbslpm_autotune).
   PairShape shape{};
@@ -435,7 +436,7 @@ int sbslpm_ctx_enable_timing(sbslpm_ctx* c, int enable) {
 }
 
 int sbslpm_kernel_time(sbslpm_ctx* c, int kernel, double* ms, int64_t* launches, int reset) {
-  if (!c || kernel < 0 || kernel > 7) return SBSLPM_E_ARG;
+  if (!c || kernel < 0 || kernel > 8) return SBSLPM_E_ARG;
   for (auto& p : c->pending) {
     HIP_TRY(c, hipEventSynchronize(p.b));
     float t = 0;
@@ -723,7 +724,15 @@ int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const d
   // need clearing.
   launch_upward_fused(c->dt, area, vor, w.qpart, w.qproxy, c->stream, nmem);
   launch_zero_rows(c->zero_rows, c->nzero_rows, t.n, 3, w.phiproxy, c->stream, nmem, Pn);
-  if (c->prm.precision == SBSLPM_MIXED) {
+  if (c->prm.precision == SBSLPM_MIXED && nmem > 1 && w.qcol && !dpart && !dprox) {
+    // all members at the undisplaced positions: member 0's records give the
+    // geometry, the members' charges become columns of one product
+    launch_source_records_mfma(c->dt, w.qpart, w.qproxy, nullptr, nullptr, w.recmf, c->stream, 1);
+    launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, w.qcol, c->stream);
+    KernelTimer tm(c, SBSLPM_KERNEL_BVE_MRHS);
+    launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, c->prm.kernel_eps,
+                             w.phipart, w.phiproxy, c->stream, true, c->shape);
+  } else if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records_mfma(c->dt, w.qpart, w.qproxy, dpart, dprox, w.recmf, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
     launch_bve_interact_mfma(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, c->prm.kernel_eps, w.phipart,
@@ -1792,6 +1801,8 @@ static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
   const size_t o_r64 = A.take<SrcRec64>(c->prm.precision == SBSLPM_FP64 ? U : 0);
   const size_t o_r32 = A.take<SrcRec32>(c->prm.precision == SBSLPM_MIXED_VALU ? U : 0);
   const size_t o_rmf = A.take<SrcRecMfma>(c->prm.precision == SBSLPM_MIXED ? U16 : 0);
+  const bool mrhs = c->prm.precision == SBSLPM_MIXED && (c->prm.flags & SBSLPM_FLAG_MULTI_RHS) && nmem > 1;
+  const size_t o_qc = A.take<float>(mrhs ? (size_t)t.U16 * 16 * mrhs_chunks(nmem) : 0);
   const size_t o_qp = A.take<double>(NP), o_qx = A.take<double>(Pn);
   const size_t o_pp = A.take<double>(3 * NP), o_px = A.take<double>(3 * Pn);
   const size_t o_dp = A.take<double>(3 * NP), o_dx = A.take<double>(3 * Pn);
@@ -1811,6 +1822,7 @@ static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
   Work& w = c->wens;
   w.members = nmem;
   w.rec64 = (SrcRec64*)(b + o_r64); w.rec32 = (SrcRec32*)(b + o_r32); w.recmf = (SrcRecMfma*)(b + o_rmf);
+  w.qcol = mrhs ? (float*)(b + o_qc) : nullptr;
   w.qpart = (double*)(b + o_qp); w.qproxy = (double*)(b + o_qx);
   for (int f = 0; f < 3; ++f) {
     w.phipart[f] = (double*)(b + o_pp) + f * NP;

@@ -546,6 +546,222 @@ __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* _
   }
 }
 
+// ---- multi-right-hand-side form (ensembles with shared positions)
+//
+// When every member of an ensemble has its sources and targets at the same
+// positions (the undisplaced first RK4 stage), R_ij = 1/(eps + |p_j - b_i|^2/2)
+// is the same matrix for all members and the pair sums are the product
+// W[j][n] = sum_i R_ij Q[i][n] with one column n per member and component
+// (q b_x, q b_y, q b_z, q).  R comes from the 16x16x4 product of the
+// single-member kernel once per wave and is used again as the A operand of
+// 16x16x4 products against 16 charge columns (four members): register r of
+// lane l holds R[4 (l/16) + r][l % 16], i.e. A[j][a] = R[4 a + r][j], so
+// product r takes sources {4 a + r}, and its B operand is Q[4 a + r][n] from
+// lane n + 16 a.  The result W[j][n] lands in lane n + 16 (j/4), register
+// j % 4.
+//
+// Charge columns: for 16-source tile g and chunk k of four members, 64 lanes x
+// float4,  qcol[((g * nchunk + k) * 64 + n + 16 a) * 4 + r] = Q[4 a + r][n],
+// n = 4 * (member % 4) + c.  Members beyond the ensemble (padding of the last
+// chunk group) have zero columns.
+__global__ void k_src_mrhs(DevTree t, const double* __restrict__ q_part, const double* __restrict__ q_proxy,
+                           const float* __restrict__ geo, int nmem, int nchunk, float* __restrict__ qcol) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= t.U16) return;
+  const int mem = (int)blockIdx.y;
+  const int64_t lp = (int64_t)t.Pl * t.m2p;
+  int64_t u;
+  bool valid;
+  if (s < lp) {
+    const int k = (int)(s % t.m2p);
+    valid = k < t.m2;
+    u = (s / t.m2p) * t.m2 + k;
+  } else {
+    const int64_t v = s - lp;
+    const int k = (int)(v % t.np16);
+    valid = k < t.n;
+    u = t.npart + (v / t.np16) * t.n + k;
+  }
+  double q = 0.0;
+  if (valid && mem < nmem)
+    q = u < t.npart ? q_part[(int64_t)mem * t.npart + u]
+                    : q_proxy[(int64_t)mem * t.P * t.n + (u - t.npart)];
+  const int64_t g = s >> 4;
+  const int i = (int)(s & 15);
+  const float* tile = geo + g * 128;  // (b_x, b_y, b_z) of source i at [16 k + i]
+  const double qq = kNegInv4Pi * q;
+  const float v[4] = {(float)(qq * tile[i]), (float)(qq * tile[16 + i]), (float)(qq * tile[32 + i]), (float)qq};
+  float* o = qcol + ((g * nchunk + (mem >> 2)) * 64 + 16 * (i >> 2) + 4 * (mem & 3)) * 4 + (i & 3);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) o[4 * c] = v[c];
+}
+
+// Lane SEL of each quad of lanes (DPP quad_perm, all four lanes read SEL).
+template <int SEL>
+__device__ inline float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), SEL * 0x55, 0xF, 0xF, false));
+}
+template <int SEL>
+__device__ inline double quad_bcast(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), SEL * 0x55, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), SEL * 0x55, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Pair sums of NTT target tiles of 16 for CB chunks of four members.  Lane l
+// accumulates, per source set, w[tt][cb][r] += W_c - p_c W_q for target
+// tt*16 + 4 (l/16) + r, member 4 (chunk0 + cb) + (l%16)/4, component c = l % 4
+// (lanes with c = 3 carry W_q and are discarded).
+template <int NTT, int CB>
+__device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, const SourceSet* __restrict__ sets,
+                                      const float* __restrict__ geo, const float* __restrict__ qcol, int nchunk,
+                                      int chunk0, float eps, int lane, double (&w)[4][CB][4]) {
+  const int col = lane & 15, grp = lane >> 4, comp = lane & 3;
+  const DevPanel& PT = t.panels[T.panel];
+  const double* __restrict__ pc_arr = comp == 0 ? t.px : comp == 1 ? t.py : t.pz;
+  const double pc_ctr = comp == 0 ? PT.cx : comp == 1 ? PT.cy : PT.cz;
+  float ax[NTT], ay[NTT], az[NTT], ac[NTT][4];
+#pragma unroll
+  for (int tt = 0; tt < NTT; ++tt) {
+    const int j = min(tt * 16 + col, T.count - 1);
+    const int64_t u = (int64_t)T.first + j;
+    ax[tt] = (float)(t.px[u] - PT.cx);
+    ay[tt] = (float)(t.py[u] - PT.cy);
+    az[tt] = (float)(t.pz[u] - PT.cz);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int jj = min(tt * 16 + 4 * grp + r, T.count - 1);
+      ac[tt][r] = comp == 3 ? 0.f : (float)(pc_arr[(int64_t)T.first + jj] - pc_ctr);
+    }
+  }
+  if (T.src_begin >= T.src_end) return;
+  const float sx = grp == 0 ? -1.f : 0.f, sy = grp == 1 ? -1.f : 0.f, sz = grp == 2 ? -1.f : 0.f;
+  const float s1 = grp == 3 ? 1.f : 0.f;
+  float b1[NTT], alpha[NTT], pc[NTT][4];
+  v4f acc[NTT][CB];
+  auto begin_set = [&](const SourceSet& S) {
+    const float so = comp == 0 ? S.ox : comp == 1 ? S.oy : comp == 2 ? S.oz : 0.f;
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt) {
+      const float px = ax[tt] + S.ox, py = ay[tt] + S.oy, pz = az[tt] + S.oz;
+      alpha[tt] = fmaf(0.5f, fmaf(px, px, fmaf(py, py, pz * pz)), eps);
+      b1[tt] = fmaf(sx, px, fmaf(sy, py, fmaf(sz, pz, s1)));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pc[tt][r] = comp == 3 ? 0.f : ac[tt][r] + so;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[tt][cb] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  auto qptr = [&](int64_t g) { return qcol + ((g * nchunk + chunk0) * 64 + lane) * 4; };
+  const int last = T.src_end - 1;
+  int e = T.src_begin;
+  SourceSet S = sets[e], Sn = sets[min(e + 1, last)], Snn = sets[min(e + 2, last)];
+  begin_set(S);
+  int nst = (S.count + 15) >> 4, st = 0;
+  int64_t g = S.first16 >> 4;
+  float a1 = geo[g * 128 + lane];
+  v4f qv[CB];
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) qv[cb] = *reinterpret_cast<const v4f*>(qptr(g) + cb * 256);
+  while (true) {
+    const int64_t gn = st + 1 < nst ? g + 1 : (int64_t)(Sn.first16 >> 4);
+    const float na1 = geo[gn * 128 + lane];
+    v4f nqv[CB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) nqv[cb] = *reinterpret_cast<const v4f*>(qptr(gn) + cb * 256);
+    v4f d[NTT];
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt)
+      d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) d[tt][r] = __builtin_amdgcn_rcpf(d[tt][r] + alpha[tt]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          acc[tt][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(d[tt][r], qv[cb][r], acc[tt][cb], 0, 0, 0);
+    a1 = na1;
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) qv[cb] = nqv[cb];
+    g = gn;
+    if (++st == nst) {
+#pragma unroll
+      for (int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float wq = quad_bcast<3>(acc[tt][cb][r]);
+            w[tt][cb][r] += (double)fmaf(-pc[tt][r], wq, acc[tt][cb][r]);
+          }
+      if (++e > last) break;
+      S = Sn;
+      Sn = Snn;
+      Snn = sets[min(e + 2, last)];
+      begin_set(S);
+      nst = (S.count + 15) >> 4;
+      st = 0;
+    }
+  }
+}
+
+// One wave per target tile and group of CB member chunks (blockIdx.y).  Every
+// (target, member, component) is written by exactly one lane.
+template <int CB>
+__global__ __launch_bounds__(256) void k_bve_mrhs(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                  const SourceSet* __restrict__ sets, const float* __restrict__ geo,
+                                                  const float* __restrict__ qcol, int nchunk, int nmem, float eps,
+                                                  Ptr3 part, Ptr3 proxy, int assign, int xcd) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4, comp = lane & 3;
+  const int tile = shaped_tile(wave, xcd);
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int ntt = (T.count + 15) >> 4;
+  const int chunk0 = (int)blockIdx.y * CB;
+  double w[4][CB][4];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) w[tt][cb][r] = 0.0;
+  switch (ntt) {
+    case 4: bve_mrhs_tiles<4, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 3: bve_mrhs_tiles<3, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 2: bve_mrhs_tiles<2, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    default: bve_mrhs_tiles<1, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+  }
+  // u = x cross w: the three components of one (target, member) sit in one quad
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) {
+    if (tt >= ntt) break;
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double w0 = quad_bcast<0>(w[tt][cb][r]), w1 = quad_bcast<1>(w[tt][cb][r]),
+                     w2 = quad_bcast<2>(w[tt][cb][r]);
+        const int j = tt * 16 + 4 * grp + r;
+        const int mem = 4 * (chunk0 + cb) + (col >> 2);
+        if (comp < 3 && j < T.count && mem < nmem) {
+          const int64_t u = (int64_t)T.first + j;
+          const double x = t.px[u], y = t.py[u], z = t.pz[u];
+          const double val = comp == 0 ? y * w2 - z * w1 : comp == 1 ? z * w0 - x * w2 : x * w1 - y * w0;
+          double* o = u < t.npart ? part.p[comp] + (int64_t)mem * t.npart + u
+                                  : proxy.p[comp] + (int64_t)mem * t.P * t.n + (u - t.npart);
+          if (assign) *o = val;
+          else *o += val;
+        }
+      }
+  }
+}
+
 inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
 inline CPtr3 cptr3(const double* const* a) {
@@ -636,6 +852,28 @@ void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int nti
   hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t,
                      tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy),
                      assign ? 1 : 0, shape.xcd);
+}
+
+void launch_source_columns_mrhs(const DevTree& t, const double* q_part, const double* q_proxy,
+                                const SrcRecMfma* geo, int nmem, float* qcol, hipStream_t s) {
+  const int nchunk = mrhs_chunks(nmem);
+  hipLaunchKernelGGL(k_src_mrhs, dim3(blocks_for(t.U16, 256), 4 * nchunk), dim3(256), 0, s, t, q_part, q_proxy,
+                     (const float*)geo, nmem, nchunk, qcol);
+}
+
+void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
+                              const SrcRecMfma* geo, const float* qcol, int nmem, double eps,
+                              double* const* out_part, double* const* out_proxy, hipStream_t s, bool assign,
+                              PairShape shape) {
+  if (ntiles == 0) return;
+  const int nchunk = mrhs_chunks(nmem), cb = mrhs_chunks_per_wave(nmem);
+  const dim3 grid(blocks_for(ntiles, shape.waves), nchunk / cb), block(64 * shape.waves);
+  if (cb == 2)
+    hipLaunchKernelGGL(k_bve_mrhs<2>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
+                       nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
+  else
+    hipLaunchKernelGGL(k_bve_mrhs<1>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
+                       nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,

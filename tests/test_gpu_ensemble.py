@@ -2,7 +2,9 @@
 (gridDim.y = members) and the tree, interaction list and work tiles.  Each
 member must come out bit for bit as the single-member bve_back_rk4_step
 (bve_time_step.cpp:311-419) leaves it, in every precision, eagerly and as a
-replayed hipGraph."""
+replayed hipGraph.  With the multi-right-hand-side first stage (mixed
+precision, members' charges as columns of one product) members agree with
+the single-member steps to FP32 rounding."""
 import numpy as np
 import pytest
 
@@ -43,11 +45,64 @@ def test_ensemble_members_match_single_steps(sb, torch_gpu, precision):
             np.testing.assert_array_equal(to_np(evel[f][j]), single_vel[j][f])
 
 
-def test_ensemble_graph_replays_match_eager(sb, torch_gpu):
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("nmem", [3, 6, 9])
+def test_ensemble_multi_rhs_first_stage(sb, torch_gpu, nmem):
+    """nmem = 3: one chunk of four members (one padded); 6: two chunks in one
+    wave (two padded); 9: two waves of two chunks (seven padded).  Stage-0
+    velocities and stepped vorticities against the single-member mixed and FP64
+    steps; the mixed-precision bar is rel-L2 1e-6 against FP64."""
     t = torch_gpu
-    print("\n[case ensemble graph replay]", flush=True)
+    print(f"\n[case ensemble multi_rhs nmem={nmem}]", flush=True)
+    kw = dict(levels=3, interp_degree=8, leaf_degree=7, weights="clenshaw_curtis", kernel_eps=EPS, delta_t=DT,
+              omega=OMEGA, precision="mixed")
+    ctx = sb.Context(sb.RunConfig(multi_rhs=True, **kw))
+    cmx = sb.Context(sb.RunConfig(**kw))
+    c64 = sb.Context(sb.RunConfig(**dict(kw, precision="fp64")))
+    x, y, z, a = ctx.spec_points()
+    gx, gy, gz, ga = [to_gpu(t, q) for q in (x, y, z, a)]
+    members = [sb.workloads.gaussian_vortices(x, y, z, a, seed=s) * (1 + 0.5 * s) for s in range(1, nmem + 1)]
+    single = {}
+    for name, c in (("mixed", cmx), ("fp64", c64)):
+        single[name] = []
+        for vor in members:
+            gv = to_gpu(t, vor)
+            vel = c.zeros_part(3)
+            c.bve_back_rk4_step(gx, gy, gz, ga, gv, vel)
+            c.synchronize()
+            single[name].append((to_np(gv), np.stack([to_np(v) for v in vel])))
+    ctx.enable_timing(True)
+    for _ in range(2):  # the second call reuses the workspace and the charge columns
+        ens = to_gpu(t, np.stack(members))
+        evel = [t.zeros_like(ens) for _ in range(3)]
+        ctx.bve_back_rk4_step_ensemble(gx, gy, gz, ga, ens, evel)
+        ctx.synchronize()
+        for j in range(nmem):
+            vel = np.stack([to_np(evel[f][j]) for f in range(3)])
+            vor = to_np(ens[j])
+            errs = (rel_l2(vel, single["mixed"][j][1]), rel_l2(vel, single["fp64"][j][1]),
+                    rel_l2(vor, single["mixed"][j][0]), rel_l2(vor, single["fp64"][j][0]))
+            base = rel_l2(single["mixed"][j][1], single["fp64"][j][1])
+            print(f"member {j}: vel vs mixed {errs[0]:.2e} vs fp64 {errs[1]:.2e} (single mixed vs fp64 "
+                  f"{base:.2e}); vor vs mixed {errs[2]:.2e} vs fp64 {errs[3]:.2e}", flush=True)
+            # two FP32 summation orders, each at the FP32 rounding level of the eps = 1e-4 near field
+            assert errs[0] < 1e-6 and errs[1] < 1e-6, j
+            assert errs[2] < 1e-6 and errs[3] < 1e-6, j
+    _, launches = ctx.kernel_time("bve_mrhs", reset=True)
+    assert launches == 2  # one first-stage launch per ensemble step
+    _, launches = ctx.kernel_time("bve_mixed", reset=True)
+    assert launches == 6  # stages 1..3 on the per-member kernel
+
+
+@pytest.mark.parametrize("multi_rhs", [False, True])
+def test_ensemble_graph_replays_match_eager(sb, torch_gpu, multi_rhs):
+    t = torch_gpu
+    print(f"\n[case ensemble graph replay multi_rhs={multi_rhs}]", flush=True)
     kw = dict(levels=4, interp_degree=8, leaf_degree=7, weights="clenshaw_curtis", kernel_eps=EPS, delta_t=DT,
-              omega=OMEGA)
+              omega=OMEGA, multi_rhs=multi_rhs, precision="mixed" if multi_rhs else "fp64")
     eager = sb.Context(sb.RunConfig(**kw))
     s = t.cuda.Stream()
     with t.cuda.stream(s):
