@@ -144,7 +144,7 @@ def bench_ensemble(args, torch, sb):
     nmem = 16
     stream = torch.cuda.Stream()
     dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
-    groups = []
+    groups, mshapes = [], {}
     for side, levels, m in ENSEMBLE:
         kw = dict(levels=levels, interp_degree=8, leaf_degree=m - 1, fmm_theta=0.7, weights="clenshaw_curtis",
                   kernel_eps=1e-4, delta_t=3600.0, omega=7.292e-5, rk4_combine="reference",
@@ -162,8 +162,13 @@ def bench_ensemble(args, torch, sb):
         if not args.no_tune:
             with torch.cuda.stream(stream):
                 _, shape = ctx.autotune(dev(a), vors[0].contiguous())
+                if not args.no_multi_rhs:
+                    _, mshape = ctx.autotune_ensemble(dev(a), vors)
             torch.cuda.synchronize()
             log(f"6x{side}^2: pair-kernel launch shape {shape}")
+            if not args.no_multi_rhs:
+                log(f"6x{side}^2: multi-RHS launch shape (waves, xcd, chunks per wave) {mshape}")
+                mshapes[f"6x{side}^2"] = list(mshape)
         g = dict(side=side, ctx=ctx, kw=kw, xyz=[dev(v) for v in (x, y, z, a)], vors0=vors,
                  vors=vors.clone(), vel=[torch.zeros_like(vors) for _ in range(3)])
         groups.append(g)
@@ -230,7 +235,8 @@ def bench_ensemble(args, torch, sb):
             "one_member_at_a_time_ms_per_step": round(1e3 * seq, 3),
             "speedup_vs_one_member_at_a_time": round(seq / (sec / args.steps), 3),
             "member0_vor_rel_l2_vs_fp64": acc,
-            "pair_kernel_launch_shapes": {f"6x{g['side']}^2": list(g["ctx"].pair_shape()) for g in groups}}
+            "pair_kernel_launch_shapes": {f"6x{g['side']}^2": list(g["ctx"].pair_shape()) for g in groups},
+            "multi_rhs_launch_shapes": mshapes}
     print(json.dumps(line), flush=True)
 
 

@@ -471,6 +471,19 @@ class Context:
         self._check(self._lib.sbslpm_autotune(self.h, _ptr(area), _ptr(vor), _dp(rep), ctypes.byref(n)), "autotune")
         return [tuple(r) for r in rep[:n.value]], self.pair_shape()
 
+    def autotune_ensemble(self, area, vors):
+        """Times the multi-right-hand-side pair kernel of ensemble steps (mixed context with
+        multi_rhs=True) over waves per workgroup x XCD mapping x chunks of four members per
+        wave, with the charges of vors [members][Pl][m2], and keeps the fastest; returns
+        [(waves, xcd, chunks, ms per launch), ...] and the chosen (waves, xcd, chunks)."""
+        rep = np.zeros((12, 4))
+        n = ctypes.c_int(0)
+        self._check(self._lib.sbslpm_autotune_ensemble(self.h, int(vors.shape[0]), _ptr(area), _ptr(vors), _dp(rep),
+                                                       ctypes.byref(n)), "autotune_ensemble")
+        rows = [tuple(r) for r in rep[:n.value]]
+        best = min(rows, key=lambda r: r[3])
+        return rows, (int(best[0]), int(best[1]), int(best[2]))
+
     def set_pair_shape(self, waves, xcd):
         self._check(self._lib.sbslpm_set_pair_shape(self.h, int(waves), int(xcd)), "set_pair_shape")
 

@@ -126,6 +126,10 @@ struct sbslpm_ctx {
 
   // Launch shape of the Biot-Savart pair kernels (sbslpm_autotune).
   PairShape shape{};
+  // Launch shape and chunks of four members per wave of the multi-RHS kernel
+  // (sbslpm_autotune_ensemble); mrhs_cb = 0: untuned, `shape` and the default.
+  PairShape mrhs_shape{};
+  int mrhs_cb = 0;
 
   // hipGraph of one BVE step, keyed by the caller's pointers.
   hipGraphExec_t graph = nullptr;
@@ -728,10 +732,11 @@ int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const d
     // all members at the undisplaced positions: member 0's records give the
     // geometry, the members' charges become columns of one product
     launch_source_records_mfma(c->dt, w.qpart, w.qproxy, nullptr, nullptr, w.recmf, c->stream, 1);
-    launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, w.qcol, c->stream);
+    const int cb = c->mrhs_cb ? c->mrhs_cb : mrhs_chunks_per_wave(nmem);
+    launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, cb, w.qcol, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MRHS);
-    launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, c->prm.kernel_eps,
-                             w.phipart, w.phiproxy, c->stream, true, c->shape);
+    launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb, c->prm.kernel_eps,
+                             w.phipart, w.phiproxy, c->stream, true, c->mrhs_cb ? c->mrhs_shape : c->shape);
   } else if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records_mfma(c->dt, w.qpart, w.qproxy, dpart, dprox, w.recmf, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
@@ -1802,7 +1807,7 @@ static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
   const size_t o_r32 = A.take<SrcRec32>(c->prm.precision == SBSLPM_MIXED_VALU ? U : 0);
   const size_t o_rmf = A.take<SrcRecMfma>(c->prm.precision == SBSLPM_MIXED ? U16 : 0);
   const bool mrhs = c->prm.precision == SBSLPM_MIXED && (c->prm.flags & SBSLPM_FLAG_MULTI_RHS) && nmem > 1;
-  const size_t o_qc = A.take<float>(mrhs ? (size_t)t.U16 * 16 * mrhs_chunks(nmem) : 0);
+  const size_t o_qc = A.take<float>(mrhs ? (size_t)t.U16 * 16 * mrhs_chunks(nmem, 2) : 0);
   const size_t o_qp = A.take<double>(NP), o_qx = A.take<double>(Pn);
   const size_t o_pp = A.take<double>(3 * NP), o_px = A.take<double>(3 * Pn);
   const size_t o_dp = A.take<double>(3 * NP), o_dx = A.take<double>(3 * Pn);
@@ -1880,6 +1885,72 @@ int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* c, int nmem, const double* x, 
     std::memcpy(c->ens_key, key, sizeof(key));
   }
   HIP_TRY(c, hipGraphLaunch(c->ens_graph, c->stream));
+  return SBSLPM_OK;
+}
+
+int sbslpm_autotune_ensemble(sbslpm_ctx* c, int nmem, const double* area, const double* vor, double* report,
+                             int* ncand) {
+  if (!c || !area || !vor) return fail(c, SBSLPM_E_ARG, "autotune_ensemble: null argument");
+  if (c->capturing) return fail(c, SBSLPM_E_ARG, "autotune_ensemble: not allowed during graph capture");
+  if (c->prm.precision != SBSLPM_MIXED || !(c->prm.flags & SBSLPM_FLAG_MULTI_RHS))
+    return fail(c, SBSLPM_E_ARG, "autotune_ensemble: needs a mixed-precision context with SBSLPM_FLAG_MULTI_RHS");
+  if (nmem < 2 || nmem > 65535) return fail(c, SBSLPM_E_ARG, "autotune_ensemble: needs 2 <= members <= 65535");
+  int rc = ensure_ensemble(c, nmem);
+  if (rc) return rc;
+  const Work& w = c->wens;
+  launch_upward_fused(c->dt, area, vor, w.qpart, w.qproxy, c->stream, nmem);
+  launch_source_records_mfma(c->dt, w.qpart, w.qproxy, nullptr, nullptr, w.recmf, c->stream, 1);
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  const int waves[3] = {1, 2, 4};
+  PairShape best = c->shape;
+  int best_cb = mrhs_chunks_per_wave(nmem);
+  float best_ms = 1e30f;
+  int k = 0;
+  for (int cb = 1; cb <= 2; ++cb) {
+    launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, cb, w.qcol, c->stream);
+    for (int xcd = 0; xcd < 2; ++xcd)
+      for (int wi = 0; wi < 3; ++wi) {
+        const PairShape sh{waves[wi], xcd};
+        auto launch = [&] {
+          launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb,
+                                   c->prm.kernel_eps, w.phipart, w.phiproxy, c->stream, true, sh);
+        };
+        launch();  // warm-up
+        hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3; ++r) launch();
+        hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) break;
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= 3;
+        if (report) {
+          report[4 * k] = sh.waves;
+          report[4 * k + 1] = sh.xcd;
+          report[4 * k + 2] = cb;
+          report[4 * k + 3] = ms;
+        }
+        ++k;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = sh;
+          best_cb = cb;
+        }
+      }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ncand) *ncand = k;
+  rc = launched(c);
+  if (rc) return rc;
+  c->mrhs_shape = best;
+  c->mrhs_cb = best_cb;
+  // a captured ensemble step holds the old launch shape
+  if (c->ens_graph) {
+    hipGraphExecDestroy(c->ens_graph);
+    c->ens_graph = nullptr;
+  }
   return SBSLPM_OK;
 }
 

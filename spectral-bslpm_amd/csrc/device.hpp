@@ -191,19 +191,17 @@ void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int nti
 
 // Multi-right-hand-side MFMA pair kernel for nmem members whose sources and
 // targets share positions: one kernel matrix per wave, applied to the charge
-// columns of 4 * mrhs_chunks_per_wave members.  The columns hold
-// mrhs_chunks(nmem) chunks of four members (the last chunk group zero-padded),
-// 16 floats per record slot and chunk: U16 * 16 * mrhs_chunks(nmem) floats.
+// columns of 4 * cb members (cb = 1 or 2 chunks of four per wave).  The columns
+// hold mrhs_chunks(nmem, cb) chunks of four members (the last chunk group
+// zero-padded), 16 floats per record slot and chunk:
+// U16 * 16 * mrhs_chunks(nmem, cb) floats, at most U16 * 16 * mrhs_chunks(nmem, 2).
 // geo is member 0's SrcRecMfma array (positions only are read).
 inline int mrhs_chunks_per_wave(int nmem) { return nmem > 4 ? 2 : 1; }
-inline int mrhs_chunks(int nmem) {
-  const int cb = mrhs_chunks_per_wave(nmem);
-  return (nmem + 4 * cb - 1) / (4 * cb) * cb;
-}
+inline int mrhs_chunks(int nmem, int cb) { return (nmem + 4 * cb - 1) / (4 * cb) * cb; }
 void launch_source_columns_mrhs(const DevTree& t, const double* q_part, const double* q_proxy,
-                                const SrcRecMfma* geo, int nmem, float* qcol, hipStream_t s);
+                                const SrcRecMfma* geo, int nmem, int cb, float* qcol, hipStream_t s);
 void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                              const SrcRecMfma* geo, const float* qcol, int nmem, double eps,
+                              const SrcRecMfma* geo, const float* qcol, int nmem, int cb, double eps,
                               double* const* out_part, double* const* out_proxy, hipStream_t s, bool assign,
                               PairShape shape);
 

@@ -855,18 +855,18 @@ void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int nti
 }
 
 void launch_source_columns_mrhs(const DevTree& t, const double* q_part, const double* q_proxy,
-                                const SrcRecMfma* geo, int nmem, float* qcol, hipStream_t s) {
-  const int nchunk = mrhs_chunks(nmem);
+                                const SrcRecMfma* geo, int nmem, int cb, float* qcol, hipStream_t s) {
+  const int nchunk = mrhs_chunks(nmem, cb);
   hipLaunchKernelGGL(k_src_mrhs, dim3(blocks_for(t.U16, 256), 4 * nchunk), dim3(256), 0, s, t, q_part, q_proxy,
                      (const float*)geo, nmem, nchunk, qcol);
 }
 
 void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                              const SrcRecMfma* geo, const float* qcol, int nmem, double eps,
+                              const SrcRecMfma* geo, const float* qcol, int nmem, int cb, double eps,
                               double* const* out_part, double* const* out_proxy, hipStream_t s, bool assign,
                               PairShape shape) {
   if (ntiles == 0) return;
-  const int nchunk = mrhs_chunks(nmem), cb = mrhs_chunks_per_wave(nmem);
+  const int nchunk = mrhs_chunks(nmem, cb);
   const dim3 grid(blocks_for(ntiles, shape.waves), nchunk / cb), block(64 * shape.waves);
   if (cb == 2)
     hipLaunchKernelGGL(k_bve_mrhs<2>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,

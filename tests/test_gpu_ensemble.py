@@ -97,6 +97,33 @@ def test_ensemble_multi_rhs_first_stage(sb, torch_gpu, nmem):
     assert launches == 6  # stages 1..3 on the per-member kernel
 
 
+def test_ensemble_multi_rhs_autotune_keeps_results(sb, torch_gpu):
+    """The multi-RHS launch shape (waves, XCD mapping, chunks per wave) changes no bit."""
+    t = torch_gpu
+    print("\n[case ensemble multi_rhs autotune]", flush=True)
+    kw = dict(levels=3, interp_degree=8, leaf_degree=7, weights="clenshaw_curtis", kernel_eps=EPS, delta_t=DT,
+              omega=OMEGA, precision="mixed", multi_rhs=True)
+    plain, tuned = sb.Context(sb.RunConfig(**kw)), sb.Context(sb.RunConfig(**kw))
+    x, y, z, a = plain.spec_points()
+    gx, gy, gz, ga = [to_gpu(t, q) for q in (x, y, z, a)]
+    vors = np.stack([sb.workloads.gaussian_vortices(x, y, z, a, seed=s) for s in range(6)])
+    rows, best = tuned.autotune_ensemble(ga, to_gpu(t, vors))
+    assert len(rows) == 12 and all(r[3] > 0 for r in rows)
+    assert best[2] in (1, 2) and best[0] in (1, 2, 4)
+    out = []
+    for c in (plain, tuned):
+        v = to_gpu(t, vors)
+        vel = [t.zeros_like(v) for _ in range(3)]
+        c.bve_back_rk4_step_ensemble(gx, gy, gz, ga, v, vel)
+        c.synchronize()
+        out.append([to_np(v)] + [to_np(q) for q in vel])
+    for p, q in zip(*out):
+        np.testing.assert_array_equal(p, q)
+    c64 = sb.Context(sb.RunConfig(**dict(kw, precision="fp64")))
+    with pytest.raises(sb.SbslpmError):
+        c64.autotune_ensemble(ga, to_gpu(t, vors))
+
+
 @pytest.mark.parametrize("multi_rhs", [False, True])
 def test_ensemble_graph_replays_match_eager(sb, torch_gpu, multi_rhs):
     t = torch_gpu
