@@ -691,7 +691,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+              "(more than one GPU is out of scope: members and panels would shard with no change to the kernels)", file=sys.stderr)
         sys.exit(2)
 
     import torch

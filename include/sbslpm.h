@@ -3,7 +3,7 @@
  * and Biot-Savart problems, the direct sums, and the backward semi-Lagrangian
  * BVE step.  Library: spectral-bslpm_amd/libsbslpm.so.
  *
- * Conventions (shared with the reference operator API, SURVEY of
+ * Conventions (shared with the reference operator API: see
  * fmm_funcs.hpp:16-54, bve_vel.hpp:10-14, inverse_poisson.hpp:10-12,
  * direct_sum.hpp:8-11, bve_time_step.hpp:7-13):
  *   - All array arguments are DEVICE pointers owned by the caller, FP64,
@@ -460,6 +460,13 @@ int sbslpm_swe_tendency(sbslpm_ctx* ctx, const double* x, const double* y, const
 int sbslpm_swe_height_correction(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
                                  const double* vel_x, const double* vel_y, const double* vel_z, const double* height,
                                  const double* h_t, double dt, double* height_lap);
+/* unify_boundary_vals (initialize_cubed_sphere.cpp:364-370, functors :328-362): vals [Pl][m2]
+ * (device) is replaced, entry by entry, by the mean over all entries with the same unique index
+ * two_d_to_1d [Pl][m2] (device int32, from sbslpm_tree_dedupe); counts [n1d] (device int32) are
+ * the multiplicities.  Sums use FP64 atomics like the reference, so the order of the at most
+ * four terms is not fixed.  Entries whose index lies outside [0, n1d) are left unchanged. */
+int sbslpm_unify_boundary_vals(sbslpm_ctx* ctx, const int* two_d_to_1d, const int* counts, int64_t n1d,
+                               double* vals);
 /* swe_back_rk4_step (swe_time_step.cpp:606-811): vors, divs, height [Pl][m2] and tracers
  * [Pl][m2][tracer_count] updated in place; vel_{x,y,z} receive the stage-1 velocity and height_lap
  * the stage-1 surface Laplacian.  Variant-1 SWE kernel with the context's kernel_eps, omega,

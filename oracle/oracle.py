@@ -107,6 +107,8 @@ def lib():
         L.orc_xyz_vel_to_uv_vel.argtypes = [ctypes.c_int64] + [_dp] * 8
         L.orc_dedupe.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _ip, _ip, _dp, _dp, _dp]
         L.orc_dedupe.restype = ctypes.c_int64
+        L.orc_unify_boundary_vals.argtypes = [ctypes.c_int64, _ip, _ip, ctypes.c_int64, _dp]
+        L.orc_unify_boundary_vals.restype = ctypes.c_int
         L.orc_irreg_create.restype = ctypes.c_void_p
         L.orc_irreg_create.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.orc_irreg_destroy.argtypes = [ctypes.c_void_p]
@@ -516,6 +518,18 @@ def dedupe(x, y, z):
     o = [np.zeros(n) for _ in range(3)]
     k = lib().orc_dedupe(n, _p(x), _p(y), _p(z), m.ctypes.data_as(_ip), c.ctypes.data_as(_ip), *[_p(t) for t in o])
     return m, c[:k].copy(), o[0][:k].copy(), o[1][:k].copy(), o[2][:k].copy()
+
+
+def unify_boundary_vals(two_d_to_1d, counts, vals):
+    """unify_boundary_vals (initialize_cubed_sphere.cpp:328-370): every copy of a unique point
+    takes the mean of the copies.  Returns the new array."""
+    m = np.ascontiguousarray(two_d_to_1d, dtype=np.int32).ravel()
+    c = np.ascontiguousarray(counts, dtype=np.int32).ravel()
+    v = np.array(vals, dtype=np.float64).ravel()
+    rc = lib().orc_unify_boundary_vals(m.size, m.ctypes.data_as(_ip), c.ctypes.data_as(_ip), c.size, _p(v))
+    if rc:
+        raise ValueError("unify_boundary_vals: map index out of range")
+    return v.reshape(np.shape(vals))
 
 
 class OracleIrreg:

@@ -1231,6 +1231,20 @@ int64_t orc_dedupe(int64_t n, const double* x, const double* y, const double* z,
   return found;
 }
 
+// unify_boundary_vals (initialize_cubed_sphere.cpp:328-370): accumulate_vals sums every entry
+// into its unique point (two_d_to_1d), average_vals gives every entry the sum over its unique
+// point divided by that point's multiplicity.  Sequential sums in entry order.
+int orc_unify_boundary_vals(int64_t n, const int* two_d_to_1d, const int* counts, int64_t n1d, double* vals) {
+  std::vector<double> sum(n1d, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int k = two_d_to_1d[i];
+    if (k < 0 || k >= n1d) return ORC_E_ARG;
+    sum[k] += vals[i];
+  }
+  for (int64_t i = 0; i < n; ++i) vals[i] = sum[two_d_to_1d[i]] / counts[two_d_to_1d[i]];
+  return ORC_OK;
+}
+
 // ---- irregular particle tree and its _ll passes -------------------------------
 // initialize_cube_sphere_irreg_points (initialize_cubed_sphere.cpp:446-592),
 // dual_tree_traversal_irreg (fmm_funcs.cpp:125-231), base_pots_ll /

@@ -96,6 +96,7 @@ SIGNATURES = {
     "sbslpm_interp_points_adjoint": (_I, [_V, ctypes.c_int64, _V, _V, _V, _I, _V, _V]),
     "sbslpm_autotune": (_I, [_V, _V, _V, _PD, _PI]),
     "sbslpm_autotune_ensemble": (_I, [_V, _I, _V, _V, _PD, _PI]),
+    "sbslpm_unify_boundary_vals": (_I, [_V, _V, _V, ctypes.c_int64, _V]),
     "sbslpm_get_pair_shape": (_I, [_V, _PI, _PI]),
     "sbslpm_set_pair_shape": (_I, [_V, _I, _I]),
     "sbslpm_irreg_create": (_I, [_V, ctypes.c_int64, _PD, _PD, _PD, _I, _I, _V]),

@@ -197,6 +197,14 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _iptr(t):
+    """Device pointer of a contiguous int32 CUDA tensor."""
+    import torch
+    if not (t.is_cuda and t.dtype == torch.int32 and t.is_contiguous()):
+        raise ValueError("expected a contiguous int32 tensor on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
 class Context:
     """Device context: tree, tables, interaction lists and workspace on one GPU."""
 
@@ -483,6 +491,14 @@ class Context:
         rows = [tuple(r) for r in rep[:n.value]]
         best = min(rows, key=lambda r: r[3])
         return rows, (int(best[0]), int(best[1]), int(best[2]))
+
+    def unify_boundary_vals(self, two_d_to_1d, counts, vals):
+        """unify_boundary_vals (initialize_cubed_sphere.cpp:364-370): every copy of a unique
+        node takes the mean of its copies, in place.  two_d_to_1d [Pl, m2] and counts [n1d]
+        are int32 device tensors (from HostTree.dedupe); vals [Pl, m2] float64."""
+        self._check(self._lib.sbslpm_unify_boundary_vals(self.h, _iptr(two_d_to_1d), _iptr(counts), int(counts.numel()),
+                                                         _ptr(vals)), "unify_boundary_vals")
+        return vals
 
     def set_pair_shape(self, waves, xcd):
         self._check(self._lib.sbslpm_set_pair_shape(self.h, int(waves), int(xcd)), "set_pair_shape")

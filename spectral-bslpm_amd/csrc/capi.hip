@@ -1058,6 +1058,19 @@ int sbslpm_swe_height_correction(sbslpm_ctx* c, const double* x, const double* y
   return launched(c);
 }
 
+int sbslpm_unify_boundary_vals(sbslpm_ctx* c, const int* two_d_to_1d, const int* counts, int64_t n1d,
+                               double* vals) {
+  if (!c || !two_d_to_1d || !counts || !vals) return fail(c, SBSLPM_E_ARG, "unify_boundary_vals: null pointer");
+  const int64_t n = (int64_t)c->tree.Pl * c->tree.m2;
+  if (n1d < 1 || n1d > n) return fail(c, SBSLPM_E_ARG, "unify_boundary_vals: n1d must lie in [1, Pl*m2]");
+  if (c->capturing && n1d > c->scratch_len)
+    return fail(c, SBSLPM_E_ARG, "unify_boundary_vals: would allocate during graph capture");
+  int rc = ensure_scratch(c, n1d);
+  if (rc) return rc;
+  launch_unify_boundary_vals(n, two_d_to_1d, counts, n1d, c->scratch, vals, c->stream);
+  return launched(c);
+}
+
 static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                          double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
                          double* tracers, double* height_lap);

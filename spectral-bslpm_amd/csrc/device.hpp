@@ -322,6 +322,9 @@ void launch_apply_tendencies(int64_t n, const double* const* f, const double* co
 void launch_height_correction(const DevTree& t, const double* x, const double* y, const double* z,
                               const double* const* vel, const double* h, const double* ht, double tau, double* lap,
                               hipStream_t s);
+// Mean over the copies of every unique node; sum1d [n1d] is scratch.
+void launch_unify_boundary_vals(int64_t n, const int* map, const int* counts, int64_t n1d, double* sum1d,
+                                double* vals, hipStream_t s);
 
 // ---- direct sums (direct.hip)
 int64_t direct_partials_size(int64_t nt, int64_t ns, int ncomp);

@@ -312,4 +312,34 @@ void launch_height_correction(const DevTree& t, const double* x, const double* y
   if (t.npart > 0) dispatch_m<HeightCorr>(t.m, t, x, y, z, cp3(vel), h, ht, tau, lap, s);
 }
 
+// unify_boundary_vals (initialize_cubed_sphere.cpp:328-370): every copy of a
+// unique node (two_d_to_1d, cubed_sphere_2d_to_1d) takes the mean of its
+// copies.  As in the reference's accumulate_vals the copies are summed with
+// FP64 atomics, so the order of the (at most four) terms is not fixed.
+// Entries whose index lies outside [0, n1d) are left as they are.
+namespace {
+__global__ void k_unify_sum(int64_t n, const int* __restrict__ map, const double* __restrict__ vals, int64_t n1d,
+                            double* __restrict__ sum1d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = map[i];
+  if (k >= 0 && k < n1d) atomicAdd(sum1d + k, vals[i]);
+}
+__global__ void k_unify_mean(int64_t n, const int* __restrict__ map, const int* __restrict__ counts, int64_t n1d,
+                             const double* __restrict__ sum1d, double* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = map[i];
+  if (k >= 0 && k < n1d) vals[i] = sum1d[k] / counts[k];
+}
+}  // namespace
+
+void launch_unify_boundary_vals(int64_t n, const int* map, const int* counts, int64_t n1d, double* sum1d,
+                                double* vals, hipStream_t s) {
+  launch_fill_zero(sum1d, n1d, s);
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_unify_sum, dim3(blocks), dim3(256), 0, s, n, map, vals, n1d, sum1d);
+  hipLaunchKernelGGL(k_unify_mean, dim3(blocks), dim3(256), 0, s, n, map, counts, n1d, sum1d, vals);
+}
+
 }  // namespace sbslpm

@@ -251,3 +251,23 @@ def test_step_graph_replay_matches_eager(case, sb, torch_gpu):
         print(f"  replay {rep}: bit-identical to the eager step: {same}")
         assert same
         _ = (st[0] * 2).sum().item()  # eager work between replays
+
+
+@pytest.mark.parametrize("levels,m", [(2, 8), (3, 5)])
+def test_unify_boundary_vals_matches_oracle(sb, orc, torch_gpu, levels, m):
+    """unify_boundary_vals (initialize_cubed_sphere.cpp:364-370) on the GPU against the oracle
+    (FP64 atomics: the at most four copies may be summed in another order)."""
+    t = torch_gpu
+    print(f"\n[case unify_boundary_vals levels={levels} m={m}]", flush=True)
+    ctx = sb.Context(sb.RunConfig(levels=levels, interp_degree=8, leaf_degree=m - 1))
+    mp, cnt, *_ = sb.HostTree(sb.RunConfig(levels=levels, interp_degree=8, leaf_degree=m - 1)).dedupe()
+    vals = np.random.default_rng(7).standard_normal(mp.shape)
+    want = orc.unify_boundary_vals(mp, cnt, vals)
+    gm, gc = [t.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).cuda() for a in (mp, cnt)]
+    gv = to_gpu(t, vals)
+    for _ in range(2):  # the second call reuses the scratch and must change nothing
+        ctx.unify_boundary_vals(gm, gc, gv)
+        ctx.synchronize()
+        np.testing.assert_allclose(to_np(gv), want, rtol=1e-15, atol=1e-15)
+    with pytest.raises(sb.SbslpmError):
+        ctx.unify_boundary_vals(gm, gc[:0], gv)

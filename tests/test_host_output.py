@@ -1,4 +1,4 @@
-"""Output-side host logic (SURVEY 8(f) row 3 mirrors bve_time_evo.cpp:90-350):
+"""Output-side host logic (mirrors bve_time_evo.cpp:90-350):
 unique particle positions (cubed_sphere_2d_to_1d), 2-D to 1-D vectors, the
 lat-lon output grid and the NetCDF-classic history writer."""
 import math
