@@ -224,6 +224,41 @@ def bench_ensemble(args, torch, sb):
         c64.synchronize()
         acc[f"6x{g['side']}^2"] = float(f"{W.rel_l2(g['vors'][0].cpu().numpy(), v64.cpu().numpy()):.3e}")
         del c64
+    # pair-kernel rooflines per resolution, from HIP events on an eager context (graph replays
+    # carry no event timers): algorithmic flop = pairs per evaluation x members x 27
+    roofs = {}
+    for g in groups:
+        ce = sb.Context(sb.RunConfig(**dict(g["kw"], use_graph=False)))
+        ce.set_pair_shape(*g["ctx"].pair_shape())
+        if not args.no_multi_rhs:
+            ce.autotune_ensemble(g["xyz"][3], g["vors0"])
+        s_ = ce.sizes
+        pairs = pair_count(ce.interaction_list(), s_.P, s_.Pl, s_.n, s_.m2)
+        v, vel = g["vors0"].clone(), [torch.zeros_like(g["vors0"]) for _ in range(3)]
+        ce.enable_timing(True)
+        ce.bve_back_rk4_step_ensemble(*g["xyz"], v, vel)
+        ce.synchronize()
+        r = {}
+        for kern, name in (("bve_mrhs", "first_stage_multi_rhs"), ("bve_mixed", "per_member_kernel")):
+            ms, nl = ce.kernel_time(kern)
+            if nl == 0:
+                continue
+            achieved = pairs * nmem * FLOP_BS / (ms / nl / 1e3) / 1e12
+            r[name] = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32, unit="TFLOP/s",
+                           frac=round(achieved / PEAK_FP32, 4), avg_launch_ms=round(ms / nl, 4), launches=nl,
+                           members_per_launch=nmem, pairs_per_member=pairs, flop_per_pair=FLOP_BS)
+            if kern == "bve_mrhs":
+                # executed work: per pair and member the 16x16x4 products against four charge
+                # columns (8 flop); the kernel matrix (16x16x4 product, 8 flop, and the
+                # reciprocal) once per pair for the 4 * cb members of a wave
+                cb = mshapes.get(f"6x{g['side']}^2", [0, 0, 2])[2]
+                exe = 8.0 + 9.0 / (4 * cb)
+                ach = pairs * nmem * exe / (ms / nl / 1e3) / 1e12
+                r[name].update(executed_flop_per_pair_member=exe, achieved_executed=round(ach, 3),
+                               frac_executed=round(ach / PEAK_FP32, 4))
+        roofs[f"6x{g['side']}^2"] = r
+        del ce
+    log("c4 rooflines", roofs)
     line = {"metric": "BVE ensemble member-steps/sec, 48 members at 6x96^2, 6x128^2, 6x192^2",
             "value": round(value, 3), "unit": "member-steps/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(1e3 * sec / args.steps, 3), "higher_is_better": True,
@@ -236,7 +271,7 @@ def bench_ensemble(args, torch, sb):
             "speedup_vs_one_member_at_a_time": round(seq / (sec / args.steps), 3),
             "member0_vor_rel_l2_vs_fp64": acc,
             "pair_kernel_launch_shapes": {f"6x{g['side']}^2": list(g["ctx"].pair_shape()) for g in groups},
-            "multi_rhs_launch_shapes": mshapes}
+            "multi_rhs_launch_shapes": mshapes, "pair_kernel_rooflines": roofs}
     print(json.dumps(line), flush=True)
 
 
