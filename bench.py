@@ -167,7 +167,7 @@ def bench_ensemble(args, torch, sb):
             torch.cuda.synchronize()
             log(f"6x{side}^2: pair-kernel launch shape {shape}")
             if not args.no_multi_rhs:
-                log(f"6x{side}^2: multi-RHS launch shape (waves, xcd, chunks per wave) {mshape}")
+                log(f"6x{side}^2: multi-RHS launch shape (waves, xcd, chunks per wave, sub-tiles per pass) {mshape}")
                 mshapes[f"6x{side}^2"] = list(mshape)
         g = dict(side=side, ctx=ctx, kw=kw, xyz=[dev(v) for v in (x, y, z, a)], vors0=vors,
                  vors=vors.clone(), vel=[torch.zeros_like(vors) for _ in range(3)])
