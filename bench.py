@@ -167,7 +167,7 @@ def bench_ensemble(args, torch, sb):
             torch.cuda.synchronize()
             log(f"6x{side}^2: pair-kernel launch shape {shape}")
             if not args.no_multi_rhs:
-                log(f"6x{side}^2: multi-RHS launch shape (waves, xcd, chunks per wave, sub-tiles per pass) {mshape}")
+                log(f"6x{side}^2: multi-RHS launch shape (waves, xcd, chunks per wave) {mshape}")
                 mshapes[f"6x{side}^2"] = list(mshape)
         g = dict(side=side, ctx=ctx, kw=kw, xyz=[dev(v) for v in (x, y, z, a)], vors0=vors,
                  vors=vors.clone(), vel=[torch.zeros_like(vors) for _ in range(3)])
@@ -230,7 +230,7 @@ def bench_ensemble(args, torch, sb):
     for g in groups:
         ce = sb.Context(sb.RunConfig(**dict(g["kw"], use_graph=False)))
         ce.set_pair_shape(*g["ctx"].pair_shape())
-        if not args.no_multi_rhs:
+        if not args.no_multi_rhs and not args.no_tune:
             ce.autotune_ensemble(g["xyz"][3], g["vors0"])
         s_ = ce.sizes
         pairs = pair_count(ce.interaction_list(), s_.P, s_.Pl, s_.n, s_.m2)

@@ -348,20 +348,15 @@ int sbslpm_autotune(sbslpm_ctx* ctx, const double* area, const double* vor, doub
  * counterpart; mixed-precision context with SBSLPM_FLAG_MULTI_RHS, 2 <= nmem <= 65535): times
  * it on the context's interaction list with the charges of vor [nmem][Pl][m2] for every
  * candidate of waves per workgroup (1, 2, 4) x XCD-contiguous tile mapping (off, on) x chunks
- * of four members per wave (1, 2) x 16-target sub-tiles per pass (2, 4), and keeps the fastest
- * for later ensemble steps (a captured ensemble step is dropped and recaptured).  report
- * [ncand][5] (may be NULL) receives waves, xcd, chunks per wave, sub-tiles per pass, ms per
- * launch (24 rows).  Allocates the ensemble workspace for nmem
+ * of four members per wave (1, 2), and keeps the fastest for later ensemble steps (a captured
+ * ensemble step is dropped and recaptured).  report [ncand][4] (may be NULL) receives waves,
+ * xcd, chunks per wave, ms per launch (12 rows).  Allocates the ensemble workspace for nmem
  * members.  Results do not depend on the shape: every (target, member) is summed by one lane
  * in the same order. */
 int sbslpm_autotune_ensemble(sbslpm_ctx* ctx, int nmem, const double* area, const double* vor, double* report,
                              int* ncand);
 /* Sets the launch shape directly (waves 1, 2 or 4; xcd 0 or 1), e.g. a shape tuned earlier. */
 int sbslpm_set_pair_shape(sbslpm_ctx* ctx, int waves, int xcd);
-/* Sets the multi-right-hand-side kernel's launch shape directly (no reference counterpart):
- * waves per workgroup (1, 2, 4), XCD-contiguous mapping (0, 1), chunks of four members per wave
- * (1, 2), 16-target sub-tiles per pass (2, 4).  The result does not depend on it. */
-int sbslpm_set_multi_rhs_shape(sbslpm_ctx* ctx, int waves, int xcd, int chunks, int pass_tiles);
 int sbslpm_get_pair_shape(const sbslpm_ctx* ctx, int* waves, int* xcd);
 
 /* ---- Irregular particle tree (the lat-lon path) -------------------------

@@ -99,7 +99,6 @@ SIGNATURES = {
     "sbslpm_unify_boundary_vals": (_I, [_V, _V, _V, ctypes.c_int64, _V]),
     "sbslpm_get_pair_shape": (_I, [_V, _PI, _PI]),
     "sbslpm_set_pair_shape": (_I, [_V, _I, _I]),
-    "sbslpm_set_multi_rhs_shape": (_I, [_V, _I, _I, _I, _I]),
     "sbslpm_irreg_create": (_I, [_V, ctypes.c_int64, _PD, _PD, _PD, _I, _I, _V]),
     "sbslpm_irreg_destroy": (None, [_V]),
     "sbslpm_irreg_sizes": (_I, [_V, _PI64]),

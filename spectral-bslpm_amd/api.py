@@ -482,16 +482,15 @@ class Context:
     def autotune_ensemble(self, area, vors):
         """Times the multi-right-hand-side pair kernel of ensemble steps (mixed context with
         multi_rhs=True) over waves per workgroup x XCD mapping x chunks of four members per
-        wave x 16-target sub-tiles per pass, with the charges of vors [members][Pl][m2], and keeps
-        the fastest; returns [(waves, xcd, chunks, sub-tiles, ms per launch), ...] and the chosen
-        (waves, xcd, chunks, sub-tiles)."""
-        rep = np.zeros((24, 5))
+        wave, with the charges of vors [members][Pl][m2], and keeps the fastest; returns
+        [(waves, xcd, chunks, ms per launch), ...] and the chosen (waves, xcd, chunks)."""
+        rep = np.zeros((12, 4))
         n = ctypes.c_int(0)
         self._check(self._lib.sbslpm_autotune_ensemble(self.h, int(vors.shape[0]), _ptr(area), _ptr(vors), _dp(rep),
                                                        ctypes.byref(n)), "autotune_ensemble")
         rows = [tuple(r) for r in rep[:n.value]]
-        best = min(rows, key=lambda r: r[4])
-        return rows, (int(best[0]), int(best[1]), int(best[2]), int(best[3]))
+        best = min(rows, key=lambda r: r[3])
+        return rows, (int(best[0]), int(best[1]), int(best[2]))
 
     def unify_boundary_vals(self, two_d_to_1d, counts, vals):
         """unify_boundary_vals (initialize_cubed_sphere.cpp:364-370): every copy of a unique
@@ -500,10 +499,6 @@ class Context:
         self._check(self._lib.sbslpm_unify_boundary_vals(self.h, _iptr(two_d_to_1d), _iptr(counts), int(counts.numel()),
                                                          _ptr(vals)), "unify_boundary_vals")
         return vals
-
-    def set_multi_rhs_shape(self, waves, xcd, chunks, pass_tiles):
-        self._check(self._lib.sbslpm_set_multi_rhs_shape(self.h, int(waves), int(xcd), int(chunks), int(pass_tiles)),
-                    "set_multi_rhs_shape")
 
     def set_pair_shape(self, waves, xcd):
         self._check(self._lib.sbslpm_set_pair_shape(self.h, int(waves), int(xcd)), "set_pair_shape")

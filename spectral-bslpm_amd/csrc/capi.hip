@@ -130,7 +130,6 @@ struct sbslpm_ctx {
   // (sbslpm_autotune_ensemble); mrhs_cb = 0: untuned, `shape` and the default.
   PairShape mrhs_shape{};
   int mrhs_cb = 0;
-  int mrhs_pass = 4;  // 16-target sub-tiles per pass
 
   // hipGraph of one BVE step, keyed by the caller's pointers.
   hipGraphExec_t graph = nullptr;
@@ -736,9 +735,8 @@ int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const d
     const int cb = c->mrhs_cb ? c->mrhs_cb : mrhs_chunks_per_wave(nmem);
     launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, cb, w.qcol, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MRHS);
-    launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb, c->mrhs_pass,
-                             c->prm.kernel_eps, w.phipart, w.phiproxy, c->stream, true,
-                             c->mrhs_cb ? c->mrhs_shape : c->shape);
+    launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb, c->prm.kernel_eps,
+                             w.phipart, w.phiproxy, c->stream, true, c->mrhs_cb ? c->mrhs_shape : c->shape);
   } else if (c->prm.precision == SBSLPM_MIXED) {
     launch_source_records_mfma(c->dt, w.qpart, w.qproxy, dpart, dprox, w.recmf, c->stream, nmem);
     KernelTimer tm(c, SBSLPM_KERNEL_BVE_MIXED);
@@ -1269,23 +1267,6 @@ int sbslpm_autotune(sbslpm_ctx* c, const double* area, const double* vor, double
       hipGraphExecDestroy(c->ens_graph);
       c->ens_graph = nullptr;
     }
-  }
-  return SBSLPM_OK;
-}
-
-int sbslpm_set_multi_rhs_shape(sbslpm_ctx* c, int waves, int xcd, int chunks, int pass_tiles) {
-  if (!c || !(waves == 1 || waves == 2 || waves == 4) || !(xcd == 0 || xcd == 1) || !(chunks == 1 || chunks == 2) ||
-      !(pass_tiles == 2 || pass_tiles == 4))
-    return fail(c, SBSLPM_E_ARG,
-                "set_multi_rhs_shape: waves 1, 2 or 4; xcd 0 or 1; chunks 1 or 2; pass_tiles 2 or 4");
-  if (c->capturing) return fail(c, SBSLPM_E_ARG, "set_multi_rhs_shape: not allowed during graph capture");
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  c->mrhs_shape = PairShape{waves, xcd};
-  c->mrhs_cb = chunks;
-  c->mrhs_pass = pass_tiles;
-  if (c->ens_graph) {  // a captured ensemble step holds the old launch shape
-    hipGraphExecDestroy(c->ens_graph);
-    c->ens_graph = nullptr;
   }
   return SBSLPM_OK;
 }
@@ -1937,17 +1918,16 @@ int sbslpm_autotune_ensemble(sbslpm_ctx* c, int nmem, const double* area, const 
   HIP_TRY(c, hipEventCreate(&e1));
   const int waves[3] = {1, 2, 4};
   PairShape best = c->shape;
-  int best_cb = mrhs_chunks_per_wave(nmem), best_pass = 4;
+  int best_cb = mrhs_chunks_per_wave(nmem);
   float best_ms = 1e30f;
   int k = 0;
   for (int cb = 1; cb <= 2; ++cb) {
     launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, cb, w.qcol, c->stream);
-    for (int pass = 2; pass <= 4; pass += 2)
     for (int xcd = 0; xcd < 2; ++xcd)
       for (int wi = 0; wi < 3; ++wi) {
         const PairShape sh{waves[wi], xcd};
         auto launch = [&] {
-          launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb, pass,
+          launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb,
                                    c->prm.kernel_eps, w.phipart, w.phiproxy, c->stream, true, sh);
         };
         launch();  // warm-up
@@ -1959,18 +1939,16 @@ int sbslpm_autotune_ensemble(sbslpm_ctx* c, int nmem, const double* area, const 
         hipEventElapsedTime(&ms, e0, e1);
         ms /= 3;
         if (report) {
-          report[5 * k] = sh.waves;
-          report[5 * k + 1] = sh.xcd;
-          report[5 * k + 2] = cb;
-          report[5 * k + 3] = pass;
-          report[5 * k + 4] = ms;
+          report[4 * k] = sh.waves;
+          report[4 * k + 1] = sh.xcd;
+          report[4 * k + 2] = cb;
+          report[4 * k + 3] = ms;
         }
         ++k;
         if (ms < best_ms) {
           best_ms = ms;
           best = sh;
           best_cb = cb;
-          best_pass = pass;
         }
       }
   }
@@ -1981,7 +1959,6 @@ int sbslpm_autotune_ensemble(sbslpm_ctx* c, int nmem, const double* area, const 
   if (rc) return rc;
   c->mrhs_shape = best;
   c->mrhs_cb = best_cb;
-  c->mrhs_pass = best_pass;
   // a captured ensemble step holds the old launch shape
   if (c->ens_graph) {
     hipGraphExecDestroy(c->ens_graph);

@@ -200,11 +200,10 @@ inline int mrhs_chunks_per_wave(int nmem) { return nmem > 4 ? 2 : 1; }
 inline int mrhs_chunks(int nmem, int cb) { return (nmem + 4 * cb - 1) / (4 * cb) * cb; }
 void launch_source_columns_mrhs(const DevTree& t, const double* q_part, const double* q_proxy,
                                 const SrcRecMfma* geo, int nmem, int cb, float* qcol, hipStream_t s);
-// pass_tiles (2 or 4): 16-target sub-tiles of a 64-target tile per pass.
 void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                              const SrcRecMfma* geo, const float* qcol, int nmem, int cb, int pass_tiles,
-                              double eps, double* const* out_part, double* const* out_proxy, hipStream_t s,
-                              bool assign, PairShape shape);
+                              const SrcRecMfma* geo, const float* qcol, int nmem, int cb, double eps,
+                              double* const* out_part, double* const* out_proxy, hipStream_t s, bool assign,
+                              PairShape shape);
 
 // ---- shallow-water pair sums (swe.hip); variant 1 or 2, 3 = the lat-lon form
 // (swe_vel_impl.hpp:265-276: variant 1's kernel with the divergence term's sign reversed)

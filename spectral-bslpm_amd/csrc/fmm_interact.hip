@@ -612,10 +612,10 @@ __device__ inline double quad_bcast(double v) {
 // accumulates, per source set, w[tt][cb][r] += W_c - p_c W_q for target
 // tt*16 + 4 (l/16) + r, member 4 (chunk0 + cb) + (l%16)/4, component c = l % 4
 // (lanes with c = 3 carry W_q and are discarded).
-template <int NTT, int CB, int WN>
+template <int NTT, int CB>
 __device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, const SourceSet* __restrict__ sets,
                                       const float* __restrict__ geo, const float* __restrict__ qcol, int nchunk,
-                                      int chunk0, float eps, int lane, int tt0, double (&w)[WN][CB][4]) {
+                                      int chunk0, float eps, int lane, double (&w)[4][CB][4]) {
   const int col = lane & 15, grp = lane >> 4, comp = lane & 3;
   const DevPanel& PT = t.panels[T.panel];
   const double* __restrict__ pc_arr = comp == 0 ? t.px : comp == 1 ? t.py : t.pz;
@@ -623,14 +623,14 @@ __device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, con
   float ax[NTT], ay[NTT], az[NTT], ac[NTT][4];
 #pragma unroll
   for (int tt = 0; tt < NTT; ++tt) {
-    const int j = min((tt0 + tt) * 16 + col, T.count - 1);
+    const int j = min(tt * 16 + col, T.count - 1);
     const int64_t u = (int64_t)T.first + j;
     ax[tt] = (float)(t.px[u] - PT.cx);
     ay[tt] = (float)(t.py[u] - PT.cy);
     az[tt] = (float)(t.pz[u] - PT.cz);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int jj = min((tt0 + tt) * 16 + 4 * grp + r, T.count - 1);
+      const int jj = min(tt * 16 + 4 * grp + r, T.count - 1);
       ac[tt][r] = comp == 3 ? 0.f : (float)(pc_arr[(int64_t)T.first + jj] - pc_ctr);
     }
   }
@@ -709,11 +709,9 @@ __device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, con
   }
 }
 
-// One wave per target tile and group of CB member chunks (blockIdx.y).  The
-// tile's 16-target sub-tiles are taken NTMAX at a time (NTMAX = 2 halves the
-// accumulator registers and reads the source columns once more).  Every
+// One wave per target tile and group of CB member chunks (blockIdx.y).  Every
 // (target, member, component) is written by exactly one lane.
-template <int CB, int NTMAX>
+template <int CB>
 __global__ __launch_bounds__(256) void k_bve_mrhs(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                   const SourceSet* __restrict__ sets, const float* __restrict__ geo,
                                                   const float* __restrict__ qcol, int nchunk, int nmem, float eps,
@@ -726,46 +724,41 @@ __global__ __launch_bounds__(256) void k_bve_mrhs(DevTree t, const TargetTile* _
   const TargetTile T = tiles[tile];
   const int ntt = (T.count + 15) >> 4;
   const int chunk0 = (int)blockIdx.y * CB;
-  for (int tt0 = 0; tt0 < ntt; tt0 += NTMAX) {
-    const int nt = min(NTMAX, ntt - tt0);
-    double w[NTMAX][CB][4];
+  double w[4][CB][4];
 #pragma unroll
-    for (int tt = 0; tt < NTMAX; ++tt)
+  for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb)
+    for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) w[tt][cb][r] = 0.0;
-    if (NTMAX == 4 && nt == 4)
-      bve_mrhs_tiles<(NTMAX >= 4 ? 4 : 1), CB, NTMAX>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, tt0, w);
-    else if (NTMAX == 4 && nt == 3)
-      bve_mrhs_tiles<(NTMAX >= 4 ? 3 : 1), CB, NTMAX>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, tt0, w);
-    else if (nt == 2)
-      bve_mrhs_tiles<2, CB, NTMAX>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, tt0, w);
-    else
-      bve_mrhs_tiles<1, CB, NTMAX>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, tt0, w);
-    // u = x cross w: the three components of one (target, member) sit in one quad
+      for (int r = 0; r < 4; ++r) w[tt][cb][r] = 0.0;
+  switch (ntt) {
+    case 4: bve_mrhs_tiles<4, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 3: bve_mrhs_tiles<3, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 2: bve_mrhs_tiles<2, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    default: bve_mrhs_tiles<1, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+  }
+  // u = x cross w: the three components of one (target, member) sit in one quad
 #pragma unroll
-    for (int tt = 0; tt < NTMAX; ++tt) {
-      if (tt >= nt) break;
+  for (int tt = 0; tt < 4; ++tt) {
+    if (tt >= ntt) break;
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb)
+    for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double w0 = quad_bcast<0>(w[tt][cb][r]), w1 = quad_bcast<1>(w[tt][cb][r]),
-                       w2 = quad_bcast<2>(w[tt][cb][r]);
-          const int j = (tt0 + tt) * 16 + 4 * grp + r;
-          const int mem = 4 * (chunk0 + cb) + (col >> 2);
-          if (comp < 3 && j < T.count && mem < nmem) {
-            const int64_t u = (int64_t)T.first + j;
-            const double x = t.px[u], y = t.py[u], z = t.pz[u];
-            const double val = comp == 0 ? y * w2 - z * w1 : comp == 1 ? z * w0 - x * w2 : x * w1 - y * w0;
-            double* o = u < t.npart ? part.p[comp] + (int64_t)mem * t.npart + u
-                                    : proxy.p[comp] + (int64_t)mem * t.P * t.n + (u - t.npart);
-            if (assign) *o = val;
-            else *o += val;
-          }
+      for (int r = 0; r < 4; ++r) {
+        const double w0 = quad_bcast<0>(w[tt][cb][r]), w1 = quad_bcast<1>(w[tt][cb][r]),
+                     w2 = quad_bcast<2>(w[tt][cb][r]);
+        const int j = tt * 16 + 4 * grp + r;
+        const int mem = 4 * (chunk0 + cb) + (col >> 2);
+        if (comp < 3 && j < T.count && mem < nmem) {
+          const int64_t u = (int64_t)T.first + j;
+          const double x = t.px[u], y = t.py[u], z = t.pz[u];
+          const double val = comp == 0 ? y * w2 - z * w1 : comp == 1 ? z * w0 - x * w2 : x * w1 - y * w0;
+          double* o = u < t.npart ? part.p[comp] + (int64_t)mem * t.npart + u
+                                  : proxy.p[comp] + (int64_t)mem * t.P * t.n + (u - t.npart);
+          if (assign) *o = val;
+          else *o += val;
         }
-    }
+      }
   }
 }
 
@@ -869,20 +862,18 @@ void launch_source_columns_mrhs(const DevTree& t, const double* q_part, const do
 }
 
 void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
-                              const SrcRecMfma* geo, const float* qcol, int nmem, int cb, int pass_tiles,
-                              double eps, double* const* out_part, double* const* out_proxy, hipStream_t s,
-                              bool assign, PairShape shape) {
+                              const SrcRecMfma* geo, const float* qcol, int nmem, int cb, double eps,
+                              double* const* out_part, double* const* out_proxy, hipStream_t s, bool assign,
+                              PairShape shape) {
   if (ntiles == 0) return;
   const int nchunk = mrhs_chunks(nmem, cb);
   const dim3 grid(blocks_for(ntiles, shape.waves), nchunk / cb), block(64 * shape.waves);
-  auto go = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk, nmem,
-                       (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
-  };
   if (cb == 2)
-    pass_tiles == 2 ? go(k_bve_mrhs<2, 2>) : go(k_bve_mrhs<2, 4>);
+    hipLaunchKernelGGL(k_bve_mrhs<2>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
+                       nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
   else
-    pass_tiles == 2 ? go(k_bve_mrhs<1, 2>) : go(k_bve_mrhs<1, 4>);
+    hipLaunchKernelGGL(k_bve_mrhs<1>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
+                       nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 
 void launch_poisson_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
