@@ -247,11 +247,11 @@ int sbslpm_bve_velocity_targets(sbslpm_ctx* ctx, const double* area, const doubl
 int sbslpm_direct_sum_inv_lap(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                               const double* area, const double* pots, double* soln, int64_t lb, int64_t ub);
 /* Log direct sum at nt separate targets over ns sources; pairs with x.y >= 1 - 1e-15 skipped
- * (the FMM kernel's guard, inverse_poisson_impl.hpp:205).  soln [nt] accumulated. */
+ * (the FMM kernel's guard, inverse_poisson_impl.hpp:47).  soln [nt] accumulated. */
 int sbslpm_direct_sum_inv_lap_targets(sbslpm_ctx* ctx, int64_t nt, const double* tx, const double* ty,
                                       const double* tz, int64_t ns, const double* sx, const double* sy,
                                       const double* sz, const double* area, const double* pots, double* soln);
-/* Biot-Savart direct sum (no reference counterpart; pair kernel of bve_vel_impl.hpp:90-103):
+/* Biot-Savart direct sum (no reference counterpart; pair kernel of bve_vel_impl.hpp:54-66):
  * vel[i] += sum_j -1/(4 pi) area_j vor_j (x_i cross y_j)/(1 - x_i.y_j + eps), y_j = P(x_j + disp_j),
  * for i in [lb, ub); disp may be NULL.  Uses the context's eps. */
 int sbslpm_direct_sum_bve_vel(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
@@ -303,7 +303,7 @@ int sbslpm_bve_back_rk4_step_ensemble(sbslpm_ctx* ctx, int nmem, const double* x
  * to the FMM truncation.  On a mixed-precision context the transposed pair sums run in FP32
  * (panel-centred |x - y|^2 / 2 form, FP64 accumulation across interactions), the passes in
  * FP64.  No reference counterpart; it transposes the operator of bve_vel_interactions
- * (bve_vel.cpp:9-37, bve_vel_impl.hpp:71-106) with the passes of fmm_funcs.cpp:341-392. */
+ * (bve_vel.cpp:9-37, bve_vel_impl.hpp:34-70) with the passes of fmm_funcs.cpp:341-392. */
 int sbslpm_bve_velocity_adjoint(sbslpm_ctx* ctx, const double* area, const double* ubar_x, const double* ubar_y,
                                 const double* ubar_z, double* vor_bar);
 /* Gradient of one bve_back_rk4_step (bve_time_step.cpp:311-419, no tracers) with respect to the
@@ -392,7 +392,7 @@ int sbslpm_irreg_swe_vel_interactions(sbslpm_irreg* h, double* target_pots_1, do
 int sbslpm_irreg_swe_velocity(sbslpm_irreg* h, const double* area, const double* vors, const double* divs,
                               double* vel_x, double* vel_y, double* vel_z);
 /* Biot-Savart velocity of a vorticity field on the irregular tree (the kernel of
- * bve_vel_impl.hpp:90-103 with undisplaced sources); vel assigned.  No reference counterpart
+ * bve_vel_impl.hpp:54-66 with undisplaced sources); vel assigned.  No reference counterpart
  * on this tree. */
 int sbslpm_irreg_bve_velocity(sbslpm_irreg* h, const double* area, const double* vor, double* vel_x, double* vel_y,
                               double* vel_z);

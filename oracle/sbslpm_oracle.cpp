@@ -410,7 +410,7 @@ void dual_tree_traversal(Ctx& c) {
 }
 
 // Panel label-space scale/offset in radians, as the kernels compute them
-// (bve_vel_impl.hpp:75-83).
+// (bve_vel_impl.hpp:38-46).
 inline void panel_frame(const Panel& p, double& xo, double& xs, double& eo, double& es) {
   xo = kPi / 4 * 0.5 * (p.max_xi + p.min_xi);
   xs = kPi / 4 * 0.5 * (p.max_xi - p.min_xi);
@@ -508,7 +508,7 @@ void orc_spec_points(void* h, double* x, double* y, double* z, double* area) {
 }
 
 // Proxy node positions of every panel, [P][n] (the coordinates the pair
-// kernels recompute per pair, bve_vel_impl.hpp:86-93).
+// kernels recompute per pair, bve_vel_impl.hpp:50-56).
 void orc_proxy_points(void* h, double* x, double* y, double* z) {
   Ctx& c = *static_cast<Ctx*>(h);
   for (int i = 0; i < c.P; i++) {
@@ -652,9 +652,9 @@ static inline Side side_of(const Ctx& c, int panel, bool particles) {
   return s;
 }
 
-// fmm_interactions/bve_vel.cpp:9-37 + bve_vel_impl.hpp:71-106, with the
+// fmm_interactions/bve_vel.cpp:9-37 + bve_vel_impl.hpp:34-70, with the
 // type-dependent point sets of swe_vel_impl.hpp:188-255.  Sources sit at
-// P(node + disp) (bve_vel_impl.hpp:90-97); targets are undisplaced.
+// P(node + disp) (bve_vel_impl.hpp:54-60); targets are undisplaced.
 // phi_part[3][Pl][m2] and phi_proxy[3][P][n] are accumulated.
 // Target panels t with t % stride == 0 only (stride 1: all); used to time a
 // bounded sample of the interaction phase.
@@ -792,7 +792,7 @@ void orc_swe_vel_interactions(void* h, int variant, const double* const* disp_pa
   }
 }
 
-// fmm_interactions/inverse_poisson.cpp:9-33 + inverse_poisson_impl.hpp:181-210,
+// fmm_interactions/inverse_poisson.cpp:9-33 + inverse_poisson_impl.hpp:23-52,
 // with the type-dependent point sets.  psi_part[Pl][m2], psi_proxy[P][n] accumulated.
 void orc_poisson_fmm_interactions(void* h, const double* q_part, const double* q_proxy,
                                   double* psi_part, double* psi_proxy) {
@@ -842,7 +842,7 @@ void orc_direct_sum_inv_lap(int64_t N, const double* x, const double* y, const d
   }
 }
 
-// Biot-Savart direct sum with the pair kernel of bve_vel_impl.hpp:90-103 (no
+// Biot-Savart direct sum with the pair kernel of bve_vel_impl.hpp:54-66 (no
 // reference counterpart: the reference has only the log direct sum).  Sources
 // optionally displaced (disp may be null); vel[3][N] accumulated.
 void orc_direct_sum_bve_vel(int64_t N, const double* x, const double* y, const double* z,
@@ -911,7 +911,7 @@ void orc_direct_sum_bve_vel_targets(int64_t NT, const double* tx, const double* 
 //   near: the kernel over the sources of L's own interactions, types 0 and 1
 //         (the particle-target point sets of swe_vel_impl.hpp:188-255).
 // kind 0: Poisson, -1/(4 pi) log(1 - x.y) with the dp < 1 - 1e-15 guard on
-// unprojected nodes (inverse_poisson_impl.hpp:47-48, 199-204); out[0][NT].
+// unprojected nodes (inverse_poisson_impl.hpp:42-48); out[0][NT].
 // kind 1: Biot-Savart with undisplaced, re-projected sources
 // (bve_vel_impl.hpp:54-66); out[0..2][NT].
 int orc_eval_targets(void* h, int kind, const double* area, const double* f, double eps, int64_t NT,

@@ -187,22 +187,58 @@ class HostIrregTree:
         return out
 
 
-def _ptr(t):
-    """Device pointer of a contiguous float64 CUDA tensor (None -> NULL)."""
+def _on_gpu(t):
+    return t.is_cuda
+
+
+def _ptr(t, count=None, what="array"):
+    """Device pointer of a contiguous float64 GPU tensor (None -> NULL).
+
+    With `count` the tensor must hold exactly that many elements.  Anything
+    else (a host tensor, another dtype, a strided view, a short array) raises
+    ValueError before the library sees the pointer, so no kernel reads or
+    writes outside a caller's buffer.
+    """
     if t is None:
         return None
     import torch
-    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
-        raise ValueError("expected a contiguous float64 tensor on the GPU")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if count is not None and t.numel() != count:
+        raise ValueError(f"{what}: expected {count} elements, got {t.numel()} (shape {tuple(t.shape)})")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{what}: expected float64, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous tensor")
+    if not _on_gpu(t):
+        raise ValueError(f"{what}: expected a tensor on the GPU, got {t.device}")
     return ctypes.c_void_p(t.data_ptr())
 
 
-def _iptr(t):
-    """Device pointer of a contiguous int32 CUDA tensor."""
+def _iptr(t, count=None, what="index array"):
+    """Device pointer of a contiguous int32 GPU tensor, checked like _ptr."""
     import torch
-    if not (t.is_cuda and t.dtype == torch.int32 and t.is_contiguous()):
-        raise ValueError("expected a contiguous int32 tensor on the GPU")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if count is not None and t.numel() != count:
+        raise ValueError(f"{what}: expected {count} elements, got {t.numel()} (shape {tuple(t.shape)})")
+    if not (t.dtype == torch.int32 and t.is_contiguous() and _on_gpu(t)):
+        raise ValueError(f"{what}: expected a contiguous int32 tensor on the GPU")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _seq(ts, k, what):
+    """ts as a list of exactly k entries (tensors or None)."""
+    ts = list(ts)
+    if len(ts) != k:
+        raise ValueError(f"{what}: expected {k} arrays, got {len(ts)}")
+    return ts
+
+
+def _ptr_array(ptrs):
+    """A C array of device pointers (const double* const*) as one void* argument."""
+    arr = (ctypes.c_void_p * len(ptrs))(*[p.value for p in ptrs])
+    return arr, ctypes.cast(arr, ctypes.c_void_p)
 
 
 class Context:
@@ -236,6 +272,31 @@ class Context:
         if rc:
             msg = self._lib.sbslpm_last_error(self.h)
             raise SbslpmError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # Argument checks.  Particle arrays hold Pl * m2 values per member, proxy
+    # arrays P * n; every pointer handed to the library goes through one of these.
+    def _p(self, t, k=1, what="particle array"):
+        return _ptr(t, k * self.sizes.Pl * self.sizes.m2, what)
+
+    def _x(self, t, what="proxy array"):
+        return _ptr(t, self.sizes.P * self.sizes.n, what)
+
+    def _p3(self, ts, what, k=1):
+        return [self._p(t, k, what) for t in _seq(ts, 3, what)]
+
+    def _x3(self, ts, what):
+        return [self._x(t, what) for t in _seq(ts, 3, what)]
+
+    def _opt3(self, ts, what, proxy=False):
+        """Optional three-component array (displacements): None or three tensors."""
+        if ts is None:
+            return [None] * 3
+        return self._x3(ts, what) if proxy else self._p3(ts, what)
+
+    def _stages(self, vels, what):
+        """The 12 stage velocities [v0, v1, v2, v3] (three tensors each) as a pointer array."""
+        flat = [t for stage in _seq(vels, 4, what) for t in _seq(stage, 3, what)]
+        return _ptr_array([self._p(t, what=what) for t in flat])
 
     def set_stream(self, stream):
         self.stream = stream
@@ -290,7 +351,8 @@ class Context:
         """upward_pass, fmm_funcs.cpp:341-355."""
         q_part = self.zeros_part() if q_part is None else q_part
         q_proxy = self.zeros_proxy() if q_proxy is None else q_proxy
-        self._check(self._lib.sbslpm_upward_pass(self.h, _ptr(area), _ptr(pots), _ptr(q_part), _ptr(q_proxy)),
+        self._check(self._lib.sbslpm_upward_pass(self.h, self._p(area, what="area"), self._p(pots, what="pots"),
+                                                 self._p(q_part, what="q_part"), self._x(q_proxy, what="q_proxy")),
                     "upward_pass")
         return q_part, q_proxy
 
@@ -299,9 +361,9 @@ class Context:
         """bve_vel_interactions, bve_vel.cpp:9-37 (accumulates)."""
         phi_part = self.zeros_part(3) if phi_part is None else phi_part
         phi_proxy = self.zeros_proxy(3) if phi_proxy is None else phi_proxy
-        dp = disp_part if disp_part is not None else [None] * 3
-        dx = disp_proxy if disp_proxy is not None else [None] * 3
-        args = [_ptr(a) for a in list(dp) + list(dx) + [q_part, q_proxy] + list(phi_part) + list(phi_proxy)]
+        args = (self._opt3(disp_part, "disp_part") + self._opt3(disp_proxy, "disp_proxy", proxy=True)
+                + [self._p(q_part, what="q_part"), self._x(q_proxy, what="q_proxy")]
+                + self._p3(phi_part, "phi_part") + self._x3(phi_proxy, "phi_proxy"))
         self._check(self._lib.sbslpm_bve_vel_interactions(self.h, *args), "bve_vel_interactions")
         return phi_part, phi_proxy
 
@@ -311,10 +373,10 @@ class Context:
         swe_vel.cpp:37-67); accumulates."""
         phi_part = self.zeros_part(3) if phi_part is None else phi_part
         phi_proxy = self.zeros_proxy(3) if phi_proxy is None else phi_proxy
-        dp = disp_part if disp_part is not None else [None] * 3
-        dx = disp_proxy if disp_proxy is not None else [None] * 3
-        args = [_ptr(a) for a in list(dp) + list(dx) + [qz_part, qz_proxy, qd_part, qd_proxy] + list(phi_part)
-                + list(phi_proxy)]
+        args = (self._opt3(disp_part, "disp_part") + self._opt3(disp_proxy, "disp_proxy", proxy=True)
+                + [self._p(qz_part, what="qz_part"), self._x(qz_proxy, what="qz_proxy"),
+                   self._p(qd_part, what="qd_part"), self._x(qd_proxy, what="qd_proxy")]
+                + self._p3(phi_part, "phi_part") + self._x3(phi_proxy, "phi_proxy"))
         fn = self._lib.sbslpm_swe_vel_interactions if variant == 1 else self._lib.sbslpm_swe_vel_interactions_2
         self._check(fn(self.h, *args), "swe_vel_interactions")
         return phi_part, phi_proxy
@@ -322,9 +384,9 @@ class Context:
     def swe_velocity(self, area, vor, div, disp_part=None, disp_proxy=None, vel=None, variant=1):
         """Upward passes of vor and div + SWE interactions + downward pass (swe_time_step.cpp:640-653)."""
         vel = self.zeros_part(3) if vel is None else vel
-        dp = disp_part if disp_part is not None else [None] * 3
-        dx = disp_proxy if disp_proxy is not None else [None] * 3
-        args = [_ptr(a) for a in [area, vor, div] + list(dp) + list(dx) + list(vel)]
+        args = ([self._p(area, what="area"), self._p(vor, what="vor"), self._p(div, what="div")]
+                + self._opt3(disp_part, "disp_part") + self._opt3(disp_proxy, "disp_proxy", proxy=True)
+                + self._p3(vel, "vel"))
         self._check(self._lib.sbslpm_swe_velocity(self.h, variant, *args), "swe_velocity")
         return vel
 
@@ -332,38 +394,39 @@ class Context:
         """poisson_fmm_interactions, inverse_poisson.cpp:9-33 (accumulates)."""
         psi_part = self.zeros_part() if psi_part is None else psi_part
         psi_proxy = self.zeros_proxy() if psi_proxy is None else psi_proxy
-        self._check(self._lib.sbslpm_poisson_fmm_interactions(self.h, _ptr(q_part), _ptr(q_proxy), _ptr(psi_part),
-                                                              _ptr(psi_proxy)), "poisson_fmm_interactions")
+        self._check(self._lib.sbslpm_poisson_fmm_interactions(
+            self.h, self._p(q_part, what="q_part"), self._x(q_proxy, what="q_proxy"),
+            self._p(psi_part, what="psi_part"), self._x(psi_proxy, what="psi_proxy")), "poisson_fmm_interactions")
         return psi_part, psi_proxy
 
     def downward_pass(self, phi_proxy, phi_part=None, sols=None):
         """downward_pass, fmm_funcs.cpp:357-372."""
         sols = self.zeros_part() if sols is None else sols
-        self._check(self._lib.sbslpm_downward_pass(self.h, _ptr(phi_proxy), _ptr(phi_part), _ptr(sols)),
+        self._check(self._lib.sbslpm_downward_pass(self.h, self._x(phi_proxy, what="phi_proxy"),
+                                                   self._p(phi_part, what="phi_part"), self._p(sols, what="sols")),
                     "downward_pass")
         return sols
 
     def downward_pass_3(self, phi_proxy, phi_part=None, sols=None):
         """downward_pass_3, fmm_funcs.cpp:374-392."""
         sols = self.zeros_part(3) if sols is None else sols
-        pp = phi_part if phi_part is not None else [None] * 3
-        args = [_ptr(a) for a in list(phi_proxy) + list(pp) + list(sols)]
+        args = self._x3(phi_proxy, "phi_proxy") + self._opt3(phi_part, "phi_part") + self._p3(sols, "sols")
         self._check(self._lib.sbslpm_downward_pass_3(self.h, *args), "downward_pass_3")
         return sols
 
     def bve_velocity(self, area, vor, disp_part=None, disp_proxy=None, vel=None):
         """Upward pass + interactions + downward pass (bve_time_step.cpp:321-344)."""
         vel = self.zeros_part(3) if vel is None else vel
-        dp = disp_part if disp_part is not None else [None] * 3
-        dx = disp_proxy if disp_proxy is not None else [None] * 3
-        args = [_ptr(a) for a in [area, vor] + list(dp) + list(dx) + list(vel)]
+        args = ([self._p(area, what="area"), self._p(vor, what="vor")] + self._opt3(disp_part, "disp_part")
+                + self._opt3(disp_proxy, "disp_proxy", proxy=True) + self._p3(vel, "vel"))
         self._check(self._lib.sbslpm_bve_velocity(self.h, *args), "bve_velocity")
         return vel
 
     def poisson_solve(self, area, f, psi=None):
         """poisson_eq_fmm.cpp:129-147."""
         psi = self.zeros_part() if psi is None else psi
-        self._check(self._lib.sbslpm_poisson_solve(self.h, _ptr(area), _ptr(f), _ptr(psi)), "poisson_solve")
+        self._check(self._lib.sbslpm_poisson_solve(self.h, self._p(area, what="area"), self._p(f, what="f"),
+                                                   self._p(psi, what="psi")), "poisson_solve")
         return psi
 
     def poisson_solve_targets(self, area, f, tx, ty, tz, psi=None):
@@ -372,16 +435,19 @@ class Context:
         plus the leaf's near field."""
         nt = tx.numel()
         psi = self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") if psi is None else psi
-        self._check(self._lib.sbslpm_poisson_solve_targets(self.h, _ptr(area), _ptr(f), nt, _ptr(tx), _ptr(ty),
-                                                           _ptr(tz), _ptr(psi)), "poisson_solve_targets")
+        t3 = [_ptr(t, nt, "targets") for t in (tx, ty, tz)]
+        self._check(self._lib.sbslpm_poisson_solve_targets(self.h, self._p(area, what="area"), self._p(f, what="f"),
+                                                           nt, *t3, _ptr(psi, nt, "psi")), "poisson_solve_targets")
         return psi
 
     def bve_velocity_targets(self, area, vor, tx, ty, tz, vel=None):
         """Biot-Savart velocity at arbitrary unit-sphere targets (1-D arrays)."""
         nt = tx.numel()
         vel = [self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") for _ in range(3)] if vel is None else vel
-        self._check(self._lib.sbslpm_bve_velocity_targets(self.h, _ptr(area), _ptr(vor), nt, _ptr(tx), _ptr(ty),
-                                                          _ptr(tz), *[_ptr(v) for v in vel]), "bve_velocity_targets")
+        t3 = [_ptr(t, nt, "targets") for t in (tx, ty, tz)]
+        v3 = [_ptr(v, nt, "vel") for v in _seq(vel, 3, "vel")]
+        self._check(self._lib.sbslpm_bve_velocity_targets(self.h, self._p(area, what="area"), self._p(vor, what="vor"),
+                                                          nt, *t3, *v3), "bve_velocity_targets")
         return vel
 
     def direct_sum_inv_lap(self, x, y, z, area, pots, soln=None, lb=0, ub=None):
@@ -389,27 +455,28 @@ class Context:
         n = x.numel()
         ub = n if ub is None else ub
         soln = self.torch.zeros(n, dtype=self.torch.float64, device="cuda") if soln is None else soln
-        self._check(self._lib.sbslpm_direct_sum_inv_lap(self.h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(area),
-                                                        _ptr(pots), _ptr(soln), lb, ub), "direct_sum_inv_lap")
+        a = [_ptr(t, n, "direct_sum_inv_lap input") for t in (x, y, z, area, pots, soln)]
+        self._check(self._lib.sbslpm_direct_sum_inv_lap(self.h, n, *a, lb, ub), "direct_sum_inv_lap")
         return soln
 
     def direct_sum_inv_lap_targets(self, tx, ty, tz, sx, sy, sz, area, pots, soln=None):
-        nt = tx.numel()
+        nt, ns = tx.numel(), sx.numel()
         soln = self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") if soln is None else soln
-        self._check(self._lib.sbslpm_direct_sum_inv_lap_targets(self.h, nt, _ptr(tx), _ptr(ty), _ptr(tz),
-                                                                sx.numel(), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(area),
-                                                                _ptr(pots), _ptr(soln)),
+        t3 = [_ptr(t, nt, "targets") for t in (tx, ty, tz)]
+        s5 = [_ptr(t, ns, "sources") for t in (sx, sy, sz, area, pots)]
+        self._check(self._lib.sbslpm_direct_sum_inv_lap_targets(self.h, nt, *t3, ns, *s5, _ptr(soln, nt, "soln")),
                     "direct_sum_inv_lap_targets")
         return soln
 
     def direct_sum_bve_vel_targets(self, tx, ty, tz, sx, sy, sz, area, vor, vel=None):
         """Biot-Savart direct sum at separate targets over undisplaced sources (the
         context's eps); vel (three 1-D tensors) accumulated."""
-        nt = tx.numel()
+        nt, ns = tx.numel(), sx.numel()
         vel = [self.torch.zeros(nt, dtype=self.torch.float64, device="cuda") for _ in range(3)] if vel is None else vel
-        self._check(self._lib.sbslpm_direct_sum_bve_vel_targets(self.h, nt, _ptr(tx), _ptr(ty), _ptr(tz), sx.numel(),
-                                                                _ptr(sx), _ptr(sy), _ptr(sz), _ptr(area), _ptr(vor),
-                                                                *[_ptr(v) for v in vel]),
+        t3 = [_ptr(t, nt, "targets") for t in (tx, ty, tz)]
+        s5 = [_ptr(t, ns, "sources") for t in (sx, sy, sz, area, vor)]
+        v3 = [_ptr(v, nt, "vel") for v in _seq(vel, 3, "vel")]
+        self._check(self._lib.sbslpm_direct_sum_bve_vel_targets(self.h, nt, *t3, ns, *s5, *v3),
                     "direct_sum_bve_vel_targets")
         return vel
 
@@ -417,9 +484,10 @@ class Context:
         n = x.numel()
         ub = n if ub is None else ub
         vel = [self.torch.zeros(n, dtype=self.torch.float64, device="cuda") for _ in range(3)] if vel is None else vel
-        d = disp if disp is not None else [None] * 3
-        args = [_ptr(a) for a in [x, y, z, area, vor] + list(d) + list(vel)]
-        self._check(self._lib.sbslpm_direct_sum_bve_vel(self.h, n, *args, lb, ub), "direct_sum_bve_vel")
+        d = [None] * 3 if disp is None else [_ptr(t, n, "disp") for t in _seq(disp, 3, "disp")]
+        a = [_ptr(t, n, "direct_sum_bve_vel input") for t in (x, y, z, area, vor)]
+        v3 = [_ptr(v, n, "vel") for v in _seq(vel, 3, "vel")]
+        self._check(self._lib.sbslpm_direct_sum_bve_vel(self.h, n, *a, *d, *v3, lb, ub), "direct_sum_bve_vel")
         return vel
 
     def bve_displacement(self, x, y, z, vel, vor, tau, new_vor=None, disp_part=None, disp_proxy=None):
@@ -427,19 +495,18 @@ class Context:
         new_vor = self.zeros_part() if new_vor is None else new_vor
         disp_part = self.zeros_part(3) if disp_part is None else disp_part
         disp_proxy = self.zeros_proxy(3) if disp_proxy is None else disp_proxy
-        a = [_ptr(t) for t in [x, y, z] + list(vel) + [vor]]
-        b = [_ptr(t) for t in [new_vor] + list(disp_part) + list(disp_proxy)]
+        a = [self._p(t, what="x, y, z") for t in (x, y, z)] + self._p3(vel, "vel") + [self._p(vor, what="vor")]
+        b = [self._p(new_vor, what="new_vor")] + self._p3(disp_part, "disp_part") + self._x3(disp_proxy, "disp_proxy")
         self._check(self._lib.sbslpm_bve_displacement(self.h, *a, float(tau), *b), "bve_displacement")
         return new_vor, disp_part, disp_proxy
 
     def bve_find_departure_points(self, x, y, z, vels, dt, dep=None):
         """bve_find_departure_points, bve_time_step.cpp:172-309; vels = [v0, v1, v2, v3], each 3 tensors."""
         dep = self.zeros_part(3) if dep is None else dep
-        flat = [t for stage in vels for t in stage]
-        arr = (ctypes.c_void_p * 12)(*[t.data_ptr() for t in flat])
-        self._check(self._lib.sbslpm_bve_find_departure_points(self.h, _ptr(x), _ptr(y), _ptr(z),
-                                                               ctypes.cast(arr, ctypes.c_void_p), float(dt),
-                                                               *[_ptr(t) for t in dep]), "departure_points")
+        _keep, arr = self._stages(vels, "vels")
+        self._check(self._lib.sbslpm_bve_find_departure_points(
+            self.h, *[self._p(t, what="x, y, z") for t in (x, y, z)], arr, float(dt), *self._p3(dep, "dep")),
+            "departure_points")
         return dep
 
     def bve_departure_to_target(self, z, dep, vor, tracers=None, new_vor=None, new_tracers=None):
@@ -448,18 +515,22 @@ class Context:
         ntr = 0 if tracers is None else tracers.shape[-1]
         if ntr and new_tracers is None:
             new_tracers = self.torch.zeros_like(tracers)
-        self._check(self._lib.sbslpm_bve_departure_to_target(self.h, _ptr(z), *[_ptr(t) for t in dep], _ptr(vor),
-                                                             _ptr(tracers), ntr, _ptr(new_vor), _ptr(new_tracers)),
-                    "departure_to_target")
+        tr = [self._p(tracers, ntr, "tracers"), self._p(new_tracers, ntr, "new_tracers")] if ntr else [None, None]
+        self._check(self._lib.sbslpm_bve_departure_to_target(
+            self.h, self._p(z, what="z"), *self._p3(dep, "dep"), self._p(vor, what="vor"), tr[0], ntr,
+            self._p(new_vor, what="new_vor"), tr[1]), "departure_to_target")
         return new_vor, new_tracers
 
     def bve_back_rk4_step_ensemble(self, x, y, z, area, vors, vel, time=0.0):
         """One backward RK4 step of every member of an ensemble (vors [members, Pl, m2] updated in
         place, vel = 3 tensors [members, Pl, m2] receive the stage-0 velocities); all members share
         each kernel launch."""
-        nmem = vors.shape[0]
-        self._check(self._lib.sbslpm_bve_back_rk4_step_ensemble(self.h, nmem, _ptr(x), _ptr(y), _ptr(z), _ptr(area),
-                                                                _ptr(vors), *[_ptr(t) for t in vel], float(time)),
+        nmem = int(vors.shape[0]) if vors.dim() >= 2 else 0
+        if nmem < 1:
+            raise ValueError("vors: expected shape [members, Pl, m2]")
+        args = ([self._p(t, what="x, y, z, area") for t in (x, y, z, area)] + [self._p(vors, nmem, "vors")]
+                + self._p3(vel, "vel", k=nmem))
+        self._check(self._lib.sbslpm_bve_back_rk4_step_ensemble(self.h, nmem, *args, float(time)),
                     "bve_back_rk4_step_ensemble")
         return vors
 
@@ -467,8 +538,8 @@ class Context:
         """Transpose of the undisplaced velocity evaluation (exact transpose of the FP64 FMM operator):
         vor_bar = K^T ubar (ubar = 3 tensors)."""
         vor_bar = self.zeros_part() if vor_bar is None else vor_bar
-        self._check(self._lib.sbslpm_bve_velocity_adjoint(self.h, _ptr(area), *[_ptr(u) for u in ubar],
-                                                          _ptr(vor_bar)), "bve_velocity_adjoint")
+        self._check(self._lib.sbslpm_bve_velocity_adjoint(self.h, self._p(area, what="area"), *self._p3(ubar, "ubar"),
+                                                          self._p(vor_bar, what="vor_bar")), "bve_velocity_adjoint")
         return vor_bar
 
     def autotune(self, area, vor):
@@ -476,7 +547,8 @@ class Context:
         fastest; returns [(waves, xcd, ms per launch), ...] and the chosen (waves, xcd)."""
         rep = np.zeros((6, 3))
         n = ctypes.c_int(0)
-        self._check(self._lib.sbslpm_autotune(self.h, _ptr(area), _ptr(vor), _dp(rep), ctypes.byref(n)), "autotune")
+        self._check(self._lib.sbslpm_autotune(self.h, self._p(area, what="area"), self._p(vor, what="vor"), _dp(rep),
+                                              ctypes.byref(n)), "autotune")
         return [tuple(r) for r in rep[:n.value]], self.pair_shape()
 
     def autotune_ensemble(self, area, vors):
@@ -486,8 +558,12 @@ class Context:
         [(waves, xcd, chunks, ms per launch), ...] and the chosen (waves, xcd, chunks)."""
         rep = np.zeros((12, 4))
         n = ctypes.c_int(0)
-        self._check(self._lib.sbslpm_autotune_ensemble(self.h, int(vors.shape[0]), _ptr(area), _ptr(vors), _dp(rep),
-                                                       ctypes.byref(n)), "autotune_ensemble")
+        nmem = int(vors.shape[0]) if vors.dim() >= 2 else 0
+        if nmem < 1:
+            raise ValueError("vors: expected shape [members, Pl, m2]")
+        self._check(self._lib.sbslpm_autotune_ensemble(self.h, nmem, self._p(area, what="area"),
+                                                       self._p(vors, nmem, "vors"), _dp(rep), ctypes.byref(n)),
+                    "autotune_ensemble")
         rows = [tuple(r) for r in rep[:n.value]]
         best = min(rows, key=lambda r: r[3])
         return rows, (int(best[0]), int(best[1]), int(best[2]))
@@ -496,8 +572,10 @@ class Context:
         """unify_boundary_vals (initialize_cubed_sphere.cpp:364-370): every copy of a unique
         node takes the mean of its copies, in place.  two_d_to_1d [Pl, m2] and counts [n1d]
         are int32 device tensors (from HostTree.dedupe); vals [Pl, m2] float64."""
-        self._check(self._lib.sbslpm_unify_boundary_vals(self.h, _iptr(two_d_to_1d), _iptr(counts), int(counts.numel()),
-                                                         _ptr(vals)), "unify_boundary_vals")
+        npart = self.sizes.Pl * self.sizes.m2
+        self._check(self._lib.sbslpm_unify_boundary_vals(self.h, _iptr(two_d_to_1d, npart, "two_d_to_1d"),
+                                                         _iptr(counts, what="counts"), int(counts.numel()),
+                                                         self._p(vals, what="vals")), "unify_boundary_vals")
         return vals
 
     def set_pair_shape(self, waves, xcd):
@@ -517,32 +595,33 @@ class Context:
         """BLI of leaf fields [Pl, m2] (a list of up to 8 tensors) at the points (x, y, z) (1-D
         tensors of one length): latlon_interp_host, interp_funcs.hpp:22-60."""
         n = x.numel()
-        out = [self.torch.zeros(n, dtype=self.torch.float64, device=x.device) for _ in fields] if out is None else out
-        fa = (ctypes.c_void_p * len(fields))(*[f.data_ptr() for f in fields])
-        oa = (ctypes.c_void_p * len(out))(*[o.data_ptr() for o in out])
-        for t in list(fields) + list(out):
-            _ptr(t)
-        self._check(self._lib.sbslpm_interp_points(self.h, n, _ptr(x), _ptr(y), _ptr(z), len(fields),
-                                                   ctypes.cast(fa, ctypes.c_void_p), ctypes.cast(oa, ctypes.c_void_p)),
-                    "interp_points")
+        fields = list(fields)
+        out = [self.torch.zeros(n, dtype=self.torch.float64, device="cuda") for _ in fields] if out is None else out
+        _kf, fa = _ptr_array([self._p(f, what="fields") for f in fields])
+        _ko, oa = _ptr_array([_ptr(o, n, "out") for o in _seq(out, len(fields), "out")])
+        self._check(self._lib.sbslpm_interp_points(self.h, n, *[_ptr(t, n, "points") for t in (x, y, z)],
+                                                   len(fields), fa, oa), "interp_points")
         return out
 
     def interp_points_adjoint(self, x, y, z, out_bar, field_bar=None):
         """Transpose of interp_points: field_bar[q] [Pl, m2] += B^T out_bar[q]."""
+        n = x.numel()
+        out_bar = list(out_bar)
         field_bar = [self.zeros_part() for _ in out_bar] if field_bar is None else field_bar
-        ob = (ctypes.c_void_p * len(out_bar))(*[_ptr(o).value for o in out_bar])
-        fb = (ctypes.c_void_p * len(field_bar))(*[_ptr(f).value for f in field_bar])
-        self._check(self._lib.sbslpm_interp_points_adjoint(self.h, x.numel(), _ptr(x), _ptr(y), _ptr(z), len(out_bar),
-                                                           ctypes.cast(ob, ctypes.c_void_p),
-                                                           ctypes.cast(fb, ctypes.c_void_p)), "interp_points_adjoint")
+        _ko, ob = _ptr_array([_ptr(o, n, "out_bar") for o in out_bar])
+        _kf, fb = _ptr_array([self._p(f, what="field_bar") for f in _seq(field_bar, len(out_bar), "field_bar")])
+        self._check(self._lib.sbslpm_interp_points_adjoint(self.h, n, *[_ptr(t, n, "points") for t in (x, y, z)],
+                                                           len(out_bar), ob, fb), "interp_points_adjoint")
         return field_bar
 
     def xyz_vel_to_uv_vel(self, x, y, z, vel, u=None, v=None):
         """Zonal and meridional velocity, xyz_vel_to_uv_vel (general_utils_impl.hpp:195-229)."""
+        vel = _seq(vel, 3, "vel")
+        n = vel[0].numel()
         u = self.torch.zeros_like(vel[0]) if u is None else u
         v = self.torch.zeros_like(vel[0]) if v is None else v
-        self._check(self._lib.sbslpm_xyz_vel_to_uv_vel(self.h, vel[0].numel(), _ptr(x), _ptr(y), _ptr(z),
-                                                       *[_ptr(t) for t in vel], _ptr(u), _ptr(v)),
+        a = [_ptr(t, n, "xyz_vel_to_uv_vel input") for t in (x, y, z, *vel)]
+        self._check(self._lib.sbslpm_xyz_vel_to_uv_vel(self.h, n, *a, _ptr(u, n, "u"), _ptr(v, n, "v")),
                     "xyz_vel_to_uv_vel")
         return u, v
 
@@ -551,8 +630,8 @@ class Context:
         (d new_vor / d vor)^T new_vor_bar at the state vor (not modified; the step's forward
         pass is recomputed inside)."""
         vor_bar = self.zeros_part() if vor_bar is None else vor_bar
-        self._check(self._lib.sbslpm_bve_back_rk4_step_adjoint(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area),
-                                                               _ptr(vor), _ptr(new_vor_bar), _ptr(vor_bar)),
+        a = [self._p(t, what="bve_back_rk4_step_adjoint input") for t in (x, y, z, area, vor, new_vor_bar)]
+        self._check(self._lib.sbslpm_bve_back_rk4_step_adjoint(self.h, *a, self._p(vor_bar, what="vor_bar")),
                     "bve_back_rk4_step_adjoint")
         return vor_bar
 
@@ -561,40 +640,43 @@ class Context:
         vor_out = self.zeros_part() if vor_out is None else vor_out
         if stages is None:
             stages = self.torch.zeros((12,) + tuple(vor.shape), dtype=self.torch.float64, device="cuda")
-        self._check(self._lib.sbslpm_bve_back_rk4_step_record(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vor),
-                                                              _ptr(vor_out), _ptr(stages)), "bve_back_rk4_step_record")
+        a = [self._p(t, what="bve_back_rk4_step_record input") for t in (x, y, z, area, vor)]
+        self._check(self._lib.sbslpm_bve_back_rk4_step_record(self.h, *a, self._p(vor_out, what="vor_out"),
+                                                              self._p(stages, 12, "stages")),
+                    "bve_back_rk4_step_record")
         return vor_out, stages
 
     def bve_back_rk4_step_adjoint_stages(self, x, y, z, area, vor, stages, new_vor_bar, vor_bar=None):
         """bve_back_rk4_step_adjoint with the step's recorded stage velocities."""
         vor_bar = self.zeros_part() if vor_bar is None else vor_bar
+        a = [self._p(t, what="bve_back_rk4_step_adjoint_stages input") for t in (x, y, z, area, vor)]
         self._check(self._lib.sbslpm_bve_back_rk4_step_adjoint_stages(
-            self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vor), _ptr(stages), _ptr(new_vor_bar), _ptr(vor_bar)),
-            "bve_back_rk4_step_adjoint_stages")
+            self.h, *a, self._p(stages, 12, "stages"), self._p(new_vor_bar, what="new_vor_bar"),
+            self._p(vor_bar, what="vor_bar")), "bve_back_rk4_step_adjoint_stages")
         return vor_bar
 
     def bve_departure_to_target_adjoint(self, dep, new_vor_bar, vor_bar=None):
         """Transpose of the remap interpolation: vor_bar += B^T new_vor_bar."""
         vor_bar = self.zeros_part() if vor_bar is None else vor_bar
-        self._check(self._lib.sbslpm_bve_departure_to_target_adjoint(self.h, *[_ptr(d) for d in dep],
-                                                                     _ptr(new_vor_bar), _ptr(vor_bar)),
-                    "bve_departure_to_target_adjoint")
+        self._check(self._lib.sbslpm_bve_departure_to_target_adjoint(
+            self.h, *self._p3(dep, "dep"), self._p(new_vor_bar, what="new_vor_bar"),
+            self._p(vor_bar, what="vor_bar")), "bve_departure_to_target_adjoint")
         return vor_bar
 
     def swe_displacement(self, vel, tau, disp_part=None, disp_proxy=None):
         """swe_displacement_upward_pass, swe_time_step.cpp:265-276."""
         disp_part = self.zeros_part(3) if disp_part is None else disp_part
         disp_proxy = self.zeros_proxy(3) if disp_proxy is None else disp_proxy
-        self._check(self._lib.sbslpm_swe_displacement(self.h, *[_ptr(v) for v in vel], float(tau),
-                                                      *[_ptr(v) for v in list(disp_part) + list(disp_proxy)]),
-                    "swe_displacement")
+        self._check(self._lib.sbslpm_swe_displacement(self.h, *self._p3(vel, "vel"), float(tau),
+                                                      *self._p3(disp_part, "disp_part"),
+                                                      *self._x3(disp_proxy, "disp_proxy")), "swe_displacement")
         return disp_part, disp_proxy
 
     # ---- shallow-water step (swe_time_step.cpp) ----
     def laplacian(self, f, lap=None):
         """laplacian (deriv_funcs.cpp:27-37): spectral surface Laplacian on each leaf."""
         lap = self.zeros_part() if lap is None else lap
-        self._check(self._lib.sbslpm_laplacian(self.h, _ptr(f), _ptr(lap)), "laplacian")
+        self._check(self._lib.sbslpm_laplacian(self.h, self._p(f, what="f"), self._p(lap, what="lap")), "laplacian")
         return lap
 
     def laplacian_interactions(self, q_part, q_proxy, disp_part=None, disp_proxy=None, phi_part=None,
@@ -602,34 +684,35 @@ class Context:
         """laplacian_interactions (laplacian.cpp:6-35); accumulates."""
         phi_part = self.zeros_part() if phi_part is None else phi_part
         phi_proxy = self.zeros_proxy() if phi_proxy is None else phi_proxy
-        dp = [None] * 3 if disp_part is None else list(disp_part)
-        dx = [None] * 3 if disp_proxy is None else list(disp_proxy)
-        self._check(self._lib.sbslpm_laplacian_interactions(self.h, *[_ptr(d) for d in dp + dx], _ptr(q_part),
-                                                            _ptr(q_proxy), _ptr(phi_part), _ptr(phi_proxy)),
-                    "laplacian_interactions")
+        self._check(self._lib.sbslpm_laplacian_interactions(
+            self.h, *self._opt3(disp_part, "disp_part"), *self._opt3(disp_proxy, "disp_proxy", proxy=True),
+            self._p(q_part, what="q_part"), self._x(q_proxy, what="q_proxy"), self._p(phi_part, what="phi_part"),
+            self._x(phi_proxy, what="phi_proxy")), "laplacian_interactions")
         return phi_part, phi_proxy
 
     def apply_topography(self, x, y, z, height, disp_part=None, disp_proxy=None, eff=None):
         """apply_topography (topography.cpp:47-56) with the context's topography and swe_rule."""
         eff = self.zeros_part() if eff is None else eff
-        dp = [None] * 3 if disp_part is None else list(disp_part)
-        dx = [None] * 3 if disp_proxy is None else list(disp_proxy)
-        self._check(self._lib.sbslpm_apply_topography(self.h, _ptr(x), _ptr(y), _ptr(z), *[_ptr(d) for d in dp + dx],
-                                                      _ptr(height), _ptr(eff)), "apply_topography")
+        self._check(self._lib.sbslpm_apply_topography(
+            self.h, *[self._p(t, what="x, y, z") for t in (x, y, z)], *self._opt3(disp_part, "disp_part"),
+            *self._opt3(disp_proxy, "disp_proxy", proxy=True), self._p(height, what="height"),
+            self._p(eff, what="eff")), "apply_topography")
         return eff
 
     def swe_tendency(self, x, y, z, vel, vor, div, height, height_lap, out=None):
         """swe_tendency_computation (swe_time_step.cpp:76-161): (vor_t, div_t, h_t)."""
         out = self.zeros_part(3) if out is None else out
-        self._check(self._lib.sbslpm_swe_tendency(self.h, _ptr(x), _ptr(y), _ptr(z), *[_ptr(v) for v in vel],
-                                                  _ptr(vor), _ptr(div), _ptr(height), _ptr(height_lap),
-                                                  *[_ptr(o) for o in out]), "swe_tendency")
+        a = ([self._p(t, what="x, y, z") for t in (x, y, z)] + self._p3(vel, "vel")
+             + [self._p(t, what="vor, div, height, height_lap") for t in (vor, div, height, height_lap)])
+        self._check(self._lib.sbslpm_swe_tendency(self.h, *a, *self._p3(out, "out")), "swe_tendency")
         return out
 
     def swe_height_correction(self, x, y, z, vel, height, h_t, dt, height_lap):
         """swe_height_correction (swe_time_step.cpp:558-604); height_lap updated in place."""
-        self._check(self._lib.sbslpm_swe_height_correction(self.h, _ptr(x), _ptr(y), _ptr(z), *[_ptr(v) for v in vel],
-                                                           _ptr(height), _ptr(h_t), float(dt), _ptr(height_lap)),
+        a = ([self._p(t, what="x, y, z") for t in (x, y, z)] + self._p3(vel, "vel")
+             + [self._p(height, what="height"), self._p(h_t, what="h_t")])
+        self._check(self._lib.sbslpm_swe_height_correction(self.h, *a, float(dt),
+                                                           self._p(height_lap, what="height_lap")),
                     "swe_height_correction")
         return height_lap
 
@@ -639,20 +722,21 @@ class Context:
         in place; returns (stage-1 velocity, stage-1 surface Laplacian)."""
         vel = self.zeros_part(3) if vel is None else vel
         height_lap = self.zeros_part() if height_lap is None else height_lap
-        self._check(self._lib.sbslpm_swe_back_rk4_step(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vors),
-                                                       _ptr(divs), _ptr(height), *[_ptr(v) for v in vel],
-                                                       _ptr(tracers), _ptr(height_lap), float(time)),
+        ntr = self.cfg.tracer_count
+        tr = self._p(tracers, ntr, "tracers") if ntr > 0 else None
+        a = [self._p(t, what="swe_back_rk4_step input") for t in (x, y, z, area, vors, divs, height)]
+        self._check(self._lib.sbslpm_swe_back_rk4_step(self.h, *a, *self._p3(vel, "vel"), tr,
+                                                       self._p(height_lap, what="height_lap"), float(time)),
                     "swe_back_rk4_step")
         return vel, height_lap
 
     def swe_find_departure_points(self, x, y, z, vels, dt, dep=None):
         """swe_find_departure_points, swe_time_step.cpp:422-556."""
         dep = self.zeros_part(3) if dep is None else dep
-        flat = [t for stage in vels for t in stage]
-        arr = (ctypes.c_void_p * 12)(*[t.data_ptr() for t in flat])
-        self._check(self._lib.sbslpm_swe_find_departure_points(self.h, _ptr(x), _ptr(y), _ptr(z),
-                                                               ctypes.cast(arr, ctypes.c_void_p), float(dt),
-                                                               *[_ptr(t) for t in dep]), "swe_departure_points")
+        _keep, arr = self._stages(vels, "vels")
+        self._check(self._lib.sbslpm_swe_find_departure_points(
+            self.h, *[self._p(t, what="x, y, z") for t in (x, y, z)], arr, float(dt), *self._p3(dep, "dep")),
+            "swe_departure_points")
         return dep
 
     def swe_departure_to_target(self, dep, vor, div, h, tracers=None):
@@ -660,16 +744,20 @@ class Context:
         out = [self.zeros_part() for _ in range(3)]
         ntr = 0 if tracers is None else tracers.shape[-1]
         new_tracers = self.torch.zeros_like(tracers) if ntr else None
-        self._check(self._lib.sbslpm_swe_departure_to_target(self.h, *[_ptr(t) for t in dep], _ptr(vor), _ptr(div),
-                                                             _ptr(h), _ptr(tracers), ntr, *[_ptr(o) for o in out],
-                                                             _ptr(new_tracers)), "swe_departure_to_target")
+        tr = [self._p(tracers, ntr, "tracers"), self._p(new_tracers, ntr, "new_tracers")] if ntr else [None, None]
+        self._check(self._lib.sbslpm_swe_departure_to_target(
+            self.h, *self._p3(dep, "dep"), *[self._p(t, what="vor, div, h") for t in (vor, div, h)], tr[0], ntr,
+            *self._p3(out, "out"), tr[1]), "swe_departure_to_target")
         return out[0], out[1], out[2], new_tracers
 
     def bve_back_rk4_step(self, x, y, z, area, vors, vel, tracers=None, time=0.0, proxy_source_vors=None):
         """bve_back_rk4_step, bve_time_step.cpp:311-419: vors/tracers updated in place, vel <- stage-0 velocity."""
-        self._check(self._lib.sbslpm_bve_back_rk4_step(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(vors),
-                                                       _ptr(proxy_source_vors), *[_ptr(t) for t in vel],
-                                                       _ptr(tracers), float(time)), "bve_back_rk4_step")
+        ntr = self.cfg.tracer_count
+        tr = self._p(tracers, ntr, "tracers") if ntr > 0 else None
+        psv = None if proxy_source_vors is None else self._x(proxy_source_vors, "proxy_source_vors")
+        a = [self._p(t, what="bve_back_rk4_step input") for t in (x, y, z, area, vors)]
+        self._check(self._lib.sbslpm_bve_back_rk4_step(self.h, *a, psv, *self._p3(vel, "vel"), tr, float(time)),
+                    "bve_back_rk4_step")
         return vors
 
 
@@ -695,18 +783,25 @@ class IrregContext:
     def _zeros(self, *shape):
         return self.torch.zeros(shape, dtype=self.torch.float64, device="cuda")
 
+    def _pt(self, t, what="point array"):
+        return _ptr(t, self.N, what)
+
+    def _px(self, t, what="proxy array"):
+        return _ptr(t, self.P * self.n, what)
+
     def upward_pass(self, area, pots, proxy=None):
         """upward_pass_ll: proxy charges [P, n] (assigned)."""
         proxy = self._zeros(self.P, self.n) if proxy is None else proxy
-        self.ctx._check(self._lib.sbslpm_irreg_upward_pass(self.h, _ptr(area), _ptr(pots), _ptr(proxy)),
-                        "irreg_upward_pass")
+        self.ctx._check(self._lib.sbslpm_irreg_upward_pass(self.h, self._pt(area, "area"), self._pt(pots, "pots"),
+                                                           self._px(proxy, "proxy")), "irreg_upward_pass")
         return proxy
 
     def downward_pass_3(self, proxy, sols=None):
         """downward_pass_3_ll: proxy (3 tensors [P, n]) pushed down in place; sols += L2P."""
         sols = [self._zeros(self.N) for _ in range(3)] if sols is None else sols
-        self.ctx._check(self._lib.sbslpm_irreg_downward_pass_3(self.h, *[_ptr(p) for p in proxy],
-                                                               *[_ptr(t) for t in sols]), "irreg_downward_pass_3")
+        self.ctx._check(self._lib.sbslpm_irreg_downward_pass_3(
+            self.h, *[self._px(p, "proxy") for p in _seq(proxy, 3, "proxy")],
+            *[self._pt(t, "sols") for t in _seq(sols, 3, "sols")]), "irreg_downward_pass_3")
         return sols
 
     def swe_vel_interactions(self, proxy_vors, proxy_divs, vors, divs, area, target_pots=None, vel=None):
@@ -714,18 +809,22 @@ class IrregContext:
         target_pots = [self._zeros(self.P, self.n) for _ in range(3)] if target_pots is None else target_pots
         vel = [self._zeros(self.N) for _ in range(3)] if vel is None else vel
         self.ctx._check(self._lib.sbslpm_irreg_swe_vel_interactions(
-            self.h, *[_ptr(t) for t in target_pots], *[_ptr(t) for t in vel], _ptr(proxy_vors), _ptr(proxy_divs),
-            _ptr(vors), _ptr(divs), _ptr(area)), "irreg_swe_vel_interactions")
+            self.h, *[self._px(t, "target_pots") for t in _seq(target_pots, 3, "target_pots")],
+            *[self._pt(t, "vel") for t in _seq(vel, 3, "vel")], self._px(proxy_vors, "proxy_vors"),
+            self._px(proxy_divs, "proxy_divs"), self._pt(vors, "vors"), self._pt(divs, "divs"),
+            self._pt(area, "area")), "irreg_swe_vel_interactions")
         return target_pots, vel
 
     def swe_velocity(self, area, vors, divs, vel=None):
         vel = [self._zeros(self.N) for _ in range(3)] if vel is None else vel
-        self.ctx._check(self._lib.sbslpm_irreg_swe_velocity(self.h, _ptr(area), _ptr(vors), _ptr(divs),
-                                                            *[_ptr(t) for t in vel]), "irreg_swe_velocity")
+        self.ctx._check(self._lib.sbslpm_irreg_swe_velocity(
+            self.h, self._pt(area, "area"), self._pt(vors, "vors"), self._pt(divs, "divs"),
+            *[self._pt(t, "vel") for t in _seq(vel, 3, "vel")]), "irreg_swe_velocity")
         return vel
 
     def bve_velocity(self, area, vor, vel=None):
         vel = [self._zeros(self.N) for _ in range(3)] if vel is None else vel
-        self.ctx._check(self._lib.sbslpm_irreg_bve_velocity(self.h, _ptr(area), _ptr(vor), *[_ptr(t) for t in vel]),
-                        "irreg_bve_velocity")
+        self.ctx._check(self._lib.sbslpm_irreg_bve_velocity(
+            self.h, self._pt(area, "area"), self._pt(vor, "vor"), *[self._pt(t, "vel") for t in _seq(vel, 3, "vel")]),
+            "irreg_bve_velocity")
         return vel

@@ -1,5 +1,5 @@
 // adjoint.hip -- exact transpose of the FP64 Biot-Savart FMM evaluation
-// (velocity: upward pass, pair sums, downward pass; bve_vel_impl.hpp:71-106,
+// (velocity: upward pass, pair sums, downward pass; bve_vel_impl.hpp:34-70,
 // fmm_funcs.cpp:341-392) with respect to the source charges and the source
 // displacements.
 //

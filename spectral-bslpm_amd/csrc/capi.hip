@@ -20,6 +20,10 @@ using namespace sbslpm;
 // arrays [members][Pl][m2], proxy arrays [members][P][n], records per member.
 struct Work {
   int members = 0;
+  // precision and multi-RHS columns the record buffers were sized for (an
+  // ensemble workspace holds only the record kind of its precision)
+  int precision = -1;
+  bool mrhs = false;
   SrcRec64* rec64 = nullptr;
   SrcRec32* rec32 = nullptr;
   SrcRecMfma* recmf = nullptr;
@@ -107,7 +111,7 @@ struct sbslpm_ctx {
   // displaced state, surface Laplacian, arrival values), allocated on the first step
   char* sws_arena = nullptr;
   hipGraphExec_t swe_graph = nullptr;  // captured shallow-water step, keyed by the caller's pointers
-  const void* swe_graph_key[12] = {};
+  const void* swe_graph_key[13] = {};
   struct SweStep {
     double *dpart[3], *dprox[3], *eff, *lap, *v[3][3], *tend[3][3], *st[3], *da, *arr[3], *dep[3], *ntr;
     double *qp, *qx, *pp, *px;
@@ -409,9 +413,17 @@ int sbslpm_ctx_set_stream(sbslpm_ctx* c, void* stream) {
 
 int sbslpm_ctx_set_precision(sbslpm_ctx* c, int precision) {
   if (!c || precision < SBSLPM_FP64 || precision > SBSLPM_MIXED_VALU) return SBSLPM_E_ARG;
-  if (precision != c->prm.precision && c->graph) {
-    hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
+  if (c->capturing) return fail(c, SBSLPM_E_ARG, "set_precision: not allowed during graph capture");
+  if (precision != c->prm.precision) {
+    // captured steps hold the old precision's kernels; the ensemble workspace
+    // holds its record kind only (ensure_ensemble reallocates it on next use)
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    hipGraphExec_t* graphs[3] = {&c->graph, &c->ens_graph, &c->swe_graph};
+    for (hipGraphExec_t* g : graphs)
+      if (*g) {
+        hipGraphExecDestroy(*g);
+        *g = nullptr;
+      }
   }
   c->prm.precision = precision;
   return SBSLPM_OK;
@@ -528,7 +540,7 @@ int sbslpm_poisson_fmm_interactions(sbslpm_ctx* c, const double* q_part, const d
     KernelTimer tm(c, SBSLPM_KERNEL_LOG_MIXED);
     launch_poisson_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, psi_part, psi_proxy, c->stream);
   } else {
-    // the Poisson kernel reads the node coordinates unprojected (inverse_poisson_impl.hpp:199-204)
+    // the Poisson kernel reads the node coordinates unprojected (inverse_poisson_impl.hpp:42-46)
     launch_source_records64(c->dt, q_part, q_proxy, nullptr, nullptr, false, c->rec64, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_LOG_FP64);
     launch_poisson_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, psi_part, psi_proxy, c->stream);
@@ -671,7 +683,7 @@ static int eval_targets(sbslpm_ctx* c, int kind, const double* area, const doubl
   const int nc = kind == 0 ? 1 : 3;
   launch_upward_pass(c->dt, area, f, c->qpart, c->qproxy, c->stream);
   launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, nc, c->phiproxy, c->stream);
-  // the Poisson kernel reads unprojected nodes, Biot-Savart re-projects (inverse_poisson_impl.hpp:199-204,
+  // the Poisson kernel reads unprojected nodes, Biot-Savart re-projects (inverse_poisson_impl.hpp:42-46,
   // bve_vel_impl.hpp:54-60)
   launch_source_records64(c->dt, c->qpart, c->qproxy, nullptr, nullptr, kind == 1, c->rec64, c->stream);
   if (kind == 0)
@@ -709,6 +721,14 @@ namespace {
 
 int ensure_scratch(sbslpm_ctx* c, int64_t need) {
   if (need <= c->scratch_len) return SBSLPM_OK;
+  // growing the buffer synchronizes, allocates and frees the old one, none of
+  // which may happen while the stream is captured (the library's own capture
+  // or a caller's, e.g. a torch CUDA graph): an already captured graph may
+  // still reference the old buffer
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (c->capturing || (c->stream && hipStreamIsCapturing(c->stream, &cs) == hipSuccess &&
+                       cs != hipStreamCaptureStatusNone))
+    return fail(c, SBSLPM_E_ARG, "scratch buffer would grow during stream capture; call once outside the capture first");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->scratch) hipFree(c->scratch);
   c->scratch = nullptr;
@@ -1063,8 +1083,6 @@ int sbslpm_unify_boundary_vals(sbslpm_ctx* c, const int* two_d_to_1d, const int*
   if (!c || !two_d_to_1d || !counts || !vals) return fail(c, SBSLPM_E_ARG, "unify_boundary_vals: null pointer");
   const int64_t n = (int64_t)c->tree.Pl * c->tree.m2;
   if (n1d < 1 || n1d > n) return fail(c, SBSLPM_E_ARG, "unify_boundary_vals: n1d must lie in [1, Pl*m2]");
-  if (c->capturing && n1d > c->scratch_len)
-    return fail(c, SBSLPM_E_ARG, "unify_boundary_vals: would allocate during graph capture");
   int rc = ensure_scratch(c, n1d);
   if (rc) return rc;
   launch_unify_boundary_vals(n, two_d_to_1d, counts, n1d, c->scratch, vals, c->stream);
@@ -1100,7 +1118,8 @@ int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
   // Graph capture needs a non-default stream; on the default stream the step runs eagerly.
   if (!(c->prm.flags & SBSLPM_FLAG_GRAPH) || !c->stream)
     return swe_step_body(c, x, y, z, area, vors, divs, height, vx, vy, vz, tracers, height_lap);
-  const void* key[12] = {x, y, z, area, vors, divs, height, vx, vy, vz, tracers, height_lap};
+  const void* key[13] = {x,  y,  z,  area,    vors,       divs, height, vx, vy, vz, tracers, height_lap,
+                         (const void*)(intptr_t)c->prm.precision};
   if (!c->swe_graph || std::memcmp(key, c->swe_graph_key, sizeof(key)) != 0) {
     if (c->swe_graph) {
       hipGraphExecDestroy(c->swe_graph);
@@ -1811,7 +1830,13 @@ int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* c, const double* dep_x, c
 
 // Ensemble workspace for nmem members (grown on demand, never shrunk).
 static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
-  if (c->ens_arena && c->wens.members >= nmem) return SBSLPM_OK;
+  const bool mrhs = c->prm.precision == SBSLPM_MIXED && (c->prm.flags & SBSLPM_FLAG_MULTI_RHS) && nmem > 1;
+  // reuse only a workspace whose record buffers match the context's current
+  // precision (set_precision may have changed it since) and that has the
+  // multi-RHS columns when they are needed
+  if (c->ens_arena && c->wens.members >= nmem && c->wens.precision == c->prm.precision && (c->wens.mrhs || !mrhs))
+    return SBSLPM_OK;
+  if (c->capturing) return fail(c, SBSLPM_E_ARG, "ensemble workspace: cannot allocate during graph capture");
   const Tree& t = c->tree;
   const size_t NP = (size_t)t.npart * nmem, Pn = (size_t)t.P * t.n * nmem, U = (size_t)t.U * nmem,
                U16 = (size_t)t.U16 * nmem;
@@ -1819,7 +1844,6 @@ static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
   const size_t o_r64 = A.take<SrcRec64>(c->prm.precision == SBSLPM_FP64 ? U : 0);
   const size_t o_r32 = A.take<SrcRec32>(c->prm.precision == SBSLPM_MIXED_VALU ? U : 0);
   const size_t o_rmf = A.take<SrcRecMfma>(c->prm.precision == SBSLPM_MIXED ? U16 : 0);
-  const bool mrhs = c->prm.precision == SBSLPM_MIXED && (c->prm.flags & SBSLPM_FLAG_MULTI_RHS) && nmem > 1;
   const size_t o_qc = A.take<float>(mrhs ? (size_t)t.U16 * 16 * mrhs_chunks(nmem, 2) : 0);
   const size_t o_qp = A.take<double>(NP), o_qx = A.take<double>(Pn);
   const size_t o_pp = A.take<double>(3 * NP), o_px = A.take<double>(3 * Pn);
@@ -1839,6 +1863,8 @@ static int ensure_ensemble(sbslpm_ctx* c, int nmem) {
   c->ens_arena = b;
   Work& w = c->wens;
   w.members = nmem;
+  w.precision = c->prm.precision;
+  w.mrhs = mrhs;
   w.rec64 = (SrcRec64*)(b + o_r64); w.rec32 = (SrcRec32*)(b + o_r32); w.recmf = (SrcRecMfma*)(b + o_rmf);
   w.qcol = mrhs ? (float*)(b + o_qc) : nullptr;
   w.qpart = (double*)(b + o_qp); w.qproxy = (double*)(b + o_qx);

@@ -1,6 +1,6 @@
 // direct.hip -- O(N^2) direct sums (direct_sum.cpp:8-15, direct_sum_impl.hpp:18-27)
 // and the Biot-Savart direct sum the reference lacks (same pair kernel as
-// bve_vel_impl.hpp:90-103).
+// bve_vel_impl.hpp:54-66).
 //
 // One wave64 per 64 targets and one source chunk per grid row; sources are
 // read as wave-uniform loads.  Each (target tile, chunk) writes a partial sum
@@ -106,7 +106,7 @@ __global__ void k_reduce(int64_t nt, int ncomp, int64_t chunks, const double* __
   }
 }
 
-// Displaced, re-projected source positions (bve_vel_impl.hpp:94-97).
+// Displaced, re-projected source positions (bve_vel_impl.hpp:57-60).
 __global__ void k_displace(int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
                            const double* __restrict__ sz, const double* __restrict__ dx,
                            const double* __restrict__ dy, const double* __restrict__ dz, double* __restrict__ out) {

@@ -1,7 +1,7 @@
 // fmm_interact.hip -- pair kernels over the target-major interaction lists.
 //
-// Reference: bve_vel_panel_interaction (fmm_interactions/bve_vel_impl.hpp:71-106)
-// and poisson_panel_interaction (inverse_poisson_impl.hpp:181-210) run one
+// Reference: bve_vel_panel_interaction (fmm_interactions/bve_vel_impl.hpp:34-70)
+// and poisson_panel_interaction (inverse_poisson_impl.hpp:23-52) run one
 // thread per interaction, recompute every source node's map per pair and
 // scatter three atomics per pair.  Here:
 //   * one wave64 owns up to 64 target points of one target panel and walks
@@ -76,7 +76,7 @@ __device__ inline void store_s(const DevTree& t, int64_t u, double* part, double
   *o = assign ? a : *o + a;
 }
 
-// Source records, FP64: y = P(node + disp) (bve_vel_impl.hpp:90-97), q' = -q/(4 pi).
+// Source records, FP64: y = P(node + disp) (bve_vel_impl.hpp:54-60), q' = -q/(4 pi).
 // Leaf proxy rows are never sources and are left untouched.
 __global__ void k_src64(DevTree t, const double* __restrict__ q_part, const double* __restrict__ q_proxy,
                         CPtr3 dpart, CPtr3 dprox, int project, SrcRec64* __restrict__ rec) {
@@ -160,7 +160,7 @@ struct HwRcp64 {
 };
 
 // FP64 log pair kernel: psi += -1/(4 pi) log(1 - x.y) q, pairs with
-// x.y >= 1 - 1e-15 skipped (inverse_poisson_impl.hpp:204-207).  The logarithm
+// x.y >= 1 - 1e-15 skipped (inverse_poisson_impl.hpp:46-49).  The logarithm
 // is log_pos (within 2 ulp of the correctly rounded one).
 __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,

@@ -67,7 +67,7 @@ void clenshaw_curtis(int N, std::vector<double>& w) {
 }
 
 // Panel frame in radians as the interaction kernels compute it
-// (bve_vel_impl.hpp:75-83).
+// (bve_vel_impl.hpp:38-46).
 inline void frame(const HostPanel& p, double& xo, double& xs, double& eo, double& es) {
   xo = kPi / 4 * 0.5 * (p.max_xi + p.min_xi);
   xs = kPi / 4 * 0.5 * (p.max_xi - p.min_xi);

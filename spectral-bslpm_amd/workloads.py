@@ -4,7 +4,7 @@ All functions take unit-sphere coordinates (numpy arrays of any shape).
 Sign conventions of the reference kernels: psi = int G f with
 G(x, y) = -1/(4 pi) log(1 - x.y) solves  Lap psi = -(f - mean f), so a degree-l
 spherical harmonic maps to Y/(l(l+1)); the Biot-Savart velocity is
-u = -x cross grad psi (bve_vel_impl.hpp:100-103 is -x cross grad_x of G).
+u = -x cross grad psi (bve_vel_impl.hpp:61-66 is -x cross grad_x of G).
 """
 import numpy as np
 

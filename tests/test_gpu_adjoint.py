@@ -2,7 +2,7 @@
 torch.autograd wrappers.
 
 * velocity: the FMM adjoint against the transpose of the FP64 direct sum
-  (bve_vel_impl.hpp:90-103 summed over all particles, numpy) to the FMM
+  (bve_vel_impl.hpp:54-66 summed over all particles, numpy) to the FMM
   truncation, and the dot-product test <K w, ubar> = <w, K^T ubar> to
   rounding (the adjoint is the exact transpose of the FMM operator);
 * remap: the dot-product test <B w, v> = <w, B^T v> to rounding;

@@ -161,3 +161,41 @@ def test_ensemble_rejects_bad_arguments(sb, torch_gpu):
     vors = to_gpu(t, np.stack([np.zeros_like(x)] * 2))
     with pytest.raises(sb.SbslpmError):
         ctx.bve_back_rk4_step_ensemble(*g, vors, [t.zeros_like(vors) for _ in range(3)])
+
+
+@pytest.mark.parametrize("first,second", [("fp64", "mixed"), ("mixed", "fp64"), ("mixed", "mixed_valu")])
+def test_ensemble_after_set_precision(sb, torch_gpu, first, second):
+    """An ensemble workspace holds the source records of the precision it was
+    built for; after set_precision the next ensemble step must rebuild it.
+    The step on the switched context equals the step of a fresh context of the
+    second precision, bit for bit, eagerly and as a replayed graph."""
+    t = torch_gpu
+    print(f"\n[case ensemble set_precision {first}->{second}]", flush=True)
+    kw = dict(levels=3, interp_degree=8, leaf_degree=7, weights="clenshaw_curtis", kernel_eps=EPS, delta_t=DT,
+              omega=OMEGA, multi_rhs=True)
+    stream = t.cuda.Stream()
+    with t.cuda.stream(stream):
+        sw = sb.Context(sb.RunConfig(precision=first, use_graph=True, **kw))
+        fresh = sb.Context(sb.RunConfig(precision=second, use_graph=True, **kw))
+    x, y, z, a = sw.spec_points()
+    gx, gy, gz, ga = [to_gpu(t, q) for q in (x, y, z, a)]
+    members = np.stack([sb.workloads.gaussian_vortices(x, y, z, a, seed=s) for s in (1, 2, 3, 4, 5)])
+    with t.cuda.stream(stream):
+        e = to_gpu(t, members)
+        sw.bve_back_rk4_step_ensemble(gx, gy, gz, ga, e, [t.zeros_like(e) for _ in range(3)])
+    sw.synchronize()
+    sw.set_precision(second)
+    out = {}
+    for name, c in (("switched", sw), ("fresh", fresh)):
+        with t.cuda.stream(stream):
+            e = to_gpu(t, members)
+            vel = [t.zeros_like(e) for _ in range(3)]
+            for _ in range(2):  # capture, then replay
+                e.copy_(to_gpu(t, members))
+                c.bve_back_rk4_step_ensemble(gx, gy, gz, ga, e, vel)
+        c.synchronize()
+        t.cuda.synchronize()
+        out[name] = (to_np(e), [to_np(v) for v in vel])
+    np.testing.assert_array_equal(out["switched"][0], out["fresh"][0])
+    for f in range(3):
+        np.testing.assert_array_equal(out["switched"][1][f], out["fresh"][1][f])

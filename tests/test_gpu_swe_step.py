@@ -253,6 +253,42 @@ def test_step_graph_replay_matches_eager(case, sb, torch_gpu):
         _ = (st[0] * 2).sum().item()  # eager work between replays
 
 
+def test_step_graph_after_set_precision(case, sb, torch_gpu):
+    """A captured step holds the kernels of the precision it was captured in.  After
+    set_precision the graph-mode step must run the new precision: it equals, to the
+    bit, the eager step of a fresh context of that precision (before and after the
+    switch)."""
+    t, o = torch_gpu, case["o"]
+    x, y, z, a = case["x"], case["y"], case["z"], case["a"]
+    kw = CASES[[k for k in CASES if (CASES[k]["leaf_degree"] == o.md and CASES[k]["interp_degree"] == o.d)][0]]
+    common = dict(kernel_eps=EPS, omega=OMEGA, delta_t=DT, swe_rule="leaf", topography="cone", **kw)
+    stream = t.cuda.Stream()
+    with t.cuda.stream(stream):
+        cg = sb.Context(sb.RunConfig(precision="fp64", use_graph=True, **common))
+        eager = {p: sb.Context(sb.RunConfig(precision=p, **common)) for p in ("fp64", "mixed")}
+    geo = g(case, x, y, z, a)
+    state0 = g(case, case["vor"], case["div"], case["h"])
+    # the graph context keeps one set of buffers, so its graph key (the pointers)
+    # is the same before and after every switch
+    st_g = [v.clone() for v in state0]
+    vel_g, lap_g = cg.zeros_part(3), cg.zeros_part()
+    for prec in ("fp64", "fp64", "mixed", "mixed", "fp64"):
+        if prec != cg.cfg.precision:
+            cg.set_precision(prec)
+        print(f"  graph step in {prec}", flush=True)
+        with t.cuda.stream(stream):
+            for v, v0 in zip(st_g, state0):
+                v.copy_(v0)
+            cg.swe_back_rk4_step(*geo, *st_g, vel=vel_g, height_lap=lap_g)
+            st = [v.clone() for v in state0]
+            vel, lap = eager[prec].swe_back_rk4_step(*geo, *st)
+        stream.synchronize()
+        cg.synchronize()
+        eager[prec].synchronize()
+        for gv, e in zip((*st_g, *vel_g, lap_g), (*st, *vel, lap)):
+            np.testing.assert_array_equal(to_np(gv), to_np(e))
+
+
 @pytest.mark.parametrize("levels,m", [(2, 8), (3, 5)])
 def test_unify_boundary_vals_matches_oracle(sb, orc, torch_gpu, levels, m):
     """unify_boundary_vals (initialize_cubed_sphere.cpp:364-370) on the GPU against the oracle

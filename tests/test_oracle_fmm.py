@@ -1,7 +1,7 @@
 """Pins the oracle's tree, traversal, passes and pair sums against invariants,
 brute force and closed forms (initialize_cubed_sphere.cpp:64-288,
 fmm_funcs.cpp:17-123, 341-392, fmm_funcs_impl.hpp:17-188,
-bve_vel_impl.hpp:71-106, inverse_poisson_impl.hpp:181-210, direct_sum_impl.hpp:18-27)."""
+bve_vel_impl.hpp:34-70, inverse_poisson_impl.hpp:23-52, direct_sum_impl.hpp:18-27)."""
 import numpy as np
 import pytest
 

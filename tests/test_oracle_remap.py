@@ -134,3 +134,41 @@ def test_swe_remap_identity_and_rotation(orc, setup):
     for g, h in zip(out, f):
         nv, _ = o.bve_departure_to_target(z, dep, h, 0.0)
         np.testing.assert_allclose(g, nv, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("combine", [0, 1])
+def test_combine_first_order_limit(orc, setup, combine):
+    """Pins the RK4 combine of bve_find_departure_points (bve_time_step.cpp:288-303)
+    without restating it.  The four stages read four different solid rotations
+    v_s = w_s x X (k1 from stage 3, k2 from stage 2 at the first half point, k3 from
+    stage 1, k4 from stage 0: bve_time_step.cpp:212-286).  As dt -> 0 every evaluation
+    point tends to X and every projection removes only O(dt^2), so
+        (dep - X) / dt -> (c3 w3 + c2 w2 + c1 w1 + c0 w0) x X,
+    with (c3, c2, c1, c0) = (-1/6, +1/3, +1/3, +1/6) for the reference's signs
+    (x - dt/6 k1, then x -= -dt/3 k2, -= -dt/3 k3, -= -dt/6 k4) and
+    -(1/6, 1/3, 1/3, 1/6) for the textbook combine.  The rotations are not
+    parallel, so flipping any one sign, or swapping which stage feeds which k,
+    moves the limit by O(1); the remainder shrinks like dt."""
+    o, x, y, z, a = setup
+    ws = [np.array(v) * 1e-5 for v in ([0.3, -0.2, 1.0], [-0.7, 0.5, 0.1], [0.2, 0.9, -0.4], [1.1, 0.3, 0.6])]
+    X = np.stack([x, y, z], -1)
+    stages = []
+    for w in ws:  # stage s of the caller's list holds v_s
+        V = np.cross(np.broadcast_to(w, X.shape), X)
+        stages.append([V[..., 0].copy(), V[..., 1].copy(), V[..., 2].copy()])
+    coef = (1 / 6, 1 / 3, 1 / 3, -1 / 6) if combine == 0 else (-1 / 6, -1 / 3, -1 / 3, -1 / 6)  # c0, c1, c2, c3
+    wl = sum(c * w for c, w in zip(coef, ws))
+    limit = np.cross(np.broadcast_to(wl, X.shape), X)
+    scale = np.max(np.abs(limit))
+    errs = []
+    for dt in (3600.0, 360.0, 36.0):
+        dep = np.stack(o.bve_find_departure_points(x, y, z, stages, dt, combine), -1)
+        errs.append(np.max(np.abs((dep - X) / dt - limit)) / scale)
+    # first order: each factor 10 in dt divides the remainder by about 10
+    assert errs[2] < 2e-3, errs
+    assert errs[1] / errs[2] > 7 and errs[0] / errs[1] > 7, errs
+    # the other combine's limit is O(1) away: the pin tells the two apart
+    other = (-1 / 6, -1 / 3, -1 / 3, -1 / 6) if combine == 0 else (1 / 6, 1 / 3, 1 / 3, -1 / 6)
+    wo = sum(c * w for c, w in zip(other, ws))
+    lo = np.cross(np.broadcast_to(wo, X.shape), X)
+    assert np.max(np.abs(lo - limit)) / scale > 0.5

@@ -3,7 +3,7 @@ differentiable by torch.autograd: the reference gradient for the library's
 hand-written step adjoint.
 
 It follows bve_back_rk4_step (bve_time_step.cpp:311-419) with every velocity
-evaluation done as the O(N^2) Biot-Savart sum of bve_vel_impl.hpp:90-103 over
+evaluation done as the O(N^2) Biot-Savart sum of bve_vel_impl.hpp:54-66 over
 all particles (sources displaced by their own displacement, no proxies), the
 displacement update of :91-104, the backward RK4 departure search with the
 reference's combine sequence (:206-303) and the remap of :38-66.  Point
