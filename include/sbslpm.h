@@ -246,6 +246,18 @@ int sbslpm_bve_velocity_targets(sbslpm_ctx* ctx, const double* area, const doubl
  * area_j pots_j for i in [lb, ub) over n points. */
 int sbslpm_direct_sum_inv_lap(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                               const double* area, const double* pots, double* soln, int64_t lb, int64_t ub);
+/* Allocation-free sbslpm_direct_sum_inv_lap (no reference counterpart).  For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, same chunking and the same result to the bit; the partial sums live
+ * in a fixed 8 MiB device array (one per GPU, held until the process ends, its address looked up
+ * by sbslpm_ctx_create) instead of the context's on-demand scratch.  Calls no hipMalloc/hipFree.
+ * The partial sums of a slice [lb, ub) must fit 2^20 doubles, which holds for every slice of at
+ * most 2^20 = 1,048,576 targets; a longer slice returns SBSLPM_E_ARG: split it.  Every call on one GPU shares the array, so calls that overlap
+ * on one GPU (other streams, other contexts, other host threads) race; calls in order on one
+ * stream do not. */
+int sbslpm_direct_sum_inv_lap_noalloc(sbslpm_ctx* ctx, int64_t n, const double* x, const double* y,
+                                      const double* z, const double* area, const double* pots, double* soln,
+                                      int64_t lb, int64_t ub);
 /* Log direct sum at nt separate targets over ns sources; pairs with x.y >= 1 - 1e-15 skipped
  * (the FMM kernel's guard, inverse_poisson_impl.hpp:47).  soln [nt] accumulated. */
 int sbslpm_direct_sum_inv_lap_targets(sbslpm_ctx* ctx, int64_t nt, const double* tx, const double* ty,

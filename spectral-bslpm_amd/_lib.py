@@ -113,6 +113,7 @@ SIGNATURES = {
     "sbslpm_poisson_solve_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V]),
     "sbslpm_bve_velocity_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
     "sbslpm_direct_sum_inv_lap": (_I, [_V, _I64, _V, _V, _V, _V, _V, _V, _I64, _I64]),
+    "sbslpm_direct_sum_inv_lap_noalloc": (_I, [_V, _I64, _V, _V, _V, _V, _V, _V, _I64, _I64]),
     "sbslpm_direct_sum_inv_lap_targets": (_I, [_V, _I64, _V, _V, _V, _I64, _V, _V, _V, _V, _V, _V]),
     "sbslpm_direct_sum_bve_vel": (_I, [_V, _I64] + [_V] * 11 + [_I64, _I64]),
     "sbslpm_direct_sum_bve_vel_targets": (_I, [_V, _I64, _V, _V, _V, _I64] + [_V] * 8),

@@ -261,6 +261,12 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
     delete c;
     return SBSLPM_E_HIP;
   }
+  // device address of the fixed scratch of sbslpm_direct_sum_inv_lap_noalloc,
+  // looked up once per device while it is current
+  if (direct_noalloc_init(c->device) != 0) {
+    delete c;
+    return SBSLPM_E_HIP;
+  }
 
   // Device copies of the constant tables + workspace in one allocation.
   const size_t Pn = (size_t)t.P * t.n, NP = (size_t)t.npart, U = (size_t)t.U;
@@ -823,6 +829,26 @@ int sbslpm_direct_sum_inv_lap(sbslpm_ctx* c, int64_t n, const double* x, const d
   if (rc) return rc;
   launch_direct_inv_lap(nt, x + lb, y + lb, z + lb, n, x, y, z, area, pots, lb, false, c->scratch, soln + lb,
                         c->stream);
+  return launched(c);
+}
+
+int sbslpm_direct_sum_inv_lap_noalloc(sbslpm_ctx* c, int64_t n, const double* x, const double* y,
+                                      const double* z, const double* area, const double* pots, double* soln,
+                                      int64_t lb, int64_t ub) {
+  if (!c || !x || !y || !z || !area || !pots || !soln || n < 0 || lb < 0 || ub > n || lb > ub)
+    return fail(c, SBSLPM_E_ARG, "direct_sum_inv_lap_noalloc: bad argument");
+  const int64_t nt = ub - lb;
+  if (nt == 0) return SBSLPM_OK;
+  // same chunking as sbslpm_direct_sum_inv_lap, so the sums agree to the bit
+  const int64_t need = direct_partials_size(nt, n, 1);
+  if (need > kNoAllocScratchDoubles)
+    return fail(c, SBSLPM_E_ARG,
+                "direct_sum_inv_lap_noalloc: " + std::to_string(nt) + " targets need " + std::to_string(need) +
+                    " scratch doubles, above the fixed " + std::to_string(kNoAllocScratchDoubles) +
+                    "; split [lb, ub) into smaller slices");
+  double* buf = direct_noalloc_scratch(c->device);
+  if (!buf) return fail(c, SBSLPM_E_HIP, "direct_sum_inv_lap_noalloc: fixed scratch not set up on this device");
+  launch_direct_inv_lap(nt, x + lb, y + lb, z + lb, n, x, y, z, area, pots, lb, false, buf, soln + lb, c->stream);
   return launched(c);
 }
 

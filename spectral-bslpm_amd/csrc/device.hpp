@@ -328,6 +328,13 @@ void launch_unify_boundary_vals(int64_t n, const int* map, const int* counts, in
 
 // ---- direct sums (direct.hip)
 int64_t direct_partials_size(int64_t nt, int64_t ns, int ncomp);
+// Doubles in the fixed device scratch of the allocation-free direct sum (8 MiB).
+constexpr int64_t kNoAllocScratchDoubles = int64_t(1) << 20;
+// Looks up the fixed scratch's address on `device` (which must be current);
+// 0 on success, else the HIP error code.  Idempotent, thread-safe.
+int direct_noalloc_init(int device);
+// The address direct_noalloc_init found for `device`, or null before it ran.
+double* direct_noalloc_scratch(int device);
 void launch_direct_inv_lap(int64_t nt, const double* tx, const double* ty, const double* tz, int64_t ns,
                            const double* sx, const double* sy, const double* sz, const double* area,
                            const double* pots, int64_t self_offset, bool guard_coincident, double* partials,

@@ -7,6 +7,7 @@
 // and a second kernel adds the chunks in a fixed order, so the result is
 // deterministic and needs no atomics (the reference does one atomic per pair).
 #include <algorithm>
+#include <mutex>
 
 #include "device.hpp"
 
@@ -118,6 +119,36 @@ __global__ void k_displace(int64_t ns, const double* __restrict__ sx, const doub
 }
 
 }  // namespace
+
+// Fixed device scratch of the allocation-free direct sum
+// (sbslpm_direct_sum_inv_lap_noalloc): the per-chunk partial sums of
+// k_direct_log.  Each device holds its own copy until the process ends.  In
+// host code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, once per device, in direct_noalloc_init.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int direct_noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return (int)(e != hipSuccess ? e : hipErrorInvalidSymbol);
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return 0;
+}
+
+double* direct_noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 // Scratch doubles needed by a direct sum with nt targets, ns sources.
 int64_t direct_partials_size(int64_t nt, int64_t ns, int ncomp) {
