@@ -74,9 +74,24 @@ typedef struct sbslpm_params {
   double kernel_eps;  /* RunConfig::kernel_eps */
   double omega;       /* RunConfig::omega */
   double delta_t;     /* RunConfig::delta_t */
-  int topography;     /* RunConfig::topo_type: 0 none, 1 "cone" (topography.cpp:6-56) */
+  int topography;     /* SBSLPM_TOPO_*: RunConfig::topo_type 0 none, 1 "cone" (topography.cpp:6-56); 2 Gaussian */
   int swe_rule;       /* SBSLPM_SWE_RULE_*: the shallow-water step's gradient/topography indexing */
+  /* Shallow-water physics beyond the reference (all zero = the reference's step): */
+  double gravity;     /* coefficient of the free-surface Laplacian in the divergence tendency: g/a^2
+                       * for heights in metres on the unit sphere; 0 = the reference's 1 (no g/a^2) */
+  double mountain_height, mountain_width, mountain_lat, mountain_lon; /* SBSLPM_TOPO_GAUSSIAN:
+                       * h_s = height exp(-(r/width)^2), r the great-circle distance (radians) to (lat, lon) */
+  int swe_form;       /* SBSLPM_SWE_FORM_* bits, 0 = the reference */
 } sbslpm_params;
+
+/* Topography (RunConfig::topo_type, topography.cpp:47-56; 2 has no reference counterpart). */
+enum { SBSLPM_TOPO_NONE = 0, SBSLPM_TOPO_CONE = 1, SBSLPM_TOPO_GAUSSIAN = 2 };
+/* Shallow-water step forms (no reference counterpart; 0 is swe_back_rk4_step as written):
+ * REG_LAPLACIAN: the stage-1 surface Laplacian is the regularised-kernel FMM less 8/0.1 times each
+ *   node's own height (the operator of stages 2-3 at rest) instead of the spectral panel Laplacian;
+ * SURFACE_CORRECTION: the height correction subtracts 8/0.1 times the free surface (height plus
+ *   topography at the moved point) instead of the height alone (swe_time_step.cpp:598-600). */
+enum { SBSLPM_SWE_FORM_REG_LAPLACIAN = 1, SBSLPM_SWE_FORM_SURFACE_CORRECTION = 2 };
 
 /* ---- Host-only set-up (no device needed) ------------------------------------------- */
 typedef struct sbslpm_tree sbslpm_tree;
@@ -490,6 +505,34 @@ int sbslpm_unify_boundary_vals(sbslpm_ctx* ctx, const int* two_d_to_1d, const in
 int sbslpm_swe_back_rk4_step(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
                              const double* area, double* vors, double* divs, double* height, double* vel_x,
                              double* vel_y, double* vel_z, double* tracers, double* height_lap, double time);
+
+/* Shallow-water step adjoint (no reference counterpart; it differentiates swe_back_rk4_step,
+ * swe_time_step.cpp:606-811, in every form the context selects).  Needs tracer_count == 0.
+ * sbslpm_swe_back_rk4_step_record: one step out of place, new_{vors,divs,height} [Pl][m2]
+ * assigned, and stages [21][Pl][m2] = the stage velocities V0..V3 (x,y,z each) and the stage
+ * tendencies T0..T2 (vor, div, h each) the reverse pass reads. */
+int sbslpm_swe_back_rk4_step_record(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                    const double* area, const double* vors, const double* divs,
+                                    const double* height, double* new_vors, double* new_divs, double* new_height,
+                                    double* stages);
+/* Gradient of one step with respect to (vors, divs, height) before it, for the cotangents of
+ * (vors, divs, height) after it: {vors,divs,height}_bar [Pl][m2] assigned.  Reverse of the
+ * remap and departure search, then of stages 4..1: the shallow-water and regularised-Laplacian
+ * FMM evaluations transposed in their charges and displaced source positions (FP32 pair sums on
+ * mixed-precision contexts, like the forward), the height correction, the topography at displaced
+ * points, the tendencies with their leaf differentiation, the spectral panel Laplacian, the state
+ * and area updates.  Point location is piecewise constant and contributes nothing.  The forward
+ * is recomputed; the _stages form takes the record of sbslpm_swe_back_rk4_step_record instead. */
+int sbslpm_swe_back_rk4_step_adjoint(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                     const double* area, const double* vors, const double* divs,
+                                     const double* height, const double* new_vors_bar, const double* new_divs_bar,
+                                     const double* new_height_bar, double* vors_bar, double* divs_bar,
+                                     double* height_bar);
+int sbslpm_swe_back_rk4_step_adjoint_stages(sbslpm_ctx* ctx, const double* x, const double* y, const double* z,
+                                            const double* area, const double* vors, const double* divs,
+                                            const double* height, const double* stages, const double* new_vors_bar,
+                                            const double* new_divs_bar, const double* new_height_bar,
+                                            double* vors_bar, double* divs_bar, double* height_bar);
 
 /* bve_back_rk4_step (bve_time_step.cpp:311-419), no forcing (bve_forcing copy,
  * forcing_funcs.cpp:93-95).  vor [Pl][m2] and tracers [Pl][m2][tracer_count] updated in place;

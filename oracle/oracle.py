@@ -121,6 +121,8 @@ def lib():
         L.orc_irreg_downward_pass_3.restype = ctypes.c_int
         L.orc_irreg_swe_vel_interactions.argtypes = [ctypes.c_void_p, _ppd, _ppd, _dp, _dp, _dp, _dp, _dp,
                                                      ctypes.c_double]
+        L.orc_set_swe_physics.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5 + [ctypes.c_int]
+        L.orc_set_swe_physics.restype = None
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -437,6 +439,13 @@ class OracleFMM:
         if lib().orc_velocity_gradient(self.h, _pp([self._f(v) for v in vel]), rule, _pp(g)):
             raise RuntimeError("oracle velocity_gradient: geometry error")
         return g
+
+    def set_swe_physics(self, gravity=1.0, mountain=(0.0, 1.0, 0.0, 0.0), form=0):
+        """Coefficient of the free-surface Laplacian in the divergence tendency (1 in the
+        reference; g/a^2 for heights in metres), the Gaussian mountain of topography 2:
+        (height, width in radians, centre latitude, centre longitude), and the step form
+        bits (1: regularised stage-1 Laplacian, 2: free-surface height correction)."""
+        lib().orc_set_swe_physics(self.h, float(gravity), *[float(v) for v in mountain], int(form))
 
     def swe_tendency(self, x, y, z, vel, vor, div, height, lap, omega, rule=0):
         """swe_tendency_computation (swe_time_step.cpp:76-161): (vor_t, div_t, h_t)."""

@@ -282,6 +282,17 @@ struct Ctx {
   std::vector<int> ilist;            // [I][3] = target, source, type (reference BFS order)
   std::vector<int> by_target;        // interaction indices grouped by target, list order kept
   std::vector<int> tgt_start;        // [P+1]
+  // shallow-water physics beyond the reference: the coefficient of the
+  // free-surface Laplacian in the divergence tendency (the reference's 1, or
+  // g/a^2 for heights in metres) and the Gaussian mountain of topography 2
+  // (height, width in radians, centre latitude and longitude)
+  double gravity = 1.0;
+  double mtn[4] = {0, 1, 0, 0};
+  // shallow-water step form: bit 0, the stage-1 surface Laplacian by the
+  // regularised kernel with the correction at the nodes (the stages-2-3
+  // operator); bit 1, the height correction subtracts the free surface
+  int swe_form = 0;
+  int topo_kind = 0;  // topography of the running step (for the free-surface correction)
 };
 
 // initialize_cubed_sphere.cpp:73-140 (children and bounds)
@@ -1680,7 +1691,8 @@ int orc_velocity_gradient(void* h, const double* const* vel, int rule, double* c
 
 // swe_tendency_computation (swe_time_step.cpp:76-161), per particle:
 //   vor_t = -div (vor + 2 Omega z) - 2 Omega u_z
-//   div_t = 2 Omega z vor - sum_ab G_ab G_ba - lap - |u|^2 - 2 Omega (-y u_x + x u_y) [x^2+y^2 > 1e-16]
+//   div_t = 2 Omega z vor - sum_ab G_ab G_ba - gravity lap - |u|^2 - 2 Omega (-y u_x + x u_y) [x^2+y^2 > 1e-16]
+// (gravity = 1 in the reference, which omits g/a^2)
 //   h_t   = -h div
 int orc_swe_tendency(void* h, const double* x, const double* y, const double* z, const double* const* vel,
                      const double* vor, const double* div, const double* height, const double* lap, double omega,
@@ -1702,7 +1714,7 @@ int orc_swe_tendency(void* h, const double* x, const double* y, const double* z,
     const double gc = G11 * G11 + G12 * G21 + G13 * G31 + G21 * G12 + G22 * G22 + G23 * G32 + G31 * G13 +
                       G32 * G23 + G33 * G33;
     d -= gc;
-    d -= lap[k];
+    d -= c.gravity * lap[k];
     d -= ux * ux + uy * uy + uz * uz;
     if (x[k] * x[k] + y[k] * y[k] > 1e-16) d -= 2.0 * omega * (-y[k] * ux + x[k] * uy);
     div_t[k] = d;
@@ -1713,6 +1725,8 @@ int orc_swe_tendency(void* h, const double* x, const double* y, const double* z,
 
 // cone_mountain (topography.cpp:6-45) at P(x + disp): eff = h + h0 exp(-16 r^2),
 // h0 = 2000 / 6371000, r^2 = (lon - pi/2)^2 + (lat - pi/6)^2, lon in [0, 2 pi).
+// topo 2 (no reference counterpart): the Gaussian mountain
+// eff = h + H exp(-(r / w)^2), r the great-circle distance to the centre.
 // topo 0: eff = h (apply_topography's copy, :47-55).  The displacement read at
 // leaf-local i, node j is disp[i][j] of the full [P][n] array (rule 0, the
 // reference: panel i, which is not leaf i; needs m2 == n) or the leaf's own
@@ -1726,7 +1740,11 @@ int orc_apply_topography(void* h, const double* x, const double* y, const double
     return ORC_OK;
   }
   if (rule == 0 && c.m2 != c.n) return ORC_E_ARG;
+  if (topo != 1 && topo != 2) return ORC_E_ARG;
   const double h0 = 2000.0 / 6371000.0, lc = 0.5 * kPi, tc = kPi / 6.0;
+  // Gaussian mountain centre (topography 2)
+  const double mc[3] = {std::cos(c.mtn[2]) * std::cos(c.mtn[3]), std::cos(c.mtn[2]) * std::sin(c.mtn[3]),
+                        std::sin(c.mtn[2])};
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < c.Pl; i++)
     for (int j = 0; j < c.m2; j++) {
@@ -1742,6 +1760,14 @@ int orc_apply_topography(void* h, const double* x, const double* y, const double
       }
       double a = x[k] + d[0], b = y[k] + d[1], cc = z[k] + d[2];
       project_to_sphere(a, b, cc);
+      if (topo == 2) {
+        // great-circle distance to the centre as atan2(|p x c|, p.c)
+        const double cx = b * mc[2] - cc * mc[1], cy = cc * mc[0] - a * mc[2], cz = a * mc[1] - b * mc[0];
+        const double r = std::atan2(std::sqrt(cx * cx + cy * cy + cz * cz), a * mc[0] + b * mc[1] + cc * mc[2]);
+        const double q = r / c.mtn[1];
+        eff[k] = height[k] + c.mtn[0] * std::exp(-q * q);
+        continue;
+      }
       const double lat = kPi / 2.0 - std::atan2(std::sqrt(a * a + b * b), cc);
       double lon = std::atan2(b, a);
       lon += 2 * kPi;
@@ -1817,6 +1843,17 @@ void orc_swe_apply_tendencies(void* h, const double* const* f, const double* con
 
 // swe_height_correction (swe_time_step.cpp:558-604): at P(x + dt vel) the BLI
 // of h + dt h_t; lap -= 8 new_h / 0.1.
+// Height of topography 2 at a unit vector (0 for other topographies).
+static double topo_height(const Ctx& c, double a, double b, double cc) {
+  if (c.topo_kind != 2) return 0.0;
+  const double mc[3] = {std::cos(c.mtn[2]) * std::cos(c.mtn[3]), std::cos(c.mtn[2]) * std::sin(c.mtn[3]),
+                        std::sin(c.mtn[2])};
+  const double cx = b * mc[2] - cc * mc[1], cy = cc * mc[0] - a * mc[2], cz = a * mc[1] - b * mc[0];
+  const double r = std::atan2(std::sqrt(cx * cx + cy * cy + cz * cz), a * mc[0] + b * mc[1] + cc * mc[2]);
+  const double q = r / c.mtn[1];
+  return c.mtn[0] * std::exp(-q * q);
+}
+
 int orc_swe_height_correction(void* h, const double* x, const double* y, const double* z, const double* const* vel,
                               const double* height, const double* h_t, double dt, double* lap) {
   Ctx& c = *static_cast<Ctx*>(h);
@@ -1831,6 +1868,7 @@ int orc_swe_height_correction(void* h, const double* x, const double* y, const d
     project_to_sphere(a, b, cc);
     double nh;
     err |= interp_at(c, a, b, cc, 1, f1, &nh);
+    if (c.swe_form & 2) nh += topo_height(c, a, b, cc);
     lap[k] -= 8.0 * nh / 0.1;
   }
   return err ? ORC_E_GEOMETRY : ORC_OK;
@@ -1906,8 +1944,20 @@ int orc_swe_back_rk4_step(void* h, const double* x, const double* y, const doubl
   double* S[3] = {st.data(), st.data() + NP, st.data() + 2 * NP};  // displaced vor, div, h
   const double* state[3] = {vors, divs, height};
   // stage 1
+  c.topo_kind = topo;
   int err = orc_apply_topography(h, x, y, z, dp3, dx3, height, topo, rule, eff.data());
-  err |= orc_panel_laplacian(h, eff.data(), height_lap);
+  if (c.swe_form & 1) {
+    // the regularised-kernel Laplacian of stages 2-3 at the undisplaced nodes
+    // less 8/0.1 times each node's own height (or free surface)
+    err |= lap_fmm(c, area, eff.data(), dp3, dx3, height_lap);
+    for (size_t k = 0; k < NP; k++) {
+      double v = height[k];
+      if (c.swe_form & 2) v += topo_height(c, x[k], y[k], z[k]);
+      height_lap[k] -= 8.0 * v / 0.1;
+    }
+  } else {
+    err |= orc_panel_laplacian(h, eff.data(), height_lap);
+  }
   err |= swe_fmm_velocity(c, area, vors, divs, dp3, dx3, eps, V[0]);
   const double* cv[3] = {V[0][0], V[0][1], V[0][2]};
   err |= orc_swe_tendency(h, x, y, z, cv, vors, divs, height, height_lap, omega, rule, T[0][0], T[0][1], T[0][2]);
@@ -1944,6 +1994,19 @@ int orc_swe_back_rk4_step(void* h, const double* x, const double* y, const doubl
   if (ntr > 0) std::memcpy(tracers, ntr_buf.data(), NP * ntr * sizeof(double));
   if (err & ORC_E_ARG) return ORC_E_ARG;
   return err ? ORC_E_GEOMETRY : ORC_OK;
+}
+
+// Shallow-water physics beyond the reference (no counterpart there): the
+// coefficient of the free-surface Laplacian in the divergence tendency, the
+// Gaussian mountain of topography 2 and the step form bits.
+void orc_set_swe_physics(void* h, double gravity, double height, double width, double lat, double lon, int form) {
+  Ctx& c = *static_cast<Ctx*>(h);
+  c.swe_form = form;
+  c.gravity = gravity;
+  c.mtn[0] = height;
+  c.mtn[1] = width;
+  c.mtn[2] = lat;
+  c.mtn[3] = lon;
 }
 
 void orc_set_threads(int n) { omp_set_num_threads(n); }

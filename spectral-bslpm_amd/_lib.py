@@ -35,6 +35,12 @@ class Params(ctypes.Structure):
         ("delta_t", ctypes.c_double),
         ("topography", ctypes.c_int),
         ("swe_rule", ctypes.c_int),
+        ("gravity", ctypes.c_double),
+        ("mountain_height", ctypes.c_double),
+        ("mountain_width", ctypes.c_double),
+        ("mountain_lat", ctypes.c_double),
+        ("mountain_lon", ctypes.c_double),
+        ("swe_form", ctypes.c_int),
     ]
 
 
@@ -123,6 +129,9 @@ SIGNATURES = {
     "sbslpm_swe_tendency": (_I, [_V] * 14),
     "sbslpm_swe_height_correction": (_I, [_V] * 9 + [_D, _V]),
     "sbslpm_swe_back_rk4_step": (_I, [_V] * 13 + [_D]),
+    "sbslpm_swe_back_rk4_step_record": (_I, [_V] * 12),
+    "sbslpm_swe_back_rk4_step_adjoint_stages": (_I, [_V] * 15),
+    "sbslpm_swe_back_rk4_step_adjoint": (_I, [_V] * 14),
     "sbslpm_bve_displacement": (_I, [_V] + [_V] * 7 + [_D] + [_V] * 7),
     "sbslpm_bve_find_departure_points": (_I, [_V, _V, _V, _V, _V, _D, _V, _V, _V]),
     "sbslpm_bve_departure_to_target": (_I, [_V, _V, _V, _V, _V, _V, _V, _I, _V, _V]),

@@ -17,7 +17,8 @@ from ._lib import Params, SbslpmError, load
 
 WEIGHTS = {"reference": 0, "clenshaw_curtis": 1, "equal_area": 2}
 COMBINE = {"reference": 0, "textbook": 1}
-TOPOGRAPHY = {"none": 0, "cone": 1}
+TOPOGRAPHY = {"none": 0, "cone": 1, "gaussian": 2}
+SWE_FORM = {"reg_laplacian": 1, "surface_correction": 2}
 SWE_RULE = {"reference": 0, "leaf": 1}
 PRECISION = {"fp64": 0, "mixed": 1, "mixed_valu": 2}
 
@@ -38,8 +39,11 @@ class RunConfig:
     tracer_count: int = 0
     use_graph: bool = False
     multi_rhs: bool = False          # ensembles: first stage as one multi-RHS product (mixed only)
-    topography: str = "none"         # RunConfig::topo_type: none | cone
+    topography: str = "none"         # RunConfig::topo_type: none | cone | gaussian
     swe_rule: str = "reference"      # reference | leaf (shallow-water step indexing, see sbslpm.h)
+    gravity: float = 0.0             # free-surface Laplacian coefficient (g/a^2 in metres); 0: the reference's 1
+    mountain: tuple = (0.0, 1.0, 0.0, 0.0)  # gaussian topography: height, width (rad), centre lat, lon
+    swe_form: tuple = ()             # subset of SWE_FORM: departures from swe_back_rk4_step as written
 
     def to_params(self):
         p = Params()
@@ -57,6 +61,9 @@ class RunConfig:
         p.delta_t = self.delta_t
         p.topography = TOPOGRAPHY[self.topography]
         p.swe_rule = SWE_RULE[self.swe_rule]
+        p.gravity = self.gravity
+        p.mountain_height, p.mountain_width, p.mountain_lat, p.mountain_lon = [float(v) for v in self.mountain]
+        p.swe_form = sum(SWE_FORM[f] for f in self.swe_form)
         return p
 
 
@@ -729,6 +736,36 @@ class Context:
                                                        self._p(height_lap, what="height_lap"), float(time)),
                     "swe_back_rk4_step")
         return vel, height_lap
+
+    def swe_back_rk4_step_record(self, x, y, z, area, vors, divs, height, out=None, stages=None):
+        """One shallow-water step out of place, keeping what its reverse pass reads: returns
+        (new_vors, new_divs, new_height, stages), stages [21][Pl][m2] = stage velocities V0..V3
+        and tendencies T0..T2.  Needs tracer_count == 0."""
+        out = [self.zeros_part() for _ in range(3)] if out is None else list(_seq(out, 3, "out"))
+        if stages is None:
+            stages = self.torch.zeros((21, self.sizes.Pl, self.sizes.m2), dtype=self.torch.float64, device="cuda")
+        st = self._p(stages, 21, "stages")
+        a = [self._p(t, what="swe_back_rk4_step_record input") for t in (x, y, z, area, vors, divs, height)]
+        self._check(self._lib.sbslpm_swe_back_rk4_step_record(self.h, *a, *[self._p(t, what="out") for t in out], st),
+                    "swe_back_rk4_step_record")
+        return out[0], out[1], out[2], stages
+
+    def swe_back_rk4_step_adjoint(self, x, y, z, area, vors, divs, height, bars, stages=None, out=None):
+        """Gradient of one shallow-water step (swe_back_rk4_step, swe_time_step.cpp:606-811) with
+        respect to (vors, divs, height) before it, for the cotangents bars = (vors, divs, height)
+        after it.  With stages (from swe_back_rk4_step_record) the forward is not recomputed.
+        Returns (vors_bar, divs_bar, height_bar)."""
+        out = [self.zeros_part() for _ in range(3)] if out is None else list(_seq(out, 3, "out"))
+        a = [self._p(t, what="swe_back_rk4_step_adjoint input") for t in (x, y, z, area, vors, divs, height)]
+        nb = [self._p(t, what="bars") for t in _seq(bars, 3, "bars")]
+        ob = [self._p(t, what="out") for t in out]
+        if stages is None:
+            self._check(self._lib.sbslpm_swe_back_rk4_step_adjoint(self.h, *a, *nb, *ob), "swe_back_rk4_step_adjoint")
+        else:
+            self._check(self._lib.sbslpm_swe_back_rk4_step_adjoint_stages(self.h, *a, self._p(stages, 21, "stages"),
+                                                                          *nb, *ob),
+                        "swe_back_rk4_step_adjoint_stages")
+        return out[0], out[1], out[2]
 
     def swe_find_departure_points(self, x, y, z, vels, dt, dep=None):
         """swe_find_departure_points, swe_time_step.cpp:422-556."""

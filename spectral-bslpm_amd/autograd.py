@@ -13,6 +13,9 @@ sbslpm_bve_departure_to_target_adjoint, sbslpm_bve_back_rk4_step_adjoint).
   bve_back_rk4_step calls (bve_time_step.cpp:311-419) from ``vor0``;
   differentiable in ``vor0`` through the full nonlinear step (trajectories
   included), checkpointed every ``every`` steps.
+* ``swe_steps(ctx, x, y, z, area, vor0, div0, h0, nsteps, every)`` -- the
+  same for swe_back_rk4_step (swe_time_step.cpp:606-811), differentiable in
+  the three initial fields (sbslpm_swe_back_rk4_step_adjoint_stages).
 
 The first two are linear in the differentiated argument; their backward
 passes are the exact transposes of the library's FP64 operators.
@@ -121,6 +124,84 @@ def bve_steps(sbctx, x, y, z, area, vor0, nsteps, every=1):
     if nsteps < 1 or every < 1:
         raise ValueError("nsteps and every must be positive")
     return _BVESteps.apply(sbctx, x, y, z, area, vor0, nsteps, every)
+
+
+class _SWESteps(torch.autograd.Function):
+    """nsteps shallow-water steps (swe_back_rk4_step, swe_time_step.cpp:606-811) from the state
+    (vor, div, h), checkpointed like _BVESteps: the forward keeps the state every `every` steps
+    and, for the last segment, each step's record (sbslpm_swe_back_rk4_step_record: the stage
+    velocities and tendencies); the backward reruns each earlier segment from its checkpoint
+    with records and runs the reverse passes (sbslpm_swe_back_rk4_step_adjoint_stages) in
+    reverse order."""
+
+    @staticmethod
+    def forward(fctx, sbctx, x, y, z, area, vor0, div0, h0, nsteps, every):
+        nseg = (nsteps + every - 1) // every
+        last = (nseg - 1) * every
+        st = tuple(t.detach().clone() for t in (vor0, div0, h0))
+        checkpoints, tail = [tuple(t.clone() for t in st)], []
+        vel = lap = None
+        for n in range(nsteps):
+            if n >= last:
+                nv, nd, nh, rec = sbctx.swe_back_rk4_step_record(x, y, z, area, *st)
+                tail.append((st, rec))
+                st = (nv, nd, nh)
+            else:
+                if vel is None:
+                    vel, lap = sbctx.zeros_part(3), sbctx.zeros_part()
+                sbctx.swe_back_rk4_step(x, y, z, area, *st, vel=vel, height_lap=lap)
+            if (n + 1) % every == 0 and n + 1 < nsteps:
+                checkpoints.append(tuple(t.clone() for t in st))
+        fctx.sbctx, fctx.nsteps, fctx.every = sbctx, nsteps, every
+        fctx.geom = (x, y, z, area)
+        fctx.checkpoints, fctx.tail = checkpoints, tail
+        return st
+
+    @staticmethod
+    def backward(fctx, gv, gd, gh):
+        sbctx, nsteps, every = fctx.sbctx, fctx.nsteps, fctx.every
+        x, y, z, area = fctx.geom
+        ref = fctx.checkpoints[0][0]
+        gbar = [torch.zeros_like(ref) if g is None else g.contiguous() for g in (gv, gd, gh)]
+        for seg in reversed(range(len(fctx.checkpoints))):
+            start, end = seg * every, min((seg + 1) * every, nsteps)
+            if seg == len(fctx.checkpoints) - 1:
+                rec = fctx.tail
+            else:
+                rec, st = [], fctx.checkpoints[seg]
+                for _ in range(start, end):
+                    nv, nd, nh, stages = sbctx.swe_back_rk4_step_record(x, y, z, area, *st)
+                    rec.append((st, stages))
+                    st = (nv, nd, nh)
+            for state, stages in reversed(rec):
+                gbar = list(sbctx.swe_back_rk4_step_adjoint(x, y, z, area, *state, gbar, stages=stages))
+        fctx.tail = None
+        return None, None, None, None, None, gbar[0], gbar[1], gbar[2], None, None
+
+
+def swe_steps(sbctx, x, y, z, area, vor0, div0, h0, nsteps, every=1):
+    """(vor, div, h) after nsteps shallow-water steps; backward: the hand-written step adjoints
+    with checkpoints every `every` steps.  The context needs tracer_count == 0."""
+    if nsteps < 1 or every < 1:
+        raise ValueError("nsteps and every must be positive")
+    return _SWESteps.apply(sbctx, x, y, z, area, vor0, div0, h0, nsteps, every)
+
+
+def swe_4dvar(sbctx, x, y, z, area, state0, nsteps, every, obs_xyz, obs_vor, obs_h, sigma_vor=1.0, sigma_h=1.0):
+    """Loss and gradient of a shallow-water 4D-Var problem: the initial state (vor, div, h) is
+    advanced nsteps steps and its vorticity and height are compared with observations at the
+    points obs_xyz (three 1-D tensors) at the final time,
+      L = 1/2 sum ((H(vor_N) - obs_vor) / sigma_vor)^2 + 1/2 sum ((H(h_N) - obs_h) / sigma_h)^2,
+    H the BLI at the points.  The gradient comes from the hand-written step adjoints
+    (checkpoints every `every` steps) and the transposed observation operator.  Returns
+    (L, (dL/dvor0, dL/ddiv0, dL/dh0))."""
+    w = [t.detach().clone().requires_grad_(True) for t in state0]
+    vn, dn, hn = swe_steps(sbctx, x, y, z, area, w[0], w[1], w[2], nsteps, every)
+    hv = interp_points(sbctx, obs_xyz[0], obs_xyz[1], obs_xyz[2], vn)
+    hh = interp_points(sbctx, obs_xyz[0], obs_xyz[1], obs_xyz[2], hn)
+    loss = 0.5 * (((hv - obs_vor) / sigma_vor) ** 2).sum() + 0.5 * (((hh - obs_h) / sigma_h) ** 2).sum()
+    loss.backward()
+    return loss.detach(), tuple(t.grad.detach() for t in w)
 
 
 class _InterpPoints(torch.autograd.Function):

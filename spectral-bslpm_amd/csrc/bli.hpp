@@ -244,6 +244,51 @@ __device__ inline void wave_scatter_leaf(double* __restrict__ lds, int leaf, con
   }
 }
 
+// Reverse of k = BLI[f](p) for NF leaf fields at point p, first part:
+// pbar += sum_q kbar_q grad_p f_q(p); returns the leaf (-1 when location
+// fails) and the 1-D weights for the transposed interpolation fbar += B kbar,
+// which the caller applies with wave_scatter_leaf.
+template <int M, int NF>
+__device__ inline int interp_grad(const DevTree& t, double x, double y, double z, const double* const* f,
+                                  const double* kbar, double* pbar, double* wx, double* we, bool report) {
+  double xe[2];
+  const int leaf = locate_leaf(t, x, y, z, xe);
+  if (leaf < 0) {
+    if (report) flag(t, kErrLocate);
+    return -1;
+  }
+  const DevPanel& P = t.panels[leaf];
+  double dwx[M], dwe[M];
+  bool ok = bli_wd<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx, dwx);
+  ok = bli_wd<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we, dwe) && ok;
+  if (!ok && report) flag(t, kErrBliBounds);
+  double gxi[3], geta[3];
+  xieta_gradient(x, y, z, P.face, gxi, geta);
+  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
+  double sxi = 0, seta = 0;  // sum_q kbar_q d f_q / d xi, d eta
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const double* __restrict__ F = f[q] + base;
+    double dxi = 0, deta = 0;
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+      double s = 0, sd = 0;
+#pragma unroll
+      for (int c = 0; c < M; ++c) {
+        s = fma(F[a * M + c], we[c], s);
+        sd = fma(F[a * M + c], dwe[c], sd);
+      }
+      dxi = fma(dwx[a], s, dxi);
+      deta = fma(wx[a], sd, deta);
+    }
+    sxi = fma(kbar[q], dxi, sxi);
+    seta = fma(kbar[q], deta, seta);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) pbar[k] += sxi * gxi[k] + seta * geta[k];
+  return leaf;
+}
+
 inline unsigned blocks_for(int64_t count, int bs) { return (unsigned)((count + bs - 1) / bs); }
 
 // Runs F<M>::run(args...) for the particle-grid side m (2..11).

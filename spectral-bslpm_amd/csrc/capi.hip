@@ -116,6 +116,14 @@ struct sbslpm_ctx {
     double *dpart[3], *dprox[3], *eff, *lap, *v[3][3], *tend[3][3], *st[3], *da, *arr[3], *dep[3], *ntr;
     double *qp, *qx, *pp, *px;
   } sws{};
+  // shallow-water step adjoint workspace, allocated on the first SWE adjoint call
+  char* swa_arena = nullptr;
+  struct SweAdj {
+    double *vb[12], *tb[9], *rb[3], *lapbar, *qzp, *qzx, *qdp, *qdx, *qep, *qex, *qz, *qd, *qe, *dbp[3], *dbx[3];
+    double *hscat, *hh, *stages;
+    void* trec;
+    SrcRec64* srec;
+  } swa{};
 
   // Kernel timing (sbslpm_ctx_enable_timing).
   bool timing = false;
@@ -247,6 +255,12 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
     return SBSLPM_E_ARG;
   }
   if (p->tracer_count < 0 || !(p->kernel_eps >= 0)) {
+    delete c;
+    return SBSLPM_E_ARG;
+  }
+  if (p->topography < SBSLPM_TOPO_NONE || p->topography > SBSLPM_TOPO_GAUSSIAN || !(p->gravity >= 0) ||
+      (p->swe_form & ~(SBSLPM_SWE_FORM_REG_LAPLACIAN | SBSLPM_SWE_FORM_SURFACE_CORRECTION)) ||
+      (p->topography == SBSLPM_TOPO_GAUSSIAN && !(p->mountain_width > 0))) {
     delete c;
     return SBSLPM_E_ARG;
   }
@@ -399,6 +413,7 @@ void sbslpm_ctx_destroy(sbslpm_ctx* c) {
   if (c->swe_arena) hipFree(c->swe_arena);
   if (c->swe_graph) hipGraphExecDestroy(c->swe_graph);
   if (c->sws_arena) hipFree(c->sws_arena);
+  if (c->swa_arena) hipFree(c->swa_arena);
   if (c->adj_arena) hipFree(c->adj_arena);
   if (c->ens_graph) hipGraphExecDestroy(c->ens_graph);
   if (c->ens_arena) hipFree(c->ens_arena);
@@ -1037,6 +1052,21 @@ static int laplacian_fmm(sbslpm_ctx* c, const double* area, const double* f, con
   return launched(c);
 }
 
+// The context's shallow-water physics for the kernels.
+static SwePhys swe_phys(const sbslpm_ctx* c) {
+  const sbslpm_params& p = c->prm;
+  SwePhys ph{};
+  ph.topo = p.topography;
+  ph.surface_correction = (p.swe_form & SBSLPM_SWE_FORM_SURFACE_CORRECTION) ? 1 : 0;
+  ph.gravity = p.gravity > 0 ? p.gravity : 1.0;
+  ph.mtn_h = p.mountain_height;
+  ph.mtn_w = p.mountain_width > 0 ? p.mountain_width : 1.0;
+  ph.mc[0] = cos(p.mountain_lat) * cos(p.mountain_lon);
+  ph.mc[1] = cos(p.mountain_lat) * sin(p.mountain_lon);
+  ph.mc[2] = sin(p.mountain_lat);
+  return ph;
+}
+
 static bool swe_rule_ok(sbslpm_ctx* c) {
   return !(c->prm.topography && c->prm.swe_rule == SBSLPM_SWE_RULE_REFERENCE && c->tree.m2 != c->tree.n);
 }
@@ -1079,7 +1109,7 @@ int sbslpm_apply_topography(sbslpm_ctx* c, const double* x, const double* y, con
   }
   const double* dp[3] = {dpx, dpy, dpz};
   const double* dx[3] = {dxx, dxy, dxz};
-  launch_topography(c->dt, x, y, z, dp, dx, height, c->prm.swe_rule, eff, c->stream);
+  launch_topography(c->dt, x, y, z, dp, dx, height, c->prm.swe_rule, swe_phys(c), eff, c->stream);
   return launched(c);
 }
 
@@ -1090,7 +1120,8 @@ int sbslpm_swe_tendency(sbslpm_ctx* c, const double* x, const double* y, const d
     return fail(c, SBSLPM_E_ARG, "swe_tendency: null pointer");
   const double* v[3] = {vx, vy, vz};
   double* o[3] = {vor_t, div_t, h_t};
-  launch_swe_tendency(c->dt, x, y, z, v, vor, div, h, lap, c->prm.omega, c->prm.swe_rule, o, c->stream);
+  launch_swe_tendency(c->dt, x, y, z, v, vor, div, h, lap, c->prm.omega, c->prm.swe_rule, swe_phys(c).gravity, o,
+                      c->stream);
   return launched(c);
 }
 
@@ -1100,7 +1131,7 @@ int sbslpm_swe_height_correction(sbslpm_ctx* c, const double* x, const double* y
   if (!c || !x || !y || !z || !vx || !vy || !vz || !h || !h_t || !lap)
     return fail(c, SBSLPM_E_ARG, "swe_height_correction: null pointer");
   const double* v[3] = {vx, vy, vz};
-  launch_height_correction(c->dt, x, y, z, v, h, h_t, dt, lap, c->stream);
+  launch_height_correction(c->dt, x, y, z, v, h, h_t, dt, swe_phys(c), lap, c->stream);
   return launched(c);
 }
 
@@ -1117,7 +1148,7 @@ int sbslpm_unify_boundary_vals(sbslpm_ctx* c, const int* two_d_to_1d, const int*
 
 static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                          double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
-                         double* tracers, double* height_lap);
+                         double* tracers, double* height_lap, double* record = nullptr);
 
 int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                              double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
@@ -1175,15 +1206,17 @@ int sbslpm_swe_back_rk4_step(sbslpm_ctx* c, const double* x, const double* y, co
 }
 
 // The kernels of one shallow-water step (eager or under capture); the
-// workspaces exist already.
+// workspaces exist already.  record (optional, [21][Pl][m2]): the stage
+// velocities V0..V3 and the tendencies T0..T2 the reverse pass reads.
 static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
                          double* vors, double* divs, double* height, double* vx, double* vy, double* vz,
-                         double* tracers, double* height_lap) {
+                         double* tracers, double* height_lap, double* record) {
   int rc = SBSLPM_OK;
   const int ntr = c->prm.tracer_count;
   const Tree& t = c->tree;
   auto& w = c->sws;
   const int rule = c->prm.swe_rule, topo = c->prm.topography;
+  const SwePhys ph = swe_phys(c);
   const double dt = c->prm.delta_t;
   const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
   const double* state[3] = {vors, divs, height};
@@ -1193,14 +1226,22 @@ static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const 
   // stage 1: undisplaced; the spectral Laplacian of the effective height
   const double* eff = height;
   if (topo) {
-    launch_topography(c->dt, x, y, z, nullptr, nullptr, height, rule, w.eff, c->stream);
+    launch_topography(c->dt, x, y, z, nullptr, nullptr, height, rule, ph, w.eff, c->stream);
     eff = w.eff;
   }
-  launch_panel_laplacian(c->dt, eff, height_lap, c->stream);
+  if (c->prm.swe_form & SBSLPM_SWE_FORM_REG_LAPLACIAN) {
+    // the operator of stages 2-3 at the undisplaced nodes: regularised-kernel
+    // sums of area * eff, less 8/0.1 times the node's own height (or free surface)
+    rc = laplacian_fmm(c, area, eff, nullptr, nullptr, height_lap);
+    if (rc) return rc;
+    launch_node_correction(t.npart, x, y, z, height, ph, height_lap, c->stream);
+  } else {
+    launch_panel_laplacian(c->dt, eff, height_lap, c->stream);
+  }
   rc = sbslpm_swe_velocity(c, 1, area, vors, divs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vx, vy, vz);
   if (rc) return rc;
-  launch_swe_tendency(c->dt, x, y, z, vel1, vors, divs, height, height_lap, c->prm.omega, rule, w.tend[0],
-                      c->stream);
+  launch_swe_tendency(c->dt, x, y, z, vel1, vors, divs, height, height_lap, c->prm.omega, rule, ph.gravity,
+                      w.tend[0], c->stream);
   // stages 2..4
   for (int s = 1; s <= 3; ++s) {
     double* const* vprev = s == 1 ? vel1 : w.v[s - 2];
@@ -1217,14 +1258,15 @@ static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const 
     if (s == 3) break;
     const double* effs = w.st[2];
     if (topo) {
-      launch_topography(c->dt, x, y, z, cdp, cdx, w.st[2], rule, w.eff, c->stream);
+      launch_topography(c->dt, x, y, z, cdp, cdx, w.st[2], rule, ph, w.eff, c->stream);
       effs = w.eff;
     }
     rc = laplacian_fmm(c, w.da, effs, cdp, cdx, w.lap);
     if (rc) return rc;
-    launch_height_correction(c->dt, x, y, z, cvprev, height, w.tend[s - 1][2], taus[s - 1], w.lap, c->stream);
+    launch_height_correction(c->dt, x, y, z, cvprev, height, w.tend[s - 1][2], taus[s - 1], ph, w.lap, c->stream);
     const double* cv[3] = {w.v[s - 1][0], w.v[s - 1][1], w.v[s - 1][2]};
-    launch_swe_tendency(c->dt, x, y, z, cv, vors, divs, height, w.lap, c->prm.omega, rule, w.tend[s], c->stream);
+    launch_swe_tendency(c->dt, x, y, z, cv, vors, divs, height, w.lap, c->prm.omega, rule, ph.gravity, w.tend[s],
+                        c->stream);
   }
   // arrival values (state + dt k1), departure points, remap
   const double* t1[3] = {w.tend[0][0], w.tend[0][1], w.tend[0][2]};
@@ -1234,10 +1276,305 @@ static int swe_step_body(sbslpm_ctx* c, const double* x, const double* y, const 
   launch_bve_departure(c->dt, x, y, z, vels, dt, c->prm.rk4_combine, w.dep[0], w.dep[1], w.dep[2], c->stream);
   const double* arr[3] = {w.arr[0], w.arr[1], w.arr[2]};
   double* out[3] = {vors, divs, height};
+  if (record) {
+    const double* src[21] = {vx, vy, vz};
+    for (int s2 = 0; s2 < 3; ++s2)
+      for (int q = 0; q < 3; ++q) {
+        src[3 + 3 * s2 + q] = w.v[s2][q];
+        src[12 + 3 * s2 + q] = w.tend[s2][q];
+      }
+    for (int i = 0; i < 21; ++i) launch_copy(record + (int64_t)i * t.npart, src[i], t.npart, c->stream);
+  }
   launch_swe_departure_to_target(c->dt, w.dep[0], w.dep[1], w.dep[2], arr, tracers, ntr, out, w.ntr, c->stream);
   if (ntr > 0) launch_copy(tracers, w.ntr, t.npart * ntr, c->stream);
   return launched(c);
 }
+
+// ---- shallow-water step adjoint ---------------------------------------------------
+
+static int ensure_adjoint(sbslpm_ctx* c);
+
+static int ensure_swe_adjoint(sbslpm_ctx* c) {
+  int rc = ensure_adjoint(c);
+  if (!rc) rc = ensure_swe(c);
+  if (!rc) rc = ensure_swe_step(c);
+  if (rc || c->swa_arena) return rc;
+  const Tree& t = c->tree;
+  const size_t N = (size_t)t.npart, Pn = (size_t)t.P * t.n;
+  Arena A;
+  const size_t o_vb = A.take<double>(12 * N), o_tb = A.take<double>(9 * N), o_rb = A.take<double>(3 * N),
+               o_lb = A.take<double>(N), o_qp = A.take<double>(3 * N), o_qx = A.take<double>(3 * Pn),
+               o_q = A.take<double>(3 * N), o_dbp = A.take<double>(3 * N), o_dbx = A.take<double>(3 * Pn),
+               o_hs = A.take<double>(N), o_hh = A.take<double>(N), o_st = A.take<double>(21 * N),
+               o_tr = A.take<char>(t.U * swe_adj_target_record_bytes()), o_sr = A.take<SrcRec64>(t.U),
+               total = A.take<char>(256);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  char* b = nullptr;
+  if (hipMalloc(&b, total) != hipSuccess)
+    return fail(c, SBSLPM_E_NOMEM, "shallow-water adjoint workspace: out of device memory");
+  c->swa_arena = b;
+  auto& a = c->swa;
+  for (int i = 0; i < 12; ++i) a.vb[i] = (double*)(b + o_vb) + i * N;
+  for (int i = 0; i < 9; ++i) a.tb[i] = (double*)(b + o_tb) + i * N;
+  for (int i = 0; i < 3; ++i) {
+    a.rb[i] = (double*)(b + o_rb) + i * N;
+    a.dbp[i] = (double*)(b + o_dbp) + i * N;
+    a.dbx[i] = (double*)(b + o_dbx) + i * Pn;
+  }
+  a.lapbar = (double*)(b + o_lb);
+  a.qzp = (double*)(b + o_qp);
+  a.qdp = a.qzp + N;
+  a.qep = a.qdp + N;
+  a.qzx = (double*)(b + o_qx);
+  a.qdx = a.qzx + Pn;
+  a.qex = a.qdx + Pn;
+  a.qz = (double*)(b + o_q);
+  a.qd = a.qz + N;
+  a.qe = a.qd + N;
+  a.hscat = (double*)(b + o_hs);
+  a.hh = (double*)(b + o_hh);
+  a.stages = (double*)(b + o_st);
+  a.trec = b + o_tr;
+  a.srec = (SrcRec64*)(b + o_sr);
+  return SBSLPM_OK;
+}
+
+// Exact transpose of the shallow-water velocity evaluation (sbslpm_swe_velocity,
+// variant 1, sources displaced by dpart/dprox or not) for the particle-velocity
+// cotangent vbar: swa.qz, swa.qd = dL/d(area_s vor_s), dL/d(area_s div_s) at
+// the particles; with displacements, swa.dbp/dbx = dL/d disp, assigned or (acc)
+// added.  The pair sums run in FP32 on mixed-precision contexts.
+static int swe_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* vor_s, const double* div_s,
+                           const double* const* dpart, const double* const* dprox, const double* const* vbar,
+                           bool acc) {
+  const Tree& t = c->tree;
+  auto& a = c->swa;
+  const bool pos = dpart != nullptr, fp32 = c->prm.precision != SBSLPM_FP64;
+  for (int f = 0; f < 3; ++f)
+    launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
+  const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
+  launch_swe_adj_target_records(c->dt, fp32, vbar, psix, a.trec, c->stream);
+  launch_upward_fused(c->dt, area_s, vor_s, c->qpart, c->qproxy, c->stream);
+  launch_upward_fused(c->dt, area_s, div_s, c->qdpart, c->qdproxy, c->stream);
+  launch_source_records_swe(c->dt, 1, c->qpart, c->qproxy, c->qdpart, c->qdproxy, dpart, dprox, c->recswe, c->stream);
+  double* qx2[2] = {a.qzx, a.qdx};
+  launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 2, qx2, c->stream);
+  if (pos && !acc) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, a.dbx, c->stream);
+  launch_swe_adjoint(c->dt, fp32, c->adj_tiles, c->adj_ntiles, c->adj_targets, a.trec, c->recswe, c->prm.kernel_eps,
+                     dpart, dprox, a.qzp, a.qzx, a.qdp, a.qdx, a.dbp, a.dbx, acc, c->stream, c->shape);
+  const double* qzp[1] = {a.qzp};
+  const double* qdp[1] = {a.qdp};
+  launch_downward_pass(c->dt, 1, &a.qzx, qzp, &a.qz, c->stream, true);
+  launch_downward_pass(c->dt, 1, &a.qdx, qdp, &a.qd, c->stream, true);
+  return launched(c);
+}
+
+// Exact transpose of laplacian_fmm for the cotangent lbar of its output:
+// swa.qe = dL/d(area_s eff) at the particles; with displacements swa.dbp/dbx
+// = dL/d disp assigned.
+static int lap_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* eff, const double* const* dpart,
+                           const double* const* dprox, const double* lbar) {
+  const Tree& t = c->tree;
+  auto& a = c->swa;
+  auto& w = c->sws;
+  const bool pos = dpart != nullptr, fp32 = c->prm.precision != SBSLPM_FP64;
+  launch_upward_pass(c->dt, c->adj_ones, lbar, c->adj_psip[0], c->adj_psix[0], c->stream);
+  launch_lap_adj_target_records(c->dt, fp32, lbar, c->adj_psix[0], a.trec, c->stream);
+  launch_upward_fused(c->dt, area_s, eff, w.qp, w.qx, c->stream);
+  launch_source_records64(c->dt, w.qp, w.qx, dpart, dprox, true, a.srec, c->stream);
+  launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 1, &a.qex, c->stream);
+  if (pos) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, a.dbx, c->stream);
+  launch_laplacian_adjoint(c->dt, fp32, c->adj_tiles, c->adj_ntiles, c->adj_targets, a.trec, a.srec, dpart, dprox,
+                           a.qep, a.qex, a.dbp, a.dbx, false, c->stream, c->shape);
+  const double* qep[1] = {a.qep};
+  launch_downward_pass(c->dt, 1, &a.qex, qep, &a.qe, c->stream, true);
+  return launched(c);
+}
+
+// Reverse pass of one shallow-water step from its record (stage velocities
+// V0..V3 = rec[0..11], tendencies T0..T2 = rec[12..20], each [Pl][m2]):
+// sbar (vor, div, h before the step) assigned from nbar (after the step).
+static int swe_step_reverse(sbslpm_ctx* c, const double* x, const double* y, const double* z, const double* area,
+                            const double* vors, const double* divs, const double* height, const double* rec,
+                            const double* const* nbar, double* const* sbar) {
+  const Tree& t = c->tree;
+  const int64_t N = t.npart;
+  auto& w = c->sws;
+  auto& a = c->swa;
+  const SwePhys ph = swe_phys(c);
+  const int rule = c->prm.swe_rule, topo = c->prm.topography;
+  const bool reg = (c->prm.swe_form & SBSLPM_SWE_FORM_REG_LAPLACIAN) != 0;
+  const double dt = c->prm.delta_t, om = c->prm.omega;
+  const double taus[3] = {0.5 * dt, 0.5 * dt, dt};
+  const double* V[12];
+  const double* T[9];
+  for (int i = 0; i < 12; ++i) V[i] = rec + i * N;
+  for (int i = 0; i < 9; ++i) T[i] = rec + (12 + i) * N;
+  const double* state[3] = {vors, divs, height};
+  const double* csbar[3] = {sbar[0], sbar[1], sbar[2]};
+  for (int q = 0; q < 3; ++q) launch_fill_zero(sbar[q], N, c->stream);
+  launch_fill_zero(a.vb[0], 12 * N, c->stream);
+  launch_fill_zero(a.tb[0], 9 * N, c->stream);
+  launch_fill_zero(a.rb[0], 3 * N, c->stream);
+  // arrival values R = S0 + dt T0, departure search and remap
+  launch_apply_tendencies(N, state, T, nullptr, dt, w.arr, nullptr, c->stream);
+  const double* arr[3] = {w.arr[0], w.arr[1], w.arr[2]};
+  launch_departure_adjoint3(c->dt, x, y, z, V, arr, nbar, dt, c->prm.rk4_combine, a.vb, a.rb, c->stream);
+  const double* crb[3] = {a.rb[0], a.rb[1], a.rb[2]};
+  launch_apply_tendencies(N, csbar, crb, nullptr, 1.0, sbar, nullptr, c->stream);
+  const double* ctb0[3] = {a.tb[0], a.tb[1], a.tb[2]};
+  launch_apply_tendencies(N, ctb0, crb, nullptr, dt, a.tb, nullptr, c->stream);
+  const double* cdp[3] = {w.dpart[0], w.dpart[1], w.dpart[2]};
+  const double* cdx[3] = {w.dprox[0], w.dprox[1], w.dprox[2]};
+  int rc = SBSLPM_OK;
+  for (int s = 3; s >= 1 && rc == SBSLPM_OK; --s) {
+    const double* const* vprev = V + 3 * (s - 1);
+    const double* const* tprev = T + 3 * (s - 1);
+    const double tau = taus[s - 1];
+    const double taua = (s == 3 && rule == SBSLPM_SWE_RULE_REFERENCE) ? 0.5 * dt : tau;
+    // the stage's displacement, displaced state and areas, as the forward
+    rc = sbslpm_swe_displacement(c, vprev[0], vprev[1], vprev[2], tau, w.dpart[0], w.dpart[1], w.dpart[2], w.dprox[0],
+                                 w.dprox[1], w.dprox[2]);
+    if (rc) break;
+    launch_apply_tendencies(N, state, tprev, area, taua, w.st, w.da, c->stream);
+    if (s < 3) {
+      // tendencies of this stage
+      launch_swe_tendency_adjoint(c->dt, x, y, z, V + 3 * s, vors, divs, height, a.tb + 3 * s, om, rule, ph.gravity,
+                                  a.vb + 3 * s, a.lapbar, sbar, c->stream);
+      // height correction: lap -= 80 BLI[h + tau T_h](P(x + tau vprev)) (+ h_s)
+      launch_lincomb(N, height, tau, tprev[2], a.hh, c->stream);
+      launch_fill_zero(a.hscat, N, c->stream);
+      launch_height_correction_adjoint(c->dt, x, y, z, vprev, a.hh, tau, ph, a.lapbar, a.hscat, a.vb + 3 * (s - 1),
+                                       c->stream);
+      launch_lincomb(N, sbar[2], 1.0, a.hscat, sbar[2], c->stream);
+      launch_lincomb(N, a.tb[3 * (s - 1) + 2], tau, a.hscat, a.tb[3 * (s - 1) + 2], c->stream);
+      // regularised-Laplacian sums of area_s * eff at displaced sources
+      const double* eff = w.st[2];
+      if (topo) {
+        launch_topography(c->dt, x, y, z, cdp, cdx, w.st[2], rule, ph, w.eff, c->stream);
+        eff = w.eff;
+      }
+      rc = lap_fmm_adjoint(c, w.da, eff, cdp, cdx, a.lapbar);
+      if (rc) break;
+    }
+    const double* vb[3] = {a.vb[3 * s], a.vb[3 * s + 1], a.vb[3 * s + 2]};
+    rc = swe_fmm_adjoint(c, w.da, w.st[0], w.st[1], cdp, cdx, vb, s < 3);
+    if (rc) break;
+    launch_swe_stage_adjoint(c->dt, x, y, z, state, tprev, area, taua, a.qz, a.qd, s < 3 ? a.qe : nullptr,
+                             (s < 3 && topo) ? cdp : nullptr, (s < 3 && topo) ? cdx : nullptr, rule, ph, sbar,
+                             a.tb + 3 * (s - 1), a.dbp, a.dbx, c->stream);
+    const double* dbx[3] = {a.dbx[0], a.dbx[1], a.dbx[2]};
+    launch_disp_interp_adjoint(c->dt, dbx, a.dbp, c->stream);
+    const double* cvb[3] = {a.vb[3 * (s - 1)], a.vb[3 * (s - 1) + 1], a.vb[3 * (s - 1) + 2]};
+    const double* cdbp[3] = {a.dbp[0], a.dbp[1], a.dbp[2]};
+    launch_apply_tendencies(N, cvb, cdbp, nullptr, tau, a.vb + 3 * (s - 1), nullptr, c->stream);
+  }
+  if (rc) return rc;
+  // stage 1: tendencies, surface Laplacian of h (+ h_s), undisplaced velocity
+  launch_swe_tendency_adjoint(c->dt, x, y, z, V, vors, divs, height, a.tb, om, rule, ph.gravity, a.vb, a.lapbar, sbar,
+                              c->stream);
+  if (reg) {
+    // lap -= 8/0.1 (h + h_s(x)) at the nodes
+    launch_lincomb(N, sbar[2], -8.0 / 0.1, a.lapbar, sbar[2], c->stream);
+    const double* eff = height;
+    if (topo) {
+      launch_topography(c->dt, x, y, z, nullptr, nullptr, height, rule, ph, w.eff, c->stream);
+      eff = w.eff;
+    }
+    rc = lap_fmm_adjoint(c, area, eff, nullptr, nullptr, a.lapbar);
+    if (rc) return rc;
+  } else {
+    launch_panel_laplacian_adjoint(c->dt, a.lapbar, sbar[2], c->stream);
+  }
+  const double* vb0[3] = {a.vb[0], a.vb[1], a.vb[2]};
+  rc = swe_fmm_adjoint(c, area, vors, divs, nullptr, nullptr, vb0, false);
+  if (rc) return rc;
+  launch_swe_stage_adjoint(c->dt, x, y, z, state, nullptr, area, 0.0, a.qz, a.qd, reg ? a.qe : nullptr, nullptr,
+                           nullptr, rule, ph, sbar, nullptr, nullptr, nullptr, c->stream);
+  return launched(c);
+}
+
+// Checks shared by the shallow-water record and adjoint entry points.
+static int swe_adjoint_args(sbslpm_ctx* c, const char* what) {
+  if (c->prm.tracer_count != 0)
+    return fail(c, SBSLPM_E_ARG, std::string(what) + ": needs tracer_count == 0 (tracers are not differentiated)");
+  if (!swe_rule_ok(c))
+    return fail(c, SBSLPM_E_ARG, std::string(what) + ": the reference topography rule needs leaf_degree == interp_degree");
+  return ensure_swe_adjoint(c);
+}
+
+extern "C" {
+
+int sbslpm_swe_back_rk4_step_record(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                    const double* area, const double* vors, const double* divs, const double* height,
+                                    double* new_vors, double* new_divs, double* new_height, double* stages) {
+  if (!c || !x || !y || !z || !area || !vors || !divs || !height || !new_vors || !new_divs || !new_height || !stages)
+    return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step_record: null pointer");
+  const double* ins[4] = {vors, divs, height, area};
+  const double* outs[4] = {new_vors, new_divs, new_height, stages};
+  for (const double* o : outs)
+    for (const double* i : ins)
+      if (o == i) return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step_record: outputs must not alias the inputs");
+  int rc = swe_adjoint_args(c, "swe_back_rk4_step_record");
+  if (rc) return rc;
+  const int64_t N = c->tree.npart;
+  launch_copy(new_vors, vors, N, c->stream);
+  launch_copy(new_divs, divs, N, c->stream);
+  launch_copy(new_height, height, N, c->stream);
+  return swe_step_body(c, x, y, z, area, new_vors, new_divs, new_height, stages, stages + N, stages + 2 * N, nullptr,
+                       c->sws.lap, stages);
+}
+
+int sbslpm_swe_back_rk4_step_adjoint_stages(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                            const double* area, const double* vors, const double* divs,
+                                            const double* height, const double* stages, const double* new_vors_bar,
+                                            const double* new_divs_bar, const double* new_height_bar,
+                                            double* vors_bar, double* divs_bar, double* height_bar) {
+  if (!c || !x || !y || !z || !area || !vors || !divs || !height || !stages || !new_vors_bar || !new_divs_bar ||
+      !new_height_bar || !vors_bar || !divs_bar || !height_bar)
+    return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step_adjoint_stages: null pointer");
+  const double* ins[8] = {vors, divs, height, area, stages, new_vors_bar, new_divs_bar, new_height_bar};
+  const double* outs[3] = {vors_bar, divs_bar, height_bar};
+  for (const double* o : outs)
+    for (const double* i : ins)
+      if (o == i) return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step_adjoint_stages: cotangents must not alias the inputs");
+  int rc = swe_adjoint_args(c, "swe_back_rk4_step_adjoint_stages");
+  if (rc) return rc;
+  const double* nb[3] = {new_vors_bar, new_divs_bar, new_height_bar};
+  double* sb[3] = {vors_bar, divs_bar, height_bar};
+  return swe_step_reverse(c, x, y, z, area, vors, divs, height, stages, nb, sb);
+}
+
+int sbslpm_swe_back_rk4_step_adjoint(sbslpm_ctx* c, const double* x, const double* y, const double* z,
+                                     const double* area, const double* vors, const double* divs, const double* height,
+                                     const double* new_vors_bar, const double* new_divs_bar,
+                                     const double* new_height_bar, double* vors_bar, double* divs_bar,
+                                     double* height_bar) {
+  if (!c || !x || !y || !z || !area || !vors || !divs || !height || !new_vors_bar || !new_divs_bar ||
+      !new_height_bar || !vors_bar || !divs_bar || !height_bar)
+    return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step_adjoint: null pointer");
+  const double* ins[7] = {vors, divs, height, area, new_vors_bar, new_divs_bar, new_height_bar};
+  const double* outs[3] = {vors_bar, divs_bar, height_bar};
+  for (const double* o : outs)
+    for (const double* i : ins)
+      if (o == i) return fail(c, SBSLPM_E_ARG, "swe_back_rk4_step_adjoint: cotangents must not alias the inputs");
+  int rc = swe_adjoint_args(c, "swe_back_rk4_step_adjoint");
+  if (rc) return rc;
+  // forward recompute into the workspace's record; the stepped state itself is
+  // not needed (written to the arrival-value buffers, then dropped)
+  const int64_t N = c->tree.npart;
+  auto& w = c->sws;
+  double* tmp[3] = {c->swa.rb[0], c->swa.rb[1], c->swa.rb[2]};
+  for (int q = 0; q < 3; ++q) launch_copy(tmp[q], (q == 0 ? vors : q == 1 ? divs : height), N, c->stream);
+  rc = swe_step_body(c, x, y, z, area, tmp[0], tmp[1], tmp[2], c->swa.stages, c->swa.stages + N,
+                     c->swa.stages + 2 * N, nullptr, w.lap, c->swa.stages);
+  if (rc) return rc;
+  const double* nb[3] = {new_vors_bar, new_divs_bar, new_height_bar};
+  double* sb[3] = {vors_bar, divs_bar, height_bar};
+  return swe_step_reverse(c, x, y, z, area, vors, divs, height, c->swa.stages, nb, sb);
+}
+
+}  // extern "C"
 
 // ---- irregular particle tree (lat-lon path) ----------------------------------
 

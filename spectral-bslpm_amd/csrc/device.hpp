@@ -98,6 +98,54 @@ __device__ inline int locate_leaf(const DevTree& t, double x, double y, double z
   return locate_uniform(t.levels, x, y, z, xe);
 }
 
+// Shallow-water physics of a context: topography kind (0 none, 1 the reference's
+// cone, 2 Gaussian), the Gaussian mountain (height, width, unit centre), the
+// coefficient of the free-surface Laplacian in the divergence tendency, and
+// whether the height correction subtracts the free surface.
+struct SwePhys {
+  int topo;
+  int surface_correction;
+  double gravity;
+  double mtn_h, mtn_w, mc[3];
+};
+// Topography height at a unit vector (0 for topo 0) and, when grad is given, its
+// gradient in the point (unit-sphere components; the caller projects).
+__host__ __device__ inline double topo_height(const SwePhys& ph, double a, double b, double c, double* grad = nullptr) {
+  if (grad) grad[0] = grad[1] = grad[2] = 0.0;
+  if (ph.topo == 2) {
+    // r = atan2(|p x c|, p.c), h = H exp(-(r/w)^2)
+    const double cx = b * ph.mc[2] - c * ph.mc[1], cy = c * ph.mc[0] - a * ph.mc[2], cz = a * ph.mc[1] - b * ph.mc[0];
+    const double sn = sqrt(cx * cx + cy * cy + cz * cz), cs = a * ph.mc[0] + b * ph.mc[1] + c * ph.mc[2];
+    const double r = atan2(sn, cs), q = r / ph.mtn_w, e = ph.mtn_h * exp(-q * q);
+    if (grad && sn > 0.0) {
+      // dr/dp = (cs (p - c cs) / sn - sn c) / (sn^2 + cs^2)
+      const double k = -2.0 * e * r / (ph.mtn_w * ph.mtn_w) / (sn * sn + cs * cs);
+      const double p[3] = {a, b, c};
+      for (int i = 0; i < 3; ++i) grad[i] = k * (cs * (p[i] - ph.mc[i] * cs) / sn - sn * ph.mc[i]);
+    }
+    return e;
+  }
+  if (ph.topo == 1) {
+    // cone_mountain (topography.cpp:6-45): h0 exp(-16 r^2), r^2 = (lon - pi/2)^2 + (lat - pi/6)^2
+    const double pi = 3.14159265358979323846;
+    const double rho2 = a * a + b * b, rho = sqrt(rho2);
+    const double lat = pi / 2.0 - atan2(rho, c);
+    double lon = atan2(b, a);
+    lon += 2 * pi;
+    lon = fmod(lon, 2 * pi);
+    const double dl = lon - 0.5 * pi, dtt = lat - pi / 6.0;
+    const double e = (2000.0 / 6371000.0) * exp(-16.0 * (dl * dl + dtt * dtt));
+    if (grad && rho2 > 0.0) {
+      // d lon = (-b, a, 0) / rho^2;  d lat = -(c d rho - rho d c) / (rho^2 + c^2), d rho = (a, b, 0) / rho
+      const double n2 = rho2 + c * c, k = -32.0 * e;
+      const double glon[3] = {-b / rho2, a / rho2, 0.0};
+      const double glat[3] = {-c * a / (rho * n2), -c * b / (rho * n2), rho / n2};
+      for (int i = 0; i < 3; ++i) grad[i] = k * (dl * glon[i] + dtt * glat[i]);
+    }
+    return e;
+  }
+  return 0.0;
+}
 // Source record of the FP64 pair kernels: displaced, re-projected source
 // position and its charge pre-multiplied by -1/(4 pi).
 struct SrcRec64 {
@@ -244,6 +292,52 @@ void launch_departure_adjoint(const DevTree& t, const double* x, const double* y
                               hipStream_t s);
 void launch_disp_interp_adjoint(const DevTree& t, const double* const* dbar_proxy, double* const* dbar_part,
                                 hipStream_t s);
+// Reverse of the departure search and the three-field remap of the shallow-water step
+// (vel_bar as launch_departure_adjoint; field_bar[3] accumulated).
+void launch_departure_adjoint3(const DevTree& t, const double* x, const double* y, const double* z,
+                               const double* const* vels /*12*/, const double* const* fields /*3*/,
+                               const double* const* new_bar /*3*/, double dt, int combine,
+                               double* const* vel_bar /*12*/, double* const* field_bar /*3*/, hipStream_t s);
+
+// ---- shallow-water step adjoint (swe_adjoint.hip, swe_step.hip)
+size_t swe_adj_target_record_bytes();
+void launch_swe_adj_target_records(const DevTree& t, bool fp32, const double* const* psi_part,
+                                   const double* const* psi_prox, void* rec, hipStream_t s);
+// Transposed shallow-water pair sums (variant 1): charge cotangents of the vorticity and
+// divergence charges assigned at every source row; with dpart the displacement cotangents
+// assigned (or added with accumulate) as well.
+void launch_swe_adjoint(const DevTree& t, bool fp32, const TargetTile* tiles, int ntiles, const SourceSet* targets,
+                        const void* trec, const SrcRecSwe* srec, double eps, const double* const* dpart,
+                        const double* const* dprox, double* qz_part, double* qz_prox, double* qd_part,
+                        double* qd_prox, double* const* dbar_part, double* const* dbar_prox, bool accumulate,
+                        hipStream_t s, PairShape shape = {});
+void launch_lap_adj_target_records(const DevTree& t, bool fp32, const double* psi_part, const double* psi_prox,
+                                   void* rec, hipStream_t s);
+// Transposed regularised-Laplacian pair sums, likewise.
+void launch_laplacian_adjoint(const DevTree& t, bool fp32, const TargetTile* tiles, int ntiles,
+                              const SourceSet* targets, const void* trec, const SrcRec64* srec,
+                              const double* const* dpart, const double* const* dprox, double* q_part, double* q_prox,
+                              double* const* dbar_part, double* const* dbar_prox, bool accumulate, hipStream_t s,
+                              PairShape shape = {});
+void launch_swe_tendency_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                                 const double* const* vel, const double* vor, const double* div, const double* h,
+                                 const double* const* tbar, double omega, int rule, double gravity,
+                                 double* const* vbar, double* lapbar, double* const* sbar, hipStream_t s);
+void launch_panel_laplacian_adjoint(const DevTree& t, const double* lapbar, double* fbar, hipStream_t s);
+// lap -= 8/0.1 (h + h_s(x)) at the nodes (h_s with the surface correction only).
+void launch_node_correction(int64_t n, const double* x, const double* y, const double* z, const double* h,
+                            const SwePhys& ph, double* lap, hipStream_t s);
+// out = x + alpha y (out may alias x or y).
+void launch_lincomb(int64_t n, const double* x, double alpha, const double* y, double* out, hipStream_t s);
+void launch_height_correction_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                                      const double* const* vel, const double* hh, double tau, const SwePhys& ph,
+                                      const double* lapbar, double* hscat, double* const* vbar, hipStream_t s);
+void launch_swe_stage_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                              const double* const* s0, const double* const* tprev, const double* area, double taua,
+                              const double* qz, const double* qd, const double* qe, const double* const* dpart,
+                              const double* const* dprox, int rule, const SwePhys& ph, double* const* sbar,
+                              double* const* tbar, double* const* dbar_part, double* const* dbar_prox,
+                              hipStream_t s);
 void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
                           const double* const* vel, double tau, double omega, const double* area, const double* qbar,
                           const double* const* dbar, double* vor_bar, double* const* vbar, hipStream_t s);
@@ -313,15 +407,16 @@ void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, 
 void launch_panel_laplacian(const DevTree& t, const double* f, double* lap, hipStream_t s);
 void launch_swe_tendency(const DevTree& t, const double* x, const double* y, const double* z,
                          const double* const* vel, const double* vor, const double* div, const double* h,
-                         const double* lap, double omega, int rule, double* const* out, hipStream_t s);
+                         const double* lap, double omega, int rule, double gravity, double* const* out,
+                         hipStream_t s);
 void launch_topography(const DevTree& t, const double* x, const double* y, const double* z,
                        const double* const* disp_part, const double* const* disp_proxy, const double* h, int rule,
-                       double* eff, hipStream_t s);
+                       const SwePhys& ph, double* eff, hipStream_t s);
 void launch_apply_tendencies(int64_t n, const double* const* f, const double* const* ft, const double* area,
                              double tau, double* const* out, double* new_area, hipStream_t s);
 void launch_height_correction(const DevTree& t, const double* x, const double* y, const double* z,
-                              const double* const* vel, const double* h, const double* ht, double tau, double* lap,
-                              hipStream_t s);
+                              const double* const* vel, const double* h, const double* ht, double tau,
+                              const SwePhys& ph, double* lap, hipStream_t s);
 // Mean over the copies of every unique node; sum1d [n1d] is scratch.
 void launch_unify_boundary_vals(int64_t n, const int* map, const int* counts, int64_t n1d, double* sum1d,
                                 double* vals, hipStream_t s);

@@ -78,62 +78,25 @@ __device__ inline void interp3_at(const DevTree& t, double x, double y, double z
   for (int q = 0; q < 3; ++q) k[q] = tensor_gather<M>(f[q] + base, wx, we);
 }
 
-// Reverse of k = BLI[f](p) for NF leaf fields at point p, first part:
-// pbar += sum_q kbar_q grad_p f_q(p); returns the leaf (-1 when location
-// fails) and the 1-D weights for the transposed interpolation fbar += B kbar,
-// which the caller applies with wave_scatter_leaf.
-template <int M, int NF>
-__device__ inline int interp_grad(const DevTree& t, double x, double y, double z, const double* const* f,
-                                  const double* kbar, double* pbar, double* wx, double* we, bool report) {
-  double xe[2];
-  const int leaf = locate_leaf(t, x, y, z, xe);
-  if (leaf < 0) {
-    if (report) flag(t, kErrLocate);
-    return -1;
-  }
-  const DevPanel& P = t.panels[leaf];
-  double dwx[M], dwe[M];
-  bool ok = bli_wd<M>(xe[0], P.min_xi, P.max_xi, t.cheb_m, wx, dwx);
-  ok = bli_wd<M>(xe[1], P.min_eta, P.max_eta, t.cheb_m, we, dwe) && ok;
-  if (!ok && report) flag(t, kErrBliBounds);
-  double gxi[3], geta[3];
-  xieta_gradient(x, y, z, P.face, gxi, geta);
-  const int64_t base = (int64_t)(leaf - t.offset) * (M * M);
-  double sxi = 0, seta = 0;  // sum_q kbar_q d f_q / d xi, d eta
-#pragma unroll
-  for (int q = 0; q < NF; ++q) {
-    const double* __restrict__ F = f[q] + base;
-    double dxi = 0, deta = 0;
-#pragma unroll
-    for (int a = 0; a < M; ++a) {
-      double s = 0, sd = 0;
-#pragma unroll
-      for (int c = 0; c < M; ++c) {
-        s = fma(F[a * M + c], we[c], s);
-        sd = fma(F[a * M + c], dwe[c], sd);
-      }
-      dxi = fma(dwx[a], s, dxi);
-      deta = fma(wx[a], sd, deta);
-    }
-    sxi = fma(kbar[q], dxi, sxi);
-    seta = fma(kbar[q], deta, seta);
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) pbar[k] += sxi * gxi[k] + seta * geta[k];
-  return leaf;
-}
+struct CF3 {
+  const double* p[3];
+};
+struct F3 {
+  double* p[3];
+};
 
-// Reverse of k_departure followed by k_dep_to_target (Omega terms included)
-// for one arrival particle.  Inputs: the four stage velocities v (v0..v3,
-// as in the departure search), the field before the step and the cotangent
-// of the field after it.  vor_bar and vbar[0..8] (v0..v2) are accumulated,
-// vbar[9..11] (v3, read only at the arrival node) assigned.
-template <int M>
+// Reverse of k_departure followed by the remap of NF fields for one arrival
+// particle: k_dep_to_target (NF = 1, the Omega terms included) or the
+// three-field remap of the shallow-water step (NF = 3, omega = 0).  Inputs:
+// the four stage velocities v (v0..v3, as in the departure search), the
+// fields before the remap and the cotangents of the fields after it.
+// fbar and vbar[0..8] (v0..v2) are accumulated, vbar[9..11] (v3, read only at
+// the arrival node) assigned.
+template <int M, int NF>
 __global__ __launch_bounds__(128) void k_departure_adj(DevTree t, const double* __restrict__ x,
                                                        const double* __restrict__ y, const double* __restrict__ z,
-                                                       CVel12 v, const double* __restrict__ vor,
-                                                       const double* __restrict__ nbar, double dt, int combine,
-                                                       double omega, Vel12 vbar, double* __restrict__ vor_bar) {
+                                                       CVel12 v, CF3 fld, CF3 nbar, double dt, int combine,
+                                                       double omega, Vel12 vbar, F3 fbar) {
   __shared__ double lds[2][wave_scatter_lds<M, 3>()];
   double* wl = lds[threadIdx.x >> 6];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,14 +142,14 @@ __global__ __launch_bounds__(128) void k_departure_adj(DevTree t, const double* 
     project_rcp(a, b, c);
   }
   double wx[M], we[M];
-  // remap: new = BLI[vor](dep) + 2 Omega dep_z - 2 Omega z
-  const double g = nbar[u];
-  double db[3] = {0, 0, 2 * omega * g};
+  // remap: new_q = BLI[f_q](dep) (+ 2 Omega dep_z - 2 Omega z for the vorticity, NF = 1)
+  double g[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) g[q] = nbar.p[q][u];
+  double db[3] = {0, 0, 2 * omega * g[0]};
   {
-    const double* f[1] = {vor};
-    double* fb[1] = {vor_bar};
-    int leaf = interp_grad<M, 1>(t, a, b, c, f, &g, db, wx, we, valid);
-    wave_scatter_leaf<M, 1>(wl, valid ? leaf : -1, wx, we, &g, fb, t.offset);
+    int leaf = interp_grad<M, NF>(t, a, b, c, fld.p, g, db, wx, we, valid);
+    wave_scatter_leaf<M, NF>(wl, valid ? leaf : -1, wx, we, g, fbar.p, t.offset);
   }
   // combine
   double kb1[3], kb2[3], kb3[3], kb4[3];
@@ -283,10 +246,14 @@ struct RemapAdj {
 
 template <int M>
 struct DepartureAdj {
-  static void run(const DevTree& t, const double* x, const double* y, const double* z, CVel12 v, const double* vor,
-                  const double* nb, double dt, int combine, double omega, Vel12 vb, double* wb, hipStream_t s) {
-    hipLaunchKernelGGL(k_departure_adj<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, v, vor, nb,
-                       dt, combine, omega, vb, wb);
+  static void run(const DevTree& t, const double* x, const double* y, const double* z, CVel12 v, int nf, CF3 f,
+                  CF3 nb, double dt, int combine, double omega, Vel12 vb, F3 fb, hipStream_t s) {
+    if (nf == 1)
+      hipLaunchKernelGGL((k_departure_adj<M, 1>), dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, v, f,
+                         nb, dt, combine, omega, vb, fb);
+    else
+      hipLaunchKernelGGL((k_departure_adj<M, 3>), dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, v, f,
+                         nb, dt, combine, omega, vb, fb);
   }
 };
 
@@ -319,7 +286,29 @@ void launch_departure_adjoint(const DevTree& t, const double* x, const double* y
     v.p[i] = vels[i];
     vb.p[i] = vel_bar[i];
   }
-  dispatch_m<DepartureAdj>(t.m, t, x, y, z, v, vor, new_vor_bar, dt, combine, omega, vb, vor_bar, s);
+  CF3 f{{vor, nullptr, nullptr}}, nb{{new_vor_bar, nullptr, nullptr}};
+  F3 fb{{vor_bar, nullptr, nullptr}};
+  dispatch_m<DepartureAdj>(t.m, t, x, y, z, v, 1, f, nb, dt, combine, omega, vb, fb, s);
+}
+
+void launch_departure_adjoint3(const DevTree& t, const double* x, const double* y, const double* z,
+                               const double* const* vels, const double* const* fields, const double* const* new_bar,
+                               double dt, int combine, double* const* vel_bar, double* const* field_bar,
+                               hipStream_t s) {
+  CVel12 v{};
+  Vel12 vb{};
+  for (int i = 0; i < 12; ++i) {
+    v.p[i] = vels[i];
+    vb.p[i] = vel_bar[i];
+  }
+  CF3 f{}, nb{};
+  F3 fb{};
+  for (int q = 0; q < 3; ++q) {
+    f.p[q] = fields[q];
+    nb.p[q] = new_bar[q];
+    fb.p[q] = field_bar[q];
+  }
+  dispatch_m<DepartureAdj>(t.m, t, x, y, z, v, 3, f, nb, dt, combine, 0.0, vb, fb, s);
 }
 
 void launch_disp_interp_adjoint(const DevTree& t, const double* const* dbar_proxy, double* const* dbar_part,

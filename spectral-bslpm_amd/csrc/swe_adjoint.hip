@@ -1,0 +1,516 @@
+// swe_adjoint.hip -- exact transposes of the shallow-water step's pair sums
+// with respect to their source charges and displaced source positions, over
+// the interaction list seen from its sources (one wave per tile of at most 64
+// source rows, each lane owning one row, target records read wave-uniformly):
+//   * the shallow-water velocity sums, variant 1 (swe_vel_panel_interaction,
+//     swe_vel_impl.hpp:12-76): per pair
+//       u_t += r (Qz (x cross y) + Qd (y - x (x.y))),  r = 1 / (1 - x.y + eps),
+//     Qz, Qd the charges times -1/(4 pi).  For a velocity cotangent psi at the
+//     target, psi.(x cross y) = y.c1 and psi.(y - x (x.y)) = y.c2 with
+//     c1 = psi cross x, c2 = psi - x (x.psi), so
+//       Qz_bar += r (y.c1),  Qd_bar += r (y.c2),
+//       y_bar  += r (r (Qz y.c1 + Qd y.c2) x + Qz c1 + Qd c2)     (dr/dy = r^2 x);
+//   * the regularised-Laplacian sums (lap_panel_interaction,
+//     laplacian_impl.hpp:46-66): phi_t += K(x.y) q, so
+//       q_bar += psi K(t),  y_bar += q psi K'(t) x,
+//     K' from the derivative of each factor of K.
+// Source-side finish as in the Biot-Savart adjoint: the position cotangent
+// goes back through the projection y = P(node + d).  FP64 (k_*_adj64) and,
+// for mixed-precision contexts, FP32 pair arithmetic in panel-centred
+// coordinates with FP32 partial sums per target set flushed into FP64
+// (k_*_adj32).
+#include <algorithm>
+#include <cmath>
+
+#include "device.hpp"
+
+namespace sbslpm {
+namespace {
+
+struct CPtr3 {
+  const double* p[3];
+};
+struct Ptr3 {
+  double* p[3];
+};
+
+// ---- shallow-water velocity sums ----------------------------------------------
+
+// Target record: node, c1 = psi cross x, c2 = psi - x (x.psi) (72 bytes).
+struct SweTgt {
+  double x, y, z, a0, a1, a2, b0, b1, b2;
+};
+
+__global__ void k_swe_tgt(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, SweTgt* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double a, b, d;
+  if (u < t.npart) {
+    a = psi_part.p[0][u]; b = psi_part.p[1][u]; d = psi_part.p[2][u];
+  } else {
+    const int64_t v = u - t.npart;
+    if (v / t.n >= t.offset) return;  // leaf proxy rows are never targets
+    a = psi_prox.p[0][v]; b = psi_prox.p[1][v]; d = psi_prox.p[2][v];
+  }
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const double xp = x * a + y * b + z * d;
+  rec[u] = SweTgt{x, y, z, b * z - d * y, d * x - a * z, a * y - b * x, a - x * xp, b - y * xp, d - z * xp};
+}
+
+// Stores of one source row: charge cotangents (times -1/(4 pi): the records
+// carry the scaled charges), and the displacement cotangent through the
+// projection.  acc: add the displacement cotangent to what is there.
+template <bool POS>
+__device__ inline void swe_store(const DevTree& t, int64_t u, const double* S3, double az, double ad, const double* pb,
+                                 CPtr3 dpart, CPtr3 dprox, double* qz_part, double* qz_prox, double* qd_part,
+                                 double* qd_prox, Ptr3 dbar_part, Ptr3 dbar_prox, int acc) {
+  const bool part = u < t.npart;
+  const int64_t v = part ? u : u - t.npart;
+  (part ? qz_part : qz_prox)[v] = kNegInv4Pi * az;
+  (part ? qd_part : qd_prox)[v] = kNegInv4Pi * ad;
+  if (POS) {
+    const double w0 = t.px[u] + (part ? dpart.p[0] : dprox.p[0])[v];
+    const double w1 = t.py[u] + (part ? dpart.p[1] : dprox.p[1])[v];
+    const double w2 = t.pz[u] + (part ? dpart.p[2] : dprox.p[2])[v];
+    const double ir = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const double yy = S3[0] * pb[0] + S3[1] * pb[1] + S3[2] * pb[2];
+    double* const* o = part ? dbar_part.p : dbar_prox.p;
+    for (int f = 0; f < 3; ++f) {
+      const double d = (pb[f] - S3[f] * yy) * ir;
+      o[f][v] = acc ? o[f][v] + d : d;
+    }
+  }
+}
+
+template <bool POS>
+__global__ __launch_bounds__(256) void k_swe_adj64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ targets, const SweTgt* __restrict__ trec,
+                                                   const SrcRecSwe* __restrict__ srec, double eps, CPtr3 dpart,
+                                                   CPtr3 dprox, double* __restrict__ qz_part,
+                                                   double* __restrict__ qz_prox, double* __restrict__ qd_part,
+                                                   double* __restrict__ qd_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
+                                                   int acc, int xcd) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = shaped_tile(wave, xcd);
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const SrcRecSwe S = srec[u];
+  double az = 0, ad = 0, p0 = 0, p1 = 0, p2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet G = targets[e];
+    const SweTgt* __restrict__ r = trec + G.first;
+#pragma unroll 2
+    for (int k = 0; k < G.count; ++k) {
+      const SweTgt R = r[k];
+      const double g = rcp64_pair((1.0 - (R.x * S.x + R.y * S.y + R.z * S.z)) + eps);
+      const double a1 = S.x * R.a0 + S.y * R.a1 + S.z * R.a2;
+      const double a2 = S.x * R.b0 + S.y * R.b1 + S.z * R.b2;
+      az = fma(g, a1, az);
+      ad = fma(g, a2, ad);
+      if (POS) {
+        const double w = g * fma(S.qz, a1, S.qd * a2);
+        p0 = fma(g, fma(w, R.x, fma(S.qz, R.a0, S.qd * R.b0)), p0);
+        p1 = fma(g, fma(w, R.y, fma(S.qz, R.a1, S.qd * R.b1)), p1);
+        p2 = fma(g, fma(w, R.z, fma(S.qz, R.a2, S.qd * R.b2)), p2);
+      }
+    }
+  }
+  if (lane >= T.count) return;
+  const double S3[3] = {S.x, S.y, S.z}, pb[3] = {p0, p1, p2};
+  swe_store<POS>(t, u, S3, az, ad, pb, dpart, dprox, qz_part, qz_prox, qd_part, qd_prox, dbar_part, dbar_prox, acc);
+}
+
+// FP32 target record: node relative to its panel centre, absolute node, c1, c2.
+struct SweTgt32 {
+  float ax, ay, az, x, y, z, a0, a1, a2, b0, b1, b2;
+};
+
+__global__ void k_swe_tgt32(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, SweTgt32* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double a, b, d;
+  int panel;
+  if (u < t.npart) {
+    a = psi_part.p[0][u]; b = psi_part.p[1][u]; d = psi_part.p[2][u];
+    panel = t.offset + (int)(u / t.m2);
+  } else {
+    const int64_t v = u - t.npart;
+    panel = (int)(v / t.n);
+    if (panel >= t.offset) return;
+    a = psi_prox.p[0][v]; b = psi_prox.p[1][v]; d = psi_prox.p[2][v];
+  }
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const DevPanel& P = t.panels[panel];
+  const double xp = x * a + y * b + z * d;
+  rec[u] = SweTgt32{(float)(x - P.cx), (float)(y - P.cy), (float)(z - P.cz), (float)x, (float)y, (float)z,
+                    (float)(b * z - d * y), (float)(d * x - a * z), (float)(a * y - b * x),
+                    (float)(a - x * xp), (float)(b - y * xp), (float)(d - z * xp)};
+}
+
+// FP32 pairs: d = y - x from panel-centred coordinates, h = |d|^2/2 = 1 - x.y,
+// and since c1, c2 are orthogonal to x, y.c1 = d.c1 and y.c2 = d.c2.
+template <bool POS>
+__global__ __launch_bounds__(256) void k_swe_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ targets,
+                                                   const SweTgt32* __restrict__ trec,
+                                                   const SrcRecSwe* __restrict__ srec, float eps, CPtr3 dpart,
+                                                   CPtr3 dprox, double* __restrict__ qz_part,
+                                                   double* __restrict__ qz_prox, double* __restrict__ qd_part,
+                                                   double* __restrict__ qd_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
+                                                   int acc, int xcd) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = shaped_tile(wave, xcd);
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const SrcRecSwe S = srec[u];
+  const DevPanel& PS = t.panels[T.panel];
+  const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
+  const float qz = (float)S.qz, qd = (float)S.qd;
+  double az = 0, ad = 0, p0 = 0, p1 = 0, p2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet G = targets[e];
+    const SweTgt32* __restrict__ r = trec + G.first;
+    // source relative to the target panel's centre
+    const float sx = bx - G.ox, sy = by - G.oy, sz = bz - G.oz;
+    float fz = 0.f, fd = 0.f, f0 = 0.f, f1 = 0.f, f2 = 0.f;
+#pragma unroll 2
+    for (int k = 0; k < G.count; ++k) {
+      const SweTgt32 R = r[k];
+      const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;  // y_s - x_t
+      const float g = __builtin_amdgcn_rcpf(fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps));
+      const float a1 = fmaf(d0, R.a0, fmaf(d1, R.a1, d2 * R.a2));
+      const float a2 = fmaf(d0, R.b0, fmaf(d1, R.b1, d2 * R.b2));
+      fz = fmaf(g, a1, fz);
+      fd = fmaf(g, a2, fd);
+      if (POS) {
+        const float w = g * fmaf(qz, a1, qd * a2);
+        f0 = fmaf(g, fmaf(w, R.x, fmaf(qz, R.a0, qd * R.b0)), f0);
+        f1 = fmaf(g, fmaf(w, R.y, fmaf(qz, R.a1, qd * R.b1)), f1);
+        f2 = fmaf(g, fmaf(w, R.z, fmaf(qz, R.a2, qd * R.b2)), f2);
+      }
+    }
+    az += fz;
+    ad += fd;
+    if (POS) {
+      p0 += f0; p1 += f1; p2 += f2;
+    }
+  }
+  if (lane >= T.count) return;
+  const double S3[3] = {S.x, S.y, S.z}, pb[3] = {p0, p1, p2};
+  swe_store<POS>(t, u, S3, az, ad, pb, dpart, dprox, qz_part, qz_prox, qd_part, qd_prox, dbar_part, dbar_prox, acc);
+}
+
+// ---- regularised-Laplacian sums ------------------------------------------------
+
+constexpr double kLapEps = 0.1;
+constexpr double kPiL = 3.14159265358979323846;
+
+struct LapTgt {
+  double x, y, z, psi;
+};
+
+__global__ void k_lap_tgt(DevTree t, const double* __restrict__ psi_part, const double* __restrict__ psi_prox,
+                          LapTgt* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double p;
+  if (u < t.npart) {
+    p = psi_part[u];
+  } else {
+    const int64_t v = u - t.npart;
+    if (v / t.n >= t.offset) return;
+    p = psi_prox[v];
+  }
+  rec[u] = LapTgt{t.px[u], t.py[u], t.pz[u], p};
+}
+
+// K(t) eps^{5/2} and its derivative (t > 1 - 30 eps, eps = 0.1):
+//   h1 = c1 2t(-1 + 3 eps + t^2),                  h1' = c1 (6t^2 - 2 + 6 eps)
+//   h2 = c2 A E Q,  A = 1 + erf(t/eps), E = e^{(t^2-1)/eps},
+//        Q = 2 eps^2 + 2t^2(t^2-1) + eps(7t^2-1),
+//   h2' = c2 (A' E Q + A (2t/eps) E Q + A E Q'),  A' = 2/(sqrt(pi) eps) e^{-t^2/eps^2},
+//        Q' = 8t^3 - 4t + 14 eps t.
+// For |t/eps| >= 6, erf is +-1 in FP64 and the A' term is below an ulp of the
+// others, as in the forward kernel.
+__device__ inline void lap_kernel_d(double tt, double c1, double c2, double& k, double& kd) {
+  const double eps = kLapEps, inv_eps = 1.0 / kLapEps;
+  const double t2 = tt * tt;
+  const double h1 = c1 * (2.0 * tt * (-1.0 + 3.0 * eps + t2));
+  const double h1d = c1 * (6.0 * t2 - 2.0 + 6.0 * eps);
+  const double s = tt * inv_eps;
+  const double A = s >= 6.0 ? 2.0 : (s <= -6.0 ? 0.0 : 1.0 + erf(s));
+  const double E = exp((t2 - 1.0) * inv_eps);
+  const double Q = 2.0 * eps * eps + 2.0 * t2 * (t2 - 1.0) + eps * (7.0 * t2 - 1.0);
+  const double Qd = 8.0 * t2 * tt - 4.0 * tt + 14.0 * eps * tt;
+  double h2d = A * E * (2.0 * tt * inv_eps * Q + Qd);
+  if (fabs(s) < 6.0) h2d += 2.0 / sqrt(kPiL) * inv_eps * exp(-s * s) * E * Q;
+  k = h1 + c2 * A * E * Q;
+  kd = h1d + c2 * h2d;
+}
+
+// Records are the forward's (SrcRec64, q times -1/(4 pi)); c3 = -4 pi / eps^{5/2}
+// undoes the scaling.  q_bar = eps^{-5/2} sum psi K;  y_bar = q' c3 sum psi K' x.
+template <bool POS>
+__global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ targets, const LapTgt* __restrict__ trec,
+                                                   const SrcRec64* __restrict__ srec, double c1, double c2, double c3,
+                                                   CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
+                                                   double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
+                                                   int acc, int xcd) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = shaped_tile(wave, xcd);
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const SrcRec64 S = srec[u];
+  const double cut = 1.0 - 30 * kLapEps;
+  double aq = 0, p0 = 0, p1 = 0, p2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet G = targets[e];
+    const LapTgt* __restrict__ r = trec + G.first;
+    for (int k = 0; k < G.count; ++k) {
+      const LapTgt R = r[k];
+      const double tt = R.x * S.x + R.y * S.y + R.z * S.z;
+      if (tt > cut) {
+        double kv, kd;
+        lap_kernel_d(tt, c1, c2, kv, kd);
+        aq = fma(R.psi, kv, aq);
+        if (POS) {
+          const double w = R.psi * kd;
+          p0 = fma(w, R.x, p0);
+          p1 = fma(w, R.y, p1);
+          p2 = fma(w, R.z, p2);
+        }
+      }
+    }
+  }
+  if (lane >= T.count) return;
+  const bool part = u < t.npart;
+  const int64_t v = part ? u : u - t.npart;
+  (part ? q_part : q_prox)[v] = c3 * kNegInv4Pi * aq;
+  if (POS) {
+    const double w0 = t.px[u] + (part ? dpart.p[0] : dprox.p[0])[v];
+    const double w1 = t.py[u] + (part ? dpart.p[1] : dprox.p[1])[v];
+    const double w2 = t.pz[u] + (part ? dpart.p[2] : dprox.p[2])[v];
+    const double ir = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const double sc = S.q * c3;
+    const double yb[3] = {sc * p0, sc * p1, sc * p2};
+    const double yy = S.x * yb[0] + S.y * yb[1] + S.z * yb[2];
+    double* const* o = part ? dbar_part.p : dbar_prox.p;
+    const double s3[3] = {S.x, S.y, S.z};
+    for (int f = 0; f < 3; ++f) {
+      const double d = (yb[f] - s3[f] * yy) * ir;
+      o[f][v] = acc ? o[f][v] + d : d;
+    }
+  }
+}
+
+// FP32 target record for the Laplacian: node relative to its panel centre,
+// absolute node, psi.
+struct LapTgt32 {
+  float ax, ay, az, x, y, z, psi, pad;
+};
+
+__global__ void k_lap_tgt32(DevTree t, const double* __restrict__ psi_part, const double* __restrict__ psi_prox,
+                            LapTgt32* __restrict__ rec) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.U) return;
+  double p;
+  int panel;
+  if (u < t.npart) {
+    p = psi_part[u];
+    panel = t.offset + (int)(u / t.m2);
+  } else {
+    const int64_t v = u - t.npart;
+    panel = (int)(v / t.n);
+    if (panel >= t.offset) return;
+    p = psi_prox[v];
+  }
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const DevPanel& P = t.panels[panel];
+  rec[u] = LapTgt32{(float)(x - P.cx), (float)(y - P.cy), (float)(z - P.cz), (float)x, (float)y, (float)z, (float)p,
+                    0.f};
+}
+
+// FP32 pairs as k_lap_mixed: h = |x - y|^2/2 = 1 - t, t^2 - 1 = -h(2 - h),
+// the exponentials on the hardware base-2 exponential, erf and the A' term
+// only where |t| < 6 eps.
+template <bool POS>
+__global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                   const SourceSet* __restrict__ targets,
+                                                   const LapTgt32* __restrict__ trec,
+                                                   const SrcRec64* __restrict__ srec, float c1, float c2, double c3,
+                                                   CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
+                                                   double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
+                                                   int acc, int xcd) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = shaped_tile(wave, xcd);
+  if (tile >= ntiles) return;
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const SrcRec64 S = srec[u];
+  const DevPanel& PS = t.panels[T.panel];
+  const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
+  const float eps = (float)kLapEps, inv_eps = 1.0f / eps, k_exp = 1.44269504088896341f / eps;
+  const float cut = 1.0f - 30.0f * eps, k_a = 1.12837916709551257f * inv_eps;  // 2/sqrt(pi)/eps
+  double aq = 0, p0 = 0, p1 = 0, p2 = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet G = targets[e];
+    const LapTgt32* __restrict__ r = trec + G.first;
+    const float sx = bx - G.ox, sy = by - G.oy, sz = bz - G.oz;
+    float fq = 0.f, f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    for (int k = 0; k < G.count; ++k) {
+      const LapTgt32 R = r[k];
+      const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;
+      const float h = 0.5f * fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+      const float tt = 1.0f - h, tm1 = -h * (2.0f - h), t2 = 1.0f + tm1;
+      if (tt > cut) {
+        const float sv = tt * inv_eps;
+        const bool mid = fabsf(sv) < 6.0f;
+        const float A = sv >= 6.0f ? 2.0f : (sv <= -6.0f ? 0.0f : 1.0f + erff(sv));
+        const float E = __builtin_amdgcn_exp2f(tm1 * k_exp);
+        const float Q = 2.0f * eps * eps + 2.0f * t2 * tm1 + eps * (7.0f * t2 - 1.0f);
+        const float kv = c1 * (2.0f * tt * (3.0f * eps + tm1)) + c2 * A * E * Q;
+        fq = fmaf(R.psi, kv, fq);
+        if (POS) {
+          const float Qd = tt * (8.0f * t2 - 4.0f + 14.0f * eps);
+          float h2d = A * E * (2.0f * tt * inv_eps * Q + Qd);
+          if (mid) h2d += k_a * __builtin_amdgcn_exp2f(-sv * sv * 1.44269504088896341f) * E * Q;
+          const float w = R.psi * (c1 * (6.0f * t2 - 2.0f + 6.0f * eps) + c2 * h2d);
+          f0 = fmaf(w, R.x, f0);
+          f1 = fmaf(w, R.y, f1);
+          f2 = fmaf(w, R.z, f2);
+        }
+      }
+    }
+    aq += fq;
+    if (POS) {
+      p0 += f0; p1 += f1; p2 += f2;
+    }
+  }
+  if (lane >= T.count) return;
+  const bool part = u < t.npart;
+  const int64_t v = part ? u : u - t.npart;
+  (part ? q_part : q_prox)[v] = c3 * kNegInv4Pi * aq;
+  if (POS) {
+    const double w0 = t.px[u] + (part ? dpart.p[0] : dprox.p[0])[v];
+    const double w1 = t.py[u] + (part ? dpart.p[1] : dprox.p[1])[v];
+    const double w2 = t.pz[u] + (part ? dpart.p[2] : dprox.p[2])[v];
+    const double ir = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const double sc = S.q * c3;
+    const double yb[3] = {sc * p0, sc * p1, sc * p2};
+    const double yy = S.x * yb[0] + S.y * yb[1] + S.z * yb[2];
+    double* const* o = part ? dbar_part.p : dbar_prox.p;
+    const double s3[3] = {S.x, S.y, S.z};
+    for (int f = 0; f < 3; ++f) {
+      const double d = (yb[f] - s3[f] * yy) * ir;
+      o[f][v] = acc ? o[f][v] + d : d;
+    }
+  }
+}
+
+CPtr3 cptr(const double* const* p) {
+  CPtr3 r{};
+  for (int i = 0; i < 3; ++i) r.p[i] = p ? p[i] : nullptr;
+  return r;
+}
+Ptr3 ptr(double* const* p) {
+  Ptr3 r{};
+  for (int i = 0; i < 3; ++i) r.p[i] = p ? p[i] : nullptr;
+  return r;
+}
+
+}  // namespace
+
+size_t swe_adj_target_record_bytes() {
+  size_t b = sizeof(SweTgt);
+  b = std::max(b, sizeof(SweTgt32));
+  b = std::max(b, sizeof(LapTgt));
+  return std::max(b, sizeof(LapTgt32));
+}
+
+void launch_swe_adj_target_records(const DevTree& t, bool fp32, const double* const* psi_part,
+                                   const double* const* psi_prox, void* rec, hipStream_t s) {
+  const dim3 grid((unsigned)((t.U + 255) / 256)), block(256);
+  if (fp32)
+    hipLaunchKernelGGL(k_swe_tgt32, grid, block, 0, s, t, cptr(psi_part), cptr(psi_prox), static_cast<SweTgt32*>(rec));
+  else
+    hipLaunchKernelGGL(k_swe_tgt, grid, block, 0, s, t, cptr(psi_part), cptr(psi_prox), static_cast<SweTgt*>(rec));
+}
+
+void launch_swe_adjoint(const DevTree& t, bool fp32, const TargetTile* tiles, int ntiles, const SourceSet* targets,
+                        const void* trec, const SrcRecSwe* srec, double eps, const double* const* dpart,
+                        const double* const* dprox, double* qz_part, double* qz_prox, double* qd_part,
+                        double* qd_prox, double* const* dbar_part, double* const* dbar_prox, bool accumulate,
+                        hipStream_t s, PairShape shape) {
+  if (ntiles == 0) return;
+  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
+  const int acc = accumulate ? 1 : 0;
+  if (fp32) {
+    const SweTgt32* r = static_cast<const SweTgt32*>(trec);
+    if (dpart)
+      hipLaunchKernelGGL(k_swe_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps,
+                         cptr(dpart), cptr(dprox), qz_part, qz_prox, qd_part, qd_prox, ptr(dbar_part), ptr(dbar_prox),
+                         acc, shape.xcd);
+    else
+      hipLaunchKernelGGL(k_swe_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps,
+                         CPtr3{}, CPtr3{}, qz_part, qz_prox, qd_part, qd_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+  } else {
+    const SweTgt* r = static_cast<const SweTgt*>(trec);
+    if (dpart)
+      hipLaunchKernelGGL(k_swe_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, cptr(dpart),
+                         cptr(dprox), qz_part, qz_prox, qd_part, qd_prox, ptr(dbar_part), ptr(dbar_prox), acc,
+                         shape.xcd);
+    else
+      hipLaunchKernelGGL(k_swe_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, CPtr3{},
+                         CPtr3{}, qz_part, qz_prox, qd_part, qd_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+  }
+}
+
+void launch_lap_adj_target_records(const DevTree& t, bool fp32, const double* psi_part, const double* psi_prox,
+                                   void* rec, hipStream_t s) {
+  const dim3 grid((unsigned)((t.U + 255) / 256)), block(256);
+  if (fp32)
+    hipLaunchKernelGGL(k_lap_tgt32, grid, block, 0, s, t, psi_part, psi_prox, static_cast<LapTgt32*>(rec));
+  else
+    hipLaunchKernelGGL(k_lap_tgt, grid, block, 0, s, t, psi_part, psi_prox, static_cast<LapTgt*>(rec));
+}
+
+void launch_laplacian_adjoint(const DevTree& t, bool fp32, const TargetTile* tiles, int ntiles,
+                              const SourceSet* targets, const void* trec, const SrcRec64* srec,
+                              const double* const* dpart, const double* const* dprox, double* q_part, double* q_prox,
+                              double* const* dbar_part, double* const* dbar_prox, bool accumulate, hipStream_t s,
+                              PairShape shape) {
+  if (ntiles == 0) return;
+  const double eps = kLapEps;
+  const double c1 = std::exp(-1.0 / eps) * std::pow(kPiL, -1.5), c2 = 1.0 / (std::sqrt(eps) * kPiL);
+  const double c3 = -4.0 * kPiL / std::pow(eps, 2.5);
+  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
+  const int acc = accumulate ? 1 : 0;
+  if (fp32) {
+    const LapTgt32* r = static_cast<const LapTgt32*>(trec);
+    if (dpart)
+      hipLaunchKernelGGL(k_lap_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)c1,
+                         (float)c2, c3, cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part), ptr(dbar_prox), acc,
+                         shape.xcd);
+    else
+      hipLaunchKernelGGL(k_lap_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)c1,
+                         (float)c2, c3, CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+  } else {
+    const LapTgt* r = static_cast<const LapTgt*>(trec);
+    if (dpart)
+      hipLaunchKernelGGL(k_lap_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, c1, c2, c3,
+                         cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part), ptr(dbar_prox), acc, shape.xcd);
+    else
+      hipLaunchKernelGGL(k_lap_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, c1, c2, c3,
+                         CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+  }
+}
+
+}  // namespace sbslpm

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(128) void k_swe_tendency(DevTree t, const double* _
                                                       CP3 vel, const double* __restrict__ vor,
                                                       const double* __restrict__ div, const double* __restrict__ h,
                                                       const double* __restrict__ lap, double omega, int rule,
-                                                      P3 out) {
+                                                      double gravity, P3 out) {
   __shared__ double sv[3][M * M], xn[M], en[M];
   const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
   const DevPanel& P = t.panels[leaf + t.offset];
@@ -162,20 +162,21 @@ __global__ __launch_bounds__(128) void k_swe_tendency(DevTree t, const double* _
   double d = 2.0 * omega * z * vk;
   d -= G[0][0] * G[0][0] + G[0][1] * G[1][0] + G[0][2] * G[2][0] + G[1][0] * G[0][1] + G[1][1] * G[1][1] +
        G[1][2] * G[2][1] + G[2][0] * G[0][2] + G[2][1] * G[1][2] + G[2][2] * G[2][2];
-  d -= lap[k];
+  d -= gravity * lap[k];
   d -= ux * ux + uy * uy + uz * uz;
   if (x * x + y * y > 1e-16) d -= 2.0 * omega * (-y * ux + x * uy);
   out.p[1][k] = d;
   out.p[2][k] = -h[k] * dk;
 }
 
-// cone_mountain (topography.cpp:6-45): eff = h + h0 exp(-16 r^2) at P(x + disp).
-// The displacement of leaf-local i, node j: rule 0 reads row i of the full
-// [P][n] displacement array (the reference's index; leaf rows live at
-// i + offset there), rule 1 the leaf's own row.
+// apply_topography (topography.cpp:47-56): eff = h + h_s at P(x + disp), h_s the
+// cone_mountain (:6-45) or the Gaussian mountain (topo_height).  The
+// displacement of leaf-local i, node j: rule 0 reads row i of the full [P][n]
+// displacement array (the reference's index; leaf rows live at i + offset
+// there), rule 1 the leaf's own row.
 __global__ void k_topography(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, CP3 dpart, CP3 dprox, const double* __restrict__ h,
-                             int rule, double* __restrict__ eff) {
+                             int rule, SwePhys ph, double* __restrict__ eff) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= t.npart) return;
   const int i = (int)(k / t.m2), j = (int)(k % t.m2);
@@ -190,12 +191,7 @@ __global__ void k_topography(DevTree t, const double* __restrict__ x, const doub
   }
   double a = x[k] + d[0], b = y[k] + d[1], c = z[k] + d[2];
   project_to_sphere(a, b, c);
-  const double lat = kPiD / 2.0 - atan2(sqrt(a * a + b * b), c);
-  double lon = atan2(b, a);
-  lon += 2 * kPiD;
-  lon = fmod(lon, 2 * kPiD);
-  const double dl = lon - 0.5 * kPiD, dt = lat - kPiD / 6.0;
-  eff[k] = h[k] + (2000.0 / 6371000.0) * exp(-16.0 * (dl * dl + dt * dt));
+  eff[k] = h[k] + topo_height(ph, a, b, c);
 }
 
 // out_q = f_q + tau ft_q (q = vor, div, h); new_area = area (1 + tau div) when area is given.
@@ -208,16 +204,21 @@ __global__ void k_apply_tend(int64_t n, CP3 f, CP3 ft, const double* __restrict_
 }
 
 // swe_height_correction: at P(x + tau vel) the BLI of h + tau h_t over the
-// containing leaf; lap -= 8 new_h / 0.1.
+// containing leaf; lap -= 8 new_h / 0.1.  With the surface correction the
+// topography at that point is added to new_h.  tau = 0 reads the nodes
+// themselves (vel may then be null).
 template <int M>
 __global__ __launch_bounds__(128) void k_height_corr(DevTree t, const double* __restrict__ x,
                                                      const double* __restrict__ y, const double* __restrict__ z,
                                                      CP3 vel, const double* __restrict__ h,
-                                                     const double* __restrict__ ht, double tau,
+                                                     const double* __restrict__ ht, double tau, SwePhys ph,
                                                      double* __restrict__ lap) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= t.npart) return;
-  double a = x[k] + tau * vel.p[0][k], b = y[k] + tau * vel.p[1][k], c = z[k] + tau * vel.p[2][k];
+  double a = x[k], b = y[k], c = z[k];
+  if (tau != 0.0) {
+    a += tau * vel.p[0][k]; b += tau * vel.p[1][k]; c += tau * vel.p[2][k];
+  }
   project_to_sphere(a, b, c);
   double xe[2];
   const int leaf = locate_leaf(t, a, b, c, xe);
@@ -236,11 +237,250 @@ __global__ __launch_bounds__(128) void k_height_corr(DevTree t, const double* __
   for (int i = 0; i < M; ++i) {
     double r = 0;
 #pragma unroll
-    for (int j = 0; j < M; ++j) r += (h[base + i * M + j] + tau * ht[base + i * M + j]) * we[j];
+    for (int j = 0; j < M; ++j) r += (h[base + i * M + j] + (ht ? tau * ht[base + i * M + j] : 0.0)) * we[j];
     v += wx[i] * r;
   }
+  if (ph.surface_correction) v += topo_height(ph, a, b, c);
   lap[k] -= 8.0 * v / 0.1;
 }
+
+// ---- reverse mode -------------------------------------------------------------
+
+// Transpose of tile_deriv: the contribution of the derivative cotangents g
+// (along dir, one per node) to node (a, b) of the differentiated tile.  The
+// derivative at node i is sum_{k != i} (f_k - f_i) w_k / (w_i (x_i - x_k)).
+template <int M>
+__device__ inline double tile_deriv_t(const double* g, int a, int b, int dir, const double* nodes) {
+  constexpr int D = M - 1;
+  const int kk = dir == 0 ? a : b;
+  const double xk = nodes[kk], wk = bary_w(kk, D);
+  double acc = 0, self = 0;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    if (i == kk) continue;
+    const double gi = dir == 0 ? g[i * M + b] : g[a * M + i];
+    const double wi = bary_w(i, D);
+    acc += gi * wk / (wi * (nodes[i] - xk));
+    self += wi / (xk - nodes[i]);
+  }
+  return acc - g[a * M + b] * self / wk;
+}
+
+// The velocity-gradient map of k_swe_tendency at one node is linear in the
+// label derivatives (f_xi, f_eta): G = f_xi col0 + f_eta col1.
+__device__ inline void grad_map(int rule, int face, double xi, double eta, double* col0, double* col1) {
+  const double X = tan(xi), Y = tan(eta), C = sqrt(1 + X * X), D = sqrt(1 + Y * Y);
+  const double xd0 = rule == 0 ? D : 1.0, ed0 = rule == 0 ? X * Y / C : 0.0;
+  const double xd1 = rule == 0 ? X * Y / D : 0.0, ed1 = rule == 0 ? C : 1.0;
+  xyzvec_from_xietavec(xd0, ed0, face, xi, eta, rule != 0, col0);
+  xyzvec_from_xietavec(xd1, ed1, face, xi, eta, rule != 0, col1);
+}
+
+// Reverse of k_swe_tendency for the cotangents tbar = (vor_t, div_t, h_t)bar:
+// vbar (velocity) and sbar (vor, div, h before the step) accumulated, lapbar
+// assigned.  sum_ab G_ab G_ba has dG_ab = 2 G_ba; the label-derivative
+// cotangents go back through the transposed leaf differentiation.
+template <int M>
+__global__ __launch_bounds__(128) void k_swe_tendency_adj(DevTree t, const double* __restrict__ px,
+                                                          const double* __restrict__ py,
+                                                          const double* __restrict__ pz, CP3 vel,
+                                                          const double* __restrict__ vor,
+                                                          const double* __restrict__ div,
+                                                          const double* __restrict__ h, CP3 tbar, double omega,
+                                                          int rule, double gravity, P3 vbar,
+                                                          double* __restrict__ lapbar, P3 sbar) {
+  __shared__ double sv[3][M * M], gx[3][M * M], ge[3][M * M], xn[M], en[M];
+  const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
+  const DevPanel& P = t.panels[leaf + t.offset];
+  leaf_nodes<M>(P, xn, en);
+  const int64_t k = (int64_t)leaf * (M * M) + o;
+  if (o < M * M)
+    for (int c = 0; c < 3; ++c) sv[c][o] = vel.p[c][k];
+  __syncthreads();
+  double vb[3] = {0, 0, 0};
+  if (o < M * M) {
+    const int face = rule == 0 ? t.panels[leaf].face : P.face;
+    const double xi = xn[a], eta = en[b];
+    double col0[3], col1[3], G[3][3];
+    grad_map(rule, face, xi, eta, col0, col1);
+    for (int c = 0; c < 3; ++c) {
+      const double fx = tile_deriv<M>(sv[c], a, b, 0, xn), fe = tile_deriv<M>(sv[c], a, b, 1, en);
+      for (int i = 0; i < 3; ++i) G[c][i] = fx * col0[i] + fe * col1[i];
+    }
+    const double tv = tbar.p[0][k], td = tbar.p[1][k], th = tbar.p[2][k];
+    for (int c = 0; c < 3; ++c) {
+      double bx = 0, be = 0;
+      for (int i = 0; i < 3; ++i) {
+        const double gb = -2.0 * td * G[i][c];
+        bx += gb * col0[i];
+        be += gb * col1[i];
+      }
+      gx[c][o] = bx;
+      ge[c][o] = be;
+    }
+    const double x = px[k], y = py[k], z = pz[k];
+    const double ux = sv[0][o], uy = sv[1][o], uz = sv[2][o];
+    const double vk = vor[k], dk = div[k];
+    vb[0] = -2.0 * ux * td;
+    vb[1] = -2.0 * uy * td;
+    vb[2] = -2.0 * uz * td - 2.0 * omega * tv;
+    if (x * x + y * y > 1e-16) {
+      vb[0] += 2.0 * omega * y * td;
+      vb[1] -= 2.0 * omega * x * td;
+    }
+    sbar.p[0][k] += -dk * tv + 2.0 * omega * z * td;
+    sbar.p[1][k] += -(vk + 2.0 * omega * z) * tv - h[k] * th;
+    sbar.p[2][k] += -dk * th;
+    lapbar[k] = -gravity * td;
+  }
+  __syncthreads();
+  if (o < M * M)
+    for (int c = 0; c < 3; ++c)
+      vbar.p[c][k] += vb[c] + tile_deriv_t<M>(gx[c], a, b, 0, xn) + tile_deriv_t<M>(ge[c], a, b, 1, en);
+}
+
+// Transpose of k_panel_lap: fbar += L^T lapbar.
+template <int M>
+__global__ __launch_bounds__(128) void k_panel_lap_adj(DevTree t, const double* __restrict__ lapbar,
+                                                       double* __restrict__ fbar) {
+  __shared__ double sa[M * M], sb[M * M], sc[M * M], sx[M * M], se[M * M], xn[M], en[M];
+  const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
+  const DevPanel& P = t.panels[leaf + t.offset];
+  leaf_nodes<M>(P, xn, en);
+  __syncthreads();
+  const int64_t k = (int64_t)leaf * (M * M) + o;
+  if (o < M * M) {
+    const double X = tan(xn[a]), Y = tan(en[b]);
+    const double C2 = 1 + X * X, D2 = 1 + Y * Y, delta = 1 + X * X + Y * Y, l = lapbar[k];
+    sa[o] = delta / C2 * l;                       // d/d f_xixi
+    sb[o] = delta / D2 * l;                       // d/d f_etaeta
+    sc[o] = delta * 2.0 * X * Y / (C2 * D2) * l;  // d/d f_xieta
+  }
+  __syncthreads();
+  if (o < M * M) {
+    sx[o] = tile_deriv_t<M>(sa, a, b, 0, xn);
+    se[o] = tile_deriv_t<M>(sc, a, b, 0, xn) + tile_deriv_t<M>(sb, a, b, 1, en);
+  }
+  __syncthreads();
+  if (o < M * M) fbar[k] += tile_deriv_t<M>(sx, a, b, 0, xn) + tile_deriv_t<M>(se, a, b, 1, en);
+}
+
+// Reverse of k_height_corr for lapbar: with kb = -8/0.1 lapbar, the transposed
+// interpolation kb B is scattered into hscat (h + tau h_t's cotangent; the
+// caller splits it), and the moved point's cotangent (interpolant and, with the
+// surface correction, topography gradients) goes back through the projection
+// into vbar += tau pbar.  tau = 0: the nodes themselves, no velocity term.
+template <int M>
+__global__ __launch_bounds__(128) void k_height_corr_adj(DevTree t, const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z, CP3 vel,
+                                                         const double* __restrict__ hh, double tau, SwePhys ph,
+                                                         const double* __restrict__ lapbar,
+                                                         double* __restrict__ hscat, P3 vbar) {
+  __shared__ double lds[2][wave_scatter_lds<M, 1>()];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < t.npart;
+  const int64_t u = valid ? i : t.npart - 1;
+  double w[3] = {x[u], y[u], z[u]};
+  if (tau != 0.0) {
+    w[0] += tau * vel.p[0][u]; w[1] += tau * vel.p[1][u]; w[2] += tau * vel.p[2][u];
+  }
+  double a = w[0], b = w[1], c = w[2];
+  project_to_sphere(a, b, c);
+  const double kb = -8.0 / 0.1 * lapbar[u];
+  double pb[3] = {0, 0, 0}, wx[M], we[M];
+  const double* f[1] = {hh};
+  int leaf = interp_grad<M, 1>(t, a, b, c, f, &kb, pb, wx, we, valid);
+  if (ph.surface_correction) {
+    double g[3];
+    topo_height(ph, a, b, c, g);
+    for (int q = 0; q < 3; ++q) pb[q] += kb * g[q];
+  }
+  double* dst[1] = {hscat};
+  wave_scatter_leaf<M, 1>(lds[threadIdx.x >> 6], valid ? leaf : -1, wx, we, &kb, dst, t.offset);
+  if (valid && tau != 0.0 && vbar.p[0]) {
+    project_adjoint(w[0], w[1], w[2], pb);
+    for (int q = 0; q < 3; ++q) vbar.p[q][u] += tau * pb[q];
+  }
+}
+
+// Reverse of one stage's state, area and (stages 2-3) effective-height
+// updates, given the charge cotangents of its velocity evaluation (qz, qd:
+// q = A_s S_vor, A_s S_div) and of its surface-Laplacian evaluation (qe:
+// q = A_s eff, eff = S_h + h_s(P(x + d_row)); qe null at stage 4):
+//   S_s = S0 + taua T,  A_s = area (1 + taua S0_div).
+// sbar (vor, div, h of S0) and tbar accumulated; with displacements given,
+// the topography's displacement cotangent is added to dbar at the row the
+// forward read (rule 0: row i of the [P][n] array, rule 1: the particle).
+__global__ void k_swe_stage_adj(DevTree t, const double* __restrict__ x, const double* __restrict__ y,
+                                const double* __restrict__ z, CP3 s0, CP3 tprev, const double* __restrict__ area,
+                                double taua, const double* __restrict__ qz, const double* __restrict__ qd,
+                                const double* __restrict__ qe, CP3 dpart, CP3 dprox, int rule, SwePhys ph, P3 sbar,
+                                P3 tbar, P3 dbar_part, P3 dbar_prox) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t.npart) return;
+  double st[3];
+  for (int q = 0; q < 3; ++q) st[q] = s0.p[q][k] + (tprev.p[0] ? taua * tprev.p[q][k] : 0.0);
+  const double as = area[k] * (1.0 + taua * s0.p[1][k]);
+  const double bz = as * qz[k], bd = as * qd[k], be = qe ? as * qe[k] : 0.0;
+  double ab = st[0] * qz[k] + st[1] * qd[k];
+  if (qe) {
+    double eff = st[2];
+    if (ph.topo) {
+      // the forward's displaced point
+      const int i = (int)(k / t.m2), j = (int)(k % t.m2);
+      const bool prox = rule == 0 && i < t.offset;
+      const int64_t row = rule != 0 ? k : (prox ? (int64_t)i * t.n + j : (int64_t)(i - t.offset) * t.m2 + j);
+      double w[3] = {x[k], y[k], z[k]};
+      if (dpart.p[0])
+        for (int q = 0; q < 3; ++q) w[q] += (prox ? dprox.p[q] : dpart.p[q])[row];
+      double a = w[0], b = w[1], c = w[2];
+      project_to_sphere(a, b, c);
+      double g[3];
+      eff += topo_height(ph, a, b, c, g);
+      if (dpart.p[0]) {
+        for (int q = 0; q < 3; ++q) g[q] *= be;
+        project_adjoint(w[0], w[1], w[2], g);
+        for (int q = 0; q < 3; ++q) (prox ? dbar_prox.p[q] : dbar_part.p[q])[row] += g[q];
+      }
+    }
+    ab += eff * qe[k];
+  }
+  sbar.p[0][k] += bz;
+  sbar.p[1][k] += bd + area[k] * taua * ab;
+  sbar.p[2][k] += be;
+  if (tbar.p[0]) {
+    tbar.p[0][k] += taua * bz;
+    tbar.p[1][k] += taua * bd;
+    tbar.p[2][k] += taua * be;
+  }
+}
+
+template <int M>
+struct TendencyAdj {
+  static void run(const DevTree& t, const double* x, const double* y, const double* z, CP3 vel, const double* vor,
+                  const double* div, const double* h, CP3 tbar, double omega, int rule, double gravity, P3 vbar,
+                  double* lapbar, P3 sbar, hipStream_t s) {
+    hipLaunchKernelGGL(k_swe_tendency_adj<M>, dim3(t.Pl), dim3(128), 0, s, t, x, y, z, vel, vor, div, h, tbar, omega,
+                       rule, gravity, vbar, lapbar, sbar);
+  }
+};
+
+template <int M>
+struct PanelLapAdj {
+  static void run(const DevTree& t, const double* lapbar, double* fbar, hipStream_t s) {
+    hipLaunchKernelGGL(k_panel_lap_adj<M>, dim3(t.Pl), dim3(128), 0, s, t, lapbar, fbar);
+  }
+};
+
+template <int M>
+struct HeightCorrAdj {
+  static void run(const DevTree& t, const double* x, const double* y, const double* z, CP3 vel, const double* hh,
+                  double tau, SwePhys ph, const double* lapbar, double* hscat, P3 vbar, hipStream_t s) {
+    hipLaunchKernelGGL(k_height_corr_adj<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, vel, hh,
+                       tau, ph, lapbar, hscat, vbar);
+  }
+};
 
 template <int M>
 struct PanelLap {
@@ -252,19 +492,19 @@ struct PanelLap {
 template <int M>
 struct Tendency {
   static void run(const DevTree& t, const double* x, const double* y, const double* z, CP3 vel, const double* vor,
-                  const double* div, const double* h, const double* lap, double omega, int rule, P3 out,
-                  hipStream_t s) {
+                  const double* div, const double* h, const double* lap, double omega, int rule, double gravity,
+                  P3 out, hipStream_t s) {
     hipLaunchKernelGGL(k_swe_tendency<M>, dim3(t.Pl), dim3(128), 0, s, t, x, y, z, vel, vor, div, h, lap, omega,
-                       rule, out);
+                       rule, gravity, out);
   }
 };
 
 template <int M>
 struct HeightCorr {
   static void run(const DevTree& t, const double* x, const double* y, const double* z, CP3 vel, const double* h,
-                  const double* ht, double tau, double* lap, hipStream_t s) {
+                  const double* ht, double tau, SwePhys ph, double* lap, hipStream_t s) {
     hipLaunchKernelGGL(k_height_corr<M>, dim3(blocks_for(t.npart, 128)), dim3(128), 0, s, t, x, y, z, vel, h, ht,
-                       tau, lap);
+                       tau, ph, lap);
   }
 };
 
@@ -287,16 +527,17 @@ void launch_panel_laplacian(const DevTree& t, const double* f, double* lap, hipS
 
 void launch_swe_tendency(const DevTree& t, const double* x, const double* y, const double* z,
                          const double* const* vel, const double* vor, const double* div, const double* h,
-                         const double* lap, double omega, int rule, double* const* out, hipStream_t s) {
-  if (t.Pl > 0) dispatch_m<Tendency>(t.m, t, x, y, z, cp3(vel), vor, div, h, lap, omega, rule, p3(out), s);
+                         const double* lap, double omega, int rule, double gravity, double* const* out,
+                         hipStream_t s) {
+  if (t.Pl > 0) dispatch_m<Tendency>(t.m, t, x, y, z, cp3(vel), vor, div, h, lap, omega, rule, gravity, p3(out), s);
 }
 
 void launch_topography(const DevTree& t, const double* x, const double* y, const double* z,
                        const double* const* disp_part, const double* const* disp_proxy, const double* h, int rule,
-                       double* eff, hipStream_t s) {
+                       const SwePhys& ph, double* eff, hipStream_t s) {
   if (t.npart > 0)
     hipLaunchKernelGGL(k_topography, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, x, y, z, cp3(disp_part),
-                       cp3(disp_proxy), h, rule, eff);
+                       cp3(disp_proxy), h, rule, ph, eff);
 }
 
 void launch_apply_tendencies(int64_t n, const double* const* f, const double* const* ft, const double* area,
@@ -307,9 +548,78 @@ void launch_apply_tendencies(int64_t n, const double* const* f, const double* co
 }
 
 void launch_height_correction(const DevTree& t, const double* x, const double* y, const double* z,
-                              const double* const* vel, const double* h, const double* ht, double tau, double* lap,
+                              const double* const* vel, const double* h, const double* ht, double tau,
+                              const SwePhys& ph, double* lap, hipStream_t s) {
+  if (t.npart > 0) dispatch_m<HeightCorr>(t.m, t, x, y, z, cp3(vel), h, ht, tau, ph, lap, s);
+}
+
+void launch_swe_tendency_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                                 const double* const* vel, const double* vor, const double* div, const double* h,
+                                 const double* const* tbar, double omega, int rule, double gravity,
+                                 double* const* vbar, double* lapbar, double* const* sbar, hipStream_t s) {
+  if (t.Pl > 0)
+    dispatch_m<TendencyAdj>(t.m, t, x, y, z, cp3(vel), vor, div, h, cp3(tbar), omega, rule, gravity, p3(vbar), lapbar,
+                            p3(sbar), s);
+}
+
+void launch_panel_laplacian_adjoint(const DevTree& t, const double* lapbar, double* fbar, hipStream_t s) {
+  if (t.Pl > 0) dispatch_m<PanelLapAdj>(t.m, t, lapbar, fbar, s);
+}
+
+void launch_height_correction_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                                      const double* const* vel, const double* hh, double tau, const SwePhys& ph,
+                                      const double* lapbar, double* hscat, double* const* vbar, hipStream_t s) {
+  P3 vb{};
+  if (vbar)
+    for (int i = 0; i < 3; ++i) vb.p[i] = vbar[i];
+  if (t.npart > 0) dispatch_m<HeightCorrAdj>(t.m, t, x, y, z, cp3(vel), hh, tau, ph, lapbar, hscat, vb, s);
+}
+
+void launch_swe_stage_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
+                              const double* const* s0, const double* const* tprev, const double* area, double taua,
+                              const double* qz, const double* qd, const double* qe, const double* const* dpart,
+                              const double* const* dprox, int rule, const SwePhys& ph, double* const* sbar,
+                              double* const* tbar, double* const* dbar_part, double* const* dbar_prox,
                               hipStream_t s) {
-  if (t.npart > 0) dispatch_m<HeightCorr>(t.m, t, x, y, z, cp3(vel), h, ht, tau, lap, s);
+  P3 tb{}, dbp{}, dbx{};
+  for (int i = 0; i < 3; ++i) {
+    tb.p[i] = tbar ? tbar[i] : nullptr;
+    dbp.p[i] = dbar_part ? dbar_part[i] : nullptr;
+    dbx.p[i] = dbar_prox ? dbar_prox[i] : nullptr;
+  }
+  if (t.npart > 0)
+    hipLaunchKernelGGL(k_swe_stage_adj, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, x, y, z, cp3(s0),
+                       cp3(tprev), area, taua, qz, qd, qe, cp3(dpart), cp3(dprox), rule, ph, p3(sbar), tb, dbp, dbx);
+}
+
+namespace {
+// The height correction at the nodes themselves (the regularised stage-1
+// Laplacian): lap -= 8/0.1 (h + h_s(x)), h_s only with the surface correction.
+__global__ void k_node_corr(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, const double* __restrict__ h, SwePhys ph,
+                            double* __restrict__ lap) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v = h[k];
+  if (ph.surface_correction) v += topo_height(ph, x[k], y[k], z[k]);
+  lap[k] -= 8.0 * v / 0.1;
+}
+
+// out = x + alpha y (out may alias x or y)
+__global__ void k_lincomb(int64_t n, const double* __restrict__ x, double alpha, const double* __restrict__ y,
+                          double* out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = x[k] + alpha * y[k];
+}
+}  // namespace
+
+void launch_node_correction(int64_t n, const double* x, const double* y, const double* z, const double* h,
+                            const SwePhys& ph, double* lap, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_node_corr, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, y, z, h, ph, lap);
+}
+
+void launch_lincomb(int64_t n, const double* x, double alpha, const double* y, double* out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, alpha, y, out);
 }
 
 // unify_boundary_vals (initialize_cubed_sphere.cpp:328-370): every copy of a

@@ -113,6 +113,43 @@ def gaussian_vortices(x, y, z, area, count=12, seed=0):
     return w
 
 
+# ---------------------------------------------------------------- shallow water on a rotating sphere
+
+EARTH_RADIUS = 6.37122e6   # m
+GRAVITY = 9.80616          # m s^-2
+
+
+def gaussian_mountain(seed=0):
+    """Seeded Gaussian mountain: height U(1000, 2500) m, width U(0.1, 0.3) rad, centre
+    uniform on the sphere.  Returns (height, width, lat, lon)."""
+    rng = np.random.default_rng(seed)
+    h0 = rng.uniform(1000.0, 2500.0)
+    w = rng.uniform(0.1, 0.3)
+    c = rng.normal(size=3)
+    c /= np.linalg.norm(c)
+    return float(h0), float(w), float(np.arcsin(c[2])), float(np.arctan2(c[1], c[0]))
+
+
+def mountain_height(x, y, z, mountain):
+    """h_s = H exp(-(r/w)^2), r the great-circle distance to the centre."""
+    h0, w, lat, lon = mountain
+    c = np.array([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)])
+    cx, cy, cz = y * c[2] - z * c[1], z * c[0] - x * c[2], x * c[1] - y * c[0]
+    r = np.arctan2(np.sqrt(cx * cx + cy * cy + cz * cz), x * c[0] + y * c[1] + z * c[2])
+    return h0 * np.exp(-(r / w) ** 2)
+
+
+def swe_zonal_flow(x, y, z, mountain=None, u0=20.0, depth=5960.0, omega=7.292e-5, a=EARTH_RADIUS, g=GRAVITY):
+    """Balanced zonal flow u = u0 cos(lat) on the unit sphere (velocities in rad/s, time in s,
+    heights in m): relative vorticity 2 (u0/a) sin(lat), zero divergence, and the fluid depth
+    under the free surface depth - (a Omega u0 + u0^2/2) sin^2(lat) / g (Williamson et al. 1992,
+    test cases 2 and 5), minus the mountain.  Returns (vor, div, h)."""
+    vor = 2.0 * (u0 / a) * z
+    surface = depth - (a * omega * u0 + 0.5 * u0 * u0) * z * z / g
+    hs = 0.0 if mountain is None else mountain_height(x, y, z, mountain)
+    return vor, np.zeros_like(z), surface - hs
+
+
 def random_unit_vectors(n, seed=0):
     rng = np.random.default_rng(seed)
     v = rng.normal(size=(n, 3))

@@ -183,3 +183,67 @@ def test_step_at_rest_keeps_the_state(orc, sb, rule):
     assert np.abs(v).max() == 0 and np.abs(d).max() == 0
     assert np.abs(h / H - 1).max() < 1e-14
     assert max(np.abs(u).max() for u in vel) == 0 and np.abs(lap).max() == 0
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_gaussian_topography(orc, sb, rule):
+    """h_s = H exp(-(r/w)^2) against the great-circle distance taken as arccos(p.c)."""
+    o = orc.OracleFMM(3, 6, 6, 0.7, 0)
+    x, y, z, _ = o.spec_points()
+    mtn = (1500.0, 0.2, 0.4, -1.1)
+    o.set_swe_physics(1.0, mtn, 0)
+    h = 5000.0 + 10.0 * x
+    rng = np.random.default_rng(4)
+    dp = [1e-2 * rng.normal(size=x.shape) for _ in range(3)]
+    dx = [1e-2 * rng.normal(size=(o.P, o.n)) for _ in range(3)]
+    eff = o.apply_topography(x.ravel(), y.ravel(), z.ravel(), h, 2, rule, dp, dx)
+    if rule == 1:
+        d = [v.copy() for v in dp]
+    else:
+        off = o.P - o.Pl
+        d = [np.concatenate([dx[f][:off], dp[f]])[: o.Pl] for f in range(3)]
+    p = np.stack([x + d[0], y + d[1], z + d[2]])
+    p /= np.linalg.norm(p, axis=0)
+    c = np.array([np.cos(mtn[2]) * np.cos(mtn[3]), np.cos(mtn[2]) * np.sin(mtn[3]), np.sin(mtn[2])])
+    r = np.arccos(np.clip(np.tensordot(c, p, 1), -1, 1))
+    assert np.abs(eff - (h + 1500.0 * np.exp(-(r / 0.2) ** 2))).max() < 1e-8
+    assert (eff - h).max() > 1000.0  # some particles sit near the top
+
+
+def test_gravity_scales_the_surface_laplacian_term(orc, sb):
+    o = orc.OracleFMM(2, 6, 6, 0.7, 0)
+    x, y, z, _ = o.spec_points()
+    vel = [1e-6 * (-y), 1e-6 * x, 0 * z]
+    vor, div, h, lap = 2e-6 * z, 1e-7 * x * y, 5000 + 100 * z, 3.0 + x
+    t1 = o.swe_tendency(x, y, z, vel, vor, div, h, lap, 7.292e-5, 1)
+    o.set_swe_physics(2.5e-13)
+    tg = o.swe_tendency(x, y, z, vel, vor, div, h, lap, 7.292e-5, 1)
+    assert np.array_equal(t1[0], tg[0]) and np.array_equal(t1[2], tg[2])
+    assert np.abs((tg[1] - t1[1]) - (1 - 2.5e-13) * lap).max() < 1e-12 * np.abs(lap).max()
+
+
+def test_regularised_stage1_laplacian_and_surface_correction(orc, sb):
+    """Stage-1 surface Laplacian of the regularised form at rest.  A flat free surface over a
+    mountain (h = c - h_s) has zero Laplacian with the surface correction; without it the
+    correction leaves 8/0.1 h_s.  A degree-2 harmonic height has eigenvalue near -6."""
+    W = sb.workloads
+    o = orc.OracleFMM(4, 6, 6, 0.7, 1)
+    x, y, z, a = o.spec_points()
+    mtn = (1500.0, 0.25, 0.3, 2.0)
+    hs = W.mountain_height(x, y, z, mtn)
+    h = 5000.0 - hs
+    args = (x.ravel(), y.ravel(), z.ravel(), a, 0 * x, 0 * x)
+    lap = {}
+    for form in (1, 3):
+        o.set_swe_physics(1.0, mtn, form)
+        lap[form] = o.swe_back_rk4_step(*args, h, 900.0, 7.292e-5, 1e-4, 0, 2, 1)[4]
+    # K * 1 = 8/0.1 holds to the quadrature's ~1% at this size (K * c - 80 c leaves 3.2% of
+    # 80 max h_s at c = 5000), against the full 80 h_s the height-only correction leaves
+    scale = 80.0 * hs.max()
+    assert np.abs(lap[3]).max() < 0.05 * scale
+    assert np.abs(lap[1] - 80.0 * hs).max() < 0.05 * scale and np.abs(lap[1]).max() > 0.9 * scale
+    o.set_swe_physics(1.0, mtn, 3)
+    f = sh(W, 2, 1, x, y, z)
+    got = o.swe_back_rk4_step(*args, f, 900.0, 7.292e-5, 1e-4, 0, 0, 1)[4]
+    lam = np.sum(got * f * a) / np.sum(f * f * a)
+    assert abs(lam / -6.0 - 1) < 0.03
