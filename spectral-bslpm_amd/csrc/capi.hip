@@ -215,6 +215,23 @@ int upload(sbslpm_ctx* c, void* dst, const void* src, size_t bytes) {
   return SBSLPM_OK;
 }
 
+// The logarithm and exponential tables of the FP64 pair kernels, built once
+// per process.
+struct HostTables {
+  std::vector<LogTabEntry> log = std::vector<LogTabEntry>(kLogTabSize);
+  std::vector<double> exp = std::vector<double>(kExpTabSize);
+  std::vector<double> exp_neg = std::vector<double>(kExpNegTabSize);
+  HostTables() {
+    build_log_table(log.data());
+    build_exp_table(exp.data());
+    build_exp_neg_table(exp_neg.data());
+  }
+};
+const HostTables& host_tables() {
+  static const HostTables tables;
+  return tables;
+}
+
 int velocity(sbslpm_ctx* c, const Work& w, int nmem, const double* area, const double* vor,
              const double* const* dpart, const double* const* dprox, double* const* vel);
 
@@ -311,6 +328,8 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   size_t o_disppart = A.take<double>(3 * NP), o_dispproxy = A.take<double>(3 * Pn);
   size_t o_vstage = A.take<double>(9 * NP), o_newvor = A.take<double>(NP), o_dep = A.take<double>(3 * NP);
   size_t o_vnext = A.take<double>(NP), o_trnext = A.take<double>(NP * ntr);
+  size_t o_ltab = A.take<LogTabEntry>(kLogTabSize), o_etab = A.take<double>(kExpTabSize),
+         o_entab = A.take<double>(kExpNegTabSize);
   size_t total = A.take<char>(256);
   if (hipMalloc(&c->arena, total) != hipSuccess) {
     delete c;
@@ -337,6 +356,12 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   rc |= upload(c, b + o_sets, t.srcsets.data(), t.srcsets.size() * sizeof(SourceSet));
   rc |= upload(c, b + o_zr, t.zero_rows.data(), t.zero_rows.size() * sizeof(int));
   rc |= upload(c, b + o_ss, t.set_start.data(), t.set_start.size() * sizeof(int));
+  {
+    const HostTables& ht = host_tables();
+    rc |= upload(c, b + o_ltab, ht.log.data(), kLogTabSize * sizeof(LogTabEntry));
+    rc |= upload(c, b + o_etab, ht.exp.data(), kExpTabSize * sizeof(double));
+    rc |= upload(c, b + o_entab, ht.exp_neg.data(), kExpNegTabSize * sizeof(double));
+  }
   if (rc != SBSLPM_OK || hipMemset(b + o_err, 0, 4 * sizeof(int)) != hipSuccess) {
     hipFree(c->arena);
     delete c;
@@ -351,6 +376,9 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   d.p2m = (const double*)(b + o_p2m); d.p2mT = (const double*)(b + o_p2mT);
   d.proxy_leaf = (const int*)(b + o_pl); d.proxy_xieta = (const double*)(b + o_pxe);
   d.proxy_order = (const int*)(b + o_po);
+  d.logtab = (const LogTabEntry*)(b + o_ltab);
+  d.exptab = (const double*)(b + o_etab);
+  d.expnegtab = (const double*)(b + o_entab);
   d.err = (int*)(b + o_err);
   d.P = t.P; d.Pl = t.Pl; d.offset = t.offset; d.n = t.n; d.dp1 = t.dp1; d.m = t.m; d.m2 = t.m2;
   d.levels = t.cfg.levels; d.npart = t.npart; d.U = t.U;

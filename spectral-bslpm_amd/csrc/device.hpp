@@ -1,5 +1,6 @@
 // device.hpp -- device-side view of the tree and the kernel launchers.
 #pragma once
+#include "fasttab.hpp"
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -32,6 +33,9 @@ struct DevTree {
   const int* proxy_leaf;      // [offset*n]
   const double* proxy_xieta;  // [offset*n][2]
   const int* proxy_order;     // [offset*n] proxy nodes sorted by leaf
+  const LogTabEntry* logtab;  // [kLogTabSize] table of log_tab (fasttab.hpp)
+  const double* exptab;       // [kExpTabSize] table of exp_tab
+  const double* expnegtab;    // [kExpNegTabSize] table of exp_neg
   int* err;
   int P, Pl, offset, n, dp1, m, m2, levels;
   int64_t npart, U;

@@ -19,7 +19,6 @@
 #include <cmath>
 
 #include "device.hpp"
-#include "fastlog.hpp"
 
 namespace sbslpm {
 namespace {
@@ -29,6 +28,21 @@ struct Ptr3 {
 };
 struct CPtr3 {
   const double* p[3];
+};
+
+// A uniform constant moved into a vector register: the first step of a Horner
+// scheme whose multiplier is such a register is one three-operand fma (with
+// two constants in scalar registers the instruction set needs a copy).
+__device__ inline double in_vgpr(double v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
+// Constants of the FP64 regularised-Laplacian pair kernel (k_lap64).
+struct LapPoly {
+  double a1, a0;      // A(t) = t (a1 t^2 + a0)
+  double b2, b1, b0;  // B(t) = (b2 t^2 + b1) t^2 + b0
+  double inv_eps, cut, t_sat;
 };
 
 // Member slices of a batched launch (gridDim.y members): charges and
@@ -155,17 +169,20 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
   if (lane < T.count) store_u(t, u, part, proxy, y * w2 - z * w1, z * w0 - x * w2, x * w1 - y * w0, assign);
 }
 
-struct HwRcp64 {
-  __device__ double operator()(double d) const { return __builtin_amdgcn_rcp(d); }
-};
-
 // FP64 log pair kernel: psi += -1/(4 pi) log(1 - x.y) q, pairs with
 // x.y >= 1 - 1e-15 skipped (inverse_poisson_impl.hpp:46-49).  The logarithm
-// is log_pos (within 2 ulp of the correctly rounded one).
+// is log_tab (fasttab.hpp, within 1.1 ulp of the correctly rounded one) on
+// the context's table, staged in LDS.  A skipped pair (a source that coincides
+// with a target: the target's own node, or a node on a shared leaf edge) is
+// rare, so the wave tests the whole source at once (a ballot, one compare per
+// pair) and only a wave holding such a pair takes the guarded form.
 __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
                                                const SrcRec64* __restrict__ rec, double* __restrict__ part,
                                                double* __restrict__ proxy, int assign) {
+  __shared__ LogTabEntry ltab[kLogTabSize];
+  for (int i = threadIdx.x; i < kLogTabSize; i += blockDim.x) ltab[i] = t.logtab[i];
+  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -173,19 +190,61 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
   const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  constexpr double kSkip = 1.0 - 1e-15;
+  const double c6 = in_vgpr(-1.0 / 6.0);
   double acc = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
     const SrcRec64* __restrict__ r = rec + S.first;
-#pragma unroll 2
-    for (int k = 0; k < S.count; ++k) {
-      const SrcRec64 s = r[k];
-      const double dp = x * s.x + y * s.y + z * s.z;
-      const double l = log_pos(fmax(1.0 - dp, 1e-300), HwRcp64{});
-      acc += (dp < 1.0 - 1e-15) ? l * s.q : 0.0;
+    int k = 0;
+    for (; k + 1 < S.count; k += 2) {
+      const SrcRec64 s0 = r[k], s1 = r[k + 1];
+      const double dp0 = x * s0.x + y * s0.y + z * s0.z;
+      const double dp1 = x * s1.x + y * s1.y + z * s1.z;
+      double l0 = log_tab(1.0 - dp0, ltab, c6), l1 = log_tab(1.0 - dp1, ltab, c6);
+      const bool sk0 = dp0 >= kSkip, sk1 = dp1 >= kSkip;
+      if ((__builtin_amdgcn_ballot_w64(sk0) | __builtin_amdgcn_ballot_w64(sk1)) != 0) {
+        // a real branch (the empty asm keeps the compiler from turning it into
+        // selects on every pair); log_tab's index is masked, so the discarded
+        // values were read in bounds
+        __asm__ volatile("");
+        l0 = sk0 ? 0.0 : l0;
+        l1 = sk1 ? 0.0 : l1;
+      }
+      acc = fma(l0, s0.q, acc);
+      acc = fma(l1, s1.q, acc);
+    }
+    if (k < S.count) {
+      const SrcRec64 s0 = r[k];
+      const double dp0 = x * s0.x + y * s0.y + z * s0.z;
+      const double l0 = log_tab(1.0 - dp0, ltab, c6);
+      acc = fma(dp0 >= kSkip ? 0.0 : l0, s0.q, acc);
     }
   }
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
+}
+
+// Constants of k_lap64 with the Horner multipliers in vector registers.
+struct LapPolyV {
+  double a1, b2, c24;
+};
+
+// K(t) of k_lap64 where t >= t_sat (erf factor exactly 1).
+__device__ inline double lap_pair(double dp, const LapPoly& kp, const LapPolyV& kv, const double* etab) {
+  const double t2 = dp * dp;
+  const double A = dp * fma(kv.a1, t2, kp.a0);
+  const double B = fma(fma(kv.b2, t2, kp.b1), t2, kp.b0);
+  return fma(exp_neg(fma(t2, kp.inv_eps, -kp.inv_eps), etab, kv.c24), B, A);
+}
+// K(t) for any t, zero at or below the cut.
+__device__ inline double lap_pair_slow(double dp, const LapPoly& kp, const LapPolyV& kv, const double* etab) {
+  const double t2 = dp * dp;
+  const double A = dp * fma(kv.a1, t2, kp.a0);
+  const double B = fma(fma(kv.b2, t2, kp.b1), t2, kp.b0);
+  const double sv = dp * kp.inv_eps;
+  const double h = sv >= 6.0 ? 1.0 : (sv <= -6.0 ? 0.0 : 0.5 * (1.0 + erf(sv)));
+  const double K = fma(exp_neg(fma(t2, kp.inv_eps, -kp.inv_eps), etab, kv.c24) * h, B, A);
+  return dp > kp.cut ? K : 0.0;
 }
 
 // Regularised-Laplacian pair kernel (lap_panel_interaction,
@@ -194,15 +253,23 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
 //           + (1 + erf(t/eps)) e^{(t^2-1)/eps} c2 (2 eps^2 + 2t^2(t^2-1) + eps(7t^2-1))] c3,
 // c1 = e^{-1/eps} pi^{-3/2}, c2 = 1/(sqrt(eps) pi), c3 = eps^{-5/2}; phi += K q.
 // Sources are displaced and projected; the records hold q' = -q/(4 pi), so c3
-// arrives multiplied by -4 pi.  erf(s) rounds to exactly +-1 in FP64 for
-// |s| >= 6 (erfc(6) = 2.2e-17 is below half an ulp of 1), so for |t| >= 6 eps the
-// erf call is skipped with bit-identical results; at the tuned sizes over 99% of
-// the pair work has t > 0.6.
+// arrives multiplied by -4 pi.  The kernel is evaluated as
+//   K(t) = A(t) + e^{(t^2-1)/eps} B(t) (1 + erf(t/eps))/2,
+//   A = t (a1 t^2 + a0),  B = (b2 t^2 + b1) t^2 + b0 (LapPoly: the constants
+//   folded on the host, B carrying the factor 2 of the saturated erf).
+// erf(s) rounds to exactly +-1 in FP64 for |s| >= 6 (erfc(6) = 2.2e-17 is below
+// half an ulp of 1), so for t >= 6 eps the erf factor is exactly 1; over 99% of
+// the pair work has t > 0.6, so a wave tests the whole source at once (a
+// ballot) and evaluates erf only when one of its pairs needs it.  The
+// exponential is exp_neg (fasttab.hpp: its argument lies in [-1/eps, 0]) on the
+// context's table, staged in LDS.
 __global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
-                                               const SrcRec64* __restrict__ rec, double eps, double c1, double c2,
-                                               double c3, double* __restrict__ part, double* __restrict__ proxy,
-                                               int assign) {
+                                               const SrcRec64* __restrict__ rec, LapPoly kp,
+                                               double* __restrict__ part, double* __restrict__ proxy, int assign) {
+  __shared__ double etab[kExpNegTabSize];
+  for (int i = threadIdx.x; i < kExpNegTabSize; i += blockDim.x) etab[i] = t.expnegtab[i];
+  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -210,23 +277,33 @@ __global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __re
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
   const double x = t.px[u], y = t.py[u], z = t.pz[u];
-  const double cut = 1.0 - 30 * eps, inv_eps = 1.0 / eps;
+  // every pair passes the cut when 1 - 30 eps < -1 (eps = 0.1)
+  const bool no_cut = kp.cut < -1.0;
+  const LapPolyV kv{in_vgpr(kp.a1), in_vgpr(kp.b2), in_vgpr(1.0 / 24.0)};
   double acc = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
     const SrcRec64* __restrict__ r = rec + S.first;
-    for (int k = 0; k < S.count; ++k) {
-      const SrcRec64 s = r[k];
-      const double dp = x * s.x + y * s.y + z * s.z;
-      if (dp > cut) {
-        const double dp2 = dp * dp;
-        const double h1 = c1 * (2.0 * dp * (-1.0 + 3.0 * eps + dp2));
-        const double s_ = dp * inv_eps;
-        const double ef = s_ >= 6.0 ? 1.0 : (s_ <= -6.0 ? -1.0 : erf(s_));
-        const double h2 = (1.0 + ef) * exp((dp2 - 1) * inv_eps) * c2 *
-                          (2.0 * eps * eps + 2.0 * dp2 * (dp2 - 1.0) + eps * (7.0 * dp2 - 1.0));
-        acc = fma((h1 + h2) * c3, s.q, acc);
+    int k = 0;
+    for (; k + 1 < S.count; k += 2) {
+      const SrcRec64 s0 = r[k], s1 = r[k + 1];
+      const double dp0 = x * s0.x + y * s0.y + z * s0.z;
+      const double dp1 = x * s1.x + y * s1.y + z * s1.z;
+      double K0 = lap_pair(dp0, kp, kv, etab), K1 = lap_pair(dp1, kp, kv, etab);
+      if (!no_cut || (__builtin_amdgcn_ballot_w64(dp0 < kp.t_sat) | __builtin_amdgcn_ballot_w64(dp1 < kp.t_sat)) != 0) {
+        // a real branch (the empty asm keeps the compiler from evaluating erf
+        // on every pair)
+        __asm__ volatile("");
+        K0 = lap_pair_slow(dp0, kp, kv, etab);
+        K1 = lap_pair_slow(dp1, kp, kv, etab);
       }
+      acc = fma(K0, s0.q, acc);
+      acc = fma(K1, s1.q, acc);
+    }
+    if (k < S.count) {
+      const SrcRec64 s0 = r[k];
+      const double dp0 = x * s0.x + y * s0.y + z * s0.z;
+      acc = fma(lap_pair_slow(dp0, kp, kv, etab), s0.q, acc);
     }
   }
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
@@ -815,8 +892,18 @@ void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int 
   const double eps = 0.1, pi = 3.14159265358979323846;
   const double c1 = std::exp(-1.0 / eps) * std::pow(pi, -1.5), c2 = 1.0 / (std::sqrt(eps) * pi);
   const double c3 = -4.0 * pi / std::pow(eps, 2.5);
+  LapPoly kp;
+  kp.a1 = 2.0 * c1 * c3;
+  kp.a0 = 2.0 * c1 * c3 * (3.0 * eps - 1.0);
+  kp.b2 = 4.0 * c2 * c3;
+  kp.b1 = 2.0 * c2 * c3 * (7.0 * eps - 2.0);
+  kp.b0 = 2.0 * c2 * c3 * (2.0 * eps * eps - eps);
+  kp.inv_eps = 1.0 / eps;
+  kp.cut = 1.0 - 30 * eps;
+  // t >= t_sat gives t / eps >= 6 after rounding (the margin covers the product's rounding)
+  kp.t_sat = 6.0 * eps * (1.0 + 1e-12);
   hipLaunchKernelGGL(k_lap64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, eps, c1, c2, c3, out_part, out_proxy, assign ? 1 : 0);
+                     ntiles, sets, rec, kp, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,

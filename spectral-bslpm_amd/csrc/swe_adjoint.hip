@@ -235,19 +235,19 @@ __global__ void k_lap_tgt(DevTree t, const double* __restrict__ psi_part, const 
 //   h2' = c2 (A' E Q + A (2t/eps) E Q + A E Q'),  A' = 2/(sqrt(pi) eps) e^{-t^2/eps^2},
 //        Q' = 8t^3 - 4t + 14 eps t.
 // For |t/eps| >= 6, erf is +-1 in FP64 and the A' term is below an ulp of the
-// others, as in the forward kernel.
-__device__ inline void lap_kernel_d(double tt, double c1, double c2, double& k, double& kd) {
+// others, as in the forward kernel.  Exponentials by exp_tab, as the forward.
+__device__ inline void lap_kernel_d(double tt, double c1, double c2, const double* etab, double& k, double& kd) {
   const double eps = kLapEps, inv_eps = 1.0 / kLapEps;
   const double t2 = tt * tt;
   const double h1 = c1 * (2.0 * tt * (-1.0 + 3.0 * eps + t2));
   const double h1d = c1 * (6.0 * t2 - 2.0 + 6.0 * eps);
   const double s = tt * inv_eps;
   const double A = s >= 6.0 ? 2.0 : (s <= -6.0 ? 0.0 : 1.0 + erf(s));
-  const double E = exp((t2 - 1.0) * inv_eps);
+  const double E = exp_tab((t2 - 1.0) * inv_eps, etab);
   const double Q = 2.0 * eps * eps + 2.0 * t2 * (t2 - 1.0) + eps * (7.0 * t2 - 1.0);
   const double Qd = 8.0 * t2 * tt - 4.0 * tt + 14.0 * eps * tt;
   double h2d = A * E * (2.0 * tt * inv_eps * Q + Qd);
-  if (fabs(s) < 6.0) h2d += 2.0 / sqrt(kPiL) * inv_eps * exp(-s * s) * E * Q;
+  if (fabs(s) < 6.0) h2d += 2.0 / sqrt(kPiL) * inv_eps * exp_tab(-s * s, etab) * E * Q;
   k = h1 + c2 * A * E * Q;
   kd = h1d + c2 * h2d;
 }
@@ -261,6 +261,9 @@ __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* 
                                                    CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
                                                    double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
                                                    int acc, int xcd) {
+  __shared__ double etab[kExpTabSize];
+  for (int i = threadIdx.x; i < kExpTabSize; i += blockDim.x) etab[i] = t.exptab[i];
+  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int tile = shaped_tile(wave, xcd);
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* 
       const double tt = R.x * S.x + R.y * S.y + R.z * S.z;
       if (tt > cut) {
         double kv, kd;
-        lap_kernel_d(tt, c1, c2, kv, kd);
+        lap_kernel_d(tt, c1, c2, etab, kv, kd);
         aq = fma(R.psi, kv, aq);
         if (POS) {
           const double w = R.psi * kd;
