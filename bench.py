@@ -17,6 +17,7 @@ oracle restatement of the reference algorithm timed on a bounded sample on
 this host's cores).
 """
 import argparse
+import ctypes
 import importlib
 import json
 import os
@@ -543,15 +544,23 @@ def bench_c5(args, torch, sb):
         t_fwd, st = timed(lambda: forward(ctx, first), args.steps)
         t_grad, (L, grad) = timed(lambda: ag.swe_4dvar(ctx, *g, first, nsteps, every, obs, ov, oh, sig_v, sig_h),
                                   args.steps)
+        # the same gradient with every step's stage record kept in HBM by the forward run (one
+        # segment: no rerun; 24 x 21 fields of 221,184 doubles = 0.89 GB)
+        t_hbm, (L_hbm, g_hbm) = timed(lambda: ag.swe_4dvar(ctx, *g, first, nsteps, nsteps, obs, ov, oh, sig_v,
+                                                          sig_h), args.steps)
+        # (the reverse passes sum transposed interpolations with atomics, so gradients agree to
+        # rounding, not bit for bit)
+        same = max(W.rel_l2(v.cpu().numpy(), u.cpu().numpy()) for u, v in zip(grad, g_hbm))
         bounded = all(bool(torch.isfinite(v).all()) for v in st)
         res[p] = dict(fwd=t_fwd, grad=t_grad, L=float(L), grad_fields=grad, bounded=bounded,
-                      hmin=float(st[2].min()), hmax=float(st[2].max()), vmax=float(st[0].abs().max()))
+                      hmin=float(st[2].min()), hmax=float(st[2].max()), vmax=float(st[0].abs().max()),
+                      grad_hbm=t_hbm, hbm_same=same)
         log(f"c5 {p}: forward {nsteps} steps {1e3 * t_fwd:.1f} ms, gradient {1e3 * t_grad:.1f} ms "
-            f"({t_grad / t_fwd:.2f}x), loss {float(L):.6e}, h in [{res[p]['hmin']:.0f}, {res[p]['hmax']:.0f}]")
+            f"({t_grad / t_fwd:.2f}x), loss {float(L):.6e}, h in [{res[p]['hmin']:.0f}, {res[p]['hmax']:.0f}]; "
+            f"records in HBM: {1e3 * t_hbm:.1f} ms ({t_hbm / t_fwd:.2f}x, gradient rel-L2 {same:.1e} from the "
+            f"checkpointed one)")
     gerr = [W.rel_l2(u.cpu().numpy(), v.cpu().numpy()) for u, v in zip(res["mixed"]["grad_fields"],
                                                                         res["fp64"]["grad_fields"])]
-    # finite differences of the FP64 run; a direction whose quotients at h and h/8 disagree has a
-    # leaf-edge crossing inside its stencil (the step is piecewise smooth) and is replaced
     # finite differences of the FP64 run (relative step 1.25e-3 of the direction; leaf-edge
     # crossings inside the stencil set a floor near 1e-5, see tests/test_gpu_swe_4dvar.py)
     rng = np.random.default_rng(74)
@@ -597,6 +606,13 @@ def bench_c5(args, torch, sb):
             "forward_ms": round(1e3 * m["fwd"], 3), "gradient_ms": round(1e3 * m["grad"], 3), "loss": m["L"],
             "fp64": {"forward_ms": round(1e3 * f["fwd"], 3), "gradient_ms": round(1e3 * f["grad"], 3),
                      "ratio": round(f["grad"] / f["fwd"], 4), "loss": f["L"]},
+            "records_in_hbm": {"what": "every step's stage record kept by the forward run (one segment, no "
+                                       "rerun; 0.89 GB) instead of checkpoints every 6 steps",
+                               "mixed_gradient_ms": round(1e3 * m["grad_hbm"], 3),
+                               "mixed_ratio": round(m["grad_hbm"] / m["fwd"], 4),
+                               "fp64_gradient_ms": round(1e3 * f["grad_hbm"], 3),
+                               "fp64_ratio": round(f["grad_hbm"] / f["fwd"], 4),
+                               "gradient_rel_l2_vs_checkpointed": float(f"{max(m['hbm_same'], f['hbm_same']):.3e}")},
             "mixed_vs_fp64_gradient_rel_l2": [float(f"{e:.3e}") for e in gerr],
             "forward_bounded": m["bounded"] and f["bounded"],
             "final_height_range_m": [round(m["hmin"], 1), round(m["hmax"], 1)],
@@ -605,6 +621,52 @@ def bench_c5(args, torch, sb):
             if rels else None,
             "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
+
+
+PROBE_MIXES = {
+    0: "16 v_fma_f32 (independent)",
+    1: "16 v_rcp_f32 (independent)",
+    2: "4 v_mfma_f32_16x16x4_f32 (independent accumulators)",
+    3: "4 v_mfma_f32_4x4x1_16b_f32 (independent accumulators)",
+    4: "k_bve_mfma 256-pair step: 16x16x4 + 4 add + 4 rcp + 4 x 4x4x1",
+    5: "the same with the 4 reciprocals batched: 16x16x4 + 4 add + 9 mul + 1 rcp + 4 x 4x4x1",
+    6: "half the waves mix 2, half mix 0 (vector/matrix co-issue)",
+    7: "4 v_mfma_f32_16x16x32_bf16 (independent accumulators)",
+    8: "step with a 16x16x32 bf16 first product, alpha as its accumulator, batched reciprocals",
+    9: "step with a 16x16x32 bf16 first product, alpha as its accumulator, 4 reciprocals",
+}
+
+
+def bench_probe(args, torch):
+    """Register-only issue-rate probes (spectral-bslpm_amd/csrc/probe/issue_rates.hip) behind the
+    floor table of the mixed pair kernel: shader-clock cycles per iteration of each instruction
+    mix, per wave and per SIMD, at 1 and 4 waves per SIMD (256 compute units x 4 SIMDs)."""
+    lib_path = os.path.join(REPO, "spectral-bslpm_amd", "libsbslpm_probe.so")
+    if not os.path.exists(lib_path):
+        raise SystemExit("libsbslpm_probe.so is not built: run __graft_entry__.build()")
+    torch.cuda.init()
+    lib = ctypes.CDLL(lib_path)
+    lib.sbslpm_probe_issue.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.sbslpm_probe_issue.restype = ctypes.c_int
+    iters = 20000
+    out = {}
+    for mix, what in PROBE_MIXES.items():
+        row = {"mix": what}
+        for wps in (1, 4):
+            blocks, wpb = 256 * wps, 4  # one block of 4 waves per compute unit and wps blocks per unit
+            cyc, ms = ctypes.c_double(), ctypes.c_double()
+            lib.sbslpm_probe_issue(mix, blocks, wpb, iters // 10, ctypes.byref(cyc), ctypes.byref(ms))  # warm-up
+            rc = lib.sbslpm_probe_issue(mix, blocks, wpb, iters, ctypes.byref(cyc), ctypes.byref(ms))
+            if rc:
+                raise SystemExit(f"probe mix {mix} failed with {rc}")
+            row[f"{wps}_wave_per_simd"] = {"cycles_per_iter_per_wave": round(cyc.value, 2),
+                                           "cycles_per_iter_per_simd": round(cyc.value / wps, 2),
+                                           "launch_ms": round(ms.value, 3)}
+        out[str(mix)] = row
+        log(f"probe {mix}: {what}: {row}")
+    print(json.dumps({"metric": "register-only issue probes", "unit": "shader cycles per iteration",
+                      "iterations": iters, "probes": out}), flush=True)
 
 
 C2 = dict(levels=6, interp_degree=10, leaf_degree=7, fmm_theta=0.6)
@@ -892,7 +954,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "c5", "adjoint", "4dvar", "swe"],
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "c5", "adjoint", "4dvar", "swe", "probe"],
                     help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c2: the 6x256^2 Poisson solve, "
                          "swe: the shallow-water step at 6x192^2, "
                          "c4: the 48-member ensemble, "
@@ -936,6 +998,9 @@ def main():
         return
     if args.config == "c5":
         bench_c5(args, torch, sb)
+        return
+    if args.config == "probe":
+        bench_probe(args, torch)
         return
     W = sb.workloads
     cfgd = CONFIGS[args.config]

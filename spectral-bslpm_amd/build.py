@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbslpm.so")
+PROBE_LIB = os.path.join(HERE, "libsbslpm_probe.so")
 ARCH = os.environ.get("SBSLPM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["fmm_passes.hip", "fmm_interact.hip", "remap.hip", "direct.hip", "targets.hip", "swe.hip", "adjoint.hip", "remap_adjoint.hip", "output.hip", "irreg.hip", "swe_step.hip", "swe_adjoint.hip", "capi.hip"]
@@ -50,6 +51,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # register-only issue-rate probes (bench.py --config probe), a library of their own
+    src = os.path.join(CSRC, "probe", "issue_rates.hip")
+    ptmp = PROBE_LIB + ".tmp"
+    _run(["hipcc", "-x", "hip", src, "-shared", "-o", ptmp] + COMMON + DEVICE)
+    os.replace(ptmp, PROBE_LIB)
     return LIB
 
 

@@ -1367,6 +1367,28 @@ static int ensure_swe_adjoint(sbslpm_ctx* c) {
   return SBSLPM_OK;
 }
 
+// Anterpolation of a particle cotangent with unit weights, the first step of
+// every transposed FMM: adj_psip[f] = vbar[f] and the non-leaf rows of
+// adj_psix[f] (no target-record kernel reads the leaf rows).  The three
+// components of one array of 3 x npart (consecutive) go through one batched
+// launch sequence (gridDim.y = 3; adj_psip / adj_psix are laid out that way).
+static void adjoint_anterpolate(sbslpm_ctx* c, const double* const* vbar, int ncomp) {
+  const Tree& t = c->tree;
+  if (t.cfg.levels < 2) {
+    for (int f = 0; f < ncomp; ++f)
+      launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
+    return;
+  }
+  bool packed = ncomp == 3;
+  for (int f = 1; f < ncomp && packed; ++f) packed = vbar[f] == vbar[0] + f * t.npart;
+  if (packed) {
+    launch_upward_fused(c->dt, c->adj_ones, vbar[0], c->adj_psip[0], c->adj_psix[0], c->stream, 3);
+    return;
+  }
+  for (int f = 0; f < ncomp; ++f)
+    launch_upward_fused(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream, 1);
+}
+
 // Exact transpose of the shallow-water velocity evaluation (sbslpm_swe_velocity,
 // variant 1, sources displaced by dpart/dprox or not) for the particle-velocity
 // cotangent vbar: swa.qz, swa.qd = dL/d(area_s vor_s), dL/d(area_s div_s) at
@@ -1378,8 +1400,7 @@ static int swe_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* vo
   const Tree& t = c->tree;
   auto& a = c->swa;
   const bool pos = dpart != nullptr, fp32 = c->prm.precision != SBSLPM_FP64;
-  for (int f = 0; f < 3; ++f)
-    launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
+  adjoint_anterpolate(c, vbar, 3);
   const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
   launch_swe_adj_target_records(c->dt, fp32, vbar, psix, a.trec, c->stream);
   launch_upward_fused(c->dt, area_s, vor_s, c->qpart, c->qproxy, c->stream);
@@ -1406,7 +1427,7 @@ static int lap_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* ef
   auto& a = c->swa;
   auto& w = c->sws;
   const bool pos = dpart != nullptr, fp32 = c->prm.precision != SBSLPM_FP64;
-  launch_upward_pass(c->dt, c->adj_ones, lbar, c->adj_psip[0], c->adj_psix[0], c->stream);
+  adjoint_anterpolate(c, &lbar, 1);
   launch_lap_adj_target_records(c->dt, fp32, lbar, c->adj_psix[0], a.trec, c->stream);
   launch_upward_fused(c->dt, area_s, eff, w.qp, w.qx, c->stream);
   launch_source_records64(c->dt, w.qp, w.qx, dpart, dprox, true, a.srec, c->stream);
@@ -2059,8 +2080,7 @@ static int fmm_adjoint(sbslpm_ctx* c, const double* area, const double* vor_eff,
                 const double* const* dprox, const double* const* vbar) {
   const Tree& t = c->tree;
   const bool pos = dpart != nullptr;
-  for (int f = 0; f < 3; ++f)
-    launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
+  adjoint_anterpolate(c, vbar, 3);
   const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
   const bool fp32 = c->prm.precision != SBSLPM_FP64;  // mixed contexts take the FP32 pair sums
   if (fp32)

@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "lapkernel.hpp"
 
 namespace sbslpm {
 namespace {
@@ -37,13 +38,11 @@ __device__ inline double in_vgpr(double v) {
   __asm__ volatile("" : "+v"(v));
   return v;
 }
+__device__ inline float in_vgpr_f(float v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
 
-// Constants of the FP64 regularised-Laplacian pair kernel (k_lap64).
-struct LapPoly {
-  double a1, a0;      // A(t) = t (a1 t^2 + a0)
-  double b2, b1, b0;  // B(t) = (b2 t^2 + b1) t^2 + b0
-  double inv_eps, cut, t_sat;
-};
 
 // Member slices of a batched launch (gridDim.y members): charges and
 // displacements of the sources, outputs of the targets.
@@ -224,48 +223,41 @@ __global__ __launch_bounds__(256) void k_log64(DevTree t, const TargetTile* __re
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
 
-// Constants of k_lap64 with the Horner multipliers in vector registers.
-struct LapPolyV {
+// Horner multipliers of the FP64 regularised-Laplacian kernel in vector registers.
+struct LapV64 {
   double a1, b2, c24;
 };
 
-// K(t) of k_lap64 where t >= t_sat (erf factor exactly 1).
-__device__ inline double lap_pair(double dp, const LapPoly& kp, const LapPolyV& kv, const double* etab) {
+// K(t) of k_lap64 where t >= t_sat (erf factor exactly 1): A + E B.
+__device__ inline double lap_pair(double dp, const LapCoef64& kp, const LapV64& kv, const double* etab) {
   const double t2 = dp * dp;
   const double A = dp * fma(kv.a1, t2, kp.a0);
   const double B = fma(fma(kv.b2, t2, kp.b1), t2, kp.b0);
   return fma(exp_neg(fma(t2, kp.inv_eps, -kp.inv_eps), etab, kv.c24), B, A);
 }
-// K(t) for any t, zero at or below the cut.
-__device__ inline double lap_pair_slow(double dp, const LapPoly& kp, const LapPolyV& kv, const double* etab) {
+// K(t) for any t: A + f E B, zero at or below the cut.
+__device__ inline double lap_pair_slow(double dp, const LapCoef64& kp, const LapV64& kv, const double* etab) {
   const double t2 = dp * dp;
   const double A = dp * fma(kv.a1, t2, kp.a0);
   const double B = fma(fma(kv.b2, t2, kp.b1), t2, kp.b0);
   const double sv = dp * kp.inv_eps;
-  const double h = sv >= 6.0 ? 1.0 : (sv <= -6.0 ? 0.0 : 0.5 * (1.0 + erf(sv)));
-  const double K = fma(exp_neg(fma(t2, kp.inv_eps, -kp.inv_eps), etab, kv.c24) * h, B, A);
+  const double f = sv >= 6.0 ? 1.0 : (sv <= -6.0 ? 0.0 : 0.5 * (1.0 + erf(sv)));
+  const double K = fma(exp_neg(fma(t2, kp.inv_eps, -kp.inv_eps), etab, kv.c24) * f, B, A);
   return dp > kp.cut ? K : 0.0;
 }
 
 // Regularised-Laplacian pair kernel (lap_panel_interaction,
-// laplacian_impl.hpp:46-66, eps = 0.1): for t = x.y > 1 - 30 eps,
-//   K(t) = [c1 2t(-1 + 3 eps + t^2)
-//           + (1 + erf(t/eps)) e^{(t^2-1)/eps} c2 (2 eps^2 + 2t^2(t^2-1) + eps(7t^2-1))] c3,
-// c1 = e^{-1/eps} pi^{-3/2}, c2 = 1/(sqrt(eps) pi), c3 = eps^{-5/2}; phi += K q.
-// Sources are displaced and projected; the records hold q' = -q/(4 pi), so c3
-// arrives multiplied by -4 pi.  The kernel is evaluated as
-//   K(t) = A(t) + e^{(t^2-1)/eps} B(t) (1 + erf(t/eps))/2,
-//   A = t (a1 t^2 + a0),  B = (b2 t^2 + b1) t^2 + b0 (LapPoly: the constants
-//   folded on the host, B carrying the factor 2 of the saturated erf).
-// erf(s) rounds to exactly +-1 in FP64 for |s| >= 6 (erfc(6) = 2.2e-17 is below
-// half an ulp of 1), so for t >= 6 eps the erf factor is exactly 1; over 99% of
-// the pair work has t > 0.6, so a wave tests the whole source at once (a
-// ballot) and evaluates erf only when one of its pairs needs it.  The
-// exponential is exp_neg (fasttab.hpp: its argument lies in [-1/eps, 0]) on the
-// context's table, staged in LDS.
+// laplacian_impl.hpp:46-66, eps = 0.1): phi += K(x.y) q with K in the folded
+// form of lapkernel.hpp (A + E B where t >= 6 eps, erf only below).  Sources
+// are displaced and projected; the records hold q' = -q/(4 pi), so the
+// coefficients arrive multiplied by -4 pi / eps^{5/2}.  Over 99% of the pair
+// work has t > 0.6, so a wave tests the whole source at once (a ballot) and
+// evaluates erf only when one of its pairs needs it.  The exponential is
+// exp_neg (fasttab.hpp: its argument lies in [-1/eps, 0]) on the context's
+// table, staged in LDS.
 __global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                const SourceSet* __restrict__ sets,
-                                               const SrcRec64* __restrict__ rec, LapPoly kp,
+                                               const SrcRec64* __restrict__ rec, LapCoef64 kp,
                                                double* __restrict__ part, double* __restrict__ proxy, int assign) {
   __shared__ double etab[kExpNegTabSize];
   for (int i = threadIdx.x; i < kExpNegTabSize; i += blockDim.x) etab[i] = t.expnegtab[i];
@@ -279,7 +271,7 @@ __global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __re
   const double x = t.px[u], y = t.py[u], z = t.pz[u];
   // every pair passes the cut when 1 - 30 eps < -1 (eps = 0.1)
   const bool no_cut = kp.cut < -1.0;
-  const LapPolyV kv{in_vgpr(kp.a1), in_vgpr(kp.b2), in_vgpr(1.0 / 24.0)};
+  const LapV64 kv{in_vgpr(kp.a1), in_vgpr(kp.b2), in_vgpr(1.0 / 24.0)};
   double acc = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
@@ -309,16 +301,36 @@ __global__ __launch_bounds__(256) void k_lap64(DevTree t, const TargetTile* __re
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
 
-// Mixed-precision regularised-Laplacian pairs: h = |x - y|^2 / 2 = 1 - t in
-// FP32 from panel-centred coordinates, t^2 - 1 = -h (2 - h), the exponential on
-// the hardware base-2 exponential (its argument lies in [-10 log2 e, 0]), erf
-// only where |t| < 0.6 (elsewhere it is +-1 in FP32 too).  FP32 sums per
-// source set; c3 (times -4 pi, undoing the record's -1/(4 pi)) applied per set
-// in FP64.
+// Mixed-precision regularised-Laplacian pairs in the folded form of
+// lapkernel.hpp: the squared chord h2 = |x - y|^2 = 2(1 - t) in FP32 from
+// panel-centred coordinates, t^2 - 1 = h2 (h2/4 - 1) (no cancellation), the
+// exponential on the hardware base-2 exponential of u = (t^2 - 1) log2(e)/eps
+// (u in [-2/(eps ln 2), 0]), the polynomials in u; erf only when a wave-wide
+// ballot finds a pair with t < 0.6 (elsewhere it is +-1 in FP32 too).  FP32
+// sums per source set; c3 (times -4 pi, undoing the record's -1/(4 pi))
+// applied per set in FP64.
+__device__ inline float lap32_K(float h2, const LapCoef32& kp, float a1v, float b2v) {
+  const float tt = fmaf(h2, -0.5f, 1.0f);
+  const float u = h2 * fmaf(h2, 0.25f * kp.k, -kp.k);
+  const float A = tt * fmaf(a1v, u, kp.a0);
+  const float B = fmaf(fmaf(b2v, u, kp.b1), u, kp.b0);
+  return fmaf(__builtin_amdgcn_exp2f(u), B, A);
+}
+__device__ inline float lap32_K_slow(float h2, const LapCoef32& kp, float a1v, float b2v) {
+  const float tt = fmaf(h2, -0.5f, 1.0f);
+  const float u = h2 * fmaf(h2, 0.25f * kp.k, -kp.k);
+  const float A = tt * fmaf(a1v, u, kp.a0);
+  const float B = fmaf(fmaf(b2v, u, kp.b1), u, kp.b0);
+  const float sv = tt * kp.inv_eps;
+  const float f = sv >= 6.0f ? 1.0f : (sv <= -6.0f ? 0.0f : 0.5f * (1.0f + erff(sv)));
+  const float K = fmaf(__builtin_amdgcn_exp2f(u) * f, B, A);
+  return tt > kp.cut ? K : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_lap_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ sets,
-                                                   const SrcRec32* __restrict__ rec, float eps, float c1, float c2,
-                                                   double c3, double* __restrict__ part, double* __restrict__ proxy,
+                                                   const SrcRec32* __restrict__ rec, LapCoef32 kp, double c3,
+                                                   double* __restrict__ part, double* __restrict__ proxy,
                                                    int assign) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -328,25 +340,37 @@ __global__ __launch_bounds__(256) void k_lap_mixed(DevTree t, const TargetTile* 
   const DevPanel& PT = t.panels[T.panel];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
   const float ax = (float)(t.px[u] - PT.cx), ay = (float)(t.py[u] - PT.cy), az = (float)(t.pz[u] - PT.cz);
-  const float inv_eps = 1.0f / eps, k_exp = 1.44269504088896341f / eps, cut = 1.0f - 30.0f * eps;
+  const bool no_cut = kp.cut < -1.0f;
+  // t < t_sat  <=>  h2 > h_sat
+  const float h_sat = 2.0f * (1.0f - kp.t_sat);
+  const float a1v = in_vgpr_f(kp.a1), b2v = in_vgpr_f(kp.b2);
   double acc = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
     const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
     const SrcRec32* __restrict__ r = rec + S.first;
     float v = 0.f;
-#pragma unroll 2
-    for (int k = 0; k < S.count; ++k) {
-      const SrcRec32 s = r[k];
-      const float d0 = s.x - px, d1 = s.y - py, d2 = s.z - pz;
-      const float h = 0.5f * fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-      const float tt = 1.0f - h, tm1 = -h * (2.0f - h), t2 = 1.0f + tm1;
-      const float sv = tt * inv_eps;
-      const float ef = sv >= 6.0f ? 1.0f : (sv <= -6.0f ? -1.0f : erff(sv));
-      const float ex = __builtin_amdgcn_exp2f(tm1 * k_exp);
-      const float h1 = c1 * (2.0f * tt * (3.0f * eps + tm1));
-      const float h2 = (1.0f + ef) * ex * c2 * (2.0f * eps * eps + 2.0f * t2 * tm1 + eps * (7.0f * t2 - 1.0f));
-      v = (tt > cut) ? fmaf(h1 + h2, s.q, v) : v;
+    int k = 0;
+    for (; k + 1 < S.count; k += 2) {
+      const SrcRec32 s0 = r[k], s1 = r[k + 1];
+      const float d0 = s0.x - px, d1 = s0.y - py, d2 = s0.z - pz;
+      const float g0 = s1.x - px, g1 = s1.y - py, g2 = s1.z - pz;
+      const float h0 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+      const float h1 = fmaf(g0, g0, fmaf(g1, g1, g2 * g2));
+      float K0 = lap32_K(h0, kp, a1v, b2v), K1 = lap32_K(h1, kp, a1v, b2v);
+      if (!no_cut || (__builtin_amdgcn_ballot_w64(h0 > h_sat) | __builtin_amdgcn_ballot_w64(h1 > h_sat)) != 0) {
+        // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+        __asm__ volatile("");
+        K0 = lap32_K_slow(h0, kp, a1v, b2v);
+        K1 = lap32_K_slow(h1, kp, a1v, b2v);
+      }
+      v = fmaf(K0, s0.q, v);
+      v = fmaf(K1, s1.q, v);
+    }
+    if (k < S.count) {
+      const SrcRec32 s0 = r[k];
+      const float d0 = s0.x - px, d1 = s0.y - py, d2 = s0.z - pz;
+      v = fmaf(lap32_K_slow(fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), kp, a1v, b2v), s0.q, v);
     }
     acc += c3 * (double)v;
   }
@@ -890,18 +914,8 @@ void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int 
                                  bool assign) {
   if (ntiles == 0) return;
   const double eps = 0.1, pi = 3.14159265358979323846;
-  const double c1 = std::exp(-1.0 / eps) * std::pow(pi, -1.5), c2 = 1.0 / (std::sqrt(eps) * pi);
-  const double c3 = -4.0 * pi / std::pow(eps, 2.5);
-  LapPoly kp;
-  kp.a1 = 2.0 * c1 * c3;
-  kp.a0 = 2.0 * c1 * c3 * (3.0 * eps - 1.0);
-  kp.b2 = 4.0 * c2 * c3;
-  kp.b1 = 2.0 * c2 * c3 * (7.0 * eps - 2.0);
-  kp.b0 = 2.0 * c2 * c3 * (2.0 * eps * eps - eps);
-  kp.inv_eps = 1.0 / eps;
-  kp.cut = 1.0 - 30 * eps;
-  // t >= t_sat gives t / eps >= 6 after rounding (the margin covers the product's rounding)
-  kp.t_sat = 6.0 * eps * (1.0 + 1e-12);
+  // the records carry -q/(4 pi): undo it with the kernel's eps^{-5/2}
+  const LapCoef64 kp = lap_coef64(eps, -4.0 * pi / std::pow(eps, 2.5));
   hipLaunchKernelGGL(k_lap64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
                      ntiles, sets, rec, kp, out_part, out_proxy, assign ? 1 : 0);
 }
@@ -911,10 +925,9 @@ void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, 
                                      bool assign) {
   if (ntiles == 0) return;
   const double eps = 0.1, pi = 3.14159265358979323846;
-  const double c1 = std::exp(-1.0 / eps) * std::pow(pi, -1.5), c2 = 1.0 / (std::sqrt(eps) * pi);
   const double c3 = -4.0 * pi / std::pow(eps, 2.5);
   hipLaunchKernelGGL(k_lap_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, (float)eps, (float)c1, (float)c2, c3, out_part, out_proxy, assign ? 1 : 0);
+                     ntiles, sets, rec, lap_coef32(eps, 1.0), c3, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,

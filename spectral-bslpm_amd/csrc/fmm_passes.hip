@@ -300,6 +300,19 @@ void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
   if (count > 0) hipLaunchKernelGGL(k_copy, dim3(blocks_for(count, 256)), dim3(256), 0, s, dst, src, count);
 }
 
+// M2M for parent levels lfirst .. 0, a launch per level (level l starts at
+// panel 2 (4^l - 1) and holds 6 4^l panels).  A launch per level measured
+// faster than one workgroup per cube face walking the top levels with a
+// barrier between them: that kernel's waves wait on a chain of dependent
+// loads with too few waves per compute unit to hide them (178 us against
+// about 30 us for the four launches it replaced at 6 x 512^2).
+static void launch_m2m_levels(const DevTree& t, int lfirst, double* q_proxy, hipStream_t s, int nmem) {
+  for (int l = lfirst; l >= 0; --l) {
+    const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
+    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
+  }
+}
+
 // fmm_funcs.cpp:341-355.  Assigns q_part and every row of q_proxy (the
 // reference accumulates into caller-zeroed rows: bve_time_step.cpp:321).
 void launch_upward_pass(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
@@ -310,11 +323,7 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
   else
     hipLaunchKernelGGL(k_p2m, dim3(t.Pl), dim3(128), 0, s, t, (const double*)q_part, q_proxy);
   // Parents level L-2 .. 0; level l starts at 2*(4^l - 1).
-  for (int l = t.levels - 2; l >= 0; --l) {
-    int pbegin = 2 * ((1 << (2 * l)) - 1);
-    int count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
-  }
+  launch_m2m_levels(t, t.levels - 2, q_proxy, s, 1);
 }
 
 // Upward pass with base_pots, P2M and the first M2M level fused: q_part and
@@ -330,10 +339,7 @@ void launch_upward_fused(const DevTree& t, const double* area, const double* pot
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
     hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
   }
-  for (int l = t.levels - 3; l >= 0; --l) {
-    const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
-  }
+  launch_m2m_levels(t, t.levels - 3, q_proxy, s, nmem);
 }
 
 // fmm_funcs.cpp:357-392.  phi_proxy rows are accumulated top-down; sols are

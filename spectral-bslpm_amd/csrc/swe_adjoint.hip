@@ -23,6 +23,7 @@
 #include <cmath>
 
 #include "device.hpp"
+#include "lapkernel.hpp"
 
 namespace sbslpm {
 namespace {
@@ -228,41 +229,61 @@ __global__ void k_lap_tgt(DevTree t, const double* __restrict__ psi_part, const 
   rec[u] = LapTgt{t.px[u], t.py[u], t.pz[u], p};
 }
 
-// K(t) eps^{5/2} and its derivative (t > 1 - 30 eps, eps = 0.1):
-//   h1 = c1 2t(-1 + 3 eps + t^2),                  h1' = c1 (6t^2 - 2 + 6 eps)
-//   h2 = c2 A E Q,  A = 1 + erf(t/eps), E = e^{(t^2-1)/eps},
-//        Q = 2 eps^2 + 2t^2(t^2-1) + eps(7t^2-1),
-//   h2' = c2 (A' E Q + A (2t/eps) E Q + A E Q'),  A' = 2/(sqrt(pi) eps) e^{-t^2/eps^2},
-//        Q' = 8t^3 - 4t + 14 eps t.
-// For |t/eps| >= 6, erf is +-1 in FP64 and the A' term is below an ulp of the
-// others, as in the forward kernel.  Exponentials by exp_tab, as the forward.
-__device__ inline void lap_kernel_d(double tt, double c1, double c2, const double* etab, double& k, double& kd) {
-  const double eps = kLapEps, inv_eps = 1.0 / kLapEps;
+// Horner multipliers of the FP64 transposed Laplacian in vector registers (the
+// first step of each Horner scheme is then one three-operand fma).
+__device__ inline double in_vgpr(double v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+__device__ inline float in_vgpr_f(float v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
+// K(t) and K'(t) eps^{5/2} in the folded form of lapkernel.hpp.  fast: t >= 6 eps
+// (K = A + E B, K' = A' + E D); otherwise the erf factor f and the e^{-s^2}
+// term g, and zero at or below the cut.
+template <bool POS>
+__device__ inline void lap_kd(double tt, const LapCoef64& kp, double a1v, double b2v, double e1v, double d2v,
+                              double c24v, const double* etab, bool fast, double& k, double& kd) {
   const double t2 = tt * tt;
-  const double h1 = c1 * (2.0 * tt * (-1.0 + 3.0 * eps + t2));
-  const double h1d = c1 * (6.0 * t2 - 2.0 + 6.0 * eps);
-  const double s = tt * inv_eps;
-  const double A = s >= 6.0 ? 2.0 : (s <= -6.0 ? 0.0 : 1.0 + erf(s));
-  const double E = exp_tab((t2 - 1.0) * inv_eps, etab);
-  const double Q = 2.0 * eps * eps + 2.0 * t2 * (t2 - 1.0) + eps * (7.0 * t2 - 1.0);
-  const double Qd = 8.0 * t2 * tt - 4.0 * tt + 14.0 * eps * tt;
-  double h2d = A * E * (2.0 * tt * inv_eps * Q + Qd);
-  if (fabs(s) < 6.0) h2d += 2.0 / sqrt(kPiL) * inv_eps * exp_tab(-s * s, etab) * E * Q;
-  k = h1 + c2 * A * E * Q;
-  kd = h1d + c2 * h2d;
+  const double A = tt * fma(a1v, t2, kp.a0);
+  const double B = fma(fma(b2v, t2, kp.b1), t2, kp.b0);
+  const double E = exp_neg(fma(t2, kp.inv_eps, -kp.inv_eps), etab, c24v);
+  double Ad = 0, D = 0;
+  if (POS) {
+    Ad = fma(e1v, t2, kp.e0);
+    D = tt * fma(fma(d2v, t2, kp.d1), t2, kp.d0);
+  }
+  if (fast) {
+    k = fma(E, B, A);
+    if (POS) kd = fma(E, D, Ad);
+    return;
+  }
+  const double sv = tt * kp.inv_eps;
+  const bool mid = fabs(sv) < 6.0;
+  const double f = sv >= 6.0 ? 1.0 : (sv <= -6.0 ? 0.0 : 0.5 * (1.0 + erf(sv)));
+  const bool in = tt > kp.cut;
+  k = in ? fma(E * f, B, A) : 0.0;
+  if (POS) {
+    const double g = mid ? 2.0 / (sqrt(kPiL) * kLapEps) * exp(-sv * sv) : 0.0;
+    kd = in ? fma(E, fma(f, D, 0.5 * g * B), Ad) : 0.0;
+  }
 }
 
 // Records are the forward's (SrcRec64, q times -1/(4 pi)); c3 = -4 pi / eps^{5/2}
 // undoes the scaling.  q_bar = eps^{-5/2} sum psi K;  y_bar = q' c3 sum psi K' x.
+// A wave tests each source at once (a ballot) and takes the erf form only when
+// one of its pairs has t < 6 eps.
 template <bool POS>
 __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ targets, const LapTgt* __restrict__ trec,
-                                                   const SrcRec64* __restrict__ srec, double c1, double c2, double c3,
+                                                   const SrcRec64* __restrict__ srec, LapCoef64 kp, double c3,
                                                    CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
                                                    double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
                                                    int acc, int xcd) {
-  __shared__ double etab[kExpTabSize];
-  for (int i = threadIdx.x; i < kExpTabSize; i += blockDim.x) etab[i] = t.exptab[i];
+  __shared__ double etab[kExpNegTabSize];
+  for (int i = threadIdx.x; i < kExpNegTabSize; i += blockDim.x) etab[i] = t.expnegtab[i];
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -271,7 +292,9 @@ __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* 
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
   const SrcRec64 S = srec[u];
-  const double cut = 1.0 - 30 * kLapEps;
+  const bool no_cut = kp.cut < -1.0;
+  const double a1v = in_vgpr(kp.a1), b2v = in_vgpr(kp.b2), e1v = in_vgpr(kp.e1), d2v = in_vgpr(kp.d2);
+  const double c24v = in_vgpr(1.0 / 24.0);
   double aq = 0, p0 = 0, p1 = 0, p2 = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet G = targets[e];
@@ -279,16 +302,20 @@ __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* 
     for (int k = 0; k < G.count; ++k) {
       const LapTgt R = r[k];
       const double tt = R.x * S.x + R.y * S.y + R.z * S.z;
-      if (tt > cut) {
-        double kv, kd;
-        lap_kernel_d(tt, c1, c2, etab, kv, kd);
-        aq = fma(R.psi, kv, aq);
-        if (POS) {
-          const double w = R.psi * kd;
-          p0 = fma(w, R.x, p0);
-          p1 = fma(w, R.y, p1);
-          p2 = fma(w, R.z, p2);
-        }
+      double kv, kd = 0;
+      if (no_cut && __builtin_amdgcn_ballot_w64(tt < kp.t_sat) == 0) {
+        lap_kd<POS>(tt, kp, a1v, b2v, e1v, d2v, c24v, etab, true, kv, kd);
+      } else {
+        // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+        __asm__ volatile("");
+        lap_kd<POS>(tt, kp, a1v, b2v, e1v, d2v, c24v, etab, false, kv, kd);
+      }
+      aq = fma(R.psi, kv, aq);
+      if (POS) {
+        const double w = R.psi * kd;
+        p0 = fma(w, R.x, p0);
+        p1 = fma(w, R.y, p1);
+        p2 = fma(w, R.z, p2);
       }
     }
   }
@@ -340,14 +367,45 @@ __global__ void k_lap_tgt32(DevTree t, const double* __restrict__ psi_part, cons
                     0.f};
 }
 
-// FP32 pairs as k_lap_mixed: h = |x - y|^2/2 = 1 - t, t^2 - 1 = -h(2 - h),
-// the exponentials on the hardware base-2 exponential, erf and the A' term
-// only where |t| < 6 eps.
+// FP32 pairs as k_lap_mixed (lapkernel.hpp's polynomials in u, the base-2
+// exponent of E, from the squared chord h2), erf and the e^{-s^2} term only
+// when a ballot finds a pair with t < 6 eps.
+template <bool POS>
+__device__ inline void lap_kd32(float h2, const LapCoef32& kp, float a1v, float b2v, float e1v, float d2v, bool fast,
+                                float& k, float& kd) {
+  const float tt = fmaf(h2, -0.5f, 1.0f);
+  const float uu = h2 * fmaf(h2, 0.25f * kp.k, -kp.k);
+  const float A = tt * fmaf(a1v, uu, kp.a0);
+  const float B = fmaf(fmaf(b2v, uu, kp.b1), uu, kp.b0);
+  const float E = __builtin_amdgcn_exp2f(uu);
+  float Ad = 0.f, D = 0.f;
+  if (POS) {
+    Ad = fmaf(e1v, uu, kp.e0);
+    D = tt * fmaf(fmaf(d2v, uu, kp.d1), uu, kp.d0);
+  }
+  if (fast) {
+    k = fmaf(E, B, A);
+    if (POS) kd = fmaf(E, D, Ad);
+    return;
+  }
+  const float sv = tt * kp.inv_eps;
+  const bool mid = fabsf(sv) < 6.0f;
+  const float f = sv >= 6.0f ? 1.0f : (sv <= -6.0f ? 0.0f : 0.5f * (1.0f + erff(sv)));
+  const bool in = tt > kp.cut;
+  k = in ? fmaf(E * f, B, A) : 0.0f;
+  if (POS) {
+    const float g = mid ? (float)(2.0 / (1.7724538509055160273 * kLapEps)) *
+                              __builtin_amdgcn_exp2f(-sv * sv * 1.44269504088896341f)
+                        : 0.0f;
+    kd = in ? fmaf(E, fmaf(f, D, 0.5f * g * B), Ad) : 0.0f;
+  }
+}
+
 template <bool POS>
 __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                    const SourceSet* __restrict__ targets,
                                                    const LapTgt32* __restrict__ trec,
-                                                   const SrcRec64* __restrict__ srec, float c1, float c2, double c3,
+                                                   const SrcRec64* __restrict__ srec, LapCoef32 kp, double c3,
                                                    CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
                                                    double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
                                                    int acc, int xcd) {
@@ -360,8 +418,10 @@ __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* 
   const SrcRec64 S = srec[u];
   const DevPanel& PS = t.panels[T.panel];
   const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
-  const float eps = (float)kLapEps, inv_eps = 1.0f / eps, k_exp = 1.44269504088896341f / eps;
-  const float cut = 1.0f - 30.0f * eps, k_a = 1.12837916709551257f * inv_eps;  // 2/sqrt(pi)/eps
+  const bool no_cut = kp.cut < -1.0f;
+  // t < t_sat  <=>  h2 > h_sat
+  const float h_sat = 2.0f * (1.0f - kp.t_sat);
+  const float a1v = in_vgpr_f(kp.a1), b2v = in_vgpr_f(kp.b2), e1v = in_vgpr_f(kp.e1), d2v = in_vgpr_f(kp.d2);
   double aq = 0, p0 = 0, p1 = 0, p2 = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet G = targets[e];
@@ -371,25 +431,21 @@ __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* 
     for (int k = 0; k < G.count; ++k) {
       const LapTgt32 R = r[k];
       const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;
-      const float h = 0.5f * fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-      const float tt = 1.0f - h, tm1 = -h * (2.0f - h), t2 = 1.0f + tm1;
-      if (tt > cut) {
-        const float sv = tt * inv_eps;
-        const bool mid = fabsf(sv) < 6.0f;
-        const float A = sv >= 6.0f ? 2.0f : (sv <= -6.0f ? 0.0f : 1.0f + erff(sv));
-        const float E = __builtin_amdgcn_exp2f(tm1 * k_exp);
-        const float Q = 2.0f * eps * eps + 2.0f * t2 * tm1 + eps * (7.0f * t2 - 1.0f);
-        const float kv = c1 * (2.0f * tt * (3.0f * eps + tm1)) + c2 * A * E * Q;
-        fq = fmaf(R.psi, kv, fq);
-        if (POS) {
-          const float Qd = tt * (8.0f * t2 - 4.0f + 14.0f * eps);
-          float h2d = A * E * (2.0f * tt * inv_eps * Q + Qd);
-          if (mid) h2d += k_a * __builtin_amdgcn_exp2f(-sv * sv * 1.44269504088896341f) * E * Q;
-          const float w = R.psi * (c1 * (6.0f * t2 - 2.0f + 6.0f * eps) + c2 * h2d);
-          f0 = fmaf(w, R.x, f0);
-          f1 = fmaf(w, R.y, f1);
-          f2 = fmaf(w, R.z, f2);
-        }
+      const float h2 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+      float kv, kd = 0.f;
+      if (no_cut && __builtin_amdgcn_ballot_w64(h2 > h_sat) == 0) {
+        lap_kd32<POS>(h2, kp, a1v, b2v, e1v, d2v, true, kv, kd);
+      } else {
+        // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+        __asm__ volatile("");
+        lap_kd32<POS>(h2, kp, a1v, b2v, e1v, d2v, false, kv, kd);
+      }
+      fq = fmaf(R.psi, kv, fq);
+      if (POS) {
+        const float w = R.psi * kd;
+        f0 = fmaf(w, R.x, f0);
+        f1 = fmaf(w, R.y, f1);
+        f2 = fmaf(w, R.z, f2);
       }
     }
     aq += fq;
@@ -492,27 +548,27 @@ void launch_laplacian_adjoint(const DevTree& t, bool fp32, const TargetTile* til
                               PairShape shape) {
   if (ntiles == 0) return;
   const double eps = kLapEps;
-  const double c1 = std::exp(-1.0 / eps) * std::pow(kPiL, -1.5), c2 = 1.0 / (std::sqrt(eps) * kPiL);
   const double c3 = -4.0 * kPiL / std::pow(eps, 2.5);
   const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const int acc = accumulate ? 1 : 0;
   if (fp32) {
     const LapTgt32* r = static_cast<const LapTgt32*>(trec);
     if (dpart)
-      hipLaunchKernelGGL(k_lap_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)c1,
-                         (float)c2, c3, cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part), ptr(dbar_prox), acc,
-                         shape.xcd);
+      hipLaunchKernelGGL(k_lap_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
+                         lap_coef32(eps, 1.0), c3, cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part),
+                         ptr(dbar_prox), acc, shape.xcd);
     else
-      hipLaunchKernelGGL(k_lap_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)c1,
-                         (float)c2, c3, CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+      hipLaunchKernelGGL(k_lap_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
+                         lap_coef32(eps, 1.0), c3, CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
   } else {
     const LapTgt* r = static_cast<const LapTgt*>(trec);
     if (dpart)
-      hipLaunchKernelGGL(k_lap_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, c1, c2, c3,
-                         cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part), ptr(dbar_prox), acc, shape.xcd);
+      hipLaunchKernelGGL(k_lap_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
+                         lap_coef64(eps, 1.0), c3, cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part),
+                         ptr(dbar_prox), acc, shape.xcd);
     else
-      hipLaunchKernelGGL(k_lap_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, c1, c2, c3,
-                         CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+      hipLaunchKernelGGL(k_lap_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
+                         lap_coef64(eps, 1.0), c3, CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
   }
 }
 

@@ -18,3 +18,15 @@ def test_log_and_exp_accuracy(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
+
+
+def test_folded_laplacian_kernel(tmp_path):
+    """lapkernel.hpp's folded coefficients (FP64 in powers of t, FP32 in powers of the
+    exponent u) against the reference's K(t) in long double and its central-difference
+    derivative, in the saturated form (t >= 0.6) and the erf form over [-1, 1]."""
+    exe = str(tmp_path / "lapkernel_check")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(REPO, "spectral-bslpm_amd", "csrc"),
+                           os.path.join(REPO, "tests", "cpp", "lapkernel_check.cpp"), "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
