@@ -1022,11 +1022,12 @@ def main():
         if args.shape:
             w_, x_ = [int(v) for v in args.shape.split(",")]
             ctx.set_pair_shape(w_, x_)
-            tuned[name] = {"waves_per_workgroup": w_, "xcd_contiguous": bool(x_), "fixed": True}
+            tuned[name] = {"waves_per_workgroup": w_, "xcd_mapping": x_, "fixed": True}
         elif not args.no_tune:
             with torch.cuda.stream(stream):
                 table, shape = ctx.autotune(ga, gvor0)
-            tuned[name] = {"waves_per_workgroup": shape[0], "xcd_contiguous": bool(shape[1]),
+            # xcd mapping 0: round-robin over the XCDs, 1: equal contiguous ranges, 2: cost-balanced ranges
+            tuned[name] = {"waves_per_workgroup": shape[0], "xcd_mapping": shape[1],
                            "candidates_ms": [[int(w), int(c), round(ms, 4)] for w, c, ms in table]}
     torch.cuda.synchronize()
     log("launch shape", tuned)

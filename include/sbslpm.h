@@ -364,8 +364,10 @@ int sbslpm_bve_departure_to_target_adjoint(sbslpm_ctx* ctx, const double* dep_x,
 
 /* Autotuner of the Biot-Savart pair kernels' launch shape (no reference counterpart): times
  * the context's pair kernel (its precision) on its own interaction list with the records of
- * (area, vor), for every candidate of waves per workgroup (1, 2, 4) x XCD-contiguous tile
- * mapping (off, on), and keeps the fastest for all later evaluations, steps and ensembles
+ * (area, vor), for every candidate of waves per workgroup (1, 2, 4) x tile-to-XCD mapping
+ * (0: workgroups dealt round-robin over the XCDs by the hardware; 1: a contiguous range of
+ * equally many tiles per XCD; 2: a contiguous range of equal estimated work per XCD), and
+ * keeps the fastest for all later evaluations, steps and ensembles
  * and the step adjoint's transposed pair sums (captured graphs are dropped and recaptured).
  * report [ncand][3] (may be NULL) receives
  * waves, xcd, ms per launch; *ncand the number of candidates timed.  Results do not depend on
@@ -374,15 +376,15 @@ int sbslpm_autotune(sbslpm_ctx* ctx, const double* area, const double* vor, doub
 /* Autotuner of the multi-right-hand-side pair kernel of ensemble steps (no reference
  * counterpart; mixed-precision context with SBSLPM_FLAG_MULTI_RHS, 2 <= nmem <= 65535): times
  * it on the context's interaction list with the charges of vor [nmem][Pl][m2] for every
- * candidate of waves per workgroup (1, 2, 4) x XCD-contiguous tile mapping (off, on) x chunks
+ * candidate of waves per workgroup (1, 2, 4) x tile-to-XCD mapping (0, 1, 2 as above) x chunks
  * of four members per wave (1, 2), and keeps the fastest for later ensemble steps (a captured
  * ensemble step is dropped and recaptured).  report [ncand][4] (may be NULL) receives waves,
- * xcd, chunks per wave, ms per launch (12 rows).  Allocates the ensemble workspace for nmem
+ * xcd, chunks per wave, ms per launch (18 rows).  Allocates the ensemble workspace for nmem
  * members.  Results do not depend on the shape: every (target, member) is summed by one lane
  * in the same order. */
 int sbslpm_autotune_ensemble(sbslpm_ctx* ctx, int nmem, const double* area, const double* vor, double* report,
                              int* ncand);
-/* Sets the launch shape directly (waves 1, 2 or 4; xcd 0 or 1), e.g. a shape tuned earlier. */
+/* Sets the launch shape directly (waves 1, 2 or 4; xcd 0, 1 or 2), e.g. a shape tuned earlier. */
 int sbslpm_set_pair_shape(sbslpm_ctx* ctx, int waves, int xcd);
 int sbslpm_get_pair_shape(const sbslpm_ctx* ctx, int* waves, int* xcd);
 

@@ -552,7 +552,7 @@ class Context:
     def autotune(self, area, vor):
         """Times the pair kernel's launch shapes on this context's interaction list and keeps the
         fastest; returns [(waves, xcd, ms per launch), ...] and the chosen (waves, xcd)."""
-        rep = np.zeros((6, 3))
+        rep = np.zeros((9, 3))
         n = ctypes.c_int(0)
         self._check(self._lib.sbslpm_autotune(self.h, self._p(area, what="area"), self._p(vor, what="vor"), _dp(rep),
                                               ctypes.byref(n)), "autotune")
@@ -563,7 +563,7 @@ class Context:
         multi_rhs=True) over waves per workgroup x XCD mapping x chunks of four members per
         wave, with the charges of vors [members][Pl][m2], and keeps the fastest; returns
         [(waves, xcd, chunks, ms per launch), ...] and the chosen (waves, xcd, chunks)."""
-        rep = np.zeros((12, 4))
+        rep = np.zeros((18, 4))
         n = ctypes.c_int(0)
         nmem = int(vors.shape[0]) if vors.dim() >= 2 else 0
         if nmem < 1:

@@ -215,6 +215,21 @@ int upload(sbslpm_ctx* c, void* dst, const void* src, size_t bytes) {
   return SBSLPM_OK;
 }
 
+// A launch shape for the context's full target-major tile list: with the
+// cost-balanced XCD mapping (xcd 2) the grid is 8 x the largest XCD range in
+// workgroups of `waves` tiles, for the ranges of the matrix-core kernels
+// (mfma) or of the vector-unit ones.
+static PairShape full_list_shape(const Tree& t, int waves, int xcd, bool mfma) {
+  PairShape sh{waves, xcd, 0};
+  if (xcd == 2) {
+    const int* xs = t.xcd_start.data() + (mfma ? 0 : 9);
+    int most = 0;
+    for (int x = 0; x < 8; ++x) most = std::max(most, (xs[x + 1] - xs[x] + waves - 1) / waves);
+    sh.grid = 8 * most;
+  }
+  return sh;
+}
+
 // The logarithm and exponential tables of the FP64 pair kernels, built once
 // per process.
 struct HostTables {
@@ -329,7 +344,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   size_t o_vstage = A.take<double>(9 * NP), o_newvor = A.take<double>(NP), o_dep = A.take<double>(3 * NP);
   size_t o_vnext = A.take<double>(NP), o_trnext = A.take<double>(NP * ntr);
   size_t o_ltab = A.take<LogTabEntry>(kLogTabSize), o_etab = A.take<double>(kExpTabSize),
-         o_entab = A.take<double>(kExpNegTabSize);
+         o_entab = A.take<double>(kExpNegTabSize), o_xcd = A.take<int>(18);
   size_t total = A.take<char>(256);
   if (hipMalloc(&c->arena, total) != hipSuccess) {
     delete c;
@@ -362,6 +377,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
     rc |= upload(c, b + o_etab, ht.exp.data(), kExpTabSize * sizeof(double));
     rc |= upload(c, b + o_entab, ht.exp_neg.data(), kExpNegTabSize * sizeof(double));
   }
+  rc |= upload(c, b + o_xcd, t.xcd_start.data(), 18 * sizeof(int));
   if (rc != SBSLPM_OK || hipMemset(b + o_err, 0, 4 * sizeof(int)) != hipSuccess) {
     hipFree(c->arena);
     delete c;
@@ -379,6 +395,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   d.logtab = (const LogTabEntry*)(b + o_ltab);
   d.exptab = (const double*)(b + o_etab);
   d.expnegtab = (const double*)(b + o_entab);
+  d.xcd_start = (const int*)(b + o_xcd);
   d.err = (int*)(b + o_err);
   d.P = t.P; d.Pl = t.Pl; d.offset = t.offset; d.n = t.n; d.dp1 = t.dp1; d.m = t.m; d.m2 = t.m2;
   d.levels = t.cfg.levels; d.npart = t.npart; d.U = t.U;
@@ -475,6 +492,8 @@ int sbslpm_ctx_set_precision(sbslpm_ctx* c, int precision) {
       }
   }
   c->prm.precision = precision;
+  // the balanced XCD ranges (and so the grid) differ between matrix-core and vector-unit kernels
+  c->shape = full_list_shape(c->tree, c->shape.waves, c->shape.xcd, precision == SBSLPM_MIXED);
   return SBSLPM_OK;
 }
 
@@ -1660,9 +1679,9 @@ int sbslpm_autotune(sbslpm_ctx* c, const double* area, const double* vor, double
   PairShape best = c->shape;
   float best_ms = 1e30f;
   int k = 0;
-  for (int xcd = 0; xcd < 2; ++xcd)
+  for (int xcd = 0; xcd < 3; ++xcd)
     for (int wi = 0; wi < 3; ++wi) {
-      const PairShape sh{waves[wi], xcd};
+      const PairShape sh = full_list_shape(c->tree, waves[wi], xcd, prec == SBSLPM_MIXED);
       launch(sh);  // warm-up
       hipEventRecord(e0, c->stream);
       for (int r = 0; r < 3; ++r) launch(sh);
@@ -1703,12 +1722,12 @@ int sbslpm_autotune(sbslpm_ctx* c, const double* area, const double* vor, double
 }
 
 int sbslpm_set_pair_shape(sbslpm_ctx* c, int waves, int xcd) {
-  if (!c || !(waves == 1 || waves == 2 || waves == 4) || !(xcd == 0 || xcd == 1))
-    return fail(c, SBSLPM_E_ARG, "set_pair_shape: waves must be 1, 2 or 4 and xcd 0 or 1");
+  if (!c || !(waves == 1 || waves == 2 || waves == 4) || !(xcd == 0 || xcd == 1 || xcd == 2))
+    return fail(c, SBSLPM_E_ARG, "set_pair_shape: waves must be 1, 2 or 4 and xcd 0, 1 or 2");
   if (c->capturing) return fail(c, SBSLPM_E_ARG, "set_pair_shape: not allowed during graph capture");
   if (waves != c->shape.waves || xcd != c->shape.xcd) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->shape = PairShape{waves, xcd};
+    c->shape = full_list_shape(c->tree, waves, xcd, c->prm.precision == SBSLPM_MIXED);
     if (c->graph) {
       hipGraphExecDestroy(c->graph);
       c->graph = nullptr;
@@ -2360,9 +2379,9 @@ int sbslpm_autotune_ensemble(sbslpm_ctx* c, int nmem, const double* area, const 
   int k = 0;
   for (int cb = 1; cb <= 2; ++cb) {
     launch_source_columns_mrhs(c->dt, w.qpart, w.qproxy, w.recmf, nmem, cb, w.qcol, c->stream);
-    for (int xcd = 0; xcd < 2; ++xcd)
+    for (int xcd = 0; xcd < 3; ++xcd)
       for (int wi = 0; wi < 3; ++wi) {
-        const PairShape sh{waves[wi], xcd};
+        const PairShape sh = full_list_shape(c->tree, waves[wi], xcd, true);
         auto launch = [&] {
           launch_bve_interact_mrhs(c->dt, c->tiles, c->ntiles, c->sets, w.recmf, w.qcol, nmem, cb,
                                    c->prm.kernel_eps, w.phipart, w.phiproxy, c->stream, true, sh);

@@ -36,6 +36,7 @@ struct DevTree {
   const LogTabEntry* logtab;  // [kLogTabSize] table of log_tab (fasttab.hpp)
   const double* exptab;       // [kExpTabSize] table of exp_tab
   const double* expnegtab;    // [kExpNegTabSize] table of exp_neg
+  const int* xcd_start;       // [9] cost-balanced XCD ranges of the target-major tile list (or null)
   int* err;
   int P, Pl, offset, n, dp1, m, m2, levels;
   int64_t npart, U;
@@ -54,13 +55,20 @@ __device__ inline int64_t member_proxy(const DevTree& t) { return (int64_t)block
 constexpr int kWavesPerBlock = 4;
 
 // Launch shape of the Biot-Savart pair kernels, chosen by the context's
-// autotuner: waves (tiles) per workgroup and whether workgroups are remapped
-// so that each XCD takes a contiguous range of tiles.  The default is the
-// shape the autotuner picks for the 6 x 512^2 workload (four waves, tiles
-// dealt round-robin over the XCDs by the hardware).
+// autotuner: waves (tiles) per workgroup and how workgroups map to tiles:
+//   xcd 0: workgroup b takes tiles b*waves.. (the hardware deals workgroups
+//          round-robin over the XCDs, so neighbouring tiles land on different XCDs);
+//   xcd 1: each XCD takes a contiguous range of equally many tiles;
+//   xcd 2: each XCD takes a contiguous range of equal estimated work
+//          (DevTree::xcd_start), for the context's full target-major list
+//          only; `grid` is then the launch's workgroup count, 8 x the largest
+//          range in workgroups.
+// The default (four waves, mapping 0) needs no ranges; at 6 x 512^2 the
+// autotuner picks mapping 2.
 struct PairShape {
   int waves = kWavesPerBlock;
   int xcd = 0;
+  int grid = 0;
 };
 
 // Hardware dispatches workgroup b to XCD b % 8.  Give each XCD a contiguous
@@ -71,10 +79,27 @@ __device__ inline int xcd_remap(int b, int nb) {
   return x < rem ? x * (per + 1) + y : rem * (per + 1) + (x - rem) * per + y;
 }
 
-// Tile of this wave under a PairShape launch (blockDim.x = 64 * waves).
+// Tile of this wave under a PairShape launch (blockDim.x = 64 * waves) with
+// mapping 0 or 1 (a mapping-2 request on a list without ranges launches as 0).
 __device__ inline int shaped_tile(int wave, int xcd) {
-  const int b = xcd ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int b = xcd == 1 ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   return b * (int)(blockDim.x >> 6) + wave;
+}
+// The same for the context's target-major list, where mapping 2 takes the
+// y-th group of `waves` tiles of XCD x's range, x = b % 8, y = b / 8 (a
+// workgroup past its range gets no tile: INT_MAX).
+__device__ inline int shaped_tile(const int* __restrict__ xcd_start, int wave, int xcd) {
+  if (xcd == 2) {
+    const int x = (int)blockIdx.x & 7, y = (int)blockIdx.x >> 3;
+    const int lo = xcd_start[x], hi = xcd_start[x + 1];
+    const int tile = lo + y * (int)(blockDim.x >> 6) + wave;
+    return tile < hi ? tile : 0x7fffffff;
+  }
+  return shaped_tile(wave, xcd);
+}
+// Workgroups of a pair-kernel launch over ntiles tiles.
+inline int pair_grid(int ntiles, const PairShape& shape) {
+  return shape.xcd == 2 && shape.grid > 0 ? shape.grid : (ntiles + shape.waves - 1) / shape.waves;
 }
 
 // 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_bve64(DevTree t, const TargetTile* __re
   rec += (int64_t)blockIdx.y * t.U;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
+  const int tile = shaped_tile(t.xcd_start ? t.xcd_start + 9 : nullptr, wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void k_bve_mixed(DevTree t, const TargetTile* 
   rec += (int64_t)blockIdx.y * t.U;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
+  const int tile = shaped_tile(t.xcd_start ? t.xcd_start + 9 : nullptr, wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const DevPanel& PT = t.panels[T.panel];
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* _
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
-  const int tile = shaped_tile(wave, xcd);
+  const int tile = shaped_tile(t.xcd_start, wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int ntt = (T.count + 15) >> 4;
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(256) void k_bve_mrhs(DevTree t, const TargetTile* _
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4, comp = lane & 3;
-  const int tile = shaped_tile(wave, xcd);
+  const int tile = shaped_tile(t.xcd_start, wave, xcd);
   if (tile >= ntiles) return;
   const TargetTile T = tiles[tile];
   const int ntt = (T.count + 15) >> 4;
@@ -879,6 +879,13 @@ inline Ptr3 ptr3(double* const* a) {
 
 }  // namespace
 
+// Mapping 2 needs the tree's XCD ranges and the grid the context computed for
+// its full list; otherwise the launch uses mapping 0.
+static PairShape checked_shape(const DevTree& t, PairShape shape) {
+  if (shape.xcd == 2 && (!t.xcd_start || shape.grid <= 0)) shape.xcd = 0;
+  return shape;
+}
+
 void launch_source_records64(const DevTree& t, const double* q_part, const double* q_proxy,
                              const double* const* disp_part, const double* const* disp_proxy, bool project,
                              SrcRec64* rec, hipStream_t s, int nmem) {
@@ -897,7 +904,8 @@ void launch_bve_interact64(const DevTree& t, const TargetTile* tiles, int ntiles
                            const SrcRec64* rec, double eps, double* const* out_part, double* const* out_proxy,
                            hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve64, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t, tiles,
+  shape = checked_shape(t, shape);
+  hipLaunchKernelGGL(k_bve64, dim3(pair_grid(ntiles, shape), nmem), dim3(64 * shape.waves), 0, s, t, tiles,
                      ntiles, sets, rec, eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 
@@ -934,7 +942,8 @@ void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int nt
                                const SrcRec32* rec, double eps, double* const* out_part,
                                double* const* out_proxy, hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve_mixed, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t,
+  shape = checked_shape(t, shape);
+  hipLaunchKernelGGL(k_bve_mixed, dim3(pair_grid(ntiles, shape), nmem), dim3(64 * shape.waves), 0, s, t,
                      tiles, ntiles, sets, rec, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 
@@ -949,7 +958,8 @@ void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int nti
                               const SrcRecMfma* rec, double eps, double* const* out_part, double* const* out_proxy,
                               hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
-  hipLaunchKernelGGL(k_bve_mfma, dim3(blocks_for(ntiles, shape.waves), nmem), dim3(64 * shape.waves), 0, s, t,
+  shape = checked_shape(t, shape);
+  hipLaunchKernelGGL(k_bve_mfma, dim3(pair_grid(ntiles, shape), nmem), dim3(64 * shape.waves), 0, s, t,
                      tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy),
                      assign ? 1 : 0, shape.xcd);
 }
@@ -967,7 +977,8 @@ void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int nti
                               PairShape shape) {
   if (ntiles == 0) return;
   const int nchunk = mrhs_chunks(nmem, cb);
-  const dim3 grid(blocks_for(ntiles, shape.waves), nchunk / cb), block(64 * shape.waves);
+  shape = checked_shape(t, shape);
+  const dim3 grid(pair_grid(ntiles, shape), nchunk / cb), block(64 * shape.waves);
   if (cb == 2)
     hipLaunchKernelGGL(k_bve_mrhs<2>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
                        nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);

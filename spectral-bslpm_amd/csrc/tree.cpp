@@ -305,6 +305,38 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
   t.nonleaf_tiles = 0;
   while (t.nonleaf_tiles < (int)t.tiles.size() && !t.pn[t.tiles[t.nonleaf_tiles].panel].is_leaf) ++t.nonleaf_tiles;
 
+  // Eight contiguous ranges of the tile list with equal estimated work, one per
+  // XCD (the list is in breadth-first panel order, so a range of leaf tiles is a
+  // compact patch of the sphere and its sources stay in that XCD's L2), for two
+  // work models: xcd_start[0..8] for the matrix-core kernels (a tile's 16 x 16
+  // pair blocks over its source sets plus one block per set for the per-set
+  // flush) and xcd_start[9..17] for the vector-unit kernels, whose wave walks
+  // every source of a set whatever the tile's target count (the set's sources
+  // plus 16 for the per-set work).
+  {
+    const size_t nt = t.tiles.size();
+    t.xcd_start.assign(18, (int)nt);
+    for (int model = 0; model < 2; ++model) {
+      std::vector<double> pre(nt + 1, 0.0);
+      for (size_t k = 0; k < nt; ++k) {
+        const TargetTile& tl = t.tiles[k];
+        const double tb = (tl.count + 15) / 16;
+        double w = 0;
+        for (int e = tl.src_begin; e < tl.src_end; ++e)
+          w += model == 0 ? tb * ((t.srcsets[e].count + 15) / 16) + 1.0 : t.srcsets[e].count + 16.0;
+        pre[k + 1] = pre[k] + w;
+      }
+      int* xs = t.xcd_start.data() + 9 * model;
+      xs[0] = 0;
+      size_t k = 0;
+      for (int x = 1; x < 8; ++x) {
+        const double goal = pre[nt] * x / 8.0;
+        while (k < nt && pre[k] < goal) ++k;
+        xs[x] = (int)k;
+      }
+    }
+  }
+
   // Proxy rows that are not interaction targets: non-leaf panels without
   // tiles, and the leaf rows when there is no L2L level to write them.
   t.zero_rows.clear();

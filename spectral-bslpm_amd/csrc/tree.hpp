@@ -77,6 +77,9 @@ struct Tree {
   std::vector<SourceSet> srcsets;       // per-tile source lists (CSR via tiles)
   std::vector<int> set_start;           // [P+1]: source sets of target panel p are [set_start[p], set_start[p+1])
   int nonleaf_tiles = 0;                // tiles of non-leaf target panels (they come first)
+  std::vector<int> xcd_start;           // [2][9]: tiles [xcd_start[x], xcd_start[x+1]) of XCD x in the
+                                        // cost-balanced contiguous launch mapping (PairShape xcd 2),
+                                        // for matrix-core kernels, then for vector-unit kernels
   std::vector<double> px, py, pz;       // unified point coordinates [U] (targets, undisplaced sources)
   std::vector<double> m2m_hi, m2m_lo;   // [d+1][d+1]: parent basis a at child node k, A[a*(d+1)+k]
   std::vector<double> p2m;              // [d+1][m]: proxy basis a at particle node k

@@ -108,7 +108,7 @@ def test_ensemble_multi_rhs_autotune_keeps_results(sb, torch_gpu):
     gx, gy, gz, ga = [to_gpu(t, q) for q in (x, y, z, a)]
     vors = np.stack([sb.workloads.gaussian_vortices(x, y, z, a, seed=s) for s in range(6)])
     rows, best = tuned.autotune_ensemble(ga, to_gpu(t, vors))
-    assert len(rows) == 12 and all(r[3] > 0 for r in rows)
+    assert len(rows) == 18 and all(r[3] > 0 for r in rows)
     assert best[2] in (1, 2) and best[0] in (1, 2, 4)
     out = []
     for c in (plain, tuned):
