@@ -278,6 +278,46 @@ def bench_ensemble(args, torch, sb):
         c64.synchronize()
         acc[f"6x{g['side']}^2"] = float(f"{W.rel_l2(g['vors'][0].cpu().numpy(), v64.cpu().numpy()):.3e}")
         del c64
+    # bf16 M2L (opt-in): the same step with bf16 kernel-product operands for proxy-to-proxy sets;
+    # member 0 against the FP64 run and the members' own spread (each member's distance from
+    # member 0 after the same steps)
+    m2l = {}
+    if not args.no_graph:
+        for g in groups:
+            kwb = dict(g["kw"], bf16_m2l=True)
+            with torch.cuda.stream(stream):
+                cb_ = sb.Context(sb.RunConfig(**kwb))
+            cb_.set_pair_shape(*g["ctx"].pair_shape())
+            if not args.no_multi_rhs and not args.no_tune:
+                cb_.autotune_ensemble(g["xyz"][3], g["vors0"])
+            vb = g["vors0"].clone()
+            velb = [torch.zeros_like(vb) for _ in range(3)]
+            with torch.cuda.stream(stream):
+                cb_.bve_back_rk4_step_ensemble(*g["xyz"], vb, velb)  # capture
+            torch.cuda.synchronize()
+            vb.copy_(g["vors0"])
+            torch.cuda.synchronize()
+            b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            with torch.cuda.stream(stream):
+                b0.record(stream)
+                for _ in range(args.steps):
+                    cb_.bve_back_rk4_step_ensemble(*g["xyz"], vb, velb)
+                b1.record(stream)
+            torch.cuda.synchronize()
+            kw64 = dict(g["kw"], precision="fp64", use_graph=False)
+            c64 = sb.Context(sb.RunConfig(**kw64))
+            v64 = g["vors0"][0].clone()
+            vel64 = c64.zeros_part(3)
+            for _ in range(args.steps):
+                c64.bve_back_rk4_step(*g["xyz"], v64, vel64)
+            c64.synchronize()
+            vh = vb.cpu().numpy()
+            spread = float(np.median([W.rel_l2(vh[j], vh[0]) for j in range(1, nmem)]))
+            m2l[f"6x{g['side']}^2"] = {"ms_per_step": round(b0.elapsed_time(b1) / args.steps, 3),
+                                       "member0_vor_rel_l2_vs_fp64": float(f"{W.rel_l2(vh[0], v64.cpu().numpy()):.3e}"),
+                                       "member_spread_rel_l2": float(f"{spread:.3e}")}
+            del cb_, c64
+        log("bf16 M2L", m2l)
     # pair-kernel rooflines per resolution, from HIP events on an eager context (graph replays
     # carry no event timers): algorithmic flop = pairs per evaluation x members x 27
     roofs = {}
@@ -325,6 +365,7 @@ def bench_ensemble(args, torch, sb):
             "one_member_at_a_time_ms_per_step": round(1e3 * seq, 3),
             "speedup_vs_one_member_at_a_time": round(seq / (sec / args.steps), 3),
             "member0_vor_rel_l2_vs_fp64": acc,
+            "bf16_m2l": m2l,
             "pair_kernel_launch_shapes": {f"6x{g['side']}^2": list(g["ctx"].pair_shape()) for g in groups},
             "multi_rhs_launch_shapes": mshapes, "pair_kernel_rooflines": roofs}
     print(json.dumps(line), flush=True)

@@ -57,8 +57,12 @@ enum { SBSLPM_FP64 = 0, SBSLPM_MIXED = 1, SBSLPM_MIXED_VALU = 2 };
  * (sources and targets at the same positions for every member) computes each kernel matrix once
  * and applies it to all members' charges as one multi-right-hand-side product on the matrix
  * cores.  Members then agree with the single-member step to FP32 rounding (another summation
- * order) instead of bit for bit. */
-enum { SBSLPM_FLAG_GRAPH = 1, SBSLPM_FLAG_MULTI_RHS = 2 };
+ * order) instead of bit for bit.
+ * SBSLPM_FLAG_BF16_M2L: on a SBSLPM_MIXED context, the kernel product 1 - x.y of proxy-to-proxy
+ * (type 3, M2L) interactions takes bf16 operands (FP32 accumulation) in the matrix-core pair
+ * kernels; every other interaction stays FP32.  An opt-in measurement of reduced-precision
+ * M2L: its error is far above the FP32 form's (see the ensemble benchmark). */
+enum { SBSLPM_FLAG_GRAPH = 1, SBSLPM_FLAG_MULTI_RHS = 2, SBSLPM_FLAG_BF16_M2L = 4 };
 
 /* Run parameters: the hot-path subset of RunConfig (run_config.hpp:6-44). */
 typedef struct sbslpm_params {

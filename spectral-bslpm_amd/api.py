@@ -39,6 +39,7 @@ class RunConfig:
     tracer_count: int = 0
     use_graph: bool = False
     multi_rhs: bool = False          # ensembles: first stage as one multi-RHS product (mixed only)
+    bf16_m2l: bool = False           # mixed: bf16 kernel-product operands for proxy-to-proxy sets
     topography: str = "none"         # RunConfig::topo_type: none | cone | gaussian
     swe_rule: str = "reference"      # reference | leaf (shallow-water step indexing, see sbslpm.h)
     gravity: float = 0.0             # free-surface Laplacian coefficient (g/a^2 in metres); 0: the reference's 1
@@ -54,7 +55,7 @@ class RunConfig:
         p.rk4_combine = COMBINE[self.rk4_combine]
         p.precision = PRECISION[self.precision]
         p.tracer_count = self.tracer_count
-        p.flags = (1 if self.use_graph else 0) | (2 if self.multi_rhs else 0)
+        p.flags = (1 if self.use_graph else 0) | (2 if self.multi_rhs else 0) | (4 if self.bf16_m2l else 0)
         p.theta = self.fmm_theta
         p.kernel_eps = self.kernel_eps
         p.omega = self.omega

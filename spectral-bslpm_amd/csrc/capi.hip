@@ -397,6 +397,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   d.expnegtab = (const double*)(b + o_entab);
   d.xcd_start = (const int*)(b + o_xcd);
   d.err = (int*)(b + o_err);
+  d.bf16_m2l = (c->prm.flags & SBSLPM_FLAG_BF16_M2L) ? 1 : 0;
   d.P = t.P; d.Pl = t.Pl; d.offset = t.offset; d.n = t.n; d.dp1 = t.dp1; d.m = t.m; d.m2 = t.m2;
   d.levels = t.cfg.levels; d.npart = t.npart; d.U = t.U;
   d.m2p = t.m2p; d.np16 = t.np16; d.U16 = t.U16;

@@ -36,8 +36,9 @@ struct DevTree {
   const LogTabEntry* logtab;  // [kLogTabSize] table of log_tab (fasttab.hpp)
   const double* exptab;       // [kExpTabSize] table of exp_tab
   const double* expnegtab;    // [kExpNegTabSize] table of exp_neg
-  const int* xcd_start;       // [9] cost-balanced XCD ranges of the target-major tile list (or null)
+  const int* xcd_start;       // [2][9] cost-balanced XCD ranges of the target-major tile list (or null)
   int* err;
+  int bf16_m2l;               // matrix-core kernels take bf16 kernel-product operands for M2L sets
   int P, Pl, offset, n, dp1, m, m2, levels;
   int64_t npart, U;
   int m2p, np16;  // m2, n rounded up to 16

@@ -453,6 +453,28 @@ __global__ __launch_bounds__(256) void k_log_mixed(DevTree t, const TargetTile* 
 }
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+// The kernel products of NTT 16 x 16 tiles, D_ij = beta_i - p_j.b_i: FP32
+// (v_mfma_f32_16x16x4_f32, lane i + 16 k holding component k), or, for an M2L
+// set on a context with bf16 M2L, bf16 operands with FP32 accumulation
+// (v_mfma_f32_16x16x16_bf16: lane i + 16 k holds K slots 4k..4k+3, the
+// component in the first and zeros in the others, the same pairing of A and B).
+// One branch for all NTT products.
+template <int NTT>
+__device__ inline void kernel_products(float a1, const float (&b1)[NTT], bool lowp, v4f (&d)[NTT]) {
+  if (lowp) {
+    const v4bf a = v4bf{(__bf16)a1, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f};
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt)
+      d[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+          a, v4bf{(__bf16)b1[tt], (__bf16)0.f, (__bf16)0.f, (__bf16)0.f}, v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int tt = 0; tt < NTT; ++tt)
+      d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  }
+}
 
 // Records of the MFMA kernel (16-padded per source set; padding slots zero).
 __global__ void k_src_mfma(DevTree t, const double* __restrict__ q_part, const double* __restrict__ q_proxy,
@@ -524,7 +546,7 @@ __global__ void k_src_mfma(DevTree t, const double* __restrict__ q_part, const d
 // 4x4x1 form uses lanes 4 b .. 4 b + 3 (A row / B column = lane % 4, D row =
 // register).  The next source tile's operands are loaded before the current
 // tile's arithmetic (every index stays inside the set's padded records).
-template <int NTT>
+template <int NTT, bool M2L>
 __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, const SourceSet* __restrict__ sets,
                                       const float* __restrict__ rec, float eps, int lane, double (&w)[4][3]) {
   const int col = lane & 15, grp = lane >> 4;
@@ -572,9 +594,7 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
     const float na1 = rn[lane];
     const v4f nbq = *reinterpret_cast<const v4f*>(rn + off_b);
     v4f d[NTT];
-#pragma unroll
-    for (int tt = 0; tt < NTT; ++tt)
-      d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    kernel_products<NTT>(a1, b1, M2L && S.m2l, d);
 #pragma unroll
     for (int tt = 0; tt < NTT; ++tt)
 #pragma unroll
@@ -605,6 +625,7 @@ __device__ inline void bve_mfma_tiles(const DevTree& t, const TargetTile& T, con
   }
 }
 
+template <bool M2L>
 __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                   const SourceSet* __restrict__ sets,
                                                   const float* __restrict__ rec, float eps, Ptr3 part,
@@ -622,10 +643,10 @@ __global__ __launch_bounds__(256) void k_bve_mfma(DevTree t, const TargetTile* _
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) w[tt][0] = w[tt][1] = w[tt][2] = 0.0;
   switch (ntt) {
-    case 4: bve_mfma_tiles<4>(t, T, sets, rec, eps, lane, w); break;
-    case 3: bve_mfma_tiles<3>(t, T, sets, rec, eps, lane, w); break;
-    case 2: bve_mfma_tiles<2>(t, T, sets, rec, eps, lane, w); break;
-    default: bve_mfma_tiles<1>(t, T, sets, rec, eps, lane, w); break;
+    case 4: bve_mfma_tiles<4, M2L>(t, T, sets, rec, eps, lane, w); break;
+    case 3: bve_mfma_tiles<3, M2L>(t, T, sets, rec, eps, lane, w); break;
+    case 2: bve_mfma_tiles<2, M2L>(t, T, sets, rec, eps, lane, w); break;
+    default: bve_mfma_tiles<1, M2L>(t, T, sets, rec, eps, lane, w); break;
   }
   // sum the four source quarters (lanes col, col+16, col+32, col+48)
 #pragma unroll
@@ -713,7 +734,7 @@ __device__ inline double quad_bcast(double v) {
 // accumulates, per source set, w[tt][cb][r] += W_c - p_c W_q for target
 // tt*16 + 4 (l/16) + r, member 4 (chunk0 + cb) + (l%16)/4, component c = l % 4
 // (lanes with c = 3 carry W_q and are discarded).
-template <int NTT, int CB>
+template <int NTT, int CB, bool M2L>
 __device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, const SourceSet* __restrict__ sets,
                                       const float* __restrict__ geo, const float* __restrict__ qcol, int nchunk,
                                       int chunk0, float eps, int lane, double (&w)[4][CB][4]) {
@@ -771,9 +792,7 @@ __device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, con
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) nqv[cb] = *reinterpret_cast<const v4f*>(qptr(gn) + cb * 256);
     v4f d[NTT];
-#pragma unroll
-    for (int tt = 0; tt < NTT; ++tt)
-      d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[tt], v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    kernel_products<NTT>(a1, b1, M2L && S.m2l, d);
 #pragma unroll
     for (int tt = 0; tt < NTT; ++tt)
 #pragma unroll
@@ -812,7 +831,7 @@ __device__ inline void bve_mrhs_tiles(const DevTree& t, const TargetTile& T, con
 
 // One wave per target tile and group of CB member chunks (blockIdx.y).  Every
 // (target, member, component) is written by exactly one lane.
-template <int CB>
+template <int CB, bool M2L>
 __global__ __launch_bounds__(256) void k_bve_mrhs(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
                                                   const SourceSet* __restrict__ sets, const float* __restrict__ geo,
                                                   const float* __restrict__ qcol, int nchunk, int nmem, float eps,
@@ -833,10 +852,10 @@ __global__ __launch_bounds__(256) void k_bve_mrhs(DevTree t, const TargetTile* _
 #pragma unroll
       for (int r = 0; r < 4; ++r) w[tt][cb][r] = 0.0;
   switch (ntt) {
-    case 4: bve_mrhs_tiles<4, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
-    case 3: bve_mrhs_tiles<3, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
-    case 2: bve_mrhs_tiles<2, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
-    default: bve_mrhs_tiles<1, CB>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 4: bve_mrhs_tiles<4, CB, M2L>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 3: bve_mrhs_tiles<3, CB, M2L>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    case 2: bve_mrhs_tiles<2, CB, M2L>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
+    default: bve_mrhs_tiles<1, CB, M2L>(t, T, sets, geo, qcol, nchunk, chunk0, eps, lane, w); break;
   }
   // u = x cross w: the three components of one (target, member) sit in one quad
 #pragma unroll
@@ -878,6 +897,11 @@ inline Ptr3 ptr3(double* const* a) {
 }
 
 }  // namespace
+
+// The matrix-core kernels in their FP32 form or with bf16 M2L products.
+static auto mfma_kernel(const DevTree& t) { return t.bf16_m2l ? k_bve_mfma<true> : k_bve_mfma<false>; }
+template <int CB>
+static auto mrhs_kernel(const DevTree& t) { return t.bf16_m2l ? k_bve_mrhs<CB, true> : k_bve_mrhs<CB, false>; }
 
 // Mapping 2 needs the tree's XCD ranges and the grid the context computed for
 // its full list; otherwise the launch uses mapping 0.
@@ -959,7 +983,7 @@ void launch_bve_interact_mfma(const DevTree& t, const TargetTile* tiles, int nti
                               hipStream_t s, bool assign, int nmem, PairShape shape) {
   if (ntiles == 0) return;
   shape = checked_shape(t, shape);
-  hipLaunchKernelGGL(k_bve_mfma, dim3(pair_grid(ntiles, shape), nmem), dim3(64 * shape.waves), 0, s, t,
+  hipLaunchKernelGGL(mfma_kernel(t), dim3(pair_grid(ntiles, shape), nmem), dim3(64 * shape.waves), 0, s, t,
                      tiles, ntiles, sets, (const float*)rec, (float)eps, ptr3(out_part), ptr3(out_proxy),
                      assign ? 1 : 0, shape.xcd);
 }
@@ -980,10 +1004,10 @@ void launch_bve_interact_mrhs(const DevTree& t, const TargetTile* tiles, int nti
   shape = checked_shape(t, shape);
   const dim3 grid(pair_grid(ntiles, shape), nchunk / cb), block(64 * shape.waves);
   if (cb == 2)
-    hipLaunchKernelGGL(k_bve_mrhs<2>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
+    hipLaunchKernelGGL(mrhs_kernel<2>(t), grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
                        nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
   else
-    hipLaunchKernelGGL(k_bve_mrhs<1>, grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
+    hipLaunchKernelGGL(mrhs_kernel<1>(t), grid, block, 0, s, t, tiles, ntiles, sets, (const float*)geo, qcol, nchunk,
                        nmem, (float)eps, ptr3(out_part), ptr3(out_proxy), assign ? 1 : 0, shape.xcd);
 }
 

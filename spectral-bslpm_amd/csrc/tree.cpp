@@ -272,6 +272,7 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
     int s = t.ilist[3 * order[k] + 1], tg = t.ilist[3 * order[k]];
     SourceSet ss{};
     ss.panel = s;
+    ss.m2l = (!t.pn[s].is_leaf && !t.pn[tg].is_leaf) ? 1 : 0;
     ss.ox = (float)(t.dev[tg].cx - t.dev[s].cx);
     ss.oy = (float)(t.dev[tg].cy - t.dev[s].cy);
     ss.oz = (float)(t.dev[tg].cz - t.dev[s].cz);

@@ -55,7 +55,7 @@ struct TargetTile {
 struct SourceSet {
   int first, count, panel, first16;
   float ox, oy, oz;
-  int pad;
+  int m2l;  // 1 when both the source and the target panel are non-leaf (proxy to proxy, type 3)
 };
 
 struct Tree {
