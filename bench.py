@@ -1139,8 +1139,9 @@ def main():
     #   unit (2) = 18.
     executed = {"bve_fp64": 19, "bve_mixed": 18}
     limiter = {"bve_fp64": "FP64 vector FMA and reciprocal issue (78.6 TF/s vector peak)",
-               "bve_mixed": "MFMA products and v_rcp_f32 at quarter rate, serialized on each SIMD "
-                            "(no vector/matrix co-issue measured: SQ_VALU_MFMA_COEXEC_CYCLES = 0)"}
+               "bve_mixed": "MFMA products (64 cycles per 256 pairs) and vector work with four v_rcp_f32 "
+                            "(about 90 cycles), serialized on each SIMD (register-only probe: no vector/matrix "
+                            "co-issue; profiles/bve_mixed_issue_rates.md)"}
     for ctx, name, peak in ((ctx64, "bve_fp64", PEAK_FP64), (ctxmx, "bve_mixed", PEAK_FP32)):
         ctx.enable_timing(True)
         with torch.cuda.stream(stream):

@@ -1431,10 +1431,10 @@ static int swe_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* vo
   if (pos && !acc) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, a.dbx, c->stream);
   launch_swe_adjoint(c->dt, fp32, c->adj_tiles, c->adj_ntiles, c->adj_targets, a.trec, c->recswe, c->prm.kernel_eps,
                      dpart, dprox, a.qzp, a.qzx, a.qdp, a.qdx, a.dbp, a.dbx, acc, c->stream, c->shape);
-  const double* qzp[1] = {a.qzp};
-  const double* qdp[1] = {a.qdp};
-  launch_downward_pass(c->dt, 1, &a.qzx, qzp, &a.qz, c->stream, true);
-  launch_downward_pass(c->dt, 1, &a.qdx, qdp, &a.qd, c->stream, true);
+  // both charge cotangents in one downward pass (two components per launch)
+  const double* qp2[2] = {a.qzp, a.qdp};
+  double* q2[2] = {a.qz, a.qd};
+  launch_downward_pass(c->dt, 2, qx2, qp2, q2, c->stream, true);
   return launched(c);
 }
 
