@@ -430,6 +430,66 @@ int sbslpm_irreg_swe_velocity(sbslpm_irreg* h, const double* area, const double*
 int sbslpm_irreg_bve_velocity(sbslpm_irreg* h, const double* area, const double* vor, double* vel_x, double* vel_y,
                               double* vel_z);
 
+/* ---- Lat-lon shallow-water step (swe_time_step_ll.cpp) ---------------------------------------
+ * Fields are device arrays [lat_count][lon_count], row-major (index i*lon_count + j), on the
+ * cell-centred grid of latlon_point (general_utils_impl.hpp:252-276): row i at latitude
+ * -90 + grid_spacing (i + 1/2), column j at longitude grid_spacing (j + 1/2), in degrees; x, y, z
+ * are its points.  lat_count and lon_count are at least 3.  Velocities are on the unit sphere per
+ * second and heights in Earth radii, as in the reference.  Two rules where the reference is
+ * undefined: the polar filter's Fourier coefficients start at zero (polar_filtering leaves all
+ * but the first uninitialised, :628-631, :644-645), and field_averaging's longitude neighbours
+ * are the periodic j+1 and j-1 (it reads them before setting them, :322-323).  Grid indices are
+ * kept inside the grid where the reference's would leave it.  Outputs are assigned and may not
+ * alias the inputs (the stencils read neighbouring points). */
+/* swe_tendency_computation_ll (swe_time_step_ll.cpp:365-387): vor_tend_ll, height_tend_ll,
+ * div_tend_ll (:14-231), then div_tend values below 2e-12 in magnitude set to zero (filter_vals,
+ * general_utils_impl.hpp:278-293). */
+int sbslpm_ll_tendencies(sbslpm_ctx* ctx, int lat_count, int lon_count, double grid_spacing, double omega,
+                         const double* x, const double* y, const double* z, const double* vors, const double* divs,
+                         const double* height, const double* topo, const double* vel_x, const double* vel_y,
+                         const double* vel_z, double* vor_tend, double* div_tend, double* height_tend);
+/* polar_filtering (swe_time_step_ll.cpp:600-664), in place: on each row with |lat| > 60 degrees
+ * the values (clamped as the reference does: above 1e-11 and below -1e-11 both to +1e-11) are
+ * replaced by their series in cos(k lat), sin(k lat), k < min(floor(90 - |lat|) + 1, 29). */
+int sbslpm_ll_polar_filter(sbslpm_ctx* ctx, int lat_count, int lon_count, double grid_spacing, const double* x,
+                           const double* y, const double* z, double* field);
+/* swe_compute_arrival_values_ll (swe_time_step_ll.cpp:389-598): quadratic semi-Lagrangian
+ * arrival values of vorticity and height (1-D along the row where |v| < 1e-16, else 3 x 3 at the
+ * departure point x - delta_t vel, with the ring means of the first and last rows at the poles);
+ * a_div = div + delta_t div_tend. */
+int sbslpm_ll_arrival_values(sbslpm_ctx* ctx, int lat_count, int lon_count, double grid_spacing, double delta_t,
+                             const double* x, const double* y, const double* z, const double* vors,
+                             const double* divs, const double* height, const double* vel_x, const double* vel_y,
+                             const double* vel_z, const double* vor_tend, const double* div_tend, double* a_vor,
+                             double* a_div, double* a_height);
+/* field_averaging (swe_time_step_ll.cpp:270-363): 5-point average with weight epsilon on rows
+ * with |lat| > 60 degrees, times cos^2(lat); other rows copied. */
+int sbslpm_ll_field_averaging(sbslpm_ctx* ctx, int lat_count, int lon_count, const double* x, const double* y,
+                              const double* z, const double* field, double* new_field, double epsilon);
+/* laplacian_diffusion (swe_time_step_ll.cpp:233-268): interior rows plus
+ * dt * 0.89/6371000 * the 5-point spherical Laplacian; first and last rows copied. */
+int sbslpm_ll_laplacian_diffusion(sbslpm_ctx* ctx, int lat_count, int lon_count, double grid_spacing, double dt,
+                                  const double* x, const double* y, const double* z, const double* field,
+                                  double* new_field);
+/* swe_vel_tend_computation (swe_time_step_ll.cpp:687-735) on an irregular tree built from the
+ * grid's points (x, y, z here are their device copies, in the same order): the FMM velocity
+ * (sbslpm_irreg_swe_velocity), the tendencies, and the polar filter of the height tendency. */
+int sbslpm_irreg_swe_vel_tend_ll(sbslpm_irreg* h, int lat_count, int lon_count, double grid_spacing, double omega,
+                                 const double* x, const double* y, const double* z, const double* area,
+                                 const double* vors, const double* divs, const double* height, const double* topo,
+                                 double* vel_x, double* vel_y, double* vel_z, double* vor_tend, double* div_tend,
+                                 double* height_tend);
+/* swe_back_rk4_step_ll (swe_time_step_ll.cpp:737-774): one semi-Lagrangian step in place --
+ * velocity and tendencies, arrival values, field_averaging (epsilon 0.16) and laplacian_diffusion
+ * of the divergence, polar filter of the divergence.  vel receives the start state's velocity.
+ * The height tendency and its filter (:728), which the step does not use, are skipped.  time is
+ * unused, as in the reference. */
+int sbslpm_irreg_swe_back_rk4_step_ll(sbslpm_irreg* h, int lat_count, int lon_count, double grid_spacing,
+                                      double delta_t, double omega, const double* x, const double* y,
+                                      const double* z, const double* area, double* vors, double* divs,
+                                      double* height, const double* topo, double* vel_x, double* vel_y,
+                                      double* vel_z, double time);
+
 /* BLI interpolation of nfields (1..8) leaf fields [Pl][m2] at n arbitrary points (x, y, z on
  * the unit sphere): out[q][i] = field q interpolated at point i on the leaf that contains it.
  * The functor of latlon_interp_host / interp_to_latlon_host (interp_funcs.hpp:22-77) for any

@@ -124,6 +124,14 @@ def lib():
         L.orc_set_swe_physics.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 5 + [ctypes.c_int]
         L.orc_set_swe_physics.restype = None
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        _i, _d = ctypes.c_int, ctypes.c_double
+        L.orc_ll_tendencies.argtypes = [_i, _i, _d, _d] + [_dp] * 13
+        L.orc_ll_tendencies.restype = ctypes.c_int
+        L.orc_ll_polar_filter.argtypes = [_i, _i, _d] + [_dp] * 4
+        L.orc_ll_arrival_values.argtypes = [_i, _i, _d, _d] + [_dp] * 14
+        L.orc_ll_arrival_values.restype = ctypes.c_int
+        L.orc_ll_field_averaging.argtypes = [_i, _i] + [_dp] * 5 + [_d]
+        L.orc_ll_laplacian_diffusion.argtypes = [_i, _i, _d, _d] + [_dp] * 5
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -518,6 +526,54 @@ def xyz_vel_to_uv_vel(x, y, z, vx, vy, vz):
     return u, v
 
 
+def _grid_arrays(nlat, nlon, *arrs):
+    out = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in arrs]
+    for a in out:
+        if a.size != nlat * nlon:
+            raise ValueError(f"lat-lon field of {a.size} values, expected {nlat} x {nlon}")
+    return out
+
+
+def ll_tendencies(nlat, nlon, dx, omega, x, y, z, vor, div, h, topo, vel):
+    """swe_tendency_computation_ll: (vor_tend, div_tend, height_tend), each [nlat*nlon]."""
+    a = _grid_arrays(nlat, nlon, x, y, z, vor, div, h, topo, *vel)
+    out = [np.zeros(nlat * nlon) for _ in range(3)]
+    if lib().orc_ll_tendencies(nlat, nlon, float(dx), float(omega), *[_p(t) for t in a + out]):
+        raise ValueError("ll_tendencies: grid under 3 x 3")
+    return out
+
+
+def ll_polar_filter(nlat, nlon, dx, x, y, z, f):
+    """polar_filtering with zero-initialised coefficients; returns the filtered copy."""
+    a = _grid_arrays(nlat, nlon, x, y, z)
+    g = np.array(f, dtype=np.float64).ravel().copy()
+    lib().orc_ll_polar_filter(nlat, nlon, float(dx), *[_p(t) for t in a], _p(g))
+    return g
+
+
+def ll_arrival_values(nlat, nlon, dx, dt, x, y, z, vor, div, h, vel, vor_t, div_t):
+    """swe_compute_arrival_values_ll: (a_vor, a_div, a_height)."""
+    a = _grid_arrays(nlat, nlon, x, y, z, vor, div, h, *vel, vor_t, div_t)
+    out = [np.zeros(nlat * nlon) for _ in range(3)]
+    if lib().orc_ll_arrival_values(nlat, nlon, float(dx), float(dt), *[_p(t) for t in a + out]):
+        raise ValueError("ll_arrival_values: grid under 3 x 3")
+    return out
+
+
+def ll_field_averaging(nlat, nlon, x, y, z, f, epsilon):
+    a = _grid_arrays(nlat, nlon, x, y, z, f)
+    out = np.zeros(nlat * nlon)
+    lib().orc_ll_field_averaging(nlat, nlon, *[_p(t) for t in a], _p(out), float(epsilon))
+    return out
+
+
+def ll_laplacian_diffusion(nlat, nlon, dx, dt, x, y, z, f):
+    a = _grid_arrays(nlat, nlon, x, y, z, f)
+    out = np.zeros(nlat * nlon)
+    lib().orc_ll_laplacian_diffusion(nlat, nlon, float(dx), float(dt), *[_p(t) for t in a], _p(out))
+    return out
+
+
 def dedupe(x, y, z):
     """cubed_sphere_2d_to_1d, the reference's O(N^2) scan: (map, counts, x1d, y1d, z1d)."""
     x, y, z = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x, y, z)]
@@ -603,3 +659,22 @@ class OracleIrreg:
         tp, vel = self.swe_vel_interactions(qv, qd, vors, divs, area, eps)
         _, vel = self.downward_pass_3(tp, vel)
         return vel
+
+    def swe_vel_tend_ll(self, nlat, nlon, dx, omega, area, vors, divs, height, topo, eps):
+        """swe_vel_tend_computation (swe_time_step_ll.cpp:687-735): the FMM velocity, the
+        tendencies and the polar filter of the height tendency.  Returns (vel, vor_t,
+        div_t, h_t)."""
+        vel = self.swe_velocity(area, vors, divs, eps)
+        vt, dt_, ht = ll_tendencies(nlat, nlon, dx, omega, self.x, self.y, self.z, vors, divs, height, topo, vel)
+        ht = ll_polar_filter(nlat, nlon, dx, self.x, self.y, self.z, ht)
+        return vel, vt, dt_, ht
+
+    def swe_back_rk4_step_ll(self, nlat, nlon, dx, dt, omega, area, vors, divs, height, topo, eps):
+        """swe_back_rk4_step_ll (swe_time_step_ll.cpp:737-774): one semi-Lagrangian step
+        on the lat-lon grid.  Returns (vors, divs, height, vel) -- vel of the start state."""
+        vel, vt, dvt, _ = self.swe_vel_tend_ll(nlat, nlon, dx, omega, area, vors, divs, height, topo, eps)
+        av, ad, ah = ll_arrival_values(nlat, nlon, dx, dt, self.x, self.y, self.z, vors, divs, height, vel, vt, dvt)
+        d2 = ll_field_averaging(nlat, nlon, self.x, self.y, self.z, ad, 0.16)
+        d3 = ll_laplacian_diffusion(nlat, nlon, dx, dt, self.x, self.y, self.z, d2)
+        d3 = ll_polar_filter(nlat, nlon, dx, self.x, self.y, self.z, d3)
+        return av, d3, ah, vel

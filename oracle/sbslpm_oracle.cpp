@@ -2009,6 +2009,363 @@ void orc_set_swe_physics(void* h, double gravity, double height, double width, d
   c.mtn[3] = lon;
 }
 
+// ---- Lat-lon shallow-water step (swe_time_step_ll.cpp) ----------------------
+// Fields are [lat][lon] row-major on the cell-centred grid of latlon_point
+// (general_utils_impl.hpp:252-276): lat_i = -90 + dx (i + 1/2), lon_j = dx (j + 1/2)
+// degrees.  Sequential loops in the reference's operation order.
+
+struct LlGrid {
+  int nlat, nlon;
+  double dx, dxr;  // spacing in degrees and radians (dx*pi/180 as written)
+  int64_t at(int i, int j) const { return int64_t(i) * nlon + j; }
+  int jp(int j) const { return j + 1 == nlon ? 0 : j + 1; }
+  int jm(int j) const { return j == 0 ? nlon - 1 : j - 1; }
+};
+
+static LlGrid ll_grid(int nlat, int nlon, double dx) { return LlGrid{nlat, nlon, dx, dx * kPi / 180.0}; }
+
+// xyz_to_latlon (general_utils_impl.hpp:232-236).
+static void ll_latlon(double x, double y, double z, double& lat, double& lon) {
+  lat = kPi / 2.0 - std::atan2(std::sqrt(x * x + y * y), z);
+  lon = std::atan2(y, x);
+}
+
+static double ll_lat(const double* x, const double* y, const double* z, int64_t k) {
+  double lat, lon;
+  ll_latlon(x[k], y[k], z[k], lat, lon);
+  return lat;
+}
+
+// xyz_vel_to_uv_vel's per-point body (general_utils_impl.hpp:214-226).
+static void ll_uv(double x, double y, double z, double vx, double vy, double vz, double& u, double& v) {
+  if (std::abs(z) < 1 - 1e-16) {
+    double r = std::sqrt(x * x + y * y);
+    u = (-y * vx + x * vy) / r;
+    v = -((x * vx + y * vy) * z - (x * x + y * y) * vz) / r;
+  } else {
+    u = 0;
+    v = 0;
+  }
+}
+
+// swe_tendency_computation_ll (swe_time_step_ll.cpp:365-387): u, v and |vel|^2
+// (xyz_vel_to_uv_vel, vel_dot_prod), then vor_tend_ll (:14-32), height_tend_ll
+// (:34-95), div_tend_ll (:97-231) and filter_vals(div, 2e-12)
+// (general_utils_impl.hpp:278-293).  Returns 1 for a grid under 3 x 3.
+int orc_ll_tendencies(int nlat, int nlon, double dx_deg, double omega, const double* x, const double* y,
+                      const double* z, const double* vor, const double* div, const double* h, const double* topo,
+                      const double* vx, const double* vy, const double* vz, double* vor_t, double* div_t,
+                      double* h_t) {
+  if (nlat < 3 || nlon < 3) return 1;
+  const LlGrid g = ll_grid(nlat, nlon, dx_deg);
+  const int64_t n = int64_t(nlat) * nlon;
+  std::vector<double> u(n), v(n), dp(n);
+  for (int64_t k = 0; k < n; k++) {
+    ll_uv(x[k], y[k], z[k], vx[k], vy[k], vz[k], u[k], v[k]);
+    dp[k] = vx[k] * vx[k] + vy[k] * vy[k] + vz[k] * vz[k];
+  }
+  const double grav = 9.81 / 6371000.0;
+  auto bern = [&](int64_t k) { return grav * (h[k] + topo[k]) + 0.5 * dp[k]; };
+  for (int i = 0; i < nlat; i++) {
+    for (int j = 0; j < nlon; j++) {
+      const int64_t k = g.at(i, j), kp = g.at(i, g.jp(j)), km = g.at(i, g.jm(j));
+      const double lat = ll_lat(x, y, z, k);
+      // vorticity: -(zeta + 2 Omega z) delta - 2 Omega w
+      const double avor = vor[k] + 2.0 * omega * z[k];
+      vor_t[k] = -avor * div[k] - 2.0 * omega * vz[k];
+      // height: -(d_lon(h u) / (2 dx in degrees) + d_lat(h v cos)) / cos
+      const double d1 = (h[kp] * u[kp] - h[km] * u[km]) / (2.0 * dx_deg);
+      double d2;
+      if (i == 0) {
+        const int64_t kn = g.at(1, j);
+        d2 = (3.0 * h[k] * v[k] * std::cos(lat) + h[kn] * v[kn] * std::cos(lat + g.dxr)) / (3.0 * g.dxr);
+      } else if (i == nlat - 1) {
+        const int64_t ks = g.at(i - 1, j);
+        d2 = (-h[ks] * v[ks] * std::cos(lat - g.dxr) - 3.0 * h[k] * v[k] * std::cos(lat)) / (3.0 * g.dxr);
+      } else {
+        const int64_t kn = g.at(i + 1, j), ks = g.at(i - 1, j);
+        d2 = (h[kn] * v[kn] * std::cos(lat + g.dxr) - h[ks] * v[ks] * std::cos(lat - g.dxr)) / (2.0 * g.dxr);
+      }
+      h_t[k] = -(d1 + d2) / std::cos(lat);
+      // divergence
+      const double e1 = (vor[kp] + 2.0 * omega * z[kp]) * v[kp];
+      const double e2 = (vor[km] + 2.0 * omega * z[km]) * v[km];
+      const double b0 = bern(k);
+      double curl = (e1 - e2) / (2.0 * g.dxr);
+      double lap = (bern(kp) - 2.0 * b0 + bern(km)) / (g.dxr * g.dxr * std::cos(lat) * std::cos(lat));
+      if (i == 0 || i == nlat - 1) {
+        // one-sided rows: the pole term enters with weight 0 (the stencil's -1/2 or +1/2
+        // node is the pole); note the product 2 Omega z u, not (zeta + 2 Omega z) u
+        const int64_t kq = g.at(i == 0 ? 1 : i - 1, j);
+        const double lq = i == 0 ? lat + g.dxr : lat - g.dxr;
+        const double w1 = (vor[k] + 2.0 * omega * z[k] * u[k]) * std::cos(lat);
+        const double w2 = (vor[kq] + 2.0 * omega * z[kq] * u[kq]) * std::cos(lq);
+        if (i == 0)
+          curl -= (3.0 * w1 + w2) / (3.0 * g.dxr);
+        else
+          curl -= (-3.0 * w1 - w2) / (3.0 * g.dxr);
+        lap += curl / std::cos(lat);
+      } else {
+        const int64_t kn = g.at(i + 1, j), ks = g.at(i - 1, j);
+        const double w1 = (vor[kn] + 2.0 * omega * z[kn]) * u[kn] * std::cos(lat + g.dxr);
+        const double w2 = (vor[ks] + 2.0 * omega * z[ks]) * u[ks] * std::cos(lat - g.dxr);
+        curl -= (w1 - w2) / (2.0 * g.dxr);
+        const double bn = bern(kn), bs = bern(ks);
+        lap += -std::tan(lat) * (bn - bs) / (2.0 * g.dxr) + (bn - 2.0 * b0 + bs) / (g.dxr * g.dxr);
+      }
+      double dt_k = curl / std::cos(lat) - lap;
+      if (std::abs(lat) * 180.0 / M_PI > 60) dt_k *= std::cos(lat) * std::cos(lat);
+      if (std::abs(dt_k) < 2e-12) dt_k = 0.0;
+      div_t[k] = dt_k;
+    }
+  }
+  return 0;
+}
+
+// polar_filtering (swe_time_step_ll.cpp:600-664) on rows with |lat| > 60 degrees,
+// in place.  DEVIATION: every Fourier coefficient starts at zero; the reference
+// zeroes only coefficient 0 inside its loop (:628-631) and accumulates into the
+// others uninitialised (:644-645), which is undefined.  The clamp keeps the
+// reference's rule: values above 1e-11 and below -1e-11 both become +1e-11 (:640-643).
+void orc_ll_polar_filter(int nlat, int nlon, double dx_deg, const double* x, const double* y, const double* z,
+                         double* f) {
+  const LlGrid g = ll_grid(nlat, nlon, dx_deg);
+  for (int i = 0; i < nlat; i++) {
+    const double lat0 = ll_lat(x, y, z, g.at(i, 0));
+    if (!(std::abs(lat0) > kPi / 3.0)) continue;
+    int spec = int(std::floor(90.0 - std::abs(lat0 * 180.0 / kPi)));
+    spec = std::min(spec + 1, 29);
+    double cs[30] = {}, sn[30] = {};
+    for (int j = 0; j < nlon; j++) {
+      const int64_t k = g.at(i, j);
+      const double lat = ll_lat(x, y, z, k);
+      if (f[k] > 1e-11)
+        f[k] = 1e-11;
+      else if (f[k] < -1e-11)
+        f[k] = 1e-11;
+      cs[0] += f[k];
+      for (int m = 1; m < spec; m++) {
+        cs[m] += f[k] * std::cos(m * lat) * g.dxr;
+        sn[m] += f[k] * std::sin(m * lat) * g.dxr;
+      }
+    }
+    cs[0] /= nlon;
+    for (int m = 1; m < spec; m++) {
+      cs[m] /= nlon;
+      sn[m] /= nlon;
+    }
+    for (int m = 0; m < spec; m++) {
+      if (cs[m] != cs[m]) cs[m] = 0;
+      if (sn[m] != sn[m]) sn[m] = 0;
+    }
+    for (int j = 0; j < nlon; j++) {
+      const int64_t k = g.at(i, j);
+      const double lat = ll_lat(x, y, z, k);
+      double val = cs[0];
+      for (int m = 1; m < spec; m++) val += std::cos(m * lat) * cs[m] + std::sin(m * lat) * sn[m];
+      f[k] = val;
+    }
+  }
+}
+
+// swe_compute_arrival_values_ll (swe_time_step_ll.cpp:389-598): quadratic
+// semi-Lagrangian arrival values on the lat-lon grid.  |v| < 1e-16: 1-D
+// interpolation along the row at lon - u dt (:446-474); else the departure point
+// x - dt vel projected to the sphere, 3 x 3 tensor Lagrange interpolation, with
+// the ring mean of row 0 (row nlat-1) standing at the south (north) pole node
+// (:476-593).  Indices are kept inside the grid where the reference's would leave
+// it (a departure latitude of exactly 90 degrees, a longitude index rounded up to
+// lon_count, or a row shift of more than 360 degrees); elsewhere they are the
+// reference's.
+int orc_ll_arrival_values(int nlat, int nlon, double dx, double dt, const double* x, const double* y,
+                          const double* z, const double* vor, const double* div, const double* h, const double* vx,
+                          const double* vy, const double* vz, const double* vor_t, const double* div_t,
+                          double* a_vor, double* a_div, double* a_h) {
+  if (nlat < 3 || nlon < 3) return 1;
+  const LlGrid g = ll_grid(nlat, nlon, dx);
+  const double vs = 1.0, ds = 1.0, hs = 1.0;  // vor_scale, div_scale, h_scale (:430-432)
+  auto vval = [&](int i, int j) { int64_t k = g.at(i, j); return vor[k] + vs * dt * vor_t[k]; };
+  auto hval = [&](int i, int j) { int64_t k = g.at(i, j); return h[k] - dt * hs * h[k] * div[k]; };
+  auto ring = [&](int i, double out[2]) {  // pole means (:532-547, :565-580)
+    double sv = 0.0, sh = 0.0;
+    for (int m = 0; m < nlon; m++) {
+      const int64_t k = g.at(i, m);
+      sv += vor[k] + vs * dt * vor_t[k];
+    }
+    for (int m = 0; m < nlon; m++) {
+      const int64_t k = g.at(i, m);
+      sh += h[k] - hs * h[k] * div[k] * dt;
+    }
+    out[0] = sv / nlon;
+    out[1] = sh / nlon;
+  };
+  double south[2], north[2];
+  ring(0, south);
+  ring(nlat - 1, north);
+  auto lon_coeffs = [&](double lon_d, int li, double c[3]) {
+    c[0] = (lon_d - dx * (li + 0.5)) * (lon_d - dx * (li + 1.5)) / (2.0 * dx * dx);
+    c[1] = -(lon_d - dx * (li - 0.5)) * (lon_d - dx * (li + 1.5)) / (dx * dx);
+    c[2] = (lon_d - dx * (li - 0.5)) * (lon_d - dx * (li + 0.5)) / (2.0 * dx * dx);
+  };
+  // a grid index from a floored coordinate, kept inside [0, hi) (NaN -> 0)
+  auto index = [](double fl, int hi) {
+    if (!(fl >= 0.0)) return 0;
+    if (fl > double(hi - 1)) return hi - 1;
+    return int(fl);
+  };
+  auto lon_index = [&](double& lon_d) {
+    if (lon_d < 0.0) lon_d += 360.0;
+    return index(std::floor(lon_d / dx), nlon);
+  };
+  for (int i = 0; i < nlat; i++) {
+    for (int j = 0; j < nlon; j++) {
+      const int64_t k = g.at(i, j);
+      double u, v;
+      ll_uv(x[k], y[k], z[k], vx[k], vy[k], vz[k], u, v);
+      if (std::abs(v) < 1e-16) {
+        double lat, lon;
+        ll_latlon(x[k], y[k], z[k], lat, lon);
+        lon -= u * dt;
+        double lon_d = 180.0 / kPi * lon;
+        lon_d = std::fmod(lon_d + 360.0, 360.0);
+        const int li = lon_index(lon_d);
+        const int lm = ((li - 1) + nlon) % nlon, lp = (li + 1) % nlon;
+        double c[3];
+        lon_coeffs(lon_d, li, c);
+        const double vv[3] = {vval(i, lm), vval(i, li), vval(i, lp)};
+        const double hh[3] = {hval(i, lm), hval(i, li), hval(i, lp)};
+        double av = 0, ah = 0;
+        for (int l = 0; l < 3; l++) {
+          av += vv[l] * c[l];
+          ah += hh[l] * c[l];
+        }
+        a_vor[k] = av;
+        a_h[k] = ah;
+      } else {
+        double px = x[k] - dt * vx[k], py = y[k] - dt * vy[k], pz = z[k] - dt * vz[k];
+        const double nrm = std::sqrt(px * px + py * py + pz * pz);
+        px /= nrm;
+        py /= nrm;
+        pz /= nrm;
+        double lat, lon;
+        ll_latlon(px, py, pz, lat, lon);
+        const double lat_d = 180.0 / kPi * lat;
+        double lon_d = 180.0 / kPi * lon;
+        lon_d = std::fmod(lon_d + 360.0, 360.0);
+        int ti = index(std::floor((lat_d + 90.0) / dx), nlat);
+        const int li = lon_index(lon_d);
+        double a[3];
+        bool done = false;
+        if (ti == 0) {
+          if (lat_d > -90.0 + 0.5 * dx) {
+            ti += 1;
+          } else {  // nodes: south pole, row 0, row 1
+            a[0] = (lat_d - (-90.0 + dx * (ti + 0.5))) * (lat_d - (-90.0 + dx * (ti + 1.5))) / (dx * dx * 0.5 * 1.5);
+            a[1] = -(lat_d - (-90.0)) * (lat_d - (-90.0 + dx * (ti + 1.5))) / (dx * dx * 0.5);
+            a[2] = (lat_d - (-90.0)) * (lat_d - (-90.0 + dx * (ti + 0.5))) / (dx * dx * 1.5);
+            done = true;
+          }
+        }
+        if (ti == nlat - 1) {
+          if (lat_d < 90.0 - 0.5 * dx) {
+            ti -= 1;
+          } else {  // nodes: row nlat-2, row nlat-1, north pole
+            a[0] = (lat_d - (-90.0 + dx * (ti + 0.5))) * (lat_d - 90.0) / (dx * dx * 1.5);
+            a[1] = -(lat_d - (-90.0 + dx * (ti - 0.5))) * (lat_d - 90.0) / (dx * dx * 0.5);
+            a[2] = (lat_d - (-90.0 + dx * (ti - 0.5))) * (lat_d - (-90.0 + dx * (ti + 0.5))) / (dx * dx * 0.5 * 1.5);
+            done = true;
+          }
+        }
+        if (!done) {
+          a[0] = (lat_d - (-90.0 + dx * (ti + 0.5))) * (lat_d - (-90.0 + dx * (ti + 1.5))) / (2.0 * dx * dx);
+          a[1] = -(lat_d - (-90.0 + dx * (ti - 0.5))) * (lat_d - (-90.0 + dx * (ti + 1.5))) / (dx * dx);
+          a[2] = (lat_d - (-90.0 + dx * (ti - 0.5))) * (lat_d - (-90.0 + dx * (ti + 0.5))) / (2.0 * dx * dx);
+        }
+        double c[3];
+        lon_coeffs(lon_d, li, c);
+        const int lm = ((li - 1) + nlon) % nlon, lp = (li + 1) % nlon;
+        const int cols[3] = {lm, li, lp};
+        double vv[3][3], hh[3][3];
+        for (int l = 0; l < 3; l++) {
+          vv[0][l] = ti == 0 ? south[0] : vval(ti - 1, cols[l]);
+          hh[0][l] = ti == 0 ? south[1] : hval(ti - 1, cols[l]);
+          vv[1][l] = vval(ti, cols[l]);
+          hh[1][l] = hval(ti, cols[l]);
+          vv[2][l] = ti == nlat - 1 ? north[0] : vval(ti + 1, cols[l]);
+          hh[2][l] = ti == nlat - 1 ? north[1] : hval(ti + 1, cols[l]);
+        }
+        double av = 0, ah = 0;
+        for (int r = 0; r < 3; r++)
+          for (int l = 0; l < 3; l++) {
+            av += vv[r][l] * a[r] * c[l];
+            ah += hh[r][l] * a[r] * c[l];
+          }
+        a_vor[k] = av;
+        a_h[k] = ah;
+      }
+      a_div[k] = div[k] + ds * dt * div_t[k];
+    }
+  }
+  return 0;
+}
+
+// field_averaging (swe_time_step_ll.cpp:270-363): rows with |lat| > 60 degrees
+// (lat of column 0) get the 5-point average with weight epsilon, times cos^2(lat);
+// the other rows the same form with weight 0 (:314).  DEVIATION: the reference
+// reads field(i, jp) and field(i, jm) before it sets jp and jm (:322-323, :346-347),
+// which is undefined; here they are the periodic neighbours j+1 and j-1 that it
+// sets just after.  The first and last rows use one latitude neighbour, weight 3.
+void orc_ll_field_averaging(int nlat, int nlon, const double* x, const double* y, const double* z,
+                            const double* f, double* out, double epsilon) {
+  const LlGrid g = ll_grid(nlat, nlon, 1.0);
+  for (int i = 0; i < nlat; i++) {
+    const double lat = ll_lat(x, y, z, g.at(i, 0));
+    const bool polar = std::abs(lat) > kPi / 3.0;
+    const double e = polar ? epsilon : 0.00;
+    for (int j = 0; j < nlon; j++) {
+      const int64_t k = g.at(i, j);
+      const double f1 = e * f[g.at(i, g.jp(j))], f2 = e * f[g.at(i, g.jm(j))];
+      double f3, f4, coeff;
+      if (i > 0 && i < nlat - 1) {
+        f3 = e * f[g.at(i + 1, j)];
+        f4 = e * f[g.at(i - 1, j)];
+        coeff = 4;
+      } else {
+        f3 = e * f[g.at(i == 0 ? 1 : i - 1, j)];
+        f4 = 0;
+        coeff = 3;
+      }
+      double val = (1.0 - coeff * e) * f[k] + f1 + f2 + f3 + f4;
+      if (polar) val *= std::cos(lat) * std::cos(lat);
+      out[k] = val;
+    }
+  }
+}
+
+// laplacian_diffusion (swe_time_step_ll.cpp:233-268): interior rows get
+// dt * 0.89/6371000 * (the 5-point spherical Laplacian); the first and last rows
+// are copied.
+void orc_ll_laplacian_diffusion(int nlat, int nlon, double dx_deg, double dt, const double* x, const double* y,
+                                const double* z, const double* f, double* out) {
+  const LlGrid g = ll_grid(nlat, nlon, dx_deg);
+  const double coeff = 0.89 / 6371000.0;
+  for (int i = 0; i < nlat; i++) {
+    for (int j = 0; j < nlon; j++) {
+      const int64_t k = g.at(i, j);
+      out[k] = f[k];
+      if (i > 0 && i < nlat - 1) {
+        const double c = f[k], e = f[g.at(i, g.jp(j))], w = f[g.at(i, g.jm(j))];
+        const double nn = f[g.at(i + 1, j)], ss = f[g.at(i - 1, j)];
+        const double lat = ll_lat(x, y, z, k);
+        const double l1 = 1.0 / (g.dxr * g.dxr * std::cos(lat) * std::cos(lat)) * (e - 2.0 * c + w);
+        const double l2 = std::tan(lat) * (nn - ss) / (2.0 * g.dxr);
+        const double l3 = (nn - 2.0 * c + ss) / (g.dxr * g.dxr);
+        out[k] += (l1 - l2 + l3) * dt * coeff;
+      }
+    }
+  }
+}
+
 void orc_set_threads(int n) { omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 

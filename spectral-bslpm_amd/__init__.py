@@ -3,4 +3,4 @@ sums and the backward semi-Lagrangian BVE step, as hand-written HIP kernels
 behind a C ABI (include/sbslpm.h, libsbslpm.so)."""
 from ._lib import SbslpmError, exported_symbols, load  # noqa: F401
 from .api import Context, HostIrregTree, HostTree, IrregContext, RunConfig, cubed_sphere_midpoints  # noqa: F401
-from . import output, workloads  # noqa: F401
+from . import latlon, output, workloads  # noqa: F401

@@ -114,6 +114,13 @@ SIGNATURES = {
     "sbslpm_irreg_swe_velocity": (_I, [_V] * 7),
     "sbslpm_irreg_bve_velocity": (_I, [_V] * 6),
     "sbslpm_xyz_vel_to_uv_vel": (_I, [_V, ctypes.c_int64] + [_V] * 8),
+    "sbslpm_ll_tendencies": (_I, [_V, _I, _I, _D, _D] + [_V] * 13),
+    "sbslpm_ll_polar_filter": (_I, [_V, _I, _I, _D] + [_V] * 4),
+    "sbslpm_ll_arrival_values": (_I, [_V, _I, _I, _D, _D] + [_V] * 14),
+    "sbslpm_ll_field_averaging": (_I, [_V, _I, _I] + [_V] * 5 + [_D]),
+    "sbslpm_ll_laplacian_diffusion": (_I, [_V, _I, _I, _D, _D] + [_V] * 5),
+    "sbslpm_irreg_swe_vel_tend_ll": (_I, [_V, _I, _I, _D, _D] + [_V] * 14),
+    "sbslpm_irreg_swe_back_rk4_step_ll": (_I, [_V, _I, _I, _D, _D, _D] + [_V] * 13 + [_D]),
     "sbslpm_swe_find_departure_points": (_I, [_V, _V, _V, _V, _V, ctypes.c_double, _V, _V, _V]),
     "sbslpm_swe_departure_to_target": (_I, [_V] * 8 + [_I] + [_V] * 4),
     "sbslpm_poisson_solve_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V]),

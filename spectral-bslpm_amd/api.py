@@ -633,6 +633,53 @@ class Context:
                     "xyz_vel_to_uv_vel")
         return u, v
 
+    # ---- lat-lon shallow-water grid operators (swe_time_step_ll.cpp); grid = (lat_count,
+    # lon_count, grid_spacing in degrees), fields [lat_count * lon_count] float64 on the GPU
+    def _ll(self, grid, *ts, what):
+        nlat, nlon, dx = int(grid[0]), int(grid[1]), float(grid[2])
+        return (nlat, nlon, dx), [_ptr(t, nlat * nlon, what) for t in ts]
+
+    def _ll_new(self, like, k):
+        return [self.torch.empty_like(like) for _ in range(k)]
+
+    def ll_tendencies(self, grid, x, y, z, vors, divs, height, topo, vel, omega=None, out=None):
+        """swe_tendency_computation_ll: (vor_tend, div_tend, height_tend)."""
+        from .latlon import OMEGA
+        out = self._ll_new(vors, 3) if out is None else _seq(out, 3, "out")
+        (nlat, nlon, dx), a = self._ll(grid, x, y, z, vors, divs, height, topo, *_seq(vel, 3, "vel"), *out,
+                                       what="ll_tendencies array")
+        self._check(self._lib.sbslpm_ll_tendencies(self.h, nlat, nlon, dx, float(OMEGA if omega is None else omega),
+                                                   *a), "ll_tendencies")
+        return out
+
+    def ll_polar_filter(self, grid, x, y, z, field):
+        """polar_filtering in place (zero-initialised coefficients)."""
+        (nlat, nlon, dx), a = self._ll(grid, x, y, z, field, what="ll_polar_filter array")
+        self._check(self._lib.sbslpm_ll_polar_filter(self.h, nlat, nlon, dx, *a), "ll_polar_filter")
+        return field
+
+    def ll_arrival_values(self, grid, delta_t, x, y, z, vors, divs, height, vel, vor_tend, div_tend, out=None):
+        """swe_compute_arrival_values_ll: (a_vor, a_div, a_height)."""
+        out = self._ll_new(vors, 3) if out is None else _seq(out, 3, "out")
+        (nlat, nlon, dx), a = self._ll(grid, x, y, z, vors, divs, height, *_seq(vel, 3, "vel"), vor_tend, div_tend,
+                                       *out, what="ll_arrival_values array")
+        self._check(self._lib.sbslpm_ll_arrival_values(self.h, nlat, nlon, dx, float(delta_t), *a),
+                    "ll_arrival_values")
+        return out
+
+    def ll_field_averaging(self, grid, x, y, z, field, epsilon, out=None):
+        out = self.torch.empty_like(field) if out is None else out
+        (nlat, nlon, _), a = self._ll(grid, x, y, z, field, out, what="ll_field_averaging array")
+        self._check(self._lib.sbslpm_ll_field_averaging(self.h, nlat, nlon, *a, float(epsilon)), "ll_field_averaging")
+        return out
+
+    def ll_laplacian_diffusion(self, grid, dt, x, y, z, field, out=None):
+        out = self.torch.empty_like(field) if out is None else out
+        (nlat, nlon, dx), a = self._ll(grid, x, y, z, field, out, what="ll_laplacian_diffusion array")
+        self._check(self._lib.sbslpm_ll_laplacian_diffusion(self.h, nlat, nlon, dx, float(dt), *a),
+                    "ll_laplacian_diffusion")
+        return out
+
     def bve_back_rk4_step_adjoint(self, x, y, z, area, vor, new_vor_bar, vor_bar=None):
         """Gradient of one bve_back_rk4_step with respect to the vorticity before it: vor_bar =
         (d new_vor / d vor)^T new_vor_bar at the state vor (not modified; the step's forward
@@ -858,6 +905,31 @@ class IrregContext:
         self.ctx._check(self._lib.sbslpm_irreg_swe_velocity(
             self.h, self._pt(area, "area"), self._pt(vors, "vors"), self._pt(divs, "divs"),
             *[self._pt(t, "vel") for t in _seq(vel, 3, "vel")]), "irreg_swe_velocity")
+        return vel
+
+    def swe_vel_tend_ll(self, grid, x, y, z, area, vors, divs, height, topo, omega=None, vel=None, tend=None):
+        """swe_vel_tend_computation on this tree (built from the grid's points): returns
+        (vel [3], (vor_tend, div_tend, height_tend))."""
+        from .latlon import OMEGA
+        vel = [self._zeros(self.N) for _ in range(3)] if vel is None else _seq(vel, 3, "vel")
+        tend = [self._zeros(self.N) for _ in range(3)] if tend is None else _seq(tend, 3, "tend")
+        nlat, nlon, dx = int(grid[0]), int(grid[1]), float(grid[2])
+        a = [self._pt(t, "swe_vel_tend_ll array") for t in (x, y, z, area, vors, divs, height, topo, *vel, *tend)]
+        self.ctx._check(self._lib.sbslpm_irreg_swe_vel_tend_ll(
+            self.h, nlat, nlon, dx, float(OMEGA if omega is None else omega), *a), "irreg_swe_vel_tend_ll")
+        return vel, tend
+
+    def swe_back_rk4_step_ll(self, grid, delta_t, x, y, z, area, vors, divs, height, topo, omega=None, vel=None,
+                             time=0.0):
+        """swe_back_rk4_step_ll: one step of (vors, divs, height) in place; returns vel, the
+        start state's velocity."""
+        from .latlon import OMEGA
+        vel = [self._zeros(self.N) for _ in range(3)] if vel is None else _seq(vel, 3, "vel")
+        nlat, nlon, dx = int(grid[0]), int(grid[1]), float(grid[2])
+        a = [self._pt(t, "swe_back_rk4_step_ll array") for t in (x, y, z, area, vors, divs, height, topo, *vel)]
+        self.ctx._check(self._lib.sbslpm_irreg_swe_back_rk4_step_ll(
+            self.h, nlat, nlon, dx, float(delta_t), float(OMEGA if omega is None else omega), *a, float(time)),
+            "irreg_swe_back_rk4_step_ll")
         return vel
 
     def bve_velocity(self, area, vor, vel=None):

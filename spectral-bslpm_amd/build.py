@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libsbslpm.so")
 PROBE_LIB = os.path.join(HERE, "libsbslpm_probe.so")
 ARCH = os.environ.get("SBSLPM_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["fmm_passes.hip", "fmm_interact.hip", "remap.hip", "direct.hip", "targets.hip", "swe.hip", "adjoint.hip", "remap_adjoint.hip", "output.hip", "irreg.hip", "swe_step.hip", "swe_adjoint.hip", "capi.hip"]
+HIP_SOURCES = ["fmm_passes.hip", "fmm_interact.hip", "remap.hip", "direct.hip", "targets.hip", "swe.hip", "adjoint.hip", "remap_adjoint.hip", "output.hip", "irreg.hip", "swe_step.hip", "swe_adjoint.hip", "swe_ll.hip", "capi.hip"]
 # MFMA results in VGPRs (gfx950 reads them from the unified register file), so
 # the vector unit can take the reciprocal without accumulator-register copies.
 EXTRA = {"fmm_interact.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}

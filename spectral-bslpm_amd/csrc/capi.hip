@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <initializer_list>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -174,6 +176,9 @@ struct sbslpm_irreg {
   double* prox[3] = {};
   SrcRecSwe* recswe = nullptr;
   SrcRec64* rec64 = nullptr;
+  // lat-lon step workspace (tendencies, arrival values, pole means), allocated on
+  // the first step
+  double* ll_ws = nullptr;
 };
 
 namespace {
@@ -1847,6 +1852,7 @@ void sbslpm_irreg_destroy(sbslpm_irreg* h) {
   if (!h) return;
   if (h->ctx && h->ctx->stream) hipStreamSynchronize(h->ctx->stream);
   if (h->arena) hipFree(h->arena);
+  if (h->ll_ws) hipFree(h->ll_ws);
   delete h;
 }
 
@@ -1898,6 +1904,17 @@ void irreg_swe_pairs(sbslpm_irreg* h, const double* qz, const double* qzx, const
                         s, true);
 }
 
+// The lat-lon shallow-water velocity at the points in the caller's order (vel assigned):
+// upward passes of both charges, pair sums, downward pass.
+void irreg_swe_vel(sbslpm_irreg* h, const double* area, const double* vors, const double* divs, double* const* vel) {
+  irreg_upward(h, area, vors, h->q[0], h->qx[0]);
+  irreg_upward(h, area, divs, h->q[1], h->qx[1]);
+  irreg_swe_pairs(h, h->q[0], h->qx[0], h->q[1], h->qx[1]);
+  const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
+  irreg_downward(h, h->prox, cp, h->part);
+  launch_scatter(h->tree.N, 3, h->perm, cp, vel, false, h->ctx->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1945,13 +1962,8 @@ int sbslpm_irreg_swe_velocity(sbslpm_irreg* h, const double* area, const double*
                               double* vel_x, double* vel_y, double* vel_z) {
   if (!h || !area || !vors || !divs || !vel_x || !vel_y || !vel_z)
     return h ? fail(h->ctx, SBSLPM_E_ARG, "irreg_swe_velocity: null") : SBSLPM_E_ARG;
-  irreg_upward(h, area, vors, h->q[0], h->qx[0]);
-  irreg_upward(h, area, divs, h->q[1], h->qx[1]);
-  irreg_swe_pairs(h, h->q[0], h->qx[0], h->q[1], h->qx[1]);
-  const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
-  irreg_downward(h, h->prox, cp, h->part);
   double* vo[3] = {vel_x, vel_y, vel_z};
-  launch_scatter(h->tree.N, 3, h->perm, cp, vo, false, h->ctx->stream);
+  irreg_swe_vel(h, area, vors, divs, vo);
   return launched(h->ctx);
 }
 
@@ -2000,6 +2012,160 @@ int sbslpm_xyz_vel_to_uv_vel(sbslpm_ctx* c, int64_t n, const double* x, const do
   if (!c || n < 0 || (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !u || !v)))
     return fail(c, SBSLPM_E_ARG, "xyz_vel_to_uv_vel: bad argument");
   launch_xyz_vel_to_uv(n, x, y, z, vx, vy, vz, u, v, c->stream);
+  return launched(c);
+}
+
+}  // extern "C"
+
+namespace {
+
+// A lat-lon grid the operators accept: at least 3 x 3 points and a positive spacing.
+int ll_grid_check(sbslpm_ctx* c, int nlat, int nlon, double dx, const char* what) {
+  if (nlat < 3 || nlon < 3 || !(dx > 0.0) || !std::isfinite(dx))
+    return fail(c, SBSLPM_E_ARG,
+                std::string(what) + ": lat_count and lon_count must be >= 3 and grid_spacing positive");
+  return SBSLPM_OK;
+}
+
+// Null and aliasing checks: every pointer set, no output equal to an input or to
+// another output (the stencils read neighbouring points while writing).
+int ll_ptr_check(sbslpm_ctx* c, const char* what, std::initializer_list<const void*> in,
+                 std::initializer_list<const void*> out) {
+  for (const void* p : in)
+    if (!p) return fail(c, SBSLPM_E_ARG, std::string(what) + ": null array");
+  for (auto o = out.begin(); o != out.end(); ++o) {
+    if (!*o) return fail(c, SBSLPM_E_ARG, std::string(what) + ": null array");
+    for (const void* p : in)
+      if (p == *o) return fail(c, SBSLPM_E_ARG, std::string(what) + ": an output aliases an input");
+    for (auto q = out.begin(); q != o; ++q)
+      if (*q == *o) return fail(c, SBSLPM_E_ARG, std::string(what) + ": two outputs alias");
+  }
+  return SBSLPM_OK;
+}
+
+int irreg_ll_check(sbslpm_irreg* h, int nlat, int nlon, double dx, const char* what) {
+  if (int rc = ll_grid_check(h->ctx, nlat, nlon, dx, what)) return rc;
+  if ((int64_t)nlat * nlon != h->tree.N)
+    return fail(h->ctx, SBSLPM_E_ARG,
+                std::string(what) + ": the grid has " + std::to_string((int64_t)nlat * nlon) +
+                    " points, the tree " + std::to_string(h->tree.N));
+  return SBSLPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbslpm_ll_tendencies(sbslpm_ctx* c, int nlat, int nlon, double dx, double omega, const double* x,
+                         const double* y, const double* z, const double* vors, const double* divs,
+                         const double* height, const double* topo, const double* vx, const double* vy,
+                         const double* vz, double* vor_t, double* div_t, double* h_t) {
+  if (!c) return SBSLPM_E_ARG;
+  if (int rc = ll_grid_check(c, nlat, nlon, dx, "ll_tendencies")) return rc;
+  if (int rc = ll_ptr_check(c, "ll_tendencies", {x, y, z, vors, divs, height, topo, vx, vy, vz}, {vor_t, div_t, h_t}))
+    return rc;
+  const double* vel[3] = {vx, vy, vz};
+  launch_ll_tendencies(ll_grid(nlat, nlon, dx), omega, x, y, z, vors, divs, height, topo, vel, vor_t, div_t, h_t,
+                       c->stream);
+  return launched(c);
+}
+
+int sbslpm_ll_polar_filter(sbslpm_ctx* c, int nlat, int nlon, double dx, const double* x, const double* y,
+                           const double* z, double* field) {
+  if (!c) return SBSLPM_E_ARG;
+  if (int rc = ll_grid_check(c, nlat, nlon, dx, "ll_polar_filter")) return rc;
+  if (int rc = ll_ptr_check(c, "ll_polar_filter", {x, y, z}, {field})) return rc;
+  launch_ll_polar_filter(ll_grid(nlat, nlon, dx), x, y, z, field, c->stream);
+  return launched(c);
+}
+
+int sbslpm_ll_arrival_values(sbslpm_ctx* c, int nlat, int nlon, double dx, double dt, const double* x,
+                             const double* y, const double* z, const double* vors, const double* divs,
+                             const double* height, const double* vx, const double* vy, const double* vz,
+                             const double* vor_t, const double* div_t, double* a_vor, double* a_div,
+                             double* a_h) {
+  if (!c) return SBSLPM_E_ARG;
+  if (int rc = ll_grid_check(c, nlat, nlon, dx, "ll_arrival_values")) return rc;
+  if (int rc = ll_ptr_check(c, "ll_arrival_values", {x, y, z, vors, divs, height, vx, vy, vz, vor_t, div_t},
+                            {a_vor, a_div, a_h}))
+    return rc;
+  if (int rc = ensure_scratch(c, 4)) return rc;  // the four pole means
+  const LlGrid g = ll_grid(nlat, nlon, dx);
+  const double* vel[3] = {vx, vy, vz};
+  launch_ll_pole_means(g, dt, vors, divs, height, vor_t, c->scratch, c->stream);
+  launch_ll_arrival(g, dt, x, y, z, vors, divs, height, vel, vor_t, div_t, c->scratch, a_vor, a_div, a_h, c->stream);
+  return launched(c);
+}
+
+int sbslpm_ll_field_averaging(sbslpm_ctx* c, int nlat, int nlon, const double* x, const double* y, const double* z,
+                              const double* field, double* new_field, double epsilon) {
+  if (!c) return SBSLPM_E_ARG;
+  if (int rc = ll_grid_check(c, nlat, nlon, 1.0, "ll_field_averaging")) return rc;
+  if (int rc = ll_ptr_check(c, "ll_field_averaging", {x, y, z, field}, {new_field})) return rc;
+  launch_ll_average(ll_grid(nlat, nlon, 1.0), x, y, z, field, new_field, epsilon, c->stream);
+  return launched(c);
+}
+
+int sbslpm_ll_laplacian_diffusion(sbslpm_ctx* c, int nlat, int nlon, double dx, double dt, const double* x,
+                                  const double* y, const double* z, const double* field, double* new_field) {
+  if (!c) return SBSLPM_E_ARG;
+  if (int rc = ll_grid_check(c, nlat, nlon, dx, "ll_laplacian_diffusion")) return rc;
+  if (int rc = ll_ptr_check(c, "ll_laplacian_diffusion", {x, y, z, field}, {new_field})) return rc;
+  launch_ll_diffuse(ll_grid(nlat, nlon, dx), dt, x, y, z, field, new_field, c->stream);
+  return launched(c);
+}
+
+int sbslpm_irreg_swe_vel_tend_ll(sbslpm_irreg* h, int nlat, int nlon, double dx, double omega, const double* x,
+                                 const double* y, const double* z, const double* area, const double* vors,
+                                 const double* divs, const double* height, const double* topo, double* vx,
+                                 double* vy, double* vz, double* vor_t, double* div_t, double* h_t) {
+  if (!h) return SBSLPM_E_ARG;
+  sbslpm_ctx* c = h->ctx;
+  if (int rc = irreg_ll_check(h, nlat, nlon, dx, "irreg_swe_vel_tend_ll")) return rc;
+  if (int rc = ll_ptr_check(c, "irreg_swe_vel_tend_ll", {x, y, z, area, vors, divs, height, topo},
+                            {vx, vy, vz, vor_t, div_t, h_t}))
+    return rc;
+  const LlGrid g = ll_grid(nlat, nlon, dx);
+  double* vel[3] = {vx, vy, vz};
+  irreg_swe_vel(h, area, vors, divs, vel);
+  const double* cv[3] = {vx, vy, vz};
+  launch_ll_tendencies(g, omega, x, y, z, vors, divs, height, topo, cv, vor_t, div_t, h_t, c->stream);
+  launch_ll_polar_filter(g, x, y, z, h_t, c->stream);
+  return launched(c);
+}
+
+int sbslpm_irreg_swe_back_rk4_step_ll(sbslpm_irreg* h, int nlat, int nlon, double dx, double dt, double omega,
+                                      const double* x, const double* y, const double* z, const double* area,
+                                      double* vors, double* divs, double* height, const double* topo, double* vx,
+                                      double* vy, double* vz, double time) {
+  (void)time;
+  if (!h) return SBSLPM_E_ARG;
+  sbslpm_ctx* c = h->ctx;
+  if (int rc = irreg_ll_check(h, nlat, nlon, dx, "irreg_swe_back_rk4_step_ll")) return rc;
+  if (int rc = ll_ptr_check(c, "irreg_swe_back_rk4_step_ll", {x, y, z, area, topo}, {vors, divs, height, vx, vy, vz}))
+    return rc;
+  const int64_t n = h->tree.N;
+  if (!h->ll_ws) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (c->stream && hipStreamIsCapturing(c->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+      return fail(c, SBSLPM_E_ARG, "irreg_swe_back_rk4_step_ll: call once outside stream capture first (workspace)");
+    HIP_TRY(c, hipMalloc(&h->ll_ws, (5 * n + 8) * sizeof(double)));
+  }
+  double* vor_t = h->ll_ws;
+  double* div_t = vor_t + n;
+  double* a_vor = div_t + n;
+  double* a_div = a_vor + n;
+  double* a_h = a_div + n;
+  double* pole = a_h + n;
+  const LlGrid g = ll_grid(nlat, nlon, dx);
+  double* vel[3] = {vx, vy, vz};
+  irreg_swe_vel(h, area, vors, divs, vel);
+  const double* cv[3] = {vx, vy, vz};
+  launch_ll_tendencies(g, omega, x, y, z, vors, divs, height, topo, cv, vor_t, div_t, nullptr, c->stream);
+  launch_ll_pole_means(g, dt, vors, divs, height, vor_t, pole, c->stream);
+  launch_ll_arrival(g, dt, x, y, z, vors, divs, height, cv, vor_t, div_t, pole, a_vor, a_div, a_h, c->stream);
+  launch_ll_smooth(g, dt, x, y, z, a_vor, a_div, a_h, 0.16, vors, divs, height, c->stream);
+  launch_ll_polar_filter(g, x, y, z, divs, c->stream);
   return launched(c);
 }
 

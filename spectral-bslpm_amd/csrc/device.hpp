@@ -386,6 +386,35 @@ void launch_p2m_ll(const DevTree& t, const int* leaves, int nleaves, const int* 
 void launch_l2p_ll(const DevTree& t, const int* point_leaf, int ncomp, const double* const* phi,
                    const double* const* part, double* const* out, hipStream_t s);
 
+// ---- lat-lon shallow-water grid operators (swe_ll.hip); fields [nlat][nlon]
+// row-major on the cell-centred grid, spacing dx degrees (dxr = dx*pi/180)
+struct LlGrid {
+  int nlat, nlon;
+  double dx, dxr;
+};
+LlGrid ll_grid(int nlat, int nlon, double dx_deg);
+// vor_t, div_t, h_t (h_t may be null) from the state and vel[3]
+void launch_ll_tendencies(const LlGrid& g, double omega, const double* x, const double* y, const double* z,
+                          const double* vor, const double* div, const double* h, const double* topo,
+                          const double* const* vel, double* vor_t, double* div_t, double* h_t, hipStream_t s);
+// means[4] = {south vor, south h, north vor, north h} of the first and last rows
+void launch_ll_pole_means(const LlGrid& g, double dt, const double* vor, const double* div, const double* h,
+                          const double* vor_t, double* means, hipStream_t s);
+void launch_ll_arrival(const LlGrid& g, double dt, const double* x, const double* y, const double* z,
+                       const double* vor, const double* div, const double* h, const double* const* vel,
+                       const double* vor_t, const double* div_t, const double* means, double* a_vor, double* a_div,
+                       double* a_h, hipStream_t s);
+void launch_ll_average(const LlGrid& g, const double* x, const double* y, const double* z, const double* f,
+                       double* out, double eps, hipStream_t s);
+void launch_ll_diffuse(const LlGrid& g, double dt, const double* x, const double* y, const double* z,
+                       const double* f, double* out, hipStream_t s);
+// div = diffuse(average(a_div)), vor = a_vor, h = a_h
+void launch_ll_smooth(const LlGrid& g, double dt, const double* x, const double* y, const double* z,
+                      const double* a_vor, const double* a_div, const double* a_h, double eps, double* vor,
+                      double* div, double* h, hipStream_t s);
+void launch_ll_polar_filter(const LlGrid& g, const double* x, const double* y, const double* z, double* f,
+                            hipStream_t s);
+
 // ---- output operators (output.hip)
 int max_interp_fields();
 // out[q][i] = BLI of leaf field q [Pl][m2] at point i (n points, nf <= max_interp_fields()).
