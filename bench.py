@@ -990,14 +990,105 @@ def bench_swe(args, torch, sb):
     print(json.dumps(line), flush=True)
 
 
+def bench_ll(args, torch, sb):
+    """Lat-lon shallow-water step (swe_back_rk4_step_ll) on the reference's grid: 1 degree,
+    180 x 360 = 64,800 points (swe_time_evo_ll.cpp:48-80), interp_degree 10, leaf size
+    4 (d+1)^2 = 484, theta 0.5, kernel_eps 1e-4, delta_t 50 s (namelist.txt), the lat-lon
+    test case 5 with the cone mountain (initial_conditions.cpp:486-506, topography.cpp:111-131).
+    One step = the irregular-tree FMM velocity (two upward passes, the FP64 pair kernel, the
+    downward pass), the tendencies, the pole means, the arrival values, the divergence
+    smoothing and the polar filter; timed steps advance the state.  The CPU baseline is one
+    oracle step from the same state on the host cores, also checked against the GPU's."""
+    L = sb.latlon
+    dx, deg, leaf, dt = 1.0, 10, 484, 50.0
+    lats, lons, x, y, z, area = L.cell_grid(dx)
+    topo = L.cone_topography(lats, lons)
+    vor, div, h = L.initial_state("tc5", lats, lons, topo)
+    nlat, nlon = x.shape
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx = sb.Context(sb.RunConfig(levels=2, interp_degree=deg, kernel_eps=1e-4, fmm_theta=0.5))
+        irr = ctx.irregular(x, y, z, leaf)
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    g = [dev(v) for v in (x, y, z, area, topo)]
+    st0 = [dev(v) for v in (vor, div, h)]
+    st = [v.clone() for v in st0]
+    vel = [torch.zeros(nlat * nlon, dtype=torch.float64, device="cuda") for _ in range(3)]
+    grid = (nlat, nlon, dx)
+
+    def run(n):
+        with torch.cuda.stream(stream):
+            for _ in range(n):
+                irr.swe_back_rk4_step_ll(grid, dt, g[0], g[1], g[2], g[3], *st, g[4], vel=vel)
+
+    run(max(args.warmup, 1))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+    run(args.steps)
+    with torch.cuda.stream(stream):
+        e1.record(stream)
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / args.steps
+    finite = all(bool(torch.isfinite(v).all()) for v in st)
+    log(f"lat-lon step: {1e3 * sec:.3f} ms ({1 / sec:.1f} steps/s), {irr.P} panels, {irr.I} interactions")
+
+    cpu, parity = None, None
+    if args.cpu_budget > 0:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        O.set_threads(threads)
+        o = O.OracleIrreg(x, y, z, leaf, deg, 0.5)
+        one = [v.clone() for v in st0]
+        with torch.cuda.stream(stream):
+            gvel = irr.swe_back_rk4_step_ll(grid, dt, g[0], g[1], g[2], g[3], *one, g[4])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        *ref, rvel = o.swe_back_rk4_step_ll(nlat, nlon, dx, dt, L.OMEGA, area.ravel(), vor.ravel(), div.ravel(),
+                                            h.ravel(), topo.ravel(), 1e-4)
+        osec = time.perf_counter() - t0
+        cpu = dict(value=round(1.0 / osec, 4), unit="steps/s", cores=threads, kind="port",
+                   sample=f"one full oracle lat-lon step at 1 degree ({osec:.2f} s)")
+        # The step interpolates along the row where |v| < 1e-16 (swe_time_step_ll.cpp:446).  In
+        # this zonal state v is rounding noise (|v| < 1e-16 at every point in the oracle), so the
+        # two sides of that test are picked by rounding at some points; those points are counted
+        # and left out of the comparison of vorticity and height (divergence and velocity do not
+        # depend on the branch).
+        _, v_gpu = O.xyz_vel_to_uv_vel(x, y, z, *[t.cpu().numpy() for t in gvel])
+        _, v_orc = O.xyz_vel_to_uv_vel(x, y, z, *rvel)
+        flip = (np.abs(v_gpu) < 1e-16) != (np.abs(v_orc) < 1e-16)
+        errs = []
+        vscale = max(np.abs(v).max() for v in rvel)  # one scale for the three velocity components
+        for i, (a, b) in enumerate(zip(one + gvel, ref + list(rvel))):
+            keep = ~flip if i in (0, 2) else np.ones_like(flip)
+            a = a.cpu().numpy().ravel()
+            scale = vscale if i >= 3 else np.abs(b).max()
+            errs.append(float(np.abs(a - b)[keep].max() / max(scale, 1e-300)))
+        parity = dict(max_rel=float(f"{max(errs):.3e}"), branch_flips=int(flip.sum()),
+                      oracle_points_on_row_branch=int((np.abs(v_orc) < 1e-16).sum()))
+        log(f"cpu baseline: {cpu}; GPU vs oracle step: {parity}")
+
+    line = {"metric": "lat-lon SWE steps/sec at 1 degree = 64,800 points (FP64)",
+            "value": round(1.0 / sec, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(1e3 * sec, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (the reference's lat-lon test case 5 with its cone mountain)",
+            "config": {"workload": "Lat-lon shallow-water step: 180 x 360 points", "points": nlat * nlon,
+                       "interp_degree": deg, "leaf_size": leaf, "theta": 0.5, "panels": irr.P,
+                       "interactions": irr.I, "dt": dt},
+            "state_finite_after_steps": finite, "gpu_vs_oracle_step": parity, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "c5", "adjoint", "4dvar", "swe", "probe"],
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c2", "c4", "c5", "adjoint", "4dvar", "swe", "ll", "probe"],
                     help="c3: the 6x512^2 step (the metric), c1: 6x64^2, c2: the 6x256^2 Poisson solve, "
-                         "swe: the shallow-water step at 6x192^2, "
+                         "swe: the shallow-water step at 6x192^2, ll: the lat-lon shallow-water step at 1 degree, "
                          "c4: the 48-member ensemble, "
                          "adjoint: cost of the step gradient at 6x512^2, 4dvar: BVE 4D-Var gradient at 6x192^2, "
                          "c5: shallow-water 4D-Var gradient at 6x192^2")
@@ -1039,6 +1130,9 @@ def main():
         return
     if args.config == "c5":
         bench_c5(args, torch, sb)
+        return
+    if args.config == "ll":
+        bench_ll(args, torch, sb)
         return
     if args.config == "probe":
         bench_probe(args, torch)

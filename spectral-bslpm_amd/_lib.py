@@ -120,7 +120,7 @@ SIGNATURES = {
     "sbslpm_ll_field_averaging": (_I, [_V, _I, _I] + [_V] * 5 + [_D]),
     "sbslpm_ll_laplacian_diffusion": (_I, [_V, _I, _I, _D, _D] + [_V] * 5),
     "sbslpm_irreg_swe_vel_tend_ll": (_I, [_V, _I, _I, _D, _D] + [_V] * 14),
-    "sbslpm_irreg_swe_back_rk4_step_ll": (_I, [_V, _I, _I, _D, _D, _D] + [_V] * 13 + [_D]),
+    "sbslpm_irreg_swe_back_rk4_step_ll": (_I, [_V, _I, _I, _D, _D, _D] + [_V] * 11 + [_D]),
     "sbslpm_swe_find_departure_points": (_I, [_V, _V, _V, _V, _V, ctypes.c_double, _V, _V, _V]),
     "sbslpm_swe_departure_to_target": (_I, [_V] * 8 + [_I] + [_V] * 4),
     "sbslpm_poisson_solve_targets": (_I, [_V, _V, _V, _I64, _V, _V, _V, _V]),

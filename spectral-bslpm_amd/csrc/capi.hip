@@ -176,6 +176,7 @@ struct sbslpm_irreg {
   double* prox[3] = {};
   SrcRecSwe* recswe = nullptr;
   SrcRec64* rec64 = nullptr;
+  double* w = nullptr;  // [N][2 (d+1)] BLI basis of each sorted point on its leaf
   // lat-lon step workspace (tendencies, arrival values, pole means), allocated on
   // the first step
   double* ll_ws = nullptr;
@@ -1780,7 +1781,8 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
                o_tl = A.take<TargetTile>(t.tiles.size()), o_st = A.take<SourceSet>(t.sets.size()),
                o_zr = A.take<int>(t.zero_rows.size()), o_q = A.take<double>(2 * N), o_qx = A.take<double>(2 * Pn),
                o_part = A.take<double>(3 * N), o_prox = A.take<double>(3 * Pn), o_rs = A.take<SrcRecSwe>(U),
-               o_r64 = A.take<SrcRec64>(U), total = A.take<char>(256);
+               o_r64 = A.take<SrcRec64>(U), o_w = A.take<double>((size_t)N * 2 * t.dp1),
+               total = A.take<char>(256);
   char* b = nullptr;
   if (hipMalloc(&b, total) != hipSuccess) {
     delete h;
@@ -1801,7 +1803,25 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
   if (!rc) rc = upload(c, b + o_pl, t.point_leaf.data(), N * sizeof(int));
   if (!rc) rc = upload(c, b + o_lv, leaves.data(), leaves.size() * sizeof(int));
   if (!rc) rc = upload(c, b + o_nl, t.nonleaf.data(), t.nonleaf.size() * sizeof(int));
-  if (!rc) rc = upload(c, b + o_tl, t.tiles.data(), t.tiles.size() * sizeof(TargetTile));
+  // tiles heaviest first (sum of their source-set sizes): the pair kernels take one tile per
+  // workgroup in dispatch order, so the long tiles start first and the short ones fill in
+  std::vector<TargetTile> tiles_by_cost = t.tiles;
+  {
+    auto cost = [&](const TargetTile& tl) {
+      int64_t w = 0;
+      for (int e = tl.src_begin; e < tl.src_end; ++e) w += t.sets[e].count;
+      return w;
+    };
+    std::vector<int64_t> w(tiles_by_cost.size());
+    std::vector<int> order(tiles_by_cost.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+      order[i] = (int)i;
+      w[i] = cost(t.tiles[i]);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return w[a] > w[b2]; });
+    for (size_t i = 0; i < order.size(); ++i) tiles_by_cost[i] = t.tiles[order[i]];
+  }
+  if (!rc) rc = upload(c, b + o_tl, tiles_by_cost.data(), tiles_by_cost.size() * sizeof(TargetTile));
   if (!rc) rc = upload(c, b + o_st, t.sets.data(), t.sets.size() * sizeof(SourceSet));
   if (!rc) rc = upload(c, b + o_zr, t.zero_rows.data(), t.zero_rows.size() * sizeof(int));
   if (rc) {
@@ -1844,6 +1864,12 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
   }
   h->recswe = (SrcRecSwe*)(b + o_rs);
   h->rec64 = (SrcRec64*)(b + o_r64);
+  h->w = (double*)(b + o_w);
+  launch_point_weights(h->dt, h->point_leaf, h->w, c->stream);
+  if (int lrc = launched(c)) {
+    sbslpm_irreg_destroy(h);
+    return lrc;
+  }
   *out = h;
   return SBSLPM_OK;
 }
@@ -1873,10 +1899,25 @@ void irreg_upward(sbslpm_irreg* h, const double* area, const double* pots, doubl
   const IrregTree& t = h->tree;
   hipStream_t s = h->ctx->stream;
   launch_gather_charges(t.N, h->perm, area, pots, q_sorted, s);
-  launch_p2m_ll(h->dt, h->leaves, h->nleaves, h->leaf_first, h->leaf_count, q_sorted, proxy, s);
+  launch_p2m_ll(h->dt, h->leaves, h->nleaves, h->leaf_first, h->leaf_count, h->w, 1, q_sorted, 0, proxy, 0, s);
   for (int l = t.levels; l >= 0; --l) {
     const int a = t.level_nonleaf[l], b = t.level_nonleaf[l + 1];
     launch_m2m_list(h->dt, h->nonleaf + a, b - a, proxy, s);
+  }
+}
+
+// The same for vorticity and divergence together, into h->q[0..1] and h->qx[0..1]
+// (each pair N and P*n apart): one P2M and one M2M launch per level for both.
+void irreg_upward2(sbslpm_irreg* h, const double* area, const double* vors, const double* divs) {
+  const IrregTree& t = h->tree;
+  hipStream_t s = h->ctx->stream;
+  launch_gather_charges(t.N, h->perm, area, vors, h->q[0], s);
+  launch_gather_charges(t.N, h->perm, area, divs, h->q[1], s);
+  launch_p2m_ll(h->dt, h->leaves, h->nleaves, h->leaf_first, h->leaf_count, h->w, 2, h->q[0], h->q[1] - h->q[0],
+                h->qx[0], h->qx[1] - h->qx[0], s);
+  for (int l = t.levels; l >= 0; --l) {
+    const int a = t.level_nonleaf[l], b = t.level_nonleaf[l + 1];
+    launch_m2m_list(h->dt, h->nonleaf + a, b - a, h->qx[0], s, 2);
   }
 }
 
@@ -1891,7 +1932,7 @@ void irreg_downward(sbslpm_irreg* h, double* const* proxy, const double* const* 
     launch_l2l_list(h->dt, h->nonleaf + a, b - a, 3, proxy, s);
   }
   const double* cp[3] = {proxy[0], proxy[1], proxy[2]};
-  launch_l2p_ll(h->dt, h->point_leaf, 3, cp, part_sorted, out_sorted, s);
+  launch_l2p_ll(h->dt, h->point_leaf, h->w, 3, cp, part_sorted, out_sorted, s);
 }
 
 // Pair sums of the lat-lon shallow-water kernel into the workspace (assigned).
@@ -1901,14 +1942,13 @@ void irreg_swe_pairs(sbslpm_irreg* h, const double* qz, const double* qzx, const
   launch_source_records_swe(h->dt, 3, qz, qzx, qd, qdx, nullptr, nullptr, h->recswe, s);
   launch_zero_rows(h->zero_rows, h->nzero, t.n, 3, h->prox, s);
   launch_swe_interact64(h->dt, 3, h->tiles, h->ntiles, h->sets, h->recswe, h->ctx->prm.kernel_eps, h->part, h->prox,
-                        s, true);
+                        s, true, true);
 }
 
 // The lat-lon shallow-water velocity at the points in the caller's order (vel assigned):
 // upward passes of both charges, pair sums, downward pass.
 void irreg_swe_vel(sbslpm_irreg* h, const double* area, const double* vors, const double* divs, double* const* vel) {
-  irreg_upward(h, area, vors, h->q[0], h->qx[0]);
-  irreg_upward(h, area, divs, h->q[1], h->qx[1]);
+  irreg_upward2(h, area, vors, divs);
   irreg_swe_pairs(h, h->q[0], h->qx[0], h->q[1], h->qx[1]);
   const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
   irreg_downward(h, h->prox, cp, h->part);

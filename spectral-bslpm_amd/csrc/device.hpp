@@ -290,7 +290,7 @@ void launch_source_records_swe(const DevTree& t, int variant, const double* qz_p
                                const double* const* disp_proxy, SrcRecSwe* rec, hipStream_t s);
 void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRecSwe* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s, bool assign = false);
+                           hipStream_t s, bool assign = false, bool shared_tiles = false);
 
 // ---- adjoints (adjoint.hip: FMM transpose; remap_adjoint.hip: remap side)
 size_t adj_target_record_bytes();
@@ -373,7 +373,8 @@ void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, co
                           const double* const* dbar, double* vor_bar, double* const* vbar, hipStream_t s);
 
 // ---- irregular particle tree (irreg.hip; M2M / L2L over panel lists in fmm_passes.hip)
-void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s);
+// nmem proxy arrays, P*n apart
+void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s, int nmem = 1);
 void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp, double* const* phi_proxy,
                      hipStream_t s);
 void launch_gather_charges(int64_t n, const int* perm, const double* area, const double* pots, double* q,
@@ -381,9 +382,13 @@ void launch_gather_charges(int64_t n, const int* perm, const double* area, const
 void launch_scatter(int64_t n, int ncomp, const int* perm, const double* const* in, double* const* out,
                     bool accumulate, hipStream_t s);
 void launch_add(int64_t n, int ncomp, double* const* a, const double* const* b, hipStream_t s);
+// w [npart][2 dp1]: each sorted point's 1-D BLI basis at xi, then at eta, on its leaf
+void launch_point_weights(const DevTree& t, const int* point_leaf, double* w, hipStream_t s);
+// nq (1 or 2) charge arrays q + j*qstride -> proxy rows proxy + j*pstride
 void launch_p2m_ll(const DevTree& t, const int* leaves, int nleaves, const int* leaf_first, const int* leaf_count,
-                   const double* q, double* proxy, hipStream_t s);
-void launch_l2p_ll(const DevTree& t, const int* point_leaf, int ncomp, const double* const* phi,
+                   const double* w, int nq, const double* q, int64_t qstride, double* proxy, int64_t pstride,
+                   hipStream_t s);
+void launch_l2p_ll(const DevTree& t, const int* point_leaf, const double* w, int ncomp, const double* const* phi,
                    const double* const* part, double* const* out, hipStream_t s);
 
 // ---- lat-lon shallow-water grid operators (swe_ll.hip); fields [nlat][nlon]

@@ -374,8 +374,8 @@ void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy,
     hipLaunchKernelGGL(k_l2p, dim3(t.Pl, nmem), dim3(128), 0, s, t, ncomp, cphi, part, out);
 }
 
-void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s) {
-  if (count > 0) hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, t, 0, parents, q_proxy);
+void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s, int nmem) {
+  if (count > 0) hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, 0, parents, q_proxy);
 }
 
 void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp, double* const* phi_proxy,

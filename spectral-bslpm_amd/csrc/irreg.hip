@@ -1,5 +1,6 @@
 // irreg.hip -- passes of the irregular particle tree (the lat-lon path's _ll
 // functors, fmm_funcs_impl.hpp:190-364) on leaf-sorted points:
+//   * k_point_weights: each point's 1-D BLI basis on its leaf, once per tree;
 //   * k_p2m_ll: base_pots_ll -- proxy charges of a leaf by BLI anterpolation
 //     of its points (one workgroup per leaf, point weights staged in LDS);
 //   * k_l2p_ll: child_panel_interp_3_ll -- proxy potentials of the point's
@@ -50,54 +51,84 @@ __global__ void k_add(int64_t n, int ncomp, Ptr3 a, CPtr3 b) {
   for (int f = 0; f < ncomp; ++f) a.p[f][i] += b.p[f][i];
 }
 
+// The 1-D BLI basis of every sorted point on its leaf: w[k][0..d] at xi, then
+// w[k][d+1..2d+1] at eta.  The points do not move, so P2M and L2P read these
+// instead of recomputing them every pass (same values).
+__global__ void k_point_weights(DevTree t, const int* __restrict__ point_leaf, double* __restrict__ w) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t.npart) return;
+  const DevPanel& P = t.panels[point_leaf[k]];
+  const int dp1 = t.dp1;
+  double xe[2];
+  xieta_auto(t.px[k], t.py[k], t.pz[k], xe);
+  double* wk = w + k * 2 * dp1;
+  bool ok = bli_weights_1d(xe[0], P.min_xi, P.max_xi, dp1 - 1, t.cheb_d, wk);
+  ok = bli_weights_1d(xe[1], P.min_eta, P.max_eta, dp1 - 1, t.cheb_d, wk + dp1) && ok;
+  if (!ok) flag(t, kErrBliBounds);
+}
+
 // base_pots_ll (fmm_funcs_impl.hpp:207-223): proxy[leaf][a*(d+1)+b] =
-// sum over the leaf's points of l_a(xi) l_b(eta) q (assigned).
+// sum over the leaf's points of l_a(xi) l_b(eta) q (assigned), for nq = 1 or 2
+// charge arrays at once (q + j*qstride -> proxy + j*pstride); the points'
+// weights and charges are staged in LDS 64 points at a time.
 __global__ __launch_bounds__(128) void k_p2m_ll(DevTree t, const int* __restrict__ leaves,
                                                 const int* __restrict__ leaf_first, const int* __restrict__ leaf_count,
-                                                const double* __restrict__ q, double* __restrict__ proxy) {
+                                                const double* __restrict__ w, int nq, const double* __restrict__ q,
+                                                int64_t qstride, double* __restrict__ proxy, int64_t pstride) {
   __shared__ double sw[64 * 2 * (kMaxDeg + 1)];
-  __shared__ double sq[64];
+  __shared__ double sq[2][64];
   const int leaf = leaves[blockIdx.x];
-  const DevPanel& P = t.panels[leaf];
   const int first = leaf_first[leaf], cnt = leaf_count[leaf];
   const int dp1 = t.dp1, n = t.n, o = threadIdx.x;
-  double acc = 0;
+  double acc0 = 0, acc1 = 0;
   for (int c0 = 0; c0 < cnt; c0 += 64) {
     const int m = min(64, cnt - c0);
+    const double* wc = w + ((int64_t)first + c0) * 2 * dp1;
+    for (int i = o; i < m * 2 * dp1; i += blockDim.x) sw[i] = wc[i];
     if (o < m) {
       const int64_t k = (int64_t)first + c0 + o;
-      double xe[2];
-      xieta_auto(t.px[k], t.py[k], t.pz[k], xe);
-      double* w = sw + o * 2 * dp1;
-      bool ok = bli_weights_1d(xe[0], P.min_xi, P.max_xi, dp1 - 1, t.cheb_d, w);
-      ok = bli_weights_1d(xe[1], P.min_eta, P.max_eta, dp1 - 1, t.cheb_d, w + dp1) && ok;
-      if (!ok) flag(t, kErrBliBounds);
-      sq[o] = q[k];
+      sq[0][o] = q[k];
+      if (nq > 1) sq[1][o] = q[qstride + k];
     }
     __syncthreads();
     if (o < n) {
       const int a = o / dp1, b = o % dp1;
-      for (int j = 0; j < m; ++j) acc = fma(sw[j * 2 * dp1 + a] * sw[j * 2 * dp1 + dp1 + b], sq[j], acc);
+      if (nq > 1) {
+#pragma unroll 4
+        for (int j = 0; j < m; ++j) {
+          const double ab = sw[j * 2 * dp1 + a] * sw[j * 2 * dp1 + dp1 + b];
+          acc0 = fma(ab, sq[0][j], acc0);
+          acc1 = fma(ab, sq[1][j], acc1);
+        }
+      } else {
+#pragma unroll 4
+        for (int j = 0; j < m; ++j) acc0 = fma(sw[j * 2 * dp1 + a] * sw[j * 2 * dp1 + dp1 + b], sq[0][j], acc0);
+      }
     }
     __syncthreads();
   }
-  if (o < n) proxy[(int64_t)leaf * n + o] = acc;
+  if (o < n) {
+    proxy[(int64_t)leaf * n + o] = acc0;
+    if (nq > 1) proxy[pstride + (int64_t)leaf * n + o] = acc1;
+  }
 }
 
 // child_panel_interp_3_ll (fmm_funcs_impl.hpp:341-362) for sorted point k:
 // out[f][k] = sum_ab l_a(xi) l_b(eta) phi[f][leaf][a*(d+1)+b] (+ part[f][k]).
 template <int DP1>
-__global__ __launch_bounds__(256) void k_l2p_ll(DevTree t, const int* __restrict__ point_leaf, int ncomp, CPtr3 phi,
-                                                CPtr3 part, Ptr3 out) {
+__global__ __launch_bounds__(256) void k_l2p_ll(DevTree t, const int* __restrict__ point_leaf,
+                                                const double* __restrict__ w, int ncomp, CPtr3 phi, CPtr3 part,
+                                                Ptr3 out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= t.npart) return;
   const int leaf = point_leaf[k];
-  const DevPanel& P = t.panels[leaf];
-  double xe[2], wx[DP1], we[DP1];
-  xieta_auto(t.px[k], t.py[k], t.pz[k], xe);
-  bool ok = bli_weights_1d(xe[0], P.min_xi, P.max_xi, DP1 - 1, t.cheb_d, wx);
-  ok = bli_weights_1d(xe[1], P.min_eta, P.max_eta, DP1 - 1, t.cheb_d, we) && ok;
-  if (!ok) flag(t, kErrBliBounds);
+  double wx[DP1], we[DP1];
+  const double* wk = w + k * 2 * DP1;
+#pragma unroll
+  for (int a = 0; a < DP1; ++a) {
+    wx[a] = wk[a];
+    we[a] = wk[DP1 + a];
+  }
   for (int f = 0; f < ncomp; ++f) {
     double v = tensor_gather<DP1>(phi.p[f] + (int64_t)leaf * (DP1 * DP1), wx, we);
     if (part.p[f]) v += part.p[f][k];
@@ -107,9 +138,9 @@ __global__ __launch_bounds__(256) void k_l2p_ll(DevTree t, const int* __restrict
 
 template <int DP1>
 struct L2PLL {
-  static void run(const DevTree& t, const int* point_leaf, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 out,
-                  hipStream_t s) {
-    hipLaunchKernelGGL(k_l2p_ll<DP1>, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, point_leaf, ncomp, phi,
+  static void run(const DevTree& t, const int* point_leaf, const double* w, int ncomp, CPtr3 phi, CPtr3 part,
+                  Ptr3 out, hipStream_t s) {
+    hipLaunchKernelGGL(k_l2p_ll<DP1>, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, point_leaf, w, ncomp, phi,
                        part, out);
   }
 };
@@ -143,13 +174,20 @@ void launch_add(int64_t n, int ncomp, double* const* a, const double* const* b, 
   if (n > 0) hipLaunchKernelGGL(k_add, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ncomp, x, y);
 }
 
-void launch_p2m_ll(const DevTree& t, const int* leaves, int nleaves, const int* leaf_first, const int* leaf_count,
-                   const double* q, double* proxy, hipStream_t s) {
-  if (nleaves > 0)
-    hipLaunchKernelGGL(k_p2m_ll, dim3(nleaves), dim3(128), 0, s, t, leaves, leaf_first, leaf_count, q, proxy);
+void launch_point_weights(const DevTree& t, const int* point_leaf, double* w, hipStream_t s) {
+  if (t.npart > 0)
+    hipLaunchKernelGGL(k_point_weights, dim3(blocks_for(t.npart, 256)), dim3(256), 0, s, t, point_leaf, w);
 }
 
-void launch_l2p_ll(const DevTree& t, const int* point_leaf, int ncomp, const double* const* phi,
+void launch_p2m_ll(const DevTree& t, const int* leaves, int nleaves, const int* leaf_first, const int* leaf_count,
+                   const double* w, int nq, const double* q, int64_t qstride, double* proxy, int64_t pstride,
+                   hipStream_t s) {
+  if (nleaves > 0)
+    hipLaunchKernelGGL(k_p2m_ll, dim3(nleaves), dim3(128), 0, s, t, leaves, leaf_first, leaf_count, w, nq, q, qstride,
+                       proxy, pstride);
+}
+
+void launch_l2p_ll(const DevTree& t, const int* point_leaf, const double* w, int ncomp, const double* const* phi,
                    const double* const* part, double* const* out, hipStream_t s) {
   CPtr3 a{}, b{};
   Ptr3 o{};
@@ -158,7 +196,7 @@ void launch_l2p_ll(const DevTree& t, const int* point_leaf, int ncomp, const dou
     b.p[f] = part ? part[f] : nullptr;
     o.p[f] = out[f];
   }
-  if (t.npart > 0) dispatch_m<L2PLL>(t.dp1, t, point_leaf, ncomp, a, b, o, s);
+  if (t.npart > 0) dispatch_m<L2PLL>(t.dp1, t, point_leaf, w, ncomp, a, b, o, s);
 }
 
 }  // namespace sbslpm

@@ -108,6 +108,81 @@ __global__ __launch_bounds__(256) void k_swe64(DevTree t, const TargetTile* __re
   }
 }
 
+// k_swe64 with the source sets of one tile shared by the workgroup's four waves
+// (wave w takes sets w, w+4, ...), the four partial sums added in wave order
+// through LDS.  For tile lists too short to fill the chip with one wave per
+// tile (the lat-lon path's irregular tree: ~2,000 tiles of long source lists),
+// this gives each SIMD four times the waves to hide the record loads behind.
+// Workgroup b takes tile b: a list sorted heaviest first is dispatched in that
+// order (longest-processing-time first), which evens the CUs' loads.
+template <int VARIANT>
+__global__ __launch_bounds__(256) void k_swe64_shared(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                      const SourceSet* __restrict__ sets,
+                                                      const SrcRecSwe* __restrict__ rec, double eps, Ptr3 part,
+                                                      Ptr3 proxy, int assign) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;  // uniform over the workgroup
+  const TargetTile T = tiles[tile];
+  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // W (3), D (3), S
+  for (int e = T.src_begin + wave; e < T.src_end; e += kWavesPerBlock) {
+    const SourceSet S = sets[e];
+    const SrcRecSwe* __restrict__ r = rec + S.first;
+#pragma unroll 2
+    for (int k = 0; k < S.count; ++k) {
+      const SrcRecSwe s = r[k];
+      const double dp = x * s.x + y * s.y + z * s.z;
+      const double g = rcp64_pair((1.0 - dp) + eps);
+      const double gz = g * s.qz;
+      acc[0] = fma(gz, s.x, acc[0]);
+      acc[1] = fma(gz, s.y, acc[1]);
+      acc[2] = fma(gz, s.z, acc[2]);
+      if (VARIANT == 1) {
+        const double gd = g * s.qd;
+        acc[3] = fma(gd, s.x, acc[3]);
+        acc[4] = fma(gd, s.y, acc[4]);
+        acc[5] = fma(gd, s.z, acc[5]);
+        acc[6] = fma(gd, dp, acc[6]);
+      } else {
+        const double dp0 = x * s.x0 + y * s.y0 + z * s.z0;
+        const double gd = rcp64_pair((1.0 - dp0) + eps) * s.qd;
+        acc[3] = fma(gd, s.x0, acc[3]);
+        acc[4] = fma(gd, s.y0, acc[4]);
+        acc[5] = fma(gd, s.z0, acc[5]);
+        acc[6] = fma(gd, dp0, acc[6]);
+      }
+    }
+  }
+  __shared__ double red[7][kWavesPerBlock - 1][64];
+  if (wave > 0)
+#pragma unroll
+    for (int q = 0; q < 7; ++q) red[q][wave - 1][lane] = acc[q];
+  __syncthreads();
+  if (wave > 0 || lane >= T.count) return;
+#pragma unroll
+  for (int q = 0; q < 7; ++q)
+    for (int w = 0; w < kWavesPerBlock - 1; ++w) acc[q] += red[q][w][lane];
+  const double a = (y * acc[2] - z * acc[1]) + (acc[3] - x * acc[6]);
+  const double b = (z * acc[0] - x * acc[2]) + (acc[4] - y * acc[6]);
+  const double c = (x * acc[1] - y * acc[0]) + (acc[5] - z * acc[6]);
+  double* o[3];
+  int64_t v = u;
+  if (u < t.npart) {
+    o[0] = part.p[0]; o[1] = part.p[1]; o[2] = part.p[2];
+  } else {
+    v = u - t.npart;
+    o[0] = proxy.p[0]; o[1] = proxy.p[1]; o[2] = proxy.p[2];
+  }
+  if (assign) {
+    o[0][v] = a; o[1][v] = b; o[2][v] = c;
+  } else {
+    o[0][v] += a; o[1][v] += b; o[2][v] += c;
+  }
+}
+
 // Mixed-precision records: position relative to the source panel centre (FP32)
 // and the two charges with the variant's constants, FP32.
 __global__ void k_src_swe32(DevTree t, const double* __restrict__ qz_part, const double* __restrict__ qz_proxy,
@@ -244,12 +319,21 @@ void launch_source_records_swe(const DevTree& t, int variant, const double* qz_p
 
 void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tiles, int ntiles, const SourceSet* sets,
                            const SrcRecSwe* rec, double eps, double* const* out_part, double* const* out_proxy,
-                           hipStream_t s, bool assign) {
+                           hipStream_t s, bool assign, bool shared_tiles) {
   if (ntiles == 0) return;
   Ptr3 op{}, ox{};
   for (int f = 0; f < 3; ++f) {
     op.p[f] = out_part[f];
     ox.p[f] = out_proxy[f];
+  }
+  if (shared_tiles) {
+    if (variant != 2)
+      hipLaunchKernelGGL(k_swe64_shared<1>, dim3(ntiles), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec,
+                         eps, op, ox, assign ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_swe64_shared<2>, dim3(ntiles), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec,
+                         eps, op, ox, assign ? 1 : 0);
+    return;
   }
   const unsigned blocks = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
   if (variant != 2)

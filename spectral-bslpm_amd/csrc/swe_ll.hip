@@ -373,6 +373,9 @@ __device__ inline double wave_sum(double v) {
 // uninitialised, :628-631).  Values are clamped as the reference does (above
 // 1e-11 and below -1e-11 both to +1e-11), the sums over the row are reduced
 // across the workgroup, then each point is rebuilt from the coefficients.
+// cos(m lat) and sin(m lat) come from one sincos and the angle-addition
+// recurrence (error growing by about an ulp per term, 28 terms at most) in place
+// of a library call per term.
 __global__ __launch_bounds__(kLlBlock) void k_ll_polar(LlGrid g, const double* __restrict__ x,
                                                        const double* __restrict__ y, const double* __restrict__ z,
                                                        double* __restrict__ f) {
@@ -394,14 +397,18 @@ __global__ __launch_bounds__(kLlBlock) void k_ll_polar(LlGrid g, const double* _
     else if (fv < -1e-11)
       fv = 1e-11;
     cs[0] += fv;
+    double s1, c1;
+    sincos(lat, &s1, &c1);
+    double sm = s1, cm = c1;  // sin(m lat), cos(m lat) by the angle-addition recurrence
 #pragma unroll
     for (int m = 1; m < kSpecMax; ++m) {
       if (m < spec) {
-        double s, c;
-        sincos(m * lat, &s, &c);
-        cs[m] += fv * c * g.dxr;
-        sn[m] += fv * s * g.dxr;
+        cs[m] += fv * cm * g.dxr;
+        sn[m] += fv * sm * g.dxr;
       }
+      const double cn = cm * c1 - sm * s1;
+      sm = sm * c1 + cm * s1;
+      cm = cn;
     }
   }
   __shared__ double part[kLlBlock / 64][2 * kSpecMax];
@@ -428,10 +435,14 @@ __global__ __launch_bounds__(kLlBlock) void k_ll_polar(LlGrid g, const double* _
     const int64_t k = ll_at(g, i, j);
     const double lat = ll_lat(x[k], y[k], z[k]);
     double val = coef[0];
+    double s1, c1;
+    sincos(lat, &s1, &c1);
+    double sm = s1, cm = c1;
     for (int m = 1; m < spec; ++m) {
-      double s, c;
-      sincos(m * lat, &s, &c);
-      val += c * coef[m] + s * coef[kSpecMax + m];
+      val += cm * coef[m] + sm * coef[kSpecMax + m];
+      const double cn = cm * c1 - sm * s1;
+      sm = sm * c1 + cm * s1;
+      cm = cn;
     }
     f[k] = val;
   }

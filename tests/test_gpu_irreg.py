@@ -85,4 +85,6 @@ def test_bve_velocity_equals_divergence_free_swe(setup, torch_gpu):
     bve = np.stack([to_np(v) for v in irr.bve_velocity(ga, gv)], 1)
     swe = np.stack([to_np(v) for v in irr.swe_velocity(ga, gv, t.zeros_like(gv))], 1)
     ctx.synchronize()
-    np.testing.assert_allclose(bve, swe, rtol=0, atol=1e-14 * np.abs(swe).max())
+    # same pairs, different summation order (the shallow-water kernel splits each tile's
+    # source sets over four waves): equal to rounding
+    np.testing.assert_allclose(bve, swe, rtol=0, atol=1e-13 * np.abs(swe).max())

@@ -174,7 +174,7 @@ def test_one_degree_step_matches_oracle(sb, orc, torch_gpu):
     leaf size 484, theta 0.5, delta_t 50 s (namelist.txt), test case 5 with the cone."""
     print("\n[case lat-lon 1 degree, d=10]", flush=True)
     s = make_state(sb, 1.0, seed=4)
-    ctx = sb.Context(sb.RunConfig(levels=2, interp_degree=10, kernel_eps=EPS, theta=0.5))
+    ctx = sb.Context(sb.RunConfig(levels=2, interp_degree=10, kernel_eps=EPS, fmm_theta=0.5))
     irr = ctx.irregular(s["x"], s["y"], s["z"], 484)
     o = orc.OracleIrreg(s["x"], s["y"], s["z"], 484, 10, 0.5)
     g = {k: to_gpu(torch_gpu, s[k]) for k in ("x", "y", "z", "area", "topo", "vor", "div", "h")}
