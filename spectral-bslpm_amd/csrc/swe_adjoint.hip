@@ -1,7 +1,8 @@
 // swe_adjoint.hip -- exact transposes of the shallow-water step's pair sums
 // with respect to their source charges and displaced source positions, over
 // the interaction list seen from its sources (one wave per tile of at most 64
-// source rows, each lane owning one row, target records read wave-uniformly):
+// source rows, each lane owning one row, target records read wave-uniformly;
+// tiles that leave lanes idle run in lane groups, see RowGroups):
 //   * the shallow-water velocity sums, variant 1 (swe_vel_panel_interaction,
 //     swe_vel_impl.hpp:12-76): per pair
 //       u_t += r (Qz (x cross y) + Qd (y - x (x.y))),  r = 1 / (1 - x.y + eps),
@@ -34,6 +35,58 @@ struct CPtr3 {
 struct Ptr3 {
   double* p[3];
 };
+
+// Lane groups.  A tile of R source rows leaves 64 - R lanes of its wave idle
+// (28 of 64 at m = 6).  With groups, the tile is cut into H sub-tiles of at
+// most T rows, one per wave, and the wave's lanes hold G = 64 / T copies of the
+// sub-tile (lane = g T + i: row h T + i, copy g); copy g takes targets g, g + G,
+// ... of every target set, and the copies' sums are added in copy order at the
+// end.  Target records are then read per lane (G addresses per load) instead of
+// wave-uniformly.  {1, 64, 1} is the plain one-tile-per-wave form.
+struct RowGroups {
+  int H, T, G;
+};
+
+// The wave's tile, sub-tile row offset and copy for a launch over ntiles tiles;
+// false if the wave has no work.
+template <bool GR>
+__device__ inline bool group_tile(RowGroups gs, int ntiles, int wave, int xcd, int& tile, int& row0, int& g, int& i) {
+  const int lane = threadIdx.x & 63;
+  if (!GR) {
+    tile = shaped_tile(wave, xcd);
+    row0 = 0;
+    g = 0;
+    i = lane;
+    return tile < ntiles;
+  }
+  const int task = shaped_tile(wave, xcd);
+  if (task >= ntiles * gs.H) return false;
+  tile = task / gs.H;
+  row0 = (task - tile * gs.H) * gs.T;
+  g = lane / gs.T;
+  i = lane - g * gs.T;
+  return true;
+}
+
+// Grouped waves stage each chunk of up to 64 target records in a per-wave LDS
+// slice (one coalesced record per lane), then read them from there; LDS
+// operations of one wave complete in order, so the fences only keep the
+// compiler from moving the accesses across each other.
+template <typename R>
+__device__ inline void stage_records(R* dst, const R* __restrict__ src, int cn) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int lane = threadIdx.x & 63;
+  if (lane < cn) dst[lane] = src[lane];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Sum of a lane's value over the G copies of its row, in copy order (valid in
+// copy 0's lanes; every lane must call it).
+__device__ inline double group_sum(double v, RowGroups gs, int i) {
+  double s = v;
+  for (int q = 1; q < gs.G; ++q) s += __shfl(v, i + q * gs.T, 64);
+  return s;
+}
 
 // ---- shallow-water velocity sums ----------------------------------------------
 
@@ -75,7 +128,10 @@ __device__ inline void swe_store(const DevTree& t, int64_t u, const double* S3, 
     const double w2 = t.pz[u] + (part ? dpart.p[2] : dprox.p[2])[v];
     const double ir = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
     const double yy = S3[0] * pb[0] + S3[1] * pb[1] + S3[2] * pb[2];
-    double* const* o = part ? dbar_part.p : dbar_prox.p;
+    // one select per component (selecting the pointer array itself puts it in scratch)
+    double* o[3] = {part ? dbar_part.p[0] : dbar_prox.p[0], part ? dbar_part.p[1] : dbar_prox.p[1],
+                    part ? dbar_part.p[2] : dbar_prox.p[2]};
+#pragma unroll
     for (int f = 0; f < 3; ++f) {
       const double d = (pb[f] - S3[f] * yy) * ir;
       o[f][v] = acc ? o[f][v] + d : d;
@@ -83,42 +139,62 @@ __device__ inline void swe_store(const DevTree& t, int64_t u, const double* S3, 
   }
 }
 
-template <bool POS>
+template <bool POS, bool GR>
 __global__ __launch_bounds__(256) void k_swe_adj64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ targets, const SweTgt* __restrict__ trec,
+                                                   RowGroups gs, const SourceSet* __restrict__ targets,
+                                                   const SweTgt* __restrict__ trec,
                                                    const SrcRecSwe* __restrict__ srec, double eps, CPtr3 dpart,
                                                    CPtr3 dprox, double* __restrict__ qz_part,
                                                    double* __restrict__ qz_prox, double* __restrict__ qd_part,
                                                    double* __restrict__ qd_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
                                                    int acc, int xcd) {
+  __shared__ SweTgt stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, xcd, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const SrcRecSwe S = srec[u];
   double az = 0, ad = 0, p0 = 0, p1 = 0, p2 = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet G = targets[e];
     const SweTgt* __restrict__ r = trec + G.first;
+    for (int c0 = 0; c0 < G.count; c0 += GR ? 64 : G.count) {
+      const int cn = GR ? min(64, G.count - c0) : G.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
 #pragma unroll 2
-    for (int k = 0; k < G.count; ++k) {
-      const SweTgt R = r[k];
-      const double g = rcp64_pair((1.0 - (R.x * S.x + R.y * S.y + R.z * S.z)) + eps);
-      const double a1 = S.x * R.a0 + S.y * R.a1 + S.z * R.a2;
-      const double a2 = S.x * R.b0 + S.y * R.b1 + S.z * R.b2;
-      az = fma(g, a1, az);
-      ad = fma(g, a2, ad);
-      if (POS) {
-        const double w = g * fma(S.qz, a1, S.qd * a2);
-        p0 = fma(g, fma(w, R.x, fma(S.qz, R.a0, S.qd * R.b0)), p0);
-        p1 = fma(g, fma(w, R.y, fma(S.qz, R.a1, S.qd * R.b1)), p1);
-        p2 = fma(g, fma(w, R.z, fma(S.qz, R.a2, S.qd * R.b2)), p2);
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const SweTgt R = GR ? stage[wave][k] : r[k];
+        const double g = rcp64_pair((1.0 - (R.x * S.x + R.y * S.y + R.z * S.z)) + eps);
+        const double a1 = S.x * R.a0 + S.y * R.a1 + S.z * R.a2;
+        const double a2 = S.x * R.b0 + S.y * R.b1 + S.z * R.b2;
+        az = fma(g, a1, az);
+        ad = fma(g, a2, ad);
+        if (POS) {
+          const double w = g * fma(S.qz, a1, S.qd * a2);
+          p0 = fma(g, fma(w, R.x, fma(S.qz, R.a0, S.qd * R.b0)), p0);
+          p1 = fma(g, fma(w, R.y, fma(S.qz, R.a1, S.qd * R.b1)), p1);
+          p2 = fma(g, fma(w, R.z, fma(S.qz, R.a2, S.qd * R.b2)), p2);
+        }
       }
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    az = group_sum(az, gs, ci);
+    ad = group_sum(ad, gs, ci);
+    if (POS) {
+      p0 = group_sum(p0, gs, ci);
+      p1 = group_sum(p1, gs, ci);
+      p2 = group_sum(p2, gs, ci);
+    }
+  }
+  if (cg != 0 || row >= T.count) return;
   const double S3[3] = {S.x, S.y, S.z}, pb[3] = {p0, p1, p2};
   swe_store<POS>(t, u, S3, az, ad, pb, dpart, dprox, qz_part, qz_prox, qd_part, qd_prox, dbar_part, dbar_prox, acc);
 }
@@ -152,21 +228,24 @@ __global__ void k_swe_tgt32(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, SweTgt32*
 
 // FP32 pairs: d = y - x from panel-centred coordinates, h = |d|^2/2 = 1 - x.y,
 // and since c1, c2 are orthogonal to x, y.c1 = d.c1 and y.c2 = d.c2.
-template <bool POS>
+template <bool POS, bool GR>
 __global__ __launch_bounds__(256) void k_swe_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ targets,
+                                                   RowGroups gs, const SourceSet* __restrict__ targets,
                                                    const SweTgt32* __restrict__ trec,
                                                    const SrcRecSwe* __restrict__ srec, float eps, CPtr3 dpart,
                                                    CPtr3 dprox, double* __restrict__ qz_part,
                                                    double* __restrict__ qz_prox, double* __restrict__ qd_part,
                                                    double* __restrict__ qd_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
                                                    int acc, int xcd) {
+  __shared__ SweTgt32 stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, xcd, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const SrcRecSwe S = srec[u];
   const DevPanel& PS = t.panels[T.panel];
   const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
@@ -178,20 +257,27 @@ __global__ __launch_bounds__(256) void k_swe_adj32(DevTree t, const TargetTile* 
     // source relative to the target panel's centre
     const float sx = bx - G.ox, sy = by - G.oy, sz = bz - G.oz;
     float fz = 0.f, fd = 0.f, f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    for (int c0 = 0; c0 < G.count; c0 += GR ? 64 : G.count) {
+      const int cn = GR ? min(64, G.count - c0) : G.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
 #pragma unroll 2
-    for (int k = 0; k < G.count; ++k) {
-      const SweTgt32 R = r[k];
-      const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;  // y_s - x_t
-      const float g = __builtin_amdgcn_rcpf(fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps));
-      const float a1 = fmaf(d0, R.a0, fmaf(d1, R.a1, d2 * R.a2));
-      const float a2 = fmaf(d0, R.b0, fmaf(d1, R.b1, d2 * R.b2));
-      fz = fmaf(g, a1, fz);
-      fd = fmaf(g, a2, fd);
-      if (POS) {
-        const float w = g * fmaf(qz, a1, qd * a2);
-        f0 = fmaf(g, fmaf(w, R.x, fmaf(qz, R.a0, qd * R.b0)), f0);
-        f1 = fmaf(g, fmaf(w, R.y, fmaf(qz, R.a1, qd * R.b1)), f1);
-        f2 = fmaf(g, fmaf(w, R.z, fmaf(qz, R.a2, qd * R.b2)), f2);
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const SweTgt32 R = GR ? stage[wave][k] : r[k];
+        const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;  // y_s - x_t
+        const float g = __builtin_amdgcn_rcpf(fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps));
+        const float a1 = fmaf(d0, R.a0, fmaf(d1, R.a1, d2 * R.a2));
+        const float a2 = fmaf(d0, R.b0, fmaf(d1, R.b1, d2 * R.b2));
+        fz = fmaf(g, a1, fz);
+        fd = fmaf(g, a2, fd);
+        if (POS) {
+          const float w = g * fmaf(qz, a1, qd * a2);
+          f0 = fmaf(g, fmaf(w, R.x, fmaf(qz, R.a0, qd * R.b0)), f0);
+          f1 = fmaf(g, fmaf(w, R.y, fmaf(qz, R.a1, qd * R.b1)), f1);
+          f2 = fmaf(g, fmaf(w, R.z, fmaf(qz, R.a2, qd * R.b2)), f2);
+        }
       }
     }
     az += fz;
@@ -200,7 +286,16 @@ __global__ __launch_bounds__(256) void k_swe_adj32(DevTree t, const TargetTile* 
       p0 += f0; p1 += f1; p2 += f2;
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    az = group_sum(az, gs, ci);
+    ad = group_sum(ad, gs, ci);
+    if (POS) {
+      p0 = group_sum(p0, gs, ci);
+      p1 = group_sum(p1, gs, ci);
+      p2 = group_sum(p2, gs, ci);
+    }
+  }
+  if (cg != 0 || row >= T.count) return;
   const double S3[3] = {S.x, S.y, S.z}, pb[3] = {p0, p1, p2};
   swe_store<POS>(t, u, S3, az, ad, pb, dpart, dprox, qz_part, qz_prox, qd_part, qd_prox, dbar_part, dbar_prox, acc);
 }
@@ -275,9 +370,10 @@ __device__ inline void lap_kd(double tt, const LapCoef64& kp, double a1v, double
 // undoes the scaling.  q_bar = eps^{-5/2} sum psi K;  y_bar = q' c3 sum psi K' x.
 // A wave tests each source at once (a ballot) and takes the erf form only when
 // one of its pairs has t < 6 eps.
-template <bool POS>
+template <bool POS, bool GR>
 __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ targets, const LapTgt* __restrict__ trec,
+                                                   RowGroups gs, const SourceSet* __restrict__ targets,
+                                                   const LapTgt* __restrict__ trec,
                                                    const SrcRec64* __restrict__ srec, LapCoef64 kp, double c3,
                                                    CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
                                                    double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
@@ -285,12 +381,15 @@ __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* 
   __shared__ double etab[kExpNegTabSize];
   for (int i = threadIdx.x; i < kExpNegTabSize; i += blockDim.x) etab[i] = t.expnegtab[i];
   __syncthreads();
+  __shared__ LapTgt stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, xcd, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const SrcRec64 S = srec[u];
   const bool no_cut = kp.cut < -1.0;
   const double a1v = in_vgpr(kp.a1), b2v = in_vgpr(kp.b2), e1v = in_vgpr(kp.e1), d2v = in_vgpr(kp.d2);
@@ -299,27 +398,42 @@ __global__ __launch_bounds__(256) void k_lap_adj64(DevTree t, const TargetTile* 
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet G = targets[e];
     const LapTgt* __restrict__ r = trec + G.first;
-    for (int k = 0; k < G.count; ++k) {
-      const LapTgt R = r[k];
-      const double tt = R.x * S.x + R.y * S.y + R.z * S.z;
-      double kv, kd = 0;
-      if (no_cut && __builtin_amdgcn_ballot_w64(tt < kp.t_sat) == 0) {
-        lap_kd<POS>(tt, kp, a1v, b2v, e1v, d2v, c24v, etab, true, kv, kd);
-      } else {
-        // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
-        __asm__ volatile("");
-        lap_kd<POS>(tt, kp, a1v, b2v, e1v, d2v, c24v, etab, false, kv, kd);
-      }
-      aq = fma(R.psi, kv, aq);
-      if (POS) {
-        const double w = R.psi * kd;
-        p0 = fma(w, R.x, p0);
-        p1 = fma(w, R.y, p1);
-        p2 = fma(w, R.z, p2);
+    for (int c0 = 0; c0 < G.count; c0 += GR ? 64 : G.count) {
+      const int cn = GR ? min(64, G.count - c0) : G.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const LapTgt R = GR ? stage[wave][k] : r[k];
+        const double tt = R.x * S.x + R.y * S.y + R.z * S.z;
+        double kv, kd = 0;
+        if (no_cut && __builtin_amdgcn_ballot_w64(tt < kp.t_sat) == 0) {
+          lap_kd<POS>(tt, kp, a1v, b2v, e1v, d2v, c24v, etab, true, kv, kd);
+        } else {
+          // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+          __asm__ volatile("");
+          lap_kd<POS>(tt, kp, a1v, b2v, e1v, d2v, c24v, etab, false, kv, kd);
+        }
+        aq = fma(R.psi, kv, aq);
+        if (POS) {
+          const double w = R.psi * kd;
+          p0 = fma(w, R.x, p0);
+          p1 = fma(w, R.y, p1);
+          p2 = fma(w, R.z, p2);
+        }
       }
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    aq = group_sum(aq, gs, ci);
+    if (POS) {
+      p0 = group_sum(p0, gs, ci);
+      p1 = group_sum(p1, gs, ci);
+      p2 = group_sum(p2, gs, ci);
+    }
+  }
+  if (cg != 0 || row >= T.count) return;
   const bool part = u < t.npart;
   const int64_t v = part ? u : u - t.npart;
   (part ? q_part : q_prox)[v] = c3 * kNegInv4Pi * aq;
@@ -401,20 +515,23 @@ __device__ inline void lap_kd32(float h2, const LapCoef32& kp, float a1v, float 
   }
 }
 
-template <bool POS>
+template <bool POS, bool GR>
 __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ targets,
+                                                   RowGroups gs, const SourceSet* __restrict__ targets,
                                                    const LapTgt32* __restrict__ trec,
                                                    const SrcRec64* __restrict__ srec, LapCoef32 kp, double c3,
                                                    CPtr3 dpart, CPtr3 dprox, double* __restrict__ q_part,
                                                    double* __restrict__ q_prox, Ptr3 dbar_part, Ptr3 dbar_prox,
                                                    int acc, int xcd) {
+  __shared__ LapTgt32 stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, xcd, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const SrcRec64 S = srec[u];
   const DevPanel& PS = t.panels[T.panel];
   const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
@@ -428,24 +545,31 @@ __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* 
     const LapTgt32* __restrict__ r = trec + G.first;
     const float sx = bx - G.ox, sy = by - G.oy, sz = bz - G.oz;
     float fq = 0.f, f0 = 0.f, f1 = 0.f, f2 = 0.f;
-    for (int k = 0; k < G.count; ++k) {
-      const LapTgt32 R = r[k];
-      const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;
-      const float h2 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-      float kv, kd = 0.f;
-      if (no_cut && __builtin_amdgcn_ballot_w64(h2 > h_sat) == 0) {
-        lap_kd32<POS>(h2, kp, a1v, b2v, e1v, d2v, true, kv, kd);
-      } else {
-        // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
-        __asm__ volatile("");
-        lap_kd32<POS>(h2, kp, a1v, b2v, e1v, d2v, false, kv, kd);
-      }
-      fq = fmaf(R.psi, kv, fq);
-      if (POS) {
-        const float w = R.psi * kd;
-        f0 = fmaf(w, R.x, f0);
-        f1 = fmaf(w, R.y, f1);
-        f2 = fmaf(w, R.z, f2);
+    for (int c0 = 0; c0 < G.count; c0 += GR ? 64 : G.count) {
+      const int cn = GR ? min(64, G.count - c0) : G.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const LapTgt32 R = GR ? stage[wave][k] : r[k];
+        const float d0 = sx - R.ax, d1 = sy - R.ay, d2 = sz - R.az;
+        const float h2 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+        float kv, kd = 0.f;
+        if (no_cut && __builtin_amdgcn_ballot_w64(h2 > h_sat) == 0) {
+          lap_kd32<POS>(h2, kp, a1v, b2v, e1v, d2v, true, kv, kd);
+        } else {
+          // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+          __asm__ volatile("");
+          lap_kd32<POS>(h2, kp, a1v, b2v, e1v, d2v, false, kv, kd);
+        }
+        fq = fmaf(R.psi, kv, fq);
+        if (POS) {
+          const float w = R.psi * kd;
+          f0 = fmaf(w, R.x, f0);
+          f1 = fmaf(w, R.y, f1);
+          f2 = fmaf(w, R.z, f2);
+        }
       }
     }
     aq += fq;
@@ -453,7 +577,15 @@ __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* 
       p0 += f0; p1 += f1; p2 += f2;
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    aq = group_sum(aq, gs, ci);
+    if (POS) {
+      p0 = group_sum(p0, gs, ci);
+      p1 = group_sum(p1, gs, ci);
+      p2 = group_sum(p2, gs, ci);
+    }
+  }
+  if (cg != 0 || row >= T.count) return;
   const bool part = u < t.npart;
   const int64_t v = part ? u : u - t.npart;
   (part ? q_part : q_prox)[v] = c3 * kNegInv4Pi * aq;
@@ -472,6 +604,20 @@ __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* 
       o[f][v] = acc ? o[f][v] + d : d;
     }
   }
+}
+
+// Lane groups for the context's source-major tiles: rows per tile follow the
+// tile builder (a panel's m2 particles or n proxy rows, split into equal tiles
+// of at most 64).  Grouping is used only where it fills more lanes.
+RowGroups row_groups(const DevTree& t) {
+  auto tile_rows = [](int c) {
+    const int chunks = (c + 63) / 64;
+    return (c + chunks - 1) / chunks;
+  };
+  const int rows = std::max(tile_rows(t.m2), tile_rows(t.n));
+  const int H = rows <= 32 ? 1 : 2, T = (rows + H - 1) / H, G = 64 / T;
+  if (G * T <= rows) return RowGroups{1, 64, 1};
+  return RowGroups{H, T, G};
 }
 
 CPtr3 cptr(const double* const* p) {
@@ -509,27 +655,29 @@ void launch_swe_adjoint(const DevTree& t, bool fp32, const TargetTile* tiles, in
                         double* qd_prox, double* const* dbar_part, double* const* dbar_prox, bool accumulate,
                         hipStream_t s, PairShape shape) {
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
+  const RowGroups gs = row_groups(t);
+  const bool gr = gs.G > 1;
+  const int tasks = ntiles * gs.H;
+  const dim3 grid((unsigned)((tasks + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const int acc = accumulate ? 1 : 0;
+  const bool pos = dpart != nullptr;
+  const CPtr3 dp = pos ? cptr(dpart) : CPtr3{}, dx = pos ? cptr(dprox) : CPtr3{};
+  const Ptr3 bp = pos ? ptr(dbar_part) : Ptr3{}, bx = pos ? ptr(dbar_prox) : Ptr3{};
+#define SBSLPM_SWE_ADJ(KERNEL, REC, EPSV, P, G)                                                                   \
+  hipLaunchKernelGGL((KERNEL<P, G>), grid, block, 0, s, t, tiles, ntiles, gs, targets, static_cast<const REC*>(trec), \
+                     srec, EPSV, dp, dx, qz_part, qz_prox, qd_part, qd_prox, bp, bx, acc, shape.xcd)
   if (fp32) {
-    const SweTgt32* r = static_cast<const SweTgt32*>(trec);
-    if (dpart)
-      hipLaunchKernelGGL(k_swe_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps,
-                         cptr(dpart), cptr(dprox), qz_part, qz_prox, qd_part, qd_prox, ptr(dbar_part), ptr(dbar_prox),
-                         acc, shape.xcd);
-    else
-      hipLaunchKernelGGL(k_swe_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps,
-                         CPtr3{}, CPtr3{}, qz_part, qz_prox, qd_part, qd_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+    if (pos && gr) SBSLPM_SWE_ADJ(k_swe_adj32, SweTgt32, (float)eps, true, true);
+    else if (pos) SBSLPM_SWE_ADJ(k_swe_adj32, SweTgt32, (float)eps, true, false);
+    else if (gr) SBSLPM_SWE_ADJ(k_swe_adj32, SweTgt32, (float)eps, false, true);
+    else SBSLPM_SWE_ADJ(k_swe_adj32, SweTgt32, (float)eps, false, false);
   } else {
-    const SweTgt* r = static_cast<const SweTgt*>(trec);
-    if (dpart)
-      hipLaunchKernelGGL(k_swe_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, cptr(dpart),
-                         cptr(dprox), qz_part, qz_prox, qd_part, qd_prox, ptr(dbar_part), ptr(dbar_prox), acc,
-                         shape.xcd);
-    else
-      hipLaunchKernelGGL(k_swe_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, CPtr3{},
-                         CPtr3{}, qz_part, qz_prox, qd_part, qd_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+    if (pos && gr) SBSLPM_SWE_ADJ(k_swe_adj64, SweTgt, eps, true, true);
+    else if (pos) SBSLPM_SWE_ADJ(k_swe_adj64, SweTgt, eps, true, false);
+    else if (gr) SBSLPM_SWE_ADJ(k_swe_adj64, SweTgt, eps, false, true);
+    else SBSLPM_SWE_ADJ(k_swe_adj64, SweTgt, eps, false, false);
   }
+#undef SBSLPM_SWE_ADJ
 }
 
 void launch_lap_adj_target_records(const DevTree& t, bool fp32, const double* psi_part, const double* psi_prox,
@@ -549,27 +697,31 @@ void launch_laplacian_adjoint(const DevTree& t, bool fp32, const TargetTile* til
   if (ntiles == 0) return;
   const double eps = kLapEps;
   const double c3 = -4.0 * kPiL / std::pow(eps, 2.5);
-  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
+  const RowGroups gs = row_groups(t);
+  const bool gr = gs.G > 1;
+  const int tasks = ntiles * gs.H;
+  const dim3 grid((unsigned)((tasks + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const int acc = accumulate ? 1 : 0;
+  const bool pos = dpart != nullptr;
+  const CPtr3 dp = pos ? cptr(dpart) : CPtr3{}, dx = pos ? cptr(dprox) : CPtr3{};
+  const Ptr3 bp = pos ? ptr(dbar_part) : Ptr3{}, bx = pos ? ptr(dbar_prox) : Ptr3{};
+#define SBSLPM_LAP_ADJ(KERNEL, REC, COEF, P, G)                                                                   \
+  hipLaunchKernelGGL((KERNEL<P, G>), grid, block, 0, s, t, tiles, ntiles, gs, targets, static_cast<const REC*>(trec), \
+                     srec, COEF, c3, dp, dx, q_part, q_prox, bp, bx, acc, shape.xcd)
   if (fp32) {
-    const LapTgt32* r = static_cast<const LapTgt32*>(trec);
-    if (dpart)
-      hipLaunchKernelGGL(k_lap_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
-                         lap_coef32(eps, 1.0), c3, cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part),
-                         ptr(dbar_prox), acc, shape.xcd);
-    else
-      hipLaunchKernelGGL(k_lap_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
-                         lap_coef32(eps, 1.0), c3, CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+    const LapCoef32 kp = lap_coef32(eps, 1.0);
+    if (pos && gr) SBSLPM_LAP_ADJ(k_lap_adj32, LapTgt32, kp, true, true);
+    else if (pos) SBSLPM_LAP_ADJ(k_lap_adj32, LapTgt32, kp, true, false);
+    else if (gr) SBSLPM_LAP_ADJ(k_lap_adj32, LapTgt32, kp, false, true);
+    else SBSLPM_LAP_ADJ(k_lap_adj32, LapTgt32, kp, false, false);
   } else {
-    const LapTgt* r = static_cast<const LapTgt*>(trec);
-    if (dpart)
-      hipLaunchKernelGGL(k_lap_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
-                         lap_coef64(eps, 1.0), c3, cptr(dpart), cptr(dprox), q_part, q_prox, ptr(dbar_part),
-                         ptr(dbar_prox), acc, shape.xcd);
-    else
-      hipLaunchKernelGGL(k_lap_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec,
-                         lap_coef64(eps, 1.0), c3, CPtr3{}, CPtr3{}, q_part, q_prox, Ptr3{}, Ptr3{}, acc, shape.xcd);
+    const LapCoef64 kp = lap_coef64(eps, 1.0);
+    if (pos && gr) SBSLPM_LAP_ADJ(k_lap_adj64, LapTgt, kp, true, true);
+    else if (pos) SBSLPM_LAP_ADJ(k_lap_adj64, LapTgt, kp, true, false);
+    else if (gr) SBSLPM_LAP_ADJ(k_lap_adj64, LapTgt, kp, false, true);
+    else SBSLPM_LAP_ADJ(k_lap_adj64, LapTgt, kp, false, false);
   }
+#undef SBSLPM_LAP_ADJ
 }
 
 }  // namespace sbslpm
