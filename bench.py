@@ -653,6 +653,10 @@ def bench_c5(args, torch, sb):
                                "mixed_ratio": round(m["grad_hbm"] / m["fwd"], 4),
                                "fp64_gradient_ms": round(1e3 * f["grad_hbm"], 3),
                                "fp64_ratio": round(f["grad_hbm"] / f["fwd"], 4),
+                               # the 24 reverse steps alone against the 24 forward steps: the records-in-HBM
+                               # gradient less its forward run
+                               "mixed_reverse_vs_forward": round((m["grad_hbm"] - m["fwd"]) / m["fwd"], 4),
+                               "fp64_reverse_vs_forward": round((f["grad_hbm"] - f["fwd"]) / f["fwd"], 4),
                                "gradient_rel_l2_vs_checkpointed": float(f"{max(m['hbm_same'], f['hbm_same']):.3e}")},
             "mixed_vs_fp64_gradient_rel_l2": [float(f"{e:.3e}") for e in gerr],
             "forward_bounded": m["bounded"] and f["bounded"],

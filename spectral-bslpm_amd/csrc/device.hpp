@@ -103,6 +103,76 @@ inline int pair_grid(int ntiles, const PairShape& shape) {
   return shape.xcd == 2 && shape.grid > 0 ? shape.grid : (ntiles + shape.waves - 1) / shape.waves;
 }
 
+// Lane groups.  A tile of R rows (targets of a target-major tile, sources of a
+// source-major one) leaves 64 - R lanes of its wave idle (28 of 64 at m = 6).
+// With groups, the tile is cut into H sub-tiles of at most T rows, one per
+// wave, and the wave's lanes hold G = 64 / T copies of the sub-tile (lane =
+// g T + i: row h T + i, copy g); copy g takes records g, g + G, ... of every
+// set, and the copies' sums are added in copy order at the end.  Records are
+// then read per lane, so grouped waves stage them in LDS first.  {1, 64, 1} is
+// the plain one-tile-per-wave form.
+struct RowGroups {
+  int H, T, G;
+};
+
+// Groups for a context's tiles: rows per tile follow the tile builders (a
+// panel's m2 particles or n proxy rows, split into equal tiles of at most 64).
+// Grouping is used only where it fills more lanes.
+inline RowGroups row_groups(const DevTree& t) {
+  auto tile_rows = [](int c) {
+    if (c <= 0) return 0;
+    const int chunks = (c + 63) / 64;
+    return (c + chunks - 1) / chunks;
+  };
+  const int rows = tile_rows(t.m2) > tile_rows(t.n) ? tile_rows(t.m2) : tile_rows(t.n);
+  if (rows <= 0) return RowGroups{1, 64, 1};
+  const int H = rows <= 32 ? 1 : 2, T = (rows + H - 1) / H, G = 64 / T;
+  if (G * T <= rows) return RowGroups{1, 64, 1};
+  return RowGroups{H, T, G};
+}
+
+// The wave's tile, sub-tile row offset, copy and row-in-sub-tile for a launch
+// over ntiles tiles (mapping xcd 0 or 1 over the ntiles * H sub-tiles when
+// grouped); false if the wave has no work.
+template <bool GR>
+__device__ inline bool group_tile(RowGroups gs, int ntiles, int wave, int xcd, int& tile, int& row0, int& g, int& i) {
+  const int lane = threadIdx.x & 63;
+  if (!GR) {
+    tile = shaped_tile(wave, xcd);
+    row0 = 0;
+    g = 0;
+    i = lane;
+    return tile < ntiles;
+  }
+  const int task = shaped_tile(wave, xcd);
+  if (task >= ntiles * gs.H) return false;
+  tile = task / gs.H;
+  row0 = (task - tile * gs.H) * gs.T;
+  g = lane / gs.T;
+  i = lane - g * gs.T;
+  return true;
+}
+
+// A grouped wave stages a chunk of up to 64 records in its LDS slice (one
+// coalesced record per lane) before reading them per lane; LDS operations of
+// one wave complete in order, so the fences only keep the compiler from moving
+// the accesses across each other.
+template <typename R>
+__device__ inline void stage_records(R* dst, const R* __restrict__ src, int cn) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int lane = threadIdx.x & 63;
+  if (lane < cn) dst[lane] = src[lane];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Sum of a lane's value over the G copies of its row, in copy order (valid in
+// copy 0's lanes; every lane must call it).
+__device__ inline double group_sum(double v, RowGroups gs, int i) {
+  double s = v;
+  for (int q = 1; q < gs.G; ++q) s += __shfl(v, i + q * gs.T, 64);
+  return s;
+}
+
 // 1/d in FP64: hardware reciprocal estimate plus two Newton-Raphson steps
 // (each step squares the relative error; the estimate is good to ~2^-26), so
 // the result is within an ulp of the correctly rounded quotient.

@@ -377,6 +377,104 @@ __global__ __launch_bounds__(256) void k_lap_mixed(DevTree t, const TargetTile* 
   if (lane < T.count) store_s(t, u, part, proxy, acc, assign);
 }
 
+// k_lap64 and k_lap_mixed in lane groups (RowGroups): each 36-row tile at
+// m = 6 runs as two 18-row sub-tiles in three copies per wave, every copy
+// taking every third source of a set from records staged in LDS; one pair per
+// iteration behind the same wave-wide ballot for the erf form.
+__global__ __launch_bounds__(256) void k_lap64_gr(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                  RowGroups gs, const SourceSet* __restrict__ sets,
+                                                  const SrcRec64* __restrict__ rec, LapCoef64 kp,
+                                                  double* __restrict__ part, double* __restrict__ proxy, int assign) {
+  __shared__ double etab[kExpNegTabSize];
+  __shared__ SrcRec64 stage[kWavesPerBlock][64];
+  for (int i = threadIdx.x; i < kExpNegTabSize; i += blockDim.x) etab[i] = t.expnegtab[i];
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tile, row0, cg, ci;
+  if (!group_tile<true>(gs, ntiles, wave, 1, tile, row0, cg, ci)) return;
+  const TargetTile T = tiles[tile];
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const bool copy_ok = cg < gs.G;
+  const double x = t.px[u], y = t.py[u], z = t.pz[u];
+  const bool no_cut = kp.cut < -1.0;
+  const LapV64 kv{in_vgpr(kp.a1), in_vgpr(kp.b2), in_vgpr(1.0 / 24.0)};
+  double acc = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const SrcRec64* __restrict__ r = rec + S.first;
+    for (int c0 = 0; c0 < S.count; c0 += 64) {
+      const int cn = min(64, S.count - c0);
+      stage_records(stage[wave], r + c0, cn);
+      const int trips = (cn + gs.G - 1) / gs.G;
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = kk * gs.G + cg;
+        if (!(copy_ok && k < cn)) continue;
+        const SrcRec64 s0 = stage[wave][k];
+        const double dp0 = x * s0.x + y * s0.y + z * s0.z;
+        double K0 = lap_pair(dp0, kp, kv, etab);
+        if (!no_cut || __builtin_amdgcn_ballot_w64(dp0 < kp.t_sat) != 0) {
+          // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+          __asm__ volatile("");
+          K0 = lap_pair_slow(dp0, kp, kv, etab);
+        }
+        acc = fma(K0, s0.q, acc);
+      }
+    }
+  }
+  acc = group_sum(acc, gs, ci);
+  if (cg == 0 && row < T.count) store_s(t, u, part, proxy, acc, assign);
+}
+
+__global__ __launch_bounds__(256) void k_lap_mixed_gr(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
+                                                      RowGroups gs, const SourceSet* __restrict__ sets,
+                                                      const SrcRec32* __restrict__ rec, LapCoef32 kp, double c3,
+                                                      double* __restrict__ part, double* __restrict__ proxy,
+                                                      int assign) {
+  __shared__ SrcRec32 stage[kWavesPerBlock][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tile, row0, cg, ci;
+  if (!group_tile<true>(gs, ntiles, wave, 1, tile, row0, cg, ci)) return;
+  const TargetTile T = tiles[tile];
+  const DevPanel& PT = t.panels[T.panel];
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const bool copy_ok = cg < gs.G;
+  const float ax = (float)(t.px[u] - PT.cx), ay = (float)(t.py[u] - PT.cy), az = (float)(t.pz[u] - PT.cz);
+  const bool no_cut = kp.cut < -1.0f;
+  const float h_sat = 2.0f * (1.0f - kp.t_sat);
+  const float a1v = in_vgpr_f(kp.a1), b2v = in_vgpr_f(kp.b2);
+  double acc = 0;
+  for (int e = T.src_begin; e < T.src_end; ++e) {
+    const SourceSet S = sets[e];
+    const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
+    const SrcRec32* __restrict__ r = rec + S.first;
+    float v = 0.f;
+    for (int c0 = 0; c0 < S.count; c0 += 64) {
+      const int cn = min(64, S.count - c0);
+      stage_records(stage[wave], r + c0, cn);
+      const int trips = (cn + gs.G - 1) / gs.G;
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = kk * gs.G + cg;
+        if (!(copy_ok && k < cn)) continue;
+        const SrcRec32 s0 = stage[wave][k];
+        const float d0 = s0.x - px, d1 = s0.y - py, d2 = s0.z - pz;
+        const float h0 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+        float K0 = lap32_K(h0, kp, a1v, b2v);
+        if (!no_cut || __builtin_amdgcn_ballot_w64(h0 > h_sat) != 0) {
+          // a real branch (the empty asm keeps the compiler from evaluating erf on every pair)
+          __asm__ volatile("");
+          K0 = lap32_K_slow(h0, kp, a1v, b2v);
+        }
+        v = fmaf(K0, s0.q, v);
+      }
+    }
+    acc += c3 * (double)v;
+  }
+  acc = group_sum(acc, gs, ci);
+  if (cg == 0 && row < T.count) store_s(t, u, part, proxy, acc, assign);
+}
+
 // Mixed precision Biot-Savart: FP32 pair arithmetic in panel-centred
 // coordinates with 1 - x.y computed as |x - y|^2 / 2, FP32 partial sums per
 // source set flushed into FP64 accumulators, cross product in FP64.
@@ -948,8 +1046,13 @@ void launch_laplacian_interact64(const DevTree& t, const TargetTile* tiles, int 
   const double eps = 0.1, pi = 3.14159265358979323846;
   // the records carry -q/(4 pi): undo it with the kernel's eps^{-5/2}
   const LapCoef64 kp = lap_coef64(eps, -4.0 * pi / std::pow(eps, 2.5));
-  hipLaunchKernelGGL(k_lap64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, kp, out_part, out_proxy, assign ? 1 : 0);
+  const RowGroups gs = row_groups(t);
+  if (gs.G > 1)
+    hipLaunchKernelGGL(k_lap64_gr, dim3(blocks_for((int64_t)ntiles * gs.H, kWavesPerBlock)), dim3(64 * kWavesPerBlock),
+                       0, s, t, tiles, ntiles, gs, sets, rec, kp, out_part, out_proxy, assign ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_lap64, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
+                       ntiles, sets, rec, kp, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,
@@ -958,8 +1061,14 @@ void launch_laplacian_interact_mixed(const DevTree& t, const TargetTile* tiles, 
   if (ntiles == 0) return;
   const double eps = 0.1, pi = 3.14159265358979323846;
   const double c3 = -4.0 * pi / std::pow(eps, 2.5);
-  hipLaunchKernelGGL(k_lap_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t, tiles,
-                     ntiles, sets, rec, lap_coef32(eps, 1.0), c3, out_part, out_proxy, assign ? 1 : 0);
+  const RowGroups gs = row_groups(t);
+  if (gs.G > 1)
+    hipLaunchKernelGGL(k_lap_mixed_gr, dim3(blocks_for((int64_t)ntiles * gs.H, kWavesPerBlock)),
+                       dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, gs, sets, rec, lap_coef32(eps, 1.0), c3,
+                       out_part, out_proxy, assign ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_lap_mixed, dim3(blocks_for(ntiles, kWavesPerBlock)), dim3(64 * kWavesPerBlock), 0, s, t,
+                       tiles, ntiles, sets, rec, lap_coef32(eps, 1.0), c3, out_part, out_proxy, assign ? 1 : 0);
 }
 
 void launch_bve_interact_mixed(const DevTree& t, const TargetTile* tiles, int ntiles, const SourceSet* sets,

@@ -49,47 +49,63 @@ __global__ void k_src_swe(DevTree t, const double* __restrict__ qz_part, const d
   rec[u] = SrcRecSwe{x, y, z, kNegInv4Pi * qz, x0, y0, z0, qd_scale * qd};
 }
 
-template <int VARIANT>
+template <int VARIANT, bool GR>
 __global__ __launch_bounds__(256) void k_swe64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                               const SourceSet* __restrict__ sets, const SrcRecSwe* __restrict__ rec,
-                                               double eps, Ptr3 part, Ptr3 proxy, int assign) {
+                                               RowGroups gs, const SourceSet* __restrict__ sets,
+                                               const SrcRecSwe* __restrict__ rec, double eps, Ptr3 part, Ptr3 proxy,
+                                               int assign) {
+  __shared__ SrcRecSwe stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, 1, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const double x = t.px[u], y = t.py[u], z = t.pz[u];
   double w0 = 0, w1 = 0, w2 = 0, d0 = 0, d1 = 0, d2 = 0, sd = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet S = sets[e];
     const SrcRecSwe* __restrict__ r = rec + S.first;
+    for (int c0 = 0; c0 < S.count; c0 += GR ? 64 : S.count) {
+      const int cn = GR ? min(64, S.count - c0) : S.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
 #pragma unroll 2
-    for (int k = 0; k < S.count; ++k) {
-      const SrcRecSwe s = r[k];
-      const double dp = x * s.x + y * s.y + z * s.z;
-      const double g = rcp64_pair((1.0 - dp) + eps);
-      const double gz = g * s.qz;
-      w0 = fma(gz, s.x, w0);
-      w1 = fma(gz, s.y, w1);
-      w2 = fma(gz, s.z, w2);
-      if (VARIANT == 1) {
-        const double gd = g * s.qd;
-        d0 = fma(gd, s.x, d0);
-        d1 = fma(gd, s.y, d1);
-        d2 = fma(gd, s.z, d2);
-        sd = fma(gd, dp, sd);
-      } else {
-        const double dp0 = x * s.x0 + y * s.y0 + z * s.z0;
-        const double gd = rcp64_pair((1.0 - dp0) + eps) * s.qd;
-        d0 = fma(gd, s.x0, d0);
-        d1 = fma(gd, s.y0, d1);
-        d2 = fma(gd, s.z0, d2);
-        sd = fma(gd, dp0, sd);
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const SrcRecSwe s = GR ? stage[wave][k] : r[k];
+        const double dp = x * s.x + y * s.y + z * s.z;
+        const double g = rcp64_pair((1.0 - dp) + eps);
+        const double gz = g * s.qz;
+        w0 = fma(gz, s.x, w0);
+        w1 = fma(gz, s.y, w1);
+        w2 = fma(gz, s.z, w2);
+        if (VARIANT == 1) {
+          const double gd = g * s.qd;
+          d0 = fma(gd, s.x, d0);
+          d1 = fma(gd, s.y, d1);
+          d2 = fma(gd, s.z, d2);
+          sd = fma(gd, dp, sd);
+        } else {
+          const double dp0 = x * s.x0 + y * s.y0 + z * s.z0;
+          const double gd = rcp64_pair((1.0 - dp0) + eps) * s.qd;
+          d0 = fma(gd, s.x0, d0);
+          d1 = fma(gd, s.y0, d1);
+          d2 = fma(gd, s.z0, d2);
+          sd = fma(gd, dp0, sd);
+        }
       }
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    w0 = group_sum(w0, gs, ci); w1 = group_sum(w1, gs, ci); w2 = group_sum(w2, gs, ci);
+    d0 = group_sum(d0, gs, ci); d1 = group_sum(d1, gs, ci); d2 = group_sum(d2, gs, ci);
+    sd = group_sum(sd, gs, ci);
+  }
+  if (cg != 0 || row >= T.count) return;
   const double a = (y * w2 - z * w1) + (d0 - x * sd);
   const double b = (z * w0 - x * w2) + (d1 - y * sd);
   const double c = (x * w1 - y * w0) + (d2 - z * sd);
@@ -217,17 +233,21 @@ __global__ void k_src_swe32(DevTree t, const double* __restrict__ qz_part, const
 //   u = x cross sum(g qz d) + sum(g qd d) + x sum(g qd h),   g = 1/(h + eps).
 // FP32 partial sums per source set are added into FP64; the cross product and
 // the output are FP64.
+template <bool GR>
 __global__ __launch_bounds__(256) void k_swe_mixed(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ sets,
+                                                   RowGroups gs, const SourceSet* __restrict__ sets,
                                                    const SrcRecSwe32* __restrict__ rec, float eps, Ptr3 part,
                                                    Ptr3 proxy, int assign) {
+  __shared__ SrcRecSwe32 stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, 1, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
   const DevPanel& PT = t.panels[T.panel];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const double x = t.px[u], y = t.py[u], z = t.pz[u];
   const float ax = (float)(x - PT.cx), ay = (float)(y - PT.cy), az = (float)(z - PT.cz);
   double w0 = 0, w1 = 0, w2 = 0, d0 = 0, d1 = 0, d2 = 0, sd = 0;
@@ -236,26 +256,38 @@ __global__ __launch_bounds__(256) void k_swe_mixed(DevTree t, const TargetTile* 
     const float px = ax + S.ox, py = ay + S.oy, pz = az + S.oz;
     const SrcRecSwe32* __restrict__ r = rec + S.first;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, e0 = 0.f, e1 = 0.f, e2 = 0.f, eh = 0.f;
+    for (int c0 = 0; c0 < S.count; c0 += GR ? 64 : S.count) {
+      const int cn = GR ? min(64, S.count - c0) : S.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
 #pragma unroll 4
-    for (int k = 0; k < S.count; ++k) {
-      const SrcRecSwe32 s = r[k];
-      const float c0 = s.x - px, c1 = s.y - py, c2 = s.z - pz;
-      const float h = 0.5f * fmaf(c0, c0, fmaf(c1, c1, c2 * c2));
-      const float g = __builtin_amdgcn_rcpf(h + eps);
-      const float gz = g * s.qz, gd = g * s.qd;
-      v0 = fmaf(gz, c0, v0);
-      v1 = fmaf(gz, c1, v1);
-      v2 = fmaf(gz, c2, v2);
-      e0 = fmaf(gd, c0, e0);
-      e1 = fmaf(gd, c1, e1);
-      e2 = fmaf(gd, c2, e2);
-      eh = fmaf(gd, h, eh);
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const SrcRecSwe32 s = GR ? stage[wave][k] : r[k];
+        const float c0f = s.x - px, c1f = s.y - py, c2f = s.z - pz;
+        const float h = 0.5f * fmaf(c0f, c0f, fmaf(c1f, c1f, c2f * c2f));
+        const float g = __builtin_amdgcn_rcpf(h + eps);
+        const float gz = g * s.qz, gd = g * s.qd;
+        v0 = fmaf(gz, c0f, v0);
+        v1 = fmaf(gz, c1f, v1);
+        v2 = fmaf(gz, c2f, v2);
+        e0 = fmaf(gd, c0f, e0);
+        e1 = fmaf(gd, c1f, e1);
+        e2 = fmaf(gd, c2f, e2);
+        eh = fmaf(gd, h, eh);
+      }
     }
     w0 += v0; w1 += v1; w2 += v2;
     d0 += e0; d1 += e1; d2 += e2;
     sd += eh;
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    w0 = group_sum(w0, gs, ci); w1 = group_sum(w1, gs, ci); w2 = group_sum(w2, gs, ci);
+    d0 = group_sum(d0, gs, ci); d1 = group_sum(d1, gs, ci); d2 = group_sum(d2, gs, ci);
+    sd = group_sum(sd, gs, ci);
+  }
+  if (cg != 0 || row >= T.count) return;
   const double a = (y * w2 - z * w1) + (d0 + x * sd);
   const double b = (z * w0 - x * w2) + (d1 + y * sd);
   const double c = (x * w1 - y * w0) + (d2 + z * sd);
@@ -299,8 +331,14 @@ void launch_swe_interact_mixed(const DevTree& t, const TargetTile* tiles, int nt
     ox.p[f] = out_proxy[f];
   }
   const unsigned blocks = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL(k_swe_mixed, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec,
-                     (float)eps, op, ox, assign ? 1 : 0);
+  const RowGroups gs = row_groups(t);
+  if (gs.G > 1)
+    hipLaunchKernelGGL(k_swe_mixed<true>, dim3((((int64_t)ntiles * gs.H + kWavesPerBlock - 1) / kWavesPerBlock)),
+                       dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, gs, sets, rec, (float)eps, op, ox,
+                       assign ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_swe_mixed<false>, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, gs, sets,
+                       rec, (float)eps, op, ox, assign ? 1 : 0);
 }
 
 void launch_source_records_swe(const DevTree& t, int variant, const double* qz_part, const double* qz_proxy,
@@ -335,13 +373,21 @@ void launch_swe_interact64(const DevTree& t, int variant, const TargetTile* tile
                          eps, op, ox, assign ? 1 : 0);
     return;
   }
-  const unsigned blocks = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
-  if (variant != 2)
-    hipLaunchKernelGGL(k_swe64<1>, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec, eps, op, ox,
-                       assign ? 1 : 0);
-  else
-    hipLaunchKernelGGL(k_swe64<2>, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, t, tiles, ntiles, sets, rec, eps, op, ox,
-                       assign ? 1 : 0);
+  const RowGroups gs = row_groups(t);
+  const unsigned blocks = (unsigned)(((int64_t)ntiles * gs.H + kWavesPerBlock - 1) / kWavesPerBlock);
+  const dim3 block(64 * kWavesPerBlock);
+  const int as = assign ? 1 : 0;
+  if (variant != 2) {
+    if (gs.G > 1)
+      hipLaunchKernelGGL((k_swe64<1, true>), dim3(blocks), block, 0, s, t, tiles, ntiles, gs, sets, rec, eps, op, ox, as);
+    else
+      hipLaunchKernelGGL((k_swe64<1, false>), dim3(blocks), block, 0, s, t, tiles, ntiles, gs, sets, rec, eps, op, ox, as);
+  } else {
+    if (gs.G > 1)
+      hipLaunchKernelGGL((k_swe64<2, true>), dim3(blocks), block, 0, s, t, tiles, ntiles, gs, sets, rec, eps, op, ox, as);
+    else
+      hipLaunchKernelGGL((k_swe64<2, false>), dim3(blocks), block, 0, s, t, tiles, ntiles, gs, sets, rec, eps, op, ox, as);
+  }
 }
 
 }  // namespace sbslpm

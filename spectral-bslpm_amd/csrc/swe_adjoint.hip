@@ -36,58 +36,6 @@ struct Ptr3 {
   double* p[3];
 };
 
-// Lane groups.  A tile of R source rows leaves 64 - R lanes of its wave idle
-// (28 of 64 at m = 6).  With groups, the tile is cut into H sub-tiles of at
-// most T rows, one per wave, and the wave's lanes hold G = 64 / T copies of the
-// sub-tile (lane = g T + i: row h T + i, copy g); copy g takes targets g, g + G,
-// ... of every target set, and the copies' sums are added in copy order at the
-// end.  Target records are then read per lane (G addresses per load) instead of
-// wave-uniformly.  {1, 64, 1} is the plain one-tile-per-wave form.
-struct RowGroups {
-  int H, T, G;
-};
-
-// The wave's tile, sub-tile row offset and copy for a launch over ntiles tiles;
-// false if the wave has no work.
-template <bool GR>
-__device__ inline bool group_tile(RowGroups gs, int ntiles, int wave, int xcd, int& tile, int& row0, int& g, int& i) {
-  const int lane = threadIdx.x & 63;
-  if (!GR) {
-    tile = shaped_tile(wave, xcd);
-    row0 = 0;
-    g = 0;
-    i = lane;
-    return tile < ntiles;
-  }
-  const int task = shaped_tile(wave, xcd);
-  if (task >= ntiles * gs.H) return false;
-  tile = task / gs.H;
-  row0 = (task - tile * gs.H) * gs.T;
-  g = lane / gs.T;
-  i = lane - g * gs.T;
-  return true;
-}
-
-// Grouped waves stage each chunk of up to 64 target records in a per-wave LDS
-// slice (one coalesced record per lane), then read them from there; LDS
-// operations of one wave complete in order, so the fences only keep the
-// compiler from moving the accesses across each other.
-template <typename R>
-__device__ inline void stage_records(R* dst, const R* __restrict__ src, int cn) {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  const int lane = threadIdx.x & 63;
-  if (lane < cn) dst[lane] = src[lane];
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
-// Sum of a lane's value over the G copies of its row, in copy order (valid in
-// copy 0's lanes; every lane must call it).
-__device__ inline double group_sum(double v, RowGroups gs, int i) {
-  double s = v;
-  for (int q = 1; q < gs.G; ++q) s += __shfl(v, i + q * gs.T, 64);
-  return s;
-}
-
 // ---- shallow-water velocity sums ----------------------------------------------
 
 // Target record: node, c1 = psi cross x, c2 = psi - x (x.psi) (72 bytes).
@@ -604,20 +552,6 @@ __global__ __launch_bounds__(256) void k_lap_adj32(DevTree t, const TargetTile* 
       o[f][v] = acc ? o[f][v] + d : d;
     }
   }
-}
-
-// Lane groups for the context's source-major tiles: rows per tile follow the
-// tile builder (a panel's m2 particles or n proxy rows, split into equal tiles
-// of at most 64).  Grouping is used only where it fills more lanes.
-RowGroups row_groups(const DevTree& t) {
-  auto tile_rows = [](int c) {
-    const int chunks = (c + 63) / 64;
-    return (c + chunks - 1) / chunks;
-  };
-  const int rows = std::max(tile_rows(t.m2), tile_rows(t.n));
-  const int H = rows <= 32 ? 1 : 2, T = (rows + H - 1) / H, G = 64 / T;
-  if (G * T <= rows) return RowGroups{1, 64, 1};
-  return RowGroups{H, T, G};
 }
 
 CPtr3 cptr(const double* const* p) {
