@@ -508,6 +508,13 @@ def bench_4dvar(args, torch, sb):
         an = float((grad * d).sum())
         rels.append(abs(fd - an) / abs(fd))
     log(f"finite differences: max rel {max(rels):.2e}, median {float(np.median(rels)):.2e}")
+    # the same on a mixed-precision context (FP32 pair sums forward and transposed)
+    ctxm = sb.Context(sb.RunConfig(precision="mixed", **kw))
+    m_fwd, _ = timed(lambda: ag.bve_steps(ctxm, *g, first, nsteps, nsteps), args.steps)
+    m_grad, (Lm, gradm) = timed(lambda: ag.bve_4dvar(ctxm, *g, first, nsteps, every, obs, obs_val), args.steps)
+    m_err = float((gradm - grad).norm() / grad.norm())
+    log(f"mixed: forward {1e3 * m_fwd:.1f} ms, gradient {1e3 * m_grad:.1f} ms ({m_grad / m_fwd:.2f}x), "
+        f"gradient rel-L2 vs fp64 {m_err:.2e}")
     line = {"metric": "BVE 4D-Var gradient cost / forward cost, 24 steps at 6x192^2",
             "value": round(t_grad / t_fwd, 4), "unit": "x forward run", "n_gpus": 1, "steps": args.steps,
             "warmup": 1, "ms_per_step": round(1e3 * t_grad / nsteps, 3), "higher_is_better": False,
@@ -517,7 +524,10 @@ def bench_4dvar(args, torch, sb):
                        "particles": int(x.size), "observations": nobs},
             "forward_ms": round(1e3 * t_fwd, 3), "gradient_ms": round(1e3 * t_grad, 3), "loss": float(L),
             "fd_check": {"directions": ndir, "max_rel_err": float(f"{max(rels):.3e}"),
-                         "median_rel_err": float(f"{float(np.median(rels)):.3e}"), "relative_step": 1e-4}}
+                         "median_rel_err": float(f"{float(np.median(rels)):.3e}"), "relative_step": 1e-4},
+            "mixed": {"forward_ms": round(1e3 * m_fwd, 3), "gradient_ms": round(1e3 * m_grad, 3),
+                      "ratio": round(m_grad / m_fwd, 4), "loss": float(Lm),
+                      "gradient_rel_l2_vs_fp64": float(f"{m_err:.3e}")}}
     print(json.dumps(line), flush=True)
 
 

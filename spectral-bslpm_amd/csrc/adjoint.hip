@@ -55,39 +55,57 @@ __global__ void k_tgt_rec(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, TgtRec* __r
 }
 
 // Transposed pair sums.  POS: also the position cotangent (displaced stages).
-template <bool POS>
+template <bool POS, bool GR>
 __global__ __launch_bounds__(256) void k_bve_adj64(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ targets,
+                                                   RowGroups gs, const SourceSet* __restrict__ targets,
                                                    const TgtRec* __restrict__ trec, const SrcRec64* __restrict__ srec,
                                                    double eps, CPtr3 dpart, CPtr3 dprox, double* __restrict__ qbar_part,
                                                    double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox, int xcd) {
+  __shared__ TgtRec stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, xcd, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const SrcRec64 S = srec[u];
   double aq = 0, a0 = 0, a1 = 0, a2 = 0;
   for (int e = T.src_begin; e < T.src_end; ++e) {
     const SourceSet G = targets[e];
     const TgtRec* __restrict__ r = trec + G.first;
+    for (int c0 = 0; c0 < G.count; c0 += GR ? 64 : G.count) {
+      const int cn = GR ? min(64, G.count - c0) : G.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
 #pragma unroll 4
-    for (int k = 0; k < G.count; ++k) {
-      const TgtRec R = r[k];
-      // the forward kernel's denominator, x = target, y = source
-      const double g = rcp64_pair((1.0 - (R.x * S.x + R.y * S.y + R.z * S.z)) + eps);
-      const double dc = S.x * R.c0 + S.y * R.c1 + S.z * R.c2;
-      aq = fma(g, dc, aq);
-      if (POS) {
-        const double h = g * dc;
-        a0 = fma(g, fma(h, R.x, R.c0), a0);
-        a1 = fma(g, fma(h, R.y, R.c1), a1);
-        a2 = fma(g, fma(h, R.z, R.c2), a2);
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const TgtRec R = GR ? stage[wave][k] : r[k];
+        // the forward kernel's denominator, x = target, y = source
+        const double g = rcp64_pair((1.0 - (R.x * S.x + R.y * S.y + R.z * S.z)) + eps);
+        const double dc = S.x * R.c0 + S.y * R.c1 + S.z * R.c2;
+        aq = fma(g, dc, aq);
+        if (POS) {
+          const double h = g * dc;
+          a0 = fma(g, fma(h, R.x, R.c0), a0);
+          a1 = fma(g, fma(h, R.y, R.c1), a1);
+          a2 = fma(g, fma(h, R.z, R.c2), a2);
+        }
       }
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    aq = group_sum(aq, gs, ci);
+    if (POS) {
+      a0 = group_sum(a0, gs, ci);
+      a1 = group_sum(a1, gs, ci);
+      a2 = group_sum(a2, gs, ci);
+    }
+  }
+  if (cg != 0 || row >= T.count) return;
   const bool part = u < t.npart;
   const int64_t v = part ? u : u - t.npart;
   (part ? qbar_part : qbar_prox)[v] = kNegInv4Pi * aq;
@@ -133,18 +151,21 @@ __global__ void k_tgt_rec32(DevTree t, CPtr3 psi_part, CPtr3 psi_prox, TgtRec32*
 // |x - y|^2 / 2 in panel-centred coordinates like the forward mixed kernel,
 // FP32 partial sums per target set flushed into FP64 accumulators, the
 // source-side finish in FP64.
-template <bool POS>
+template <bool POS, bool GR>
 __global__ __launch_bounds__(256) void k_bve_adj32(DevTree t, const TargetTile* __restrict__ tiles, int ntiles,
-                                                   const SourceSet* __restrict__ targets,
+                                                   RowGroups gs, const SourceSet* __restrict__ targets,
                                                    const TgtRec32* __restrict__ trec, const SrcRec64* __restrict__ srec,
                                                    float eps, CPtr3 dpart, CPtr3 dprox, double* __restrict__ qbar_part,
                                                    double* __restrict__ qbar_prox, Ptr3 dbar_part, Ptr3 dbar_prox, int xcd) {
+  __shared__ TgtRec32 stage[GR ? kWavesPerBlock : 1][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int tile = shaped_tile(wave, xcd);
-  if (tile >= ntiles) return;
+  int tile, row0, cg, ci;
+  if (!group_tile<GR>(gs, ntiles, wave, xcd, tile, row0, cg, ci)) return;
   const TargetTile T = tiles[tile];
-  const int64_t u = (int64_t)T.first + (lane < T.count ? lane : T.count - 1);
+  const int row = row0 + ci;
+  const int64_t u = (int64_t)T.first + (row < T.count ? row : T.count - 1);
+  const int ng = GR ? gs.G : 1;
+  const bool copy_ok = !GR || cg < ng;
   const SrcRec64 S = srec[u];
   const DevPanel& PS = t.panels[T.panel];
   const float bx = (float)(S.x - PS.cx), by = (float)(S.y - PS.cy), bz = (float)(S.z - PS.cz);
@@ -156,18 +177,25 @@ __global__ __launch_bounds__(256) void k_bve_adj32(DevTree t, const TargetTile* 
     // source relative to the target panel's centre
     const float qx = bx - G.ox, qy = by - G.oy, qz = bz - G.oz;
     float sq = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int c0 = 0; c0 < G.count; c0 += GR ? 64 : G.count) {
+      const int cn = GR ? min(64, G.count - c0) : G.count;
+      if (GR) stage_records(stage[wave], r + c0, cn);
+      const int trips = GR ? (cn + ng - 1) / ng : cn;
 #pragma unroll 4
-    for (int k = 0; k < G.count; ++k) {
-      const TgtRec32 R = r[k];
-      const float d0 = R.ax - qx, d1 = R.ay - qy, d2 = R.az - qz;  // x_t - y_s
-      const float g = __builtin_amdgcn_rcpf(fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps));
-      const float dc = fmaf(yx, R.c0, fmaf(yy, R.c1, yz * R.c2));
-      sq = fmaf(g, dc, sq);
-      if (POS) {
-        const float h = g * dc;
-        s0 = fmaf(g, fmaf(h, d0 + yx, R.c0), s0);
-        s1 = fmaf(g, fmaf(h, d1 + yy, R.c1), s1);
-        s2 = fmaf(g, fmaf(h, d2 + yz, R.c2), s2);
+      for (int kk = 0; kk < trips; ++kk) {
+        const int k = GR ? kk * ng + cg : kk;
+        if (GR && !(copy_ok && k < cn)) continue;
+        const TgtRec32 R = GR ? stage[wave][k] : r[k];
+        const float d0 = R.ax - qx, d1 = R.ay - qy, d2 = R.az - qz;  // x_t - y_s
+        const float g = __builtin_amdgcn_rcpf(fmaf(0.5f, fmaf(d0, d0, fmaf(d1, d1, d2 * d2)), eps));
+        const float dc = fmaf(yx, R.c0, fmaf(yy, R.c1, yz * R.c2));
+        sq = fmaf(g, dc, sq);
+        if (POS) {
+          const float h = g * dc;
+          s0 = fmaf(g, fmaf(h, d0 + yx, R.c0), s0);
+          s1 = fmaf(g, fmaf(h, d1 + yy, R.c1), s1);
+          s2 = fmaf(g, fmaf(h, d2 + yz, R.c2), s2);
+        }
       }
     }
     aq += sq;
@@ -175,7 +203,15 @@ __global__ __launch_bounds__(256) void k_bve_adj32(DevTree t, const TargetTile* 
       a0 += s0; a1 += s1; a2 += s2;
     }
   }
-  if (lane >= T.count) return;
+  if (GR) {
+    aq = group_sum(aq, gs, ci);
+    if (POS) {
+      a0 = group_sum(a0, gs, ci);
+      a1 = group_sum(a1, gs, ci);
+      a2 = group_sum(a2, gs, ci);
+    }
+  }
+  if (cg != 0 || row >= T.count) return;
   const bool part = u < t.npart;
   const int64_t v = part ? u : u - t.npart;
   (part ? qbar_part : qbar_prox)[v] = kNegInv4Pi * aq;
@@ -208,17 +244,26 @@ void launch_bve_adjoint32(const DevTree& t, const TargetTile* tiles, int ntiles,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
                           double* const* dbar_prox, hipStream_t s, PairShape shape) {
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
+  const RowGroups gs = row_groups(t);
+  const bool gr = gs.G > 1;
+  const dim3 grid((unsigned)(((int64_t)ntiles * gs.H + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const TgtRec32* r = static_cast<const TgtRec32*>(trec);
+  CPtr3 dp{}, dx{};
+  Ptr3 bp{}, bx{};
   if (dpart) {
-    CPtr3 dp{{dpart[0], dpart[1], dpart[2]}}, dx{{dprox[0], dprox[1], dprox[2]}};
-    Ptr3 bp{{dbar_part[0], dbar_part[1], dbar_part[2]}}, bx{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
-    hipLaunchKernelGGL(k_bve_adj32<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps, dp, dx,
-                       qbar_part, qbar_prox, bp, bx, shape.xcd);
-  } else {
-    hipLaunchKernelGGL(k_bve_adj32<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, (float)eps, CPtr3{},
-                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{}, shape.xcd);
+    dp = CPtr3{{dpart[0], dpart[1], dpart[2]}};
+    dx = CPtr3{{dprox[0], dprox[1], dprox[2]}};
+    bp = Ptr3{{dbar_part[0], dbar_part[1], dbar_part[2]}};
+    bx = Ptr3{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
   }
+#define SBSLPM_BVE_ADJ(P, G)                                                                                        \
+  hipLaunchKernelGGL((k_bve_adj32<P, G>), grid, block, 0, s, t, tiles, ntiles, gs, targets, r, srec, (float)eps, dp, dx,     \
+                     qbar_part, qbar_prox, bp, bx, shape.xcd)
+  if (dpart && gr) SBSLPM_BVE_ADJ(true, true);
+  else if (dpart) SBSLPM_BVE_ADJ(true, false);
+  else if (gr) SBSLPM_BVE_ADJ(false, true);
+  else SBSLPM_BVE_ADJ(false, false);
+#undef SBSLPM_BVE_ADJ
 }
 
 void launch_adj_target_records(const DevTree& t, const double* const* psi_part, const double* const* psi_prox,
@@ -233,17 +278,26 @@ void launch_bve_adjoint64(const DevTree& t, const TargetTile* tiles, int ntiles,
                           const double* const* dprox, double* qbar_part, double* qbar_prox, double* const* dbar_part,
                           double* const* dbar_prox, hipStream_t s, PairShape shape) {
   if (ntiles == 0) return;
-  const dim3 grid((unsigned)((ntiles + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
+  const RowGroups gs = row_groups(t);
+  const bool gr = gs.G > 1;
+  const dim3 grid((unsigned)(((int64_t)ntiles * gs.H + shape.waves - 1) / shape.waves)), block(64 * shape.waves);
   const TgtRec* r = static_cast<const TgtRec*>(trec);
+  CPtr3 dp{}, dx{};
+  Ptr3 bp{}, bx{};
   if (dpart) {
-    CPtr3 dp{{dpart[0], dpart[1], dpart[2]}}, dx{{dprox[0], dprox[1], dprox[2]}};
-    Ptr3 bp{{dbar_part[0], dbar_part[1], dbar_part[2]}}, bx{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
-    hipLaunchKernelGGL(k_bve_adj64<true>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, dp, dx,
-                       qbar_part, qbar_prox, bp, bx, shape.xcd);
-  } else {
-    hipLaunchKernelGGL(k_bve_adj64<false>, grid, block, 0, s, t, tiles, ntiles, targets, r, srec, eps, CPtr3{},
-                       CPtr3{}, qbar_part, qbar_prox, Ptr3{}, Ptr3{}, shape.xcd);
+    dp = CPtr3{{dpart[0], dpart[1], dpart[2]}};
+    dx = CPtr3{{dprox[0], dprox[1], dprox[2]}};
+    bp = Ptr3{{dbar_part[0], dbar_part[1], dbar_part[2]}};
+    bx = Ptr3{{dbar_prox[0], dbar_prox[1], dbar_prox[2]}};
   }
+#define SBSLPM_BVE_ADJ(P, G)                                                                                        \
+  hipLaunchKernelGGL((k_bve_adj64<P, G>), grid, block, 0, s, t, tiles, ntiles, gs, targets, r, srec, eps, dp, dx,     \
+                     qbar_part, qbar_prox, bp, bx, shape.xcd)
+  if (dpart && gr) SBSLPM_BVE_ADJ(true, true);
+  else if (dpart) SBSLPM_BVE_ADJ(true, false);
+  else if (gr) SBSLPM_BVE_ADJ(false, true);
+  else SBSLPM_BVE_ADJ(false, false);
+#undef SBSLPM_BVE_ADJ
 }
 
 }  // namespace sbslpm
