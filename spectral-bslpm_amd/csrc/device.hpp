@@ -30,6 +30,9 @@ struct DevTree {
   const double* m2m_loT;
   const double* p2m;   // [d+1][m]
   const double* p2mT;  // [m][d+1]
+  const double* anc_l2l;  // composite L2L matrices (tree.hpp), or null: no fused downward pass
+  const double* l2p_hi;   // [m][d+1] parent basis at the particle nodes of an upper-half child
+  const double* l2p_lo;
   const int* proxy_leaf;      // [offset*n]
   const double* proxy_xieta;  // [offset*n][2]
   const int* proxy_order;     // [offset*n] proxy nodes sorted by leaf

@@ -235,6 +235,310 @@ __global__ __launch_bounds__(128) void k_l2p(DevTree t, int ncomp, CPtr3 phi, CP
   }
 }
 
+// The downward pass fused into one launch, one workgroup per parent of the
+// leaves (level L-2), for trees whose leaf rows are not pair-sum targets.
+// fmm_funcs.cpp:357-392 accumulates every level's rows into the next level
+// (L2L) and interpolates the leaves' rows to the particles (L2P).  A panel's
+// accumulated row is its own row plus each ancestor's own row evaluated at
+// the panel's nodes, since each level's interpolant reproduces the degree-d
+// polynomial it is given.  So the workgroup
+//   1. forms the parent's accumulated row from its own row and its ancestors'
+//      through the composite matrices (DevTree::anc_l2l), up to kAncChunk
+//      ancestors per LDS pass;
+//   2. forms the four leaves' rows by one L2L step (k_l2l's two sweeps), in LDS;
+//   3. interpolates them to the leaf particle grids (k_l2p's two sweeps) and
+//      adds the particle field.
+// Nothing is written but sols: the proxy rows stay as the pair sums left
+// them.  One launch replaces L-1 L2L levels and the L2P, and the leaf rows
+// make no round trip through HBM.
+constexpr int kAncChunk = 6;
+
+__global__ __launch_bounds__(256) void k_l2l_l2p(DevTree t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 sols) {
+  extern __shared__ double lds[];
+  const int n = t.n, D = t.dp1, DD = D * D, m = t.m, m2 = t.m2;
+  const int lp = t.levels - 2, ip = blockIdx.x;
+  const int par = 2 * ((1 << (2 * lp)) - 1) + ip;
+  const int64_t mp = member_part(t), mx = member_proxy(t);
+  const int nc = ncomp * n;
+  double* acc = lds;        // [ncomp][n] accumulated parent row
+  double* R = lds + 3 * n;  // phase 1 and phase 2 scratch
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) acc[i] = comp(phi, i / n)[mx + (int64_t)par * n + i % n];
+  for (int g0 = 0; g0 < lp; g0 += kAncChunk) {
+    const int na = min(kAncChunk, lp - g0);
+    double* sP = R;             // [na][ncomp][n] ancestors' own rows
+    double* sV = sP + na * nc;  // [na][ncomp][n] eta sweeps
+    double* sC = sV + na * nc;  // [na][xi, eta][D][D] composite matrices
+    __syncthreads();  // the previous chunk's sweeps are done with sP, sV, sC
+    for (int i = threadIdx.x; i < na * nc; i += blockDim.x) {
+      const int g = g0 + 1 + i / nc, j = i % nc;
+      const int anc = 2 * ((1 << (2 * (lp - g))) - 1) + (ip >> (2 * g));
+      sP[i] = comp(phi, j / n)[mx + (int64_t)anc * n + j % n];
+    }
+    for (int i = threadIdx.x; i < na * 2 * DD; i += blockDim.x) {
+      const int g = g0 + 1 + i / (2 * DD), eta = (i / DD) & 1;
+      // position of the parent's interval in the ancestor's: bit s-1 is the
+      // upper-half flag of the step s levels above the parent (quadrants
+      // (hi,hi), (lo,hi), (lo,lo), (hi,lo) in (xi, eta))
+      int pos = 0;
+      for (int s = 1; s <= g; ++s) {
+        const int c = (ip >> (2 * (s - 1))) & 3;
+        pos |= (eta ? (c <= 1) : (c == 0 || c == 3)) << (s - 1);
+      }
+      sC[i] = t.anc_l2l[(int64_t)((1 << g) - 2 + pos) * DD + i % DD];
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < na * nc; o += blockDim.x) {  // V[a][l] = sum_b Ceta[l][b] P[a][b]
+      const int r = o % n, a = r / D, l = r % D;
+      const double* Ce = sC + ((o / nc) * 2 + 1) * DD;
+      const double* P = sP + (o - r);
+      double v = 0;
+      for (int b = 0; b < D; ++b) v += Ce[l * D + b] * P[a * D + b];
+      sV[o] = v;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < nc; o += blockDim.x) {  // acc[k][l] += sum_g sum_a Cxi[k][a] V[a][l]
+      const int r = o % n, k = r / D, l = r % D;
+      double v = acc[o];
+      for (int gi = 0; gi < na; ++gi) {
+        const double* Cx = sC + gi * 2 * DD;
+        const double* V = sV + gi * nc + (o - r);
+        for (int a = 0; a < D; ++a) v += Cx[k * D + a] * V[a * D + l];
+      }
+      acc[o] = v;
+    }
+  }
+  __syncthreads();
+  double* shi = R;               // [D][D]
+  double* slo = shi + DD;        // [D][D]
+  double* sT = slo + DD;         // [D][m]
+  double* sVh = sT + D * m;      // [ncomp][2][n]
+  double* sL = sVh + 2 * nc;     // [ncomp][4][n] leaf rows
+  double* sW = sL + 4 * nc;      // [ncomp][4][D][m]
+  for (int i = threadIdx.x; i < DD; i += blockDim.x) { shi[i] = t.m2m_hi[i]; slo[i] = t.m2m_lo[i]; }
+  for (int i = threadIdx.x; i < D * m; i += blockDim.x) sT[i] = t.p2m[i];
+  __syncthreads();
+  for (int o = threadIdx.x; o < 2 * nc; o += blockDim.x) {  // V_h[a][l] = sum_b Y_h[b][l] acc[a][b]
+    const int f = o / (2 * n), h = (o / n) & 1, a = (o % n) / D, l = o % D;
+    const double* Y = h ? slo : shi;
+    const double* P = acc + f * n;
+    double v = 0;
+    for (int b = 0; b < D; ++b) v += Y[b * D + l] * P[a * D + b];
+    sVh[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 4 * nc; o += blockDim.x) {  // leaf_c[k][l] = sum_a X_c[a][k] V_h(c)[a][l]
+    const int f = o / (4 * n), c = (o / n) & 3, k = (o % n) / D, l = o % D;
+    const double* X = (c == 0 || c == 3) ? shi : slo;
+    const double* V = sVh + (f * 2 + (c <= 1 ? 0 : 1)) * n;
+    double v = 0;
+    for (int a = 0; a < D; ++a) v += X[a * D + k] * V[a * D + l];
+    sL[o] = v;
+  }
+  __syncthreads();
+  const int64_t leaf0 = 4 * (int64_t)ip;  // leaves are the last level: panel index - offset
+  if (m == D) {  // the reference layout: the particle grid is the proxy grid
+    for (int o = threadIdx.x; o < 4 * nc; o += blockDim.x) {
+      const int f = o / (4 * n);
+      const int64_t u = mp + leaf0 * m2 + o % (4 * n);
+      const double* pf = comp(part, f);
+      double v = sL[o];
+      if (pf) v += pf[u];
+      comp(sols, f)[u] = v;
+    }
+    return;
+  }
+  const int vn = D * m;
+  for (int o = threadIdx.x; o < 4 * ncomp * vn; o += blockDim.x) {  // W[a][l] = sum_b T[b][l] leaf[a][b]
+    const int lc = o / vn, a = (o % vn) / m, l = o % m;
+    const double* P = sL + lc * n;
+    double v = 0;
+    for (int b = 0; b < D; ++b) v += sT[b * m + l] * P[a * D + b];
+    sW[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 4 * ncomp * m2; o += blockDim.x) {  // sol[k][l] = sum_a T[a][k] W[a][l]
+    const int lc = o / m2, f = lc / 4, c = lc % 4, k = (o % m2) / m, l = o % m;
+    const double* V = sW + lc * vn;
+    double v = 0;
+    for (int a = 0; a < D; ++a) v += sT[a * m + k] * V[a * m + l];
+    const int64_t u = mp + (leaf0 + c) * m2 + (o % m2);
+    const double* pf = comp(part, f);
+    if (pf) v += pf[u];
+    comp(sols, f)[u] = v;
+  }
+}
+
+// LDS accesses of one wave complete in order; the fences keep the compiler
+// from moving a lane's LDS reads above the other lanes' writes.
+__device__ inline void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// A read-only table with a wave-uniform address: a pointer into the constant
+// address space, which the compiler reads with scalar loads.
+typedef const __attribute__((address_space(4))) double* ConstTab;
+__device__ inline ConstTab const_tab(const double* p) { return (ConstTab)p; }
+
+// k_l2l_l2p for a proxy grid of D x D nodes and a leaf grid of M x M
+// particles known at compile time, tiled for registers.  Every sweep is a
+// batch of small products out[r][k] = sum_b in[r][b] C[k][b] whose matrix C
+// is the same for all lanes of a wave: a lane owns one row r (a component and
+// a node index) and keeps it in registers, and the loop over outputs k reads
+// row k of C through the scalar unit, so a multiply-add costs no LDS read.
+//   phase 1: wave w evaluates ancestors g = w+1, w+5, ... at the parent's
+//     nodes (the eta sweep by rows, the xi sweep by columns through the
+//     wave's LDS slice, its sum kept in the slice); the parent's own row and
+//     the four waves' sums are added in a fixed order;
+//   phase 2: waves 0 and 1 evaluate the parent's row at the particle nodes of
+//     the upper and lower half in eta (l2p_hi, l2p_lo); wave c then forms
+//     leaf c's particle values by the xi sweep and adds the particle field.
+// The leaf rows and the L2P matrix are folded into l2p_hi / l2p_lo (the
+// parent's interpolant evaluated at the leaves' particle nodes).
+template <int D, int M>
+__global__ __launch_bounds__(256) void k_down_fused(DevTree t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 sols) {
+  constexpr int N = D * D, M2 = M * M;
+  __shared__ double sV[4][3 * N];
+  __shared__ double sPart[4][3 * N];
+  __shared__ double sAcc[3 * N];
+  __shared__ double sW[2][3 * D * M];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lp = t.levels - 2, ip = blockIdx.x;
+  const int par = 2 * ((1 << (2 * lp)) - 1) + ip;
+  const int64_t mp = member_part(t), mx = member_proxy(t);
+  static_assert(3 * N <= 512 && 3 * D <= 64 && 3 * M <= 64, "k_down_fused: grid too large for one workgroup");
+  const bool row_lane = lane < ncomp * D;
+  const int f = row_lane ? lane / D : 0, a = lane - f * D;
+  const double* __restrict__ phif = comp(phi, f) + mx;
+  // Loads whose latency would otherwise sit in the dependent chain: the
+  // parent's own row (two entries per thread at most) and, for the lanes of
+  // the last sweep, the particle field of leaf c = w.
+  const int nall = ncomp * N;
+  double own[2] = {0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int o = threadIdx.x + 256 * i;
+    if (o < nall) own[i] = comp(phi, o / N)[mx + (int64_t)par * N + o % N];
+  }
+  const bool col_lane = lane < ncomp * M;
+  const int fc = col_lane ? lane / M : 0, lc = lane - fc * M;
+  const int64_t u0 = mp + (4 * (int64_t)ip + w) * M2 + lc;
+  const double* pfc = comp(part, fc);
+  double pv[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) pv[k] = (col_lane && pfc) ? pfc[u0 + k * M] : 0.0;
+  double* part_w = &sPart[w][f * N + a];  // this lane's column of the wave's sum: [k * D]
+  if (row_lane)
+    for (int k = 0; k < D; ++k) part_w[k * D] = 0.0;
+  for (int g = w + 1; g <= lp; g += 4) {
+    int px = 0, py = 0;
+    for (int s = 1; s <= g; ++s) {
+      const int c = (ip >> (2 * (s - 1))) & 3;
+      px |= (c == 0 || c == 3) << (s - 1);
+      py |= (c <= 1) << (s - 1);
+    }
+    const ConstTab Cx = const_tab(t.anc_l2l) + (int64_t)((1 << g) - 2 + px) * N;
+    const ConstTab Cy = const_tab(t.anc_l2l) + (int64_t)((1 << g) - 2 + py) * N;
+    const int64_t anc = 2 * ((1 << (2 * (lp - g))) - 1) + (ip >> (2 * g));
+    if (row_lane) {  // V[a][l] = sum_b Cy[l][b] P[a][b]
+      double p[D];
+#pragma unroll
+      for (int b = 0; b < D; ++b) p[b] = phif[anc * N + a * D + b];
+#pragma unroll 1
+      for (int l = 0; l < D; ++l) {
+        double v = 0.0;
+#pragma unroll
+        for (int b = 0; b < D; ++b) v += Cy[l * D + b] * p[b];
+        sV[w][f * N + a * D + l] = v;
+      }
+    }
+    wave_lds_order();
+    if (row_lane) {  // column l = a: sum[k][l] += sum_r Cx[k][r] V[r][l]
+      double q[D];
+#pragma unroll
+      for (int r = 0; r < D; ++r) q[r] = sV[w][f * N + r * D + a];
+#pragma unroll 1
+      for (int k = 0; k < D; ++k) {
+        double v = part_w[k * D];
+#pragma unroll
+        for (int r = 0; r < D; ++r) v += Cx[k * D + r] * q[r];
+        part_w[k * D] = v;
+      }
+    }
+    wave_lds_order();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int o = threadIdx.x + 256 * i;
+    if (o < nall) sAcc[o] = own[i] + sPart[0][o] + sPart[1][o] + sPart[2][o] + sPart[3][o];
+  }
+  __syncthreads();
+  if (w < 2 && row_lane) {  // W_h[a][l] = sum_b Z_h[l][b] acc[a][b], h = w
+    const ConstTab Z = const_tab(w == 0 ? t.l2p_hi : t.l2p_lo);
+    double p[D];
+#pragma unroll
+    for (int b = 0; b < D; ++b) p[b] = sAcc[f * N + a * D + b];
+#pragma unroll 1
+    for (int l = 0; l < M; ++l) {
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < D; ++b) v += Z[l * D + b] * p[b];
+      sW[w][(f * D + a) * M + l] = v;
+    }
+  }
+  __syncthreads();
+  if (col_lane) {  // leaf c = w: sol[k][l] = sum_r Zx[k][r] W_h(c)[r][l] + part
+    const int c = w;
+    const ConstTab Zx = const_tab((c == 0 || c == 3) ? t.l2p_hi : t.l2p_lo);
+    const double* Wc = sW[c <= 1 ? 0 : 1] + fc * D * M;
+    double q[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) q[r] = Wc[r * M + lc];
+    double* sf = comp(sols, fc);
+    // unrolled for the prefetched pv[k]; the barriers keep each output's
+    // coefficient row from being loaded ahead (the rows would not fit the
+    // scalar registers together)
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      double v = 0.0;
+#pragma unroll
+      for (int r = 0; r < D; ++r) v += Zx[k * D + r] * q[r];
+      sf[u0 + k * M] = v + pv[k];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+template <int D, int M>
+bool launch_down_fused_dm(const DevTree& t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 out, hipStream_t s, int nmem) {
+  if (t.dp1 != D || t.m != M) return false;
+  hipLaunchKernelGGL((k_down_fused<D, M>), dim3(6 << (2 * (t.levels - 2)), nmem), dim3(256), 0, s, t, ncomp, phi, part,
+                     out);
+  return true;
+}
+
+// The register-tiled form for the layouts in use (degree 8 with an 8 x 8 or
+// 9 x 9 leaf grid, degree 5 with 6 x 6, ...); false for the others.
+bool launch_down_fused(const DevTree& t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 out, hipStream_t s, int nmem) {
+  return launch_down_fused_dm<9, 8>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<9, 9>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<6, 6>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<5, 5>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<5, 6>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<6, 7>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<7, 7>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<11, 11>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<11, 8>(t, ncomp, phi, part, out, s, nmem);
+}
+
+// Dynamic LDS of k_l2l_l2p (bytes).
+size_t l2l_l2p_lds(const DevTree& t, int ncomp) {
+  const size_t n = t.n, D = t.dp1, m = t.m, nc = ncomp * n;
+  const size_t na = std::min(kAncChunk, std::max(t.levels - 2, 0));
+  const size_t ph1 = na * (2 * nc + 2 * D * D);
+  const size_t ph2 = 2 * D * D + D * m + 2 * nc + 4 * nc + 4 * ncomp * D * m;
+  return (3 * n + std::max(ph1, ph2)) * sizeof(double);
+}
+
 // Identity L2P of the reference layout: sols = leaf proxy rows (+ particle field).
 __global__ void k_l2p_copy(int64_t npart, int ncomp, CPtr3 leafrows, CPtr3 part, Ptr3 sols, int64_t proxy_stride) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,9 +668,18 @@ void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy,
     leafrows.p[f] = phi_proxy[f] + (int64_t)t.offset * t.n;
     out.p[f] = sols[f];
   }
-  launch_l2l(t, ncomp, phi_proxy, s, assign_leaf_rows, nmem);
   CPtr3 cphi{};
   for (int f = 0; f < ncomp; ++f) cphi.p[f] = phi_proxy[f];
+  if (assign_leaf_rows && t.levels >= 2 && t.anc_l2l) {
+    if (launch_down_fused(t, ncomp, cphi, part, out, s, nmem)) return;
+    const size_t lds = l2l_l2p_lds(t, ncomp);
+    if (lds <= 65536) {
+      hipLaunchKernelGGL(k_l2l_l2p, dim3(6 << (2 * (t.levels - 2)), nmem), dim3(256), lds, s, t, ncomp, cphi, part,
+                         out);
+      return;
+    }
+  }
+  launch_l2l(t, ncomp, phi_proxy, s, assign_leaf_rows, nmem);
   if (is_identity_p2m(t))
     hipLaunchKernelGGL(k_l2p_copy, dim3(blocks_for(t.npart, 256), nmem), dim3(256), 0, s, t.npart, ncomp, leafrows,
                        part, out, (int64_t)t.P * t.n);

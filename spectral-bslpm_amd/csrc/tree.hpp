@@ -83,6 +83,10 @@ struct Tree {
   std::vector<double> px, py, pz;       // unified point coordinates [U] (targets, undisplaced sources)
   std::vector<double> m2m_hi, m2m_lo;   // [d+1][d+1]: parent basis a at child node k, A[a*(d+1)+k]
   std::vector<double> p2m;              // [d+1][m]: proxy basis a at particle node k
+  std::vector<double> l2p_hi, l2p_lo;   // [m][d+1]: parent basis a at particle node k of an upper/lower half
+                                        // child, l2p[k * (d+1) + a]
+  std::vector<double> anc_l2l;          // [2^(L-1)-2][d+1][d+1]: ancestor basis a at a descendant's node k,
+                                        // [k][a] in matrix 2^g - 2 + pos for gap g and interval position pos
   std::vector<int> zero_rows;           // proxy rows no pair kernel or last-level L2L writes
   std::vector<int> proxy_leaf;          // [offset*n] leaf containing each non-leaf proxy node
   std::vector<double> proxy_xieta;      // [offset*n][2] label coordinates of those nodes (radians)
