@@ -340,6 +340,7 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   size_t o_p2m = A.take<double>(t.dp1 * t.m), o_p2mT = A.take<double>(t.dp1 * t.m);
   size_t o_anc = A.take<double>(std::max<size_t>(t.anc_l2l.size(), 1));
   size_t o_l2phi = A.take<double>(t.l2p_hi.size()), o_l2plo = A.take<double>(t.l2p_lo.size());
+  size_t o_l2pThi = A.take<double>(t.l2pT_hi.size()), o_l2pTlo = A.take<double>(t.l2pT_lo.size());
   size_t o_pl = A.take<int>(t.proxy_leaf.size()), o_pxe = A.take<double>(t.proxy_xieta.size());
   size_t o_po = A.take<int>(t.proxy_order.size());
   size_t o_err = A.take<int>(4), o_zr = A.take<int>(t.zero_rows.size());
@@ -375,6 +376,8 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   rc |= upload(c, b + o_anc, t.anc_l2l.data(), t.anc_l2l.size() * sizeof(double));
   rc |= upload(c, b + o_l2phi, t.l2p_hi.data(), t.l2p_hi.size() * sizeof(double));
   rc |= upload(c, b + o_l2plo, t.l2p_lo.data(), t.l2p_lo.size() * sizeof(double));
+  rc |= upload(c, b + o_l2pThi, t.l2pT_hi.data(), t.l2pT_hi.size() * sizeof(double));
+  rc |= upload(c, b + o_l2pTlo, t.l2pT_lo.data(), t.l2pT_lo.size() * sizeof(double));
   rc |= upload(c, b + o_pl, t.proxy_leaf.data(), t.proxy_leaf.size() * sizeof(int));
   rc |= upload(c, b + o_pxe, t.proxy_xieta.data(), t.proxy_xieta.size() * sizeof(double));
   rc |= upload(c, b + o_po, t.proxy_order.data(), t.proxy_order.size() * sizeof(int));
@@ -404,6 +407,8 @@ int sbslpm_ctx_create(const sbslpm_params* p, sbslpm_ctx** out) {
   d.anc_l2l = (const double*)(b + o_anc);
   d.l2p_hi = (const double*)(b + o_l2phi);
   d.l2p_lo = (const double*)(b + o_l2plo);
+  d.l2pT_hi = (const double*)(b + o_l2pThi);
+  d.l2pT_lo = (const double*)(b + o_l2pTlo);
   d.proxy_leaf = (const int*)(b + o_pl); d.proxy_xieta = (const double*)(b + o_pxe);
   d.proxy_order = (const int*)(b + o_po);
   d.logtab = (const LogTabEntry*)(b + o_ltab);
@@ -1846,7 +1851,7 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
   d.m2m_lo = (const double*)(b + o_lo);
   d.m2m_hiT = d.m2m_loT = nullptr;
   d.p2m = d.p2mT = nullptr;
-  d.anc_l2l = d.l2p_hi = d.l2p_lo = nullptr;
+  d.anc_l2l = d.l2p_hi = d.l2p_lo = d.l2pT_hi = d.l2pT_lo = nullptr;
   d.proxy_leaf = nullptr; d.proxy_xieta = nullptr; d.proxy_order = nullptr;
   d.P = t.P; d.Pl = (int)leaves.size(); d.offset = t.P;  // every proxy row may be an interaction source
   d.n = t.n; d.dp1 = t.dp1; d.m = 0; d.m2 = 0; d.levels = t.levels;

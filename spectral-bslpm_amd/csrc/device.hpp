@@ -33,6 +33,8 @@ struct DevTree {
   const double* anc_l2l;  // composite L2L matrices (tree.hpp), or null: no fused downward pass
   const double* l2p_hi;   // [m][d+1] parent basis at the particle nodes of an upper-half child
   const double* l2p_lo;
+  const double* l2pT_hi;  // [d+1][m] transposes
+  const double* l2pT_lo;
   const int* proxy_leaf;      // [offset*n]
   const double* proxy_xieta;  // [offset*n][2]
   const int* proxy_order;     // [offset*n] proxy nodes sorted by leaf

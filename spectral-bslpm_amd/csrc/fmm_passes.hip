@@ -530,6 +530,89 @@ bool launch_down_fused(const DevTree& t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 
          launch_down_fused_dm<11, 8>(t, ncomp, phi, part, out, s, nmem);
 }
 
+// k_p2m_m2m for a proxy grid of D x D nodes and a leaf grid of M x M
+// particles known at compile time, tiled for registers like k_down_fused.
+// The P2M of each leaf and the M2M into the parent compose to one product per
+// direction: the parent's charge Q[a][b] = sum_c sum_kl Zx_c[k][a] Zy_c[l][b]
+// q_c[k][l], Z = l2p_hi / l2p_lo the parent basis at the particles of an
+// upper / lower half child (used here through the transposes l2pT, rows
+// contiguous in the particle index).
+//   - all threads: q_part = area * pots for the four leaves, staged in LDS;
+//   - wave c: lanes k < M form U_c[k][b] = sum_l q_c[k][l] Zy_c[l][b];
+//   - wave w: lanes b < D form Q[a][b] for a = w, w + 4, ..., summing leaves
+//     0..3 in order.
+template <int D, int M>
+__global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const double* __restrict__ area,
+                                                  const double* __restrict__ pots, double* __restrict__ q_part,
+                                                  double* __restrict__ q) {
+  constexpr int N = D * D, M2 = M * M;
+  __shared__ double sq[4 * M2];
+  __shared__ double sU[4][M * D];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int par = pbegin + blockIdx.x;
+  const int64_t leaf0 = 4 * (int64_t)blockIdx.x;  // the parents' children are the leaves 4i .. 4i+3
+  pots += member_part(t);
+  q_part += member_part(t);
+  q += member_proxy(t);
+  for (int i = threadIdx.x; i < 4 * M2; i += blockDim.x) {
+    const int64_t u = leaf0 * M2 + i;
+    const double v = area[u] * pots[u];
+    q_part[u] = v;
+    sq[i] = v;
+  }
+  __syncthreads();
+  if (lane < M) {  // leaf c = w, row k = lane: U_c[k][b] = sum_l Zy[b][l] q_c[k][l]
+    const ConstTab Zy = const_tab(w <= 1 ? t.l2pT_hi : t.l2pT_lo);
+    double r[M];
+#pragma unroll
+    for (int l = 0; l < M; ++l) r[l] = sq[w * M2 + lane * M + l];
+#pragma unroll 1
+    for (int b = 0; b < D; ++b) {
+      double v = 0.0;
+#pragma unroll
+      for (int l = 0; l < M; ++l) v += Zy[b * M + l] * r[l];
+      sU[w][lane * D + b] = v;
+    }
+  }
+  __syncthreads();
+  if (lane < D) {  // column b = lane: Q[a][b] = sum_c sum_k Zx_c[a][k] U_c[k][b]
+    double u[4][M];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int k = 0; k < M; ++k) u[c][k] = sU[c][k * D + lane];
+    const ConstTab Zhi = const_tab(t.l2pT_hi), Zlo = const_tab(t.l2pT_lo);
+#pragma unroll 1
+    for (int a = w; a < D; a += 4) {
+      double v = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const ConstTab Zx = (c == 0 || c == 3) ? Zhi : Zlo;
+#pragma unroll
+        for (int k = 0; k < M; ++k) v += Zx[a * M + k] * u[c][k];
+      }
+      q[(int64_t)par * N + a * D + lane] = v;
+    }
+  }
+}
+
+template <int D, int M>
+bool launch_up_fused_dm(const DevTree& t, int pbegin, int count, const double* area, const double* pots,
+                        double* q_part, double* q, hipStream_t s, int nmem) {
+  if (t.dp1 != D || t.m != M || !t.l2pT_hi) return false;
+  hipLaunchKernelGGL((k_up_fused<D, M>), dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q);
+  return true;
+}
+
+bool launch_up_fused(const DevTree& t, int pbegin, int count, const double* area, const double* pots, double* q_part,
+                     double* q, hipStream_t s, int nmem) {
+#define SBSLPM_UP(D, M) launch_up_fused_dm<D, M>(t, pbegin, count, area, pots, q_part, q, s, nmem)
+  return SBSLPM_UP(9, 8) || SBSLPM_UP(9, 9) || SBSLPM_UP(6, 6) || SBSLPM_UP(5, 5) || SBSLPM_UP(5, 6) ||
+         SBSLPM_UP(6, 7) || SBSLPM_UP(7, 7) || SBSLPM_UP(11, 11) || SBSLPM_UP(11, 8);
+#undef SBSLPM_UP
+}
+
 // Dynamic LDS of k_l2l_l2p (bytes).
 size_t l2l_l2p_lds(const DevTree& t, int ncomp) {
   const size_t n = t.n, D = t.dp1, m = t.m, nc = ncomp * n;
@@ -641,7 +724,8 @@ void launch_upward_fused(const DevTree& t, const double* area, const double* pot
   {
     const int l = t.levels - 2;
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
+    if (!launch_up_fused(t, pbegin, count, area, pots, q_part, q_proxy, s, nmem))
+      hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
   }
   launch_m2m_levels(t, t.levels - 3, q_proxy, s, nmem);
 }

@@ -365,11 +365,13 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
   // children (m2m_hi p2m and m2m_lo p2m, evaluated directly), stored [k][a].
   t.l2p_hi.assign(t.dp1 * t.m, 0.0);
   t.l2p_lo.assign(t.dp1 * t.m, 0.0);
+  t.l2pT_hi.assign(t.dp1 * t.m, 0.0);
+  t.l2pT_lo.assign(t.dp1 * t.m, 0.0);
   for (int k = 0; k < t.m; ++k) {
     bli_weights_1d(0.5 * (t.node_m[k] + 1.0), -1.0, 1.0, d, t.cheb_d.data(), w.data());
-    for (int a = 0; a <= d; ++a) t.l2p_hi[k * t.dp1 + a] = w[a];
+    for (int a = 0; a <= d; ++a) t.l2p_hi[k * t.dp1 + a] = t.l2pT_hi[a * t.m + k] = w[a];
     bli_weights_1d(0.5 * (t.node_m[k] - 1.0), -1.0, 1.0, d, t.cheb_d.data(), w.data());
-    for (int a = 0; a <= d; ++a) t.l2p_lo[k * t.dp1 + a] = w[a];
+    for (int a = 0; a <= d; ++a) t.l2p_lo[k * t.dp1 + a] = t.l2pT_lo[a * t.m + k] = w[a];
   }
   // Composite transfer matrices for the fused downward pass: the basis of an
   // ancestor g levels above a panel at that panel's nodes, one 1-D matrix per

@@ -85,6 +85,7 @@ struct Tree {
   std::vector<double> p2m;              // [d+1][m]: proxy basis a at particle node k
   std::vector<double> l2p_hi, l2p_lo;   // [m][d+1]: parent basis a at particle node k of an upper/lower half
                                         // child, l2p[k * (d+1) + a]
+  std::vector<double> l2pT_hi, l2pT_lo; // their transposes [d+1][m]
   std::vector<double> anc_l2l;          // [2^(L-1)-2][d+1][d+1]: ancestor basis a at a descendant's node k,
                                         // [k][a] in matrix 2^g - 2 + pos for gap g and interval position pos
   std::vector<int> zero_rows;           // proxy rows no pair kernel or last-level L2L writes
