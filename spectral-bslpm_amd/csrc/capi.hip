@@ -619,20 +619,29 @@ int sbslpm_bve_vel_interactions(sbslpm_ctx* c, const double* dpx, const double* 
   return launched(c);
 }
 
-int sbslpm_poisson_fmm_interactions(sbslpm_ctx* c, const double* q_part, const double* q_proxy, double* psi_part,
-                                    double* psi_proxy) {
-  if (!c || !q_part || !q_proxy || !psi_part || !psi_proxy)
-    return fail(c, SBSLPM_E_ARG, "poisson_fmm_interactions: null pointer");
+// Source records and the log pair sums over the full list, accumulated into
+// the target rows or (assign) written over them.
+static void poisson_pairs(sbslpm_ctx* c, const double* q_part, const double* q_proxy, double* psi_part,
+                          double* psi_proxy, bool assign) {
   if (c->prm.precision != SBSLPM_FP64) {
     launch_source_records32(c->dt, q_part, q_proxy, nullptr, nullptr, c->rec32, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_LOG_MIXED);
-    launch_poisson_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, psi_part, psi_proxy, c->stream);
+    launch_poisson_interact_mixed(c->dt, c->tiles, c->ntiles, c->sets, c->rec32, psi_part, psi_proxy, c->stream,
+                                  assign);
   } else {
     // the Poisson kernel reads the node coordinates unprojected (inverse_poisson_impl.hpp:42-46)
     launch_source_records64(c->dt, q_part, q_proxy, nullptr, nullptr, false, c->rec64, c->stream);
     KernelTimer tm(c, SBSLPM_KERNEL_LOG_FP64);
-    launch_poisson_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, psi_part, psi_proxy, c->stream);
+    launch_poisson_interact64(c->dt, c->tiles, c->ntiles, c->sets, c->rec64, psi_part, psi_proxy, c->stream,
+                              assign);
   }
+}
+
+int sbslpm_poisson_fmm_interactions(sbslpm_ctx* c, const double* q_part, const double* q_proxy, double* psi_part,
+                                    double* psi_proxy) {
+  if (!c || !q_part || !q_proxy || !psi_part || !psi_proxy)
+    return fail(c, SBSLPM_E_ARG, "poisson_fmm_interactions: null pointer");
+  poisson_pairs(c, q_part, q_proxy, psi_part, psi_proxy, false);
   return launched(c);
 }
 
@@ -667,13 +676,20 @@ int sbslpm_bve_velocity(sbslpm_ctx* c, const double* area, const double* vor, co
 
 int sbslpm_poisson_solve(sbslpm_ctx* c, const double* area, const double* f, double* psi) {
   if (!c || !area || !f || !psi) return fail(c, SBSLPM_E_ARG, "poisson_solve: null pointer");
-  const Tree& t = c->tree;
-  launch_upward_pass(c->dt, area, f, c->qpart, c->qproxy, c->stream);
-  launch_fill_zero(c->phipart[0], t.npart, c->stream);
-  launch_fill_zero(c->phiproxy[0], (int64_t)t.P * t.n, c->stream);
-  int rc = sbslpm_poisson_fmm_interactions(c, c->qpart, c->qproxy, c->phipart[0], c->phiproxy[0]);
-  if (rc) return rc;
-  return sbslpm_downward_pass(c, c->phiproxy[0], c->phipart[0], psi);
+  // The reference's sequence (upward pass, interactions into zeroed rows,
+  // downward pass) with the fused passes: the leaf proxy rows are neither
+  // interaction sources nor targets, so the upward pass leaves them unwritten
+  // and the downward pass goes from the non-leaf rows to the particles; the
+  // pair sums assign their target rows and the non-leaf rows without tiles
+  // are zeroed.
+  launch_upward_fused(c->dt, area, f, c->qpart, c->qproxy, c->stream);
+  launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, 1, c->phiproxy, c->stream);
+  poisson_pairs(c, c->qpart, c->qproxy, c->phipart[0], c->phiproxy[0], true);
+  double* px[1] = {c->phiproxy[0]};
+  const double* pp[1] = {c->phipart[0]};
+  double* so[1] = {psi};
+  launch_downward_pass(c->dt, 1, px, pp, so, c->stream, true);
+  return launched(c);
 }
 
 // Shallow-water workspace: records [U] and the divergence charges.
