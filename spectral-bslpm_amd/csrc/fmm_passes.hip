@@ -530,17 +530,59 @@ bool launch_down_fused(const DevTree& t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 
          launch_down_fused_dm<11, 8>(t, ncomp, phi, part, out, s, nmem);
 }
 
+// Charge of a parent from its four children's M x M grids (row-major in LDS,
+// sq[c * M * M + k * M + l]): Q[a][b] = sum_c sum_kl X_c[a][k] Y_c[b][l]
+// q_c[k][l], with X_c = hi for children 0 and 3, Y_c = hi for children 0 and 1
+// (quadrants (hi,hi), (lo,hi), (lo,lo), (hi,lo) in (xi, eta)) and the tables
+// laid out [basis a][child node k], rows contiguous in k:
+//   - wave c: lanes k < M form U_c[k][b] = sum_l Y_c[b][l] q_c[k][l];
+//   - wave w: lanes b < D form Q[a][b] for a = w, w + 4, ..., summing the
+//     children in order, and write row a of the parent.
+// Tiled for registers like k_down_fused; every thread of the workgroup calls it.
+template <int D, int M>
+__device__ inline void parent_from_children(const double* sq, ConstTab Thi, ConstTab Tlo, double (*sU)[M * D],
+                                            double* __restrict__ qrow, int lane, int w) {
+  constexpr int M2 = M * M;
+  if (lane < M) {
+    const ConstTab Y = w <= 1 ? Thi : Tlo;
+    double r[M];
+#pragma unroll
+    for (int l = 0; l < M; ++l) r[l] = sq[w * M2 + lane * M + l];
+#pragma unroll 1
+    for (int b = 0; b < D; ++b) {
+      double v = 0.0;
+#pragma unroll
+      for (int l = 0; l < M; ++l) v += Y[b * M + l] * r[l];
+      sU[w][lane * D + b] = v;
+    }
+  }
+  __syncthreads();
+  if (lane < D) {
+    double u[4][M];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int k = 0; k < M; ++k) u[c][k] = sU[c][k * D + lane];
+#pragma unroll 1
+    for (int a = w; a < D; a += 4) {
+      double v = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const ConstTab X = (c == 0 || c == 3) ? Thi : Tlo;
+#pragma unroll
+        for (int k = 0; k < M; ++k) v += X[a * M + k] * u[c][k];
+      }
+      qrow[a * D + lane] = v;
+    }
+  }
+}
+
 // k_p2m_m2m for a proxy grid of D x D nodes and a leaf grid of M x M
-// particles known at compile time, tiled for registers like k_down_fused.
-// The P2M of each leaf and the M2M into the parent compose to one product per
-// direction: the parent's charge Q[a][b] = sum_c sum_kl Zx_c[k][a] Zy_c[l][b]
-// q_c[k][l], Z = l2p_hi / l2p_lo the parent basis at the particles of an
-// upper / lower half child (used here through the transposes l2pT, rows
-// contiguous in the particle index).
-//   - all threads: q_part = area * pots for the four leaves, staged in LDS;
-//   - wave c: lanes k < M form U_c[k][b] = sum_l q_c[k][l] Zy_c[l][b];
-//   - wave w: lanes b < D form Q[a][b] for a = w, w + 4, ..., summing leaves
-//     0..3 in order.
+// particles known at compile time.  The P2M of each leaf and the M2M into
+// the parent compose to one product per direction with l2p_hi / l2p_lo, the
+// parent basis at the particles of an upper / lower half child (through their
+// transposes l2pT): q_part = area * pots for the four leaves is written and
+// staged in LDS, then parent_from_children.
 template <int D, int M>
 __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const double* __restrict__ area,
                                                   const double* __restrict__ pots, double* __restrict__ q_part,
@@ -562,39 +604,36 @@ __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const d
     sq[i] = v;
   }
   __syncthreads();
-  if (lane < M) {  // leaf c = w, row k = lane: U_c[k][b] = sum_l Zy[b][l] q_c[k][l]
-    const ConstTab Zy = const_tab(w <= 1 ? t.l2pT_hi : t.l2pT_lo);
-    double r[M];
-#pragma unroll
-    for (int l = 0; l < M; ++l) r[l] = sq[w * M2 + lane * M + l];
-#pragma unroll 1
-    for (int b = 0; b < D; ++b) {
-      double v = 0.0;
-#pragma unroll
-      for (int l = 0; l < M; ++l) v += Zy[b * M + l] * r[l];
-      sU[w][lane * D + b] = v;
-    }
-  }
+  parent_from_children<D, M>(sq, const_tab(t.l2pT_hi), const_tab(t.l2pT_lo), sU, q + (int64_t)par * N, lane, w);
+}
+
+// k_m2m (one level, parents pbegin ..) in the same tiled form, M = D.
+template <int D>
+__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, double* __restrict__ q) {
+  constexpr int N = D * D;
+  __shared__ double sq[4 * N];
+  __shared__ double sU[4][N];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int par = pbegin + blockIdx.x;
+  const int c0 = t.panels[par].child0;
+  q += member_proxy(t);
+  for (int i = threadIdx.x; i < 4 * N; i += blockDim.x) sq[i] = q[(int64_t)c0 * N + i];
   __syncthreads();
-  if (lane < D) {  // column b = lane: Q[a][b] = sum_c sum_k Zx_c[a][k] U_c[k][b]
-    double u[4][M];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int k = 0; k < M; ++k) u[c][k] = sU[c][k * D + lane];
-    const ConstTab Zhi = const_tab(t.l2pT_hi), Zlo = const_tab(t.l2pT_lo);
-#pragma unroll 1
-    for (int a = w; a < D; a += 4) {
-      double v = 0.0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const ConstTab Zx = (c == 0 || c == 3) ? Zhi : Zlo;
-#pragma unroll
-        for (int k = 0; k < M; ++k) v += Zx[a * M + k] * u[c][k];
-      }
-      q[(int64_t)par * N + a * D + lane] = v;
-    }
-  }
+  parent_from_children<D, D>(sq, const_tab(t.m2m_hi), const_tab(t.m2m_lo), sU, q + (int64_t)par * N, lane, w);
+}
+
+template <int D>
+bool launch_m2m_tiled_d(const DevTree& t, int pbegin, int count, double* q, hipStream_t s, int nmem) {
+  if (t.dp1 != D) return false;
+  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, nmem), dim3(256), 0, s, t, pbegin, q);
+  return true;
+}
+
+bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, double* q, hipStream_t s, int nmem) {
+  return launch_m2m_tiled_d<9>(t, pbegin, count, q, s, nmem) || launch_m2m_tiled_d<6>(t, pbegin, count, q, s, nmem) ||
+         launch_m2m_tiled_d<5>(t, pbegin, count, q, s, nmem) || launch_m2m_tiled_d<7>(t, pbegin, count, q, s, nmem) ||
+         launch_m2m_tiled_d<11>(t, pbegin, count, q, s, nmem);
 }
 
 template <int D, int M>
@@ -696,7 +735,8 @@ void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
 static void launch_m2m_levels(const DevTree& t, int lfirst, double* q_proxy, hipStream_t s, int nmem) {
   for (int l = lfirst; l >= 0; --l) {
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
+    if (!launch_m2m_tiled(t, pbegin, count, q_proxy, s, nmem))
+      hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
   }
 }
 
