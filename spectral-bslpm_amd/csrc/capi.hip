@@ -767,8 +767,7 @@ int sbslpm_swe_velocity(sbslpm_ctx* c, int variant, const double* area, const do
     return fail(c, SBSLPM_E_ARG, "swe_velocity: bad argument");
   int rc = ensure_swe(c);
   if (rc) return rc;
-  launch_upward_fused(c->dt, area, vor, c->qpart, c->qproxy, c->stream);
-  launch_upward_fused(c->dt, area, div, c->qdpart, c->qdproxy, c->stream);
+  launch_upward_fused2(c->dt, area, vor, c->qpart, c->qproxy, div, c->qdpart, c->qdproxy, c->stream);
   launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, 3, c->phiproxy, c->stream);
   rc = swe_interactions(c, variant, dpx, dpy, dpz, dxx, dxy, dxz, c->qpart, c->qproxy, c->qdpart, c->qdproxy,
                         c->phipart, c->phiproxy, true);
@@ -1458,8 +1457,7 @@ static int swe_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* vo
   adjoint_anterpolate(c, vbar, 3);
   const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
   launch_swe_adj_target_records(c->dt, fp32, vbar, psix, a.trec, c->stream);
-  launch_upward_fused(c->dt, area_s, vor_s, c->qpart, c->qproxy, c->stream);
-  launch_upward_fused(c->dt, area_s, div_s, c->qdpart, c->qdproxy, c->stream);
+  launch_upward_fused2(c->dt, area_s, vor_s, c->qpart, c->qproxy, div_s, c->qdpart, c->qdproxy, c->stream);
   launch_source_records_swe(c->dt, 1, c->qpart, c->qproxy, c->qdpart, c->qdproxy, dpart, dprox, c->recswe, c->stream);
   double* qx2[2] = {a.qzx, a.qdx};
   launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 2, qx2, c->stream);

@@ -582,11 +582,14 @@ __device__ inline void parent_from_children(const double* sq, ConstTab Thi, Cons
 // the parent compose to one product per direction with l2p_hi / l2p_lo, the
 // parent basis at the particles of an upper / lower half child (through their
 // transposes l2pT): q_part = area * pots for the four leaves is written and
-// staged in LDS, then parent_from_children.
+// staged in LDS, then parent_from_children.  With pots2 set, gridDim.y is 2
+// and blockIdx.y 1 takes the second charge (pots2, q_part2, q2) instead of a
+// member slice.
 template <int D, int M>
 __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const double* __restrict__ area,
                                                   const double* __restrict__ pots, double* __restrict__ q_part,
-                                                  double* __restrict__ q) {
+                                                  double* __restrict__ q, const double* __restrict__ pots2,
+                                                  double* __restrict__ q_part2, double* __restrict__ q2) {
   constexpr int N = D * D, M2 = M * M;
   __shared__ double sq[4 * M2];
   __shared__ double sU[4][M * D];
@@ -594,9 +597,15 @@ __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const d
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int par = pbegin + blockIdx.x;
   const int64_t leaf0 = 4 * (int64_t)blockIdx.x;  // the parents' children are the leaves 4i .. 4i+3
-  pots += member_part(t);
-  q_part += member_part(t);
-  q += member_proxy(t);
+  if (pots2 && blockIdx.y == 1) {
+    pots = pots2;
+    q_part = q_part2;
+    q = q2;
+  } else {
+    pots += member_part(t);
+    q_part += member_part(t);
+    q += member_proxy(t);
+  }
   for (int i = threadIdx.x; i < 4 * M2; i += blockDim.x) {
     const int64_t u = leaf0 * M2 + i;
     const double v = area[u] * pots[u];
@@ -607,9 +616,11 @@ __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const d
   parent_from_children<D, M>(sq, const_tab(t.l2pT_hi), const_tab(t.l2pT_lo), sU, q + (int64_t)par * N, lane, w);
 }
 
-// k_m2m (one level, parents pbegin ..) in the same tiled form, M = D.
+// k_m2m (one level, parents pbegin ..) in the same tiled form, M = D; with q2
+// set, blockIdx.y 1 takes q2 as k_up_fused does.
 template <int D>
-__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, double* __restrict__ q) {
+__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, double* __restrict__ q,
+                                                   double* __restrict__ q2) {
   constexpr int N = D * D;
   __shared__ double sq[4 * N];
   __shared__ double sU[4][N];
@@ -617,36 +628,41 @@ __global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, double
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int par = pbegin + blockIdx.x;
   const int c0 = t.panels[par].child0;
-  q += member_proxy(t);
+  q = (q2 && blockIdx.y == 1) ? q2 : q + member_proxy(t);
   for (int i = threadIdx.x; i < 4 * N; i += blockDim.x) sq[i] = q[(int64_t)c0 * N + i];
   __syncthreads();
   parent_from_children<D, D>(sq, const_tab(t.m2m_hi), const_tab(t.m2m_lo), sU, q + (int64_t)par * N, lane, w);
 }
 
 template <int D>
-bool launch_m2m_tiled_d(const DevTree& t, int pbegin, int count, double* q, hipStream_t s, int nmem) {
+bool launch_m2m_tiled_d(const DevTree& t, int pbegin, int count, double* q, double* q2, hipStream_t s, int nmem) {
   if (t.dp1 != D) return false;
-  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, nmem), dim3(256), 0, s, t, pbegin, q);
+  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, q2 ? 2 : nmem), dim3(256), 0, s, t, pbegin, q, q2);
   return true;
 }
 
-bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, double* q, hipStream_t s, int nmem) {
-  return launch_m2m_tiled_d<9>(t, pbegin, count, q, s, nmem) || launch_m2m_tiled_d<6>(t, pbegin, count, q, s, nmem) ||
-         launch_m2m_tiled_d<5>(t, pbegin, count, q, s, nmem) || launch_m2m_tiled_d<7>(t, pbegin, count, q, s, nmem) ||
-         launch_m2m_tiled_d<11>(t, pbegin, count, q, s, nmem);
+// The tiled M2M for the degrees in use; false for the others.  q2: a second
+// charge array in the same launch (nmem is then 1).
+bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, double* q, double* q2, hipStream_t s, int nmem) {
+#define SBSLPM_M2M(D) launch_m2m_tiled_d<D>(t, pbegin, count, q, q2, s, nmem)
+  return SBSLPM_M2M(9) || SBSLPM_M2M(6) || SBSLPM_M2M(5) || SBSLPM_M2M(7) || SBSLPM_M2M(11);
+#undef SBSLPM_M2M
 }
 
 template <int D, int M>
 bool launch_up_fused_dm(const DevTree& t, int pbegin, int count, const double* area, const double* pots,
-                        double* q_part, double* q, hipStream_t s, int nmem) {
+                        double* q_part, double* q, const double* pots2, double* q_part2, double* q2, hipStream_t s,
+                        int nmem) {
   if (t.dp1 != D || t.m != M || !t.l2pT_hi) return false;
-  hipLaunchKernelGGL((k_up_fused<D, M>), dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q);
+  hipLaunchKernelGGL((k_up_fused<D, M>), dim3(count, pots2 ? 2 : nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part,
+                     q, pots2, q_part2, q2);
   return true;
 }
 
 bool launch_up_fused(const DevTree& t, int pbegin, int count, const double* area, const double* pots, double* q_part,
-                     double* q, hipStream_t s, int nmem) {
-#define SBSLPM_UP(D, M) launch_up_fused_dm<D, M>(t, pbegin, count, area, pots, q_part, q, s, nmem)
+                     double* q, const double* pots2, double* q_part2, double* q2, hipStream_t s, int nmem) {
+#define SBSLPM_UP(D, M) \
+  launch_up_fused_dm<D, M>(t, pbegin, count, area, pots, q_part, q, pots2, q_part2, q2, s, nmem)
   return SBSLPM_UP(9, 8) || SBSLPM_UP(9, 9) || SBSLPM_UP(6, 6) || SBSLPM_UP(5, 5) || SBSLPM_UP(5, 6) ||
          SBSLPM_UP(6, 7) || SBSLPM_UP(7, 7) || SBSLPM_UP(11, 11) || SBSLPM_UP(11, 8);
 #undef SBSLPM_UP
@@ -732,11 +748,14 @@ void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
 // barrier between them: that kernel's waves wait on a chain of dependent
 // loads with too few waves per compute unit to hide them (178 us against
 // about 30 us for the four launches it replaced at 6 x 512^2).
-static void launch_m2m_levels(const DevTree& t, int lfirst, double* q_proxy, hipStream_t s, int nmem) {
+static void launch_m2m_levels(const DevTree& t, int lfirst, double* q_proxy, hipStream_t s, int nmem,
+                              double* q_proxy2 = nullptr) {
   for (int l = lfirst; l >= 0; --l) {
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    if (!launch_m2m_tiled(t, pbegin, count, q_proxy, s, nmem))
-      hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
+    if (launch_m2m_tiled(t, pbegin, count, q_proxy, q_proxy2, s, nmem)) continue;
+    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
+    if (q_proxy2)
+      hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy2);
   }
 }
 
@@ -755,19 +774,31 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
 
 // Upward pass with base_pots, P2M and the first M2M level fused: q_part and
 // the non-leaf rows of q_proxy are assigned, the leaf rows are not written.
-void launch_upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
-                         hipStream_t s, int nmem) {
+static void upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
+                         const double* pots2, double* q_part2, double* q_proxy2, hipStream_t s, int nmem) {
   if (t.levels < 2) {
     launch_upward_pass(t, area, pots, q_part, q_proxy, s);
+    if (pots2) launch_upward_pass(t, area, pots2, q_part2, q_proxy2, s);
     return;
   }
-  {
-    const int l = t.levels - 2;
-    const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    if (!launch_up_fused(t, pbegin, count, area, pots, q_part, q_proxy, s, nmem))
-      hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
+  const int l = t.levels - 2;
+  const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
+  if (!launch_up_fused(t, pbegin, count, area, pots, q_part, q_proxy, pots2, q_part2, q_proxy2, s, nmem)) {
+    hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
+    if (pots2)
+      hipLaunchKernelGGL(k_p2m_m2m, dim3(count, 1), dim3(256), 0, s, t, pbegin, area, pots2, q_part2, q_proxy2);
   }
-  launch_m2m_levels(t, t.levels - 3, q_proxy, s, nmem);
+  launch_m2m_levels(t, t.levels - 3, q_proxy, s, nmem, q_proxy2);
+}
+
+void launch_upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
+                         hipStream_t s, int nmem) {
+  upward_fused(t, area, pots, q_part, q_proxy, nullptr, nullptr, nullptr, s, nmem);
+}
+
+void launch_upward_fused2(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
+                          const double* pots2, double* q_part2, double* q_proxy2, hipStream_t s) {
+  upward_fused(t, area, pots, q_part, q_proxy, pots2, q_part2, q_proxy2, s, 1);
 }
 
 // fmm_funcs.cpp:357-392.  phi_proxy rows are accumulated top-down; sols are

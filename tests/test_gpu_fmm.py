@@ -167,3 +167,36 @@ def test_rh4_closed_form_on_gpu(sb, torch_gpu):
     vel = ctx.bve_velocity(to_gpu(torch_gpu, a), to_gpu(torch_gpu, W.rh4_vorticity(x, y, z)))
     ctx.synchronize()
     assert W.vec_rel_l2([to_np(v) for v in vel], W.rh4_velocity(x, y, z)) < 0.01
+
+
+@pytest.mark.parametrize("levels,deg,leaf", [(6, 8, 7), (5, 5, 5), (5, 6, 4)])
+def test_fused_passes_match_level_by_level(sb, torch_gpu, levels, deg, leaf):
+    """bve_velocity and poisson_solve run the fused passes: one upward launch for the P2M and
+    the first M2M level, and one downward launch for every L2L level and the L2P, with each
+    ancestor's expansion evaluated at the leaves' parent through a composite matrix.  The
+    C-ABI's upward_pass, interactions and downward_pass run the reference's passes level by
+    level (pinned against the oracle above).  With the same pair kernels the two may differ
+    only by the passes' rounding: relative 1e-14.  (8, 7) and (5, 5) take the register-tiled
+    kernels, (6, 4) the generic ones."""
+    t = torch_gpu
+    print(f"\n[case fused passes L={levels} d={deg} m={leaf + 1}]", flush=True)
+    ctx = sb.Context(sb.RunConfig(levels=levels, interp_degree=deg, leaf_degree=leaf, fmm_theta=0.7,
+                                  weights="clenshaw_curtis", kernel_eps=EPS))
+    x, y, z, a = ctx.spec_points()
+    vor = sb.workloads.gaussian_vortices(x, y, z, a, seed=3)
+    ga, gv = to_gpu(t, a), to_gpu(t, vor)
+    fused = [to_np(v) for v in ctx.bve_velocity(ga, gv)]
+    qp, qx = ctx.upward_pass(ga, gv)
+    pp, px = ctx.bve_vel_interactions(qp, qx)
+    by_level = [to_np(v) for v in ctx.downward_pass_3(px, pp)]
+    ctx.synchronize()
+    err = sb.workloads.vec_rel_l2(fused, by_level)
+    psi_f = to_np(ctx.poisson_solve(ga, gv))
+    qp, qx = ctx.upward_pass(ga, gv)
+    sp, sx = ctx.poisson_fmm_interactions(qp, qx)
+    psi_l = to_np(ctx.downward_pass(sx, sp))
+    ctx.synchronize()
+    err_psi = rel(psi_f, psi_l)
+    print(f"  velocity {err:.2e}, psi {err_psi:.2e}")
+    assert err < 1e-14 and err_psi < 1e-14
+    assert np.abs(fused[0]).max() > 0 and np.abs(psi_f).max() > 0
