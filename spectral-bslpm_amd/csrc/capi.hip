@@ -767,7 +767,8 @@ int sbslpm_swe_velocity(sbslpm_ctx* c, int variant, const double* area, const do
     return fail(c, SBSLPM_E_ARG, "swe_velocity: bad argument");
   int rc = ensure_swe(c);
   if (rc) return rc;
-  launch_upward_fused2(c->dt, area, vor, c->qpart, c->qproxy, div, c->qdpart, c->qdproxy, c->stream);
+  const UpCharge vd[2] = {{area, vor, c->qpart, c->qproxy}, {area, div, c->qdpart, c->qdproxy}};
+  launch_upward_sets(c->dt, vd, 2, c->stream);
   launch_zero_rows(c->zero_rows, c->nzero_rows, c->tree.n, 3, c->phiproxy, c->stream);
   rc = swe_interactions(c, variant, dpx, dpy, dpz, dxx, dxy, dxz, c->qpart, c->qproxy, c->qdpart, c->qdproxy,
                         c->phipart, c->phiproxy, true);
@@ -1423,24 +1424,11 @@ static int ensure_swe_adjoint(sbslpm_ctx* c) {
 
 // Anterpolation of a particle cotangent with unit weights, the first step of
 // every transposed FMM: adj_psip[f] = vbar[f] and the non-leaf rows of
-// adj_psix[f] (no target-record kernel reads the leaf rows).  The three
-// components of one array of 3 x npart (consecutive) go through one batched
-// launch sequence (gridDim.y = 3; adj_psip / adj_psix are laid out that way).
-static void adjoint_anterpolate(sbslpm_ctx* c, const double* const* vbar, int ncomp) {
-  const Tree& t = c->tree;
-  if (t.cfg.levels < 2) {
-    for (int f = 0; f < ncomp; ++f)
-      launch_upward_pass(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream);
-    return;
-  }
-  bool packed = ncomp == 3;
-  for (int f = 1; f < ncomp && packed; ++f) packed = vbar[f] == vbar[0] + f * t.npart;
-  if (packed) {
-    launch_upward_fused(c->dt, c->adj_ones, vbar[0], c->adj_psip[0], c->adj_psix[0], c->stream, 3);
-    return;
-  }
-  for (int f = 0; f < ncomp; ++f)
-    launch_upward_fused(c->dt, c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f], c->stream, 1);
+// adj_psix[f] (no target-record kernel reads the leaf rows), as upward-pass
+// charge sets, so that callers launch them together with their charges'
+// upward passes (launch_upward_sets).
+static void anterpolation_sets(sbslpm_ctx* c, const double* const* vbar, int ncomp, UpCharge* up) {
+  for (int f = 0; f < ncomp; ++f) up[f] = {c->adj_ones, vbar[f], c->adj_psip[f], c->adj_psix[f]};
 }
 
 // Exact transpose of the shallow-water velocity evaluation (sbslpm_swe_velocity,
@@ -1454,10 +1442,15 @@ static int swe_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* vo
   const Tree& t = c->tree;
   auto& a = c->swa;
   const bool pos = dpart != nullptr, fp32 = c->prm.precision != SBSLPM_FP64;
-  adjoint_anterpolate(c, vbar, 3);
+  {  // the cotangent's anterpolation and the charges' upward pass share their launches
+    UpCharge up[5];
+    anterpolation_sets(c, vbar, 3, up);
+    up[3] = {area_s, vor_s, c->qpart, c->qproxy};
+    up[4] = {area_s, div_s, c->qdpart, c->qdproxy};
+    launch_upward_sets(c->dt, up, 5, c->stream);
+  }
   const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
   launch_swe_adj_target_records(c->dt, fp32, vbar, psix, a.trec, c->stream);
-  launch_upward_fused2(c->dt, area_s, vor_s, c->qpart, c->qproxy, div_s, c->qdpart, c->qdproxy, c->stream);
   launch_source_records_swe(c->dt, 1, c->qpart, c->qproxy, c->qdpart, c->qdproxy, dpart, dprox, c->recswe, c->stream);
   double* qx2[2] = {a.qzx, a.qdx};
   launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 2, qx2, c->stream);
@@ -1480,9 +1473,11 @@ static int lap_fmm_adjoint(sbslpm_ctx* c, const double* area_s, const double* ef
   auto& a = c->swa;
   auto& w = c->sws;
   const bool pos = dpart != nullptr, fp32 = c->prm.precision != SBSLPM_FP64;
-  adjoint_anterpolate(c, &lbar, 1);
+  UpCharge up[2];
+  anterpolation_sets(c, &lbar, 1, up);
+  up[1] = {area_s, eff, w.qp, w.qx};
+  launch_upward_sets(c->dt, up, 2, c->stream);  // the cotangent's anterpolation and the charge's upward pass
   launch_lap_adj_target_records(c->dt, fp32, lbar, c->adj_psix[0], a.trec, c->stream);
-  launch_upward_fused(c->dt, area_s, eff, w.qp, w.qx, c->stream);
   launch_source_records64(c->dt, w.qp, w.qx, dpart, dprox, true, a.srec, c->stream);
   launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 1, &a.qex, c->stream);
   if (pos) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, a.dbx, c->stream);
@@ -2334,7 +2329,12 @@ static int fmm_adjoint(sbslpm_ctx* c, const double* area, const double* vor_eff,
                 const double* const* dprox, const double* const* vbar) {
   const Tree& t = c->tree;
   const bool pos = dpart != nullptr;
-  adjoint_anterpolate(c, vbar, 3);
+  {  // the cotangent's anterpolation and (displaced sources) the charge's upward pass share their launches
+    UpCharge up[4];
+    anterpolation_sets(c, vbar, 3, up);
+    up[3] = {area, vor_eff, c->qpart, c->qproxy};
+    launch_upward_sets(c->dt, up, pos ? 4 : 3, c->stream);
+  }
   const double* psix[3] = {c->adj_psix[0], c->adj_psix[1], c->adj_psix[2]};
   const bool fp32 = c->prm.precision != SBSLPM_FP64;  // mixed contexts take the FP32 pair sums
   if (fp32)
@@ -2342,7 +2342,6 @@ static int fmm_adjoint(sbslpm_ctx* c, const double* area, const double* vor_eff,
   else
     launch_adj_target_records(c->dt, vbar, psix, c->adj_trec, c->stream);
   // source positions (and, for the displacement part, charges) as the forward records
-  if (pos) launch_upward_fused(c->dt, area, vor_eff, c->qpart, c->qproxy, c->stream);
   launch_source_records64(c->dt, c->qpart, c->qproxy, dpart, dprox, true, c->adj_srec, c->stream);
   launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 1, &c->adj_qbx, c->stream);
   if (pos) launch_zero_rows(c->adj_zero_rows, c->adj_nzero, t.n, 3, c->adj_dbx, c->stream);

@@ -305,10 +305,17 @@ void launch_l2l(const DevTree& t, int ncomp, double* const* phi_proxy, hipStream
 // are not written (they are never interaction sources).
 void launch_upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
                          hipStream_t s, int nmem = 1);
-// launch_upward_fused for two charge arrays over the same points (the
-// shallow-water vorticity and divergence), one launch per level for both.
-void launch_upward_fused2(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
-                          const double* pots2, double* q_part2, double* q_proxy2, hipStream_t s);
+// launch_upward_fused for several charges over the same points (e.g. the
+// shallow-water vorticity and divergence, and the adjoint's cotangent
+// components), one launch per level for up to kMaxUpSets of them.
+constexpr int kMaxUpSets = 5;
+struct UpCharge {
+  const double* area;
+  const double* pots;
+  double* q_part;
+  double* q_proxy;
+};
+void launch_upward_sets(const DevTree& t, const UpCharge* sets, int nsets, hipStream_t s);
 void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy, const double* const* phi_part,
                           double* const* sols, hipStream_t s, bool assign_leaf_rows = false, int nmem = 1);
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);

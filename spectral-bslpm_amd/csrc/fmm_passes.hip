@@ -377,6 +377,27 @@ __device__ inline void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CS
 typedef const __attribute__((address_space(4))) double* ConstTab;
 __device__ inline ConstTab const_tab(const double* p) { return (ConstTab)p; }
 
+// Charge sets of one upward launch (UpCharge in device.hpp), entry y taken by
+// blockIdx.y; n = 0: the member slices of entry 0.
+struct UpSets {
+  const double* area[kMaxUpSets];
+  const double* pots[kMaxUpSets];
+  double* qp[kMaxUpSets];
+  double* qx[kMaxUpSets];
+  int n;
+};
+
+// a[y] for a wave-uniform y, as a chain of selects (no indexed copy of the
+// argument array).
+template <typename T>
+__device__ inline T pick_set(T const (&a)[kMaxUpSets], int y) {
+  T v = a[0];
+#pragma unroll
+  for (int i = 1; i < kMaxUpSets; ++i)
+    if (y == i) v = a[i];
+  return v;
+}
+
 // k_l2l_l2p for a proxy grid of D x D nodes and a leaf grid of M x M
 // particles known at compile time, tiled for registers.  Every sweep is a
 // batch of small products out[r][k] = sum_b in[r][b] C[k][b] whose matrix C
@@ -582,14 +603,10 @@ __device__ inline void parent_from_children(const double* sq, ConstTab Thi, Cons
 // the parent compose to one product per direction with l2p_hi / l2p_lo, the
 // parent basis at the particles of an upper / lower half child (through their
 // transposes l2pT): q_part = area * pots for the four leaves is written and
-// staged in LDS, then parent_from_children.  With pots2 set, gridDim.y is 2
-// and blockIdx.y 1 takes the second charge (pots2, q_part2, q2) instead of a
-// member slice.
+// staged in LDS, then parent_from_children.  blockIdx.y takes charge set y
+// of `sets` (sets.n > 0) or member slice y of set 0.
 template <int D, int M>
-__global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const double* __restrict__ area,
-                                                  const double* __restrict__ pots, double* __restrict__ q_part,
-                                                  double* __restrict__ q, const double* __restrict__ pots2,
-                                                  double* __restrict__ q_part2, double* __restrict__ q2) {
+__global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, UpSets sets) {
   constexpr int N = D * D, M2 = M * M;
   __shared__ double sq[4 * M2];
   __shared__ double sU[4][M * D];
@@ -597,14 +614,16 @@ __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const d
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int par = pbegin + blockIdx.x;
   const int64_t leaf0 = 4 * (int64_t)blockIdx.x;  // the parents' children are the leaves 4i .. 4i+3
-  if (pots2 && blockIdx.y == 1) {
-    pots = pots2;
-    q_part = q_part2;
-    q = q2;
-  } else {
-    pots += member_part(t);
-    q_part += member_part(t);
-    q += member_proxy(t);
+  const double* __restrict__ area = sets.area[0];
+  const double* __restrict__ pots = sets.pots[0] + member_part(t);
+  double* __restrict__ q_part = sets.qp[0] + member_part(t);
+  double* __restrict__ q = sets.qx[0] + member_proxy(t);
+  if (sets.n > 0) {
+    const int y = blockIdx.y;
+    area = pick_set(sets.area, y);
+    pots = pick_set(sets.pots, y);
+    q_part = pick_set(sets.qp, y);
+    q = pick_set(sets.qx, y);
   }
   for (int i = threadIdx.x; i < 4 * M2; i += blockDim.x) {
     const int64_t u = leaf0 * M2 + i;
@@ -616,11 +635,10 @@ __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, const d
   parent_from_children<D, M>(sq, const_tab(t.l2pT_hi), const_tab(t.l2pT_lo), sU, q + (int64_t)par * N, lane, w);
 }
 
-// k_m2m (one level, parents pbegin ..) in the same tiled form, M = D; with q2
-// set, blockIdx.y 1 takes q2 as k_up_fused does.
+// k_m2m (one level, parents pbegin ..) in the same tiled form, M = D, over
+// the charge sets as k_up_fused.
 template <int D>
-__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, double* __restrict__ q,
-                                                   double* __restrict__ q2) {
+__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, UpSets sets) {
   constexpr int N = D * D;
   __shared__ double sq[4 * N];
   __shared__ double sU[4][N];
@@ -628,41 +646,35 @@ __global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, double
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int par = pbegin + blockIdx.x;
   const int c0 = t.panels[par].child0;
-  q = (q2 && blockIdx.y == 1) ? q2 : q + member_proxy(t);
+  double* __restrict__ q = sets.n > 0 ? pick_set(sets.qx, (int)blockIdx.y) : sets.qx[0] + member_proxy(t);
   for (int i = threadIdx.x; i < 4 * N; i += blockDim.x) sq[i] = q[(int64_t)c0 * N + i];
   __syncthreads();
   parent_from_children<D, D>(sq, const_tab(t.m2m_hi), const_tab(t.m2m_lo), sU, q + (int64_t)par * N, lane, w);
 }
 
 template <int D>
-bool launch_m2m_tiled_d(const DevTree& t, int pbegin, int count, double* q, double* q2, hipStream_t s, int nmem) {
+bool launch_m2m_tiled_d(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
   if (t.dp1 != D) return false;
-  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, q2 ? 2 : nmem), dim3(256), 0, s, t, pbegin, q, q2);
+  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, sets.n > 0 ? sets.n : nmem), dim3(256), 0, s, t, pbegin, sets);
   return true;
 }
 
-// The tiled M2M for the degrees in use; false for the others.  q2: a second
-// charge array in the same launch (nmem is then 1).
-bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, double* q, double* q2, hipStream_t s, int nmem) {
-#define SBSLPM_M2M(D) launch_m2m_tiled_d<D>(t, pbegin, count, q, q2, s, nmem)
+// The tiled M2M for the degrees in use; false for the others.
+bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
+#define SBSLPM_M2M(D) launch_m2m_tiled_d<D>(t, pbegin, count, sets, s, nmem)
   return SBSLPM_M2M(9) || SBSLPM_M2M(6) || SBSLPM_M2M(5) || SBSLPM_M2M(7) || SBSLPM_M2M(11);
 #undef SBSLPM_M2M
 }
 
 template <int D, int M>
-bool launch_up_fused_dm(const DevTree& t, int pbegin, int count, const double* area, const double* pots,
-                        double* q_part, double* q, const double* pots2, double* q_part2, double* q2, hipStream_t s,
-                        int nmem) {
+bool launch_up_fused_dm(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
   if (t.dp1 != D || t.m != M || !t.l2pT_hi) return false;
-  hipLaunchKernelGGL((k_up_fused<D, M>), dim3(count, pots2 ? 2 : nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part,
-                     q, pots2, q_part2, q2);
+  hipLaunchKernelGGL((k_up_fused<D, M>), dim3(count, sets.n > 0 ? sets.n : nmem), dim3(256), 0, s, t, pbegin, sets);
   return true;
 }
 
-bool launch_up_fused(const DevTree& t, int pbegin, int count, const double* area, const double* pots, double* q_part,
-                     double* q, const double* pots2, double* q_part2, double* q2, hipStream_t s, int nmem) {
-#define SBSLPM_UP(D, M) \
-  launch_up_fused_dm<D, M>(t, pbegin, count, area, pots, q_part, q, pots2, q_part2, q2, s, nmem)
+bool launch_up_fused(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
+#define SBSLPM_UP(D, M) launch_up_fused_dm<D, M>(t, pbegin, count, sets, s, nmem)
   return SBSLPM_UP(9, 8) || SBSLPM_UP(9, 9) || SBSLPM_UP(6, 6) || SBSLPM_UP(5, 5) || SBSLPM_UP(5, 6) ||
          SBSLPM_UP(6, 7) || SBSLPM_UP(7, 7) || SBSLPM_UP(11, 11) || SBSLPM_UP(11, 8);
 #undef SBSLPM_UP
@@ -749,13 +761,17 @@ void launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
 // loads with too few waves per compute unit to hide them (178 us against
 // about 30 us for the four launches it replaced at 6 x 512^2).
 static void launch_m2m_levels(const DevTree& t, int lfirst, double* q_proxy, hipStream_t s, int nmem,
-                              double* q_proxy2 = nullptr) {
+                              const UpSets* sets = nullptr) {
+  UpSets one{};
+  one.qx[0] = q_proxy;
+  const UpSets& us = sets ? *sets : one;
   for (int l = lfirst; l >= 0; --l) {
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-    if (launch_m2m_tiled(t, pbegin, count, q_proxy, q_proxy2, s, nmem)) continue;
-    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy);
-    if (q_proxy2)
-      hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(128), 0, s, t, pbegin, (const int*)nullptr, q_proxy2);
+    if (launch_m2m_tiled(t, pbegin, count, us, s, nmem)) continue;
+    if (us.n == 0)
+      hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, pbegin, (const int*)nullptr, us.qx[0]);
+    for (int i = 0; i < us.n; ++i)
+      hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(128), 0, s, t, pbegin, (const int*)nullptr, us.qx[i]);
   }
 }
 
@@ -774,31 +790,46 @@ void launch_upward_pass(const DevTree& t, const double* area, const double* pots
 
 // Upward pass with base_pots, P2M and the first M2M level fused: q_part and
 // the non-leaf rows of q_proxy are assigned, the leaf rows are not written.
-static void upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
-                         const double* pots2, double* q_part2, double* q_proxy2, hipStream_t s, int nmem) {
+// The fused upward pass for the charge sets `us` (us.n > 0) or for nmem
+// member slices of us entry 0.
+static void upward_fused(const DevTree& t, const UpSets& us, hipStream_t s, int nmem) {
+  const int ns = us.n > 0 ? us.n : 1;
   if (t.levels < 2) {
-    launch_upward_pass(t, area, pots, q_part, q_proxy, s);
-    if (pots2) launch_upward_pass(t, area, pots2, q_part2, q_proxy2, s);
+    for (int i = 0; i < ns; ++i) launch_upward_pass(t, us.area[i], us.pots[i], us.qp[i], us.qx[i], s);
     return;
   }
   const int l = t.levels - 2;
   const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
-  if (!launch_up_fused(t, pbegin, count, area, pots, q_part, q_proxy, pots2, q_part2, q_proxy2, s, nmem)) {
-    hipLaunchKernelGGL(k_p2m_m2m, dim3(count, nmem), dim3(256), 0, s, t, pbegin, area, pots, q_part, q_proxy);
-    if (pots2)
-      hipLaunchKernelGGL(k_p2m_m2m, dim3(count, 1), dim3(256), 0, s, t, pbegin, area, pots2, q_part2, q_proxy2);
+  if (!launch_up_fused(t, pbegin, count, us, s, nmem)) {
+    for (int i = 0; i < ns; ++i)
+      hipLaunchKernelGGL(k_p2m_m2m, dim3(count, us.n > 0 ? 1 : nmem), dim3(256), 0, s, t, pbegin, us.area[i],
+                         us.pots[i], us.qp[i], us.qx[i]);
   }
-  launch_m2m_levels(t, t.levels - 3, q_proxy, s, nmem, q_proxy2);
+  launch_m2m_levels(t, t.levels - 3, us.qx[0], s, nmem, &us);
 }
 
 void launch_upward_fused(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
                          hipStream_t s, int nmem) {
-  upward_fused(t, area, pots, q_part, q_proxy, nullptr, nullptr, nullptr, s, nmem);
+  UpSets us{};
+  us.area[0] = area;
+  us.pots[0] = pots;
+  us.qp[0] = q_part;
+  us.qx[0] = q_proxy;
+  upward_fused(t, us, s, nmem);
 }
 
-void launch_upward_fused2(const DevTree& t, const double* area, const double* pots, double* q_part, double* q_proxy,
-                          const double* pots2, double* q_part2, double* q_proxy2, hipStream_t s) {
-  upward_fused(t, area, pots, q_part, q_proxy, pots2, q_part2, q_proxy2, s, 1);
+void launch_upward_sets(const DevTree& t, const UpCharge* sets, int nsets, hipStream_t s) {
+  for (int b = 0; b < nsets; b += kMaxUpSets) {
+    UpSets us{};
+    us.n = std::min(kMaxUpSets, nsets - b);
+    for (int i = 0; i < us.n; ++i) {
+      us.area[i] = sets[b + i].area;
+      us.pots[i] = sets[b + i].pots;
+      us.qp[i] = sets[b + i].q_part;
+      us.qx[i] = sets[b + i].q_proxy;
+    }
+    upward_fused(t, us, s, 1);
+  }
 }
 
 // fmm_funcs.cpp:357-392.  phi_proxy rows are accumulated top-down; sols are
