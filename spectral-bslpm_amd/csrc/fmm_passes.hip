@@ -537,11 +537,12 @@ bool launch_down_fused_dm(const DevTree& t, int ncomp, CPtr3 phi, CPtr3 part, Pt
   return true;
 }
 
-// The register-tiled form for the layouts in use (degree 8 with an 8 x 8 or
-// 9 x 9 leaf grid, degree 5 with 6 x 6, ...); false for the others.
+// The register-tiled form for the layouts in use (degree 8 with an 8 x 8,
+// 9 x 9 or 6 x 6 leaf grid, degree 5 with 6 x 6, ...); false for the others.
 bool launch_down_fused(const DevTree& t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 out, hipStream_t s, int nmem) {
   return launch_down_fused_dm<9, 8>(t, ncomp, phi, part, out, s, nmem) ||
          launch_down_fused_dm<9, 9>(t, ncomp, phi, part, out, s, nmem) ||
+         launch_down_fused_dm<9, 6>(t, ncomp, phi, part, out, s, nmem) ||
          launch_down_fused_dm<6, 6>(t, ncomp, phi, part, out, s, nmem) ||
          launch_down_fused_dm<5, 5>(t, ncomp, phi, part, out, s, nmem) ||
          launch_down_fused_dm<5, 6>(t, ncomp, phi, part, out, s, nmem) ||
@@ -675,8 +676,8 @@ bool launch_up_fused_dm(const DevTree& t, int pbegin, int count, const UpSets& s
 
 bool launch_up_fused(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
 #define SBSLPM_UP(D, M) launch_up_fused_dm<D, M>(t, pbegin, count, sets, s, nmem)
-  return SBSLPM_UP(9, 8) || SBSLPM_UP(9, 9) || SBSLPM_UP(6, 6) || SBSLPM_UP(5, 5) || SBSLPM_UP(5, 6) ||
-         SBSLPM_UP(6, 7) || SBSLPM_UP(7, 7) || SBSLPM_UP(11, 11) || SBSLPM_UP(11, 8);
+  return SBSLPM_UP(9, 8) || SBSLPM_UP(9, 9) || SBSLPM_UP(9, 6) || SBSLPM_UP(6, 6) || SBSLPM_UP(5, 5) ||
+         SBSLPM_UP(5, 6) || SBSLPM_UP(6, 7) || SBSLPM_UP(7, 7) || SBSLPM_UP(11, 11) || SBSLPM_UP(11, 8);
 #undef SBSLPM_UP
 }
 
