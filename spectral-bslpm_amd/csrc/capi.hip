@@ -177,6 +177,11 @@ struct sbslpm_irreg {
   SrcRecSwe* recswe = nullptr;
   SrcRec64* rec64 = nullptr;
   double* w = nullptr;  // [N][2 (d+1)] BLI basis of each sorted point on its leaf
+  // ancestors of leaf i: anc[3 e .. 3 e + 2] = (panel, xi matrix, eta matrix of
+  // anc_tab) for e in [anc_first[i], anc_first[i + 1]): the one-launch L2L sweep
+  int* anc_first = nullptr;
+  int* anc = nullptr;
+  double* anc_tab = nullptr;
   // lat-lon step workspace (tendencies, arrival values, pole means), allocated on
   // the first step
   double* ll_ws = nullptr;
@@ -1795,6 +1800,31 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
     count[p] = t.pn[p].point_count;
     if (t.pn[p].is_leaf) leaves.push_back(p);
   }
+  // Each leaf's ancestors with the composite matrices that evaluate them at the
+  // leaf's nodes (the position bit of each step up is the upper-half flag of
+  // the child: quadrants (hi,hi), (lo,hi), (lo,lo), (hi,lo) in (xi, eta)).
+  std::vector<int> anc_first(1, 0), anc;
+  int maxgap = 0;
+  for (int p : leaves) {
+    int node = p, px = 0, py = 0;
+    for (int g = 1; t.pn[node].parent >= 0; ++g) {
+      const int par = t.pn[node].parent;
+      int q = 0;
+      while (q < 4 && t.pn[par].child[q] != node) ++q;
+      if (q == 4) {
+        delete h;
+        return fail(c, SBSLPM_E_ARG, "irreg_create: a panel is not among its parent's children");
+      }
+      px |= (q == 0 || q == 3) << (g - 1);
+      py |= (q <= 1) << (g - 1);
+      anc.insert(anc.end(), {par, (1 << g) - 2 + px, (1 << g) - 2 + py});
+      maxgap = std::max(maxgap, g);
+      node = par;
+    }
+    anc_first.push_back((int)anc.size() / 3);
+  }
+  std::vector<double> anc_tab;
+  composite_l2l(t.degree, t.cheb.data(), t.cheb.data(), maxgap, anc_tab);
   Arena A;
   const size_t o_pan = A.take<DevPanel>(t.P), o_px = A.take<double>(3 * U), o_cheb = A.take<double>(t.dp1),
                o_hi = A.take<double>(t.dp1 * t.dp1), o_lo = A.take<double>(t.dp1 * t.dp1), o_perm = A.take<int>(N),
@@ -1804,7 +1834,8 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
                o_zr = A.take<int>(t.zero_rows.size()), o_q = A.take<double>(2 * N), o_qx = A.take<double>(2 * Pn),
                o_part = A.take<double>(3 * N), o_prox = A.take<double>(3 * Pn), o_rs = A.take<SrcRecSwe>(U),
                o_r64 = A.take<SrcRec64>(U), o_w = A.take<double>((size_t)N * 2 * t.dp1),
-               total = A.take<char>(256);
+               o_af = A.take<int>(anc_first.size()), o_anc = A.take<int>(std::max<size_t>(anc.size(), 1)),
+               o_atab = A.take<double>(std::max<size_t>(anc_tab.size(), 1)), total = A.take<char>(256);
   char* b = nullptr;
   if (hipMalloc(&b, total) != hipSuccess) {
     delete h;
@@ -1825,6 +1856,9 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
   if (!rc) rc = upload(c, b + o_pl, t.point_leaf.data(), N * sizeof(int));
   if (!rc) rc = upload(c, b + o_lv, leaves.data(), leaves.size() * sizeof(int));
   if (!rc) rc = upload(c, b + o_nl, t.nonleaf.data(), t.nonleaf.size() * sizeof(int));
+  if (!rc) rc = upload(c, b + o_af, anc_first.data(), anc_first.size() * sizeof(int));
+  if (!rc) rc = upload(c, b + o_anc, anc.data(), anc.size() * sizeof(int));
+  if (!rc) rc = upload(c, b + o_atab, anc_tab.data(), anc_tab.size() * sizeof(double));
   // tiles heaviest first (sum of their source-set sizes): the pair kernels take one tile per
   // workgroup in dispatch order, so the long tiles start first and the short ones fill in
   std::vector<TargetTile> tiles_by_cost = t.tiles;
@@ -1872,6 +1906,9 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
   h->leaves = (int*)(b + o_lv);
   h->nleaves = (int)leaves.size();
   h->nonleaf = (int*)(b + o_nl);
+  h->anc_first = (int*)(b + o_af);
+  h->anc = (int*)(b + o_anc);
+  h->anc_tab = (double*)(b + o_atab);
   h->tiles = (TargetTile*)(b + o_tl);
   h->ntiles = (int)t.tiles.size();
   h->sets = (SourceSet*)(b + o_st);
@@ -1946,14 +1983,18 @@ void irreg_upward2(sbslpm_irreg* h, const double* area, const double* vors, cons
 
 // L2L top-down over the non-leaf panels (accumulating into the children), then
 // out_sorted = L2P(leaf rows) + part_sorted (parent_to_child_3_ll +
-// child_panel_interp_3_ll).
-void irreg_downward(sbslpm_irreg* h, double* const* proxy, const double* const* part_sorted, double* const* out_sorted) {
+// child_panel_interp_3_ll).  With `fused` the sweep is one launch that forms
+// the leaf rows from their ancestors' rows (equal in exact arithmetic) and
+// leaves the non-leaf rows as they are.
+void irreg_downward(sbslpm_irreg* h, double* const* proxy, const double* const* part_sorted, double* const* out_sorted,
+                    bool fused = false) {
   const IrregTree& t = h->tree;
   hipStream_t s = h->ctx->stream;
-  for (int l = 0; l <= t.levels; ++l) {
-    const int a = t.level_nonleaf[l], b = t.level_nonleaf[l + 1];
-    launch_l2l_list(h->dt, h->nonleaf + a, b - a, 3, proxy, s);
-  }
+  if (!fused || !launch_leaf_rows(h->dt, h->leaves, h->nleaves, h->anc_first, h->anc, h->anc_tab, 3, proxy, s))
+    for (int l = 0; l <= t.levels; ++l) {
+      const int a = t.level_nonleaf[l], b = t.level_nonleaf[l + 1];
+      launch_l2l_list(h->dt, h->nonleaf + a, b - a, 3, proxy, s);
+    }
   const double* cp[3] = {proxy[0], proxy[1], proxy[2]};
   launch_l2p_ll(h->dt, h->point_leaf, h->w, 3, cp, part_sorted, out_sorted, s);
 }
@@ -1974,7 +2015,7 @@ void irreg_swe_vel(sbslpm_irreg* h, const double* area, const double* vors, cons
   irreg_upward2(h, area, vors, divs);
   irreg_swe_pairs(h, h->q[0], h->qx[0], h->q[1], h->qx[1]);
   const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
-  irreg_downward(h, h->prox, cp, h->part);
+  irreg_downward(h, h->prox, cp, h->part, true);
   launch_scatter(h->tree.N, 3, h->perm, cp, vel, false, h->ctx->stream);
 }
 
@@ -2042,7 +2083,7 @@ int sbslpm_irreg_bve_velocity(sbslpm_irreg* h, const double* area, const double*
   launch_bve_interact64(h->dt, h->tiles, h->ntiles, h->sets, h->rec64, h->ctx->prm.kernel_eps, h->part, h->prox, s,
                         true);
   const double* cp[3] = {h->part[0], h->part[1], h->part[2]};
-  irreg_downward(h, h->prox, cp, h->part);
+  irreg_downward(h, h->prox, cp, h->part, true);
   double* vo[3] = {vel_x, vel_y, vel_z};
   launch_scatter(t.N, 3, h->perm, cp, vo, false, s);
   return launched(h->ctx);

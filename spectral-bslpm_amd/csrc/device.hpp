@@ -460,6 +460,13 @@ void launch_stage_adjoint(const DevTree& t, const double* x, const double* y, co
 
 // ---- irregular particle tree (irreg.hip; M2M / L2L over panel lists in fmm_passes.hip)
 // nmem proxy arrays, P*n apart
+// The L2L sweep of a tree with leaves at several levels in one launch: the
+// leaf rows (leaves[0 .. nleaves)) += each ancestor's row evaluated at the
+// leaf's nodes (ancestor triples anc[anc_first[i] ..), composite matrices
+// anc_tab); the non-leaf rows are left as they are.  False (nothing launched)
+// for a degree without a compiled instance.
+bool launch_leaf_rows(const DevTree& t, const int* leaves, int nleaves, const int* anc_first, const int* anc,
+                      const double* anc_tab, int ncomp, double* const* phi_proxy, hipStream_t s);
 void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s, int nmem = 1);
 void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp, double* const* phi_proxy,
                      hipStream_t s);

@@ -681,6 +681,81 @@ bool launch_up_fused(const DevTree& t, int pbegin, int count, const UpSets& sets
 #undef SBSLPM_UP
 }
 
+// The L2L sweep of a tree whose leaves sit at several levels (the irregular
+// lat-lon tree) in one launch: a leaf's row becomes its own row plus each
+// ancestor's own row evaluated at the leaf's nodes through a composite matrix
+// (tree.cpp composite_l2l), one workgroup per leaf.  The ancestors of leaf i
+// are anc[3e .. 3e+2] = (panel, xi matrix, eta matrix) for e in
+// [anc_first[i], anc_first[i+1]).  Wave w takes entries w, w + 4, ... with the
+// sweeps of k_down_fused's first phase; the leaf's row and the four waves'
+// sums are added in a fixed order.  Only the leaf rows are written.
+template <int D>
+__global__ __launch_bounds__(256) void k_leaf_rows(const int* __restrict__ leaves, const int* __restrict__ anc_first,
+                                                   const int* __restrict__ anc, const double* __restrict__ anc_tab,
+                                                   int ncomp, Ptr3 phi) {
+  constexpr int N = D * D;
+  static_assert(3 * N <= 512 && 3 * D <= 64, "k_leaf_rows: grid too large for one workgroup");
+  __shared__ double sV[4][3 * N];
+  __shared__ double sPart[4][3 * N];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int leaf = leaves[blockIdx.x];
+  const int e0 = anc_first[blockIdx.x], e1 = anc_first[blockIdx.x + 1];
+  const bool row_lane = lane < ncomp * D;
+  const int f = row_lane ? lane / D : 0, a = lane - f * D;
+  const double* __restrict__ phif = comp(phi, f);
+  double* part_w = &sPart[w][f * N + a];
+  if (row_lane)
+    for (int k = 0; k < D; ++k) part_w[k * D] = 0.0;
+  for (int e = e0 + w; e < e1; e += 4) {
+    const int64_t ap = anc[3 * e];
+    const ConstTab Cx = const_tab(anc_tab) + (int64_t)anc[3 * e + 1] * N;
+    const ConstTab Cy = const_tab(anc_tab) + (int64_t)anc[3 * e + 2] * N;
+    if (row_lane) {  // V[a][l] = sum_b Cy[l][b] P[a][b]
+      double p[D];
+#pragma unroll
+      for (int b = 0; b < D; ++b) p[b] = phif[ap * N + a * D + b];
+#pragma unroll 1
+      for (int l = 0; l < D; ++l) {
+        double v = 0.0;
+#pragma unroll
+        for (int b = 0; b < D; ++b) v += Cy[l * D + b] * p[b];
+        sV[w][f * N + a * D + l] = v;
+      }
+    }
+    wave_lds_order();
+    if (row_lane) {  // column l = a: sum[k][l] += sum_r Cx[k][r] V[r][l]
+      double q[D];
+#pragma unroll
+      for (int r = 0; r < D; ++r) q[r] = sV[w][f * N + r * D + a];
+#pragma unroll 1
+      for (int k = 0; k < D; ++k) {
+        double v = part_w[k * D];
+#pragma unroll
+        for (int r = 0; r < D; ++r) v += Cx[k * D + r] * q[r];
+        part_w[k * D] = v;
+      }
+    }
+    wave_lds_order();
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < ncomp * N; o += blockDim.x) {
+    const int ff = o / N;
+    double* row = comp(phi, ff) + (int64_t)leaf * N;
+    const int j = o - ff * N;
+    row[j] = row[j] + sPart[0][o] + sPart[1][o] + sPart[2][o] + sPart[3][o];
+  }
+}
+
+template <int D>
+bool launch_leaf_rows_d(const DevTree& t, const int* leaves, int nleaves, const int* anc_first, const int* anc,
+                        const double* anc_tab, int ncomp, Ptr3 phi, hipStream_t s) {
+  if (t.dp1 != D) return false;
+  if (nleaves > 0)
+    hipLaunchKernelGGL((k_leaf_rows<D>), dim3(nleaves), dim3(256), 0, s, leaves, anc_first, anc, anc_tab, ncomp, phi);
+  return true;
+}
+
 // Dynamic LDS of k_l2l_l2p (bytes).
 size_t l2l_l2p_lds(const DevTree& t, int ncomp) {
   const size_t n = t.n, D = t.dp1, m = t.m, nc = ncomp * n;
@@ -883,6 +958,15 @@ void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp,
   Ptr3 phi{};
   for (int f = 0; f < ncomp; ++f) phi.p[f] = phi_proxy[f];
   if (count > 0) hipLaunchKernelGGL(k_l2l, dim3(count), dim3(256), 0, s, t, 0, parents, ncomp, phi, 0);
+}
+
+bool launch_leaf_rows(const DevTree& t, const int* leaves, int nleaves, const int* anc_first, const int* anc,
+                      const double* anc_tab, int ncomp, double* const* phi_proxy, hipStream_t s) {
+  Ptr3 phi{};
+  for (int f = 0; f < ncomp; ++f) phi.p[f] = phi_proxy[f];
+#define SBSLPM_LR(D) launch_leaf_rows_d<D>(t, leaves, nleaves, anc_first, anc, anc_tab, ncomp, phi, s)
+  return SBSLPM_LR(11) || SBSLPM_LR(9) || SBSLPM_LR(7) || SBSLPM_LR(6) || SBSLPM_LR(5);
+#undef SBSLPM_LR
 }
 
 }  // namespace sbslpm

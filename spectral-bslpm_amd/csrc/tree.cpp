@@ -166,6 +166,31 @@ int locate_leaf(const Tree& t, double x, double y, double z) {
   return locate_uniform(t.cfg.levels, x, y, z, xe);
 }
 
+// Composite transfer matrices of the fused downward passes: the basis of an
+// ancestor g levels above a panel at that panel's nodes, one 1-D matrix per
+// position of the panel's interval among the ancestor's 2^g equal
+// sub-intervals (counted from the lower end), g = 1 .. maxgap.  Children are
+// exact halves and the interpolants are degree-d polynomials in each
+// direction, so the reference's level-by-level L2L chain evaluates each
+// ancestor's interpolant at the panel's nodes; these matrices do it in one
+// product.  Layout [(2^g - 2 + pos)][k][a]: descendant node k, ancestor basis
+// a (the transpose of m2m_hi's layout).
+void composite_l2l(int degree, const double* node, const double* cheb, int maxgap, std::vector<double>& out) {
+  const int dp1 = degree + 1, n = dp1 * dp1;
+  std::vector<double> w(dp1);
+  out.clear();
+  for (int g = 1; g <= maxgap; ++g) {
+    const int np = 1 << g;
+    const double wd = 2.0 / np;
+    out.resize((size_t)(2 * np - 2) * n, 0.0);
+    for (int pos = 0; pos < np; ++pos)
+      for (int k = 0; k <= degree; ++k) {
+        bli_weights_1d(-1.0 + pos * wd + 0.5 * (node[k] + 1.0) * wd, -1.0, 1.0, degree, cheb, w.data());
+        for (int a = 0; a <= degree; ++a) out[(size_t)(np - 2 + pos) * n + k * dp1 + a] = w[a];
+      }
+  }
+}
+
 std::string build_tree(const TreeConfig& cfg, Tree& t) {
   if (cfg.levels < 1 || cfg.levels > 10) return "levels must be in 1..10";
   if (cfg.degree < 1 || cfg.degree > kMaxDeg) return "interp_degree must be in 1..10";
@@ -373,26 +398,7 @@ std::string build_tree(const TreeConfig& cfg, Tree& t) {
     bli_weights_1d(0.5 * (t.node_m[k] - 1.0), -1.0, 1.0, d, t.cheb_d.data(), w.data());
     for (int a = 0; a <= d; ++a) t.l2p_lo[k * t.dp1 + a] = t.l2pT_lo[a * t.m + k] = w[a];
   }
-  // Composite transfer matrices for the fused downward pass: the basis of an
-  // ancestor g levels above a panel at that panel's nodes, one 1-D matrix per
-  // position of the panel's interval among the ancestor's 2^g equal
-  // sub-intervals (counted from the lower end), g = 1 .. L-2.  Children are
-  // exact halves and the interpolants are degree-d polynomials in each
-  // direction, so the reference's level-by-level L2L chain evaluates each
-  // ancestor's interpolant at the panel's nodes; these matrices do it in one
-  // product.  Layout [(2^g - 2 + pos)][k][a]: descendant node k, ancestor
-  // basis a (the transpose of m2m_hi's layout).
-  t.anc_l2l.clear();
-  for (int g = 1; g + 2 <= t.cfg.levels; ++g) {
-    const int np = 1 << g;
-    const double wd = 2.0 / np;
-    t.anc_l2l.resize((size_t)(2 * np - 2) * t.n, 0.0);
-    for (int pos = 0; pos < np; ++pos)
-      for (int k = 0; k <= d; ++k) {
-        bli_weights_1d(-1.0 + pos * wd + 0.5 * (t.node_d[k] + 1.0) * wd, -1.0, 1.0, d, t.cheb_d.data(), w.data());
-        for (int a = 0; a <= d; ++a) t.anc_l2l[(size_t)(np - 2 + pos) * t.n + k * t.dp1 + a] = w[a];
-      }
-  }
+  composite_l2l(d, t.node_d.data(), t.cheb_d.data(), t.cfg.levels - 2, t.anc_l2l);
 
   // Leaf containing each non-leaf proxy node, located from its Cartesian
   // position exactly as bve_disp_interp does (bve_time_step.cpp:139-143).

@@ -100,6 +100,9 @@ struct Tree {
 // Builds everything above on the host; returns an empty string on success or
 // a message describing the invalid parameter.
 std::string build_tree(const TreeConfig& cfg, Tree& t);
+// Composite L2L matrices for gaps 1 .. maxgap (layout of Tree::anc_l2l); node:
+// the proxy nodes in [-1, 1], cheb: the interpolation nodes.
+void composite_l2l(int degree, const double* node, const double* cheb, int maxgap, std::vector<double>& out);
 
 // The interaction list seen from its sources, for the transposed (adjoint)
 // pair kernel: tiles of at most 64 source points of one source panel
