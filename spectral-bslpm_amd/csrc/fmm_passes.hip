@@ -636,16 +636,16 @@ __global__ __launch_bounds__(256) void k_up_fused(DevTree t, int pbegin, UpSets 
   parent_from_children<D, M>(sq, const_tab(t.l2pT_hi), const_tab(t.l2pT_lo), sU, q + (int64_t)par * N, lane, w);
 }
 
-// k_m2m (one level, parents pbegin ..) in the same tiled form, M = D, over
-// the charge sets as k_up_fused.
+// k_m2m (one level: parents pbegin .., or list[]) in the same tiled form,
+// M = D, over the charge sets as k_up_fused.
 template <int D>
-__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, UpSets sets) {
+__global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, const int* __restrict__ list, UpSets sets) {
   constexpr int N = D * D;
   __shared__ double sq[4 * N];
   __shared__ double sU[4][N];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int par = pbegin + blockIdx.x;
+  const int par = list ? list[blockIdx.x] : pbegin + blockIdx.x;
   const int c0 = t.panels[par].child0;
   double* __restrict__ q = sets.n > 0 ? pick_set(sets.qx, (int)blockIdx.y) : sets.qx[0] + member_proxy(t);
   for (int i = threadIdx.x; i < 4 * N; i += blockDim.x) sq[i] = q[(int64_t)c0 * N + i];
@@ -654,15 +654,19 @@ __global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, UpSets
 }
 
 template <int D>
-bool launch_m2m_tiled_d(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
+bool launch_m2m_tiled_d(const DevTree& t, int pbegin, const int* list, int count, const UpSets& sets, hipStream_t s,
+                        int nmem) {
   if (t.dp1 != D) return false;
-  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, sets.n > 0 ? sets.n : nmem), dim3(256), 0, s, t, pbegin, sets);
+  hipLaunchKernelGGL((k_m2m_tiled<D>), dim3(count, sets.n > 0 ? sets.n : nmem), dim3(256), 0, s, t, pbegin, list,
+                     sets);
   return true;
 }
 
-// The tiled M2M for the degrees in use; false for the others.
-bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
-#define SBSLPM_M2M(D) launch_m2m_tiled_d<D>(t, pbegin, count, sets, s, nmem)
+// The tiled M2M for the degrees in use (parents pbegin .. pbegin + count, or
+// list[0 .. count)); false for the others.
+bool launch_m2m_tiled(const DevTree& t, int pbegin, int count, const UpSets& sets, hipStream_t s, int nmem,
+                      const int* list = nullptr) {
+#define SBSLPM_M2M(D) launch_m2m_tiled_d<D>(t, pbegin, list, count, sets, s, nmem)
   return SBSLPM_M2M(9) || SBSLPM_M2M(6) || SBSLPM_M2M(5) || SBSLPM_M2M(7) || SBSLPM_M2M(11);
 #undef SBSLPM_M2M
 }
@@ -950,7 +954,11 @@ void launch_downward_pass(const DevTree& t, int ncomp, double* const* phi_proxy,
 }
 
 void launch_m2m_list(const DevTree& t, const int* parents, int count, double* q_proxy, hipStream_t s, int nmem) {
-  if (count > 0) hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, 0, parents, q_proxy);
+  if (count <= 0) return;
+  UpSets one{};
+  one.qx[0] = q_proxy;
+  if (!launch_m2m_tiled(t, 0, count, one, s, nmem, parents))
+    hipLaunchKernelGGL(k_m2m, dim3(count, nmem), dim3(128), 0, s, t, 0, parents, q_proxy);
 }
 
 void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp, double* const* phi_proxy,

@@ -69,47 +69,70 @@ __global__ void k_point_weights(DevTree t, const int* __restrict__ point_leaf, d
 
 // base_pots_ll (fmm_funcs_impl.hpp:207-223): proxy[leaf][a*(d+1)+b] =
 // sum over the leaf's points of l_a(xi) l_b(eta) q (assigned), for nq = 1 or 2
-// charge arrays at once (q + j*qstride -> proxy + j*pstride); the points'
-// weights and charges are staged in LDS 64 points at a time.
-__global__ __launch_bounds__(128) void k_p2m_ll(DevTree t, const int* __restrict__ leaves,
-                                                const int* __restrict__ leaf_first, const int* __restrict__ leaf_count,
-                                                const double* __restrict__ w, int nq, const double* __restrict__ q,
-                                                int64_t qstride, double* __restrict__ proxy, int64_t pstride) {
-  __shared__ double sw[64 * 2 * (kMaxDeg + 1)];
-  __shared__ double sq[2][64];
+// charge arrays at once (q + j*qstride -> proxy + j*pstride).  One workgroup of
+// kP2mParts x 128 threads per leaf.  The leaf's weights and charges are staged
+// in LDS in rounds of up to kP2mRound points, all threads loading (each point's
+// weights are a new cache line, so the loads must be in flight together, not
+// one point after another); thread (h, o) then sums output o over part h of
+// the round, and the parts are added through LDS in a fixed order.
+constexpr int kP2mParts = 4, kP2mRound = 256;
+
+__global__ __launch_bounds__(128 * kP2mParts) void k_p2m_ll(DevTree t, const int* __restrict__ leaves,
+                                                            const int* __restrict__ leaf_first,
+                                                            const int* __restrict__ leaf_count,
+                                                            const double* __restrict__ w, int nq,
+                                                            const double* __restrict__ q, int64_t qstride,
+                                                            double* __restrict__ proxy, int64_t pstride) {
+  static_assert((kMaxDeg + 1) * (kMaxDeg + 1) <= 128, "k_p2m_ll: one output per thread of a part");
+  __shared__ double sw[kP2mRound * 2 * (kMaxDeg + 1)];
+  __shared__ double sq[2][kP2mRound];
+  __shared__ double sacc[kP2mParts][2][128];
   const int leaf = leaves[blockIdx.x];
   const int first = leaf_first[leaf], cnt = leaf_count[leaf];
-  const int dp1 = t.dp1, n = t.n, o = threadIdx.x;
+  const int dp1 = t.dp1, n = t.n, w2 = 2 * dp1;
+  const int o = threadIdx.x & 127, h = threadIdx.x >> 7;
+  const int a = o / dp1, b = o - a * dp1;
   double acc0 = 0, acc1 = 0;
-  for (int c0 = 0; c0 < cnt; c0 += 64) {
-    const int m = min(64, cnt - c0);
-    const double* wc = w + ((int64_t)first + c0) * 2 * dp1;
-    for (int i = o; i < m * 2 * dp1; i += blockDim.x) sw[i] = wc[i];
-    if (o < m) {
-      const int64_t k = (int64_t)first + c0 + o;
-      sq[0][o] = q[k];
-      if (nq > 1) sq[1][o] = q[qstride + k];
+  for (int c0 = 0; c0 < cnt; c0 += kP2mRound) {
+    const int m = min(kP2mRound, cnt - c0);
+    const double* __restrict__ wc = w + ((int64_t)first + c0) * w2;
+    for (int i = threadIdx.x; i < m * w2; i += blockDim.x) sw[i] = wc[i];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int64_t k = (int64_t)first + c0 + i;
+      sq[0][i] = q[k];
+      sq[1][i] = nq > 1 ? q[qstride + k] : 0.0;
     }
     __syncthreads();
     if (o < n) {
-      const int a = o / dp1, b = o % dp1;
+      const int part = (m + kP2mParts - 1) / kP2mParts, j0 = h * part, j1 = min(m, j0 + part);
       if (nq > 1) {
 #pragma unroll 4
-        for (int j = 0; j < m; ++j) {
-          const double ab = sw[j * 2 * dp1 + a] * sw[j * 2 * dp1 + dp1 + b];
+        for (int j = j0; j < j1; ++j) {
+          const double ab = sw[j * w2 + a] * sw[j * w2 + dp1 + b];
           acc0 = fma(ab, sq[0][j], acc0);
           acc1 = fma(ab, sq[1][j], acc1);
         }
       } else {
 #pragma unroll 4
-        for (int j = 0; j < m; ++j) acc0 = fma(sw[j * 2 * dp1 + a] * sw[j * 2 * dp1 + dp1 + b], sq[0][j], acc0);
+        for (int j = j0; j < j1; ++j) acc0 = fma(sw[j * w2 + a] * sw[j * w2 + dp1 + b], sq[0][j], acc0);
       }
     }
     __syncthreads();
   }
   if (o < n) {
-    proxy[(int64_t)leaf * n + o] = acc0;
-    if (nq > 1) proxy[pstride + (int64_t)leaf * n + o] = acc1;
+    sacc[h][0][o] = acc0;
+    sacc[h][1][o] = acc1;
+  }
+  __syncthreads();
+  if (h == 0 && o < n) {
+    double v0 = sacc[0][0][o], v1 = sacc[0][1][o];
+#pragma unroll
+    for (int i = 1; i < kP2mParts; ++i) {
+      v0 += sacc[i][0][o];
+      v1 += sacc[i][1][o];
+    }
+    proxy[(int64_t)leaf * n + o] = v0;
+    if (nq > 1) proxy[pstride + (int64_t)leaf * n + o] = v1;
   }
 }
 
@@ -183,7 +206,7 @@ void launch_p2m_ll(const DevTree& t, const int* leaves, int nleaves, const int* 
                    const double* w, int nq, const double* q, int64_t qstride, double* proxy, int64_t pstride,
                    hipStream_t s) {
   if (nleaves > 0)
-    hipLaunchKernelGGL(k_p2m_ll, dim3(nleaves), dim3(128), 0, s, t, leaves, leaf_first, leaf_count, w, nq, q, qstride,
+    hipLaunchKernelGGL(k_p2m_ll, dim3(nleaves), dim3(128 * kP2mParts), 0, s, t, leaves, leaf_first, leaf_count, w, nq, q, qstride,
                        proxy, pstride);
 }
 
