@@ -1823,8 +1823,15 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
     }
     anc_first.push_back((int)anc.size() / 3);
   }
+  // The table holds 2^(maxgap+1) - 2 matrices; past a depth no tree of a
+  // realistic point set reaches (clustered points could), the level-by-level
+  // lists stay in use instead.
+  constexpr int kMaxCompositeGap = 14;
   std::vector<double> anc_tab;
-  composite_l2l(t.degree, t.cheb.data(), t.cheb.data(), maxgap, anc_tab);
+  if (maxgap <= kMaxCompositeGap)
+    composite_l2l(t.degree, t.cheb.data(), t.cheb.data(), maxgap, anc_tab);
+  else
+    anc.clear();
   Arena A;
   const size_t o_pan = A.take<DevPanel>(t.P), o_px = A.take<double>(3 * U), o_cheb = A.take<double>(t.dp1),
                o_hi = A.take<double>(t.dp1 * t.dp1), o_lo = A.take<double>(t.dp1 * t.dp1), o_perm = A.take<int>(N),
@@ -1908,7 +1915,7 @@ int sbslpm_irreg_create(sbslpm_ctx* c, int64_t n, const double* x, const double*
   h->nonleaf = (int*)(b + o_nl);
   h->anc_first = (int*)(b + o_af);
   h->anc = (int*)(b + o_anc);
-  h->anc_tab = (double*)(b + o_atab);
+  h->anc_tab = anc_tab.empty() ? nullptr : (double*)(b + o_atab);
   h->tiles = (TargetTile*)(b + o_tl);
   h->ntiles = (int)t.tiles.size();
   h->sets = (SourceSet*)(b + o_st);

@@ -970,6 +970,7 @@ void launch_l2l_list(const DevTree& t, const int* parents, int count, int ncomp,
 
 bool launch_leaf_rows(const DevTree& t, const int* leaves, int nleaves, const int* anc_first, const int* anc,
                       const double* anc_tab, int ncomp, double* const* phi_proxy, hipStream_t s) {
+  if (!anc_tab) return false;
   Ptr3 phi{};
   for (int f = 0; f < ncomp; ++f) phi.p[f] = phi_proxy[f];
 #define SBSLPM_LR(D) launch_leaf_rows_d<D>(t, leaves, nleaves, anc_first, anc, anc_tab, ncomp, phi, s)
