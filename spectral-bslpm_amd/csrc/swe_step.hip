@@ -55,36 +55,71 @@ __device__ inline double tile_deriv(const double* s, int a, int b, int dir, cons
 }
 
 // Node coordinates (radians) of a leaf: cos(pi i / D) * half + mid
-// (deriv_funcs_impl.hpp:14-15, 31).
+// (deriv_funcs_impl.hpp:14-15, 31); node o of each direction, o < M.
 template <int M>
-__device__ inline void leaf_nodes(const DevPanel& P, double* xn, double* en) {
+__device__ inline void leaf_nodes(const DevPanel& P, double* xn, double* en, int o) {
   constexpr int D = M - 1;
-  const int o = threadIdx.x;
   if (o < M) {
     xn[o] = cos(kPiD * o / D) * (0.5 * (P.max_xi - P.min_xi)) + 0.5 * (P.max_xi + P.min_xi);
     en[o] = cos(kPiD * o / D) * (0.5 * (P.max_eta - P.min_eta)) + 0.5 * (P.max_eta + P.min_eta);
   }
 }
 
+// The leaf kernels below keep a leaf's m x m tile in LDS and give each of its
+// nodes a thread; a 256-thread workgroup holds leaf_slots<M>() leaves (7 at
+// m = 6, 4 at m = 8), one LDS slot each.  Threads past the last slot or the
+// last leaf take part in the barriers only.
 template <int M>
-__global__ __launch_bounds__(128) void k_panel_lap(DevTree t, const double* __restrict__ f, double* __restrict__ lap) {
-  __shared__ double sf[M * M], sx[M * M], se[M * M], xn[M], en[M];
-  const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
-  const DevPanel& P = t.panels[leaf + t.offset];
-  leaf_nodes<M>(P, xn, en);
-  if (o < M * M) sf[o] = f[(int64_t)leaf * (M * M) + o];
+constexpr int leaf_slots() {
+  return 256 / (M * M);
+}
+struct LeafSlot {
+  int slot, leaf, o, a, b;
+  bool live;
+};
+template <int M>
+__device__ inline LeafSlot leaf_slot(const DevTree& t) {
+  constexpr int M2 = M * M, S = leaf_slots<M>();
+  static_assert(S >= 1, "leaf grid larger than a workgroup");
+  LeafSlot L;
+  L.slot = threadIdx.x / M2;
+  L.o = threadIdx.x - L.slot * M2;
+  L.leaf = blockIdx.x * S + L.slot;
+  L.live = L.slot < S && L.leaf < t.Pl;
+  if (!L.live) L.slot = 0;  // LDS slot 0 is only read through L.live-guarded code
+  L.a = L.o / M;
+  L.b = L.o % M;
+  return L;
+}
+template <int M>
+inline unsigned leaf_blocks(const DevTree& t) {
+  return (unsigned)((t.Pl + leaf_slots<M>() - 1) / leaf_slots<M>());
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_panel_lap(DevTree t, const double* __restrict__ f, double* __restrict__ lap) {
+  constexpr int M2 = M * M, S = leaf_slots<M>();
+  __shared__ double sf_[S][M2], sx_[S][M2], se_[S][M2], xn_[S][M], en_[S][M];
+  const LeafSlot L = leaf_slot<M>(t);
+  const int a = L.a, b = L.b, o = L.o;
+  double *sf = sf_[L.slot], *sx = sx_[L.slot], *se = se_[L.slot], *xn = xn_[L.slot], *en = en_[L.slot];
+  const int64_t k = (int64_t)L.leaf * M2 + o;
+  if (L.live) {
+    leaf_nodes<M>(t.panels[L.leaf + t.offset], xn, en, o);
+    sf[o] = f[k];
+  }
   __syncthreads();
-  if (o < M * M) {
+  if (L.live) {
     sx[o] = tile_deriv<M>(sf, a, b, 0, xn);
     se[o] = tile_deriv<M>(sf, a, b, 1, en);
   }
   __syncthreads();
-  if (o < M * M) {
+  if (L.live) {
     const double fxx = tile_deriv<M>(sx, a, b, 0, xn), fxe = tile_deriv<M>(se, a, b, 0, xn),
                  fee = tile_deriv<M>(se, a, b, 1, en);
     const double X = tan(xn[a]), Y = tan(en[b]);
     const double C2 = 1 + X * X, D2 = 1 + Y * Y, delta = 1 + X * X + Y * Y;
-    lap[(int64_t)leaf * (M * M) + o] = delta * (fxx / C2 + fee / D2 + 2.0 * X * Y / (C2 * D2) * fxe);
+    lap[k] = delta * (fxx / C2 + fee / D2 + 2.0 * X * Y / (C2 * D2) * fxe);
   }
 }
 
@@ -130,21 +165,26 @@ __device__ inline void xyzvec_from_xietavec(double xd, double ed, int face, doub
 // (deriv_funcs_impl.hpp:291-292, then again inside the conversion); rule 1: the
 // leaf's own face, the raw derivatives, the pole nodes mapped correctly.
 template <int M>
-__global__ __launch_bounds__(128) void k_swe_tendency(DevTree t, const double* __restrict__ px,
+__global__ __launch_bounds__(256) void k_swe_tendency(DevTree t, const double* __restrict__ px,
                                                       const double* __restrict__ py, const double* __restrict__ pz,
                                                       CP3 vel, const double* __restrict__ vor,
                                                       const double* __restrict__ div, const double* __restrict__ h,
                                                       const double* __restrict__ lap, double omega, int rule,
                                                       double gravity, P3 out) {
-  __shared__ double sv[3][M * M], xn[M], en[M];
-  const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
-  const DevPanel& P = t.panels[leaf + t.offset];
-  leaf_nodes<M>(P, xn, en);
-  const int64_t k = (int64_t)leaf * (M * M) + o;
-  if (o < M * M)
+  constexpr int M2 = M * M, S = leaf_slots<M>();
+  __shared__ double sv_[S][3][M2], xn_[S][M], en_[S][M];
+  const LeafSlot L = leaf_slot<M>(t);
+  const int leaf = L.leaf, o = L.o, a = L.a, b = L.b;
+  double(*sv)[M2] = sv_[L.slot];
+  double *xn = xn_[L.slot], *en = en_[L.slot];
+  const int64_t k = (int64_t)leaf * M2 + o;
+  if (L.live) {
+    leaf_nodes<M>(t.panels[leaf + t.offset], xn, en, o);
     for (int c = 0; c < 3; ++c) sv[c][o] = vel.p[c][k];
+  }
   __syncthreads();
-  if (o >= M * M) return;
+  if (!L.live) return;
+  const DevPanel& P = t.panels[leaf + t.offset];
   const int face = rule == 0 ? t.panels[leaf].face : P.face;
   const double xi = xn[a], eta = en[b];
   const double X = tan(xi), Y = tan(eta), C = sqrt(1 + X * X), D = sqrt(1 + Y * Y);
@@ -281,7 +321,7 @@ __device__ inline void grad_map(int rule, int face, double xi, double eta, doubl
 // assigned.  sum_ab G_ab G_ba has dG_ab = 2 G_ba; the label-derivative
 // cotangents go back through the transposed leaf differentiation.
 template <int M>
-__global__ __launch_bounds__(128) void k_swe_tendency_adj(DevTree t, const double* __restrict__ px,
+__global__ __launch_bounds__(256) void k_swe_tendency_adj(DevTree t, const double* __restrict__ px,
                                                           const double* __restrict__ py,
                                                           const double* __restrict__ pz, CP3 vel,
                                                           const double* __restrict__ vor,
@@ -289,16 +329,23 @@ __global__ __launch_bounds__(128) void k_swe_tendency_adj(DevTree t, const doubl
                                                           const double* __restrict__ h, CP3 tbar, double omega,
                                                           int rule, double gravity, P3 vbar,
                                                           double* __restrict__ lapbar, P3 sbar) {
-  __shared__ double sv[3][M * M], gx[3][M * M], ge[3][M * M], xn[M], en[M];
-  const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
-  const DevPanel& P = t.panels[leaf + t.offset];
-  leaf_nodes<M>(P, xn, en);
-  const int64_t k = (int64_t)leaf * (M * M) + o;
-  if (o < M * M)
+  constexpr int M2 = M * M, S = leaf_slots<M>();
+  __shared__ double sv_[S][3][M2], gx_[S][3][M2], ge_[S][3][M2], xn_[S][M], en_[S][M];
+  const LeafSlot L = leaf_slot<M>(t);
+  const int leaf = L.leaf, o = L.o, a = L.a, b = L.b;
+  double(*sv)[M2] = sv_[L.slot];
+  double(*gx)[M2] = gx_[L.slot];
+  double(*ge)[M2] = ge_[L.slot];
+  double *xn = xn_[L.slot], *en = en_[L.slot];
+  const int64_t k = (int64_t)leaf * M2 + o;
+  if (L.live) {
+    leaf_nodes<M>(t.panels[leaf + t.offset], xn, en, o);
     for (int c = 0; c < 3; ++c) sv[c][o] = vel.p[c][k];
+  }
   __syncthreads();
   double vb[3] = {0, 0, 0};
-  if (o < M * M) {
+  if (L.live) {
+    const DevPanel& P = t.panels[leaf + t.offset];
     const int face = rule == 0 ? t.panels[leaf].face : P.face;
     const double xi = xn[a], eta = en[b];
     double col0[3], col1[3], G[3][3];
@@ -334,22 +381,25 @@ __global__ __launch_bounds__(128) void k_swe_tendency_adj(DevTree t, const doubl
     lapbar[k] = -gravity * td;
   }
   __syncthreads();
-  if (o < M * M)
+  if (L.live)
     for (int c = 0; c < 3; ++c)
       vbar.p[c][k] += vb[c] + tile_deriv_t<M>(gx[c], a, b, 0, xn) + tile_deriv_t<M>(ge[c], a, b, 1, en);
 }
 
 // Transpose of k_panel_lap: fbar += L^T lapbar.
 template <int M>
-__global__ __launch_bounds__(128) void k_panel_lap_adj(DevTree t, const double* __restrict__ lapbar,
+__global__ __launch_bounds__(256) void k_panel_lap_adj(DevTree t, const double* __restrict__ lapbar,
                                                        double* __restrict__ fbar) {
-  __shared__ double sa[M * M], sb[M * M], sc[M * M], sx[M * M], se[M * M], xn[M], en[M];
-  const int leaf = blockIdx.x, o = threadIdx.x, a = o / M, b = o % M;
-  const DevPanel& P = t.panels[leaf + t.offset];
-  leaf_nodes<M>(P, xn, en);
+  constexpr int M2 = M * M, S = leaf_slots<M>();
+  __shared__ double sa_[S][M2], sb_[S][M2], sc_[S][M2], sx_[S][M2], se_[S][M2], xn_[S][M], en_[S][M];
+  const LeafSlot L = leaf_slot<M>(t);
+  const int o = L.o, a = L.a, b = L.b;
+  double *sa = sa_[L.slot], *sb = sb_[L.slot], *sc = sc_[L.slot], *sx = sx_[L.slot], *se = se_[L.slot];
+  double *xn = xn_[L.slot], *en = en_[L.slot];
+  if (L.live) leaf_nodes<M>(t.panels[L.leaf + t.offset], xn, en, o);
   __syncthreads();
-  const int64_t k = (int64_t)leaf * (M * M) + o;
-  if (o < M * M) {
+  const int64_t k = (int64_t)L.leaf * M2 + o;
+  if (L.live) {
     const double X = tan(xn[a]), Y = tan(en[b]);
     const double C2 = 1 + X * X, D2 = 1 + Y * Y, delta = 1 + X * X + Y * Y, l = lapbar[k];
     sa[o] = delta / C2 * l;                       // d/d f_xixi
@@ -357,12 +407,12 @@ __global__ __launch_bounds__(128) void k_panel_lap_adj(DevTree t, const double* 
     sc[o] = delta * 2.0 * X * Y / (C2 * D2) * l;  // d/d f_xieta
   }
   __syncthreads();
-  if (o < M * M) {
+  if (L.live) {
     sx[o] = tile_deriv_t<M>(sa, a, b, 0, xn);
     se[o] = tile_deriv_t<M>(sc, a, b, 0, xn) + tile_deriv_t<M>(sb, a, b, 1, en);
   }
   __syncthreads();
-  if (o < M * M) fbar[k] += tile_deriv_t<M>(sx, a, b, 0, xn) + tile_deriv_t<M>(se, a, b, 1, en);
+  if (L.live) fbar[k] += tile_deriv_t<M>(sx, a, b, 0, xn) + tile_deriv_t<M>(se, a, b, 1, en);
 }
 
 // Reverse of k_height_corr for lapbar: with kb = -8/0.1 lapbar, the transposed
@@ -461,7 +511,7 @@ struct TendencyAdj {
   static void run(const DevTree& t, const double* x, const double* y, const double* z, CP3 vel, const double* vor,
                   const double* div, const double* h, CP3 tbar, double omega, int rule, double gravity, P3 vbar,
                   double* lapbar, P3 sbar, hipStream_t s) {
-    hipLaunchKernelGGL(k_swe_tendency_adj<M>, dim3(t.Pl), dim3(128), 0, s, t, x, y, z, vel, vor, div, h, tbar, omega,
+    hipLaunchKernelGGL(k_swe_tendency_adj<M>, dim3(leaf_blocks<M>(t)), dim3(256), 0, s, t, x, y, z, vel, vor, div, h, tbar, omega,
                        rule, gravity, vbar, lapbar, sbar);
   }
 };
@@ -469,7 +519,7 @@ struct TendencyAdj {
 template <int M>
 struct PanelLapAdj {
   static void run(const DevTree& t, const double* lapbar, double* fbar, hipStream_t s) {
-    hipLaunchKernelGGL(k_panel_lap_adj<M>, dim3(t.Pl), dim3(128), 0, s, t, lapbar, fbar);
+    hipLaunchKernelGGL(k_panel_lap_adj<M>, dim3(leaf_blocks<M>(t)), dim3(256), 0, s, t, lapbar, fbar);
   }
 };
 
@@ -485,7 +535,7 @@ struct HeightCorrAdj {
 template <int M>
 struct PanelLap {
   static void run(const DevTree& t, const double* f, double* lap, hipStream_t s) {
-    hipLaunchKernelGGL(k_panel_lap<M>, dim3(t.Pl), dim3(128), 0, s, t, f, lap);
+    hipLaunchKernelGGL(k_panel_lap<M>, dim3(leaf_blocks<M>(t)), dim3(256), 0, s, t, f, lap);
   }
 };
 
@@ -494,7 +544,7 @@ struct Tendency {
   static void run(const DevTree& t, const double* x, const double* y, const double* z, CP3 vel, const double* vor,
                   const double* div, const double* h, const double* lap, double omega, int rule, double gravity,
                   P3 out, hipStream_t s) {
-    hipLaunchKernelGGL(k_swe_tendency<M>, dim3(t.Pl), dim3(128), 0, s, t, x, y, z, vel, vor, div, h, lap, omega,
+    hipLaunchKernelGGL(k_swe_tendency<M>, dim3(leaf_blocks<M>(t)), dim3(256), 0, s, t, x, y, z, vel, vor, div, h, lap, omega,
                        rule, gravity, out);
   }
 };
