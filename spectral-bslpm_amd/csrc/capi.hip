@@ -1514,18 +1514,18 @@ static int swe_step_reverse(sbslpm_ctx* c, const double* x, const double* y, con
   for (int i = 0; i < 9; ++i) T[i] = rec + (12 + i) * N;
   const double* state[3] = {vors, divs, height};
   const double* csbar[3] = {sbar[0], sbar[1], sbar[2]};
-  for (int q = 0; q < 3; ++q) launch_fill_zero(sbar[q], N, c->stream);
-  launch_fill_zero(a.vb[0], 12 * N, c->stream);
-  launch_fill_zero(a.tb[0], 9 * N, c->stream);
-  launch_fill_zero(a.rb[0], 3 * N, c->stream);
+  {
+    double* zero[6] = {sbar[0], sbar[1], sbar[2], a.vb[0], a.tb[0], a.rb[0]};
+    const int64_t count[6] = {N, N, N, 12 * N, 9 * N, 3 * N};
+    launch_fill_zero_sets(zero, count, 6, c->stream);
+  }
   // arrival values R = S0 + dt T0, departure search and remap
   launch_apply_tendencies(N, state, T, nullptr, dt, w.arr, nullptr, c->stream);
   const double* arr[3] = {w.arr[0], w.arr[1], w.arr[2]};
   launch_departure_adjoint3(c->dt, x, y, z, V, arr, nbar, dt, c->prm.rk4_combine, a.vb, a.rb, c->stream);
   const double* crb[3] = {a.rb[0], a.rb[1], a.rb[2]};
-  launch_apply_tendencies(N, csbar, crb, nullptr, 1.0, sbar, nullptr, c->stream);
   const double* ctb0[3] = {a.tb[0], a.tb[1], a.tb[2]};
-  launch_apply_tendencies(N, ctb0, crb, nullptr, dt, a.tb, nullptr, c->stream);
+  launch_apply_two(N, crb, csbar, 1.0, sbar, ctb0, dt, a.tb, c->stream);
   const double* cdp[3] = {w.dpart[0], w.dpart[1], w.dpart[2]};
   const double* cdx[3] = {w.dprox[0], w.dprox[1], w.dprox[2]};
   int rc = SBSLPM_OK;
@@ -1544,12 +1544,10 @@ static int swe_step_reverse(sbslpm_ctx* c, const double* x, const double* y, con
       launch_swe_tendency_adjoint(c->dt, x, y, z, V + 3 * s, vors, divs, height, a.tb + 3 * s, om, rule, ph.gravity,
                                   a.vb + 3 * s, a.lapbar, sbar, c->stream);
       // height correction: lap -= 80 BLI[h + tau T_h](P(x + tau vprev)) (+ h_s)
-      launch_lincomb(N, height, tau, tprev[2], a.hh, c->stream);
-      launch_fill_zero(a.hscat, N, c->stream);
+      launch_lincomb_zero(N, height, tau, tprev[2], a.hh, a.hscat, c->stream);
       launch_height_correction_adjoint(c->dt, x, y, z, vprev, a.hh, tau, ph, a.lapbar, a.hscat, a.vb + 3 * (s - 1),
                                        c->stream);
-      launch_lincomb(N, sbar[2], 1.0, a.hscat, sbar[2], c->stream);
-      launch_lincomb(N, a.tb[3 * (s - 1) + 2], tau, a.hscat, a.tb[3 * (s - 1) + 2], c->stream);
+      launch_add2(N, a.hscat, sbar[2], tau, a.tb[3 * (s - 1) + 2], c->stream);
       // regularised-Laplacian sums of area_s * eff at displaced sources
       const double* eff = w.st[2];
       if (topo) {
@@ -2457,8 +2455,11 @@ int step_reverse(sbslpm_ctx* c, const double* x, const double* y, const double* 
   const double* cdx[3] = {w.dispproxy[0], w.dispproxy[1], w.dispproxy[2]};
   int rc = SBSLPM_OK;
   // remap and departure search
-  launch_fill_zero(vor_bar, t.npart, c->stream);
-  launch_fill_zero(VB[0], 9 * t.npart, c->stream);  // v0..v2 cotangents (contiguous)
+  {  // vor_bar and the v0..v2 cotangents (contiguous) in one launch
+    double* zero[2] = {vor_bar, VB[0]};
+    const int64_t count[2] = {t.npart, 9 * t.npart};
+    launch_fill_zero_sets(zero, count, 2, c->stream);
+  }
   launch_departure_adjoint(c->dt, x, y, z, V, vor, new_vor_bar, dt, c->prm.rk4_combine, om, VB, vor_bar, c->stream);
   // stages 3, 2, 1: velocity at displaced sources, then the displacement update
   for (int s = 2; s >= 0 && rc == SBSLPM_OK; --s) {

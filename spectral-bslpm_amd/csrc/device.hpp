@@ -445,6 +445,15 @@ void launch_node_correction(int64_t n, const double* x, const double* y, const d
                             const SwePhys& ph, double* lap, hipStream_t s);
 // out = x + alpha y (out may alias x or y).
 void launch_lincomb(int64_t n, const double* x, double alpha, const double* y, double* out, hipStream_t s);
+// Small fused element-wise steps of the reverse passes: zero fill of n arrays
+// in one launch; out = x + alpha y with zero[] cleared; a += src, b += beta
+// src; out1 = f1 + alpha1 r, out2 = f2 + alpha2 r for three components.
+void launch_fill_zero_sets(double* const* arrays, const int64_t* counts, int n, hipStream_t s);
+void launch_lincomb_zero(int64_t n, const double* x, double alpha, const double* y, double* out, double* zero,
+                         hipStream_t s);
+void launch_add2(int64_t n, const double* src, double* a, double beta, double* b, hipStream_t s);
+void launch_apply_two(int64_t n, const double* const* r, const double* const* f1, double alpha1, double* const* out1,
+                      const double* const* f2, double alpha2, double* const* out2, hipStream_t s);
 void launch_height_correction_adjoint(const DevTree& t, const double* x, const double* y, const double* z,
                                       const double* const* vel, const double* hh, double tau, const SwePhys& ph,
                                       const double* lapbar, double* hscat, double* const* vbar, hipStream_t s);

@@ -661,7 +661,88 @@ __global__ void k_lincomb(int64_t n, const double* __restrict__ x, double alpha,
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = x[k] + alpha * y[k];
 }
+
+// Zero fill of up to kFillArrays arrays in one launch: array blockIdx.y.
+constexpr int kFillArrays = 6;
+struct FillSet {
+  double* p[kFillArrays];
+  int64_t count[kFillArrays];
+};
+__global__ void k_fill_zero_sets(FillSet f) {
+  const int y = blockIdx.y;
+  double* p = f.p[0];
+  int64_t count = f.count[0];
+#pragma unroll
+  for (int i = 1; i < kFillArrays; ++i)
+    if (y == i) {
+      p = f.p[i];
+      count = f.count[i];
+    }
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x)
+    p[k] = 0.0;
+}
+
+// out = x + alpha y, zero[k] = 0 (a scatter target cleared in the same pass).
+__global__ void k_lincomb_zero(int64_t n, const double* __restrict__ x, double alpha, const double* __restrict__ y,
+                               double* __restrict__ out, double* __restrict__ zero) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  out[k] = x[k] + alpha * y[k];
+  zero[k] = 0.0;
+}
+
+// a += src, b += beta src (two accumulations of one cotangent).
+__global__ void k_add2(int64_t n, const double* __restrict__ src, double* __restrict__ a, double beta,
+                       double* __restrict__ b) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double v = src[k];
+  a[k] += v;
+  b[k] += beta * v;
+}
+
+// out1_q = f1_q + alpha1 r_q, out2_q = f2_q + alpha2 r_q for q = 0..2.
+__global__ void k_apply_two(int64_t n, CP3 r, CP3 f1, double alpha1, P3 out1, CP3 f2, double alpha2, P3 out2) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int q = 0; q < 3; ++q) {
+    const double v = r.p[q][k];
+    out1.p[q][k] = f1.p[q][k] + alpha1 * v;
+    out2.p[q][k] = f2.p[q][k] + alpha2 * v;
+  }
+}
 }  // namespace
+
+void launch_fill_zero_sets(double* const* arrays, const int64_t* counts, int n, hipStream_t s) {
+  for (int b = 0; b < n; b += kFillArrays) {
+    FillSet f{};
+    int64_t most = 0;
+    const int m = n - b < kFillArrays ? n - b : kFillArrays;
+    for (int i = 0; i < m; ++i) {
+      f.p[i] = arrays[b + i];
+      f.count[i] = counts[b + i];
+      most = counts[b + i] > most ? counts[b + i] : most;
+    }
+    const unsigned gx = (unsigned)std::min<int64_t>(blocks_for(most, 256), 2048);
+    if (most > 0) hipLaunchKernelGGL(k_fill_zero_sets, dim3(gx, m), dim3(256), 0, s, f);
+  }
+}
+
+void launch_lincomb_zero(int64_t n, const double* x, double alpha, const double* y, double* out, double* zero,
+                         hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_lincomb_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, alpha, y, out, zero);
+}
+
+void launch_add2(int64_t n, const double* src, double* a, double beta, double* b, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_add2, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, src, a, beta, b);
+}
+
+void launch_apply_two(int64_t n, const double* const* r, const double* const* f1, double alpha1, double* const* out1,
+                      const double* const* f2, double alpha2, double* const* out2, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_apply_two, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, cp3(r), cp3(f1), alpha1, p3(out1),
+                       cp3(f2), alpha2, p3(out2));
+}
 
 void launch_node_correction(int64_t n, const double* x, const double* y, const double* z, const double* h,
                             const SwePhys& ph, double* lap, hipStream_t s) {
