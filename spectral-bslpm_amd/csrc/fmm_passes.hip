@@ -653,6 +653,96 @@ __global__ __launch_bounds__(256) void k_m2m_tiled(DevTree t, int pbegin, const 
   parent_from_children<D, D>(sq, const_tab(t.m2m_hi), const_tab(t.m2m_lo), sU, q + (int64_t)par * N, lane, w);
 }
 
+// Two M2M levels in one launch, one workgroup per grandparent g (panels
+// gbegin ..): wave c forms child c's row from its four children (the 16
+// grandchildren rows, consecutive in a uniform tree, staged in LDS; both
+// sweeps inside the wave; the eta sweep forms the sums of both halves with
+// uniform table reads and keeps the lane's own, the upper half for
+// grandchildren 0 and 1; the xi sweep spreads the rows over the wave's lanes
+// and reads the matrices from LDS), writes it and keeps it in LDS; parent_from_children
+// then forms g's row.  The arithmetic of each row is that of k_m2m_tiled.
+// Uniform trees only (every panel above the leaves has four children,
+// numbered level by level).
+template <int D>
+__global__ __launch_bounds__(256) void k_m2m_two(DevTree t, int gbegin, UpSets sets) {
+  constexpr int N = D * D;
+  static_assert(4 * D <= 64, "k_m2m_two: one lane per grandchild row");
+  __shared__ double sg[16 * N];
+  __shared__ double sW[4][4 * N];
+  __shared__ double sc[4 * N];
+  __shared__ double sU[4][N];
+  __shared__ double sX[2][N];  // m2m_hi, m2m_lo: the xi sweep reads a row per lane
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = gbegin + blockIdx.x;
+  const int c0 = t.panels[g].child0;
+  const int gc0 = t.panels[c0].child0;
+  double* __restrict__ q = sets.n > 0 ? pick_set(sets.qx, (int)blockIdx.y) : sets.qx[0] + member_proxy(t);
+  for (int i = threadIdx.x; i < 16 * N; i += blockDim.x) sg[i] = q[(int64_t)gc0 * N + i];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    sX[0][i] = t.m2m_hi[i];
+    sX[1][i] = t.m2m_lo[i];
+  }
+  __syncthreads();
+  const ConstTab Thi = const_tab(t.m2m_hi), Tlo = const_tab(t.m2m_lo);
+  if (lane < 4 * D) {  // child c = w, row k of grandchild cc: U[cc][k][b] = sum_l Y_cc[b][l] q[cc][k][l]
+    const int cc = lane / D, k = lane - cc * D;
+    const bool yhi = cc <= 1;
+    double r[D];
+#pragma unroll
+    for (int l = 0; l < D; ++l) r[l] = sg[((w * 4 + cc) * D + k) * D + l];
+#pragma unroll 1
+    for (int b = 0; b < D; ++b) {  // both halves' sums (uniform table reads), then the lane's one
+      double vh = 0.0, vl = 0.0;
+#pragma unroll
+      for (int l = 0; l < D; ++l) {
+        vh += Thi[b * D + l] * r[l];
+        vl += Tlo[b * D + l] * r[l];
+      }
+      sW[w][(cc * D + k) * D + b] = yhi ? vh : vl;
+    }
+  }
+  wave_lds_order();
+  constexpr int A = 64 / D;  // row groups of the xi sweep
+  if (lane < A * D) {  // child c = w, column b, rows a = ai, ai + A, ...: Q[a][b] = sum_cc sum_k X_cc[a][k] U[cc][k][b]
+    const int ai = lane / D, b = lane - ai * D;
+    double u[4][D];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+      for (int k = 0; k < D; ++k) u[cc][k] = sW[w][(cc * D + k) * D + b];
+#pragma unroll 1
+    for (int a = ai; a < D; a += A) {
+      double v = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        const double* X = sX[(cc == 0 || cc == 3) ? 0 : 1] + a * D;
+#pragma unroll
+        for (int k = 0; k < D; ++k) v += X[k] * u[cc][k];
+      }
+      q[(int64_t)(c0 + w) * N + a * D + b] = v;
+      sc[w * N + a * D + b] = v;
+    }
+  }
+  __syncthreads();
+  parent_from_children<D, D>(sc, Thi, Tlo, sU, q + (int64_t)g * N, lane, w);
+}
+
+template <int D>
+bool launch_m2m_two_d(const DevTree& t, int gbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
+  if (t.dp1 != D) return false;
+  hipLaunchKernelGGL((k_m2m_two<D>), dim3(count, sets.n > 0 ? sets.n : nmem), dim3(256), 0, s, t, gbegin, sets);
+  return true;
+}
+
+// Levels l and l - 1 of a uniform tree in one launch (grandparents at level
+// l - 1); false for a degree without an instance.
+bool launch_m2m_two(const DevTree& t, int gbegin, int count, const UpSets& sets, hipStream_t s, int nmem) {
+#define SBSLPM_M2M2(D) launch_m2m_two_d<D>(t, gbegin, count, sets, s, nmem)
+  return SBSLPM_M2M2(9) || SBSLPM_M2M2(6) || SBSLPM_M2M2(5) || SBSLPM_M2M2(7) || SBSLPM_M2M2(11);
+#undef SBSLPM_M2M2
+}
+
 template <int D>
 bool launch_m2m_tiled_d(const DevTree& t, int pbegin, const int* list, int count, const UpSets& sets, hipStream_t s,
                         int nmem) {
@@ -846,6 +936,13 @@ static void launch_m2m_levels(const DevTree& t, int lfirst, double* q_proxy, hip
   one.qx[0] = q_proxy;
   const UpSets& us = sets ? *sets : one;
   for (int l = lfirst; l >= 0; --l) {
+    if (l >= 1) {  // levels l and l - 1 in one launch
+      const int gbegin = 2 * ((1 << (2 * (l - 1))) - 1), gcount = 6 << (2 * (l - 1));
+      if (launch_m2m_two(t, gbegin, gcount, us, s, nmem)) {
+        --l;
+        continue;
+      }
+    }
     const int pbegin = 2 * ((1 << (2 * l)) - 1), count = 6 << (2 * l);
     if (launch_m2m_tiled(t, pbegin, count, us, s, nmem)) continue;
     if (us.n == 0)
