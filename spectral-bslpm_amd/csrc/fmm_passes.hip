@@ -463,7 +463,7 @@ __global__ __launch_bounds__(256) void k_down_fused(DevTree t, int ncomp, CPtr3 
       double p[D];
 #pragma unroll
       for (int b = 0; b < D; ++b) p[b] = phif[anc * N + a * D + b];
-#pragma unroll 1
+#pragma unroll 3
       for (int l = 0; l < D; ++l) {
         double v = 0.0;
 #pragma unroll
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256) void k_down_fused(DevTree t, int ncomp, CPtr3 
       double q[D];
 #pragma unroll
       for (int r = 0; r < D; ++r) q[r] = sV[w][f * N + r * D + a];
-#pragma unroll 1
+#pragma unroll 3
       for (int k = 0; k < D; ++k) {
         double v = part_w[k * D];
 #pragma unroll
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(256) void k_leaf_rows(const int* __restrict__ leave
       double p[D];
 #pragma unroll
       for (int b = 0; b < D; ++b) p[b] = phif[ap * N + a * D + b];
-#pragma unroll 1
+#pragma unroll 3
       for (int l = 0; l < D; ++l) {
         double v = 0.0;
 #pragma unroll
@@ -822,7 +822,7 @@ __global__ __launch_bounds__(256) void k_leaf_rows(const int* __restrict__ leave
       double q[D];
 #pragma unroll
       for (int r = 0; r < D; ++r) q[r] = sV[w][f * N + r * D + a];
-#pragma unroll 1
+#pragma unroll 3
       for (int k = 0; k < D; ++k) {
         double v = part_w[k * D];
 #pragma unroll
