@@ -498,7 +498,7 @@ __global__ __launch_bounds__(256) void k_down_fused(DevTree t, int ncomp, CPtr3 
     double p[D];
 #pragma unroll
     for (int b = 0; b < D; ++b) p[b] = sAcc[f * N + a * D + b];
-#pragma unroll 1
+#pragma unroll 3
     for (int l = 0; l < M; ++l) {
       double v = 0.0;
 #pragma unroll
@@ -570,7 +570,7 @@ __device__ inline void parent_from_children(const double* sq, ConstTab Thi, Cons
     double r[M];
 #pragma unroll
     for (int l = 0; l < M; ++l) r[l] = sq[w * M2 + lane * M + l];
-#pragma unroll 1
+#pragma unroll 3
     for (int b = 0; b < D; ++b) {
       double v = 0.0;
 #pragma unroll
@@ -585,7 +585,7 @@ __device__ inline void parent_from_children(const double* sq, ConstTab Thi, Cons
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int k = 0; k < M; ++k) u[c][k] = sU[c][k * D + lane];
-#pragma unroll 1
+#pragma unroll 3
     for (int a = w; a < D; a += 4) {
       double v = 0.0;
 #pragma unroll
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void k_m2m_two(DevTree t, int gbegin, UpSets s
     double r[D];
 #pragma unroll
     for (int l = 0; l < D; ++l) r[l] = sg[((w * 4 + cc) * D + k) * D + l];
-#pragma unroll 1
+#pragma unroll 3
     for (int b = 0; b < D; ++b) {  // both halves' sums (uniform table reads), then the lane's one
       double vh = 0.0, vl = 0.0;
 #pragma unroll
