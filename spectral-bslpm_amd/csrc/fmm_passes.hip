@@ -416,10 +416,15 @@ __device__ inline T pick_set(T const (&a)[kMaxUpSets], int y) {
 template <int D, int M>
 __global__ __launch_bounds__(256) void k_down_fused(DevTree t, int ncomp, CPtr3 phi, CPtr3 part, Ptr3 sols) {
   constexpr int N = D * D, M2 = M * M;
+  // LDS is the occupancy limit, so phase 2 reuses phase 1's buffers: the W
+  // rows go where the waves' V slices were (phase 1 is over at the barrier
+  // before they are written), and the summed row goes over wave 0's sum (each
+  // thread reads its own entries of the four sums, then overwrites the first).
   __shared__ double sV[4][3 * N];
   __shared__ double sPart[4][3 * N];
-  __shared__ double sAcc[3 * N];
-  __shared__ double sW[2][3 * D * M];
+  static_assert(2 * 3 * D * M <= 4 * 3 * N, "k_down_fused: W rows do not fit the V slices");
+  double* const sAcc = sPart[0];
+  double(*const sW)[3 * D * M] = reinterpret_cast<double(*)[3 * D * M]>(&sV[0][0]);
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lp = t.levels - 2, ip = blockIdx.x;
